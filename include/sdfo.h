@@ -1,0 +1,236 @@
+/*
+ * sdfo.h -- C ABI of the MI355X (gfx950) SdfOracle build / query / contact library.
+ *
+ * This is the drop-in boundary for the OpenMP path of the reference plugin
+ * (plugins/SdfOracle) and for the per-point contact reduction of the Python
+ * prototype.  The reference has no C ABI of its own; each entry point below
+ * names the reference interface it replaces.  Plain pointers and sizes only.
+ *
+ * Conventions
+ *  - Every function returns SDFO_OK (0) or a negative SDFO_ERR_* code; the
+ *    message text (the reference's exception text where one exists) is
+ *    available from sdfo_last_error() on the calling thread.  No C++
+ *    exception crosses this boundary.
+ *  - Builds take HOST mesh arrays and produce a grid handle owned by the
+ *    library (device resident, freed by sdfo_grid_free).
+ *  - Queries and contact calls take DEVICE pointers (e.g. torch tensor
+ *    data_ptr()) owned by the caller, and an explicit hipStream_t (NULL =
+ *    the null stream).  They enqueue work and return without synchronising.
+ *  - Dense fields are x-fastest, linear index i + nx*(j + ny*k)
+ *    (GridSpec.h:56-59).  Hessians are 6 values Hxx,Hxy,Hxz,Hyy,Hyz,Hzz.
+ */
+#ifndef SDFO_H
+#define SDFO_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SDFO_OK 0
+#define SDFO_ERR_INVALID_ARGUMENT (-1)
+#define SDFO_ERR_EMPTY_MESH (-2)
+#define SDFO_ERR_IO (-3)
+#define SDFO_ERR_FORMAT (-4)
+#define SDFO_ERR_DEVICE (-5)
+#define SDFO_ERR_OUT_OF_MEMORY (-6)
+#define SDFO_ERR_UNSUPPORTED (-7)
+
+/* QueryMode (SdfOracle.h:11-18), same order. */
+#define SDFO_FIRST_ORDER 0
+#define SDFO_SECOND_ORDER 1
+#define SDFO_TRILINEAR 2
+#define SDFO_TRICUBIC 3
+#define SDFO_CONTACT_AWARE 4
+
+typedef struct sdfo_grid sdfo_grid;
+typedef void* sdfo_stream; /* hipStream_t */
+
+/* BuilderParams / SparseBuilderParams / ContactAwareBuilderParams
+ * (SdfBuilder.h:24-55) flattened into one struct.  sdfo_default_build_params
+ * fills the reference's defaults. */
+typedef struct sdfo_build_params
+{
+    double padding;            /* fraction of the bbox diagonal (0.1) */
+    double voxel_size;         /* 0 = longest side / target_resolution */
+    int32_t target_resolution; /* 128 */
+    int32_t max_resolution;    /* 512 */
+    /* sparse narrow band */
+    int32_t block_dim;    /* 8 */
+    int32_t halo_voxels;  /* 2 */
+    int32_t band_voxels;  /* 8 */
+    int32_t store_hessian; /* 1 */
+    double band_distance; /* 0 = band_voxels * voxel */
+    /* contact aware */
+    int32_t max_candidates;             /* 4 */
+    int32_t tile_dim;                   /* 8 */
+    double feature_distance_tolerance;  /* 0 = 1.5 voxel */
+    double normal_merge_angle_degrees;  /* 5 */
+    int32_t use_tile_acceleration;      /* 1 */
+    int32_t reserved;
+} sdfo_build_params;
+
+/* Grid description (GridSpec.h:17-27 plus the sparse / contact-aware
+ * header fields of SdfIO.cpp:62-80). */
+typedef struct sdfo_grid_info
+{
+    double bmin[3], bmax[3], voxel[3];
+    int32_t shape[3];
+    int32_t sparse;          /* 0 dense, 1 sparse bricks */
+    int32_t has_hessian;
+    int32_t candidate_stride; /* 0 = no contact-aware candidates */
+    int32_t block_dim, halo_voxels;
+    int32_t block_count;      /* sparse: active bricks */
+    int32_t block_grid[3];    /* sparse: ceil(n / block_dim) */
+    double band_distance;
+    double activation_distance;
+    int64_t device_bytes;     /* device memory held by the handle */
+} sdfo_grid_info;
+
+/* Contact law parameters (contacts_sdf.py:300-305; ContactPair fields,
+ * ir/contact.py:1-18). */
+typedef struct sdfo_contact_params
+{
+    double kn;            /* normal stiffness */
+    double damping;       /* c_n, quadrature_settings['damping'] */
+    double mu;            /* friction coefficient */
+    double regularizer;   /* quadrature_settings['regularizer'] (1e-6) */
+    double narrow_margin; /* |Y.z| < margin filter (contacts_sdf.py:91-98); <= 0 disables */
+    int32_t order;        /* k_order (1 or 2) */
+    int32_t mode;         /* SDFO_TRILINEAR or SDFO_TRICUBIC */
+} sdfo_contact_params;
+
+/* ---- library ----------------------------------------------------------- */
+const char* sdfo_version(void);
+/* Message of the last failed call on this thread ("" if none). */
+const char* sdfo_last_error(void);
+/* Fills the reference defaults (SdfBuilder.h:24-55). */
+void sdfo_default_build_params(sdfo_build_params* params);
+
+/* SdfBuilder::setupGrid (SdfBuilder.cpp:994-1035) + snapToStandardResolution
+ * (GridSpec.h:65-74); host only. */
+int sdfo_setup_grid(const double* verts, int64_t n_verts, const sdfo_build_params* params,
+                    sdfo_grid_info* out);
+
+/* ---- builders ---------------------------------------------------------- */
+/* SdfBuilder::build (SdfBuilder.h:58-62, SdfBuilder.cpp:364-407).
+ * verts: n_verts x 3 doubles, tris: n_tris x 3 int32 (host). */
+int sdfo_build_dense(const double* verts, int64_t n_verts, const int32_t* tris, int64_t n_tris,
+                     const sdfo_build_params* params, sdfo_grid** out, sdfo_stream stream);
+
+/* SdfBuilder::buildSparse (SdfBuilder.h:82-86, SdfBuilder.cpp:823-983). */
+int sdfo_build_sparse(const double* verts, int64_t n_verts, const int32_t* tris, int64_t n_tris,
+                      const sdfo_build_params* params, sdfo_grid** out, sdfo_stream stream);
+
+/* SdfBuilder::buildContactAware / buildSparseContactAware (SdfBuilder.h:70-78,
+ * SdfBuilder.cpp:416-821).  corner_normals: n_tris x 3 x 3 doubles. */
+int sdfo_build_contact_aware(const double* verts, int64_t n_verts, const int32_t* tris,
+                             int64_t n_tris, const double* corner_normals,
+                             const sdfo_build_params* params, int32_t sparse, sdfo_grid** out,
+                             sdfo_stream stream);
+
+/* Batched dense build of many meshes in one persistent launch (no
+ * reference counterpart: the reference builds one mesh per call).
+ * Mesh m uses verts[vert_offsets[m]..vert_offsets[m+1]) and
+ * tris[tri_offsets[m]..tri_offsets[m+1]) with mesh-local vertex indices;
+ * params[m] is its builder configuration.  out[m] receives its grid. */
+int sdfo_build_dense_batched(int32_t n_meshes, const double* verts, const int64_t* vert_offsets,
+                             const int32_t* tris, const int64_t* tri_offsets,
+                             const sdfo_build_params* params, sdfo_grid** out,
+                             sdfo_stream stream);
+
+/* ---- grid handle ------------------------------------------------------- */
+void sdfo_grid_free(sdfo_grid* grid);
+int sdfo_grid_get_info(const sdfo_grid* grid, sdfo_grid_info* info);
+
+/* Copies the FP64 dense fields to host arrays (SdfData.h:27-118 members
+ * phi0, n0, H0, witness0; any pointer may be NULL).  tri receives the
+ * nearest-triangle index per voxel (-1 when unknown, e.g. after a load). */
+int sdfo_grid_download_dense(const sdfo_grid* grid, double* phi, double* n3, double* h6,
+                             double* witness3, int32_t* tri);
+
+/* Copies the dense contact-aware candidate arrays (SdfData.h:46-50). */
+int sdfo_grid_download_candidates(const sdfo_grid* grid, uint8_t* count, double* phi,
+                                  double* n3, int32_t* branch);
+
+/* Copies the sparse bricks (SparseSdfData, SdfData.h:154-303) in (k,j,i)
+ * order: bijk[nb*3], phi[nb*B^3], n3[nb*B^3*3], h6[nb*B^3*6],
+ * active_mask[bx*by*bz] (brick activation mask, bit exact with the
+ * reference rule), and candidates when present.  Any pointer may be NULL. */
+int sdfo_grid_download_sparse(const sdfo_grid* grid, int32_t* bijk, float* phi, float* n3,
+                              float* h6, uint8_t* active_mask, uint8_t* ccount, float* cphi,
+                              float* cn3, int32_t* cbranch);
+
+/* saveSdf / saveSparseSdf (SdfIO.cpp:93-275): byte-identical "SDFO" 0.1.0. */
+int sdfo_grid_save(const sdfo_grid* grid, const char* path);
+/* SdfOracle::loadFromFile (SdfOracle.cpp:165-172) -> loadSdf / loadSparseSdf
+ * (SdfIO.cpp:279-523); the reference's validation errors map to
+ * SDFO_ERR_IO / SDFO_ERR_FORMAT with the same message text. */
+int sdfo_grid_load(const char* path, sdfo_grid** out);
+/* isValidSdf / isSparseSdf (SdfIO.cpp:527-552): 1 true, 0 false. */
+int sdfo_is_valid_file(const char* path);
+int sdfo_is_sparse_file(const char* path);
+
+/* Creates a dense grid from host FP64 fields (SdfOracle(SdfData),
+ * SdfOracle.cpp:154-157).  h6 may be NULL (zero Hessian); candidate arrays
+ * may be NULL when stride == 0. */
+int sdfo_grid_from_dense(const double* bmin3, const double* bmax3, const double* voxel3,
+                         const int32_t* shape3, const double* phi, const double* n3,
+                         const double* h6, const double* witness3, int32_t stride,
+                         const uint8_t* ccount, const double* cphi, const double* cn3,
+                         const int32_t* cbranch, sdfo_grid** out);
+
+/* ---- queries ----------------------------------------------------------- */
+/* SdfOracle::queryBatch / queryFast (SdfOracle.cpp:176-444) for FP32 point
+ * streams, FP32 arithmetic with FP64 cell location.
+ * points: n x 3 floats (device).  gap[n], grad3[n*3] (raw gradient, the
+ * Python contract of sdf_grid.py:138-204), valid[n] (device, required);
+ * normal3 / witness3 / branch (device, optional, NULL to skip). */
+int sdfo_query(const sdfo_grid* grid, int32_t mode, const float* points, int64_t n, float* gap,
+               float* grad3, uint8_t* valid, float* normal3, float* witness3, int32_t* branch,
+               sdfo_stream stream);
+
+/* Same for FP64 point streams with FP64 outputs (FP32 field data, FP64
+ * arithmetic); used by the source-compatible C++ SdfOracle mirror. */
+int sdfo_query_f64(const sdfo_grid* grid, int32_t mode, const double* points, int64_t n,
+                   double* gap, double* grad3, uint8_t* valid, double* normal3,
+                   double* witness3, int32_t* branch, sdfo_stream stream);
+
+/* Many query streams against many grids in one launch (no reference
+ * counterpart).  Stream s queries grids[s] with points
+ * [offsets[s], offsets[s+1]) of the packed arrays; all pointers device
+ * except grids/offsets (host). */
+int sdfo_query_batched(int32_t n_streams, const sdfo_grid* const* grids, int32_t mode,
+                       const int64_t* offsets, const float* points, float* gap, float* grad3,
+                       uint8_t* valid, sdfo_stream stream);
+
+/* ---- contact ----------------------------------------------------------- */
+/* Per-pose penalty + damping + regularised Coulomb friction reduction of
+ * SDFContactEngine.evaluate (contacts_sdf.py:329-385), evaluated in the SDF
+ * (body B) frame, FP64 arithmetic, deterministic FP64 sums.
+ * X: n x 3 quadrature points in A's frame, w: n weights (device, FP64).
+ * poses: B x 30 doubles per pose: R(9 row major), t(3) [Y = R X + t];
+ *        M(9), m(3) [offset from A's centre of mass, off = M X + m];
+ *        v0(3), Om(3) [relative velocity u = v0 + Om x off].
+ * out: B x 6 (F_A, tau_A) (device).  active: B int32 (device, optional). */
+int sdfo_contact_forward(const sdfo_grid* grid, const double* X, const double* w, int64_t n,
+                         const double* poses, int64_t n_poses, const sdfo_contact_params* params,
+                         double* out, int32_t* active, sdfo_stream stream);
+
+/* Hand-written backward of sdfo_contact_forward (no reference counterpart).
+ * g_out: B x 6 upstream gradient.  Outputs (device, each optional):
+ * g_twist: B x 6 gradient w.r.t. a left twist (omega, dt) of A's pose,
+ * g_params: 3 (kn, damping, mu) summed over poses, g_X: n x 3, g_w: n. */
+int sdfo_contact_backward(const sdfo_grid* grid, const double* X, const double* w, int64_t n,
+                          const double* poses, int64_t n_poses,
+                          const sdfo_contact_params* params, const double* g_out,
+                          double* g_twist, double* g_params, double* g_X, double* g_w,
+                          sdfo_stream stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SDFO_H */

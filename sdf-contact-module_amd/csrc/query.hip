@@ -1,0 +1,271 @@
+// Point-query kernels: the five query modes over dense or sparse grids
+// (SdfOracle.cpp:176-432, 448-1092).  One lane per point; the point stream
+// and the outputs are read/written once and coalesced, the field corners
+// are gathered from the FP32 mirror (float4 = phi + normal per voxel).
+#include <hip/hip_runtime.h>
+
+#include "interp_device.hpp"
+#include "sdfo_internal.hpp"
+
+namespace sdfo {
+
+using dev::Cell;
+using dev::Interp;
+using dev::QGrid;
+using dev::V3T;
+
+namespace dev {
+
+QGrid makeQGrid(const sdfo_grid& g)
+{
+    QGrid q{};
+    for (int a = 0; a < 3; ++a)
+    {
+        q.bmin[a] = g.spec.bmin[a];
+        q.voxel[a] = g.spec.voxel[a];
+        q.n[a] = g.spec.n[a];
+    }
+    q.sparse = g.sparse ? 1 : 0;
+    q.hasHessian = g.hasHessian ? 1 : 0;
+    q.stride = g.stride;
+    q.pn = g.pn;
+    q.h6f = g.h6f;
+    q.ccount = g.ccount;
+    q.cpn = g.cpn;
+    q.cbranch = g.cbranch;
+    q.bd = g.blockDim;
+    q.nbx = g.nbx;
+    q.nby = g.nby;
+    q.nbz = g.nbz;
+    q.table = g.table;
+    q.spn = g.spn;
+    q.sh6 = g.sh6;
+    q.sccount = g.sccount;
+    q.scpn = g.scpn;
+    q.scbranch = g.scbranch;
+    return q;
+}
+
+} // namespace dev
+
+namespace {
+
+template <typename T>
+struct QOut
+{
+    T gap;
+    V3T<T> grad;
+    V3T<T> normal;
+    bool valid;
+    int branch;
+};
+
+template <typename T>
+__device__ __forceinline__ T tsqrt(T x);
+template <>
+__device__ __forceinline__ float tsqrt<float>(float x)
+{
+    return sqrtf(x);
+}
+template <>
+__device__ __forceinline__ double tsqrt<double>(double x)
+{
+    return sqrt(x);
+}
+
+// Unit normal of trilinear / tricubic (SdfOracle.cpp:585-590, 1083-1087).
+template <typename T>
+__device__ __forceinline__ void finishInterp(const Interp<T>& it, QOut<T>& o)
+{
+    o.gap = it.gap;
+    o.grad = it.grad;
+    const T len = tsqrt<T>(it.grad.x * it.grad.x + it.grad.y * it.grad.y + it.grad.z * it.grad.z);
+    if (len > T(1e-15))
+        o.normal = V3T<T>{it.grad.x / len, it.grad.y / len, it.grad.z / len};
+    else
+        o.normal = V3T<T>{T(1), T(0), T(0)};
+    o.valid = true;
+}
+
+// First / second order Taylor at the nearest voxel (SdfOracle.cpp:448-520,
+// 600-654).  Second order without a Hessian degrades to first order.
+template <typename T>
+__device__ __forceinline__ void taylorQuery(const QGrid& g, double px, double py, double pz,
+                                            bool second, QOut<T>& o)
+{
+    int i, j, k;
+    if (!dev::nearestVoxel(g, px, py, pz, i, j, k)) return;
+    const int64_t rec = dev::recordOf(g, i, j, k);
+    if (rec < 0) return;
+    const float4 f = dev::fieldAt(g, rec);
+    const T dx = T(px - (g.bmin[0] + g.voxel[0] * (i + 0.5)));
+    const T dy = T(py - (g.bmin[1] + g.voxel[1] * (j + 0.5)));
+    const T dz = T(pz - (g.bmin[2] + g.voxel[2] * (k + 0.5)));
+    const T nx = T(f.y), ny = T(f.z), nz = T(f.w);
+    const T first = nx * dx + ny * dy + nz * dz;
+    const float* hp = g.sparse ? g.sh6 : g.h6f;
+    if (!second || hp == nullptr || !g.hasHessian)
+    {
+        o.gap = T(f.x) + first;
+        o.grad = V3T<T>{nx, ny, nz};
+        o.normal = o.grad;
+        o.valid = true;
+        return;
+    }
+    const float* h = hp + 6 * rec;
+    const T hxx = T(h[0]), hxy = T(h[1]), hxz = T(h[2]), hyy = T(h[3]), hyz = T(h[4]),
+            hzz = T(h[5]);
+    const T Hx = hxx * dx + hxy * dy + hxz * dz;
+    const T Hy = hxy * dx + hyy * dy + hyz * dz;
+    const T Hz = hxz * dx + hyz * dy + hzz * dz;
+    o.gap = T(f.x) + first + T(0.5) * (Hx * dx + Hy * dy + Hz * dz);
+    o.grad = V3T<T>{nx + Hx, ny + Hy, nz + Hz};
+    const T len = tsqrt<T>(o.grad.x * o.grad.x + o.grad.y * o.grad.y + o.grad.z * o.grad.z);
+    o.normal = len > T(1e-15) ? V3T<T>{o.grad.x / len, o.grad.y / len, o.grad.z / len}
+                              : V3T<T>{nx, ny, nz};
+    o.valid = true;
+}
+
+// ContactAware candidate selection (SdfOracle.cpp:269-432): no candidate
+// arrays -> tricubic; zero candidates at the voxel -> first order.
+template <typename T>
+__device__ __forceinline__ void contactAwareQuery(const QGrid& g, double px, double py,
+                                                  double pz, QOut<T>& o)
+{
+    int i, j, k;
+    if (!dev::nearestVoxel(g, px, py, pz, i, j, k)) return;
+    const int64_t rec = dev::recordOf(g, i, j, k);
+    if (rec < 0) return;
+    const uint8_t* cnt = g.sparse ? g.sccount : g.ccount;
+    const float4* cp = g.sparse ? g.scpn : g.cpn;
+    const int32_t* cb = g.sparse ? g.scbranch : g.cbranch;
+    int count = int(cnt[rec]);
+    if (count > g.stride) count = g.stride;
+    if (count <= 0)
+    {
+        taylorQuery<T>(g, px, py, pz, false, o);
+        return;
+    }
+    if (count > 8) count = 8;
+    const T dx = T(px - (g.bmin[0] + g.voxel[0] * (i + 0.5)));
+    const T dy = T(py - (g.bmin[1] + g.voxel[1] * (j + 0.5)));
+    const T dz = T(pz - (g.bmin[2] + g.voxel[2] * (k + 0.5)));
+    T best = T(0);
+    bool have = false;
+    for (int c = 0; c < count; ++c)
+    {
+        const float4 f = cp[rec * g.stride + c];
+        T nx = T(f.y), ny = T(f.z), nz = T(f.w);
+        const T nn = tsqrt<T>(nx * nx + ny * ny + nz * nz);
+        if (nn > T(1e-15))
+        {
+            nx /= nn;
+            ny /= nn;
+            nz /= nn;
+        }
+        else
+        {
+            nx = T(1);
+            ny = T(0);
+            nz = T(0);
+        }
+        const T gap = T(f.x) + (nx * dx + ny * dy + nz * dz);
+        const T ag = gap < T(0) ? -gap : gap;
+        if (!have || ag < best)
+        {
+            have = true;
+            best = ag;
+            o.gap = gap;
+            o.normal = V3T<T>{nx, ny, nz};
+            o.branch = cb[rec * g.stride + c];
+        }
+    }
+    o.grad = o.normal;
+    o.valid = true;
+}
+
+template <typename T>
+__device__ __forceinline__ QOut<T> queryPoint(const QGrid& g, int mode, double px, double py,
+                                              double pz)
+{
+    QOut<T> o;
+    o.gap = T(0);
+    o.grad = V3T<T>{T(0), T(0), T(0)};
+    o.normal = V3T<T>{T(1), T(0), T(0)};
+    o.valid = false;
+    o.branch = -1;
+    const bool cubic = mode == SDFO_TRICUBIC || (mode == SDFO_CONTACT_AWARE && g.stride <= 0);
+    if (mode == SDFO_TRILINEAR || cubic)
+    {
+        const Cell c = dev::locateCell(g, px, py, pz);
+        if (!c.valid) return o;
+        Interp<T> it;
+        const bool ok = cubic ? dev::tricubic<T, false>(g, c, it) : dev::trilinear<T, false>(g, c, it);
+        if (ok) finishInterp<T>(it, o);
+        return o;
+    }
+    if (mode == SDFO_CONTACT_AWARE)
+    {
+        contactAwareQuery<T>(g, px, py, pz, o);
+        return o;
+    }
+    taylorQuery<T>(g, px, py, pz, mode != SDFO_FIRST_ORDER, o);
+    return o;
+}
+
+template <typename TP, typename T>
+__global__ void __launch_bounds__(256)
+queryKernel(QGrid g, int mode, const TP* __restrict__ pts, int64_t n, TP* __restrict__ gap,
+            TP* __restrict__ grad3, uint8_t* __restrict__ valid, TP* __restrict__ normal3,
+            TP* __restrict__ witness3, int32_t* __restrict__ branch)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double px = double(pts[3 * i]), py = double(pts[3 * i + 1]), pz = double(pts[3 * i + 2]);
+    const QOut<T> o = queryPoint<T>(g, mode, px, py, pz);
+    gap[i] = TP(o.gap);
+    grad3[3 * i + 0] = TP(o.grad.x);
+    grad3[3 * i + 1] = TP(o.grad.y);
+    grad3[3 * i + 2] = TP(o.grad.z);
+    valid[i] = o.valid ? 1 : 0;
+    if (normal3)
+    {
+        normal3[3 * i + 0] = TP(o.normal.x);
+        normal3[3 * i + 1] = TP(o.normal.y);
+        normal3[3 * i + 2] = TP(o.normal.z);
+    }
+    if (witness3)
+    {
+        // witness = p - gap * normal (SdfOracle.cpp:593); zero when invalid.
+        witness3[3 * i + 0] = o.valid ? TP(T(px) - o.gap * o.normal.x) : TP(0);
+        witness3[3 * i + 1] = o.valid ? TP(T(py) - o.gap * o.normal.y) : TP(0);
+        witness3[3 * i + 2] = o.valid ? TP(T(pz) - o.gap * o.normal.z) : TP(0);
+    }
+    if (branch) branch[i] = o.branch;
+}
+
+} // namespace
+
+void launchQueryF32(const sdfo_grid& g, int mode, const float* pts, int64_t n, float* gap,
+                    float* grad3, uint8_t* valid, float* normal3, float* witness3,
+                    int32_t* branch, hipStream_t s)
+{
+    if (n <= 0) return;
+    const QGrid q = dev::makeQGrid(g);
+    hipLaunchKernelGGL((queryKernel<float, float>), dim3(unsigned((n + 255) / 256)), dim3(256), 0,
+                       s, q, mode, pts, n, gap, grad3, valid, normal3, witness3, branch);
+    SDFO_HIP(hipGetLastError());
+}
+
+void launchQueryF64(const sdfo_grid& g, int mode, const double* pts, int64_t n, double* gap,
+                    double* grad3, uint8_t* valid, double* normal3, double* witness3,
+                    int32_t* branch, hipStream_t s)
+{
+    if (n <= 0) return;
+    const QGrid q = dev::makeQGrid(g);
+    hipLaunchKernelGGL((queryKernel<double, double>), dim3(unsigned((n + 255) / 256)), dim3(256),
+                       0, s, q, mode, pts, n, gap, grad3, valid, normal3, witness3, branch);
+    SDFO_HIP(hipGetLastError());
+}
+
+} // namespace sdfo

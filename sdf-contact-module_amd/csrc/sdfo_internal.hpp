@@ -1,0 +1,196 @@
+// Internal types shared by the host code and the HIP kernels of libsdfo.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "sdfo.h"
+
+namespace sdfo {
+
+// Error carried to the C ABI: code + the reference's message text.
+struct Error : public std::runtime_error
+{
+    int code;
+    Error(int c, const std::string& msg) : std::runtime_error(msg), code(c) {}
+};
+
+void setLastError(const std::string& msg);
+
+#define SDFO_HIP(call)                                                                         \
+    do                                                                                         \
+    {                                                                                          \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            throw ::sdfo::Error(SDFO_ERR_DEVICE, std::string(#call) + ": " +                   \
+                                                     hipGetErrorString(e_));                   \
+    } while (0)
+
+// Cell-centred grid (GridSpec.h:17-60).
+struct GridSpec
+{
+    double bmin[3] = {0, 0, 0};
+    double bmax[3] = {1, 1, 1};
+    double voxel[3] = {1, 1, 1};
+    int n[3] = {1, 1, 1};
+    int64_t total() const { return int64_t(n[0]) * n[1] * n[2]; }
+    double hmax() const
+    {
+        double h = voxel[0];
+        if (h < voxel[1]) h = voxel[1];
+        if (h < voxel[2]) h = voxel[2];
+        return h;
+    }
+};
+
+GridSpec setupGrid(const double* verts, int64_t nv, const sdfo_build_params& p);
+
+// Device BVH node: own box + links (64 bytes, one half cache line).
+struct alignas(16) BvhNode
+{
+    double bmin[3];
+    double bmax[3];
+    int32_t left, right, triStart, triCount;
+};
+static_assert(sizeof(BvhNode) == 64, "BvhNode layout");
+
+// Host BVH, built exactly like KdTree::build (KdTree.h:91-121, 188-242) and
+// flattened: nodes in creation (pre-)order, triangles stored per BVH slot.
+struct HostBvh
+{
+    std::vector<BvhNode> nodes;
+    std::vector<double> slotTri9; // 9 doubles (a, b, c) per slot
+    std::vector<int32_t> slotTri; // original triangle index per slot
+    int depth = 0;
+    double rayDir[3] = {0, 0, 0};
+};
+
+HostBvh buildHostBvh(const double* verts, int64_t nv, const int32_t* tris, int64_t nt);
+
+// Device copy of a BVH.
+struct DeviceBvh
+{
+    BvhNode* nodes = nullptr;
+    double* tri9 = nullptr;
+    int32_t* triIdx = nullptr;
+    int nnodes = 0;
+    int depth = 0;
+    double rayDir[3] = {0, 0, 0};
+    void upload(const HostBvh& h, hipStream_t s);
+    void release();
+};
+
+// Maximal BVH traversal stack per lane (LDS).  Host rejects deeper trees.
+constexpr int kStackDepth = 32;
+
+} // namespace sdfo
+
+// The opaque handle of the C ABI.
+struct sdfo_grid
+{
+    sdfo::GridSpec spec;
+    int device = 0;
+    bool sparse = false;
+    bool hasHessian = true;
+    int stride = 0; // contact-aware candidate stride
+
+    // ---- dense: FP64 authoritative fields (SdfData) --------------------------
+    double* phi = nullptr;     // N
+    double* n3 = nullptr;      // 3N
+    double* h6 = nullptr;      // 6N
+    double* witness3 = nullptr; // 3N
+    int32_t* tri = nullptr;    // N
+    // ---- dense: FP32 query mirror ---------------------------------------------
+    float4* pn = nullptr;      // (phi, nx, ny, nz) per voxel
+    float* h6f = nullptr;      // 6 per voxel
+    // ---- dense contact-aware candidates (FP64 authoritative + FP32 mirror) ------
+    uint8_t* ccount = nullptr;  // N
+    double* cphi = nullptr;     // N*S
+    double* cn3 = nullptr;      // N*S*3
+    int32_t* cbranch = nullptr; // N*S
+    float4* cpn = nullptr;      // N*S (phi, nx, ny, nz)
+
+    // ---- sparse bricks (SparseSdfData) -------------------------------------------
+    int blockDim = 8, halo = 2;
+    double band = 0.0, activation = 0.0;
+    int nb = 0;
+    int nbx = 0, nby = 0, nbz = 0;
+    int32_t* table = nullptr;   // nbx*nby*nbz -> brick or -1
+    uint8_t* activeMask = nullptr; // nbx*nby*nbz
+    int32_t* bijk = nullptr;    // nb*3
+    float* sphi = nullptr;      // nb*B^3
+    float* sn3 = nullptr;       // nb*B^3*3
+    float* sh6 = nullptr;       // nb*B^3*6
+    float4* spn = nullptr;      // nb*B^3 query mirror
+    uint8_t* sccount = nullptr; // nb*B^3
+    float* scphi = nullptr;     // nb*B^3*S
+    float* scn3 = nullptr;      // nb*B^3*S*3
+    int32_t* scbranch = nullptr; // nb*B^3*S
+    float4* scpn = nullptr;     // nb*B^3*S (phi, nx, ny, nz) query mirror
+
+    int64_t deviceBytes = 0;
+    std::vector<void*> allocations;
+
+    template <typename T>
+    T* alloc(size_t count)
+    {
+        if (count == 0) return nullptr;
+        void* p = nullptr;
+        const size_t bytes = count * sizeof(T);
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess)
+            throw sdfo::Error(SDFO_ERR_OUT_OF_MEMORY,
+                              "hipMalloc of " + std::to_string(bytes) + " bytes failed: " +
+                                  hipGetErrorString(e));
+        allocations.push_back(p);
+        deviceBytes += int64_t(bytes);
+        return static_cast<T*>(p);
+    }
+    ~sdfo_grid()
+    {
+        for (void* p : allocations) (void)hipFree(p);
+    }
+};
+
+namespace sdfo {
+
+// Kernel launchers (build.hip).
+void launchDenseFill(const GridSpec& g, const DeviceBvh& bvh, double* phi, double* n3,
+                     double* witness3, int32_t* tri, hipStream_t s);
+void launchDenseHessian(const GridSpec& g, const double* n3, double* h6, hipStream_t s);
+void launchDenseMirror(int64_t N, const double* phi, const double* n3, const double* h6,
+                       float4* pn, float* h6f, hipStream_t s);
+void launchCandidateMirror(int64_t count, const double* cphi, const double* cn3, float4* cpn,
+                           hipStream_t s);
+void launchSparseActivate(const GridSpec& g, const DeviceBvh& bvh, int blockDim, int nbx,
+                          int nby, int nbz, double activation, uint8_t* mask, hipStream_t s);
+void launchSparseFill(const GridSpec& g, const DeviceBvh& bvh, int blockDim, int nb,
+                      const int32_t* bijk, float* sphi, float* sn3, hipStream_t s);
+void launchSparseHessian(const GridSpec& g, int blockDim, int nb, int nbx, int nby, int nbz,
+                         const int32_t* bijk, const int32_t* table, const float* sn3, float* sh6,
+                         hipStream_t s);
+void launchSparseMirror(int64_t count, const float* sphi, const float* sn3, float4* spn,
+                        hipStream_t s);
+
+// Query launchers (query.hip).
+void launchQueryF32(const sdfo_grid& g, int mode, const float* pts, int64_t n, float* gap,
+                    float* grad3, uint8_t* valid, float* normal3, float* witness3,
+                    int32_t* branch, hipStream_t s);
+void launchQueryF64(const sdfo_grid& g, int mode, const double* pts, int64_t n, double* gap,
+                    double* grad3, uint8_t* valid, double* normal3, double* witness3,
+                    int32_t* branch, hipStream_t s);
+
+// Contact launchers (contact.hip).
+void launchContactForward(const sdfo_grid& g, const double* X, const double* w, int64_t n,
+                          const double* poses, int64_t nb, const sdfo_contact_params& p,
+                          double* out, int32_t* active, hipStream_t s);
+void launchContactBackward(const sdfo_grid& g, const double* X, const double* w, int64_t n,
+                           const double* poses, int64_t nb, const sdfo_contact_params& p,
+                           const double* gOut, double* gTwist, double* gParams, double* gX,
+                           double* gW, hipStream_t s);
+
+} // namespace sdfo
