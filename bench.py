@@ -1,0 +1,233 @@
+"""Benchmark of the SdfOracle hot path on one MI355X (ring-on-cube at full size).
+
+One step = (1) dense FP64 build of the torus ring mesh (R 50 mm, r 10 mm,
+256 x 64 segments = 32 768 triangles) at target resolution 128;
+(2) Tricubic value + raw gradient of 10^6 FP32 points uniform in the valid
+region of the 128^3 cube SDF (side 0.1 m, top face at z = 55 mm);
+(3) contact forward (penalty kn = 1e11 order 2, damping 1e4) of the 7 680
+ring-centroid quadrature points (sum w = 1) under 1 000 seeded poses;
+(4) batched contact forward + backward over 256 poses with friction (mu 0.3)
+against the same work done pose by pose (the plain per-pose GPU path).
+
+``value`` is (2) in Mpts/s; the other rates are reported beside it.  Inputs
+are resident in HBM before the timed region.  The cpu_baseline leg times the
+FP64 CPU oracle (oracle/) on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "sdf-contact-module_amd"))
+
+METRIC = "build Mvoxel/s; tricubic value+grad Mpts/s; fwd+bwd contact evals/s vs plain"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--query-points", type=int, default=1_000_000)
+    ap.add_argument("--poses", type=int, default=1000)
+    ap.add_argument("--bwd-poses", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    return ap.parse_args()
+
+
+def cpu_baseline(args, cube_spec, torus_mesh, cube_mesh, pts_np, X, w, poses):
+    """FP64 oracle on bounded samples of the same workload (rank 0, N=1)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    if args.cpu_threads > 0:
+        O.set_threads(args.cpu_threads)
+    threads = O.num_threads()
+    res = {}
+    # build: a 16-slice z slab of the torus grid (same per-voxel work as the full build)
+    tv, tt = torus_mesh
+    tg = O.setup_grid(tv, 0.1, 0.0, 128, 128)
+    slab = tg.centers()[: tg.shape[0] * tg.shape[1] * 16]
+    t0 = time.perf_counter()
+    O.signed_nearest(tv, tt, slab)
+    res["build_mvoxel_per_s"] = len(slab) / (time.perf_counter() - t0) / 1e6
+    # query: the same 1e6 points, FP64 tricubic, on the oracle's cube field
+    cv, ct = cube_mesh
+    cg = O.Grid(cube_spec.bmin, cube_spec.bmax, cube_spec.voxel, cube_spec.shape)
+    cd = O.build_dense(cv, ct, cg)
+    p64 = pts_np.astype(np.float64)
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        O.query_dense(cg, cd, "Tricubic", p64)
+    res["tricubic_mpts_per_s"] = reps * len(p64) / (time.perf_counter() - t0) / 1e6
+    # contact: 200 of the poses
+    sub = poses[:200]
+    t0 = time.perf_counter()
+    O.contact_forward(cg, cd["phi"], cd["n"], X, w, sub, kn=1e11, c=1e4, mu=0.0, reg=1e-6, order=2)
+    res["contact_fwd_evals_per_s"] = len(sub) * len(X) / (time.perf_counter() - t0)
+    return res, threads
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(every box has one GPU); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    import sdfo
+    from sdfo import contact as SC
+    from sdfo import scenes
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    stream = torch.cuda.current_stream()
+
+    torus = scenes.ring_mesh(256, 64)
+    cube = scenes.cube_mesh()
+    cube_grid = sdfo.SdfGrid.build_dense(*cube, target_resolution=128, max_resolution=128)
+    spec = cube_grid.spec
+    pts_np = scenes.valid_region_points(spec, args.query_points, seed=1)
+    pts = torch.from_numpy(pts_np).to(dev)
+    X_np, w_np = scenes.ring_quadrature(120, 32)
+    R, t = scenes.ring_poses(args.poses, seed=0)
+    poses_np = scenes.pose_array(R, t)
+    Rb, tb = scenes.ring_poses(args.bwd_poses, seed=2)
+    v0, Om = scenes.pose_velocities(args.bwd_poses, seed=3)
+    poses_b_np = scenes.pose_array(Rb, tb, v0=v0, Om=Om)
+    X = torch.from_numpy(X_np).to(dev)
+    w = torch.from_numpy(w_np).to(dev)
+    poses = torch.from_numpy(poses_np).to(dev)
+    poses_b = torch.from_numpy(poses_b_np).to(dev)
+    g_out = torch.ones((args.bwd_poses, 6), dtype=torch.float64, device=dev)
+    law = SC.ContactLaw(kn=1e11, damping=1e4, mu=0.0, regularizer=1e-6, order=2)
+    law_f = SC.ContactLaw(kn=1e11, damping=1e4, mu=0.3, regularizer=1e-6, order=2)
+    n_vox_torus = int(np.prod(sdfo.setup_grid(torus[0], target_resolution=128,
+                                              max_resolution=128).shape))
+
+    def ev():
+        return torch.cuda.Event(enable_timing=True)
+
+    def step(rec):
+        # (1) build
+        t0 = time.perf_counter()
+        g = sdfo.SdfGrid.build_dense(*torus, target_resolution=128, max_resolution=128)
+        rec["build_s"] += time.perf_counter() - t0
+        g.close()
+        # (2) 1M-point tricubic query
+        e0, e1 = ev(), ev()
+        e0.record(stream)
+        cube_grid.query(pts, "Tricubic")
+        e1.record(stream)
+        # (3) contact forward, 1000 poses
+        e2, e3 = ev(), ev()
+        e2.record(stream)
+        SC.contact_forward(cube_grid, X, w, poses, law)
+        e3.record(stream)
+        # (4) batched fwd+bwd vs per-pose plain path
+        e4, e5 = ev(), ev()
+        e4.record(stream)
+        SC.contact_forward(cube_grid, X, w, poses_b, law_f)
+        SC.contact_backward(cube_grid, X, w, poses_b, g_out, law_f)
+        e5.record(stream)
+        e6, e7 = ev(), ev()
+        e6.record(stream)
+        for b in range(args.bwd_poses):
+            pb = poses_b[b:b + 1]
+            SC.contact_forward(cube_grid, X, w, pb, law_f)
+            SC.contact_backward(cube_grid, X, w, pb, g_out[b:b + 1], law_f)
+        e7.record(stream)
+        torch.cuda.synchronize()
+        rec["query_ms"].append(e0.elapsed_time(e1))
+        rec["contact_ms"].append(e2.elapsed_time(e3))
+        rec["fwdbwd_ms"].append(e4.elapsed_time(e5))
+        rec["plain_ms"].append(e6.elapsed_time(e7))
+
+    def new_rec():
+        return dict(build_s=0.0, query_ms=[], contact_ms=[], fwdbwd_ms=[], plain_ms=[])
+
+    warm = new_rec()
+    for _ in range(args.warmup):
+        step(warm)
+    rec = new_rec()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(rec)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+
+    K = args.steps
+    q_ms = float(np.mean(rec["query_ms"]))
+    c_ms = float(np.mean(rec["contact_ms"]))
+    fb_ms = float(np.mean(rec["fwdbwd_ms"]))
+    pl_ms = float(np.mean(rec["plain_ms"]))
+    build_s = rec["build_s"] / K
+    nq = args.query_points
+    value = nq / (q_ms * 1e-3) / 1e6
+    # Algorithmic HBM bytes of one query launch: 12 B in + 16 B out + 1 B valid
+    # per point, plus the FP32 field (16 B per voxel) read once.
+    alg_bytes = 29.0 * nq + 16.0 * spec.total
+    achieved = alg_bytes / (q_ms * 1e-3) / 1e9
+    n_eval = args.poses * len(X_np)
+    n_eval_b = args.bwd_poses * len(X_np)
+    line = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Mpts/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / K * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 query / f64 build+contact",
+        "data": "synthetic (seeded meshes, points and poses)",
+        "config": {
+            "workload": "ring-on-cube full size: torus 256x64 (32768 tris) dense build at 128; "
+                        "1e6-pt Tricubic value+grad on cube 128^3; 1000 poses x 7680 pts contact; "
+                        "256-pose fwd+bwd with friction",
+            "query_points": nq, "grid": list(spec.shape), "poses": args.poses,
+            "quadrature_points": len(X_np), "bwd_poses": args.bwd_poses,
+        },
+        "build_mvoxel_per_s": round(n_vox_torus / build_s / 1e6, 3),
+        "build_ms": round(build_s * 1e3, 3),
+        "query_kernel_ms": round(q_ms, 4),
+        "contact_fwd_evals_per_s": round(n_eval / (c_ms * 1e-3), 1),
+        "contact_fwdbwd_evals_per_s": round(n_eval_b / (fb_ms * 1e-3), 1),
+        "plain_fwdbwd_evals_per_s": round(n_eval_b / (pl_ms * 1e-3), 1),
+        "fwdbwd_vs_plain": round(pl_ms / fb_ms, 3),
+        "roofline": {
+            "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "queryKernel<float,float> (Tricubic)",
+            "algorithmic_bytes": alg_bytes,
+        },
+    }
+    if not args.no_cpu:
+        cpu, threads = cpu_baseline(args, spec, torus, cube, pts_np, X_np, w_np, poses_np)
+        line["cpu_baseline"] = {
+            "value": round(cpu["tricubic_mpts_per_s"], 3), "unit": "Mpts/s", "cores": threads,
+            "kind": "port",
+            "sample": "FP64 oracle: Tricubic on the same 1e6 points x3; build rate on a 16-slice "
+                      "slab of the torus grid; contact on 200 of the 1000 poses",
+            "build_mvoxel_per_s": round(cpu["build_mvoxel_per_s"], 4),
+            "contact_fwd_evals_per_s": round(cpu["contact_fwd_evals_per_s"], 1),
+        }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -49,7 +49,7 @@ denseFillKernel(GridSpec g, const BvhNode* __restrict__ nodes, const double* __r
         wit3[3 * idx + 1] = s.cp.y;
         wit3[3 * idx + 2] = s.cp.z;
     }
-    if (tri) tri[idx] = triIdx[s.slot];
+    if (tri) tri[idx] = s.slot >= 0 ? triIdx[s.slot] : -1;
 }
 
 // Central differences of the normal field, clamped at the border, then

@@ -1,0 +1,615 @@
+// C ABI of libsdfo (include/sdfo.h): argument checking, grid handles,
+// builders and launch of the query / contact kernels.  Every entry point
+// converts exceptions into a status code plus a thread-local message.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "sdfo_internal.hpp"
+
+namespace sdfo {
+
+namespace {
+thread_local std::string g_lastError;
+}
+
+void setLastError(const std::string& msg) { g_lastError = msg; }
+
+// Builds the FP32 query mirror of a dense grid whose FP64 fields are on the device.
+void finishDense(sdfo_grid& g, hipStream_t s)
+{
+    const int64_t N = g.spec.total();
+    if (!g.pn) g.pn = g.alloc<float4>(size_t(N));
+    if (!g.h6f) g.h6f = g.alloc<float>(size_t(N) * 6);
+    launchDenseMirror(N, g.phi, g.n3, g.h6, g.pn, g.h6f, s);
+    if (g.stride > 0)
+    {
+        if (!g.cpn) g.cpn = g.alloc<float4>(size_t(N) * g.stride);
+        launchCandidateMirror(N * g.stride, g.cphi, g.cn3, g.cpn, s);
+    }
+}
+
+// Brick table from the brick list: first brick wins on duplicate keys
+// (std::unordered_map::emplace in SparseSdfData::rebuildIndex, SdfData.h:172-182);
+// bricks outside the brick grid are unreachable.
+std::vector<int32_t> brickTable(const std::vector<int32_t>& bijk, int nbx, int nby, int nbz)
+{
+    std::vector<int32_t> table(size_t(nbx) * nby * nbz, -1);
+    const int nb = int(bijk.size() / 3);
+    for (int b = 0; b < nb; ++b)
+    {
+        const int bi = bijk[3 * b], bj = bijk[3 * b + 1], bk = bijk[3 * b + 2];
+        if (bi < 0 || bi >= nbx || bj < 0 || bj >= nby || bk < 0 || bk >= nbz) continue;
+        int32_t& slot = table[bi + size_t(nbx) * (bj + size_t(nby) * bk)];
+        if (slot < 0) slot = b;
+    }
+    return table;
+}
+
+void finishSparse(sdfo_grid& g, const std::vector<int32_t>& bijk, hipStream_t s)
+{
+    const std::vector<int32_t> table = brickTable(bijk, g.nbx, g.nby, g.nbz);
+    if (!g.table) g.table = g.alloc<int32_t>(table.size());
+    SDFO_HIP(hipMemcpyAsync(g.table, table.data(), table.size() * sizeof(int32_t),
+                            hipMemcpyHostToDevice, s));
+    SDFO_HIP(hipStreamSynchronize(s)); // the host table goes out of scope
+    const int64_t bvc = int64_t(g.blockDim) * g.blockDim * g.blockDim;
+    const int64_t count = int64_t(g.nb) * bvc;
+    if (count > 0)
+    {
+        if (!g.spn) g.spn = g.alloc<float4>(size_t(count));
+        launchSparseMirror(count, g.sphi, g.sn3, g.spn, s);
+        if (g.stride > 0)
+        {
+            if (!g.scpn) g.scpn = g.alloc<float4>(size_t(count) * g.stride);
+            launchSparseMirror(count * g.stride, g.scphi, g.scn3, g.scpn, s);
+        }
+    }
+}
+
+namespace {
+
+void checkMesh(const double* verts, int64_t nv, const int32_t* tris, int64_t nt)
+{
+    if (nv <= 0 || nt <= 0 || verts == nullptr || tris == nullptr)
+        throw Error(SDFO_ERR_EMPTY_MESH, "Empty mesh provided to SdfBuilder");
+    if (nt > (int64_t(1) << 30)) throw Error(SDFO_ERR_INVALID_ARGUMENT, "too many triangles");
+}
+
+void checkParams(const sdfo_build_params* p)
+{
+    if (p == nullptr) throw Error(SDFO_ERR_INVALID_ARGUMENT, "params is NULL");
+    if (p->voxel_size <= 0.0 && p->target_resolution <= 0)
+        throw Error(SDFO_ERR_INVALID_ARGUMENT, "target_resolution must be positive");
+    if (p->max_resolution <= 0) throw Error(SDFO_ERR_INVALID_ARGUMENT, "max_resolution must be positive");
+}
+
+int currentDevice()
+{
+    int d = 0;
+    SDFO_HIP(hipGetDevice(&d));
+    return d;
+}
+
+// Keeps the grid's device current for the duration of a call.
+struct DeviceGuard
+{
+    int prev = -1;
+    explicit DeviceGuard(int dev)
+    {
+        int cur = 0;
+        SDFO_HIP(hipGetDevice(&cur));
+        if (cur != dev)
+        {
+            prev = cur;
+            SDFO_HIP(hipSetDevice(dev));
+        }
+    }
+    ~DeviceGuard()
+    {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+std::unique_ptr<sdfo_grid> buildDense(const double* verts, int64_t nv, const int32_t* tris,
+                                      int64_t nt, const sdfo_build_params& p, hipStream_t s)
+{
+    checkMesh(verts, nv, tris, nt);
+    auto g = std::make_unique<sdfo_grid>();
+    g->device = currentDevice();
+    g->spec = setupGrid(verts, nv, p);
+    const int64_t N = g->spec.total();
+    const HostBvh hb = buildHostBvh(verts, nv, tris, nt);
+    DeviceBvh db;
+    try
+    {
+        db.upload(hb, s);
+        g->phi = g->alloc<double>(size_t(N));
+        g->n3 = g->alloc<double>(size_t(N) * 3);
+        g->h6 = g->alloc<double>(size_t(N) * 6);
+        g->witness3 = g->alloc<double>(size_t(N) * 3);
+        g->tri = g->alloc<int32_t>(size_t(N));
+        launchDenseFill(g->spec, db, g->phi, g->n3, g->witness3, g->tri, s);
+        launchDenseHessian(g->spec, g->n3, g->h6, s);
+        finishDense(*g, s);
+        SDFO_HIP(hipStreamSynchronize(s));
+    }
+    catch (...)
+    {
+        (void)hipStreamSynchronize(s);
+        db.release();
+        throw;
+    }
+    db.release();
+    return g;
+}
+
+std::unique_ptr<sdfo_grid> buildSparse(const double* verts, int64_t nv, const int32_t* tris,
+                                       int64_t nt, const sdfo_build_params& p, hipStream_t s)
+{
+    checkMesh(verts, nv, tris, nt);
+    if (p.block_dim <= 0) throw Error(SDFO_ERR_INVALID_ARGUMENT, "Sparse SDF blockDim must be positive");
+    auto g = std::make_unique<sdfo_grid>();
+    g->device = currentDevice();
+    g->sparse = true;
+    g->spec = setupGrid(verts, nv, p);
+    g->blockDim = p.block_dim;
+    g->halo = p.halo_voxels;
+    g->hasHessian = p.store_hessian != 0;
+    // SdfBuilder.cpp:835-842
+    const double h = g->spec.hmax();
+    g->band = p.band_distance > 0.0 ? p.band_distance : double(p.band_voxels) * h;
+    const double blockRadius = 0.5 * std::sqrt(3.0) * double(p.block_dim) * h;
+    g->activation = g->band + double(p.halo_voxels) * h + blockRadius;
+    const int bd = p.block_dim;
+    g->nbx = (g->spec.n[0] + bd - 1) / bd;
+    g->nby = (g->spec.n[1] + bd - 1) / bd;
+    g->nbz = (g->spec.n[2] + bd - 1) / bd;
+    const int64_t cells = int64_t(g->nbx) * g->nby * g->nbz;
+    const HostBvh hb = buildHostBvh(verts, nv, tris, nt);
+    DeviceBvh db;
+    try
+    {
+        db.upload(hb, s);
+        g->activeMask = g->alloc<uint8_t>(size_t(cells));
+        launchSparseActivate(g->spec, db, bd, g->nbx, g->nby, g->nbz, g->activation, g->activeMask, s);
+        std::vector<uint8_t> mask(static_cast<size_t>(cells));
+        SDFO_HIP(hipMemcpyAsync(mask.data(), g->activeMask, mask.size(), hipMemcpyDeviceToHost, s));
+        SDFO_HIP(hipStreamSynchronize(s));
+        // Blocks in (k, j, i) order (SdfBuilder.cpp:892-920).
+        std::vector<int32_t> bijk;
+        for (int64_t c = 0; c < cells; ++c)
+        {
+            if (!mask[size_t(c)]) continue;
+            bijk.push_back(int32_t(c % g->nbx));
+            bijk.push_back(int32_t((c / g->nbx) % g->nby));
+            bijk.push_back(int32_t(c / (int64_t(g->nbx) * g->nby)));
+        }
+        g->nb = int(bijk.size() / 3);
+        const int64_t bvc = int64_t(bd) * bd * bd;
+        const int64_t count = int64_t(g->nb) * bvc;
+        if (g->nb > 0)
+        {
+            g->bijk = g->alloc<int32_t>(bijk.size());
+            SDFO_HIP(hipMemcpyAsync(g->bijk, bijk.data(), bijk.size() * sizeof(int32_t),
+                                    hipMemcpyHostToDevice, s));
+            g->sphi = g->alloc<float>(size_t(count));
+            g->sn3 = g->alloc<float>(size_t(count) * 3);
+            launchSparseFill(g->spec, db, bd, g->nb, g->bijk, g->sphi, g->sn3, s);
+        }
+        finishSparse(*g, bijk, s);
+        if (g->hasHessian && g->nb > 0)
+        {
+            g->sh6 = g->alloc<float>(size_t(count) * 6);
+            SDFO_HIP(hipMemsetAsync(g->sh6, 0, size_t(count) * 6 * sizeof(float), s));
+            launchSparseHessian(g->spec, bd, g->nb, g->nbx, g->nby, g->nbz, g->bijk, g->table,
+                                g->sn3, g->sh6, s);
+        }
+        SDFO_HIP(hipStreamSynchronize(s));
+    }
+    catch (...)
+    {
+        (void)hipStreamSynchronize(s);
+        db.release();
+        throw;
+    }
+    db.release();
+    return g;
+}
+
+template <typename F>
+int guarded(F&& f)
+{
+    try
+    {
+        g_lastError.clear();
+        f();
+        return SDFO_OK;
+    }
+    catch (const Error& e)
+    {
+        g_lastError = e.what();
+        return e.code;
+    }
+    catch (const std::bad_alloc&)
+    {
+        g_lastError = "host allocation failed";
+        return SDFO_ERR_OUT_OF_MEMORY;
+    }
+    catch (const std::exception& e)
+    {
+        g_lastError = e.what();
+        return SDFO_ERR_INVALID_ARGUMENT;
+    }
+    catch (...)
+    {
+        g_lastError = "unknown error";
+        return SDFO_ERR_INVALID_ARGUMENT;
+    }
+}
+
+void fillInfo(const sdfo_grid& g, sdfo_grid_info* info)
+{
+    std::memset(info, 0, sizeof(*info));
+    for (int a = 0; a < 3; ++a)
+    {
+        info->bmin[a] = g.spec.bmin[a];
+        info->bmax[a] = g.spec.bmax[a];
+        info->voxel[a] = g.spec.voxel[a];
+        info->shape[a] = g.spec.n[a];
+    }
+    info->sparse = g.sparse ? 1 : 0;
+    info->has_hessian = g.hasHessian ? 1 : 0;
+    info->candidate_stride = g.stride;
+    info->block_dim = g.sparse ? g.blockDim : 0;
+    info->halo_voxels = g.sparse ? g.halo : 0;
+    info->block_count = g.nb;
+    info->block_grid[0] = g.nbx;
+    info->block_grid[1] = g.nby;
+    info->block_grid[2] = g.nbz;
+    info->band_distance = g.band;
+    info->activation_distance = g.activation;
+    info->device_bytes = g.deviceBytes;
+}
+
+void checkGrid(const sdfo_grid* g)
+{
+    if (g == nullptr) throw Error(SDFO_ERR_INVALID_ARGUMENT, "grid is NULL");
+}
+
+} // namespace
+
+} // namespace sdfo
+
+using namespace sdfo;
+
+extern "C" {
+
+const char* sdfo_version(void) { return "sdfo 0.1.0 (gfx950; SDFO file format 0.1.0)"; }
+
+const char* sdfo_last_error(void) { return sdfo::g_lastError.c_str(); }
+
+void sdfo_default_build_params(sdfo_build_params* p)
+{
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->padding = 0.1;
+    p->voxel_size = 0.0;
+    p->target_resolution = 128;
+    p->max_resolution = 512;
+    p->block_dim = 8;
+    p->halo_voxels = 2;
+    p->band_voxels = 8;
+    p->store_hessian = 1;
+    p->band_distance = 0.0;
+    p->max_candidates = 4;
+    p->tile_dim = 8;
+    p->feature_distance_tolerance = 0.0;
+    p->normal_merge_angle_degrees = 5.0;
+    p->use_tile_acceleration = 1;
+}
+
+int sdfo_setup_grid(const double* verts, int64_t n_verts, const sdfo_build_params* params,
+                    sdfo_grid_info* out)
+{
+    return guarded([&] {
+        if (!verts || n_verts <= 0) throw Error(SDFO_ERR_EMPTY_MESH, "Empty mesh provided to SdfBuilder");
+        checkParams(params);
+        if (!out) throw Error(SDFO_ERR_INVALID_ARGUMENT, "out is NULL");
+        sdfo_grid tmp;
+        tmp.spec = setupGrid(verts, n_verts, *params);
+        fillInfo(tmp, out);
+    });
+}
+
+int sdfo_build_dense(const double* verts, int64_t n_verts, const int32_t* tris, int64_t n_tris,
+                     const sdfo_build_params* params, sdfo_grid** out, sdfo_stream stream)
+{
+    return guarded([&] {
+        if (!out) throw Error(SDFO_ERR_INVALID_ARGUMENT, "out is NULL");
+        *out = nullptr;
+        checkParams(params);
+        *out = buildDense(verts, n_verts, tris, n_tris, *params, hipStream_t(stream)).release();
+    });
+}
+
+int sdfo_build_sparse(const double* verts, int64_t n_verts, const int32_t* tris, int64_t n_tris,
+                      const sdfo_build_params* params, sdfo_grid** out, sdfo_stream stream)
+{
+    return guarded([&] {
+        if (!out) throw Error(SDFO_ERR_INVALID_ARGUMENT, "out is NULL");
+        *out = nullptr;
+        checkParams(params);
+        *out = buildSparse(verts, n_verts, tris, n_tris, *params, hipStream_t(stream)).release();
+    });
+}
+
+void sdfo_grid_free(sdfo_grid* grid)
+{
+    if (!grid) return;
+    try
+    {
+        DeviceGuard dg(grid->device);
+        delete grid;
+    }
+    catch (...)
+    {
+        delete grid;
+    }
+}
+
+int sdfo_grid_get_info(const sdfo_grid* grid, sdfo_grid_info* info)
+{
+    return guarded([&] {
+        checkGrid(grid);
+        if (!info) throw Error(SDFO_ERR_INVALID_ARGUMENT, "info is NULL");
+        fillInfo(*grid, info);
+    });
+}
+
+int sdfo_grid_download_dense(const sdfo_grid* grid, double* phi, double* n3, double* h6,
+                             double* witness3, int32_t* tri)
+{
+    return guarded([&] {
+        checkGrid(grid);
+        if (grid->sparse) throw Error(SDFO_ERR_INVALID_ARGUMENT, "grid is sparse");
+        DeviceGuard dg(grid->device);
+        const size_t N = size_t(grid->spec.total());
+        if (phi) SDFO_HIP(hipMemcpy(phi, grid->phi, N * sizeof(double), hipMemcpyDeviceToHost));
+        if (n3) SDFO_HIP(hipMemcpy(n3, grid->n3, 3 * N * sizeof(double), hipMemcpyDeviceToHost));
+        if (h6) SDFO_HIP(hipMemcpy(h6, grid->h6, 6 * N * sizeof(double), hipMemcpyDeviceToHost));
+        if (witness3)
+            SDFO_HIP(hipMemcpy(witness3, grid->witness3, 3 * N * sizeof(double), hipMemcpyDeviceToHost));
+        if (tri)
+        {
+            if (grid->tri)
+                SDFO_HIP(hipMemcpy(tri, grid->tri, N * sizeof(int32_t), hipMemcpyDeviceToHost));
+            else
+                std::fill(tri, tri + N, -1);
+        }
+    });
+}
+
+int sdfo_grid_download_candidates(const sdfo_grid* grid, uint8_t* count, double* phi, double* n3,
+                                  int32_t* branch)
+{
+    return guarded([&] {
+        checkGrid(grid);
+        if (grid->sparse || grid->stride <= 0)
+            throw Error(SDFO_ERR_INVALID_ARGUMENT, "grid has no dense contact-aware candidates");
+        DeviceGuard dg(grid->device);
+        const size_t N = size_t(grid->spec.total()), NS = N * size_t(grid->stride);
+        if (count) SDFO_HIP(hipMemcpy(count, grid->ccount, N, hipMemcpyDeviceToHost));
+        if (phi) SDFO_HIP(hipMemcpy(phi, grid->cphi, NS * sizeof(double), hipMemcpyDeviceToHost));
+        if (n3) SDFO_HIP(hipMemcpy(n3, grid->cn3, 3 * NS * sizeof(double), hipMemcpyDeviceToHost));
+        if (branch)
+            SDFO_HIP(hipMemcpy(branch, grid->cbranch, NS * sizeof(int32_t), hipMemcpyDeviceToHost));
+    });
+}
+
+int sdfo_grid_download_sparse(const sdfo_grid* grid, int32_t* bijk, float* phi, float* n3,
+                              float* h6, uint8_t* active_mask, uint8_t* ccount, float* cphi,
+                              float* cn3, int32_t* cbranch)
+{
+    return guarded([&] {
+        checkGrid(grid);
+        if (!grid->sparse) throw Error(SDFO_ERR_INVALID_ARGUMENT, "grid is dense");
+        DeviceGuard dg(grid->device);
+        const size_t bvc = size_t(grid->blockDim) * grid->blockDim * grid->blockDim;
+        const size_t cnt = size_t(grid->nb) * bvc;
+        const size_t cells = size_t(grid->nbx) * grid->nby * grid->nbz;
+        const size_t S = size_t(grid->stride);
+        auto cp = [](void* dst, const void* src, size_t bytes) {
+            if (dst && bytes) SDFO_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+        };
+        cp(bijk, grid->bijk, size_t(grid->nb) * 3 * sizeof(int32_t));
+        cp(phi, grid->sphi, cnt * sizeof(float));
+        cp(n3, grid->sn3, cnt * 3 * sizeof(float));
+        if (h6)
+        {
+            if (grid->sh6)
+                cp(h6, grid->sh6, cnt * 6 * sizeof(float));
+            else
+                std::memset(h6, 0, cnt * 6 * sizeof(float));
+        }
+        if (active_mask)
+        {
+            if (grid->activeMask)
+                cp(active_mask, grid->activeMask, cells);
+            else
+                std::memset(active_mask, 0, cells);
+        }
+        if (S > 0)
+        {
+            cp(ccount, grid->sccount, cnt);
+            cp(cphi, grid->scphi, cnt * S * sizeof(float));
+            cp(cn3, grid->scn3, cnt * S * 3 * sizeof(float));
+            cp(cbranch, grid->scbranch, cnt * S * sizeof(int32_t));
+        }
+    });
+}
+
+int sdfo_grid_from_dense(const double* bmin3, const double* bmax3, const double* voxel3,
+                         const int32_t* shape3, const double* phi, const double* n3,
+                         const double* h6, const double* witness3, int32_t stride,
+                         const uint8_t* ccount, const double* cphi, const double* cn3,
+                         const int32_t* cbranch, sdfo_grid** out)
+{
+    return guarded([&] {
+        if (!out || !bmin3 || !bmax3 || !voxel3 || !shape3 || !phi || !n3)
+            throw Error(SDFO_ERR_INVALID_ARGUMENT, "NULL argument");
+        *out = nullptr;
+        if (stride < 0 || stride > 8) throw Error(SDFO_ERR_INVALID_ARGUMENT, "Invalid contact-aware candidate stride");
+        if (stride > 0 && (!ccount || !cphi || !cn3 || !cbranch))
+            throw Error(SDFO_ERR_INVALID_ARGUMENT, "candidate arrays are NULL");
+        auto g = std::make_unique<sdfo_grid>();
+        g->device = currentDevice();
+        for (int a = 0; a < 3; ++a)
+        {
+            g->spec.bmin[a] = bmin3[a];
+            g->spec.bmax[a] = bmax3[a];
+            g->spec.voxel[a] = voxel3[a];
+            g->spec.n[a] = shape3[a];
+            if (shape3[a] <= 0) throw Error(SDFO_ERR_INVALID_ARGUMENT, "shape must be positive");
+        }
+        const size_t N = size_t(g->spec.total());
+        g->phi = g->alloc<double>(N);
+        g->n3 = g->alloc<double>(3 * N);
+        g->h6 = g->alloc<double>(6 * N);
+        g->witness3 = g->alloc<double>(3 * N);
+        SDFO_HIP(hipMemcpy(g->phi, phi, N * sizeof(double), hipMemcpyHostToDevice));
+        SDFO_HIP(hipMemcpy(g->n3, n3, 3 * N * sizeof(double), hipMemcpyHostToDevice));
+        if (h6)
+            SDFO_HIP(hipMemcpy(g->h6, h6, 6 * N * sizeof(double), hipMemcpyHostToDevice));
+        else
+            SDFO_HIP(hipMemset(g->h6, 0, 6 * N * sizeof(double)));
+        if (witness3)
+            SDFO_HIP(hipMemcpy(g->witness3, witness3, 3 * N * sizeof(double), hipMemcpyHostToDevice));
+        else
+            SDFO_HIP(hipMemset(g->witness3, 0, 3 * N * sizeof(double)));
+        g->stride = stride;
+        if (stride > 0)
+        {
+            const size_t NS = N * size_t(stride);
+            g->ccount = g->alloc<uint8_t>(N);
+            g->cphi = g->alloc<double>(NS);
+            g->cn3 = g->alloc<double>(3 * NS);
+            g->cbranch = g->alloc<int32_t>(NS);
+            SDFO_HIP(hipMemcpy(g->ccount, ccount, N, hipMemcpyHostToDevice));
+            SDFO_HIP(hipMemcpy(g->cphi, cphi, NS * sizeof(double), hipMemcpyHostToDevice));
+            SDFO_HIP(hipMemcpy(g->cn3, cn3, 3 * NS * sizeof(double), hipMemcpyHostToDevice));
+            SDFO_HIP(hipMemcpy(g->cbranch, cbranch, NS * sizeof(int32_t), hipMemcpyHostToDevice));
+        }
+        finishDense(*g, nullptr);
+        SDFO_HIP(hipDeviceSynchronize());
+        *out = g.release();
+    });
+}
+
+int sdfo_query(const sdfo_grid* grid, int32_t mode, const float* points, int64_t n, float* gap,
+               float* grad3, uint8_t* valid, float* normal3, float* witness3, int32_t* branch,
+               sdfo_stream stream)
+{
+    return guarded([&] {
+        checkGrid(grid);
+        if (mode < 0 || mode > 4) throw Error(SDFO_ERR_INVALID_ARGUMENT, "invalid query mode");
+        if (n < 0) throw Error(SDFO_ERR_INVALID_ARGUMENT, "negative point count");
+        if (n > 0 && (!points || !gap || !grad3 || !valid))
+            throw Error(SDFO_ERR_INVALID_ARGUMENT, "NULL point or output pointer");
+        DeviceGuard dg(grid->device);
+        launchQueryF32(*grid, mode, points, n, gap, grad3, valid, normal3, witness3, branch,
+                       hipStream_t(stream));
+    });
+}
+
+int sdfo_query_f64(const sdfo_grid* grid, int32_t mode, const double* points, int64_t n,
+                   double* gap, double* grad3, uint8_t* valid, double* normal3, double* witness3,
+                   int32_t* branch, sdfo_stream stream)
+{
+    return guarded([&] {
+        checkGrid(grid);
+        if (mode < 0 || mode > 4) throw Error(SDFO_ERR_INVALID_ARGUMENT, "invalid query mode");
+        if (n < 0) throw Error(SDFO_ERR_INVALID_ARGUMENT, "negative point count");
+        if (n > 0 && (!points || !gap || !grad3 || !valid))
+            throw Error(SDFO_ERR_INVALID_ARGUMENT, "NULL point or output pointer");
+        DeviceGuard dg(grid->device);
+        launchQueryF64(*grid, mode, points, n, gap, grad3, valid, normal3, witness3, branch,
+                       hipStream_t(stream));
+    });
+}
+
+static void checkContact(const sdfo_grid* grid, const double* X, const double* w, int64_t n,
+                         const double* poses, int64_t n_poses, const sdfo_contact_params* p)
+{
+    checkGrid(grid);
+    if (!p) throw Error(SDFO_ERR_INVALID_ARGUMENT, "params is NULL");
+    if (p->mode != SDFO_TRILINEAR && p->mode != SDFO_TRICUBIC)
+        throw Error(SDFO_ERR_UNSUPPORTED, "contact reduction supports Trilinear and Tricubic fields");
+    if (p->order < 0 || p->order > 8) throw Error(SDFO_ERR_INVALID_ARGUMENT, "order must be in [0, 8]");
+    if (n < 0 || n_poses < 0) throw Error(SDFO_ERR_INVALID_ARGUMENT, "negative size");
+    if (n_poses > 65535) throw Error(SDFO_ERR_INVALID_ARGUMENT, "at most 65535 poses per call");
+    if ((n > 0 && (!X || !w)) || (n_poses > 0 && !poses))
+        throw Error(SDFO_ERR_INVALID_ARGUMENT, "NULL input pointer");
+}
+
+int sdfo_contact_forward(const sdfo_grid* grid, const double* X, const double* w, int64_t n,
+                         const double* poses, int64_t n_poses, const sdfo_contact_params* params,
+                         double* out, int32_t* active, sdfo_stream stream)
+{
+    return guarded([&] {
+        checkContact(grid, X, w, n, poses, n_poses, params);
+        if (n_poses > 0 && !out) throw Error(SDFO_ERR_INVALID_ARGUMENT, "out is NULL");
+        DeviceGuard dg(grid->device);
+        launchContactForward(*grid, X, w, n, poses, n_poses, *params, out, active,
+                             hipStream_t(stream));
+    });
+}
+
+int sdfo_contact_backward(const sdfo_grid* grid, const double* X, const double* w, int64_t n,
+                          const double* poses, int64_t n_poses, const sdfo_contact_params* params,
+                          const double* g_out, double* g_twist, double* g_params, double* g_X,
+                          double* g_w, sdfo_stream stream)
+{
+    return guarded([&] {
+        checkContact(grid, X, w, n, poses, n_poses, params);
+        if (n_poses > 0 && !g_out) throw Error(SDFO_ERR_INVALID_ARGUMENT, "g_out is NULL");
+        DeviceGuard dg(grid->device);
+        launchContactBackward(*grid, X, w, n, poses, n_poses, *params, g_out, g_twist, g_params,
+                              g_X, g_w, hipStream_t(stream));
+    });
+}
+
+} // extern "C"
+
+// Entry points whose GPU implementation is not in this build yet.
+extern "C" {
+
+int sdfo_build_contact_aware(const double*, int64_t, const int32_t*, int64_t, const double*,
+                             const sdfo_build_params*, int32_t, sdfo_grid** out, sdfo_stream)
+{
+    return guarded([&] {
+        if (out) *out = nullptr;
+        throw Error(SDFO_ERR_UNSUPPORTED, "contact-aware build is not implemented on the GPU yet");
+    });
+}
+
+int sdfo_build_dense_batched(int32_t, const double*, const int64_t*, const int32_t*,
+                             const int64_t*, const sdfo_build_params*, sdfo_grid**, sdfo_stream)
+{
+    return guarded([&] {
+        throw Error(SDFO_ERR_UNSUPPORTED, "batched build is not implemented yet");
+    });
+}
+
+int sdfo_query_batched(int32_t, const sdfo_grid* const*, int32_t, const int64_t*, const float*,
+                       float*, float*, uint8_t*, sdfo_stream)
+{
+    return guarded([&] {
+        throw Error(SDFO_ERR_UNSUPPORTED, "batched query is not implemented yet");
+    });
+}
+
+} // extern "C"

@@ -1,0 +1,482 @@
+// Per-point penalty / damping / regularised Coulomb friction contact
+// reduction (contacts_sdf.py:329-385) and its hand-written backward.
+//
+// All arithmetic after the field gather is FP64 and every sum has a fixed
+// order (per-lane strided partials, LDS tree, then chunk partials summed in
+// chunk order), so results are bit-reproducible run to run.  The backward
+// recomputes the forward terms per point instead of storing them.
+#include <hip/hip_runtime.h>
+
+#include "interp_device.hpp"
+#include "sdfo_internal.hpp"
+
+namespace sdfo {
+
+namespace dev {
+QGrid makeQGrid(const sdfo_grid& g);
+}
+
+namespace {
+
+using dev::Cell;
+using dev::Interp;
+using dev::QGrid;
+
+constexpr int kBlock = 256;
+
+struct V
+{
+    double x, y, z;
+};
+__device__ __forceinline__ V vadd(V a, V b) { return V{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ V vsub(V a, V b) { return V{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ V vmul(double s, V a) { return V{s * a.x, s * a.y, s * a.z}; }
+__device__ __forceinline__ double vdot(V a, V b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ V vcross(V a, V b)
+{
+    return V{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+struct Pose
+{
+    double R[9], t[3], M[9], m[3], v0[3], Om[3];
+};
+
+__device__ __forceinline__ Pose loadPose(const double* __restrict__ p)
+{
+    Pose P;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) P.R[i] = p[i];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) P.t[i] = p[9 + i];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) P.M[i] = p[12 + i];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+    {
+        P.m[i] = p[21 + i];
+        P.v0[i] = p[24 + i];
+        P.Om[i] = p[27 + i];
+    }
+    return P;
+}
+
+__device__ __forceinline__ V matvec(const double* A, V x)
+{
+    return V{A[0] * x.x + A[1] * x.y + A[2] * x.z, A[3] * x.x + A[4] * x.y + A[5] * x.z,
+             A[6] * x.x + A[7] * x.y + A[8] * x.z};
+}
+__device__ __forceinline__ V matTvec(const double* A, V x)
+{
+    return V{A[0] * x.x + A[3] * x.y + A[6] * x.z, A[1] * x.x + A[4] * x.y + A[7] * x.z,
+             A[2] * x.x + A[5] * x.y + A[8] * x.z};
+}
+
+struct Law
+{
+    double kn, c, mu, reg, margin;
+    int order, mode;
+};
+
+struct Terms
+{
+    bool active, damp, fric;
+    V Y, off, u, G, fp, fd, ft, fq, ut;
+    double g, p, dp, gn, D;
+    double hess[6];
+};
+
+__device__ __forceinline__ double powi(double x, int k)
+{
+    double r = 1.0;
+    for (int i = 0; i < k; ++i) r *= x;
+    return r;
+}
+
+// Forward terms of one (pose, point) pair (contacts_sdf.py:329-379).
+template <bool WithHess>
+__device__ __forceinline__ Terms pointTerms(const QGrid& g, const Pose& P, V X, const Law& L)
+{
+    Terms t;
+    t.active = false;
+    t.Y = vadd(matvec(P.R, X), V{P.t[0], P.t[1], P.t[2]});
+    t.off = vadd(matvec(P.M, X), V{P.m[0], P.m[1], P.m[2]});
+    if (L.margin > 0.0 && !(fabs(t.Y.z) < L.margin)) return t; // contacts_sdf.py:91-98
+    const Cell c = dev::locateCell(g, t.Y.x, t.Y.y, t.Y.z);
+    if (!c.valid) return t;
+    Interp<double> it;
+    const bool ok = L.mode == SDFO_TRILINEAR ? dev::trilinear<double, WithHess>(g, c, it)
+                                             : dev::tricubic<double, WithHess>(g, c, it);
+    if (!ok) return t;
+    t.g = it.gap;
+    const double delta = t.g < 0.0 ? -t.g : 0.0;
+    t.p = L.kn * powi(delta, L.order);
+    if (!(t.p > 1e-30)) return t; // contacts_sdf.py:329
+    t.active = true;
+    t.dp = (L.order >= 1 && t.g < 0.0) ? -L.kn * double(L.order) * powi(delta, L.order - 1) : 0.0;
+    if (WithHess)
+    {
+#pragma unroll
+        for (int e = 0; e < 6; ++e) t.hess[e] = it.hess[e];
+    }
+    t.G = V{it.grad.x, it.grad.y, it.grad.z};
+    t.gn = sqrt(vdot(t.G, t.G));
+    t.u = vadd(V{P.v0[0], P.v0[1], P.v0[2]}, vcross(V{P.Om[0], P.Om[1], P.Om[2]}, t.off));
+    t.fp = vmul(t.p, t.G);
+    t.fd = V{0, 0, 0};
+    t.damp = L.c > 0.0 && t.g < 0.0; // contacts_sdf.py:355-362
+    if (t.damp) t.fd = vmul(L.c * (-vdot(t.u, t.G)), t.G);
+    const double gnc = t.gn > 1e-30 ? t.gn : 1e-30; // contacts_sdf.py:369
+    const V un = V{t.G.x / gnc, t.G.y / gnc, t.G.z / gnc};
+    t.ut = vsub(t.u, vmul(vdot(t.u, un), un));
+    t.D = sqrt(vdot(t.ut, t.ut) + L.reg * L.reg);
+    const double phat = t.p * t.gn;
+    t.ft = V{0, 0, 0};
+    t.fric = L.mu * phat > 0.0; // contacts_sdf.py:375-377
+    if (t.fric)
+    {
+        const double s = -L.mu * phat / t.D;
+        t.ft = vmul(s, t.ut);
+    }
+    t.fq = vadd(vadd(t.fp, t.fd), t.ft);
+    return t;
+}
+
+// Fixed-order LDS tree reduction of NV doubles per lane.
+template <int NV>
+__device__ __forceinline__ void blockReduce(double (&v)[NV], double* sh)
+{
+#pragma unroll
+    for (int k = 0; k < NV; ++k) sh[k * kBlock + threadIdx.x] = v[k];
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1)
+    {
+        if (int(threadIdx.x) < s)
+        {
+#pragma unroll
+            for (int k = 0; k < NV; ++k)
+                sh[k * kBlock + threadIdx.x] += sh[k * kBlock + threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = sh[k * kBlock];
+}
+
+// grid = (chunks, poses).  partial[(pose*chunks + chunk)*7 + k]: F, tau, active.
+__global__ void __launch_bounds__(kBlock)
+contactForwardKernel(QGrid g, Law L, const double* __restrict__ X, const double* __restrict__ w,
+                     int64_t n, const double* __restrict__ poses, int64_t chunkLen,
+                     double* __restrict__ partial)
+{
+    __shared__ double sh[7 * kBlock];
+    const int64_t pose = blockIdx.y;
+    const int chunk = blockIdx.x;
+    const Pose P = loadPose(poses + 30 * pose);
+    const int64_t beg = int64_t(chunk) * chunkLen;
+    const int64_t end = beg + chunkLen < n ? beg + chunkLen : n;
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int64_t i = beg + threadIdx.x; i < end; i += kBlock)
+    {
+        const V x = V{X[3 * i], X[3 * i + 1], X[3 * i + 2]};
+        const Terms t = pointTerms<false>(g, P, x, L);
+        if (!t.active) continue;
+        const double wi = w[i];
+        const V tq = vcross(t.off, t.fq);
+        acc[0] += wi * t.fq.x;
+        acc[1] += wi * t.fq.y;
+        acc[2] += wi * t.fq.z;
+        acc[3] += wi * tq.x;
+        acc[4] += wi * tq.y;
+        acc[5] += wi * tq.z;
+        acc[6] += 1.0;
+    }
+    blockReduce<7>(acc, sh);
+    if (threadIdx.x == 0)
+    {
+        double* o = partial + (pose * gridDim.x + chunk) * 7;
+#pragma unroll
+        for (int k = 0; k < 7; ++k) o[k] = acc[k];
+    }
+}
+
+__global__ void contactForwardFinalize(int64_t nposes, int chunks, const double* __restrict__ partial,
+                                       double* __restrict__ out, int32_t* __restrict__ active)
+{
+    const int64_t pose = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (pose >= nposes) return;
+    double s[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int c = 0; c < chunks; ++c)
+    {
+        const double* p = partial + (pose * chunks + c) * 7;
+#pragma unroll
+        for (int k = 0; k < 7; ++k) s[k] += p[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) out[6 * pose + k] = s[k];
+    if (active) active[pose] = int32_t(s[6]);
+}
+
+// Backward per (pose, point), given a = w (gF + gT x off) -- the derivation
+// is stated next to the CPU restatement; the same formulas are used here.
+struct Back
+{
+    V gY, gOff;
+    double skn, sc, smu;
+};
+
+__device__ __forceinline__ Back pointBackward(const Terms& t, const Law& L, V a, V gT, double wi,
+                                              V Om)
+{
+    Back b;
+    const V G = t.G;
+    const double gn = t.gn;
+    const double aG = vdot(a, G);
+    double sg = t.dp * aG;
+    if (t.fric && t.p != 0.0) sg += (t.dp / t.p) * vdot(a, t.ft);
+    V sG = vmul(t.p, a);
+    V su = V{0, 0, 0};
+    const double uG = vdot(t.u, G);
+    if (t.damp)
+    {
+        sG = vadd(sG, vmul(-L.c, vadd(vmul(aG, t.u), vmul(uG, a))));
+        su = vadd(su, vmul(-L.c * aG, G));
+    }
+    b.smu = 0.0;
+    if (t.fric)
+    {
+        const double k = -L.mu * t.p;
+        const V h = vmul(gn, t.ut);
+        const V ahG = vadd(vsub(vsub(vmul(vdot(a, t.u) / gn, G), vmul(aG / gn, t.u)), vmul(uG / gn, a)),
+                           vmul(uG * aG / (gn * gn * gn), G));
+        const V dD = vmul(-uG / (gn * gn * t.D), t.ut);
+        sG = vadd(sG, vmul(k, vsub(vmul(1.0 / t.D, ahG), vmul(vdot(a, h) / (t.D * t.D), dD))));
+        const V Pa = vsub(a, vmul(aG / (gn * gn), G));
+        su = vadd(su, vmul(k * gn, vsub(vmul(1.0 / t.D, Pa),
+                                        vmul(vdot(a, t.ut) / (t.D * t.D * t.D), t.ut))));
+        b.smu = vdot(a, t.ft) / L.mu;
+    }
+    const double* h6 = t.hess;
+    const V HsG = V{h6[0] * sG.x + h6[1] * sG.y + h6[2] * sG.z,
+                    h6[1] * sG.x + h6[3] * sG.y + h6[4] * sG.z,
+                    h6[2] * sG.x + h6[4] * sG.y + h6[5] * sG.z};
+    b.gY = vadd(vmul(sg, G), HsG);
+    b.gOff = vadd(vmul(wi, vcross(t.fq, gT)), vcross(su, Om));
+    b.skn = L.kn != 0.0 ? vdot(a, vadd(t.fp, t.ft)) / L.kn : 0.0;
+    b.sc = t.damp ? vdot(a, t.fd) / L.c : 0.0;
+    return b;
+}
+
+// Pose-major backward: per pose d/d(omega, dt) and per-pose (kn, c, mu) terms.
+// partial[(pose*chunks + chunk)*9 + k].
+__global__ void __launch_bounds__(kBlock)
+contactBackwardPoseKernel(QGrid g, Law L, const double* __restrict__ X,
+                          const double* __restrict__ w, int64_t n,
+                          const double* __restrict__ poses, const double* __restrict__ gOut,
+                          int64_t chunkLen, double* __restrict__ partial)
+{
+    __shared__ double sh[9 * kBlock];
+    const int64_t pose = blockIdx.y;
+    const int chunk = blockIdx.x;
+    const Pose P = loadPose(poses + 30 * pose);
+    const V gF = V{gOut[6 * pose], gOut[6 * pose + 1], gOut[6 * pose + 2]};
+    const V gT = V{gOut[6 * pose + 3], gOut[6 * pose + 4], gOut[6 * pose + 5]};
+    const V Om = V{P.Om[0], P.Om[1], P.Om[2]};
+    const V tv = V{P.t[0], P.t[1], P.t[2]};
+    const int64_t beg = int64_t(chunk) * chunkLen;
+    const int64_t end = beg + chunkLen < n ? beg + chunkLen : n;
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int64_t i = beg + threadIdx.x; i < end; i += kBlock)
+    {
+        const V x = V{X[3 * i], X[3 * i + 1], X[3 * i + 2]};
+        const Terms t = pointTerms<true>(g, P, x, L);
+        if (!t.active) continue;
+        const double wi = w[i];
+        const V a = vmul(wi, vadd(gF, vcross(gT, t.off)));
+        const Back b = pointBackward(t, L, a, gT, wi, Om);
+        const V gw = vadd(vcross(vsub(t.Y, tv), b.gY), vcross(t.off, b.gOff));
+        acc[0] += gw.x;
+        acc[1] += gw.y;
+        acc[2] += gw.z;
+        acc[3] += b.gY.x;
+        acc[4] += b.gY.y;
+        acc[5] += b.gY.z;
+        acc[6] += b.skn;
+        acc[7] += b.sc;
+        acc[8] += b.smu;
+    }
+    blockReduce<9>(acc, sh);
+    if (threadIdx.x == 0)
+    {
+        double* o = partial + (pose * gridDim.x + chunk) * 9;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) o[k] = acc[k];
+    }
+}
+
+__global__ void contactBackwardPoseFinalize(int64_t nposes, int chunks,
+                                            const double* __restrict__ partial,
+                                            double* __restrict__ gTwist, double* __restrict__ perPose3)
+{
+    const int64_t pose = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (pose >= nposes) return;
+    double s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int c = 0; c < chunks; ++c)
+    {
+        const double* p = partial + (pose * chunks + c) * 9;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) s[k] += p[k];
+    }
+    if (gTwist)
+    {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) gTwist[6 * pose + k] = s[k];
+    }
+    perPose3[3 * pose + 0] = s[6];
+    perPose3[3 * pose + 1] = s[7];
+    perPose3[3 * pose + 2] = s[8];
+}
+
+// Sum of the per-pose parameter terms in pose order (one block).
+__global__ void paramReduceKernel(int64_t nposes, const double* __restrict__ perPose3,
+                                  double* __restrict__ gParams)
+{
+    __shared__ double sh[3 * kBlock];
+    double acc[3] = {0, 0, 0};
+    for (int64_t p = threadIdx.x; p < nposes; p += kBlock)
+    {
+        acc[0] += perPose3[3 * p];
+        acc[1] += perPose3[3 * p + 1];
+        acc[2] += perPose3[3 * p + 2];
+    }
+    blockReduce<3>(acc, sh);
+    if (threadIdx.x == 0)
+    {
+        gParams[0] = acc[0];
+        gParams[1] = acc[1];
+        gParams[2] = acc[2];
+    }
+}
+
+// Point-major backward: d/dX and d/dw summed over poses in pose order.
+__global__ void __launch_bounds__(kBlock)
+contactBackwardPointKernel(QGrid g, Law L, const double* __restrict__ X,
+                           const double* __restrict__ w, int64_t n,
+                           const double* __restrict__ poses, int64_t nposes,
+                           const double* __restrict__ gOut, double* __restrict__ gX,
+                           double* __restrict__ gW)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const V x = V{X[3 * i], X[3 * i + 1], X[3 * i + 2]};
+    const double wi = w[i];
+    V ax = V{0, 0, 0};
+    double aw = 0.0;
+    for (int64_t pose = 0; pose < nposes; ++pose)
+    {
+        const Pose P = loadPose(poses + 30 * pose);
+        const Terms t = pointTerms<true>(g, P, x, L);
+        if (!t.active) continue;
+        const V gF = V{gOut[6 * pose], gOut[6 * pose + 1], gOut[6 * pose + 2]};
+        const V gT = V{gOut[6 * pose + 3], gOut[6 * pose + 4], gOut[6 * pose + 5]};
+        const V a = vmul(wi, vadd(gF, vcross(gT, t.off)));
+        const Back b = pointBackward(t, L, a, gT, wi, V{P.Om[0], P.Om[1], P.Om[2]});
+        ax = vadd(ax, vadd(matTvec(P.R, b.gY), matTvec(P.M, b.gOff)));
+        aw += vdot(gF, t.fq) + vdot(gT, vcross(t.off, t.fq));
+    }
+    if (gX)
+    {
+        gX[3 * i] = ax.x;
+        gX[3 * i + 1] = ax.y;
+        gX[3 * i + 2] = ax.z;
+    }
+    if (gW) gW[i] = aw;
+}
+
+Law makeLaw(const sdfo_contact_params& p)
+{
+    return Law{p.kn, p.damping, p.mu, p.regularizer, p.narrow_margin, p.order, p.mode};
+}
+
+// Split each pose's points into chunks so that the launch holds at least
+// ~2048 workgroups (8 per CU) when there are few poses.
+int chunksFor(int64_t n, int64_t nposes)
+{
+    int64_t want = (2048 + nposes - 1) / nposes;
+    const int64_t maxc = (n + kBlock - 1) / kBlock;
+    if (want > maxc) want = maxc;
+    if (want < 1) want = 1;
+    if (want > 65535) want = 65535;
+    return int(want);
+}
+
+struct Scratch
+{
+    void* p = nullptr;
+    hipStream_t s;
+    explicit Scratch(size_t bytes, hipStream_t st) : s(st)
+    {
+        SDFO_HIP(hipMallocAsync(&p, bytes, s));
+    }
+    ~Scratch()
+    {
+        if (p) (void)hipFreeAsync(p, s);
+    }
+};
+
+} // namespace
+
+void launchContactForward(const sdfo_grid& g, const double* X, const double* w, int64_t n,
+                          const double* poses, int64_t nb, const sdfo_contact_params& p,
+                          double* out, int32_t* active, hipStream_t s)
+{
+    if (nb <= 0) return;
+    const QGrid q = dev::makeQGrid(g);
+    const Law L = makeLaw(p);
+    const int chunks = n > 0 ? chunksFor(n, nb) : 1;
+    const int64_t chunkLen = n > 0 ? (n + chunks - 1) / chunks : 0;
+    Scratch part(sizeof(double) * 7 * size_t(nb) * chunks, s);
+    hipLaunchKernelGGL(contactForwardKernel, dim3(chunks, unsigned(nb)), dim3(kBlock), 0, s, q, L,
+                       X, w, n, poses, chunkLen, static_cast<double*>(part.p));
+    SDFO_HIP(hipGetLastError());
+    hipLaunchKernelGGL(contactForwardFinalize, dim3(unsigned((nb + 255) / 256)), dim3(256), 0, s,
+                       nb, chunks, static_cast<const double*>(part.p), out, active);
+    SDFO_HIP(hipGetLastError());
+}
+
+void launchContactBackward(const sdfo_grid& g, const double* X, const double* w, int64_t n,
+                           const double* poses, int64_t nb, const sdfo_contact_params& p,
+                           const double* gOut, double* gTwist, double* gParams, double* gX,
+                           double* gW, hipStream_t s)
+{
+    if (nb <= 0) return;
+    const QGrid q = dev::makeQGrid(g);
+    const Law L = makeLaw(p);
+    if (gTwist || gParams)
+    {
+        const int chunks = n > 0 ? chunksFor(n, nb) : 1;
+        const int64_t chunkLen = n > 0 ? (n + chunks - 1) / chunks : 0;
+        Scratch part(sizeof(double) * (9 * size_t(nb) * chunks + 3 * size_t(nb)), s);
+        double* partial = static_cast<double*>(part.p);
+        double* perPose = partial + 9 * size_t(nb) * chunks;
+        hipLaunchKernelGGL(contactBackwardPoseKernel, dim3(chunks, unsigned(nb)), dim3(kBlock), 0,
+                           s, q, L, X, w, n, poses, gOut, chunkLen, partial);
+        SDFO_HIP(hipGetLastError());
+        hipLaunchKernelGGL(contactBackwardPoseFinalize, dim3(unsigned((nb + 255) / 256)), dim3(256),
+                           0, s, nb, chunks, partial, gTwist, perPose);
+        SDFO_HIP(hipGetLastError());
+        if (gParams)
+        {
+            hipLaunchKernelGGL(paramReduceKernel, dim3(1), dim3(kBlock), 0, s, nb, perPose, gParams);
+            SDFO_HIP(hipGetLastError());
+        }
+    }
+    if ((gX || gW) && n > 0)
+    {
+        hipLaunchKernelGGL(contactBackwardPointKernel, dim3(unsigned((n + kBlock - 1) / kBlock)),
+                           dim3(kBlock), 0, s, q, L, X, w, n, poses, nb, gOut, gX, gW);
+        SDFO_HIP(hipGetLastError());
+    }
+}
+
+} // namespace sdfo

@@ -23,7 +23,7 @@ struct D3
     double x, y, z;
 };
 
-__device__ __forceinline__ D3 d3(double x, double y, double z) { return D3{x, y, z}; }
+__host__ __device__ __forceinline__ D3 d3(double x, double y, double z) { return D3{x, y, z}; }
 __device__ __forceinline__ D3 sub(D3 a, D3 b) { return D3{a.x - b.x, a.y - b.y, a.z - b.z}; }
 __device__ __forceinline__ D3 add(D3 a, D3 b) { return D3{a.x + b.x, a.y + b.y, a.z + b.z}; }
 __device__ __forceinline__ D3 scale(double s, D3 a) { return D3{s * a.x, s * a.y, s * a.z}; }
@@ -220,12 +220,24 @@ __device__ inline SignedNearest signedNearest(D3 p, const BvhNode* __restrict__ 
 
     SignedNearest r;
     r.slot = bestSlot;
-    const Tri t = loadTri(tri9, bestSlot);
-    r.cp = closestPoint(p, t.a, t.b, t.c);
-    const double dist = sqrt(bestSq);
-    const D3 n = cross(sub(t.b, t.a), sub(t.c, t.a));
-    const double nLen = sqrt(sqnorm(n));
-    r.face = nLen > 1.0e-15 ? divs(n, nLen) : d3(0.0, 0.0, 1.0);
+    double dist;
+    if (bestSlot >= 0)
+    {
+        const Tri t = loadTri(tri9, bestSlot);
+        r.cp = closestPoint(p, t.a, t.b, t.c);
+        dist = sqrt(bestSq);
+        const D3 n = cross(sub(t.b, t.a), sub(t.c, t.a));
+        const double nLen = sqrt(sqnorm(n));
+        r.face = nLen > 1.0e-15 ? divs(n, nLen) : d3(0.0, 0.0, 1.0);
+    }
+    else
+    {
+        // No triangle compared (non-finite point): ClosestPointResult defaults
+        // with distance = DBL_MAX (KdTree.h:18-24, 125-127).
+        r.cp = d3(0.0, 0.0, 0.0);
+        r.face = d3(0.0, 0.0, 1.0);
+        dist = DBL_MAX;
+    }
 
     int hits = 0;
     sp = 0;

@@ -37,8 +37,8 @@ struct GridSpec
     double bmax[3] = {1, 1, 1};
     double voxel[3] = {1, 1, 1};
     int n[3] = {1, 1, 1};
-    int64_t total() const { return int64_t(n[0]) * n[1] * n[2]; }
-    double hmax() const
+    __host__ __device__ int64_t total() const { return int64_t(n[0]) * n[1] * n[2]; }
+    __host__ __device__ double hmax() const
     {
         double h = voxel[0];
         if (h < voxel[1]) h = voxel[1];
@@ -157,6 +157,11 @@ struct sdfo_grid
 };
 
 namespace sdfo {
+
+// Grid completion helpers (capi.cpp): FP32 mirrors and the brick table.
+void finishDense(sdfo_grid& g, hipStream_t s);
+void finishSparse(sdfo_grid& g, const std::vector<int32_t>& bijk, hipStream_t s);
+std::vector<int32_t> brickTable(const std::vector<int32_t>& bijk, int nbx, int nby, int nbz);
 
 // Kernel launchers (build.hip).
 void launchDenseFill(const GridSpec& g, const DeviceBvh& bvh, double* phi, double* n3,

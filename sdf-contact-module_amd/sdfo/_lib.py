@@ -1,0 +1,157 @@
+"""ctypes binding of libsdfo (include/sdfo.h).
+
+The library is built in-tree (``sdf-contact-module_amd/sdfo/_lib/libsdfo.so``).
+There is no fallback: importing this module raises if the library is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libsdfo.so")
+
+SDFO_OK = 0
+ERRORS = {
+    -1: "SDFO_ERR_INVALID_ARGUMENT",
+    -2: "SDFO_ERR_EMPTY_MESH",
+    -3: "SDFO_ERR_IO",
+    -4: "SDFO_ERR_FORMAT",
+    -5: "SDFO_ERR_DEVICE",
+    -6: "SDFO_ERR_OUT_OF_MEMORY",
+    -7: "SDFO_ERR_UNSUPPORTED",
+}
+
+MODES = {"FirstOrder": 0, "SecondOrder": 1, "Trilinear": 2, "Tricubic": 3, "ContactAware": 4}
+
+
+class BuildParams(C.Structure):
+    _fields_ = [
+        ("padding", C.c_double),
+        ("voxel_size", C.c_double),
+        ("target_resolution", C.c_int32),
+        ("max_resolution", C.c_int32),
+        ("block_dim", C.c_int32),
+        ("halo_voxels", C.c_int32),
+        ("band_voxels", C.c_int32),
+        ("store_hessian", C.c_int32),
+        ("band_distance", C.c_double),
+        ("max_candidates", C.c_int32),
+        ("tile_dim", C.c_int32),
+        ("feature_distance_tolerance", C.c_double),
+        ("normal_merge_angle_degrees", C.c_double),
+        ("use_tile_acceleration", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
+class GridInfo(C.Structure):
+    _fields_ = [
+        ("bmin", C.c_double * 3),
+        ("bmax", C.c_double * 3),
+        ("voxel", C.c_double * 3),
+        ("shape", C.c_int32 * 3),
+        ("sparse", C.c_int32),
+        ("has_hessian", C.c_int32),
+        ("candidate_stride", C.c_int32),
+        ("block_dim", C.c_int32),
+        ("halo_voxels", C.c_int32),
+        ("block_count", C.c_int32),
+        ("block_grid", C.c_int32 * 3),
+        ("band_distance", C.c_double),
+        ("activation_distance", C.c_double),
+        ("device_bytes", C.c_int64),
+    ]
+
+
+class ContactParams(C.Structure):
+    _fields_ = [
+        ("kn", C.c_double),
+        ("damping", C.c_double),
+        ("mu", C.c_double),
+        ("regularizer", C.c_double),
+        ("narrow_margin", C.c_double),
+        ("order", C.c_int32),
+        ("mode", C.c_int32),
+    ]
+
+
+class SdfoError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__(f"{ERRORS.get(code, code)}: {message}")
+        self.code = code
+        self.message = message
+
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        f"libsdfo is not built ({LIB_PATH} missing); run __graft_entry__.build() "
+        "or `make -C sdf-contact-module_amd`")
+
+# torch is loaded first: its bundled HIP runtime has the same soname
+# (libamdhip64.so.7) as the one libsdfo links against, so the library binds to
+# the runtime that owns torch's device memory and streams instead of loading a
+# second HIP runtime into the process.
+import torch  # noqa: E402,F401
+
+lib = C.CDLL(LIB_PATH)
+
+_vp = C.c_void_p
+_i64 = C.c_int64
+_i32 = C.c_int32
+_P = C.POINTER
+
+
+def _sig(name, args, res=C.c_int):
+    f = getattr(lib, name)
+    f.argtypes = args
+    f.restype = res
+    return f
+
+
+_sig("sdfo_version", [], C.c_char_p)
+_sig("sdfo_last_error", [], C.c_char_p)
+_sig("sdfo_default_build_params", [_P(BuildParams)], None)
+_sig("sdfo_setup_grid", [_vp, _i64, _P(BuildParams), _P(GridInfo)])
+_sig("sdfo_build_dense", [_vp, _i64, _vp, _i64, _P(BuildParams), _P(_vp), _vp])
+_sig("sdfo_build_sparse", [_vp, _i64, _vp, _i64, _P(BuildParams), _P(_vp), _vp])
+_sig("sdfo_build_contact_aware", [_vp, _i64, _vp, _i64, _vp, _P(BuildParams), _i32, _P(_vp), _vp])
+_sig("sdfo_build_dense_batched", [_i32, _vp, _vp, _vp, _vp, _P(BuildParams), _P(_vp), _vp])
+_sig("sdfo_grid_free", [_vp], None)
+_sig("sdfo_grid_get_info", [_vp, _P(GridInfo)])
+_sig("sdfo_grid_download_dense", [_vp, _vp, _vp, _vp, _vp, _vp])
+_sig("sdfo_grid_download_candidates", [_vp, _vp, _vp, _vp, _vp])
+_sig("sdfo_grid_download_sparse", [_vp] + [_vp] * 9)
+_sig("sdfo_grid_save", [_vp, C.c_char_p])
+_sig("sdfo_grid_load", [C.c_char_p, _P(_vp)])
+_sig("sdfo_is_valid_file", [C.c_char_p])
+_sig("sdfo_is_sparse_file", [C.c_char_p])
+_sig("sdfo_grid_from_dense", [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _vp,
+                              _P(_vp)])
+_sig("sdfo_query", [_vp, _i32, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp])
+_sig("sdfo_query_f64", [_vp, _i32, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp])
+_sig("sdfo_query_batched", [_i32, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp])
+_sig("sdfo_contact_forward", [_vp, _vp, _vp, _i64, _vp, _i64, _P(ContactParams), _vp, _vp, _vp])
+_sig("sdfo_contact_backward", [_vp, _vp, _vp, _i64, _vp, _i64, _P(ContactParams), _vp, _vp, _vp,
+                               _vp, _vp, _vp])
+
+EXPORTED = [n for n in dir(lib) if n.startswith("sdfo_")]
+
+
+def check(rc: int) -> None:
+    if rc != SDFO_OK:
+        raise SdfoError(rc, lib.sdfo_last_error().decode(errors="replace"))
+
+
+def version() -> str:
+    return lib.sdfo_version().decode()
+
+
+def default_build_params(**overrides) -> BuildParams:
+    p = BuildParams()
+    lib.sdfo_default_build_params(C.byref(p))
+    for k, v in overrides.items():
+        if not hasattr(p, k):
+            raise TypeError(f"unknown builder parameter {k!r}")
+        setattr(p, k, v)
+    return p

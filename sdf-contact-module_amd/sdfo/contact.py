@@ -1,0 +1,187 @@
+"""Per-pose penalty / damping / friction contact reduction on the GPU.
+
+Mirrors the per-point law and sums of ``SDFContactEngine.evaluate``
+(prototype/src/dynamics/contacts_sdf.py:300-385) evaluated in the SDF body's
+frame, batched over poses, with a hand-written backward
+(``ContactFunction``, a ``torch.autograd.Function``).
+
+Pose record (30 float64 per pose, see include/sdfo.h):
+``R(9) t(3)``: quadrature point X (body A frame) -> SDF frame, ``Y = R X + t``;
+``M(9) m(3)``: offset from A's centre of mass, ``off = M X + m``;
+``v0(3) Om(3)``: relative velocity ``u = v0 + Om x off``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import torch
+
+from . import _lib
+from ._lib import MODES, check, lib
+from .grid import SdfGrid, _stream_handle
+
+
+@dataclass
+class ContactLaw:
+    """ContactPair law parameters (ir/contact.py:1-18, contacts_sdf.py:300-305)."""
+
+    kn: float = 1e11
+    damping: float = 1e4
+    mu: float = 0.0
+    regularizer: float = 1e-6
+    narrow_margin: float = 0.0
+    order: int = 2
+    mode: str = "Tricubic"
+
+    def ctypes(self) -> _lib.ContactParams:
+        return _lib.ContactParams(float(self.kn), float(self.damping), float(self.mu),
+                                  float(self.regularizer), float(self.narrow_margin),
+                                  int(self.order), int(MODES.get(self.mode, self.mode)))
+
+
+def pose_records(R, t, M=None, m=None, v0=None, Om=None):
+    """Packs (B,3,3)/(B,3) tensors into (B,30) float64 pose records.
+
+    Defaults: ``M = R`` and ``m = 0`` (torque about the frame origin of A),
+    zero relative velocity."""
+    R = torch.as_tensor(R, dtype=torch.float64)
+    t = torch.as_tensor(t, dtype=torch.float64, device=R.device)
+    B = R.shape[0]
+    M = R if M is None else torch.as_tensor(M, dtype=torch.float64, device=R.device)
+    z = torch.zeros((B, 3), dtype=torch.float64, device=R.device)
+    m = z if m is None else torch.as_tensor(m, dtype=torch.float64, device=R.device)
+    v0 = z if v0 is None else torch.as_tensor(v0, dtype=torch.float64, device=R.device)
+    Om = z if Om is None else torch.as_tensor(Om, dtype=torch.float64, device=R.device)
+    return torch.cat([R.reshape(B, 9), t.reshape(B, 3), M.reshape(B, 9), m.reshape(B, 3),
+                      v0.reshape(B, 3), Om.reshape(B, 3)], dim=1).contiguous()
+
+
+def _ptr(t):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def _check_inputs(X, w, poses):
+    for name, t in (("X", X), ("w", w), ("poses", poses)):
+        if not t.is_cuda or t.dtype != torch.float64 or not t.is_contiguous():
+            raise ValueError(f"{name} must be a contiguous float64 GPU tensor")
+    if X.dim() != 2 or X.shape[1] != 3 or w.shape != (X.shape[0],):
+        raise ValueError("X must be (N,3) and w (N,)")
+    if poses.dim() != 2 or poses.shape[1] != 30:
+        raise ValueError("poses must be (B,30)")
+
+
+def contact_forward(grid: SdfGrid, X, w, poses, law: ContactLaw = ContactLaw(), stream=None,
+                    active=False):
+    """(F_A, tau_A) per pose as a (B,6) float64 tensor (SDF frame)."""
+    _check_inputs(X, w, poses)
+    B = poses.shape[0]
+    out = torch.empty((B, 6), dtype=torch.float64, device=X.device)
+    act = torch.empty(B, dtype=torch.int32, device=X.device) if active else None
+    p = law.ctypes()
+    check(lib.sdfo_contact_forward(grid.handle, _ptr(X), _ptr(w), X.shape[0], _ptr(poses), B,
+                                   C.byref(p), _ptr(out), _ptr(act), _stream_handle(stream)))
+    return (out, act) if active else out
+
+
+def contact_backward(grid: SdfGrid, X, w, poses, g_out, law: ContactLaw = ContactLaw(),
+                     want_twist=True, want_params=True, want_X=True, want_w=True, stream=None):
+    """Gradients of sum(g_out * forward): (g_twist (B,6) [omega, dt] left twist,
+    g_params (3,) [kn, damping, mu], g_X (N,3), g_w (N,))."""
+    _check_inputs(X, w, poses)
+    g_out = g_out.contiguous().to(torch.float64)
+    B, N = poses.shape[0], X.shape[0]
+    dev = X.device
+    gt = torch.empty((B, 6), dtype=torch.float64, device=dev) if want_twist else None
+    gp = torch.empty(3, dtype=torch.float64, device=dev) if want_params else None
+    gx = torch.empty((N, 3), dtype=torch.float64, device=dev) if want_X else None
+    gw = torch.empty(N, dtype=torch.float64, device=dev) if want_w else None
+    p = law.ctypes()
+    check(lib.sdfo_contact_backward(grid.handle, _ptr(X), _ptr(w), N, _ptr(poses), B, C.byref(p),
+                                    _ptr(g_out), _ptr(gt), _ptr(gp), _ptr(gx), _ptr(gw),
+                                    _stream_handle(stream)))
+    return gt, gp, gx, gw
+
+
+def _hat(v):
+    z = torch.zeros_like(v[..., 0])
+    return torch.stack([torch.stack([z, -v[..., 2], v[..., 1]], -1),
+                        torch.stack([v[..., 2], z, -v[..., 0]], -1),
+                        torch.stack([-v[..., 1], v[..., 0], z], -1)], -2)
+
+
+def so3_exp(omega):
+    """Rodrigues: exp([omega]x) for (B,3) float64."""
+    th = torch.linalg.vector_norm(omega, dim=-1, keepdim=True)[..., None]
+    K = _hat(omega)
+    small = th < 1e-8
+    ths = torch.where(small, torch.ones_like(th), th)
+    a = torch.where(small, 1.0 - th ** 2 / 6.0, torch.sin(ths) / ths)
+    b = torch.where(small, 0.5 - th ** 2 / 24.0, (1.0 - torch.cos(ths)) / ths ** 2)
+    eye = torch.eye(3, dtype=omega.dtype, device=omega.device).expand_as(K)
+    return eye + a * K + b * (K @ K)
+
+
+def so3_left_jacobian(omega):
+    """J_l(omega) = I + (1 - cos t)/t^2 [w] + (t - sin t)/t^3 [w]^2."""
+    th = torch.linalg.vector_norm(omega, dim=-1, keepdim=True)[..., None]
+    K = _hat(omega)
+    small = th < 1e-6
+    ths = torch.where(small, torch.ones_like(th), th)
+    a = torch.where(small, 0.5 - th ** 2 / 24.0, (1.0 - torch.cos(ths)) / ths ** 2)
+    b = torch.where(small, 1.0 / 6.0 - th ** 2 / 120.0, (ths - torch.sin(ths)) / ths ** 3)
+    eye = torch.eye(3, dtype=omega.dtype, device=omega.device).expand_as(K)
+    return eye + a * K + b * (K @ K)
+
+
+class ContactFunction(torch.autograd.Function):
+    """F, tau (B,6) of the contact reduction as a function of a per-pose twist
+    ``(omega, dt)`` applied to base poses ``R = exp([omega]) R0``, ``t = t0 + dt``
+    (torque about A's origin, ``off = R X``), of ``kn``, ``damping``, ``mu``, of
+    the quadrature points ``X`` and weights ``w``."""
+
+    @staticmethod
+    def forward(ctx, twist, kn, damping, mu, X, w, grid, R0, t0, v0, Om, regularizer, order,
+                mode):
+        R = so3_exp(twist[:, :3]) @ R0
+        t = t0 + twist[:, 3:]
+        poses = pose_records(R, t, v0=v0, Om=Om)
+        law = ContactLaw(float(kn), float(damping), float(mu), float(regularizer), 0.0, int(order),
+                         mode)
+        Xc = X.detach().contiguous()
+        wc = w.detach().contiguous()
+        out = contact_forward(grid, Xc, wc, poses, law)
+        ctx.save_for_backward(twist, Xc, wc, poses)
+        ctx.grid = grid
+        ctx.law = law
+        return out
+
+    @staticmethod
+    def backward(ctx, g_out):
+        twist, X, w, poses = ctx.saved_tensors
+        need = ctx.needs_input_grad
+        gt, gp, gx, gw = contact_backward(ctx.grid, X, w, poses, g_out, ctx.law,
+                                          want_twist=need[0], want_params=any(need[1:4]),
+                                          want_X=need[4], want_w=need[5])
+        g_twist = None
+        if need[0]:
+            Jl = so3_left_jacobian(twist[:, :3])
+            g_omega = (Jl.transpose(1, 2) @ gt[:, :3, None])[..., 0]
+            g_twist = torch.cat([g_omega, gt[:, 3:]], dim=1)
+        gkn = gp[0] if need[1] else None
+        gc = gp[1] if need[2] else None
+        gmu = gp[2] if need[3] else None
+        return (g_twist, gkn, gc, gmu, gx if need[4] else None, gw if need[5] else None, None,
+                None, None, None, None, None, None, None)
+
+
+def contact(grid, twist, kn, damping, mu, X, w, R0, t0, v0=None, Om=None, regularizer=1e-6,
+            order=2, mode="Tricubic"):
+    """Differentiable batched contact: returns (B,6) = (F_A, tau_A) per pose."""
+    B = R0.shape[0]
+    z = torch.zeros((B, 3), dtype=torch.float64, device=R0.device)
+    v0 = z if v0 is None else v0
+    Om = z if Om is None else Om
+    as_t = lambda v: v if torch.is_tensor(v) else torch.tensor(float(v), dtype=torch.float64)  # noqa: E731
+    return ContactFunction.apply(twist, as_t(kn), as_t(damping), as_t(mu), X, w, grid, R0, t0, v0,
+                                 Om, regularizer, order, mode)

@@ -1,0 +1,93 @@
+"""GPU build parity: dense and sparse SDF builds against the FP64 oracle.
+
+The build kernels run FP64 without FMA contraction and traverse the same BVH
+in the same near-first order as SdfBuilder / KdTree, so every field is
+compared bit for bit (north-star rule (a): nearest-triangle indices and the
+sparse brick mask bit-exact; here the values are bit-exact too).
+"""
+import numpy as np
+import pytest
+
+from conftest import ograd
+from sdfo import meshes
+
+pytestmark = pytest.mark.gpu
+
+DENSE_CASES = [
+    ("cube_64", lambda: meshes.cube(0.1), 64),
+    ("cube_offcentre_32", lambda: meshes.cube(0.07, (0.01, -0.02, 0.03)), 32),
+    ("torus_48", lambda: meshes.torus(0.05, 0.01, 64, 16), 48),
+    ("sphere_32", lambda: meshes.uv_sphere(0.05, 12, 24), 32),
+    ("gear_64", lambda: meshes.gear(16, 8, 0.02, 0.002, 4), 64),
+    ("hollow_cylinder_32", lambda: meshes.hollow_cylinder(0.04, 0.03, 0.05, 48, 6), 32),
+]
+
+
+@pytest.mark.parametrize("name,mk,res", DENSE_CASES, ids=[c[0] for c in DENSE_CASES])
+def test_dense_build_bit_exact(O, sdfo_gpu, name, mk, res):
+    v, t = mk()
+    print(f"[gpu] dense build {name} res={res} tris={len(t)}", flush=True)
+    g = sdfo_gpu.SdfGrid.build_dense(v, t, target_resolution=res, max_resolution=res)
+    og = O.setup_grid(v, 0.1, 0.0, res, res)
+    assert tuple(g.spec.shape) == og.shape
+    assert np.array_equal(g.spec.bmin, og.bmin)
+    assert np.array_equal(g.spec.bmax, og.bmax)
+    assert np.array_equal(g.spec.voxel, og.voxel)
+    ref = O.build_dense(v, t, og)
+    got = g.download_dense()
+    assert np.array_equal(got["tri"], ref["tri"]), "nearest-triangle index"
+    for key in ("phi", "n", "witness", "H"):
+        assert np.array_equal(got[key], ref[key]), key
+    g.close()
+
+
+def test_dense_build_non_cubic_grid(O, sdfo_gpu):
+    # a flat box: the ladder snaps each axis separately (GridSpec.h:65-74)
+    v, t = meshes.cube(0.1)
+    v = v * np.array([1.0, 0.3, 0.1])
+    g = sdfo_gpu.SdfGrid.build_dense(v, t, target_resolution=40, max_resolution=64)
+    og = O.setup_grid(v, 0.1, 0.0, 40, 64)
+    assert tuple(g.spec.shape) == og.shape and len(set(og.shape)) > 1
+    ref = O.build_dense(v, t, og)
+    got = g.download_dense()
+    for key in ("phi", "n", "witness", "H", "tri"):
+        assert np.array_equal(got[key], ref[key]), key
+
+
+SPARSE_CASES = [
+    ("hollow_cylinder_64_b8", lambda: meshes.hollow_cylinder(0.04, 0.03, 0.05, 64, 8), 64, 8, 4, 2),
+    ("torus_64_b8", lambda: meshes.torus(0.05, 0.01, 64, 16), 64, 8, 2, 1),
+    ("sphere_48_b6", lambda: meshes.uv_sphere(0.05, 12, 24), 48, 6, 3, 2),
+]
+
+
+@pytest.mark.parametrize("name,mk,res,bd,band,halo", SPARSE_CASES, ids=[c[0] for c in SPARSE_CASES])
+def test_sparse_build_bit_exact(O, sdfo_gpu, name, mk, res, bd, band, halo):
+    v, t = mk()
+    print(f"[gpu] sparse build {name}", flush=True)
+    g = sdfo_gpu.SdfGrid.build_sparse(v, t, target_resolution=res, max_resolution=res,
+                                      block_dim=bd, band_voxels=band, halo_voxels=halo)
+    og = O.setup_grid(v, 0.1, 0.0, res, res)
+    ref = O.build_sparse(v, t, og, block_dim=bd, halo=halo, band_voxels=band)
+    got = g.download_sparse()
+    assert g.info.activation_distance == ref["activation"]
+    assert g.info.band_distance == ref["band"]
+    assert np.array_equal(got["active"], ref["active"]), "brick activation mask"
+    assert np.array_equal(got["bijk"], ref["bijk"])
+    assert 0 < len(got["bijk"]) < len(got["active"])
+    for key in ("phi", "n", "H"):
+        assert np.array_equal(got[key], ref[key]), key
+    g.close()
+
+
+def test_empty_mesh_rejected(sdfo_gpu):
+    with pytest.raises(sdfo_gpu.SdfoError, match="Empty mesh"):
+        sdfo_gpu.SdfGrid.build_dense(np.zeros((0, 3)), np.zeros((0, 3), np.int32))
+
+
+def test_invalid_vertex_rejected(sdfo_gpu):
+    v, t = meshes.cube(0.1)
+    t = t.copy()
+    t[3, 1] = 99
+    with pytest.raises(sdfo_gpu.SdfoError, match="invalid vertex"):
+        sdfo_gpu.SdfGrid.build_dense(v, t, target_resolution=16, max_resolution=16)
