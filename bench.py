@@ -41,6 +41,10 @@ def parse():
     ap.add_argument("--bwd-poses", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--parts", default="build,query,contact,bwd",
+                    help="subset of the step to run (profiling)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run one 256 MiB device copy first (scales HBM counters)")
     return ap.parse_args()
 
 
@@ -119,40 +123,49 @@ def main():
     def ev():
         return torch.cuda.Event(enable_timing=True)
 
+    parts = set(args.parts.split(","))
+    if args.copy_calibration:
+        buf = torch.empty(256 << 20, dtype=torch.uint8, device=dev)
+        buf.clone()
+        torch.cuda.synchronize()
+        del buf
+
     def step(rec):
         # (1) build
-        t0 = time.perf_counter()
-        g = sdfo.SdfGrid.build_dense(*torus, target_resolution=128, max_resolution=128)
-        rec["build_s"] += time.perf_counter() - t0
-        g.close()
+        if "build" in parts:
+            t0 = time.perf_counter()
+            g = sdfo.SdfGrid.build_dense(*torus, target_resolution=128, max_resolution=128)
+            rec["build_s"] += time.perf_counter() - t0
+            g.close()
+        e = [ev() for _ in range(8)]
         # (2) 1M-point tricubic query
-        e0, e1 = ev(), ev()
-        e0.record(stream)
-        cube_grid.query(pts, "Tricubic")
-        e1.record(stream)
+        e[0].record(stream)
+        if "query" in parts:
+            cube_grid.query(pts, "Tricubic")
+        e[1].record(stream)
         # (3) contact forward, 1000 poses
-        e2, e3 = ev(), ev()
-        e2.record(stream)
-        SC.contact_forward(cube_grid, X, w, poses, law)
-        e3.record(stream)
+        e[2].record(stream)
+        if "contact" in parts:
+            SC.contact_forward(cube_grid, X, w, poses, law)
+        e[3].record(stream)
         # (4) batched fwd+bwd vs per-pose plain path
-        e4, e5 = ev(), ev()
-        e4.record(stream)
-        SC.contact_forward(cube_grid, X, w, poses_b, law_f)
-        SC.contact_backward(cube_grid, X, w, poses_b, g_out, law_f)
-        e5.record(stream)
-        e6, e7 = ev(), ev()
-        e6.record(stream)
-        for b in range(args.bwd_poses):
-            pb = poses_b[b:b + 1]
-            SC.contact_forward(cube_grid, X, w, pb, law_f)
-            SC.contact_backward(cube_grid, X, w, pb, g_out[b:b + 1], law_f)
-        e7.record(stream)
+        e[4].record(stream)
+        if "bwd" in parts:
+            SC.contact_forward(cube_grid, X, w, poses_b, law_f)
+            SC.contact_backward(cube_grid, X, w, poses_b, g_out, law_f)
+        e[5].record(stream)
+        e[6].record(stream)
+        if "bwd" in parts:
+            for b in range(args.bwd_poses):
+                pb = poses_b[b:b + 1]
+                SC.contact_forward(cube_grid, X, w, pb, law_f)
+                SC.contact_backward(cube_grid, X, w, pb, g_out[b:b + 1], law_f)
+        e[7].record(stream)
         torch.cuda.synchronize()
-        rec["query_ms"].append(e0.elapsed_time(e1))
-        rec["contact_ms"].append(e2.elapsed_time(e3))
-        rec["fwdbwd_ms"].append(e4.elapsed_time(e5))
-        rec["plain_ms"].append(e6.elapsed_time(e7))
+        rec["query_ms"].append(e[0].elapsed_time(e[1]))
+        rec["contact_ms"].append(e[2].elapsed_time(e[3]))
+        rec["fwdbwd_ms"].append(e[4].elapsed_time(e[5]))
+        rec["plain_ms"].append(e[6].elapsed_time(e[7]))
 
     def new_rec():
         return dict(build_s=0.0, query_ms=[], contact_ms=[], fwdbwd_ms=[], plain_ms=[])
@@ -169,11 +182,11 @@ def main():
     wall = time.perf_counter() - t0
 
     K = args.steps
-    q_ms = float(np.mean(rec["query_ms"]))
-    c_ms = float(np.mean(rec["contact_ms"]))
-    fb_ms = float(np.mean(rec["fwdbwd_ms"]))
-    pl_ms = float(np.mean(rec["plain_ms"]))
-    build_s = rec["build_s"] / K
+    q_ms = max(float(np.mean(rec["query_ms"])), 1e-9)
+    c_ms = max(float(np.mean(rec["contact_ms"])), 1e-9)
+    fb_ms = max(float(np.mean(rec["fwdbwd_ms"])), 1e-9)
+    pl_ms = max(float(np.mean(rec["plain_ms"])), 1e-9)
+    build_s = max(rec["build_s"] / K, 1e-12)
     nq = args.query_points
     value = nq / (q_ms * 1e-3) / 1e6
     # Algorithmic HBM bytes of one query launch: 12 B in + 16 B out + 1 B valid

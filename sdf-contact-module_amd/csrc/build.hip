@@ -98,6 +98,22 @@ __global__ void denseMirrorKernel(int64_t N, const double* __restrict__ phi,
     }
 }
 
+// Packs the 8 corner records of every cell contiguously; one lane per
+// (cell, corner) so both the gathers and the stores are coalesced.
+__global__ void cellPackKernel(GridSpec g, const float4* __restrict__ pn, float4* __restrict__ rec)
+{
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int cx = g.n[0] - 1, cy = g.n[1] - 1, cz = g.n[2] - 1;
+    const int64_t cells = int64_t(cx) * cy * cz;
+    if (t >= cells * 8) return;
+    const int64_t cell = t >> 3;
+    const int q = int(t & 7);
+    const int i = int(cell % cx) + (q & 1);
+    const int j = int((cell / cx) % cy) + ((q >> 1) & 1);
+    const int k = int(cell / (int64_t(cx) * cy)) + (q >> 2);
+    rec[t] = pn[i + int64_t(g.n[0]) * (j + int64_t(g.n[1]) * k)];
+}
+
 __global__ void candidateMirrorKernel(int64_t count, const double* __restrict__ cphi,
                                       const double* __restrict__ cn3, float4* __restrict__ cpn)
 {
@@ -233,6 +249,15 @@ void launchDenseMirror(int64_t N, const double* phi, const double* n3, const dou
 {
     hipLaunchKernelGGL(denseMirrorKernel, dim3(blocksFor(N, 256)), dim3(256), 0, s, N, phi, n3,
                        h6, pn, h6f);
+    SDFO_HIP(hipGetLastError());
+}
+
+void launchCellPack(const GridSpec& g, const float4* pn, float4* cellRec, hipStream_t s)
+{
+    const int64_t cells = int64_t(g.n[0] - 1) * (g.n[1] - 1) * (g.n[2] - 1);
+    if (cells <= 0) return;
+    hipLaunchKernelGGL(cellPackKernel, dim3(blocksFor(cells * 8, 256)), dim3(256), 0, s, g, pn,
+                       cellRec);
     SDFO_HIP(hipGetLastError());
 }
 

@@ -25,6 +25,12 @@ void finishDense(sdfo_grid& g, hipStream_t s)
     if (!g.pn) g.pn = g.alloc<float4>(size_t(N));
     if (!g.h6f) g.h6f = g.alloc<float>(size_t(N) * 6);
     launchDenseMirror(N, g.phi, g.n3, g.h6, g.pn, g.h6f, s);
+    const int64_t cells = int64_t(g.spec.n[0] - 1) * (g.spec.n[1] - 1) * (g.spec.n[2] - 1);
+    if (cells > 0 && cells * 8 * int64_t(sizeof(float4)) <= kCellRecordLimit)
+    {
+        if (!g.cellRec) g.cellRec = g.alloc<float4>(size_t(cells) * 8);
+        launchCellPack(g.spec, g.pn, g.cellRec, s);
+    }
     if (g.stride > 0)
     {
         if (!g.cpn) g.cpn = g.alloc<float4>(size_t(N) * g.stride);

@@ -267,16 +267,20 @@ __device__ __forceinline__ Back pointBackward(const Terms& t, const Law& L, V a,
     return b;
 }
 
-// Pose-major backward: per pose d/d(omega, dt) and per-pose (kn, c, mu) terms.
-// partial[(pose*chunks + chunk)*9 + k].
+// Pose-major backward: per pose d/d(omega, dt) and per-pose (kn, c, mu) terms
+// in partial[(pose*chunks + chunk)*9 + k]; when pointPartial is given, every
+// (pose, point) pair also writes its d/dX (3) and d/dw (1) contribution to
+// pointPartial[(pose - poseBase) * n + point] (zero for inactive pairs), to be
+// summed over poses in pose order by pointReduceKernel.
 __global__ void __launch_bounds__(kBlock)
 contactBackwardPoseKernel(QGrid g, Law L, const double* __restrict__ X,
                           const double* __restrict__ w, int64_t n,
                           const double* __restrict__ poses, const double* __restrict__ gOut,
-                          int64_t chunkLen, double* __restrict__ partial)
+                          int64_t chunkLen, int64_t poseBase, double* __restrict__ partial,
+                          double4* __restrict__ pointPartial)
 {
     __shared__ double sh[9 * kBlock];
-    const int64_t pose = blockIdx.y;
+    const int64_t pose = poseBase + blockIdx.y;
     const int chunk = blockIdx.x;
     const Pose P = loadPose(poses + 30 * pose);
     const V gF = V{gOut[6 * pose], gOut[6 * pose + 1], gOut[6 * pose + 2]};
@@ -286,11 +290,16 @@ contactBackwardPoseKernel(QGrid g, Law L, const double* __restrict__ X,
     const int64_t beg = int64_t(chunk) * chunkLen;
     const int64_t end = beg + chunkLen < n ? beg + chunkLen : n;
     double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double4* pp = pointPartial ? pointPartial + int64_t(blockIdx.y) * n : nullptr;
     for (int64_t i = beg + threadIdx.x; i < end; i += kBlock)
     {
         const V x = V{X[3 * i], X[3 * i + 1], X[3 * i + 2]};
         const Terms t = pointTerms<true>(g, P, x, L);
-        if (!t.active) continue;
+        if (!t.active)
+        {
+            if (pp) pp[i] = make_double4(0.0, 0.0, 0.0, 0.0);
+            continue;
+        }
         const double wi = w[i];
         const V a = vmul(wi, vadd(gF, vcross(gT, t.off)));
         const Back b = pointBackward(t, L, a, gT, wi, Om);
@@ -304,6 +313,11 @@ contactBackwardPoseKernel(QGrid g, Law L, const double* __restrict__ X,
         acc[6] += b.skn;
         acc[7] += b.sc;
         acc[8] += b.smu;
+        if (pp)
+        {
+            const V gx = vadd(matTvec(P.R, b.gY), matTvec(P.M, b.gOff));
+            pp[i] = make_double4(gx.x, gx.y, gx.z, vdot(gF, t.fq) + vdot(gT, vcross(t.off, t.fq)));
+        }
     }
     blockReduce<9>(acc, sh);
     if (threadIdx.x == 0)
@@ -312,6 +326,37 @@ contactBackwardPoseKernel(QGrid g, Law L, const double* __restrict__ X,
 #pragma unroll
         for (int k = 0; k < 9; ++k) o[k] = acc[k];
     }
+}
+
+// Sums the per-pose point partials of one pose group in pose order and adds
+// them to (or, for the first group, stores them in) gX / gW.
+__global__ void pointReduceKernel(int64_t n, int poses, const double4* __restrict__ pointPartial,
+                                  int accumulate, double* __restrict__ gX, double* __restrict__ gW)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double sx = 0.0, sy = 0.0, sz = 0.0, sw = 0.0;
+    for (int p = 0; p < poses; ++p)
+    {
+        const double4 v = pointPartial[int64_t(p) * n + i];
+        sx += v.x;
+        sy += v.y;
+        sz += v.z;
+        sw += v.w;
+    }
+    if (gX)
+    {
+        if (accumulate)
+        {
+            sx = gX[3 * i] + sx;
+            sy = gX[3 * i + 1] + sy;
+            sz = gX[3 * i + 2] + sz;
+        }
+        gX[3 * i] = sx;
+        gX[3 * i + 1] = sy;
+        gX[3 * i + 2] = sz;
+    }
+    if (gW) gW[i] = accumulate ? gW[i] + sw : sw;
 }
 
 __global__ void contactBackwardPoseFinalize(int64_t nposes, int chunks,
@@ -356,41 +401,6 @@ __global__ void paramReduceKernel(int64_t nposes, const double* __restrict__ per
         gParams[1] = acc[1];
         gParams[2] = acc[2];
     }
-}
-
-// Point-major backward: d/dX and d/dw summed over poses in pose order.
-__global__ void __launch_bounds__(kBlock)
-contactBackwardPointKernel(QGrid g, Law L, const double* __restrict__ X,
-                           const double* __restrict__ w, int64_t n,
-                           const double* __restrict__ poses, int64_t nposes,
-                           const double* __restrict__ gOut, double* __restrict__ gX,
-                           double* __restrict__ gW)
-{
-    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const V x = V{X[3 * i], X[3 * i + 1], X[3 * i + 2]};
-    const double wi = w[i];
-    V ax = V{0, 0, 0};
-    double aw = 0.0;
-    for (int64_t pose = 0; pose < nposes; ++pose)
-    {
-        const Pose P = loadPose(poses + 30 * pose);
-        const Terms t = pointTerms<true>(g, P, x, L);
-        if (!t.active) continue;
-        const V gF = V{gOut[6 * pose], gOut[6 * pose + 1], gOut[6 * pose + 2]};
-        const V gT = V{gOut[6 * pose + 3], gOut[6 * pose + 4], gOut[6 * pose + 5]};
-        const V a = vmul(wi, vadd(gF, vcross(gT, t.off)));
-        const Back b = pointBackward(t, L, a, gT, wi, V{P.Om[0], P.Om[1], P.Om[2]});
-        ax = vadd(ax, vadd(matTvec(P.R, b.gY), matTvec(P.M, b.gOff)));
-        aw += vdot(gF, t.fq) + vdot(gT, vcross(t.off, t.fq));
-    }
-    if (gX)
-    {
-        gX[3 * i] = ax.x;
-        gX[3 * i + 1] = ax.y;
-        gX[3 * i + 2] = ax.z;
-    }
-    if (gW) gW[i] = aw;
 }
 
 Law makeLaw(const sdfo_contact_params& p)
@@ -452,29 +462,43 @@ void launchContactBackward(const sdfo_grid& g, const double* X, const double* w,
     if (nb <= 0) return;
     const QGrid q = dev::makeQGrid(g);
     const Law L = makeLaw(p);
-    if (gTwist || gParams)
+    const bool wantPoint = (gX || gW) && n > 0;
+    const int chunks = n > 0 ? chunksFor(n, nb) : 1;
+    const int64_t chunkLen = n > 0 ? (n + chunks - 1) / chunks : 0;
+    // Poses per group bounded so the point partials stay under 256 MiB.
+    int64_t group = nb;
+    if (wantPoint)
     {
-        const int chunks = n > 0 ? chunksFor(n, nb) : 1;
-        const int64_t chunkLen = n > 0 ? (n + chunks - 1) / chunks : 0;
-        Scratch part(sizeof(double) * (9 * size_t(nb) * chunks + 3 * size_t(nb)), s);
-        double* partial = static_cast<double*>(part.p);
-        double* perPose = partial + 9 * size_t(nb) * chunks;
-        hipLaunchKernelGGL(contactBackwardPoseKernel, dim3(chunks, unsigned(nb)), dim3(kBlock), 0,
-                           s, q, L, X, w, n, poses, gOut, chunkLen, partial);
+        const int64_t cap = (int64_t(256) << 20) / (n * int64_t(sizeof(double4)));
+        group = cap < 1 ? 1 : (cap < nb ? cap : nb);
+    }
+    Scratch part(sizeof(double) * (9 * size_t(nb) * chunks + 3 * size_t(nb)), s);
+    double* partial = static_cast<double*>(part.p);
+    double* perPose = partial + 9 * size_t(nb) * chunks;
+    Scratch pts(wantPoint ? sizeof(double4) * size_t(group) * size_t(n) : 16, s);
+    double4* pointPartial = wantPoint ? static_cast<double4*>(pts.p) : nullptr;
+    for (int64_t base = 0; base < nb; base += group)
+    {
+        const int64_t cnt = base + group < nb ? group : nb - base;
+        hipLaunchKernelGGL(contactBackwardPoseKernel, dim3(chunks, unsigned(cnt)), dim3(kBlock), 0,
+                           s, q, L, X, w, n, poses, gOut, chunkLen, base, partial, pointPartial);
         SDFO_HIP(hipGetLastError());
-        hipLaunchKernelGGL(contactBackwardPoseFinalize, dim3(unsigned((nb + 255) / 256)), dim3(256),
-                           0, s, nb, chunks, partial, gTwist, perPose);
-        SDFO_HIP(hipGetLastError());
-        if (gParams)
+        if (wantPoint)
         {
-            hipLaunchKernelGGL(paramReduceKernel, dim3(1), dim3(kBlock), 0, s, nb, perPose, gParams);
+            hipLaunchKernelGGL(pointReduceKernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s,
+                               n, int(cnt), pointPartial, base > 0 ? 1 : 0, gX, gW);
             SDFO_HIP(hipGetLastError());
         }
     }
-    if ((gX || gW) && n > 0)
+    if (gTwist || gParams)
     {
-        hipLaunchKernelGGL(contactBackwardPointKernel, dim3(unsigned((n + kBlock - 1) / kBlock)),
-                           dim3(kBlock), 0, s, q, L, X, w, n, poses, nb, gOut, gX, gW);
+        hipLaunchKernelGGL(contactBackwardPoseFinalize, dim3(unsigned((nb + 255) / 256)), dim3(256),
+                           0, s, nb, chunks, partial, gTwist, perPose);
+        SDFO_HIP(hipGetLastError());
+    }
+    if (gParams)
+    {
+        hipLaunchKernelGGL(paramReduceKernel, dim3(1), dim3(kBlock), 0, s, nb, perPose, gParams);
         SDFO_HIP(hipGetLastError());
     }
 }

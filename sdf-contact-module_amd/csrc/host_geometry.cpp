@@ -5,6 +5,7 @@
 #include <cmath>
 #include <limits>
 #include <numeric>
+#include <thread>
 
 #include "sdfo_internal.hpp"
 
@@ -58,16 +59,99 @@ GridSpec setupGrid(const double* verts, int64_t nv, const sdfo_build_params& p)
     return g;
 }
 
+namespace {
+
+// Node count of the subtree built over n triangles: the split is always at
+// n / 2, so the tree shape depends on n only.
+int64_t subtreeNodes(int64_t n) { return n <= 8 ? 1 : 1 + subtreeNodes(n / 2) + subtreeNodes(n - n / 2); }
+
+struct BvhBuilder
+{
+    std::vector<BvhNode>& nodes;
+    std::vector<int>& idx;
+    const std::vector<double>& box; // bmin3, bmax3 per triangle
+    const std::vector<double>& cen; // centroid component per axis: cen[axis * n + t]
+    int64_t n;
+
+    // Builds the subtree over idx[lo, hi) whose root has pre-order index me;
+    // returns the subtree depth.  Subtrees above `par` triangles are built
+    // concurrently (their node ranges are disjoint and their index ranges too).
+    int build(int lo, int hi, int64_t me, int level, int64_t par)
+    {
+        const double big = std::numeric_limits<double>::max();
+        double bmn[3] = {big, big, big}, bmx[3] = {-big, -big, -big};
+        double cmn[3] = {big, big, big}, cmx[3] = {-big, -big, -big};
+        for (int i = lo; i < hi; ++i)
+        {
+            const int t = idx[i];
+            const double* r = &box[6 * size_t(t)];
+            for (int k = 0; k < 3; ++k)
+            {
+                bmn[k] = minOf(bmn[k], r[k]);
+                bmx[k] = maxOf(bmx[k], r[3 + k]);
+                const double c = cen[size_t(k) * n + t];
+                cmn[k] = minOf(cmn[k], c);
+                cmx[k] = maxOf(cmx[k], c);
+            }
+        }
+        BvhNode nd{};
+        for (int k = 0; k < 3; ++k)
+        {
+            nd.bmin[k] = bmn[k];
+            nd.bmax[k] = bmx[k];
+        }
+        nd.left = nd.right = -1;
+        nd.triStart = -1;
+        nd.triCount = 0;
+        const int count = hi - lo;
+        if (count <= 8)
+        {
+            nd.triStart = lo;
+            nd.triCount = count;
+            nodes[me] = nd;
+            return level;
+        }
+        const double ext[3] = {cmx[0] - cmn[0], cmx[1] - cmn[1], cmx[2] - cmn[2]};
+        int axis = 0;
+        if (ext[1] > ext[0]) axis = 1;
+        if (ext[2] > ext[axis]) axis = 2;
+        const int mid = lo + count / 2;
+        const double* ca = &cen[size_t(axis) * n];
+        std::nth_element(idx.begin() + lo, idx.begin() + mid, idx.begin() + hi,
+                         [ca](int x, int y) { return ca[x] < ca[y]; });
+        const int64_t left = me + 1;
+        const int64_t right = me + 1 + subtreeNodes(mid - lo);
+        nd.left = int(left);
+        nd.right = int(right);
+        nodes[me] = nd;
+        int dl, dr;
+        if (count > par)
+        {
+            std::thread th([&] { dl = build(lo, mid, left, level + 1, par); });
+            dr = build(mid, hi, right, level + 1, par);
+            th.join();
+        }
+        else
+        {
+            dl = build(lo, mid, left, level + 1, par);
+            dr = build(mid, hi, right, level + 1, par);
+        }
+        return dl > dr ? dl : dr;
+    }
+};
+
+} // namespace
+
 // KdTree::build / buildRecursive (KdTree.h:91-121, 188-242): object-median
 // split on the largest centroid extent via std::nth_element, leaves of <= 8.
-// Node order is the recursion's creation order so indices match the
-// reference tree; the traversal order (near child first) is reproduced by
-// the kernels.
+// Node indices are the recursion's pre-order creation order, so the tree is
+// node-for-node the reference tree; the traversal order (near child first)
+// is reproduced by the kernels.
 HostBvh buildHostBvh(const double* verts, int64_t nv, const int32_t* tris, int64_t nt)
 {
     HostBvh h;
     const int n = int(nt);
-    std::vector<double> tb(size_t(n) * 9); // bmin3, bmax3, centroid3 per triangle
+    std::vector<double> box(size_t(n) * 6), cen(size_t(n) * 3);
     for (int t = 0; t < n; ++t)
     {
         const int32_t* ix = tris + 3 * size_t(t);
@@ -77,77 +161,23 @@ HostBvh buildHostBvh(const double* verts, int64_t nv, const int32_t* tris, int64
         const double* a = verts + 3 * size_t(ix[0]);
         const double* b = verts + 3 * size_t(ix[1]);
         const double* c = verts + 3 * size_t(ix[2]);
-        double* o = &tb[9 * size_t(t)];
         for (int k = 0; k < 3; ++k)
         {
-            o[k] = minOf(minOf(a[k], b[k]), c[k]);
-            o[3 + k] = maxOf(maxOf(a[k], b[k]), c[k]);
-            o[6 + k] = ((a[k] + b[k]) + c[k]) / 3.0;
+            box[6 * size_t(t) + k] = minOf(minOf(a[k], b[k]), c[k]);
+            box[6 * size_t(t) + 3 + k] = maxOf(maxOf(a[k], b[k]), c[k]);
+            cen[size_t(k) * n + t] = ((a[k] + b[k]) + c[k]) / 3.0;
         }
     }
     std::vector<int> idx(n);
     std::iota(idx.begin(), idx.end(), 0);
-    h.nodes.reserve(2 * size_t(n));
-
-    struct Rec
+    if (n > 0)
     {
-        HostBvh& h;
-        std::vector<int>& idx;
-        const std::vector<double>& tb;
-        int build(int lo, int hi, int depth)
-        {
-            if (depth > h.depth) h.depth = depth;
-            const int me = int(h.nodes.size());
-            h.nodes.emplace_back();
-            const double big = std::numeric_limits<double>::max();
-            double bmn[3] = {big, big, big}, bmx[3] = {-big, -big, -big};
-            double cmn[3] = {big, big, big}, cmx[3] = {-big, -big, -big};
-            for (int i = lo; i < hi; ++i)
-            {
-                const double* r = &tb[9 * size_t(idx[i])];
-                for (int k = 0; k < 3; ++k)
-                {
-                    bmn[k] = minOf(bmn[k], r[k]);
-                    bmx[k] = maxOf(bmx[k], r[3 + k]);
-                    cmn[k] = minOf(cmn[k], r[6 + k]);
-                    cmx[k] = maxOf(cmx[k], r[6 + k]);
-                }
-            }
-            BvhNode nd{};
-            for (int k = 0; k < 3; ++k)
-            {
-                nd.bmin[k] = bmn[k];
-                nd.bmax[k] = bmx[k];
-            }
-            nd.left = nd.right = -1;
-            nd.triStart = -1;
-            nd.triCount = 0;
-            const int count = hi - lo;
-            if (count <= 8)
-            {
-                nd.triStart = lo;
-                nd.triCount = count;
-                h.nodes[me] = nd;
-                return me;
-            }
-            const double ext[3] = {cmx[0] - cmn[0], cmx[1] - cmn[1], cmx[2] - cmn[2]};
-            int axis = 0;
-            if (ext[1] > ext[0]) axis = 1;
-            if (ext[2] > ext[axis]) axis = 2;
-            const int mid = lo + count / 2;
-            const std::vector<double>& tbr = tb;
-            std::nth_element(idx.begin() + lo, idx.begin() + mid, idx.begin() + hi,
-                             [&tbr, axis](int a, int b) {
-                                 return tbr[9 * size_t(a) + 6 + axis] < tbr[9 * size_t(b) + 6 + axis];
-                             });
-            nd.left = build(lo, mid, depth + 1);
-            nd.right = build(mid, hi, depth + 1);
-            h.nodes[me] = nd;
-            return me;
-        }
-    } rec{h, idx, tb};
-    if (n > 0) rec.build(0, n, 1);
-
+        h.nodes.resize(size_t(subtreeNodes(n)));
+        // Subtrees of more than n/16 triangles (and at least 4096) run on their own thread.
+        const int64_t par = std::max<int64_t>(4096, n / 16);
+        BvhBuilder bb{h.nodes, idx, box, cen, n};
+        h.depth = bb.build(0, n, 0, 1, par);
+    }
     h.slotTri.assign(idx.begin(), idx.end());
     h.slotTri9.resize(size_t(n) * 9);
     for (int s = 0; s < n; ++s)
@@ -195,3 +225,43 @@ void DeviceBvh::release()
 }
 
 } // namespace sdfo
+
+#include "sdfo_debug.h"
+
+extern "C" int sdfo_debug_bvh_export(const double* verts, int64_t n_verts, const int32_t* tris,
+                                     int64_t n_tris, double* nodes6, int32_t* nodeI4,
+                                     int32_t* perm, int32_t max_nodes, int32_t* depth)
+{
+    try
+    {
+        if (!verts || !tris || n_verts <= 0 || n_tris <= 0) return SDFO_ERR_EMPTY_MESH;
+        const sdfo::HostBvh h = sdfo::buildHostBvh(verts, n_verts, tris, n_tris);
+        const int nn = int(h.nodes.size());
+        if (nn > max_nodes) return -nn;
+        for (int i = 0; i < nn; ++i)
+        {
+            const sdfo::BvhNode& b = h.nodes[i];
+            for (int k = 0; k < 3; ++k)
+            {
+                nodes6[6 * i + k] = b.bmin[k];
+                nodes6[6 * i + 3 + k] = b.bmax[k];
+            }
+            nodeI4[4 * i + 0] = b.left;
+            nodeI4[4 * i + 1] = b.right;
+            nodeI4[4 * i + 2] = b.triStart;
+            nodeI4[4 * i + 3] = b.triCount;
+        }
+        for (int64_t s = 0; s < n_tris; ++s) perm[s] = h.slotTri[size_t(s)];
+        if (depth) *depth = h.depth;
+        return nn;
+    }
+    catch (const sdfo::Error& e)
+    {
+        sdfo::setLastError(e.what());
+        return e.code;
+    }
+    catch (...)
+    {
+        return SDFO_ERR_INVALID_ARGUMENT;
+    }
+}

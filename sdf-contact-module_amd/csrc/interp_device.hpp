@@ -26,6 +26,10 @@ struct QGrid
     int stride;
     // dense
     const float4* pn;
+    // dense, optional: the 8 corner records of each cell packed contiguously
+    // (cell (i,j,k) -> cellRec[8 * (i + (nx-1) * (j + (ny-1) * k)) + di + 2 dj + 4 dk]),
+    // one 128-byte line per interpolation instead of four to eight.
+    const float4* cellRec;
     const float* h6f;
     const uint8_t* ccount;
     const float4* cpn;
@@ -115,26 +119,74 @@ struct Interp
     T hess[6]; // xx, xy, xz, yy, yz, zz (only with WithHess)
 };
 
-// Hermite basis (SdfOracle.cpp:976-979, 1009-1012) and second derivatives.
+// Hermite basis (SdfOracle.cpp:976-979, 1009-1012) with first and second
+// derivatives, kept in named registers.
 template <typename T>
-__device__ __forceinline__ void hermite(T t, T* h, T* dh, T* ddh)
+struct Basis
 {
-    const T t2 = t * t;
-    h[0] = t2 * (T(2) * t - T(3)) + T(1);
-    h[1] = t * (t * (t - T(2)) + T(1));
-    h[2] = t2 * (T(3) - T(2) * t);
-    h[3] = t2 * (t - T(1));
-    dh[0] = T(6) * t * (t - T(1));
-    dh[1] = t * (T(3) * t - T(4)) + T(1);
-    dh[2] = T(6) * t * (T(1) - t);
-    dh[3] = t * (T(3) * t - T(2));
-    if (ddh)
+    T h0, h1, h2, h3;     // h00, h10, h01, h11
+    T d0, d1, d2, d3;     // derivatives
+    T e0, e1, e2, e3;     // second derivatives
+    __device__ __forceinline__ T v(int corner, int kind) const // kind 0: value weight, 1: slope weight
     {
-        ddh[0] = T(12) * t - T(6);
-        ddh[1] = T(6) * t - T(4);
-        ddh[2] = T(6) - T(12) * t;
-        ddh[3] = T(6) * t - T(2);
+        return corner ? (kind ? h3 : h2) : (kind ? h1 : h0);
     }
+    __device__ __forceinline__ T d(int corner, int kind) const
+    {
+        return corner ? (kind ? d3 : d2) : (kind ? d1 : d0);
+    }
+    __device__ __forceinline__ T e(int corner, int kind) const
+    {
+        return corner ? (kind ? e3 : e2) : (kind ? e1 : e0);
+    }
+};
+
+template <typename T, bool Second>
+__device__ __forceinline__ Basis<T> hermite(T t)
+{
+    Basis<T> b;
+    const T t2 = t * t;
+    b.h0 = t2 * (T(2) * t - T(3)) + T(1);
+    b.h1 = t * (t * (t - T(2)) + T(1));
+    b.h2 = t2 * (T(3) - T(2) * t);
+    b.h3 = t2 * (t - T(1));
+    b.d0 = T(6) * t * (t - T(1));
+    b.d1 = t * (T(3) * t - T(4)) + T(1);
+    b.d2 = T(6) * t * (T(1) - t);
+    b.d3 = t * (T(3) * t - T(2));
+    if (Second)
+    {
+        b.e0 = T(12) * t - T(6);
+        b.e1 = T(6) * t - T(4);
+        b.e2 = T(6) - T(12) * t;
+        b.e3 = T(6) * t - T(2);
+    }
+    else
+    {
+        b.e0 = b.e1 = b.e2 = b.e3 = T(0);
+    }
+    return b;
+}
+
+// The 8 corner records (phi, n) of a cell; false if a sample is missing.
+__device__ __forceinline__ bool cellCorners(const QGrid& g, const Cell& c, float4 (&f)[8])
+{
+    if (g.cellRec)
+    {
+        const float4* r =
+            g.cellRec + 8 * (int64_t(c.i0) + int64_t(g.n[0] - 1) * (c.j0 + int64_t(g.n[1] - 1) * c.k0));
+#pragma unroll
+        for (int q = 0; q < 8; ++q) f[q] = r[q];
+        return true;
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+    {
+        const int64_t rec = recordOf(g, c.i0 + (q & 1), c.j0 + ((q >> 1) & 1), c.k0 + (q >> 2));
+        if (rec < 0) return false;
+        f[q] = fieldAt(g, rec);
+    }
+    return true;
 }
 
 // Tensor-product cubic Hermite on the 2x2x2 cell corners with corner data
@@ -143,59 +195,50 @@ __device__ __forceinline__ void hermite(T t, T* h, T* dh, T* ddh)
 template <typename T, bool WithHess>
 __device__ __forceinline__ bool tricubic(const QGrid& g, const Cell& c, Interp<T>& r)
 {
-    T a[4], da[4], ea[4], b[4], db[4], eb[4], cc[4], dc[4], ec[4];
-    hermite<T>(T(c.u), a, da, WithHess ? ea : nullptr);
-    hermite<T>(T(c.v), b, db, WithHess ? eb : nullptr);
-    hermite<T>(T(c.w), cc, dc, WithHess ? ec : nullptr);
+    float4 f[8];
+    if (!cellCorners(g, c, f)) return false;
+    const Basis<T> A = hermite<T, WithHess>(T(c.u));
+    const Basis<T> B = hermite<T, WithHess>(T(c.v));
+    const Basis<T> Cz = hermite<T, WithHess>(T(c.w));
     const T vx = T(g.voxel[0]), vy = T(g.voxel[1]), vz = T(g.voxel[2]);
     T gap = 0, fu = 0, fv = 0, fw = 0;
     T huu = 0, huv = 0, huw = 0, hvv = 0, hvw = 0, hww = 0;
 #pragma unroll
-    for (int dk = 0; dk < 2; ++dk)
-#pragma unroll
-        for (int dj = 0; dj < 2; ++dj)
-#pragma unroll
-            for (int di = 0; di < 2; ++di)
-            {
-                const int64_t rec = recordOf(g, c.i0 + di, c.j0 + dj, c.k0 + dk);
-                if (rec < 0) return false;
-                const float4 f = fieldAt(g, rec);
-                const T phi = T(f.x);
-                const T gx = T(f.y) * vx, gy = T(f.z) * vy, gz = T(f.w) * vz;
-                const T P0 = a[2 * di], P1 = a[2 * di + 1];
-                const T Q0 = b[2 * dj], Q1 = b[2 * dj + 1];
-                const T R0 = cc[2 * dk], R1 = cc[2 * dk + 1];
-                const T dP0 = da[2 * di], dP1 = da[2 * di + 1];
-                const T dQ0 = db[2 * dj], dQ1 = db[2 * dj + 1];
-                const T dR0 = dc[2 * dk], dR1 = dc[2 * dk + 1];
-                // s(A,B,C) = A0 B0 C0 phi + A1 B0 C0 gx + A0 B1 C0 gy + A0 B0 C1 gz
-                gap += (P0 * Q0 * R0) * phi + (P1 * Q0 * R0) * gx + (P0 * Q1 * R0) * gy +
-                       (P0 * Q0 * R1) * gz;
-                fu += (dP0 * Q0 * R0) * phi + (dP1 * Q0 * R0) * gx + (dP0 * Q1 * R0) * gy +
-                      (dP0 * Q0 * R1) * gz;
-                fv += (P0 * dQ0 * R0) * phi + (P1 * dQ0 * R0) * gx + (P0 * dQ1 * R0) * gy +
-                      (P0 * dQ0 * R1) * gz;
-                fw += (P0 * Q0 * dR0) * phi + (P1 * Q0 * dR0) * gx + (P0 * Q1 * dR0) * gy +
-                      (P0 * Q0 * dR1) * gz;
-                if (WithHess)
-                {
-                    const T eP0 = ea[2 * di], eP1 = ea[2 * di + 1];
-                    const T eQ0 = eb[2 * dj], eQ1 = eb[2 * dj + 1];
-                    const T eR0 = ec[2 * dk], eR1 = ec[2 * dk + 1];
-                    huu += (eP0 * Q0 * R0) * phi + (eP1 * Q0 * R0) * gx + (eP0 * Q1 * R0) * gy +
-                           (eP0 * Q0 * R1) * gz;
-                    hvv += (P0 * eQ0 * R0) * phi + (P1 * eQ0 * R0) * gx + (P0 * eQ1 * R0) * gy +
-                           (P0 * eQ0 * R1) * gz;
-                    hww += (P0 * Q0 * eR0) * phi + (P1 * Q0 * eR0) * gx + (P0 * Q1 * eR0) * gy +
-                           (P0 * Q0 * eR1) * gz;
-                    huv += (dP0 * dQ0 * R0) * phi + (dP1 * dQ0 * R0) * gx +
-                           (dP0 * dQ1 * R0) * gy + (dP0 * dQ0 * R1) * gz;
-                    huw += (dP0 * Q0 * dR0) * phi + (dP1 * Q0 * dR0) * gx +
-                           (dP0 * Q1 * dR0) * gy + (dP0 * Q0 * dR1) * gz;
-                    hvw += (P0 * dQ0 * dR0) * phi + (P1 * dQ0 * dR0) * gx +
-                           (P0 * dQ1 * dR0) * gy + (P0 * dQ0 * dR1) * gz;
-                }
-            }
+    for (int q = 0; q < 8; ++q)
+    {
+        const int di = q & 1, dj = (q >> 1) & 1, dk = q >> 2;
+        const T phi = T(f[q].x);
+        const T gx = T(f[q].y) * vx, gy = T(f[q].z) * vy, gz = T(f[q].w) * vz;
+        const T P0 = A.v(di, 0), P1 = A.v(di, 1), dP0 = A.d(di, 0), dP1 = A.d(di, 1);
+        const T Q0 = B.v(dj, 0), Q1 = B.v(dj, 1), dQ0 = B.d(dj, 0), dQ1 = B.d(dj, 1);
+        const T R0 = Cz.v(dk, 0), R1 = Cz.v(dk, 1), dR0 = Cz.d(dk, 0), dR1 = Cz.d(dk, 1);
+        // s(A,B,C) = A0 B0 C0 phi + A1 B0 C0 gx + A0 B1 C0 gy + A0 B0 C1 gz
+        gap += (P0 * Q0 * R0) * phi + (P1 * Q0 * R0) * gx + (P0 * Q1 * R0) * gy + (P0 * Q0 * R1) * gz;
+        fu += (dP0 * Q0 * R0) * phi + (dP1 * Q0 * R0) * gx + (dP0 * Q1 * R0) * gy +
+              (dP0 * Q0 * R1) * gz;
+        fv += (P0 * dQ0 * R0) * phi + (P1 * dQ0 * R0) * gx + (P0 * dQ1 * R0) * gy +
+              (P0 * dQ0 * R1) * gz;
+        fw += (P0 * Q0 * dR0) * phi + (P1 * Q0 * dR0) * gx + (P0 * Q1 * dR0) * gy +
+              (P0 * Q0 * dR1) * gz;
+        if (WithHess)
+        {
+            const T eP0 = A.e(di, 0), eP1 = A.e(di, 1);
+            const T eQ0 = B.e(dj, 0), eQ1 = B.e(dj, 1);
+            const T eR0 = Cz.e(dk, 0), eR1 = Cz.e(dk, 1);
+            huu += (eP0 * Q0 * R0) * phi + (eP1 * Q0 * R0) * gx + (eP0 * Q1 * R0) * gy +
+                   (eP0 * Q0 * R1) * gz;
+            hvv += (P0 * eQ0 * R0) * phi + (P1 * eQ0 * R0) * gx + (P0 * eQ1 * R0) * gy +
+                   (P0 * eQ0 * R1) * gz;
+            hww += (P0 * Q0 * eR0) * phi + (P1 * Q0 * eR0) * gx + (P0 * Q1 * eR0) * gy +
+                   (P0 * Q0 * eR1) * gz;
+            huv += (dP0 * dQ0 * R0) * phi + (dP1 * dQ0 * R0) * gx + (dP0 * dQ1 * R0) * gy +
+                   (dP0 * dQ0 * R1) * gz;
+            huw += (dP0 * Q0 * dR0) * phi + (dP1 * Q0 * dR0) * gx + (dP0 * Q1 * dR0) * gy +
+                   (dP0 * Q0 * dR1) * gz;
+            hvw += (P0 * dQ0 * dR0) * phi + (P1 * dQ0 * dR0) * gx + (P0 * dQ1 * dR0) * gy +
+                   (P0 * dQ0 * dR1) * gz;
+        }
+    }
     r.gap = gap;
     r.grad = V3T<T>{fu / vx, fv / vy, fw / vz};
     if (WithHess)
@@ -215,18 +258,11 @@ __device__ __forceinline__ bool tricubic(const QGrid& g, const Cell& c, Interp<T
 template <typename T, bool WithHess>
 __device__ __forceinline__ bool trilinear(const QGrid& g, const Cell& c, Interp<T>& r)
 {
+    float4 f[8];
+    if (!cellCorners(g, c, f)) return false;
     T p[2][2][2];
 #pragma unroll
-    for (int dk = 0; dk < 2; ++dk)
-#pragma unroll
-        for (int dj = 0; dj < 2; ++dj)
-#pragma unroll
-            for (int di = 0; di < 2; ++di)
-            {
-                const int64_t rec = recordOf(g, c.i0 + di, c.j0 + dj, c.k0 + dk);
-                if (rec < 0) return false;
-                p[di][dj][dk] = T(fieldAt(g, rec).x);
-            }
+    for (int q = 0; q < 8; ++q) p[q & 1][(q >> 1) & 1][q >> 2] = T(f[q].x);
     const T u = T(c.u), v = T(c.v), w = T(c.w);
     const T iu = T(1) - u, iv = T(1) - v, iw = T(1) - w;
     const T c00 = p[0][0][0] * iu + p[1][0][0] * u;

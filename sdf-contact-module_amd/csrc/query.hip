@@ -29,6 +29,7 @@ QGrid makeQGrid(const sdfo_grid& g)
     q.hasHessian = g.hasHessian ? 1 : 0;
     q.stride = g.stride;
     q.pn = g.pn;
+    q.cellRec = g.sparse ? nullptr : g.cellRec;
     q.h6f = g.h6f;
     q.ccount = g.ccount;
     q.cpn = g.cpn;
@@ -184,10 +185,10 @@ __device__ __forceinline__ void contactAwareQuery(const QGrid& g, double px, dou
     o.valid = true;
 }
 
-template <typename T>
-__device__ __forceinline__ QOut<T> queryPoint(const QGrid& g, int mode, double px, double py,
-                                              double pz)
+template <typename T, int Mode>
+__device__ __forceinline__ QOut<T> queryPoint(const QGrid& g, double px, double py, double pz)
 {
+    constexpr int mode = Mode;
     QOut<T> o;
     o.gap = T(0);
     o.grad = V3T<T>{T(0), T(0), T(0)};
@@ -213,35 +214,75 @@ __device__ __forceinline__ QOut<T> queryPoint(const QGrid& g, int mode, double p
     return o;
 }
 
-template <typename TP, typename T>
+// 3-vector stored contiguously (12 or 24 bytes per lane, one vector memory
+// instruction per lane and coalesced across the wave).
+template <typename TP>
+struct P3
+{
+    TP x, y, z;
+};
+
+template <typename TP, typename T, int Mode>
 __global__ void __launch_bounds__(256)
-queryKernel(QGrid g, int mode, const TP* __restrict__ pts, int64_t n, TP* __restrict__ gap,
-            TP* __restrict__ grad3, uint8_t* __restrict__ valid, TP* __restrict__ normal3,
-            TP* __restrict__ witness3, int32_t* __restrict__ branch)
+queryKernel(QGrid g, const P3<TP>* __restrict__ pts, int64_t n, TP* __restrict__ gap,
+            P3<TP>* __restrict__ grad3, uint8_t* __restrict__ valid, P3<TP>* __restrict__ normal3,
+            P3<TP>* __restrict__ witness3, int32_t* __restrict__ branch)
 {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const double px = double(pts[3 * i]), py = double(pts[3 * i + 1]), pz = double(pts[3 * i + 2]);
-    const QOut<T> o = queryPoint<T>(g, mode, px, py, pz);
+    const P3<TP> p = pts[i];
+    const double px = double(p.x), py = double(p.y), pz = double(p.z);
+    const QOut<T> o = queryPoint<T, Mode>(g, px, py, pz);
     gap[i] = TP(o.gap);
-    grad3[3 * i + 0] = TP(o.grad.x);
-    grad3[3 * i + 1] = TP(o.grad.y);
-    grad3[3 * i + 2] = TP(o.grad.z);
+    grad3[i] = P3<TP>{TP(o.grad.x), TP(o.grad.y), TP(o.grad.z)};
     valid[i] = o.valid ? 1 : 0;
-    if (normal3)
-    {
-        normal3[3 * i + 0] = TP(o.normal.x);
-        normal3[3 * i + 1] = TP(o.normal.y);
-        normal3[3 * i + 2] = TP(o.normal.z);
-    }
+    if (normal3) normal3[i] = P3<TP>{TP(o.normal.x), TP(o.normal.y), TP(o.normal.z)};
     if (witness3)
     {
         // witness = p - gap * normal (SdfOracle.cpp:593); zero when invalid.
-        witness3[3 * i + 0] = o.valid ? TP(T(px) - o.gap * o.normal.x) : TP(0);
-        witness3[3 * i + 1] = o.valid ? TP(T(py) - o.gap * o.normal.y) : TP(0);
-        witness3[3 * i + 2] = o.valid ? TP(T(pz) - o.gap * o.normal.z) : TP(0);
+        witness3[i] = o.valid ? P3<TP>{TP(T(px) - o.gap * o.normal.x), TP(T(py) - o.gap * o.normal.y),
+                                       TP(T(pz) - o.gap * o.normal.z)}
+                              : P3<TP>{TP(0), TP(0), TP(0)};
     }
     if (branch) branch[i] = o.branch;
+}
+
+template <typename TP, typename T>
+void launchQuery(const sdfo_grid& g, int mode, const TP* pts, int64_t n, TP* gap, TP* grad3,
+                 uint8_t* valid, TP* normal3, TP* witness3, int32_t* branch, hipStream_t s)
+{
+    if (n <= 0) return;
+    const QGrid q = dev::makeQGrid(g);
+    using V = P3<TP>;
+    const dim3 grid(unsigned((n + 255) / 256)), block(256);
+    const V* p = reinterpret_cast<const V*>(pts);
+    V* gr = reinterpret_cast<V*>(grad3);
+    V* nm = reinterpret_cast<V*>(normal3);
+    V* wt = reinterpret_cast<V*>(witness3);
+    switch (mode)
+    {
+    case SDFO_FIRST_ORDER:
+        hipLaunchKernelGGL((queryKernel<TP, T, SDFO_FIRST_ORDER>), grid, block, 0, s, q, p, n, gap,
+                           gr, valid, nm, wt, branch);
+        break;
+    case SDFO_SECOND_ORDER:
+        hipLaunchKernelGGL((queryKernel<TP, T, SDFO_SECOND_ORDER>), grid, block, 0, s, q, p, n, gap,
+                           gr, valid, nm, wt, branch);
+        break;
+    case SDFO_TRILINEAR:
+        hipLaunchKernelGGL((queryKernel<TP, T, SDFO_TRILINEAR>), grid, block, 0, s, q, p, n, gap,
+                           gr, valid, nm, wt, branch);
+        break;
+    case SDFO_TRICUBIC:
+        hipLaunchKernelGGL((queryKernel<TP, T, SDFO_TRICUBIC>), grid, block, 0, s, q, p, n, gap, gr,
+                           valid, nm, wt, branch);
+        break;
+    default:
+        hipLaunchKernelGGL((queryKernel<TP, T, SDFO_CONTACT_AWARE>), grid, block, 0, s, q, p, n,
+                           gap, gr, valid, nm, wt, branch);
+        break;
+    }
+    SDFO_HIP(hipGetLastError());
 }
 
 } // namespace
@@ -250,22 +291,14 @@ void launchQueryF32(const sdfo_grid& g, int mode, const float* pts, int64_t n, f
                     float* grad3, uint8_t* valid, float* normal3, float* witness3,
                     int32_t* branch, hipStream_t s)
 {
-    if (n <= 0) return;
-    const QGrid q = dev::makeQGrid(g);
-    hipLaunchKernelGGL((queryKernel<float, float>), dim3(unsigned((n + 255) / 256)), dim3(256), 0,
-                       s, q, mode, pts, n, gap, grad3, valid, normal3, witness3, branch);
-    SDFO_HIP(hipGetLastError());
+    launchQuery<float, float>(g, mode, pts, n, gap, grad3, valid, normal3, witness3, branch, s);
 }
 
 void launchQueryF64(const sdfo_grid& g, int mode, const double* pts, int64_t n, double* gap,
                     double* grad3, uint8_t* valid, double* normal3, double* witness3,
                     int32_t* branch, hipStream_t s)
 {
-    if (n <= 0) return;
-    const QGrid q = dev::makeQGrid(g);
-    hipLaunchKernelGGL((queryKernel<double, double>), dim3(unsigned((n + 255) / 256)), dim3(256),
-                       0, s, q, mode, pts, n, gap, grad3, valid, normal3, witness3, branch);
-    SDFO_HIP(hipGetLastError());
+    launchQuery<double, double>(g, mode, pts, n, gap, grad3, valid, normal3, witness3, branch, s);
 }
 
 } // namespace sdfo

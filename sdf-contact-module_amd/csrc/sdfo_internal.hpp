@@ -107,6 +107,7 @@ struct sdfo_grid
     // ---- dense: FP32 query mirror ---------------------------------------------
     float4* pn = nullptr;      // (phi, nx, ny, nz) per voxel
     float* h6f = nullptr;      // 6 per voxel
+    float4* cellRec = nullptr; // 8 corner records per cell (optional, see kCellRecordLimit)
     // ---- dense contact-aware candidates (FP64 authoritative + FP32 mirror) ------
     uint8_t* ccount = nullptr;  // N
     double* cphi = nullptr;     // N*S
@@ -163,7 +164,12 @@ void finishDense(sdfo_grid& g, hipStream_t s);
 void finishSparse(sdfo_grid& g, const std::vector<int32_t>& bijk, hipStream_t s);
 std::vector<int32_t> brickTable(const std::vector<int32_t>& bijk, int nbx, int nby, int nbz);
 
+// Dense grids whose packed cell records fit in this many bytes get them
+// (8 x float4 per cell: a random interpolation reads one 128-byte line).
+constexpr int64_t kCellRecordLimit = int64_t(4) << 30;
+
 // Kernel launchers (build.hip).
+void launchCellPack(const GridSpec& g, const float4* pn, float4* cellRec, hipStream_t s);
 void launchDenseFill(const GridSpec& g, const DeviceBvh& bvh, double* phi, double* n3,
                      double* witness3, int32_t* tri, hipStream_t s);
 void launchDenseHessian(const GridSpec& g, const double* n3, double* h6, hipStream_t s);

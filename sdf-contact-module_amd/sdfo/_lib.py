@@ -135,7 +135,25 @@ _sig("sdfo_contact_forward", [_vp, _vp, _vp, _i64, _vp, _i64, _P(ContactParams),
 _sig("sdfo_contact_backward", [_vp, _vp, _vp, _i64, _vp, _i64, _P(ContactParams), _vp, _vp, _vp,
                                _vp, _vp, _vp])
 
-EXPORTED = [n for n in dir(lib) if n.startswith("sdfo_")]
+_sig("sdfo_debug_bvh_export", [_vp, _i64, _vp, _i64, _vp, _vp, _vp, _i32, _vp])
+
+
+def debug_bvh_export(verts, tris):
+    """Host BVH (nodes6, nodeI4, perm, depth) -- test support, no device use."""
+    import numpy as np
+
+    v = np.ascontiguousarray(verts, np.float64).reshape(-1, 3)
+    t = np.ascontiguousarray(tris, np.int32).reshape(-1, 3)
+    cap = 2 * len(t) + 1
+    n6 = np.zeros((cap, 6))
+    i4 = np.zeros((cap, 4), np.int32)
+    perm = np.zeros(len(t), np.int32)
+    depth = C.c_int32()
+    p = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+    nn = lib.sdfo_debug_bvh_export(p(v), len(v), p(t), len(t), p(n6), p(i4), p(perm), cap,
+                                   C.byref(depth))
+    check(nn if nn < 0 else 0)
+    return n6[:nn].copy(), i4[:nn].copy(), perm, depth.value
 
 
 def check(rc: int) -> None:
