@@ -20,6 +20,11 @@ int sdfo_debug_bvh_export(const double* verts, int64_t n_verts, const int32_t* t
                           int64_t n_tris, double* nodes6, int32_t* nodeI4, int32_t* perm,
                           int32_t max_nodes, int32_t* depth);
 
+/* Order of n <= 64 values by absolute value, as the contact-aware builder
+ * orders merged candidates (std::sort with |a| < |b|, SdfBuilder.cpp:348-353).
+ * order[i] = index of the i-th value.  Host only. */
+int sdfo_debug_sort_by_abs(const double* values, int32_t n, int32_t* order);
+
 #ifdef __cplusplus
 }
 #endif

@@ -373,3 +373,11 @@ def contact_forward_plane(z_top, X, w, poses, kn=1e6, c=0.0, mu=0.5, reg=0.0, ma
                                        C.c_int(len(P)), _p(prm), _p(iprm, _ip), _p(out),
                                        _p(act, _ip))
     return out, act
+
+
+def sort_by_abs(values):
+    """Index order of std::sort with |a| < |b| (the contact-aware candidate sort)."""
+    v = _f64(values)
+    order = np.zeros(len(v), np.int32)
+    lib().oracle_sort_by_abs(_p(v), C.c_int(len(v)), _p(order, _ip))
+    return order

@@ -2214,3 +2214,14 @@ void oracle_contact_forward_plane(double zTop, const double* X, const double* w,
 }
 
 } // extern "C"
+
+extern "C" void oracle_sort_by_abs(const double* values, int n, int* order)
+{
+    // std::sort over candidates with |a.phi| < |b.phi| (SdfBuilder.cpp:348-353),
+    // applied to indices: the same comparisons and swaps, hence the same order.
+    std::vector<int> v(n);
+    std::iota(v.begin(), v.end(), 0);
+    std::sort(v.begin(), v.end(),
+              [&](int a, int b) { return std::abs(values[a]) < std::abs(values[b]); });
+    for (int i = 0; i < n; ++i) order[i] = v[i];
+}

@@ -593,12 +593,17 @@ int sdfo_contact_backward(const sdfo_grid* grid, const double* X, const double* 
 // Entry points whose GPU implementation is not in this build yet.
 extern "C" {
 
-int sdfo_build_contact_aware(const double*, int64_t, const int32_t*, int64_t, const double*,
-                             const sdfo_build_params*, int32_t, sdfo_grid** out, sdfo_stream)
+int sdfo_build_contact_aware(const double* verts, int64_t n_verts, const int32_t* tris,
+                             int64_t n_tris, const double* corner_normals,
+                             const sdfo_build_params* params, int32_t sparse, sdfo_grid** out,
+                             sdfo_stream stream)
 {
     return guarded([&] {
-        if (out) *out = nullptr;
-        throw Error(SDFO_ERR_UNSUPPORTED, "contact-aware build is not implemented on the GPU yet");
+        if (!out) throw Error(SDFO_ERR_INVALID_ARGUMENT, "out is NULL");
+        *out = nullptr;
+        checkParams(params);
+        *out = buildContactAware(verts, n_verts, tris, n_tris, corner_normals, *params, sparse != 0,
+                                 hipStream_t(stream));
     });
 }
 

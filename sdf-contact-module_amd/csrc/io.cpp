@@ -197,7 +197,6 @@ std::unique_ptr<sdfo_grid> loadDense(const std::string& path)
     if ((layout == kLayoutDenseContact) != contactFlag)
         throw Error(SDFO_ERR_FORMAT, "Dense SDF 0.1.0 contact-aware flag/layout mismatch");
     auto g = std::make_unique<sdfo_grid>();
-    SDFO_HIP(hipGetDevice(&g->device));
     g->spec = specOf(h);
     for (int a = 0; a < 3; ++a)
         if (g->spec.n[a] <= 0) throw Error(SDFO_ERR_FORMAT, "SdfOracle data file has an empty grid");
@@ -236,6 +235,7 @@ std::unique_ptr<sdfo_grid> loadDense(const std::string& path)
         rd(cbr.data(), NS * 4);
     }
     if (!in) throw Error(SDFO_ERR_FORMAT, "Failed to read dense SDF payload");
+    SDFO_HIP(hipGetDevice(&g->device));
     g->phi = upload(*g, phi);
     g->n3 = upload(*g, n3);
     g->h6 = upload(*g, h6);
@@ -270,7 +270,6 @@ std::unique_ptr<sdfo_grid> loadSparse(const std::string& path)
     if ((flags & kFlagSparse) == 0 || !(layout == kLayoutSparse || contact))
         throw Error(SDFO_ERR_FORMAT, "SdfOracle data file is not sparse block 0.1.0");
     auto g = std::make_unique<sdfo_grid>();
-    SDFO_HIP(hipGetDevice(&g->device));
     g->sparse = true;
     g->spec = specOf(h);
     g->blockDim = int(h.get<uint32_t>(Header::kReserved + 4));
@@ -324,6 +323,7 @@ std::unique_ptr<sdfo_grid> loadSparse(const std::string& path)
         }
         if (!in) throw Error(SDFO_ERR_FORMAT, "Failed to read sparse SDF block payload");
     }
+    SDFO_HIP(hipGetDevice(&g->device));
     g->nb = int(nblocks);
     g->nbx = (g->spec.n[0] + bd - 1) / bd;
     g->nby = (g->spec.n[1] + bd - 1) / bd;

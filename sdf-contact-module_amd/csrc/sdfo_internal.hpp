@@ -168,6 +168,11 @@ std::vector<int32_t> brickTable(const std::vector<int32_t>& bijk, int nbx, int n
 // (8 x float4 per cell: a random interpolation reads one 128-byte line).
 constexpr int64_t kCellRecordLimit = int64_t(4) << 30;
 
+// Contact-aware build (contact_aware.hip); corner normals: 9 doubles per triangle.
+sdfo_grid* buildContactAware(const double* verts, int64_t nv, const int32_t* tris, int64_t nt,
+                             const double* cornerN, const sdfo_build_params& p, bool sparse,
+                             hipStream_t s);
+
 // Kernel launchers (build.hip).
 void launchCellPack(const GridSpec& g, const float4* pn, float4* cellRec, hipStream_t s);
 void launchDenseFill(const GridSpec& g, const DeviceBvh& bvh, double* phi, double* n3,

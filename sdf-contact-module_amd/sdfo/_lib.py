@@ -136,6 +136,18 @@ _sig("sdfo_contact_backward", [_vp, _vp, _vp, _i64, _vp, _i64, _P(ContactParams)
                                _vp, _vp, _vp])
 
 _sig("sdfo_debug_bvh_export", [_vp, _i64, _vp, _i64, _vp, _vp, _vp, _i32, _vp])
+_sig("sdfo_debug_sort_by_abs", [_vp, _i32, _vp])
+
+
+def debug_sort_by_abs(values):
+    """Order the contact-aware builder gives to merged candidates (test support)."""
+    import numpy as np
+
+    v = np.ascontiguousarray(values, np.float64)
+    order = np.zeros(len(v), np.int32)
+    check(lib.sdfo_debug_sort_by_abs(v.ctypes.data_as(C.c_void_p), len(v),
+                                     order.ctypes.data_as(C.c_void_p)))
+    return order
 
 
 def debug_bvh_export(verts, tris):
