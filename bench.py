@@ -31,6 +31,25 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 
 
+TRAFFIC_FILE = "profiles/query_tricubic_cube128_1M_traffic.json"
+
+
+def _traffic(nq, shape):
+    """HBM bytes per query launch from the committed FETCH_SIZE / WRITE_SIZE
+    summary (counters scaled by the copy-kernel calibration), when the
+    workload is the one it was measured on."""
+    try:
+        with open(os.path.join(ROOT, TRAFFIC_FILE)) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    w = t.get("workload", {})
+    if w.get("query_points") != nq or list(w.get("grid", [])) != list(shape):
+        return None
+    return float(t["fetch_kib_reported"] * 1024 * t["fetch_factor"] +
+                 t["write_kib_reported"] * 1024 * t["write_factor"])
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -120,6 +139,10 @@ def main():
     n_vox_torus = int(np.prod(sdfo.setup_grid(torus[0], target_resolution=128,
                                               max_resolution=128).shape))
 
+    q_gap = torch.empty(args.query_points, dtype=torch.float32, device=dev)
+    q_grad = torch.empty((args.query_points, 3), dtype=torch.float32, device=dev)
+    q_valid = torch.empty(args.query_points, dtype=torch.uint8, device=dev)
+
     def ev():
         return torch.cuda.Event(enable_timing=True)
 
@@ -141,7 +164,7 @@ def main():
         # (2) 1M-point tricubic query
         e[0].record(stream)
         if "query" in parts:
-            cube_grid.query(pts, "Tricubic")
+            cube_grid.query_into(pts, q_gap, q_grad, q_valid, "Tricubic")
         e[1].record(stream)
         # (3) contact forward, 1000 poses
         e[2].record(stream)
@@ -225,10 +248,14 @@ def main():
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "queryKernel<float,float> (Tricubic)",
+            "kernel": "queryKernel<float, float, 3> (Tricubic)",
             "algorithmic_bytes": alg_bytes,
         },
     }
+    tr = _traffic(nq, spec.shape)
+    if tr is not None:
+        line["roofline"]["traffic"] = tr
+        line["roofline"]["traffic_source"] = TRAFFIC_FILE
     if not args.no_cpu:
         cpu, threads = cpu_baseline(args, spec, torus, cube, pts_np, X_np, w_np, poses_np)
         line["cpu_baseline"] = {

@@ -253,6 +253,25 @@ class SdfGrid:
             return gap, grad, valid.bool(), normal, witness, branch
         return gap, grad, valid.bool()
 
+    def query_into(self, points, gap, grad, valid, mode="Tricubic", stream=None):
+        """Launches one query kernel into caller-owned outputs (no allocation,
+        no conversion): ``gap`` (N), ``grad`` (N,3) of the points' dtype and
+        ``valid`` (N) uint8, all contiguous GPU tensors."""
+        import torch
+
+        mode = MODES.get(mode, mode)
+        n = points.shape[0]
+        for t in (points, gap, grad, valid):
+            if not (t.is_cuda and t.is_contiguous()):
+                raise ValueError("query_into needs contiguous GPU tensors")
+        if gap.shape != (n,) or grad.shape != (n, 3) or valid.shape != (n,) or \
+                valid.dtype != torch.uint8 or gap.dtype != points.dtype or grad.dtype != points.dtype:
+            raise ValueError("query_into output shapes/dtypes do not match the points")
+        fn = lib.sdfo_query if points.dtype == torch.float32 else lib.sdfo_query_f64
+        ptr = lambda t: C.c_void_p(t.data_ptr())  # noqa: E731
+        check(fn(self._h, int(mode), ptr(points), n, ptr(gap), ptr(grad), ptr(valid), None, None,
+                 None, _stream_handle(stream)))
+
     def query_batch(self, Y, mode="Tricubic"):
         """The prototype's SDF contract (sdf_grid.py:138-204):
         ``(g, raw_grad, |grad|, unit_normal, valid)``; invalid points give
