@@ -110,6 +110,7 @@ def main():
     import torch
 
     import sdfo
+    from sdfo import _lib
     from sdfo import contact as SC
     from sdfo import scenes
 
@@ -143,6 +144,12 @@ def main():
     q_grad = torch.empty((args.query_points, 3), dtype=torch.float32, device=dev)
     q_valid = torch.empty(args.query_points, dtype=torch.uint8, device=dev)
 
+    def busy():
+        try:
+            torch.cuda._sleep(1_000_000)
+        except (AttributeError, RuntimeError):
+            pass
+
     def ev():
         return torch.cuda.Event(enable_timing=True)
 
@@ -159,24 +166,30 @@ def main():
             t0 = time.perf_counter()
             g = sdfo.SdfGrid.build_dense(*torus, target_resolution=128, max_resolution=128)
             rec["build_s"] += time.perf_counter() - t0
+            rec["phases"].append(_lib.debug_build_phases())
             g.close()
         e = [ev() for _ in range(8)]
-        # (2) 1M-point tricubic query
+        # (2) 1M-point tricubic query; each timed segment starts behind a short
+        # device spin so the host's launch latency is not inside the events
+        busy()
         e[0].record(stream)
         if "query" in parts:
             cube_grid.query_into(pts, q_gap, q_grad, q_valid, "Tricubic")
         e[1].record(stream)
         # (3) contact forward, 1000 poses
+        busy()
         e[2].record(stream)
         if "contact" in parts:
             SC.contact_forward(cube_grid, X, w, poses, law)
         e[3].record(stream)
         # (4) batched fwd+bwd vs per-pose plain path
+        busy()
         e[4].record(stream)
         if "bwd" in parts:
             SC.contact_forward(cube_grid, X, w, poses_b, law_f)
             SC.contact_backward(cube_grid, X, w, poses_b, g_out, law_f)
         e[5].record(stream)
+        busy()
         e[6].record(stream)
         if "bwd" in parts:
             for b in range(args.bwd_poses):
@@ -191,7 +204,7 @@ def main():
         rec["plain_ms"].append(e[6].elapsed_time(e[7]))
 
     def new_rec():
-        return dict(build_s=0.0, query_ms=[], contact_ms=[], fwdbwd_ms=[], plain_ms=[])
+        return dict(build_s=0.0, query_ms=[], contact_ms=[], fwdbwd_ms=[], plain_ms=[], phases=[])
 
     warm = new_rec()
     for _ in range(args.warmup):
@@ -239,6 +252,8 @@ def main():
             "quadrature_points": len(X_np), "bwd_poses": args.bwd_poses,
         },
         "build_mvoxel_per_s": round(n_vox_torus / build_s / 1e6, 3),
+        "build_phases_ms": {k: round(float(np.mean([p[k] for p in rec["phases"]])), 4)
+                            for k in (rec["phases"][0] if rec["phases"] else {})},
         "build_ms": round(build_s * 1e3, 3),
         "query_kernel_ms": round(q_ms, 4),
         "contact_fwd_evals_per_s": round(n_eval / (c_ms * 1e-3), 1),

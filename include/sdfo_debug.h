@@ -25,6 +25,18 @@ int sdfo_debug_bvh_export(const double* verts, int64_t n_verts, const int32_t* t
  * order[i] = index of the i-th value.  Host only. */
 int sdfo_debug_sort_by_abs(const double* values, int32_t n, int32_t* order);
 
+/* Phase timings (ms) of the calling thread's last sdfo_build_dense, in the
+ * spirit of the builder's phase log (SdfBuilder.cpp:364-407): grid set-up,
+ * host BVH, BVH upload + allocations, fill kernel, Hessian kernel, FP32
+ * mirrors + cell records, total; then the number of voxels the fill re-ran
+ * with the per-lane search.  Copies min(cap, 8) values; returns 8. */
+int sdfo_debug_build_phases(double* ms, int32_t cap);
+
+/* Test switch for the calling thread: when on, the dense / sparse fills send
+ * every voxel (and the sparse activation every brick) to the per-lane replay
+ * pass, so tests can check that pass against the CPU oracle. */
+int sdfo_debug_force_replay(int32_t on);
+
 #ifdef __cplusplus
 }
 #endif

@@ -2,6 +2,10 @@
 // arithmetic must round exactly like the CPU path it replaces.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+
 #include "geom_device.hpp"
 
 namespace sdfo {
@@ -10,46 +14,119 @@ using dev::D3;
 
 namespace {
 
-constexpr int kFillBlock = 128; // 2 waves; LDS stack = kStackDepth * 128 * 4 B = 16 KB
+constexpr int kFillBlock = 256; // 4 waves
+constexpr int kFillWaves = kFillBlock / 64;
+constexpr int kReplayBlock = 64; // LDS lane stacks: kStackDepth * 64 * 4 B = 8 KB
 
-// Dense fill: one lane per voxel, one 4x4x4 voxel brick per wave so that the
-// lanes of a wave walk nearly the same BVH path (SdfBuilder.cpp:1052-1100).
+// One wave's traversal stacks in the block's LDS.
+__device__ __forceinline__ dev::WaveStacks waveStacks(unsigned char* lds)
+{
+    return dev::carveWaveStacks(lds + (threadIdx.x >> 6) * dev::kWaveStackBytes);
+}
+
+// Work items whose nearest triangle the wave search could not certify; a
+// second pass re-runs them with the per-lane search.
+struct Replay
+{
+    int* count;
+    int64_t* ids;
+};
+
+__device__ __forceinline__ void pushReplay(const Replay& r, int64_t id)
+{
+    r.ids[atomicAdd(r.count, 1)] = id;
+}
+
+__device__ __forceinline__ D3 voxelCentre(const GridSpec& g, int i, int j, int k)
+{
+    // GridSpec.h:30-34
+    return dev::d3(g.bmin[0] + g.voxel[0] * (i + 0.5), g.bmin[1] + g.voxel[1] * (j + 0.5),
+                   g.bmin[2] + g.voxel[2] * (k + 0.5));
+}
+
+struct DenseFillOut
+{
+    const int32_t* triIdx;
+    double *phi, *n3, *wit3;
+    int32_t* tri;
+};
+
+// SdfBuilder.cpp:1070-1085: phi, normal, witness and nearest triangle of a voxel.
+__device__ __forceinline__ void writeDenseVoxel(const DenseFillOut& o, D3 c,
+                                                const dev::SignedNearest& s, int64_t idx)
+{
+    const D3 gr = dev::voxelGradient(c, s);
+    o.phi[idx] = s.sd;
+    o.n3[3 * idx + 0] = gr.x;
+    o.n3[3 * idx + 1] = gr.y;
+    o.n3[3 * idx + 2] = gr.z;
+    if (o.wit3)
+    {
+        o.wit3[3 * idx + 0] = s.cp.x;
+        o.wit3[3 * idx + 1] = s.cp.y;
+        o.wit3[3 * idx + 2] = s.cp.z;
+    }
+    if (o.tri) o.tri[idx] = s.slot >= 0 ? o.triIdx[s.slot] : -1;
+}
+
+// Next work unit of a wave from the launch's queue counter (wave-uniform).
+__device__ __forceinline__ int64_t nextUnit(unsigned long long* queue)
+{
+    unsigned long long u = 0;
+    if ((threadIdx.x & 63) == 0) u = atomicAdd(queue, 1ull);
+    return int64_t(dev::uniformMask(__shfl(u, 0)));
+}
+
+// Dense fill: one lane per voxel, one 4x4x4 voxel brick per wave; the wave
+// walks the BVH cooperatively (SdfBuilder.cpp:1052-1100).  Waves take bricks
+// from a queue until it is drained, so clusters of expensive bricks (points
+// equidistant from many triangles) spread over the whole chip.
 __global__ void __launch_bounds__(kFillBlock)
 denseFillKernel(GridSpec g, const BvhNode* __restrict__ nodes, const double* __restrict__ tri9,
-                const int32_t* __restrict__ triIdx, D3 dir, int bx, int by, int bz,
-                double* __restrict__ phi, double* __restrict__ n3, double* __restrict__ wit3,
-                int32_t* __restrict__ tri)
+                const double* __restrict__ box6, D3 dir, double mu, int bx, int by, int bz,
+                DenseFillOut o, Replay replay, unsigned long long* queue, int force)
 {
-    __shared__ int stack[kStackDepth * kFillBlock];
+    __shared__ __align__(16) unsigned char lds[kFillWaves * dev::kWaveStackBytes];
     const int lane = threadIdx.x & 63;
-    const int64_t brick = int64_t(blockIdx.x) * (kFillBlock / 64) + (threadIdx.x >> 6);
     const int64_t nbricks = int64_t(bx) * by * bz;
-    if (brick >= nbricks) return;
-    const int bi = int(brick % bx);
-    const int bj = int((brick / bx) % by);
-    const int bk = int(brick / (int64_t(bx) * by));
-    const int i = bi * 4 + (lane & 3);
-    const int j = bj * 4 + ((lane >> 2) & 3);
-    const int k = bk * 4 + (lane >> 4);
-    if (i >= g.n[0] || j >= g.n[1] || k >= g.n[2]) return;
-    // GridSpec.h:30-34
-    const D3 c = dev::d3(g.bmin[0] + g.voxel[0] * (i + 0.5), g.bmin[1] + g.voxel[1] * (j + 0.5),
-                         g.bmin[2] + g.voxel[2] * (k + 0.5));
-    dev::LaneStack st{stack + threadIdx.x, kFillBlock};
-    const dev::SignedNearest s = dev::signedNearest(c, nodes, tri9, dir, st);
-    const int64_t idx = i + int64_t(g.n[0]) * (j + int64_t(g.n[1]) * k);
-    const D3 gr = dev::voxelGradient(c, s);
-    phi[idx] = s.sd;
-    n3[3 * idx + 0] = gr.x;
-    n3[3 * idx + 1] = gr.y;
-    n3[3 * idx + 2] = gr.z;
-    if (wit3)
+    const dev::WaveStacks ws = waveStacks(lds);
+    for (int64_t brick = nextUnit(queue); brick < nbricks; brick = nextUnit(queue))
     {
-        wit3[3 * idx + 0] = s.cp.x;
-        wit3[3 * idx + 1] = s.cp.y;
-        wit3[3 * idx + 2] = s.cp.z;
+        const int bi = int(brick % bx);
+        const int bj = int((brick / bx) % by);
+        const int bk = int(brick / (int64_t(bx) * by));
+        const int i = bi * 4 + (lane & 3);
+        const int j = bj * 4 + ((lane >> 2) & 3);
+        const int k = bk * 4 + (lane >> 4);
+        const bool active = i < g.n[0] && j < g.n[1] && k < g.n[2];
+        const D3 c = voxelCentre(g, min(i, g.n[0] - 1), min(j, g.n[1] - 1), min(k, g.n[2] - 1));
+        const dev::WaveSigned w = dev::waveSignedNearest(c, active, nodes, tri9, box6, dir, mu, ws);
+        if (!active) continue;
+        const int64_t idx = i + int64_t(g.n[0]) * (j + int64_t(g.n[1]) * k);
+        if (!w.exact || force)
+            pushReplay(replay, idx);
+        else
+            writeDenseVoxel(o, c, w.s, idx);
     }
-    if (tri) tri[idx] = s.slot >= 0 ? triIdx[s.slot] : -1;
+}
+
+// Per-lane search for the voxels denseFillKernel listed.
+__global__ void __launch_bounds__(kReplayBlock)
+denseReplayKernel(GridSpec g, const BvhNode* __restrict__ nodes, const double* __restrict__ tri9,
+                  D3 dir, Replay replay, DenseFillOut o)
+{
+    __shared__ int stack[kStackDepth * kReplayBlock];
+    const int n = *replay.count;
+    for (int q = int(blockIdx.x) * kReplayBlock + int(threadIdx.x); q < n;
+         q += int(gridDim.x) * kReplayBlock)
+    {
+        const int64_t idx = replay.ids[q];
+        const int i = int(idx % g.n[0]), j = int((idx / g.n[0]) % g.n[1]),
+                  k = int(idx / (int64_t(g.n[0]) * g.n[1]));
+        const D3 c = voxelCentre(g, i, j, k);
+        dev::LaneStack st{stack + threadIdx.x, kReplayBlock};
+        writeDenseVoxel(o, c, dev::signedNearest(c, nodes, tri9, dir, st), idx);
+    }
 }
 
 // Central differences of the normal field, clamped at the border, then
@@ -124,57 +201,137 @@ __global__ void candidateMirrorKernel(int64_t count, const double* __restrict__ 
 }
 
 // Sparse brick activation: |phi(brick centre)| <= activation
-// (SdfBuilder.cpp:869-890).  One lane per brick cell.
-__global__ void __launch_bounds__(kFillBlock)
-sparseActivateKernel(GridSpec g, const BvhNode* __restrict__ nodes,
-                     const double* __restrict__ tri9, D3 dir, int bd, int nbx, int nby, int nbz,
-                     double activation, uint8_t* __restrict__ mask)
+// (SdfBuilder.cpp:869-890).  One lane per brick cell, 64 consecutive cells per wave.
+__device__ __forceinline__ D3 brickCentre(const GridSpec& g, int bd, int nbx, int nby, int64_t bl)
 {
-    __shared__ int stack[kStackDepth * kFillBlock];
-    const int64_t bl = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (bl >= int64_t(nbx) * nby * nbz) return;
     const int bi = int(bl % nbx), bj = int((bl / nbx) % nby), bk = int(bl / (int64_t(nbx) * nby));
     const int i0 = bi * bd, j0 = bj * bd, k0 = bk * bd;
     const int i1 = min(i0 + bd, g.n[0]), j1 = min(j0 + bd, g.n[1]), k1 = min(k0 + bd, g.n[2]);
-    const D3 c = dev::d3(g.bmin[0] + g.voxel[0] * (0.5 * double(i0 + i1)),
-                         g.bmin[1] + g.voxel[1] * (0.5 * double(j0 + j1)),
-                         g.bmin[2] + g.voxel[2] * (0.5 * double(k0 + k1)));
-    dev::LaneStack st{stack + threadIdx.x, kFillBlock};
-    const dev::SignedNearest s = dev::signedNearest(c, nodes, tri9, dir, st);
-    mask[bl] = fabs(s.sd) <= activation ? 1 : 0;
+    return dev::d3(g.bmin[0] + g.voxel[0] * (0.5 * double(i0 + i1)),
+                   g.bmin[1] + g.voxel[1] * (0.5 * double(j0 + j1)),
+                   g.bmin[2] + g.voxel[2] * (0.5 * double(k0 + k1)));
 }
 
-// Sparse fill: one lane per brick voxel, brick-major (SdfBuilder.cpp:925-965).
 __global__ void __launch_bounds__(kFillBlock)
-sparseFillKernel(GridSpec g, const BvhNode* __restrict__ nodes, const double* __restrict__ tri9,
-                 D3 dir, int bd, int64_t total, const int32_t* __restrict__ bijk,
-                 float* __restrict__ sphi, float* __restrict__ sn3)
+sparseActivateKernel(GridSpec g, const BvhNode* __restrict__ nodes,
+                     const double* __restrict__ tri9, const double* __restrict__ box6, D3 dir,
+                     double mu, int bd, int nbx, int nby, int nbz, double activation,
+                     uint8_t* __restrict__ mask, Replay replay, unsigned long long* queue, int force)
 {
-    __shared__ int stack[kStackDepth * kFillBlock];
-    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (t >= total) return;
+    __shared__ __align__(16) unsigned char lds[kFillWaves * dev::kWaveStackBytes];
+    const int64_t ncell = int64_t(nbx) * nby * nbz;
+    const int64_t nunits = (ncell + 63) / 64;
+    const int lane = threadIdx.x & 63;
+    const dev::WaveStacks ws = waveStacks(lds);
+    for (int64_t u = nextUnit(queue); u < nunits; u = nextUnit(queue))
+    {
+        const int64_t bl0 = u * 64 + lane;
+        const bool active = bl0 < ncell;
+        const int64_t bl = active ? bl0 : ncell - 1;
+        const D3 c = brickCentre(g, bd, nbx, nby, bl);
+        const dev::WaveSigned w = dev::waveSignedNearest(c, active, nodes, tri9, box6, dir, mu, ws);
+        if (!active) continue;
+        if (!w.exact || force)
+            pushReplay(replay, bl);
+        else
+            mask[bl] = fabs(w.s.sd) <= activation ? 1 : 0;
+    }
+}
+
+__global__ void __launch_bounds__(kReplayBlock)
+sparseActivateReplayKernel(GridSpec g, const BvhNode* __restrict__ nodes,
+                           const double* __restrict__ tri9, D3 dir, int bd, int nbx, int nby,
+                           double activation, Replay replay, uint8_t* __restrict__ mask)
+{
+    __shared__ int stack[kStackDepth * kReplayBlock];
+    const int n = *replay.count;
+    for (int q = int(blockIdx.x) * kReplayBlock + int(threadIdx.x); q < n;
+         q += int(gridDim.x) * kReplayBlock)
+    {
+        const int64_t bl = replay.ids[q];
+        const D3 c = brickCentre(g, bd, nbx, nby, bl);
+        dev::LaneStack st{stack + threadIdx.x, kReplayBlock};
+        mask[bl] = fabs(dev::signedNearest(c, nodes, tri9, dir, st).sd) <= activation ? 1 : 0;
+    }
+}
+
+__device__ __forceinline__ void brickVoxel(const int32_t* __restrict__ bijk, int bd, int64_t t,
+                                           int& i, int& j, int& k)
+{
     const int bvc = bd * bd * bd;
     const int64_t b = t / bvc;
     const int local = int(t % bvc);
-    const int lx = local % bd, ly = (local / bd) % bd, lz = local / (bd * bd);
-    const int i = bijk[3 * b] * bd + lx, j = bijk[3 * b + 1] * bd + ly, k = bijk[3 * b + 2] * bd + lz;
-    if (i < 0 || i >= g.n[0] || j < 0 || j >= g.n[1] || k < 0 || k >= g.n[2])
-    {
-        sphi[t] = 0.0f;
-        sn3[3 * t + 0] = 1.0f;
-        sn3[3 * t + 1] = 0.0f;
-        sn3[3 * t + 2] = 0.0f;
-        return;
-    }
-    const D3 c = dev::d3(g.bmin[0] + g.voxel[0] * (i + 0.5), g.bmin[1] + g.voxel[1] * (j + 0.5),
-                         g.bmin[2] + g.voxel[2] * (k + 0.5));
-    dev::LaneStack st{stack + threadIdx.x, kFillBlock};
-    const dev::SignedNearest s = dev::signedNearest(c, nodes, tri9, dir, st);
+    i = bijk[3 * b] * bd + local % bd;
+    j = bijk[3 * b + 1] * bd + (local / bd) % bd;
+    k = bijk[3 * b + 2] * bd + local / (bd * bd);
+}
+
+__device__ __forceinline__ void writeSparseVoxel(D3 c, const dev::SignedNearest& s, int64_t t,
+                                                 float* __restrict__ sphi, float* __restrict__ sn3)
+{
     const D3 gr = dev::voxelGradient(c, s);
     sphi[t] = float(s.sd);
     sn3[3 * t + 0] = float(gr.x);
     sn3[3 * t + 1] = float(gr.y);
     sn3[3 * t + 2] = float(gr.z);
+}
+
+// Sparse fill: one lane per brick voxel, brick-major (SdfBuilder.cpp:925-965);
+// 64 consecutive brick voxels per queue unit.
+__global__ void __launch_bounds__(kFillBlock)
+sparseFillKernel(GridSpec g, const BvhNode* __restrict__ nodes, const double* __restrict__ tri9,
+                 const double* __restrict__ box6, D3 dir, double mu, int bd, int64_t total,
+                 const int32_t* __restrict__ bijk, float* __restrict__ sphi,
+                 float* __restrict__ sn3, Replay replay, unsigned long long* queue, int force)
+{
+    __shared__ __align__(16) unsigned char lds[kFillWaves * dev::kWaveStackBytes];
+    const int64_t nunits = (total + 63) / 64;
+    const int lane = threadIdx.x & 63;
+    const dev::WaveStacks ws = waveStacks(lds);
+    for (int64_t u = nextUnit(queue); u < nunits; u = nextUnit(queue))
+    {
+        const int64_t t0 = u * 64 + lane;
+        const int64_t t = t0 < total ? t0 : total - 1;
+        int i, j, k;
+        brickVoxel(bijk, bd, t, i, j, k);
+        const bool inGrid = i >= 0 && i < g.n[0] && j >= 0 && j < g.n[1] && k >= 0 && k < g.n[2];
+        const bool active = t0 < total && inGrid;
+        const D3 c = voxelCentre(g, min(max(i, 0), g.n[0] - 1), min(max(j, 0), g.n[1] - 1),
+                                 min(max(k, 0), g.n[2] - 1));
+        const dev::WaveSigned w = dev::waveSignedNearest(c, active, nodes, tri9, box6, dir, mu, ws);
+        if (t0 >= total) continue;
+        if (!inGrid)
+        {
+            sphi[t] = 0.0f;
+            sn3[3 * t + 0] = 1.0f;
+            sn3[3 * t + 1] = 0.0f;
+            sn3[3 * t + 2] = 0.0f;
+        }
+        else if (!w.exact || force)
+            pushReplay(replay, t);
+        else
+            writeSparseVoxel(c, w.s, t, sphi, sn3);
+    }
+}
+
+__global__ void __launch_bounds__(kReplayBlock)
+sparseFillReplayKernel(GridSpec g, const BvhNode* __restrict__ nodes,
+                       const double* __restrict__ tri9, D3 dir, int bd,
+                       const int32_t* __restrict__ bijk, Replay replay, float* __restrict__ sphi,
+                       float* __restrict__ sn3)
+{
+    __shared__ int stack[kStackDepth * kReplayBlock];
+    const int n = *replay.count;
+    for (int q = int(blockIdx.x) * kReplayBlock + int(threadIdx.x); q < n;
+         q += int(gridDim.x) * kReplayBlock)
+    {
+        const int64_t t = replay.ids[q];
+        int i, j, k;
+        brickVoxel(bijk, bd, t, i, j, k);
+        const D3 c = voxelCentre(g, i, j, k);
+        dev::LaneStack st{stack + threadIdx.x, kReplayBlock};
+        writeSparseVoxel(c, dev::signedNearest(c, nodes, tri9, dir, st), t, sphi, sn3);
+    }
 }
 
 // Sparse Hessian: missing or out-of-grid neighbours fall back to the voxel's
@@ -225,16 +382,102 @@ inline unsigned blocksFor(int64_t n, int bs) { return unsigned((n + bs - 1) / bs
 
 } // namespace
 
+// Pruning margin of the wave-cooperative nearest search: 2^-30 times the
+// largest coordinate magnitude of the grid box (which contains the mesh).
+double tieMargin(const GridSpec& g)
+{
+    double L = 0.0;
+    for (int a = 0; a < 3; ++a) L = std::max(L, std::max(std::fabs(g.bmin[a]), std::fabs(g.bmax[a])));
+    return std::ldexp(std::max(L, DBL_MIN), -30);
+}
+
+namespace {
+
+// Stream-ordered replay list sized for every work item of a launch.
+struct ReplayBuf
+{
+    Replay r{nullptr, nullptr};
+    unsigned long long* queue = nullptr; // work-queue counter of the fill launch
+    hipStream_t s;
+    ReplayBuf(int64_t cap, hipStream_t st) : s(st)
+    {
+        SDFO_HIP(hipMallocAsync(reinterpret_cast<void**>(&queue), 16, s));
+        SDFO_HIP(hipMemsetAsync(queue, 0, 16, s));
+        r.count = reinterpret_cast<int*>(queue + 1);
+        SDFO_HIP(hipMallocAsync(reinterpret_cast<void**>(&r.ids),
+                                sizeof(int64_t) * size_t(std::max<int64_t>(cap, 1)), s));
+    }
+    // Copies the list length to the calling thread's pinned counter (read it
+    // after synchronising the stream).
+    void report()
+    {
+        int* h = replayCounterHost();
+        if (h) SDFO_HIP(hipMemcpyAsync(h, r.count, sizeof(int), hipMemcpyDeviceToHost, s));
+    }
+    ~ReplayBuf()
+    {
+        (void)hipFreeAsync(r.ids, s);
+        (void)hipFreeAsync(queue, s);
+    }
+};
+
+constexpr int kReplayGrid = 512; // grid-stride blocks of the replay kernels
+
+// Blocks of a queue-driven fill launch: enough resident waves for the chip,
+// never more than there are work units.
+unsigned queueGrid(int64_t units)
+{
+    static int cus = 0;
+    if (cus == 0)
+    {
+        int dev = 0;
+        hipDeviceProp_t prop;
+        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+            cus = prop.multiProcessorCount;
+        if (cus <= 0) cus = 256;
+    }
+    const int64_t want = int64_t(cus) * 8; // 8 blocks of 4 waves per CU
+    const int64_t need = (units + kFillWaves - 1) / kFillWaves;
+    return unsigned(std::max<int64_t>(1, std::min(want, need)));
+}
+
+} // namespace
+
+int* replayCounterHost()
+{
+    thread_local int* h = nullptr;
+    if (!h)
+    {
+        if (hipHostMalloc(reinterpret_cast<void**>(&h), sizeof(int), hipHostMallocDefault) != hipSuccess)
+            h = nullptr;
+        else
+            *h = 0;
+    }
+    return h;
+}
+
+bool& forceReplay()
+{
+    thread_local bool f = false;
+    return f;
+}
+
 void launchDenseFill(const GridSpec& g, const DeviceBvh& bvh, double* phi, double* n3,
                      double* witness3, int32_t* tri, hipStream_t s)
 {
     const int bx = (g.n[0] + 3) / 4, by = (g.n[1] + 3) / 4, bz = (g.n[2] + 3) / 4;
     const int64_t nbr = int64_t(bx) * by * bz;
     const D3 dir = dev::d3(bvh.rayDir[0], bvh.rayDir[1], bvh.rayDir[2]);
-    hipLaunchKernelGGL(denseFillKernel, dim3(blocksFor(nbr, kFillBlock / 64)), dim3(kFillBlock), 0,
-                       s, g, bvh.nodes, bvh.tri9, bvh.triIdx, dir, bx, by, bz, phi, n3, witness3,
-                       tri);
+    const DenseFillOut o{bvh.triIdx, phi, n3, witness3, tri};
+    ReplayBuf rb(g.total(), s);
+    hipLaunchKernelGGL(denseFillKernel, dim3(queueGrid(nbr)), dim3(kFillBlock), 0, s, g, bvh.nodes,
+                       bvh.tri9, bvh.box6, dir, tieMargin(g), bx, by, bz, o, rb.r, rb.queue,
+                       forceReplay() ? 1 : 0);
     SDFO_HIP(hipGetLastError());
+    hipLaunchKernelGGL(denseReplayKernel, dim3(kReplayGrid), dim3(kReplayBlock), 0, s, g, bvh.nodes,
+                       bvh.tri9, dir, rb.r, o);
+    SDFO_HIP(hipGetLastError());
+    rb.report();
 }
 
 void launchDenseHessian(const GridSpec& g, const double* n3, double* h6, hipStream_t s)
@@ -275,8 +518,13 @@ void launchSparseActivate(const GridSpec& g, const DeviceBvh& bvh, int blockDim,
 {
     const int64_t nb = int64_t(nbx) * nby * nbz;
     const D3 dir = dev::d3(bvh.rayDir[0], bvh.rayDir[1], bvh.rayDir[2]);
-    hipLaunchKernelGGL(sparseActivateKernel, dim3(blocksFor(nb, kFillBlock)), dim3(kFillBlock), 0,
-                       s, g, bvh.nodes, bvh.tri9, dir, blockDim, nbx, nby, nbz, activation, mask);
+    ReplayBuf rb(nb, s);
+    hipLaunchKernelGGL(sparseActivateKernel, dim3(queueGrid((nb + 63) / 64)), dim3(kFillBlock), 0,
+                       s, g, bvh.nodes, bvh.tri9, bvh.box6, dir, tieMargin(g), blockDim, nbx, nby,
+                       nbz, activation, mask, rb.r, rb.queue, forceReplay() ? 1 : 0);
+    SDFO_HIP(hipGetLastError());
+    hipLaunchKernelGGL(sparseActivateReplayKernel, dim3(kReplayGrid), dim3(kReplayBlock), 0, s, g,
+                       bvh.nodes, bvh.tri9, dir, blockDim, nbx, nby, activation, rb.r, mask);
     SDFO_HIP(hipGetLastError());
 }
 
@@ -286,9 +534,15 @@ void launchSparseFill(const GridSpec& g, const DeviceBvh& bvh, int blockDim, int
     const int64_t total = int64_t(nb) * blockDim * blockDim * blockDim;
     if (total <= 0) return;
     const D3 dir = dev::d3(bvh.rayDir[0], bvh.rayDir[1], bvh.rayDir[2]);
-    hipLaunchKernelGGL(sparseFillKernel, dim3(blocksFor(total, kFillBlock)), dim3(kFillBlock), 0,
-                       s, g, bvh.nodes, bvh.tri9, dir, blockDim, total, bijk, sphi, sn3);
+    ReplayBuf rb(total, s);
+    hipLaunchKernelGGL(sparseFillKernel, dim3(queueGrid((total + 63) / 64)), dim3(kFillBlock), 0,
+                       s, g, bvh.nodes, bvh.tri9, bvh.box6, dir, tieMargin(g), blockDim, total,
+                       bijk, sphi, sn3, rb.r, rb.queue, forceReplay() ? 1 : 0);
     SDFO_HIP(hipGetLastError());
+    hipLaunchKernelGGL(sparseFillReplayKernel, dim3(kReplayGrid), dim3(kReplayBlock), 0, s, g,
+                       bvh.nodes, bvh.tri9, dir, blockDim, bijk, rb.r, sphi, sn3);
+    SDFO_HIP(hipGetLastError());
+    rb.report();
 }
 
 void launchSparseHessian(const GridSpec& g, int blockDim, int nb, int nbx, int nby, int nbz,

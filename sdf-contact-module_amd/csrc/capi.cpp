@@ -2,6 +2,7 @@
 // builders and launch of the query / contact kernels.  Every entry point
 // converts exceptions into a status code plus a thread-local message.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -19,6 +20,12 @@ thread_local std::string g_lastError;
 void setLastError(const std::string& msg) { g_lastError = msg; }
 
 // Builds the FP32 query mirror of a dense grid whose FP64 fields are on the device.
+double* lastBuildPhases()
+{
+    thread_local double phases[kBuildPhases] = {0, 0, 0, 0, 0, 0, 0, 0};
+    return phases;
+}
+
 void finishDense(sdfo_grid& g, hipStream_t s)
 {
     const int64_t N = g.spec.total();
@@ -123,33 +130,63 @@ struct DeviceGuard
 std::unique_ptr<sdfo_grid> buildDense(const double* verts, int64_t nv, const int32_t* tris,
                                       int64_t nt, const sdfo_build_params& p, hipStream_t s)
 {
+    using clk = std::chrono::steady_clock;
+    auto msSince = [](clk::time_point a) {
+        return std::chrono::duration<double, std::milli>(clk::now() - a).count();
+    };
+    double* ph = lastBuildPhases();
+    for (int q = 0; q < kBuildPhases; ++q) ph[q] = 0.0;
+    const clk::time_point t0 = clk::now();
     checkMesh(verts, nv, tris, nt);
     auto g = std::make_unique<sdfo_grid>();
     g->device = currentDevice();
     g->spec = setupGrid(verts, nv, p);
     const int64_t N = g->spec.total();
+    ph[0] = msSince(t0);
+    const clk::time_point t1 = clk::now();
     const HostBvh hb = buildHostBvh(verts, nv, tris, nt);
+    ph[1] = msSince(t1);
     DeviceBvh db;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     try
     {
+        const clk::time_point t2 = clk::now();
         db.upload(hb, s);
         g->phi = g->alloc<double>(size_t(N));
         g->n3 = g->alloc<double>(size_t(N) * 3);
         g->h6 = g->alloc<double>(size_t(N) * 6);
         g->witness3 = g->alloc<double>(size_t(N) * 3);
         g->tri = g->alloc<int32_t>(size_t(N));
+        for (hipEvent_t& e : ev) SDFO_HIP(hipEventCreate(&e));
+        ph[2] = msSince(t2);
+        SDFO_HIP(hipEventRecord(ev[0], s));
         launchDenseFill(g->spec, db, g->phi, g->n3, g->witness3, g->tri, s);
+        SDFO_HIP(hipEventRecord(ev[1], s));
         launchDenseHessian(g->spec, g->n3, g->h6, s);
+        SDFO_HIP(hipEventRecord(ev[2], s));
         finishDense(*g, s);
+        SDFO_HIP(hipEventRecord(ev[3], s));
         SDFO_HIP(hipStreamSynchronize(s));
+        float f = 0.0f;
+        for (int q = 0; q < 3; ++q)
+        {
+            SDFO_HIP(hipEventElapsedTime(&f, ev[q], ev[q + 1]));
+            ph[3 + q] = f;
+        }
     }
     catch (...)
     {
         (void)hipStreamSynchronize(s);
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
         db.release();
         throw;
     }
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     db.release();
+    ph[6] = msSince(t0);
+    const int* rc = replayCounterHost();
+    ph[7] = rc ? double(*rc) : -1.0;
     return g;
 }
 

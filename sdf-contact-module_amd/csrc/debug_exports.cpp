@@ -1,6 +1,7 @@
 // Test-support entry points (include/sdfo_debug.h).
 #include "absphi_sort.hpp"
 #include "sdfo_debug.h"
+#include "sdfo_internal.hpp"
 
 extern "C" int sdfo_debug_sort_by_abs(const double* values, int32_t n, int32_t* order)
 {
@@ -9,5 +10,19 @@ extern "C" int sdfo_debug_sort_by_abs(const double* values, int32_t n, int32_t* 
     for (int i = 0; i < n; ++i) v[i] = i;
     sdfo::sortPerm(v, n, sdfo::AbsPhiOrder{values});
     for (int i = 0; i < n; ++i) order[i] = v[i];
+    return SDFO_OK;
+}
+
+extern "C" int sdfo_debug_build_phases(double* ms, int32_t cap)
+{
+    const double* ph = sdfo::lastBuildPhases();
+    const int n = cap < sdfo::kBuildPhases ? cap : sdfo::kBuildPhases;
+    for (int i = 0; i < n && ms; ++i) ms[i] = ph[i];
+    return sdfo::kBuildPhases;
+}
+
+extern "C" int sdfo_debug_force_replay(int32_t on)
+{
+    sdfo::forceReplay() = on != 0;
     return SDFO_OK;
 }

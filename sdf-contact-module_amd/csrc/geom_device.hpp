@@ -172,16 +172,14 @@ struct LaneStack
     __device__ __forceinline__ int get(int e) const { return base[e * stride]; }
 };
 
-// KdTree::findSignedNearest (KdTree.h:143-165): near-first nearest search
-// (KdTree.h:266-328; strict < in leaves, so the first minimum in traversal
-// order wins) and the ray-parity count (KdTree.h:398-441; every box the ray
-// crosses is visited).
-__device__ inline SignedNearest signedNearest(D3 p, const BvhNode* __restrict__ nodes,
-                                              const double* __restrict__ tri9, D3 dir,
-                                              LaneStack st)
+// KdTree::searchNearest (KdTree.h:266-328): near-first traversal of one lane,
+// strict < in leaves, so the first minimum in traversal order wins.
+__device__ inline void refNearest(D3 p, const BvhNode* __restrict__ nodes,
+                                  const double* __restrict__ tri9, LaneStack st, double& bestSq,
+                                  int& bestSlot)
 {
-    double bestSq = DBL_MAX;
-    int bestSlot = -1;
+    bestSq = DBL_MAX;
+    bestSlot = -1;
     int sp = 0;
     st.set(sp++, 0);
     while (sp > 0)
@@ -217,7 +215,41 @@ __device__ inline SignedNearest signedNearest(D3 p, const BvhNode* __restrict__ 
         st.set(sp++, second);
         st.set(sp++, first);
     }
+}
 
+// KdTree::countRayIntersections (KdTree.h:398-441) for one lane: every box
+// the ray crosses is visited.
+__device__ inline int refRayHits(D3 p, D3 dir, const BvhNode* __restrict__ nodes,
+                                 const double* __restrict__ tri9, LaneStack st)
+{
+    int hits = 0;
+    int sp = 0;
+    st.set(sp++, 0);
+    while (sp > 0)
+    {
+        const int ni = st.get(--sp);
+        const BvhNode nd = nodes[ni];
+        if (!rayBox(p, dir, nd)) continue;
+        if (nd.triCount > 0)
+        {
+            for (int s = nd.triStart; s < nd.triStart + nd.triCount; ++s)
+            {
+                const Tri tt = loadTri(tri9, s);
+                if (rayTri(p, dir, tt.a, tt.b, tt.c)) ++hits;
+            }
+            continue;
+        }
+        st.set(sp++, nd.right);
+        st.set(sp++, nd.left);
+    }
+    return hits;
+}
+
+// KdTree::findSignedNearest (KdTree.h:143-165) from a nearest slot, its
+// squared distance and the ray hit count.
+__device__ inline SignedNearest finishSigned(D3 p, const double* __restrict__ tri9, double bestSq,
+                                             int bestSlot, int hits)
+{
     SignedNearest r;
     r.slot = bestSlot;
     double dist;
@@ -238,32 +270,303 @@ __device__ inline SignedNearest signedNearest(D3 p, const BvhNode* __restrict__ 
         r.face = d3(0.0, 0.0, 1.0);
         dist = DBL_MAX;
     }
-
-    int hits = 0;
-    sp = 0;
-    st.set(sp++, 0);
-    while (sp > 0)
-    {
-        const int ni = st.get(--sp);
-        const BvhNode nd = nodes[ni];
-        if (!rayBox(p, dir, nd)) continue;
-        if (nd.triCount > 0)
-        {
-            for (int s = nd.triStart; s < nd.triStart + nd.triCount; ++s)
-            {
-                const Tri tt = loadTri(tri9, s);
-                if (rayTri(p, dir, tt.a, tt.b, tt.c)) ++hits;
-            }
-            continue;
-        }
-        st.set(sp++, nd.right);
-        st.set(sp++, nd.left);
-    }
     if (hits > 0)
         r.sd = (hits % 2) == 1 ? -dist : dist;
     else
         r.sd = dot(sub(p, r.cp), r.face) >= 0.0 ? dist : -dist;
     return r;
+}
+
+// Per-lane form of findSignedNearest.
+__device__ inline SignedNearest signedNearest(D3 p, const BvhNode* __restrict__ nodes,
+                                              const double* __restrict__ tri9, D3 dir,
+                                              LaneStack st)
+{
+    double bestSq;
+    int bestSlot;
+    refNearest(p, nodes, tri9, st, bestSq, bestSlot);
+    const int hits = refRayHits(p, dir, nodes, tri9, st);
+    return finishSigned(p, tri9, bestSq, bestSlot, hits);
+}
+
+// ---------------------------------------------------------------------------
+// Wave-cooperative traversal.  The 64 lanes of a wave (nearby voxels) walk
+// ONE shared stack: a node is visited when any lane needs it, node and
+// triangle records are wave-uniform (scalar loads), and no lane diverges.
+// Results are the per-lane traversal's bit for bit:
+//
+// * Nearest.  The per-lane search returns, among the triangles of minimal
+//   squared distance m, the first in its depth-first order, provided it
+//   reaches that triangle t*; at each node that order puts the child with the
+//   strictly smaller box distance first (KdTree.h:311-326).  Each lane here
+//   knows the path key of every node it is shown (one bit per level: 0 = the
+//   child this lane visits first), so its depth-first rank is known wherever
+//   the wave goes, and it keeps the lexicographic minimum of (sq, rank, slot).
+//   The wave prunes a node only when for every lane its box distance exceeds
+//   (sqrt(best) + mu)^2, mu = 2^-30 * (largest coordinate magnitude), far above
+//   the rounding of the box and triangle distances, so every triangle with
+//   sq <= m is examined.  The per-lane search reaches t* whenever the box
+//   distance of t*'s leaf is <= m (its ancestors' boxes contain the leaf box,
+//   so their rounded distances are no larger, and its best before t* is > m).
+//   A lane for which that check fails is reported (exact = false) and must be
+//   re-run with the per-lane search.
+//   Path keys need no per-lane storage: a stack entry carries the wave mask
+//   of lanes whose bit for it is 1, and the entry's parent key is a prefix of
+//   the key of the node popped just before it (that node is the parent or one
+//   of its descendants).
+// * Ray parity.  The hit count does not depend on order; the wave pushes
+//   with each node the mask of lanes whose ray crossed every box on the path,
+//   which is the set of nodes the per-lane walk visits.
+//
+// Must be called by all 64 lanes of the wave in uniform control flow;
+// `active` lanes are the ones whose results are used (inactive lanes need a
+// finite point).
+// ---------------------------------------------------------------------------
+constexpr int kWave = 64;
+
+struct WaveStacks
+{
+    BvhNode* rec;             // kStackDepth node records (nearest walk)
+    double* leaf;             // staged leaf: 8 triangle boxes (6 doubles) then 8 triangles (9)
+    unsigned long long* mask; // kStackDepth: path-bit mask (nearest) / lane mask (ray)
+    int* node;                // kStackDepth: depth (nearest) / node index (ray)
+};
+
+constexpr int kLeafMax = 8; // KdTree.h:186
+constexpr int kLeafDoubles = kLeafMax * (6 + 9);
+
+// LDS bytes per wave for WaveStacks (a multiple of 16).
+constexpr int kWaveStackBytes = kStackDepth * (64 + 8 + 4) + kLeafDoubles * 8;
+
+// Carves one wave's stacks out of 16-byte aligned LDS.
+__device__ __forceinline__ WaveStacks carveWaveStacks(unsigned char* base)
+{
+    WaveStacks ws;
+    ws.rec = reinterpret_cast<BvhNode*>(base);
+    ws.leaf = reinterpret_cast<double*>(base + kStackDepth * 64);
+    ws.mask = reinterpret_cast<unsigned long long*>(base + kStackDepth * 64 + kLeafDoubles * 8);
+    ws.node = reinterpret_cast<int*>(base + kStackDepth * 72 + kLeafDoubles * 8);
+    return ws;
+}
+
+__device__ __forceinline__ int uniformInt(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ unsigned long long uniformMask(unsigned long long v)
+{
+    const unsigned lo = unsigned(__builtin_amdgcn_readfirstlane(int(unsigned(v))));
+    const unsigned hi = unsigned(__builtin_amdgcn_readfirstlane(int(unsigned(v >> 32))));
+    return (static_cast<unsigned long long>(hi) << 32) | lo;
+}
+
+struct WaveNearest
+{
+    double bestSq;
+    int bestSlot;
+    bool exact; // false: the per-lane search must be re-run for this lane
+};
+
+// Distance^2 from p to a box given as bmin3, bmax3 (same arithmetic as boxSq).
+__device__ __forceinline__ double boxSq6(D3 p, const double* b)
+{
+    double sq = 0.0;
+    const double pc[3] = {p.x, p.y, p.z};
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+    {
+        if (pc[a] < b[a])
+        {
+            const double d = b[a] - pc[a];
+            sq += d * d;
+        }
+        else if (pc[a] > b[3 + a])
+        {
+            const double d = pc[a] - b[3 + a];
+            sq += d * d;
+        }
+    }
+    return sq;
+}
+
+// The stack holds whole node records (copied from the parent's child loads),
+// so a level costs one dependent global load (the children), issued as soon
+// as the record is popped.  Leaves are staged into LDS by one vector load.
+__device__ inline WaveNearest waveNearest(D3 p, bool active, unsigned long long act,
+                                          const BvhNode* __restrict__ nodes,
+                                          const double* __restrict__ tri9,
+                                          const double* __restrict__ box6, double mu,
+                                          WaveStacks ws, int lane)
+{
+    double bestSq = DBL_MAX, thr = DBL_MAX, bestB = 0.0;
+    uint32_t bestRank = 0xffffffffu;
+    int bestSlot = -1;
+    uint32_t curKey = 0u; // key of the last node popped
+    int curDepth = 0;
+    const int nAct = __popcll(act);
+    ws.rec[0] = nodes[0];
+    ws.node[0] = 0;
+    ws.mask[0] = 0ull;
+    int sp = 1;
+    while (sp > 0)
+    {
+        --sp;
+        const BvhNode nd = ws.rec[sp];
+        const int depth = uniformInt(ws.node[sp]);
+        const unsigned long long bits = uniformMask(ws.mask[sp]);
+        const int left = uniformInt(nd.left), right = uniformInt(nd.right);
+        const int triStart = uniformInt(nd.triStart), triCount = uniformInt(nd.triCount);
+        // children requested before the box test (a leaf reloads itself: harmless)
+        const BvhNode L = nodes[triCount == 0 ? left : 0];
+        const BvhNode R = nodes[triCount == 0 ? right : 0];
+        uint32_t key = 0u;
+        if (depth > 0)
+            key = ((curKey >> (curDepth - depth + 1)) << 1) | uint32_t((bits >> lane) & 1ull);
+        curKey = key;
+        curDepth = depth;
+        const double bs = boxSq(p, nd);
+        if (__ballot(active && bs <= thr) == 0ull) continue;
+        if (triCount > 0)
+        {
+            // stage the leaf's triangle boxes and vertices (<= 120 doubles)
+            const int nb = 6 * triCount, nt = 9 * triCount;
+            const double* gb = box6 + 6 * size_t(triStart);
+            const double* gt = tri9 + 9 * size_t(triStart);
+            if (lane < nb) ws.leaf[lane] = gb[lane];
+            if (lane < nt) ws.leaf[6 * kLeafMax + lane] = gt[lane];
+            if (lane + kWave < nt) ws.leaf[6 * kLeafMax + kWave + lane] = gt[lane + kWave];
+            // A triangle whose own box is beyond every lane's threshold can
+            // neither win nor tie for any lane (same margin argument).
+            unsigned need = 0u;
+            for (int q = 0; q < triCount; ++q)
+                if (__ballot(active && boxSq6(p, ws.leaf + 6 * q) <= thr) != 0ull) need |= 1u << q;
+            const uint32_t rank = key << (31 - depth);
+            while (need != 0u)
+            {
+                const int q = __builtin_ctz(need);
+                need &= need - 1u;
+                const double* t = ws.leaf + 6 * kLeafMax + 9 * q;
+                const D3 ta = d3(t[0], t[1], t[2]), tb = d3(t[3], t[4], t[5]), tc = d3(t[6], t[7], t[8]);
+                const double sq = sqnorm(sub(p, closestPoint(p, ta, tb, tc)));
+                const int s = triStart + q;
+                const bool better =
+                    sq < bestSq ||
+                    (sq == bestSq && (rank < bestRank || (rank == bestRank && s < bestSlot)));
+                if (better)
+                {
+                    if (sq < bestSq)
+                    {
+                        bestSq = sq;
+                        const double r = sqrt(sq) + mu;
+                        thr = r * r;
+                    }
+                    bestRank = rank;
+                    bestSlot = s;
+                    bestB = bs;
+                }
+            }
+            continue;
+        }
+        const double dl = boxSq(p, L);
+        const double dr = boxSq(p, R);
+        const bool firstRight = dr < dl; // this lane's per-lane order
+        const unsigned long long fr = __ballot(firstRight);
+        const bool needL = __ballot(active && dl <= thr) != 0ull;
+        const bool needR = __ballot(active && dr <= thr) != 0ull;
+        const bool rightFirst = 2 * __popcll(fr & act) > nAct;
+        const int childDepth = depth + 1;
+        // The child visited second by the wave is pushed first.  Path bit 1 =
+        // the child this lane visits second: left has bit 1 where firstRight.
+        if (rightFirst)
+        {
+            if (needL)
+            {
+                ws.rec[sp] = L;
+                ws.node[sp] = childDepth;
+                ws.mask[sp] = fr;
+                ++sp;
+            }
+            if (needR)
+            {
+                ws.rec[sp] = R;
+                ws.node[sp] = childDepth;
+                ws.mask[sp] = ~fr;
+                ++sp;
+            }
+        }
+        else
+        {
+            if (needR)
+            {
+                ws.rec[sp] = R;
+                ws.node[sp] = childDepth;
+                ws.mask[sp] = ~fr;
+                ++sp;
+            }
+            if (needL)
+            {
+                ws.rec[sp] = L;
+                ws.node[sp] = childDepth;
+                ws.mask[sp] = fr;
+                ++sp;
+            }
+        }
+    }
+    return WaveNearest{bestSq, bestSlot, bestSlot >= 0 && bestB <= bestSq};
+}
+
+__device__ inline int waveRayHits(D3 p, D3 dir, unsigned long long act,
+                                  const BvhNode* __restrict__ nodes,
+                                  const double* __restrict__ tri9, WaveStacks ws, int lane)
+{
+    const unsigned long long me = 1ull << lane;
+    int hits = 0;
+    ws.node[0] = 0;
+    ws.mask[0] = act;
+    int sp = 1;
+    while (sp > 0)
+    {
+        --sp;
+        const int ni = uniformInt(ws.node[sp]);
+        const unsigned long long m = uniformMask(ws.mask[sp]);
+        const BvhNode nd = nodes[ni];
+        const bool in = (m & me) != 0ull && rayBox(p, dir, nd);
+        const unsigned long long m2 = __ballot(in);
+        if (m2 == 0ull) continue;
+        if (nd.triCount > 0)
+        {
+            for (int s = nd.triStart; s < nd.triStart + nd.triCount; ++s)
+            {
+                const Tri tt = loadTri(tri9, s);
+                if (rayTri(p, dir, tt.a, tt.b, tt.c) && in) ++hits;
+            }
+            continue;
+        }
+        ws.node[sp] = nd.right;
+        ws.mask[sp] = m2;
+        ++sp;
+        ws.node[sp] = nd.left;
+        ws.mask[sp] = m2;
+        ++sp;
+    }
+    return hits;
+}
+
+struct WaveSigned
+{
+    SignedNearest s;
+    bool exact; // false: recompute this lane with signedNearest
+};
+
+// findSignedNearest for the 64 lanes of a wave (see above).
+__device__ inline WaveSigned waveSignedNearest(D3 p, bool active, const BvhNode* __restrict__ nodes,
+                                               const double* __restrict__ tri9,
+                                               const double* __restrict__ box6, D3 dir, double mu,
+                                               WaveStacks ws)
+{
+    const int lane = int(threadIdx.x) & (kWave - 1);
+    const unsigned long long act = __ballot(active);
+    const WaveNearest wn = waveNearest(p, active, act, nodes, tri9, box6, mu, ws, lane);
+    const int hits = waveRayHits(p, dir, act, nodes, tri9, ws, lane);
+    return WaveSigned{finishSigned(p, tri9, wn.bestSq, wn.bestSlot, hits), wn.exact};
 }
 
 // SdfBuilder.cpp:1070-1085: sign * (c - w) / |c - w|, or sign * face normal.

@@ -180,8 +180,10 @@ HostBvh buildHostBvh(const double* verts, int64_t nv, const int32_t* tris, int64
     }
     h.slotTri.assign(idx.begin(), idx.end());
     h.slotTri9.resize(size_t(n) * 9);
+    h.slotBox6.resize(size_t(n) * 6);
     for (int s = 0; s < n; ++s)
     {
+        for (int q = 0; q < 6; ++q) h.slotBox6[6 * size_t(s) + q] = box[6 * size_t(idx[s]) + q];
         const int32_t* ix = tris + 3 * size_t(idx[s]);
         for (int c = 0; c < 3; ++c)
             for (int k = 0; k < 3; ++k)
@@ -203,12 +205,18 @@ void DeviceBvh::upload(const HostBvh& h, hipStream_t s)
     if (depth + 2 > kStackDepth)
         throw Error(SDFO_ERR_UNSUPPORTED, "BVH depth " + std::to_string(depth) +
                                               " exceeds the traversal stack");
+    if (nnodes >= (1 << 26))
+        throw Error(SDFO_ERR_UNSUPPORTED, "BVH with " + std::to_string(nnodes) +
+                                              " nodes exceeds the traversal stack encoding");
     SDFO_HIP(hipMalloc(&nodes, sizeof(BvhNode) * std::max<size_t>(1, h.nodes.size())));
     SDFO_HIP(hipMalloc(&tri9, sizeof(double) * std::max<size_t>(1, h.slotTri9.size())));
+    SDFO_HIP(hipMalloc(&box6, sizeof(double) * std::max<size_t>(1, h.slotBox6.size())));
     SDFO_HIP(hipMalloc(&triIdx, sizeof(int32_t) * std::max<size_t>(1, h.slotTri.size())));
     SDFO_HIP(hipMemcpyAsync(nodes, h.nodes.data(), sizeof(BvhNode) * h.nodes.size(),
                             hipMemcpyHostToDevice, s));
     SDFO_HIP(hipMemcpyAsync(tri9, h.slotTri9.data(), sizeof(double) * h.slotTri9.size(),
+                            hipMemcpyHostToDevice, s));
+    SDFO_HIP(hipMemcpyAsync(box6, h.slotBox6.data(), sizeof(double) * h.slotBox6.size(),
                             hipMemcpyHostToDevice, s));
     SDFO_HIP(hipMemcpyAsync(triIdx, h.slotTri.data(), sizeof(int32_t) * h.slotTri.size(),
                             hipMemcpyHostToDevice, s));
@@ -218,9 +226,11 @@ void DeviceBvh::release()
 {
     if (nodes) (void)hipFree(nodes);
     if (tri9) (void)hipFree(tri9);
+    if (box6) (void)hipFree(box6);
     if (triIdx) (void)hipFree(triIdx);
     nodes = nullptr;
     tri9 = nullptr;
+    box6 = nullptr;
     triIdx = nullptr;
 }
 

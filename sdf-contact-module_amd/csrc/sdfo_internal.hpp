@@ -64,6 +64,7 @@ struct HostBvh
 {
     std::vector<BvhNode> nodes;
     std::vector<double> slotTri9; // 9 doubles (a, b, c) per slot
+    std::vector<double> slotBox6; // triangle box per slot: bmin3, bmax3
     std::vector<int32_t> slotTri; // original triangle index per slot
     int depth = 0;
     double rayDir[3] = {0, 0, 0};
@@ -76,6 +77,7 @@ struct DeviceBvh
 {
     BvhNode* nodes = nullptr;
     double* tri9 = nullptr;
+    double* box6 = nullptr;
     int32_t* triIdx = nullptr;
     int nnodes = 0;
     int depth = 0;
@@ -158,6 +160,19 @@ struct sdfo_grid
 };
 
 namespace sdfo {
+
+// Phase timings of the calling thread's last dense build, in ms:
+// grid set-up, host BVH, BVH upload + allocations (host clock), fill kernel
+// (wave pass + replay pass), Hessian, FP32 mirrors + cell records (device
+// events), total (host clock); then the number of replayed voxels.
+constexpr int kBuildPhases = 8;
+double* lastBuildPhases();
+
+// Build kernels (build.hip): the calling thread's pinned counter that the last
+// dense or sparse fill reports its replayed voxels to (valid after the stream
+// is synchronised), and a test switch that sends every voxel to the replay pass.
+int* replayCounterHost();
+bool& forceReplay();
 
 // Grid completion helpers (capi.cpp): FP32 mirrors and the brick table.
 void finishDense(sdfo_grid& g, hipStream_t s);

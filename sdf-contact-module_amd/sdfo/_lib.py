@@ -137,6 +137,25 @@ _sig("sdfo_contact_backward", [_vp, _vp, _vp, _i64, _vp, _i64, _P(ContactParams)
 
 _sig("sdfo_debug_bvh_export", [_vp, _i64, _vp, _i64, _vp, _vp, _vp, _i32, _vp])
 _sig("sdfo_debug_sort_by_abs", [_vp, _i32, _vp])
+_sig("sdfo_debug_build_phases", [_vp, _i32])
+_sig("sdfo_debug_force_replay", [_i32])
+
+BUILD_PHASES = ("setup_grid", "host_bvh", "upload_alloc", "fill_kernel", "hessian_kernel",
+                "mirror_kernels", "total", "replayed_voxels")
+
+
+def debug_force_replay(on: bool):
+    """Test switch: send every voxel of this thread's builds to the replay pass."""
+    check(lib.sdfo_debug_force_replay(1 if on else 0))
+
+
+def debug_build_phases():
+    """Phase timings (ms) of this thread's last dense build."""
+    import numpy as np
+
+    ms = np.zeros(len(BUILD_PHASES))
+    lib.sdfo_debug_build_phases(ms.ctypes.data_as(C.c_void_p), len(ms))
+    return dict(zip(BUILD_PHASES, ms.tolist()))
 
 
 def debug_sort_by_abs(values):

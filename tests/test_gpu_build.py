@@ -91,3 +91,56 @@ def test_invalid_vertex_rejected(sdfo_gpu):
     t[3, 1] = 99
     with pytest.raises(sdfo_gpu.SdfoError, match="invalid vertex"):
         sdfo_gpu.SdfGrid.build_dense(v, t, target_resolution=16, max_resolution=16)
+
+
+@pytest.mark.parametrize("name,mk,res", [DENSE_CASES[0], DENSE_CASES[2], DENSE_CASES[4]],
+                         ids=["cube_64", "torus_48", "gear_64"])
+def test_dense_replay_pass_bit_exact(O, sdfo_gpu, name, mk, res):
+    """Every voxel forced through the per-lane replay pass still matches."""
+    from sdfo import _lib
+
+    v, t = mk()
+    print(f"[gpu] dense build with forced replay {name}", flush=True)
+    _lib.debug_force_replay(True)
+    try:
+        g = sdfo_gpu.SdfGrid.build_dense(v, t, target_resolution=res, max_resolution=res)
+        ph = _lib.debug_build_phases()
+    finally:
+        _lib.debug_force_replay(False)
+    assert ph["replayed_voxels"] == np.prod(g.spec.shape)
+    og = O.setup_grid(v, 0.1, 0.0, res, res)
+    ref = O.build_dense(v, t, og)
+    got = g.download_dense()
+    for key in ("tri", "phi", "n", "witness", "H"):
+        assert np.array_equal(got[key], ref[key]), key
+
+
+def test_sparse_replay_pass_bit_exact(O, sdfo_gpu):
+    from sdfo import _lib
+
+    v, t = meshes.torus(0.05, 0.01, 64, 16)
+    print("[gpu] sparse build with forced replay torus_64_b8", flush=True)
+    _lib.debug_force_replay(True)
+    try:
+        g = sdfo_gpu.SdfGrid.build_sparse(v, t, target_resolution=64, max_resolution=64,
+                                          block_dim=8, band_voxels=2, halo_voxels=1)
+    finally:
+        _lib.debug_force_replay(False)
+    og = O.setup_grid(v, 0.1, 0.0, 64, 64)
+    ref = O.build_sparse(v, t, og, block_dim=8, halo=1, band_voxels=2)
+    got = g.download_sparse()
+    assert np.array_equal(got["active"], ref["active"])
+    for key in ("phi", "n", "H"):
+        assert np.array_equal(got[key], ref[key]), key
+
+
+def test_wave_search_replays_few_voxels(sdfo_gpu):
+    """The wave-cooperative search certifies almost every voxel by itself."""
+    from sdfo import _lib, scenes
+
+    v, t = scenes.ring_mesh(256, 64)
+    g = sdfo_gpu.SdfGrid.build_dense(v, t, target_resolution=128, max_resolution=128)
+    ph = _lib.debug_build_phases()
+    n = np.prod(g.spec.shape)
+    print(f"[gpu] ring 128: replayed {ph['replayed_voxels']} of {n}; phases {ph}", flush=True)
+    assert 0 <= ph["replayed_voxels"] < 0.01 * n
