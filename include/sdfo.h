@@ -201,10 +201,38 @@ int sdfo_query_f64(const sdfo_grid* grid, int32_t mode, const double* points, in
 /* Many query streams against many grids in one launch (no reference
  * counterpart).  Stream s queries grids[s] with points
  * [offsets[s], offsets[s+1]) of the packed arrays; all pointers device
- * except grids/offsets (host). */
+ * except grids/offsets (host).  Convenience form: builds a plan, runs it and
+ * synchronises the stream. */
 int sdfo_query_batched(int32_t n_streams, const sdfo_grid* const* grids, int32_t mode,
                        const int64_t* offsets, const float* points, float* gap, float* grad3,
                        uint8_t* valid, sdfo_stream stream);
+
+/* A reusable batched query (no reference counterpart): the per-stream grid
+ * and point-range table is uploaded once by sdfo_query_plan_create (host
+ * arrays grids[n_streams], offsets[n_streams + 1]); sdfo_query_plan_run is a
+ * single kernel launch that allocates, copies and synchronises nothing, so it
+ * can be captured in a HIP graph (tune it, if the tuner is on, by one run
+ * outside the capture).  The grids must outlive the plan. */
+typedef struct sdfo_query_plan sdfo_query_plan;
+int sdfo_query_plan_create(int32_t n_streams, const sdfo_grid* const* grids,
+                           const int64_t* offsets, sdfo_query_plan** out);
+int sdfo_query_plan_run(const sdfo_query_plan* plan, int32_t mode, const float* points,
+                        float* gap, float* grad3, uint8_t* valid, sdfo_stream stream);
+void sdfo_query_plan_free(sdfo_query_plan* plan);
+
+/* ---- autotuner (no reference counterpart) ------------------------------ */
+/* SDFO_TUNE_ON_FIRST_USE: the first dense build of a size class (triangle
+ * and voxel counts, powers of two) times every voxel-brick shape of the fill
+ * and keeps the fastest; the first run of a batched query plan of a size
+ * class times every block size.  Results never depend on the choice. */
+#define SDFO_TUNE_OFF 0
+#define SDFO_TUNE_ON_FIRST_USE 1
+int sdfo_tuner_set_mode(int32_t mode);
+/* Clears every decision. */
+int sdfo_tuner_reset(void);
+/* One line per decision: "key: variant=ms, ..." with the choice starred.
+ * Copies at most cap-1 bytes plus a NUL; returns the full length. */
+int64_t sdfo_tuner_report(char* buf, int64_t cap);
 
 /* ---- contact ----------------------------------------------------------- */
 /* Per-pose penalty + damping + regularised Coulomb friction reduction of

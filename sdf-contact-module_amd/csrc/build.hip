@@ -5,6 +5,9 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <string>
+#include <utility>
+#include <vector>
 
 #include "geom_device.hpp"
 
@@ -51,6 +54,47 @@ struct DenseFillOut
     int32_t* tri;
 };
 
+// Voxel bricks of one wave: 2^sx x 2^sy x 2^(6-sx-sy) voxels, lane l at
+// (l & (2^sx-1), (l >> sx) & (2^sy-1), l >> (sx+sy)) inside its brick.
+struct BrickShape
+{
+    int sx, sy;     // log2 of the brick's x and y extent
+    int bx, by, bz; // bricks per axis
+    static BrickShape make(const GridSpec& g, int sx, int sy)
+    {
+        BrickShape b;
+        b.sx = sx;
+        b.sy = sy;
+        const int ex = 1 << sx, ey = 1 << sy, ez = 64 >> (sx + sy);
+        b.bx = (g.n[0] + ex - 1) / ex;
+        b.by = (g.n[1] + ey - 1) / ey;
+        b.bz = (g.n[2] + ez - 1) / ez;
+        return b;
+    }
+    __host__ __device__ int64_t count() const { return int64_t(bx) * by * bz; }
+    __device__ __forceinline__ void voxel(int64_t brick, int lane, int& i, int& j, int& k) const
+    {
+        const int bi = int(brick % bx);
+        const int bj = int((brick / bx) % by);
+        const int bk = int(brick / (int64_t(bx) * by));
+        i = (bi << sx) + (lane & ((1 << sx) - 1));
+        j = (bj << sy) + ((lane >> sx) & ((1 << sy) - 1));
+        k = (bk << (6 - sx - sy)) + (lane >> (sx + sy));
+    }
+};
+
+// One mesh of a batched dense fill.
+struct MeshJob
+{
+    GridSpec g;
+    double mu;
+    int root;
+    BrickShape sh;
+    int64_t brickBegin; // first unit of this mesh in the batch's brick sequence
+    int64_t voxelBegin; // first voxel id of this mesh in the batch's replay ids
+    DenseFillOut o;
+};
+
 // SdfBuilder.cpp:1070-1085: phi, normal, witness and nearest triangle of a voxel.
 __device__ __forceinline__ void writeDenseVoxel(const DenseFillOut& o, D3 c,
                                                 const dev::SignedNearest& s, int64_t idx)
@@ -83,21 +127,17 @@ __device__ __forceinline__ int64_t nextUnit(unsigned long long* queue)
 // equidistant from many triangles) spread over the whole chip.
 __global__ void __launch_bounds__(kFillBlock)
 denseFillKernel(GridSpec g, const BvhNode* __restrict__ nodes, const double* __restrict__ tri9,
-                const double* __restrict__ box6, D3 dir, double mu, int bx, int by, int bz,
-                DenseFillOut o, Replay replay, unsigned long long* queue, int force)
+                const double* __restrict__ box6, D3 dir, double mu, BrickShape sh, DenseFillOut o,
+                Replay replay, unsigned long long* queue, int force)
 {
     __shared__ __align__(16) unsigned char lds[kFillWaves * dev::kWaveStackBytes];
     const int lane = threadIdx.x & 63;
-    const int64_t nbricks = int64_t(bx) * by * bz;
+    const int64_t nbricks = sh.count();
     const dev::WaveStacks ws = waveStacks(lds);
     for (int64_t brick = nextUnit(queue); brick < nbricks; brick = nextUnit(queue))
     {
-        const int bi = int(brick % bx);
-        const int bj = int((brick / bx) % by);
-        const int bk = int(brick / (int64_t(bx) * by));
-        const int i = bi * 4 + (lane & 3);
-        const int j = bj * 4 + ((lane >> 2) & 3);
-        const int k = bk * 4 + (lane >> 4);
+        int i, j, k;
+        sh.voxel(brick, lane, i, j, k);
         const bool active = i < g.n[0] && j < g.n[1] && k < g.n[2];
         const D3 c = voxelCentre(g, min(i, g.n[0] - 1), min(j, g.n[1] - 1), min(k, g.n[2] - 1));
         const dev::WaveSigned w = dev::waveSignedNearest(c, active, nodes, tri9, box6, dir, mu, ws);
@@ -107,6 +147,76 @@ denseFillKernel(GridSpec g, const BvhNode* __restrict__ nodes, const double* __r
             pushReplay(replay, idx);
         else
             writeDenseVoxel(o, c, w.s, idx);
+    }
+}
+
+// Batched dense fill: the bricks of many meshes (each with its own grid and
+// its own BVH inside one concatenated node array) in one queue-driven launch.
+__device__ __forceinline__ int findJob(const MeshJob* __restrict__ jobs, int nJobs, int64_t unit,
+                                       bool byVoxel)
+{
+    int lo = 0, hi = nJobs - 1;
+    while (lo < hi)
+    {
+        const int mid = (lo + hi + 1) >> 1;
+        const int64_t b = byVoxel ? jobs[mid].voxelBegin : jobs[mid].brickBegin;
+        if (b <= unit)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(kFillBlock)
+denseFillBatchedKernel(const MeshJob* __restrict__ jobs, int nJobs, int64_t totalBricks,
+                       const BvhNode* __restrict__ nodes, const double* __restrict__ tri9,
+                       const double* __restrict__ box6, D3 dir, Replay replay,
+                       unsigned long long* queue, int force)
+{
+    __shared__ __align__(16) unsigned char lds[kFillWaves * dev::kWaveStackBytes];
+    const int lane = threadIdx.x & 63;
+    const dev::WaveStacks ws = waveStacks(lds);
+    for (int64_t unit = nextUnit(queue); unit < totalBricks; unit = nextUnit(queue))
+    {
+        const int m = dev::uniformInt(findJob(jobs, nJobs, unit, false));
+        const MeshJob& J = jobs[m];
+        const GridSpec g = J.g;
+        const BrickShape sh = J.sh;
+        int i, j, k;
+        sh.voxel(unit - J.brickBegin, lane, i, j, k);
+        const bool active = i < g.n[0] && j < g.n[1] && k < g.n[2];
+        const D3 c = voxelCentre(g, min(i, g.n[0] - 1), min(j, g.n[1] - 1), min(k, g.n[2] - 1));
+        const dev::WaveSigned w =
+            dev::waveSignedNearest(c, active, nodes, tri9, box6, dir, J.mu, ws, J.root);
+        if (!active) continue;
+        const int64_t idx = i + int64_t(g.n[0]) * (j + int64_t(g.n[1]) * k);
+        if (!w.exact || force)
+            pushReplay(replay, J.voxelBegin + idx);
+        else
+            writeDenseVoxel(J.o, c, w.s, idx);
+    }
+}
+
+__global__ void __launch_bounds__(kReplayBlock)
+denseReplayBatchedKernel(const MeshJob* __restrict__ jobs, int nJobs,
+                         const BvhNode* __restrict__ nodes, const double* __restrict__ tri9,
+                         D3 dir, Replay replay)
+{
+    __shared__ int stack[kStackDepth * kReplayBlock];
+    const int n = *replay.count;
+    for (int q = int(blockIdx.x) * kReplayBlock + int(threadIdx.x); q < n;
+         q += int(gridDim.x) * kReplayBlock)
+    {
+        const int64_t id = replay.ids[q];
+        const MeshJob& J = jobs[findJob(jobs, nJobs, id, true)];
+        const int64_t idx = id - J.voxelBegin;
+        const GridSpec& g = J.g;
+        const int i = int(idx % g.n[0]), j = int((idx / g.n[0]) % g.n[1]),
+                  k = int(idx / (int64_t(g.n[0]) * g.n[1]));
+        const D3 c = voxelCentre(g, i, j, k);
+        dev::LaneStack st{stack + threadIdx.x, kReplayBlock};
+        writeDenseVoxel(J.o, c, dev::signedNearest(c, nodes, tri9, dir, st, J.root), idx);
     }
 }
 
@@ -423,6 +533,11 @@ struct ReplayBuf
 
 constexpr int kReplayGrid = 512; // grid-stride blocks of the replay kernels
 
+// Brick shapes the tuner chooses from: (log2 x, log2 y) extents.
+constexpr int kFillShapeCount = 4;
+constexpr int kFillShapes[kFillShapeCount][2] = {{2, 2}, {3, 3}, {3, 2}, {4, 2}};
+const char* const kFillShapeNames[kFillShapeCount] = {"4x4x4", "8x8x1", "8x4x2", "16x4x1"};
+
 // Blocks of a queue-driven fill launch: enough resident waves for the chip,
 // never more than there are work units.
 unsigned queueGrid(int64_t units)
@@ -462,22 +577,100 @@ bool& forceReplay()
     return f;
 }
 
-void launchDenseFill(const GridSpec& g, const DeviceBvh& bvh, double* phi, double* n3,
-                     double* witness3, int32_t* tri, hipStream_t s)
+namespace {
+
+void runDenseFill(const GridSpec& g, const DeviceBvh& bvh, const DenseFillOut& o, int shape,
+                  bool report, hipStream_t s)
 {
-    const int bx = (g.n[0] + 3) / 4, by = (g.n[1] + 3) / 4, bz = (g.n[2] + 3) / 4;
-    const int64_t nbr = int64_t(bx) * by * bz;
+    const BrickShape sh = BrickShape::make(g, kFillShapes[shape][0], kFillShapes[shape][1]);
     const D3 dir = dev::d3(bvh.rayDir[0], bvh.rayDir[1], bvh.rayDir[2]);
-    const DenseFillOut o{bvh.triIdx, phi, n3, witness3, tri};
     ReplayBuf rb(g.total(), s);
-    hipLaunchKernelGGL(denseFillKernel, dim3(queueGrid(nbr)), dim3(kFillBlock), 0, s, g, bvh.nodes,
-                       bvh.tri9, bvh.box6, dir, tieMargin(g), bx, by, bz, o, rb.r, rb.queue,
+    hipLaunchKernelGGL(denseFillKernel, dim3(queueGrid(sh.count())), dim3(kFillBlock), 0, s, g,
+                       bvh.nodes, bvh.tri9, bvh.box6, dir, tieMargin(g), sh, o, rb.r, rb.queue,
                        forceReplay() ? 1 : 0);
     SDFO_HIP(hipGetLastError());
     hipLaunchKernelGGL(denseReplayKernel, dim3(kReplayGrid), dim3(kReplayBlock), 0, s, g, bvh.nodes,
                        bvh.tri9, dir, rb.r, o);
     SDFO_HIP(hipGetLastError());
-    rb.report();
+    if (report) rb.report();
+}
+
+std::string fillKey(const GridSpec& g, int64_t ntris)
+{
+    return "dense_fill tris~2^" + std::to_string(log2Bucket(ntris)) + " voxels~2^" +
+           std::to_string(log2Bucket(g.total()));
+}
+
+} // namespace
+
+// Brick shape for a dense fill: the tuner's choice for this size class, else
+// 4 x 4 x 4 (the most compact, so the lanes of a wave share the most nodes).
+int fillShapeFor(const GridSpec& g, int64_t ntris)
+{
+    int v = 0;
+    if (tunerLookup(fillKey(g, ntris), v) && v >= 0 && v < kFillShapeCount) return v;
+    return 0;
+}
+
+void launchDenseFill(const GridSpec& g, const DeviceBvh& bvh, double* phi, double* n3,
+                     double* witness3, int32_t* tri, hipStream_t s)
+{
+    const DenseFillOut o{bvh.triIdx, phi, n3, witness3, tri};
+    const std::string key = fillKey(g, bvh.ntris);
+    int shape = 0;
+    if (!tunerLookup(key, shape) && tunerEnabled() && !streamCapturing(s))
+    {
+        // time every brick shape on this build (each run writes the same,
+        // exact fields) and keep the fastest for the size class
+        std::vector<std::pair<int, double>> t;
+        for (int v = 0; v < kFillShapeCount; ++v)
+            t.emplace_back(v, timeOnStream(s, [&] { runDenseFill(g, bvh, o, v, false, s); }));
+        tunerRecord(key, t, kFillShapeNames);
+        tunerLookup(key, shape);
+    }
+    runDenseFill(g, bvh, o, shape, true, s);
+}
+
+void launchDenseFillBatched(const std::vector<BatchMesh>& meshes, const DeviceBvh& bvh,
+                            hipStream_t s)
+{
+    std::vector<MeshJob> jobs(meshes.size());
+    int64_t bricks = 0, voxels = 0;
+    for (size_t m = 0; m < meshes.size(); ++m)
+    {
+        const BatchMesh& b = meshes[m];
+        MeshJob& J = jobs[m];
+        J.g = b.g;
+        J.mu = tieMargin(b.g);
+        J.root = b.root;
+        const int shape = fillShapeFor(b.g, b.ntris);
+        J.sh = BrickShape::make(b.g, kFillShapes[shape][0], kFillShapes[shape][1]);
+        J.brickBegin = bricks;
+        J.voxelBegin = voxels;
+        J.o = DenseFillOut{bvh.triIdx, b.phi, b.n3, b.witness3, b.tri};
+        bricks += J.sh.count();
+        voxels += b.g.total();
+    }
+    if (jobs.empty() || bricks == 0) return;
+    MeshJob* dj = nullptr;
+    SDFO_HIP(hipMallocAsync(reinterpret_cast<void**>(&dj), sizeof(MeshJob) * jobs.size(), s));
+    SDFO_HIP(hipMemcpyAsync(dj, jobs.data(), sizeof(MeshJob) * jobs.size(), hipMemcpyHostToDevice, s));
+    const D3 dir = dev::d3(bvh.rayDir[0], bvh.rayDir[1], bvh.rayDir[2]);
+    {
+        ReplayBuf rb(voxels, s);
+        hipLaunchKernelGGL(denseFillBatchedKernel, dim3(queueGrid(bricks)), dim3(kFillBlock), 0, s,
+                           dj, int(jobs.size()), bricks, bvh.nodes, bvh.tri9, bvh.box6, dir, rb.r,
+                           rb.queue, forceReplay() ? 1 : 0);
+        SDFO_HIP(hipGetLastError());
+        hipLaunchKernelGGL(denseReplayBatchedKernel, dim3(kReplayGrid), dim3(kReplayBlock), 0, s, dj,
+                           int(jobs.size()), bvh.nodes, bvh.tri9, dir, rb.r);
+        SDFO_HIP(hipGetLastError());
+        rb.report();
+    }
+    // the job table is read by both kernels; the host copy must live until the
+    // stream has consumed it
+    SDFO_HIP(hipStreamSynchronize(s));
+    SDFO_HIP(hipFreeAsync(dj, s));
 }
 
 void launchDenseHessian(const GridSpec& g, const double* n3, double* h6, hipStream_t s)

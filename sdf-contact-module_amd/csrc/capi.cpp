@@ -2,14 +2,23 @@
 // builders and launch of the query / contact kernels.  Every entry point
 // converts exceptions into a status code plus a thread-local message.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "sdfo_internal.hpp"
+
+// The opaque batched-query handle of the C ABI.
+struct sdfo_query_plan
+{
+    int device = 0;
+    std::unique_ptr<sdfo::QueryPlan> plan;
+};
 
 namespace sdfo {
 
@@ -100,6 +109,47 @@ void checkParams(const sdfo_build_params* p)
     if (p->max_resolution <= 0) throw Error(SDFO_ERR_INVALID_ARGUMENT, "max_resolution must be positive");
 }
 
+void checkPlanArgs(int32_t ns, const sdfo_grid* const* grids, const int64_t* offsets)
+{
+    if (ns < 0) throw Error(SDFO_ERR_INVALID_ARGUMENT, "n_streams must be >= 0");
+    if (ns == 0) return;
+    if (!grids || !offsets) throw Error(SDFO_ERR_INVALID_ARGUMENT, "grids and offsets are required");
+    if (offsets[0] != 0) throw Error(SDFO_ERR_INVALID_ARGUMENT, "offsets[0] must be 0");
+    for (int32_t q = 0; q < ns; ++q)
+    {
+        if (!grids[q]) throw Error(SDFO_ERR_INVALID_ARGUMENT, "grid is NULL");
+        if (offsets[q + 1] < offsets[q])
+            throw Error(SDFO_ERR_INVALID_ARGUMENT, "offsets must be non-decreasing");
+    }
+}
+
+const int kPlanBlocks[4] = {64, 128, 256, 512};
+const char* const kPlanBlockNames[4] = {"64", "128", "256", "512"};
+
+// Runs a plan with the tuner's block size for its size class (timing every
+// block size first when the tuner is on and the stream is not capturing).
+void runPlanTuned(const QueryPlan& plan, int mode, const float* pts, float* gap, float* grad3,
+                  uint8_t* valid, hipStream_t s)
+{
+    const std::string key = "query_plan mode=" + std::to_string(mode) + " points~2^" +
+                            std::to_string(log2Bucket(plan.points));
+    int v = 2;
+    if (!tunerLookup(key, v) && tunerEnabled() && !streamCapturing(s) && plan.points > 0)
+    {
+        std::vector<std::pair<int, double>> t;
+        for (int q = 0; q < 4; ++q)
+        {
+            runQueryPlan(plan, mode, kPlanBlocks[q], pts, gap, grad3, valid, s); // warm
+            t.emplace_back(q, timeOnStream(s, [&] {
+                               runQueryPlan(plan, mode, kPlanBlocks[q], pts, gap, grad3, valid, s);
+                           }));
+        }
+        tunerRecord(key, t, kPlanBlockNames);
+        tunerLookup(key, v);
+    }
+    runQueryPlan(plan, mode, kPlanBlocks[std::max(0, std::min(3, v))], pts, gap, grad3, valid, s);
+}
+
 int currentDevice()
 {
     int d = 0;
@@ -188,6 +238,142 @@ std::unique_ptr<sdfo_grid> buildDense(const double* verts, int64_t nv, const int
     const int* rc = replayCounterHost();
     ph[7] = rc ? double(*rc) : -1.0;
     return g;
+}
+
+// Many meshes: host set-up and BVHs in parallel, one concatenated BVH on the
+// device, one queue-driven fill launch for all bricks, then the per-mesh
+// Hessian and mirror kernels.
+std::vector<std::unique_ptr<sdfo_grid>> buildDenseBatched(int32_t nm, const double* verts,
+                                                          const int64_t* vo, const int32_t* tris,
+                                                          const int64_t* to,
+                                                          const sdfo_build_params* params,
+                                                          hipStream_t s)
+{
+    for (int32_t m = 0; m < nm; ++m)
+    {
+        if (vo[m + 1] < vo[m] || to[m + 1] < to[m])
+            throw Error(SDFO_ERR_INVALID_ARGUMENT, "batched build: offsets must be non-decreasing");
+        checkMesh(verts + 3 * vo[m], vo[m + 1] - vo[m], tris + 3 * to[m], to[m + 1] - to[m]);
+    }
+    std::vector<GridSpec> specs(static_cast<size_t>(nm));
+    std::vector<HostBvh> bvhs(static_cast<size_t>(nm));
+    {
+        // meshes are independent: a small pool of host threads, largest first
+        std::vector<int32_t> order(static_cast<size_t>(nm));
+        for (int32_t m = 0; m < nm; ++m) order[size_t(m)] = m;
+        std::sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+            return (to[a + 1] - to[a]) > (to[b + 1] - to[b]);
+        });
+        std::atomic<int32_t> next{0};
+        std::vector<std::string> errs(static_cast<size_t>(nm));
+        std::vector<int> codes(static_cast<size_t>(nm), SDFO_OK);
+        auto work = [&] {
+            for (int32_t q = next++; q < nm; q = next++)
+            {
+                const int32_t m = order[size_t(q)];
+                try
+                {
+                    const double* v = verts + 3 * vo[m];
+                    specs[size_t(m)] = setupGrid(v, vo[m + 1] - vo[m], params[m]);
+                    bvhs[size_t(m)] = buildHostBvh(v, vo[m + 1] - vo[m], tris + 3 * to[m],
+                                                   to[m + 1] - to[m]);
+                }
+                catch (const Error& e)
+                {
+                    codes[size_t(m)] = e.code;
+                    errs[size_t(m)] = e.what();
+                }
+                catch (const std::exception& e)
+                {
+                    codes[size_t(m)] = SDFO_ERR_INVALID_ARGUMENT;
+                    errs[size_t(m)] = e.what();
+                }
+            }
+        };
+        const int nth = std::max(1, std::min<int>(nm, int(std::thread::hardware_concurrency())));
+        std::vector<std::thread> pool;
+        for (int t = 1; t < nth; ++t) pool.emplace_back(work);
+        work();
+        for (std::thread& t : pool) t.join();
+        for (int32_t m = 0; m < nm; ++m)
+            if (codes[size_t(m)] != SDFO_OK)
+                throw Error(codes[size_t(m)], "mesh " + std::to_string(m) + ": " + errs[size_t(m)]);
+    }
+    // concatenate the BVHs; node links and leaf slot ranges are rebased
+    HostBvh all;
+    std::vector<int> roots(static_cast<size_t>(nm));
+    size_t nodes = 0, slots = 0;
+    for (const HostBvh& h : bvhs)
+    {
+        nodes += h.nodes.size();
+        slots += h.slotTri.size();
+    }
+    all.nodes.reserve(nodes);
+    all.slotTri9.reserve(slots * 9);
+    all.slotBox6.reserve(slots * 6);
+    all.slotTri.reserve(slots);
+    for (int32_t m = 0; m < nm; ++m)
+    {
+        const HostBvh& h = bvhs[size_t(m)];
+        const int nodeBase = int(all.nodes.size());
+        const int slotBase = int(all.slotTri.size());
+        roots[size_t(m)] = nodeBase;
+        for (BvhNode nd : h.nodes)
+        {
+            if (nd.triCount > 0)
+                nd.triStart += slotBase;
+            else
+            {
+                nd.left += nodeBase;
+                nd.right += nodeBase;
+            }
+            all.nodes.push_back(nd);
+        }
+        all.slotTri9.insert(all.slotTri9.end(), h.slotTri9.begin(), h.slotTri9.end());
+        all.slotBox6.insert(all.slotBox6.end(), h.slotBox6.begin(), h.slotBox6.end());
+        all.slotTri.insert(all.slotTri.end(), h.slotTri.begin(), h.slotTri.end());
+        all.depth = std::max(all.depth, h.depth);
+        for (int k = 0; k < 3; ++k) all.rayDir[k] = h.rayDir[k];
+    }
+    std::vector<std::unique_ptr<sdfo_grid>> grids(static_cast<size_t>(nm));
+    std::vector<BatchMesh> jobs(static_cast<size_t>(nm));
+    DeviceBvh db;
+    try
+    {
+        db.upload(all, s);
+        const int dev = currentDevice();
+        for (int32_t m = 0; m < nm; ++m)
+        {
+            auto g = std::make_unique<sdfo_grid>();
+            g->device = dev;
+            g->spec = specs[size_t(m)];
+            const int64_t N = g->spec.total();
+            g->phi = g->alloc<double>(size_t(N));
+            g->n3 = g->alloc<double>(size_t(N) * 3);
+            g->h6 = g->alloc<double>(size_t(N) * 6);
+            g->witness3 = g->alloc<double>(size_t(N) * 3);
+            g->tri = g->alloc<int32_t>(size_t(N));
+            jobs[size_t(m)] = BatchMesh{g->spec, roots[size_t(m)], to[m + 1] - to[m], g->phi, g->n3,
+                                        g->witness3, g->tri};
+            grids[size_t(m)] = std::move(g);
+        }
+        launchDenseFillBatched(jobs, db, s);
+        for (int32_t m = 0; m < nm; ++m)
+        {
+            sdfo_grid& g = *grids[size_t(m)];
+            launchDenseHessian(g.spec, g.n3, g.h6, s);
+            finishDense(g, s);
+        }
+        SDFO_HIP(hipStreamSynchronize(s));
+    }
+    catch (...)
+    {
+        (void)hipStreamSynchronize(s);
+        db.release();
+        throw;
+    }
+    db.release();
+    return grids;
 }
 
 std::unique_ptr<sdfo_grid> buildSparse(const double* verts, int64_t nv, const int32_t* tris,
@@ -627,7 +813,7 @@ int sdfo_contact_backward(const sdfo_grid* grid, const double* X, const double* 
 
 } // extern "C"
 
-// Entry points whose GPU implementation is not in this build yet.
+// Contact-aware build, batched builds and queries, autotuner.
 extern "C" {
 
 int sdfo_build_contact_aware(const double* verts, int64_t n_verts, const int32_t* tris,
@@ -644,20 +830,107 @@ int sdfo_build_contact_aware(const double* verts, int64_t n_verts, const int32_t
     });
 }
 
-int sdfo_build_dense_batched(int32_t, const double*, const int64_t*, const int32_t*,
-                             const int64_t*, const sdfo_build_params*, sdfo_grid**, sdfo_stream)
+int sdfo_build_dense_batched(int32_t n_meshes, const double* verts, const int64_t* vert_offsets,
+                             const int32_t* tris, const int64_t* tri_offsets,
+                             const sdfo_build_params* params, sdfo_grid** out, sdfo_stream stream)
 {
     return guarded([&] {
-        throw Error(SDFO_ERR_UNSUPPORTED, "batched build is not implemented yet");
+        if (!out) throw Error(SDFO_ERR_INVALID_ARGUMENT, "out is NULL");
+        if (n_meshes < 0) throw Error(SDFO_ERR_INVALID_ARGUMENT, "n_meshes must be >= 0");
+        for (int32_t m = 0; m < n_meshes; ++m) out[m] = nullptr;
+        if (n_meshes == 0) return;
+        if (!verts || !vert_offsets || !tris || !tri_offsets || !params)
+            throw Error(SDFO_ERR_INVALID_ARGUMENT, "batched build: NULL array");
+        for (int32_t m = 0; m < n_meshes; ++m) checkParams(&params[m]);
+        std::vector<std::unique_ptr<sdfo_grid>> grids =
+            buildDenseBatched(n_meshes, verts, vert_offsets, tris, tri_offsets, params,
+                              hipStream_t(stream));
+        for (int32_t m = 0; m < n_meshes; ++m) out[m] = grids[size_t(m)].release();
     });
 }
 
-int sdfo_query_batched(int32_t, const sdfo_grid* const*, int32_t, const int64_t*, const float*,
-                       float*, float*, uint8_t*, sdfo_stream)
+int sdfo_query_plan_create(int32_t n_streams, const sdfo_grid* const* grids,
+                           const int64_t* offsets, sdfo_query_plan** out)
 {
     return guarded([&] {
-        throw Error(SDFO_ERR_UNSUPPORTED, "batched query is not implemented yet");
+        if (!out) throw Error(SDFO_ERR_INVALID_ARGUMENT, "out is NULL");
+        *out = nullptr;
+        checkPlanArgs(n_streams, grids, offsets);
+        auto p = std::make_unique<sdfo_query_plan>();
+        p->device = currentDevice();
+        p->plan = makeQueryPlan(n_streams, grids, offsets);
+        *out = p.release();
     });
+}
+
+int sdfo_query_plan_run(const sdfo_query_plan* plan, int32_t mode, const float* points,
+                        float* gap, float* grad3, uint8_t* valid, sdfo_stream stream)
+{
+    return guarded([&] {
+        if (!plan) throw Error(SDFO_ERR_INVALID_ARGUMENT, "plan is NULL");
+        if (mode < SDFO_FIRST_ORDER || mode > SDFO_CONTACT_AWARE)
+            throw Error(SDFO_ERR_INVALID_ARGUMENT, "unknown query mode");
+        if (plan->plan->points > 0 && (!points || !gap || !grad3 || !valid))
+            throw Error(SDFO_ERR_INVALID_ARGUMENT, "points, gap, grad3 and valid are required");
+        DeviceGuard dg(plan->device);
+        runPlanTuned(*plan->plan, mode, points, gap, grad3, valid, hipStream_t(stream));
+    });
+}
+
+void sdfo_query_plan_free(sdfo_query_plan* plan)
+{
+    if (!plan) return;
+    try
+    {
+        DeviceGuard dg(plan->device);
+        delete plan;
+    }
+    catch (...)
+    {
+        delete plan;
+    }
+}
+
+int sdfo_query_batched(int32_t n_streams, const sdfo_grid* const* grids, int32_t mode,
+                       const int64_t* offsets, const float* points, float* gap, float* grad3,
+                       uint8_t* valid, sdfo_stream stream)
+{
+    return guarded([&] {
+        if (mode < SDFO_FIRST_ORDER || mode > SDFO_CONTACT_AWARE)
+            throw Error(SDFO_ERR_INVALID_ARGUMENT, "unknown query mode");
+        checkPlanArgs(n_streams, grids, offsets);
+        const auto plan = makeQueryPlan(n_streams, grids, offsets);
+        if (plan->points > 0 && (!points || !gap || !grad3 || !valid))
+            throw Error(SDFO_ERR_INVALID_ARGUMENT, "points, gap, grad3 and valid are required");
+        runPlanTuned(*plan, mode, points, gap, grad3, valid, hipStream_t(stream));
+        SDFO_HIP(hipStreamSynchronize(hipStream_t(stream)));
+    });
+}
+
+int sdfo_tuner_set_mode(int32_t mode)
+{
+    return guarded([&] {
+        if (mode != SDFO_TUNE_OFF && mode != SDFO_TUNE_ON_FIRST_USE)
+            throw Error(SDFO_ERR_INVALID_ARGUMENT, "unknown tuner mode");
+        tunerSetMode(mode);
+    });
+}
+
+int sdfo_tuner_reset(void)
+{
+    return guarded([&] { tunerReset(); });
+}
+
+int64_t sdfo_tuner_report(char* buf, int64_t cap)
+{
+    const std::string r = tunerReport();
+    if (buf && cap > 0)
+    {
+        const size_t n = std::min<size_t>(r.size(), size_t(cap - 1));
+        std::memcpy(buf, r.data(), n);
+        buf[n] = '\0';
+    }
+    return int64_t(r.size());
 }
 
 } // extern "C"

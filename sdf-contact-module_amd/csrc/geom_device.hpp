@@ -176,12 +176,12 @@ struct LaneStack
 // strict < in leaves, so the first minimum in traversal order wins.
 __device__ inline void refNearest(D3 p, const BvhNode* __restrict__ nodes,
                                   const double* __restrict__ tri9, LaneStack st, double& bestSq,
-                                  int& bestSlot)
+                                  int& bestSlot, int root = 0)
 {
     bestSq = DBL_MAX;
     bestSlot = -1;
     int sp = 0;
-    st.set(sp++, 0);
+    st.set(sp++, root);
     while (sp > 0)
     {
         const int ni = st.get(--sp);
@@ -220,11 +220,11 @@ __device__ inline void refNearest(D3 p, const BvhNode* __restrict__ nodes,
 // KdTree::countRayIntersections (KdTree.h:398-441) for one lane: every box
 // the ray crosses is visited.
 __device__ inline int refRayHits(D3 p, D3 dir, const BvhNode* __restrict__ nodes,
-                                 const double* __restrict__ tri9, LaneStack st)
+                                 const double* __restrict__ tri9, LaneStack st, int root = 0)
 {
     int hits = 0;
     int sp = 0;
-    st.set(sp++, 0);
+    st.set(sp++, root);
     while (sp > 0)
     {
         const int ni = st.get(--sp);
@@ -280,12 +280,12 @@ __device__ inline SignedNearest finishSigned(D3 p, const double* __restrict__ tr
 // Per-lane form of findSignedNearest.
 __device__ inline SignedNearest signedNearest(D3 p, const BvhNode* __restrict__ nodes,
                                               const double* __restrict__ tri9, D3 dir,
-                                              LaneStack st)
+                                              LaneStack st, int root = 0)
 {
     double bestSq;
     int bestSlot;
-    refNearest(p, nodes, tri9, st, bestSq, bestSlot);
-    const int hits = refRayHits(p, dir, nodes, tri9, st);
+    refNearest(p, nodes, tri9, st, bestSq, bestSlot, root);
+    const int hits = refRayHits(p, dir, nodes, tri9, st, root);
     return finishSigned(p, tri9, bestSq, bestSlot, hits);
 }
 
@@ -394,7 +394,7 @@ __device__ inline WaveNearest waveNearest(D3 p, bool active, unsigned long long 
                                           const BvhNode* __restrict__ nodes,
                                           const double* __restrict__ tri9,
                                           const double* __restrict__ box6, double mu,
-                                          WaveStacks ws, int lane)
+                                          WaveStacks ws, int lane, int root)
 {
     double bestSq = DBL_MAX, thr = DBL_MAX, bestB = 0.0;
     uint32_t bestRank = 0xffffffffu;
@@ -402,7 +402,7 @@ __device__ inline WaveNearest waveNearest(D3 p, bool active, unsigned long long 
     uint32_t curKey = 0u; // key of the last node popped
     int curDepth = 0;
     const int nAct = __popcll(act);
-    ws.rec[0] = nodes[0];
+    ws.rec[0] = nodes[root];
     ws.node[0] = 0;
     ws.mask[0] = 0ull;
     int sp = 1;
@@ -515,11 +515,12 @@ __device__ inline WaveNearest waveNearest(D3 p, bool active, unsigned long long 
 
 __device__ inline int waveRayHits(D3 p, D3 dir, unsigned long long act,
                                   const BvhNode* __restrict__ nodes,
-                                  const double* __restrict__ tri9, WaveStacks ws, int lane)
+                                  const double* __restrict__ tri9, WaveStacks ws, int lane,
+                                  int root)
 {
     const unsigned long long me = 1ull << lane;
     int hits = 0;
-    ws.node[0] = 0;
+    ws.node[0] = root;
     ws.mask[0] = act;
     int sp = 1;
     while (sp > 0)
@@ -560,12 +561,12 @@ struct WaveSigned
 __device__ inline WaveSigned waveSignedNearest(D3 p, bool active, const BvhNode* __restrict__ nodes,
                                                const double* __restrict__ tri9,
                                                const double* __restrict__ box6, D3 dir, double mu,
-                                               WaveStacks ws)
+                                               WaveStacks ws, int root = 0)
 {
     const int lane = int(threadIdx.x) & (kWave - 1);
     const unsigned long long act = __ballot(active);
-    const WaveNearest wn = waveNearest(p, active, act, nodes, tri9, box6, mu, ws, lane);
-    const int hits = waveRayHits(p, dir, act, nodes, tri9, ws, lane);
+    const WaveNearest wn = waveNearest(p, active, act, nodes, tri9, box6, mu, ws, lane, root);
+    const int hits = waveRayHits(p, dir, act, nodes, tri9, ws, lane, root);
     return WaveSigned{finishSigned(p, tri9, wn.bestSq, wn.bestSlot, hits), wn.exact};
 }
 

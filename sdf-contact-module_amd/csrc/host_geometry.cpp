@@ -200,6 +200,7 @@ void DeviceBvh::upload(const HostBvh& h, hipStream_t s)
 {
     release();
     nnodes = int(h.nodes.size());
+    ntris = int64_t(h.slotTri.size());
     depth = h.depth;
     for (int k = 0; k < 3; ++k) rayDir[k] = h.rayDir[k];
     if (depth + 2 > kStackDepth)

@@ -4,6 +4,10 @@
 // are gathered from the FP32 mirror (float4 = phi + normal per voxel).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <memory>
+#include <vector>
+
 #include "interp_device.hpp"
 #include "sdfo_internal.hpp"
 
@@ -247,6 +251,77 @@ queryKernel(QGrid g, const P3<TP>* __restrict__ pts, int64_t n, TP* __restrict__
     if (branch) branch[i] = o.branch;
 }
 
+// One point stream of a batched query: its grid, its points and its range of
+// 64-point work units (one wave each) in the launch.
+struct StreamDesc
+{
+    QGrid g;
+    int64_t pointBegin, pointEnd;
+    int64_t unitBegin;
+};
+
+template <int Mode, int Block>
+__global__ void __launch_bounds__(Block)
+queryBatchedKernel(const StreamDesc* __restrict__ sd, int ns, int64_t units,
+                   const P3<float>* __restrict__ pts, float* __restrict__ gap,
+                   P3<float>* __restrict__ grad3, uint8_t* __restrict__ valid)
+{
+    const int64_t u = int64_t(blockIdx.x) * (Block / 64) + (threadIdx.x >> 6);
+    if (u >= units) return;
+    // the stream owning this unit (wave-uniform binary search)
+    int lo = 0, hi = ns - 1;
+    while (lo < hi)
+    {
+        const int mid = (lo + hi + 1) >> 1;
+        if (sd[mid].unitBegin <= u)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const StreamDesc& D = sd[lo];
+    const int64_t i = D.pointBegin + (u - D.unitBegin) * 64 + (threadIdx.x & 63);
+    if (i >= D.pointEnd) return;
+    const P3<float> p = pts[i];
+    const QOut<float> o = queryPoint<float, Mode>(D.g, double(p.x), double(p.y), double(p.z));
+    gap[i] = o.gap;
+    grad3[i] = P3<float>{o.grad.x, o.grad.y, o.grad.z};
+    valid[i] = o.valid ? 1 : 0;
+}
+
+template <int Block>
+void launchBatchedBlock(int mode, const StreamDesc* sd, int ns, int64_t units, const float* pts,
+                        float* gap, float* grad3, uint8_t* valid, hipStream_t s)
+{
+    using V = P3<float>;
+    const V* p = reinterpret_cast<const V*>(pts);
+    V* gr = reinterpret_cast<V*>(grad3);
+    constexpr int per = Block / 64;
+    const dim3 grid(unsigned((units + per - 1) / per)), block(Block);
+    switch (mode)
+    {
+    case SDFO_FIRST_ORDER:
+        hipLaunchKernelGGL((queryBatchedKernel<SDFO_FIRST_ORDER, Block>), grid, block, 0, s, sd, ns,
+                           units, p, gap, gr, valid);
+        break;
+    case SDFO_SECOND_ORDER:
+        hipLaunchKernelGGL((queryBatchedKernel<SDFO_SECOND_ORDER, Block>), grid, block, 0, s, sd, ns,
+                           units, p, gap, gr, valid);
+        break;
+    case SDFO_TRILINEAR:
+        hipLaunchKernelGGL((queryBatchedKernel<SDFO_TRILINEAR, Block>), grid, block, 0, s, sd, ns,
+                           units, p, gap, gr, valid);
+        break;
+    case SDFO_TRICUBIC:
+        hipLaunchKernelGGL((queryBatchedKernel<SDFO_TRICUBIC, Block>), grid, block, 0, s, sd, ns,
+                           units, p, gap, gr, valid);
+        break;
+    default:
+        hipLaunchKernelGGL((queryBatchedKernel<SDFO_CONTACT_AWARE, Block>), grid, block, 0, s, sd, ns,
+                           units, p, gap, gr, valid);
+        break;
+    }
+}
+
 template <typename TP, typename T>
 void launchQuery(const sdfo_grid& g, int mode, const TP* pts, int64_t n, TP* gap, TP* grad3,
                  uint8_t* valid, TP* normal3, TP* witness3, int32_t* branch, hipStream_t s)
@@ -299,6 +374,59 @@ void launchQueryF64(const sdfo_grid& g, int mode, const double* pts, int64_t n, 
                     int32_t* branch, hipStream_t s)
 {
     launchQuery<double, double>(g, mode, pts, n, gap, grad3, valid, normal3, witness3, branch, s);
+}
+
+// ---- batched query plans ------------------------------------------------
+QueryPlan::~QueryPlan()
+{
+    if (table) (void)hipFree(table);
+}
+
+std::unique_ptr<QueryPlan> makeQueryPlan(int ns, const sdfo_grid* const* grids,
+                                         const int64_t* offsets)
+{
+    auto plan = std::make_unique<QueryPlan>();
+    plan->streams = ns;
+    std::vector<StreamDesc> t(static_cast<size_t>(ns));
+    int64_t units = 0;
+    for (int q = 0; q < ns; ++q)
+    {
+        t[size_t(q)].g = dev::makeQGrid(*grids[q]);
+        t[size_t(q)].pointBegin = offsets[q];
+        t[size_t(q)].pointEnd = offsets[q + 1];
+        t[size_t(q)].unitBegin = units;
+        units += (offsets[q + 1] - offsets[q] + 63) / 64;
+    }
+    plan->points = ns > 0 ? offsets[ns] : 0;
+    plan->units = units;
+    SDFO_HIP(hipMalloc(&plan->table, sizeof(StreamDesc) * std::max<size_t>(1, t.size())));
+    if (!t.empty())
+        SDFO_HIP(hipMemcpy(plan->table, t.data(), sizeof(StreamDesc) * t.size(),
+                           hipMemcpyHostToDevice));
+    return plan;
+}
+
+void runQueryPlan(const QueryPlan& plan, int mode, int block, const float* pts, float* gap,
+                  float* grad3, uint8_t* valid, hipStream_t s)
+{
+    if (plan.streams <= 0 || plan.units <= 0) return;
+    const StreamDesc* sd = static_cast<const StreamDesc*>(plan.table);
+    switch (block)
+    {
+    case 64:
+        launchBatchedBlock<64>(mode, sd, plan.streams, plan.units, pts, gap, grad3, valid, s);
+        break;
+    case 128:
+        launchBatchedBlock<128>(mode, sd, plan.streams, plan.units, pts, gap, grad3, valid, s);
+        break;
+    case 512:
+        launchBatchedBlock<512>(mode, sd, plan.streams, plan.units, pts, gap, grad3, valid, s);
+        break;
+    default:
+        launchBatchedBlock<256>(mode, sd, plan.streams, plan.units, pts, gap, grad3, valid, s);
+        break;
+    }
+    SDFO_HIP(hipGetLastError());
 }
 
 } // namespace sdfo

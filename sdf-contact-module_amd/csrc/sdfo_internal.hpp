@@ -4,8 +4,11 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
+#include <memory>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "sdfo.h"
@@ -80,6 +83,7 @@ struct DeviceBvh
     double* box6 = nullptr;
     int32_t* triIdx = nullptr;
     int nnodes = 0;
+    int64_t ntris = 0;
     int depth = 0;
     double rayDir[3] = {0, 0, 0};
     void upload(const HostBvh& h, hipStream_t s);
@@ -168,6 +172,22 @@ namespace sdfo {
 constexpr int kBuildPhases = 8;
 double* lastBuildPhases();
 
+// Autotuner (tuner.cpp): a process-wide table of size-class -> variant
+// decisions, filled by timing every variant on first use when enabled.
+bool tunerEnabled();
+bool tunerLookup(const std::string& key, int& value);
+// Records timings (ms per variant) and keeps the fastest variant for key.
+void tunerRecord(const std::string& key, const std::vector<std::pair<int, double>>& timings,
+                 const char* const* names);
+std::string tunerReport();
+void tunerSetMode(int mode);
+void tunerReset();
+int log2Bucket(int64_t n);
+bool streamCapturing(hipStream_t s);
+// Runs f (which enqueues work on s) and returns its device time in ms.
+double timeOnStream(hipStream_t s, const std::function<void()>& f);
+int fillShapeFor(const GridSpec& g, int64_t ntris);
+
 // Build kernels (build.hip): the calling thread's pinned counter that the last
 // dense or sparse fill reports its replayed voxels to (valid after the stream
 // is synchronised), and a test switch that sends every voxel to the replay pass.
@@ -188,7 +208,20 @@ sdfo_grid* buildContactAware(const double* verts, int64_t nv, const int32_t* tri
                              const double* cornerN, const sdfo_build_params& p, bool sparse,
                              hipStream_t s);
 
+// One mesh of a batched dense build: its grid, its BVH root inside the
+// concatenated node array, and its output fields.
+struct BatchMesh
+{
+    GridSpec g;
+    int root;
+    int64_t ntris;
+    double *phi, *n3, *witness3;
+    int32_t* tri;
+};
+
 // Kernel launchers (build.hip).
+void launchDenseFillBatched(const std::vector<BatchMesh>& meshes, const DeviceBvh& bvh,
+                            hipStream_t s);
 void launchCellPack(const GridSpec& g, const float4* pn, float4* cellRec, hipStream_t s);
 void launchDenseFill(const GridSpec& g, const DeviceBvh& bvh, double* phi, double* n3,
                      double* witness3, int32_t* tri, hipStream_t s);
@@ -214,6 +247,24 @@ void launchQueryF32(const sdfo_grid& g, int mode, const float* pts, int64_t n, f
 void launchQueryF64(const sdfo_grid& g, int mode, const double* pts, int64_t n, double* gap,
                     double* grad3, uint8_t* valid, double* normal3, double* witness3,
                     int32_t* branch, hipStream_t s);
+
+// A batched query plan (query.hip): the device table of (grid, point range)
+// per stream, uploaded once; running it is one kernel launch with no
+// allocation, copy or synchronisation, so it can be captured in a HIP graph.
+struct QueryPlan
+{
+    int streams = 0;
+    int64_t points = 0; // total points over all streams
+    int64_t units = 0;  // 64-point work units
+    void* table = nullptr;
+    ~QueryPlan();
+};
+
+std::unique_ptr<QueryPlan> makeQueryPlan(int ns, const sdfo_grid* const* grids,
+                                         const int64_t* offsets);
+// block: threads per block of the batched kernel (64, 128, 256 or 512).
+void runQueryPlan(const QueryPlan& plan, int mode, int block, const float* pts, float* gap,
+                  float* grad3, uint8_t* valid, hipStream_t s);
 
 // Contact launchers (contact.hip).
 void launchContactForward(const sdfo_grid& g, const double* X, const double* w, int64_t n,

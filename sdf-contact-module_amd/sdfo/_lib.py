@@ -131,6 +131,12 @@ _sig("sdfo_grid_from_dense", [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _vp,
 _sig("sdfo_query", [_vp, _i32, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp])
 _sig("sdfo_query_f64", [_vp, _i32, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp])
 _sig("sdfo_query_batched", [_i32, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp])
+_sig("sdfo_query_plan_create", [_i32, _vp, _vp, _P(_vp)])
+_sig("sdfo_query_plan_run", [_vp, _i32, _vp, _vp, _vp, _vp, _vp])
+_sig("sdfo_query_plan_free", [_vp], None)
+_sig("sdfo_tuner_set_mode", [_i32])
+_sig("sdfo_tuner_reset", [])
+_sig("sdfo_tuner_report", [_vp, _i64], C.c_int64)
 _sig("sdfo_contact_forward", [_vp, _vp, _vp, _i64, _vp, _i64, _P(ContactParams), _vp, _vp, _vp])
 _sig("sdfo_contact_backward", [_vp, _vp, _vp, _i64, _vp, _i64, _P(ContactParams), _vp, _vp, _vp,
                                _vp, _vp, _vp])
@@ -204,3 +210,22 @@ def default_build_params(**overrides) -> BuildParams:
             raise TypeError(f"unknown builder parameter {k!r}")
         setattr(p, k, v)
     return p
+
+
+TUNE_OFF, TUNE_ON_FIRST_USE = 0, 1
+
+
+def tuner_set_mode(mode: int):
+    """SDFO_TUNE_OFF / SDFO_TUNE_ON_FIRST_USE (process-wide)."""
+    check(lib.sdfo_tuner_set_mode(int(mode)))
+
+
+def tuner_reset():
+    check(lib.sdfo_tuner_reset())
+
+
+def tuner_report() -> str:
+    n = lib.sdfo_tuner_report(None, 0)
+    buf = C.create_string_buffer(int(n) + 1)
+    lib.sdfo_tuner_report(buf, len(buf))
+    return buf.value.decode()

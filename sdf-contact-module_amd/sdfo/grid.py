@@ -91,6 +91,32 @@ class SdfGrid:
         return cls(out.value)
 
     @classmethod
+    def build_dense_batched(cls, meshes, params=None, stream=None) -> list:
+        """Many dense builds in one persistent fill launch.
+
+        ``meshes``: list of (verts, tris); ``params``: one dict of builder
+        options for all meshes, or a list of dicts (one per mesh).  Returns
+        one grid per mesh, each equal to ``build_dense`` of that mesh."""
+        n = len(meshes)
+        if n == 0:
+            return []
+        plist = params if isinstance(params, (list, tuple)) else [params or {}] * n
+        if len(plist) != n:
+            raise ValueError("one params dict per mesh")
+        vs, ts = zip(*[_mesh(v, t) for v, t in meshes])
+        vo = np.zeros(n + 1, np.int64)
+        to = np.zeros(n + 1, np.int64)
+        vo[1:] = np.cumsum([len(v) for v in vs])
+        to[1:] = np.cumsum([len(t) for t in ts])
+        V = np.ascontiguousarray(np.concatenate(vs), np.float64)
+        T = np.ascontiguousarray(np.concatenate(ts), np.int32)
+        P = (_lib.BuildParams * n)(*[_lib.default_build_params(**q) for q in plist])
+        outs = (C.c_void_p * n)()
+        check(lib.sdfo_build_dense_batched(n, _np_ptr(V), _np_ptr(vo), _np_ptr(T), _np_ptr(to), P,
+                                           outs, _stream_handle(stream)))
+        return [cls(outs[i]) for i in range(n)]
+
+    @classmethod
     def build_sparse(cls, verts, tris, stream=None, **params) -> "SdfGrid":
         """SdfBuilder::buildSparse (SdfBuilder.cpp:823-983) on the GPU."""
         v, t = _mesh(verts, tris)
@@ -286,6 +312,55 @@ class SdfGrid:
         unit[m] = grad[m] / gn[m, None]
         gap = torch.where(valid, gap, torch.zeros_like(gap))
         return gap, grad, gn, unit, valid
+
+
+class QueryPlan:
+    """Batched FP32 query of many point streams against many grids in one
+    launch.  ``run`` allocates, copies and synchronises nothing, so it can be
+    captured in a ``torch.cuda.CUDAGraph``.  The grids must stay alive."""
+
+    def __init__(self, grids, counts):
+        import ctypes as _C
+
+        self.grids = list(grids)
+        counts = [int(c) for c in counts]
+        if len(counts) != len(self.grids):
+            raise ValueError("one point count per grid")
+        self.offsets = np.zeros(len(counts) + 1, np.int64)
+        self.offsets[1:] = np.cumsum(counts)
+        handles = (_C.c_void_p * max(1, len(self.grids)))(*[g._h for g in self.grids])
+        out = _C.c_void_p()
+        check(lib.sdfo_query_plan_create(len(self.grids), handles, _np_ptr(self.offsets),
+                                         _C.byref(out)))
+        self._h = out.value
+
+    @property
+    def total(self) -> int:
+        return int(self.offsets[-1])
+
+    def run(self, points, gap, grad, valid, mode="Tricubic", stream=None):
+        """points (N,3) f32, gap (N) f32, grad (N,3) f32, valid (N) u8: GPU
+        tensors packed stream after stream (stream s owns rows
+        offsets[s]:offsets[s+1])."""
+        mode = MODES.get(mode, mode)
+        n = self.total
+        for t, shape in ((points, (n, 3)), (gap, (n,)), (grad, (n, 3)), (valid, (n,))):
+            if tuple(t.shape) != shape or not t.is_cuda or not t.is_contiguous():
+                raise ValueError("plan tensors must be contiguous GPU tensors of the plan's size")
+        ptr = lambda t: C.c_void_p(t.data_ptr())  # noqa: E731
+        check(lib.sdfo_query_plan_run(self._h, int(mode), ptr(points), ptr(gap), ptr(grad),
+                                      ptr(valid), _stream_handle(stream)))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.sdfo_query_plan_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def library_version() -> str:

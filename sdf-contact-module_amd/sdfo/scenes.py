@@ -124,3 +124,27 @@ def near_surface_points(verts, tris, n, sigma, seed=0, dtype=np.float32):
     nrm = np.cross(v[idx, 1] - v[idx, 0], v[idx, 2] - v[idx, 0])
     nrm /= np.linalg.norm(nrm, axis=1, keepdims=True) + 1e-300
     return (p + nrm * rng.normal(0.0, sigma, (n, 1))).astype(dtype)
+
+
+def gear_batch(n_meshes=64, seed=5, min_tris=2_000, max_tris=200_000):
+    """Seeded gear-like meshes of uneven size for batched builds.
+
+    Teeth 12..120, face width 10..40 mm, module 1..3 mm, triangle counts
+    log-uniform in [min_tris, max_tris] (set by the axial and per-tooth
+    resolution), each centred at the origin.  Returns a list of
+    ``(verts, tris, target_resolution)`` with the resolution 64, 128 or 256
+    chosen by the triangle count."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n_meshes):
+        teeth = int(rng.integers(12, 121))
+        face = float(rng.uniform(0.010, 0.040))
+        module = float(rng.uniform(0.001, 0.003))
+        want = float(np.exp(rng.uniform(np.log(min_tris), np.log(max_tris))))
+        spt = int(np.clip(round(np.sqrt(want / (2.0 * teeth * 4.0))), 4, 16))
+        P = teeth * spt
+        n_axial = int(np.clip(round((want - 2 * P) / (2 * P)), 1, 400))
+        v, t = meshes.gear(teeth, spt, face, module, n_axial)
+        res = 64 if len(t) < 8_000 else (128 if len(t) < 60_000 else 256)
+        out.append((v, t, res))
+    return out
