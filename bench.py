@@ -64,6 +64,11 @@ def parse():
                     help="subset of the step to run (profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 256 MiB device copy first (scales HBM counters)")
+    ap.add_argument("--workload", default="ring_on_cube",
+                    choices=["ring_on_cube", "sparse_cylinder", "batched_gears"],
+                    help="ring_on_cube (default line); sparse_cylinder: 512^3 narrow band of a "
+                         "266k-triangle tube + 1e6 ContactAware points; batched_gears: 64 gear "
+                         "meshes built in one launch + 64 query streams in one captured graph")
     return ap.parse_args()
 
 
@@ -101,12 +106,222 @@ def cpu_baseline(args, cube_spec, torus_mesh, cube_mesh, pts_np, X, w, poses):
     return res, threads
 
 
+def _events(torch, stream):
+    return torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+
+def _busy(torch):
+    try:
+        torch.cuda._sleep(1_000_000)
+    except (AttributeError, RuntimeError):
+        pass
+
+
+def bench_sparse_cylinder(args):
+    """Tube R_o 40 mm, R_i 30 mm, H 50 mm, 1024 x 64 segments (266 240 triangles),
+    sparse 512^3, 8^3 bricks, band 4 voxels, halo 2; 1e6 FP32 points on the
+    surface offset by N(0, (2h)^2) along the normal, ContactAware value + gradient
+    (the plain sparse grid has no candidates, so ContactAware runs the sparse
+    Tricubic path, SdfOracle.cpp:191-193); then the same with a sparse
+    contact-aware build (face normals as corner normals)."""
+    import torch
+
+    import sdfo
+    from sdfo import meshes, scenes
+
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+    v, t = meshes.hollow_cylinder(0.04, 0.03, 0.05, 1024, 64)
+    prm = dict(target_resolution=512, max_resolution=512, block_dim=8, band_voxels=4,
+               halo_voxels=2)
+    fn = np.cross(v[t[:, 1]] - v[t[:, 0]], v[t[:, 2]] - v[t[:, 0]])
+    fn /= np.linalg.norm(fn, axis=1, keepdims=True)
+    cn = np.repeat(fn[:, None, :], 3, axis=1)
+    g = sdfo.SdfGrid.build_sparse(v, t, **prm)  # warm-up
+    g.close()
+    K = max(1, args.steps)
+    t0 = time.perf_counter()
+    for _ in range(K):
+        g = sdfo.SdfGrid.build_sparse(v, t, **prm)
+        if _ < K - 1:
+            g.close()
+    build_s = (time.perf_counter() - t0) / K
+    info = g.info
+    h = g.spec.voxel.max()
+    pts_np = scenes.near_surface_points(v, t, args.query_points, 2.0 * h, seed=3)
+    pts = torch.from_numpy(pts_np).to(dev)
+    n = len(pts_np)
+    gap = torch.empty(n, device=dev)
+    grad = torch.empty(n, 3, device=dev)
+    valid = torch.empty(n, dtype=torch.uint8, device=dev)
+    for _ in range(args.warmup):
+        g.query_into(pts, gap, grad, valid, "ContactAware")
+    qs = []
+    for _ in range(K):
+        a, b = _events(torch, stream)
+        _busy(torch)
+        a.record(stream)
+        g.query_into(pts, gap, grad, valid, "ContactAware")
+        b.record(stream)
+        torch.cuda.synchronize()
+        qs.append(a.elapsed_time(b))
+    q_ms = float(np.mean(qs))
+    frac_valid = float(valid.float().mean().item())
+    t0 = time.perf_counter()
+    gca = sdfo.SdfGrid.build_contact_aware(v, t, cn, sparse=True, **prm)
+    ca_build_s = time.perf_counter() - t0
+    a, b = _events(torch, stream)
+    gca.query_into(pts, gap, grad, valid, "ContactAware")
+    _busy(torch)
+    a.record(stream)
+    gca.query_into(pts, gap, grad, valid, "ContactAware")
+    b.record(stream)
+    torch.cuda.synchronize()
+    ca_q_ms = a.elapsed_time(b)
+    vox = info.block_count * 512
+    alg = 29.0 * n + 16.0 * vox  # points + the touched band read once (upper bound)
+    line = {
+        "metric": "sparse narrow-band build Mvoxel/s; ContactAware value+grad Mpts/s",
+        "value": round(n / (q_ms * 1e-3) / 1e6, 3), "unit": "Mpts/s", "n_gpus": 1,
+        "steps": K, "warmup": args.warmup, "ms_per_step": round(build_s * 1e3 + q_ms, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 query / f64 build", "data": "synthetic (seeded mesh and points)",
+        "config": {"workload": "sparse hollow cylinder 1024x64 (266240 tris) at 512^3, 8^3 bricks, "
+                               "band 4, halo 2; 1e6 near-surface ContactAware points",
+                   "active_bricks": int(info.block_count), "query_points": n},
+        "sparse_build_ms": round(build_s * 1e3, 3),
+        "sparse_build_mvoxel_per_s": round(vox / build_s / 1e6, 3),
+        "query_kernel_ms": round(q_ms, 4), "valid_fraction": round(frac_valid, 4),
+        "contact_aware_sparse_build_ms": round(ca_build_s * 1e3, 3),
+        "contact_aware_query_kernel_ms": round(ca_q_ms, 4),
+        "roofline": {"bound": "hbm", "achieved": round(alg / (q_ms * 1e-3) / 1e9, 2),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(alg / (q_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "queryKernel<float, float, 4> (ContactAware -> sparse Tricubic)",
+                     "algorithmic_bytes": alg},
+    }
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+
+        if args.cpu_threads > 0:
+            O.set_threads(args.cpu_threads)
+        bijk = g.download_sparse()["bijk"]
+        og = O.setup_grid(v, 0.1, 0.0, 512, 512)
+        rng = np.random.default_rng(0)
+        pick = bijk[rng.choice(len(bijk), size=min(len(bijk), 400), replace=False)]
+        L = np.arange(512)
+        ijk = (pick[:, None, :] * 8 + np.stack([L % 8, (L // 8) % 8, L // 64], 1)[None]).reshape(-1, 3)
+        c = og.bmin + og.voxel * (ijk + 0.5)
+        t0 = time.perf_counter()
+        O.signed_nearest(v, t, c)
+        cpu_rate = len(c) / (time.perf_counter() - t0) / 1e6
+        line["cpu_baseline"] = {
+            "value": round(cpu_rate, 4), "unit": "Mvoxel/s (sparse fill)", "cores": O.num_threads(),
+            "kind": "port", "sample": "FP64 oracle signed nearest over the voxels of 400 random "
+                                      "active bricks (204 800 voxels)"}
+    print(json.dumps(line))
+
+
+def bench_batched_gears(args):
+    """64 seeded gear meshes (2k-200k triangles, grids 64^3-256^3 by size)
+    built by one batched call (one fill launch), then 64 Tricubic query
+    streams of 10k-500k near-surface points against them in one plan launch,
+    replayed from a captured graph."""
+    import torch
+
+    import sdfo
+    from sdfo import scenes
+
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+    batch = scenes.gear_batch(64, seed=5)
+    meshes_ = [(v, t) for v, t, _ in batch]
+    prm = [dict(target_resolution=r, max_resolution=r) for _, _, r in batch]
+    grids = sdfo.SdfGrid.build_dense_batched(meshes_, prm)  # warm-up
+    for g in grids:
+        g.close()
+    K = max(1, args.steps)
+    t0 = time.perf_counter()
+    for k in range(K):
+        grids = sdfo.SdfGrid.build_dense_batched(meshes_, prm)
+        if k < K - 1:
+            for g in grids:
+                g.close()
+    build_s = (time.perf_counter() - t0) / K
+    vox = sum(int(np.prod(g.spec.shape)) for g in grids)
+    rng = np.random.default_rng(9)
+    counts = rng.integers(10_000, 500_001, size=len(grids))
+    pts_np = np.concatenate([scenes.near_surface_points(v, t, int(c), 2.0 * g.spec.voxel.max(),
+                                                        seed=i)
+                             for i, ((v, t), c, g) in enumerate(zip(meshes_, counts, grids))])
+    plan = sdfo.QueryPlan(grids, counts)
+    n = plan.total
+    pts = torch.from_numpy(pts_np.astype(np.float32)).to(dev)
+    gap = torch.empty(n, device=dev)
+    grad = torch.empty(n, 3, device=dev)
+    valid = torch.empty(n, dtype=torch.uint8, device=dev)
+    plan.run(pts, gap, grad, valid, "Tricubic")
+    torch.cuda.synchronize()
+    s = torch.cuda.Stream()
+    s.wait_stream(stream)
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph, stream=s):
+        plan.run(pts, gap, grad, valid, "Tricubic", stream=s)
+    stream.wait_stream(s)
+    qs = []
+    for _ in range(K):
+        a, b = _events(torch, stream)
+        _busy(torch)
+        a.record(stream)
+        graph.replay()
+        b.record(stream)
+        torch.cuda.synchronize()
+        qs.append(a.elapsed_time(b))
+    q_ms = float(np.mean(qs))
+    line = {
+        "metric": "batched build Mvoxel/s; batched tricubic Mpts/s",
+        "value": round(vox / build_s / 1e6, 3), "unit": "Mvoxel/s", "n_gpus": 1, "steps": K,
+        "warmup": 1, "ms_per_step": round(build_s * 1e3 + q_ms, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64 build / f32 query",
+        "data": "synthetic (seeded gear meshes and points)",
+        "config": {"workload": "64 gear meshes (12-120 teeth, 2k-200k tris, grids 64-256 per "
+                               "axis) in one batched build; 64 query streams 10k-500k points",
+                   "meshes": len(grids), "triangles": int(sum(len(t) for _, t in meshes_)),
+                   "voxels": vox, "query_points": n},
+        "batched_build_ms": round(build_s * 1e3, 3),
+        "batched_query_graph_ms": round(q_ms, 4),
+        "batched_query_mpts_per_s": round(n / (q_ms * 1e-3) / 1e6, 3),
+        "tuner": sdfo.tuner_report(),
+    }
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+
+        if args.cpu_threads > 0:
+            O.set_threads(args.cpu_threads)
+        v, t, r = min(batch, key=lambda b: len(b[1]))
+        og = O.setup_grid(v, 0.1, 0.0, r, r)
+        t0 = time.perf_counter()
+        O.build_dense(v, t, og)
+        cpu_rate = og.total / (time.perf_counter() - t0) / 1e6
+        line["cpu_baseline"] = {
+            "value": round(cpu_rate, 4), "unit": "Mvoxel/s", "cores": O.num_threads(),
+            "kind": "port", "sample": f"FP64 oracle dense build of the smallest gear "
+                                      f"({len(t)} tris, grid {list(og.shape)})"}
+    print(json.dumps(line))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(every box has one GPU); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "sparse_cylinder":
+        return bench_sparse_cylinder(args)
+    if args.workload == "batched_gears":
+        return bench_batched_gears(args)
     import torch
 
     import sdfo
