@@ -182,6 +182,31 @@ int sdfo_grid_from_dense(const double* bmin3, const double* bmax3, const double*
                          const uint8_t* ccount, const double* cphi, const double* cn3,
                          const int32_t* cbranch, sdfo_grid** out);
 
+/* Uploads a host SparseSdfData (SdfData.h:154-303; blocks in their stored
+ * order: bijk[3*nb], per block phi[B], n[3B], H[6B] when has_hessian, and the
+ * candidate arrays when stride > 0; B = block_dim^3) as a grid, as
+ * SdfOracle(SparseSdfData) does (SdfOracle.cpp:160-163). */
+int sdfo_grid_from_sparse(const double* bmin3, const double* bmax3, const double* voxel3,
+                          const int32_t* shape3, int32_t block_dim, int32_t halo, double band,
+                          int32_t has_hessian, int32_t stride, int32_t n_blocks,
+                          const int32_t* bijk, const float* phi, const float* n3, const float* h6,
+                          const uint8_t* ccount, const float* cphi, const float* cn3,
+                          const int32_t* cbranch, sdfo_grid** out);
+
+/* saveSdf / saveSparseSdf (SdfIO.cpp:93-275) of host arrays (the layouts of
+ * sdfo_grid_from_dense / sdfo_grid_from_sparse; h6 NULL writes zeros). */
+int sdfo_save_dense_host(const char* path, const double* bmin3, const double* bmax3,
+                         const double* voxel3, const int32_t* shape3, const double* phi,
+                         const double* n3, const double* h6, const double* witness3, int32_t stride,
+                         const uint8_t* ccount, const double* cphi, const double* cn3,
+                         const int32_t* cbranch);
+int sdfo_save_sparse_host(const char* path, const double* bmin3, const double* bmax3,
+                          const double* voxel3, const int32_t* shape3, int32_t block_dim,
+                          int32_t halo, double band, int32_t has_hessian, int32_t stride,
+                          int32_t n_blocks, const int32_t* bijk, const float* phi, const float* n3,
+                          const float* h6, const uint8_t* ccount, const float* cphi,
+                          const float* cn3, const int32_t* cbranch);
+
 /* ---- queries ----------------------------------------------------------- */
 /* SdfOracle::queryBatch / queryFast (SdfOracle.cpp:176-444) for FP32 point
  * streams, FP32 arithmetic with FP64 cell location.
@@ -197,6 +222,21 @@ int sdfo_query(const sdfo_grid* grid, int32_t mode, const float* points, int64_t
 int sdfo_query_f64(const sdfo_grid* grid, int32_t mode, const double* points, int64_t n,
                    double* gap, double* grad3, uint8_t* valid, double* normal3,
                    double* witness3, int32_t* branch, sdfo_stream stream);
+
+/* SdfOracle::queryFast / queryBatch (SdfOracle.cpp:176-444) on host arrays
+ * of FP64 points, for per-point callers: copies in, runs the FP64 query
+ * kernels, copies out, synchronises the stream.  gap[n], normal3[3n],
+ * witness3[3n], valid[n] are required; branch[n] optional.  With
+ * cand_count != NULL and mode SDFO_CONTACT_AWARE on a grid with candidates,
+ * every candidate is returned too (cand_gap[8n], cand_normal3[24n],
+ * cand_witness3[24n], cand_branch[8n]; cand_count[i] = 0 where the point took
+ * the tricubic / first-order fallback), computed from the FP64 (dense) or
+ * FP32 (sparse) candidate arrays without FMA contraction. */
+int sdfo_query_host_f64(const sdfo_grid* grid, int32_t mode, const double* points, int64_t n,
+                        double* gap, double* normal3, double* witness3, uint8_t* valid,
+                        int32_t* branch, uint8_t* cand_count, double* cand_gap,
+                        double* cand_normal3, double* cand_witness3, int32_t* cand_branch,
+                        sdfo_stream stream);
 
 /* Many query streams against many grids in one launch (no reference
  * counterpart).  Stream s queries grids[s] with points

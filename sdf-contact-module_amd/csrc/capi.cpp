@@ -849,6 +849,95 @@ int sdfo_build_dense_batched(int32_t n_meshes, const double* verts, const int64_
     });
 }
 
+int sdfo_query_host_f64(const sdfo_grid* grid, int32_t mode, const double* points, int64_t n,
+                        double* gap, double* normal3, double* witness3, uint8_t* valid,
+                        int32_t* branch, uint8_t* cand_count, double* cand_gap,
+                        double* cand_normal3, double* cand_witness3, int32_t* cand_branch,
+                        sdfo_stream stream)
+{
+    return guarded([&] {
+        if (!grid) throw Error(SDFO_ERR_INVALID_ARGUMENT, "grid is NULL");
+        if (mode < SDFO_FIRST_ORDER || mode > SDFO_CONTACT_AWARE)
+            throw Error(SDFO_ERR_INVALID_ARGUMENT, "unknown query mode");
+        if (n < 0) throw Error(SDFO_ERR_INVALID_ARGUMENT, "n must be >= 0");
+        if (n == 0) return;
+        if (!points || !gap || !normal3 || !witness3 || !valid)
+            throw Error(SDFO_ERR_INVALID_ARGUMENT, "points, gap, normal3, witness3, valid are required");
+        const bool cands = cand_count != nullptr;
+        if (cands && (!cand_gap || !cand_normal3 || !cand_witness3 || !cand_branch))
+            throw Error(SDFO_ERR_INVALID_ARGUMENT, "candidate outputs must all be given");
+        DeviceGuard dg(grid->device);
+        const hipStream_t s = hipStream_t(stream);
+        const size_t N = size_t(n);
+        // one device block for every array: points, gap, grad, normal, witness,
+        // branch, valid, then the candidate arrays
+        size_t off = 0;
+        auto carve = [&](size_t bytes) {
+            const size_t o = off;
+            off += (bytes + 255) & ~size_t(255);
+            return o;
+        };
+        const size_t oP = carve(24 * N), oG = carve(8 * N), oGr = carve(24 * N), oN = carve(24 * N),
+                     oW = carve(24 * N), oB = carve(4 * N), oV = carve(N);
+        size_t oCC = 0, oCG = 0, oCN = 0, oCW = 0, oCB = 0;
+        if (cands)
+        {
+            oCC = carve(N);
+            oCG = carve(64 * N);
+            oCN = carve(192 * N);
+            oCW = carve(192 * N);
+            oCB = carve(32 * N);
+        }
+        char* base = nullptr;
+        SDFO_HIP(hipMallocAsync(reinterpret_cast<void**>(&base), off, s));
+        try
+        {
+            auto D = [&](size_t o) { return base + o; };
+            SDFO_HIP(hipMemcpyAsync(D(oP), points, 24 * N, hipMemcpyHostToDevice, s));
+            launchQueryF64(*grid, mode, reinterpret_cast<double*>(D(oP)), n,
+                           reinterpret_cast<double*>(D(oG)), reinterpret_cast<double*>(D(oGr)),
+                           reinterpret_cast<uint8_t*>(D(oV)), reinterpret_cast<double*>(D(oN)),
+                           reinterpret_cast<double*>(D(oW)), reinterpret_cast<int32_t*>(D(oB)), s);
+            if (cands)
+            {
+                SDFO_HIP(hipMemsetAsync(D(oCC), 0, N, s));
+                if (mode == SDFO_CONTACT_AWARE)
+                    launchCandidatesF64(*grid, reinterpret_cast<double*>(D(oP)), n,
+                                        reinterpret_cast<uint8_t*>(D(oCC)),
+                                        reinterpret_cast<double*>(D(oCG)),
+                                        reinterpret_cast<double*>(D(oCN)),
+                                        reinterpret_cast<double*>(D(oCW)),
+                                        reinterpret_cast<int32_t*>(D(oCB)),
+                                        reinterpret_cast<double*>(D(oG)),
+                                        reinterpret_cast<double*>(D(oN)),
+                                        reinterpret_cast<double*>(D(oW)),
+                                        reinterpret_cast<uint8_t*>(D(oV)), s);
+            }
+            SDFO_HIP(hipMemcpyAsync(gap, D(oG), 8 * N, hipMemcpyDeviceToHost, s));
+            SDFO_HIP(hipMemcpyAsync(normal3, D(oN), 24 * N, hipMemcpyDeviceToHost, s));
+            SDFO_HIP(hipMemcpyAsync(witness3, D(oW), 24 * N, hipMemcpyDeviceToHost, s));
+            SDFO_HIP(hipMemcpyAsync(valid, D(oV), N, hipMemcpyDeviceToHost, s));
+            if (branch) SDFO_HIP(hipMemcpyAsync(branch, D(oB), 4 * N, hipMemcpyDeviceToHost, s));
+            if (cands)
+            {
+                SDFO_HIP(hipMemcpyAsync(cand_count, D(oCC), N, hipMemcpyDeviceToHost, s));
+                SDFO_HIP(hipMemcpyAsync(cand_gap, D(oCG), 64 * N, hipMemcpyDeviceToHost, s));
+                SDFO_HIP(hipMemcpyAsync(cand_normal3, D(oCN), 192 * N, hipMemcpyDeviceToHost, s));
+                SDFO_HIP(hipMemcpyAsync(cand_witness3, D(oCW), 192 * N, hipMemcpyDeviceToHost, s));
+                SDFO_HIP(hipMemcpyAsync(cand_branch, D(oCB), 32 * N, hipMemcpyDeviceToHost, s));
+            }
+            SDFO_HIP(hipStreamSynchronize(s));
+        }
+        catch (...)
+        {
+            (void)hipStreamSynchronize(s);
+            (void)hipFreeAsync(base, s);
+            throw;
+        }
+        SDFO_HIP(hipFreeAsync(base, s));
+    });
+}
+
 int sdfo_query_plan_create(int32_t n_streams, const sdfo_grid* const* grids,
                            const int64_t* offsets, sdfo_query_plan** out)
 {

@@ -5,6 +5,7 @@
 #include <fstream>
 #include <memory>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "sdfo_internal.hpp"
@@ -99,84 +100,162 @@ void validateCommon(const Header& h)
                                          " (only 0.1.0 is supported)");
 }
 
-// saveSdf (SdfIO.cpp:93-183).
-void saveDense(const sdfo_grid& g, std::ofstream& out)
+// Host views of a grid's arrays (the file layout's element types).
+struct DenseHost
 {
-    const size_t N = size_t(g.spec.total());
-    const bool contact = g.stride > 0;
-    Header h = makeHeader(g.spec, kFlagHessian | (contact ? kFlagContact : 0u));
+    GridSpec spec;
+    const double *phi, *n3, *h6, *wit3;
+    int stride;
+    const uint8_t* cc;
+    const double *cphi, *cn3;
+    const int32_t* cbr;
+};
+
+struct SparseHost
+{
+    GridSpec spec;
+    int bd, halo, nb;
+    double band;
+    bool hasHessian;
+    int stride;
+    const int32_t* bijk;
+    const float *phi, *n3, *h6;
+    const uint8_t* cc;
+    const float *cphi, *cn3;
+    const int32_t* cbr;
+};
+
+// saveSdf (SdfIO.cpp:93-183) from host arrays.
+void writeDense(const DenseHost& d, std::ofstream& out)
+{
+    const size_t N = size_t(d.spec.total());
+    const bool contact = d.stride > 0;
+    Header h = makeHeader(d.spec, kFlagHessian | (contact ? kFlagContact : 0u));
     h.put<uint32_t>(Header::kReserved + 0, contact ? kLayoutDenseContact : kLayoutDense);
-    if (contact) h.put<uint32_t>(Header::kReserved + 4, uint32_t(g.stride));
+    if (contact) h.put<uint32_t>(Header::kReserved + 4, uint32_t(d.stride));
     out.write(reinterpret_cast<const char*>(h.b), 128);
-    auto wr = [&](const std::vector<double>& v) {
-        out.write(reinterpret_cast<const char*>(v.data()), std::streamsize(v.size() * sizeof(double)));
+    auto wr = [&](const void* p, size_t bytes) {
+        out.write(reinterpret_cast<const char*>(p), std::streamsize(bytes));
     };
-    wr(download(g.phi, N));
-    wr(download(g.n3, 3 * N));
-    wr(download(g.h6, 6 * N));
-    wr(download(g.witness3, 3 * N));
+    const std::vector<double> zeros6(d.h6 ? 0 : 6 * N, 0.0);
+    wr(d.phi, N * 8);
+    wr(d.n3, 3 * N * 8);
+    wr(d.h6 ? d.h6 : zeros6.data(), 6 * N * 8);
+    wr(d.wit3, 3 * N * 8);
     if (contact)
     {
-        const size_t NS = N * size_t(g.stride);
-        const std::vector<uint8_t> c8 = download(g.ccount, N);
-        std::vector<uint32_t> c32(c8.begin(), c8.end());
-        out.write(reinterpret_cast<const char*>(c32.data()), std::streamsize(N * 4));
-        wr(download(g.cphi, NS));
-        wr(download(g.cn3, 3 * NS));
-        const std::vector<int32_t> br = download(g.cbranch, NS);
-        out.write(reinterpret_cast<const char*>(br.data()), std::streamsize(NS * 4));
+        const size_t NS = N * size_t(d.stride);
+        std::vector<uint32_t> c32(d.cc, d.cc + N);
+        wr(c32.data(), N * 4);
+        wr(d.cphi, NS * 8);
+        wr(d.cn3, 3 * NS * 8);
+        wr(d.cbr, NS * 4);
     }
 }
 
-// saveSparseSdf (SdfIO.cpp:185-275).
-void saveSparse(const sdfo_grid& g, std::ofstream& out)
+void saveDense(const sdfo_grid& g, std::ofstream& out)
 {
-    const int bd = g.blockDim;
+    const size_t N = size_t(g.spec.total());
+    DenseHost d{};
+    d.spec = g.spec;
+    d.stride = g.stride;
+    const std::vector<double> phi = download(g.phi, N), n3 = download(g.n3, 3 * N),
+                              h6 = download(g.h6, 6 * N), w3 = download(g.witness3, 3 * N);
+    d.phi = phi.data();
+    d.n3 = n3.data();
+    d.h6 = h6.data();
+    d.wit3 = w3.data();
+    std::vector<uint8_t> cc;
+    std::vector<double> cphi, cn3;
+    std::vector<int32_t> cbr;
+    if (g.stride > 0)
+    {
+        const size_t NS = N * size_t(g.stride);
+        cc = download(g.ccount, N);
+        cphi = download(g.cphi, NS);
+        cn3 = download(g.cn3, 3 * NS);
+        cbr = download(g.cbranch, NS);
+        d.cc = cc.data();
+        d.cphi = cphi.data();
+        d.cn3 = cn3.data();
+        d.cbr = cbr.data();
+    }
+    writeDense(d, out);
+}
+
+// saveSparseSdf (SdfIO.cpp:185-275) from host arrays.
+void writeSparse(const SparseHost& d, std::ofstream& out)
+{
+    const int bd = d.bd;
     const size_t bvc = size_t(bd) * bd * bd;
-    const bool contact = g.stride > 0 && g.nb > 0;
-    Header h = makeHeader(g.spec, kFlagSparse | (g.hasHessian ? kFlagHessian : 0u) |
+    const bool contact = d.stride > 0 && d.nb > 0;
+    Header h = makeHeader(d.spec, kFlagSparse | (d.hasHessian ? kFlagHessian : 0u) |
                                       (contact ? kFlagContact : 0u));
     h.put<uint32_t>(Header::kReserved + 0, contact ? kLayoutSparseContact : kLayoutSparse);
     h.put<uint32_t>(Header::kReserved + 4, uint32_t(bd));
-    h.put<uint32_t>(Header::kReserved + 8, uint32_t(g.halo));
-    h.put<uint32_t>(Header::kReserved + 12, uint32_t(g.nb));
-    h.put<double>(Header::kReserved + 16, g.band);
-    if (contact) h.put<uint32_t>(Header::kReserved + 24, uint32_t(g.stride));
+    h.put<uint32_t>(Header::kReserved + 8, uint32_t(d.halo));
+    h.put<uint32_t>(Header::kReserved + 12, uint32_t(d.nb));
+    h.put<double>(Header::kReserved + 16, d.band);
+    if (contact) h.put<uint32_t>(Header::kReserved + 24, uint32_t(d.stride));
     out.write(reinterpret_cast<const char*>(h.b), 128);
+    const size_t S = size_t(d.stride);
+    const std::vector<float> zeros6(d.hasHessian && !d.h6 ? 6 * bvc : 0, 0.0f);
+    for (int b = 0; b < d.nb; ++b)
+    {
+        int32_t bh[4] = {d.bijk[3 * b], d.bijk[3 * b + 1], d.bijk[3 * b + 2], 0};
+        out.write(reinterpret_cast<const char*>(bh), 16);
+        out.write(reinterpret_cast<const char*>(&d.phi[b * bvc]), std::streamsize(bvc * 4));
+        out.write(reinterpret_cast<const char*>(&d.n3[3 * b * bvc]), std::streamsize(3 * bvc * 4));
+        if (d.hasHessian)
+            out.write(reinterpret_cast<const char*>(d.h6 ? &d.h6[6 * b * bvc] : zeros6.data()),
+                      std::streamsize(6 * bvc * 4));
+        if (contact)
+        {
+            out.write(reinterpret_cast<const char*>(&d.cc[b * bvc]), std::streamsize(bvc));
+            out.write(reinterpret_cast<const char*>(&d.cphi[b * bvc * S]), std::streamsize(bvc * S * 4));
+            out.write(reinterpret_cast<const char*>(&d.cn3[3 * b * bvc * S]),
+                      std::streamsize(3 * bvc * S * 4));
+            out.write(reinterpret_cast<const char*>(&d.cbr[b * bvc * S]), std::streamsize(bvc * S * 4));
+        }
+    }
+}
+
+void saveSparse(const sdfo_grid& g, std::ofstream& out)
+{
+    const size_t bvc = size_t(g.blockDim) * g.blockDim * g.blockDim;
     const size_t cnt = size_t(g.nb) * bvc;
+    SparseHost d{};
+    d.spec = g.spec;
+    d.bd = g.blockDim;
+    d.halo = g.halo;
+    d.nb = g.nb;
+    d.band = g.band;
+    d.hasHessian = g.hasHessian;
+    d.stride = g.stride;
     const std::vector<int32_t> bijk = download(g.bijk, size_t(g.nb) * 3);
-    const std::vector<float> phi = download(g.sphi, cnt);
-    const std::vector<float> n3 = download(g.sn3, 3 * cnt);
+    const std::vector<float> phi = download(g.sphi, cnt), n3 = download(g.sn3, 3 * cnt);
     std::vector<float> h6;
-    if (g.hasHessian) h6 = g.sh6 ? download(g.sh6, 6 * cnt) : std::vector<float>(6 * cnt, 0.0f);
+    if (g.hasHessian && g.sh6) h6 = download(g.sh6, 6 * cnt);
+    d.bijk = bijk.data();
+    d.phi = phi.data();
+    d.n3 = n3.data();
+    d.h6 = h6.empty() ? nullptr : h6.data();
     const size_t S = size_t(g.stride);
     std::vector<uint8_t> cc;
     std::vector<float> cphi, cn3;
     std::vector<int32_t> cbr;
-    if (contact)
+    if (g.stride > 0 && g.nb > 0)
     {
         cc = download(g.sccount, cnt);
         cphi = download(g.scphi, cnt * S);
         cn3 = download(g.scn3, 3 * cnt * S);
         cbr = download(g.scbranch, cnt * S);
+        d.cc = cc.data();
+        d.cphi = cphi.data();
+        d.cn3 = cn3.data();
+        d.cbr = cbr.data();
     }
-    for (int b = 0; b < g.nb; ++b)
-    {
-        int32_t bh[4] = {bijk[3 * b], bijk[3 * b + 1], bijk[3 * b + 2], 0};
-        out.write(reinterpret_cast<const char*>(bh), 16);
-        out.write(reinterpret_cast<const char*>(&phi[b * bvc]), std::streamsize(bvc * 4));
-        out.write(reinterpret_cast<const char*>(&n3[3 * b * bvc]), std::streamsize(3 * bvc * 4));
-        if (g.hasHessian)
-            out.write(reinterpret_cast<const char*>(&h6[6 * b * bvc]), std::streamsize(6 * bvc * 4));
-        if (contact)
-        {
-            out.write(reinterpret_cast<const char*>(&cc[b * bvc]), std::streamsize(bvc));
-            out.write(reinterpret_cast<const char*>(&cphi[b * bvc * S]), std::streamsize(bvc * S * 4));
-            out.write(reinterpret_cast<const char*>(&cn3[3 * b * bvc * S]),
-                      std::streamsize(3 * bvc * S * 4));
-            out.write(reinterpret_cast<const char*>(&cbr[b * bvc * S]), std::streamsize(bvc * S * 4));
-        }
-    }
+    writeSparse(d, out);
 }
 
 // loadSdf (SdfIO.cpp:279-394).
@@ -253,6 +332,33 @@ std::unique_ptr<sdfo_grid> loadDense(const std::string& path)
     return g;
 }
 
+// Uploads host sparse arrays (blocks in file order) into a grid whose spec,
+// blockDim, halo, band, hasHessian, stride and nb are set.
+void uploadSparse(sdfo_grid& g, const std::vector<int32_t>& bijk, const std::vector<float>& phi,
+                  const std::vector<float>& n3, const std::vector<float>& h6,
+                  const std::vector<uint8_t>& cc, const std::vector<float>& cphi,
+                  const std::vector<float>& cn3, const std::vector<int32_t>& cbr)
+{
+    const int bd = g.blockDim;
+    SDFO_HIP(hipGetDevice(&g.device));
+    g.nbx = (g.spec.n[0] + bd - 1) / bd;
+    g.nby = (g.spec.n[1] + bd - 1) / bd;
+    g.nbz = (g.spec.n[2] + bd - 1) / bd;
+    g.bijk = upload(g, bijk);
+    g.sphi = upload(g, phi);
+    g.sn3 = upload(g, n3);
+    if (g.hasHessian) g.sh6 = upload(g, h6);
+    if (g.stride > 0)
+    {
+        g.sccount = upload(g, cc);
+        g.scphi = upload(g, cphi);
+        g.scn3 = upload(g, cn3);
+        g.scbranch = upload(g, cbr);
+    }
+    finishSparse(g, bijk, nullptr);
+    SDFO_HIP(hipDeviceSynchronize());
+}
+
 // loadSparseSdf (SdfIO.cpp:396-523).
 std::unique_ptr<sdfo_grid> loadSparse(const std::string& path)
 {
@@ -323,24 +429,8 @@ std::unique_ptr<sdfo_grid> loadSparse(const std::string& path)
         }
         if (!in) throw Error(SDFO_ERR_FORMAT, "Failed to read sparse SDF block payload");
     }
-    SDFO_HIP(hipGetDevice(&g->device));
     g->nb = int(nblocks);
-    g->nbx = (g->spec.n[0] + bd - 1) / bd;
-    g->nby = (g->spec.n[1] + bd - 1) / bd;
-    g->nbz = (g->spec.n[2] + bd - 1) / bd;
-    g->bijk = upload(*g, bijk);
-    g->sphi = upload(*g, phi);
-    g->sn3 = upload(*g, n3);
-    if (g->hasHessian) g->sh6 = upload(*g, h6);
-    if (contact)
-    {
-        g->sccount = upload(*g, cc);
-        g->scphi = upload(*g, cphi);
-        g->scn3 = upload(*g, cn3);
-        g->scbranch = upload(*g, cbr);
-    }
-    finishSparse(*g, bijk, nullptr);
-    SDFO_HIP(hipDeviceSynchronize());
+    uploadSparse(*g, bijk, phi, n3, h6, cc, cphi, cn3, cbr);
     return g;
 }
 
@@ -431,5 +521,138 @@ int sdfo_is_valid_file(const char* path)
 }
 
 int sdfo_is_sparse_file(const char* path) { return (path && isSparse(path)) ? 1 : 0; }
+
+int sdfo_save_dense_host(const char* path, const double* bmin3, const double* bmax3,
+                         const double* voxel3, const int32_t* shape3, const double* phi,
+                         const double* n3, const double* h6, const double* witness3, int32_t stride,
+                         const uint8_t* ccount, const double* cphi, const double* cn3,
+                         const int32_t* cbranch)
+{
+    return run([&] {
+        if (!path || !bmin3 || !bmax3 || !voxel3 || !shape3 || !phi || !n3 || !witness3)
+            throw Error(SDFO_ERR_INVALID_ARGUMENT, "NULL argument");
+        if (stride > 0 && (!ccount || !cphi || !cn3 || !cbranch))
+            throw Error(SDFO_ERR_INVALID_ARGUMENT, "candidate arrays are required when stride > 0");
+        DenseHost d{};
+        for (int a = 0; a < 3; ++a)
+        {
+            d.spec.bmin[a] = bmin3[a];
+            d.spec.bmax[a] = bmax3[a];
+            d.spec.voxel[a] = voxel3[a];
+            d.spec.n[a] = shape3[a];
+        }
+        d.phi = phi;
+        d.n3 = n3;
+        d.h6 = h6;
+        d.wit3 = witness3;
+        d.stride = stride > 0 ? stride : 0;
+        d.cc = ccount;
+        d.cphi = cphi;
+        d.cn3 = cn3;
+        d.cbr = cbranch;
+        std::ofstream out(path, std::ios::binary);
+        if (!out.is_open()) throw Error(SDFO_ERR_IO, std::string("Cannot open file for writing: ") + path);
+        writeDense(d, out);
+        out.flush();
+        if (!out.good()) throw Error(SDFO_ERR_IO, std::string("Failed to write ") + path);
+    });
+}
+
+int sdfo_save_sparse_host(const char* path, const double* bmin3, const double* bmax3,
+                          const double* voxel3, const int32_t* shape3, int32_t block_dim,
+                          int32_t halo, double band, int32_t has_hessian, int32_t stride,
+                          int32_t n_blocks, const int32_t* bijk, const float* phi, const float* n3,
+                          const float* h6, const uint8_t* ccount, const float* cphi,
+                          const float* cn3, const int32_t* cbranch)
+{
+    return run([&] {
+        if (!path || !bmin3 || !bmax3 || !voxel3 || !shape3)
+            throw Error(SDFO_ERR_INVALID_ARGUMENT, "NULL argument");
+        if (n_blocks < 0 || block_dim <= 0)
+            throw Error(SDFO_ERR_INVALID_ARGUMENT, "Sparse SDF blockDim must be positive");
+        if (n_blocks > 0 && (!bijk || !phi || !n3))
+            throw Error(SDFO_ERR_INVALID_ARGUMENT, "block arrays are required");
+        if (stride > 0 && n_blocks > 0 && (!ccount || !cphi || !cn3 || !cbranch))
+            throw Error(SDFO_ERR_INVALID_ARGUMENT, "candidate arrays are required when stride > 0");
+        SparseHost d{};
+        for (int a = 0; a < 3; ++a)
+        {
+            d.spec.bmin[a] = bmin3[a];
+            d.spec.bmax[a] = bmax3[a];
+            d.spec.voxel[a] = voxel3[a];
+            d.spec.n[a] = shape3[a];
+        }
+        d.bd = block_dim;
+        d.halo = halo;
+        d.nb = n_blocks;
+        d.band = band;
+        d.hasHessian = has_hessian != 0;
+        d.stride = stride > 0 ? stride : 0;
+        d.bijk = bijk;
+        d.phi = phi;
+        d.n3 = n3;
+        d.h6 = h6;
+        d.cc = ccount;
+        d.cphi = cphi;
+        d.cn3 = cn3;
+        d.cbr = cbranch;
+        std::ofstream out(path, std::ios::binary);
+        if (!out.is_open()) throw Error(SDFO_ERR_IO, std::string("Cannot open file for writing: ") + path);
+        writeSparse(d, out);
+        out.flush();
+        if (!out.good()) throw Error(SDFO_ERR_IO, std::string("Failed to write ") + path);
+    });
+}
+
+int sdfo_grid_from_sparse(const double* bmin3, const double* bmax3, const double* voxel3,
+                          const int32_t* shape3, int32_t block_dim, int32_t halo, double band,
+                          int32_t has_hessian, int32_t stride, int32_t n_blocks,
+                          const int32_t* bijk, const float* phi, const float* n3, const float* h6,
+                          const uint8_t* ccount, const float* cphi, const float* cn3,
+                          const int32_t* cbranch, sdfo_grid** out)
+{
+    return run([&] {
+        if (!out || !bmin3 || !bmax3 || !voxel3 || !shape3)
+            throw Error(SDFO_ERR_INVALID_ARGUMENT, "NULL argument");
+        *out = nullptr;
+        if (block_dim <= 0 || block_dim > 64)
+            throw Error(SDFO_ERR_INVALID_ARGUMENT, "Sparse SDF blockDim must be in 1..64");
+        if (n_blocks < 0 || stride < 0 || stride > 8)
+            throw Error(SDFO_ERR_INVALID_ARGUMENT, "invalid block count or candidate stride");
+        const size_t bvc = size_t(block_dim) * block_dim * block_dim, cnt = size_t(n_blocks) * bvc;
+        if (n_blocks > 0 && (!bijk || !phi || !n3 || (has_hessian && !h6)))
+            throw Error(SDFO_ERR_INVALID_ARGUMENT, "block arrays are required");
+        if (stride > 0 && n_blocks > 0 && (!ccount || !cphi || !cn3 || !cbranch))
+            throw Error(SDFO_ERR_INVALID_ARGUMENT, "candidate arrays are required when stride > 0");
+        auto g = std::make_unique<sdfo_grid>();
+        g->sparse = true;
+        for (int a = 0; a < 3; ++a)
+        {
+            g->spec.bmin[a] = bmin3[a];
+            g->spec.bmax[a] = bmax3[a];
+            g->spec.voxel[a] = voxel3[a];
+            g->spec.n[a] = shape3[a];
+            if (shape3[a] <= 0) throw Error(SDFO_ERR_INVALID_ARGUMENT, "empty grid");
+        }
+        g->blockDim = block_dim;
+        g->halo = halo;
+        g->band = band;
+        g->hasHessian = has_hessian != 0;
+        g->stride = stride;
+        g->nb = n_blocks;
+        const size_t S = size_t(stride);
+        auto vec = [](auto* p, size_t n) {
+            using T = std::remove_const_t<std::remove_pointer_t<decltype(p)>>;
+            return p ? std::vector<T>(p, p + n) : std::vector<T>();
+        };
+        uploadSparse(*g, vec(bijk, size_t(n_blocks) * 3), vec(phi, cnt), vec(n3, 3 * cnt),
+                     has_hessian ? vec(h6, 6 * cnt) : std::vector<float>(),
+                     stride > 0 ? vec(ccount, cnt) : std::vector<uint8_t>(),
+                     stride > 0 ? vec(cphi, cnt * S) : std::vector<float>(),
+                     stride > 0 ? vec(cn3, 3 * cnt * S) : std::vector<float>(),
+                     stride > 0 ? vec(cbranch, cnt * S) : std::vector<int32_t>());
+        *out = g.release();
+    });
+}
 
 } // extern "C"

@@ -266,6 +266,14 @@ std::unique_ptr<QueryPlan> makeQueryPlan(int ns, const sdfo_grid* const* grids,
 void runQueryPlan(const QueryPlan& plan, int mode, int block, const float* pts, float* gap,
                   float* grad3, uint8_t* valid, hipStream_t s);
 
+// Every ContactAware candidate of FP64 points (query_exact.hip); arrays of
+// 8 candidates per point; points without candidates get count 0 and keep
+// their gap / normal / witness / valid.
+bool gridHasCandidates(const sdfo_grid& g);
+void launchCandidatesF64(const sdfo_grid& g, const double* pts, int64_t n, uint8_t* count,
+                         double* cgap, double* cn3, double* cw3, int32_t* cb, double* gap,
+                         double* normal3, double* witness3, uint8_t* valid, hipStream_t s);
+
 // Contact launchers (contact.hip).
 void launchContactForward(const sdfo_grid& g, const double* X, const double* w, int64_t n,
                           const double* poses, int64_t nb, const sdfo_contact_params& p,

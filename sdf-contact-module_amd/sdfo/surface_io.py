@@ -6,6 +6,7 @@ Host-side text parsing; returns numpy arrays ready for ``SdfGrid.build_*``.
 """
 from __future__ import annotations
 
+import math
 import re
 
 import numpy as np
@@ -115,8 +116,9 @@ def load_contact_aware_surface(path: str):
     verts, tris, cnorm = [], [], []
 
     def fresh():
-        return dict(active=False, name="", ev=-1, et=-1, sv=0, st=0, T=np.eye(4), N=np.eye(3),
-                    ids={})
+        return dict(active=False, name="", ev=-1, et=-1, sv=0, st=0,
+                    T=[[1.0 if r == c else 0.0 for c in range(4)] for r in range(4)],
+                    N=[[1.0 if r == c else 0.0 for c in range(3)] for r in range(3)], ids={})
 
     st = fresh()
 
@@ -167,12 +169,17 @@ def load_contact_aware_surface(path: str):
                 vals = [s.double() for _ in range(16)]
                 if any(v is None for v in vals):
                     raise err(ln, "transform requires 16 row-major values")
-                T = np.asarray(vals, np.float64).reshape(4, 4)
-                L = T[:3, :3]
-                if abs(np.linalg.det(L)) <= 1.0e-15:
+                T = [[float(vals[4 * r + c]) for c in range(4)] for r in range(4)]
+                det = (T[0][0] * (T[1][1] * T[2][2] - T[1][2] * T[2][1]) -
+                       T[0][1] * (T[1][0] * T[2][2] - T[1][2] * T[2][0]) +
+                       T[0][2] * (T[1][0] * T[2][1] - T[1][1] * T[2][0]))
+                if abs(det) <= 1.0e-15:
                     raise err(ln, "transform linear part must be invertible")
                 st["T"] = T
-                st["N"] = np.linalg.inv(L).T
+                # inverse transpose of the linear part = cofactors / det
+                st["N"] = [[(T[(r + 1) % 3][(c + 1) % 3] * T[(r + 2) % 3][(c + 2) % 3] -
+                             T[(r + 1) % 3][(c + 2) % 3] * T[(r + 2) % 3][(c + 1) % 3]) / det
+                            for c in range(3)] for r in range(3)]
                 continue
             if token == "vertices":
                 n = s.int()
@@ -196,9 +203,11 @@ def load_contact_aware_surface(path: str):
                 if vid in st["ids"]:
                     raise err(ln, "duplicate vertex id in surface")
                 st["ids"][vid] = len(verts)
-                h = st["T"] @ np.array([*xyz, 1.0])
-                p = h[:3] / h[3] if abs(h[3]) > 1.0e-15 else h[:3]
-                verts.append(p)
+                T = st["T"]
+                h = [((T[r][0] * xyz[0] + T[r][1] * xyz[1]) + T[r][2] * xyz[2]) + T[r][3]
+                     for r in range(4)]
+                p = [h[0] / h[3], h[1] / h[3], h[2] / h[3]] if abs(h[3]) > 1.0e-15 else h[:3]
+                verts.append(np.asarray(p, np.float64))
                 st["sv"] += 1
                 continue
             if token == "tri":
@@ -214,11 +223,13 @@ def load_contact_aware_surface(path: str):
                     if vid[c] not in st["ids"]:
                         raise err(ln, "triangle references unknown vertex id")
                     tri.append(st["ids"][vid[c]])
-                    n = st["N"] @ np.asarray(nums[3 * c:3 * c + 3])
-                    ln_ = np.sqrt(n @ n)
-                    if not np.all(np.isfinite(n)) or not np.isfinite(ln_) or ln_ <= 1.0e-15:
+                    N = st["N"]
+                    x = nums[3 * c:3 * c + 3]
+                    n = [(N[r][0] * x[0] + N[r][1] * x[1]) + N[r][2] * x[2] for r in range(3)]
+                    ln_ = math.sqrt((n[0] * n[0] + n[1] * n[1]) + n[2] * n[2])
+                    if not all(math.isfinite(q) for q in n) or not math.isfinite(ln_) or ln_ <= 1.0e-15:
                         raise err(ln, "corner normal must be a finite non-zero vector")
-                    nrm.append(n / ln_)
+                    nrm.append([n[0] / ln_, n[1] / ln_, n[2] / ln_])
                 e0 = verts[tri[1]] - verts[tri[0]]
                 e1 = verts[tri[2]] - verts[tri[0]]
                 if np.linalg.norm(np.cross(e0, e1)) <= 1.0e-15:

@@ -1,0 +1,120 @@
+"""The source-compatible C++ API (include/SdfOracle, the reference's
+SdfBuilder / SdfOracle / SdfIO / mesh loaders) driven by a compiled program,
+checked against the Python / C-ABI path: same files byte for byte, same query
+results, ContactAware candidates equal to the FP64 oracle's."""
+import os
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import ROOT, ograd
+from sdfo import meshes, scenes
+from sdfo.surface_io import load_contact_aware_surface, write_contact_aware_surface
+
+pytestmark = pytest.mark.gpu
+
+BIN = os.path.join(ROOT, "sdf-contact-module_amd", "_build", "sdf_api_check")
+MODES = ["FirstOrder", "SecondOrder", "Trilinear", "Tricubic"]
+REC = 9 + 8 * 8
+
+
+def _write_obj(path, v, t):
+    with open(path, "w") as f:
+        for p in v:
+            f.write(f"v {float(p[0])!r} {float(p[1])!r} {float(p[2])!r}\n")
+        for a, b, c in t:
+            f.write(f"f {a + 1} {b + 1} {c + 1}\n")
+
+
+def _face_normals(v, t):
+    n = np.cross(v[t[:, 1]] - v[t[:, 0]], v[t[:, 2]] - v[t[:, 0]])
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    return np.repeat(n[:, None, :], 3, axis=1)
+
+
+def _read(path, n):
+    return np.fromfile(path, np.float64).reshape(n, REC)
+
+
+@pytest.fixture(scope="module")
+def run(tmp_path_factory):
+    if not os.path.exists(BIN):
+        pytest.fail("sdf_api_check is not built (make -C sdf-contact-module_amd)")
+    d = tmp_path_factory.mktemp("cpp")
+    v, t = meshes.cube(0.1, (0.003, -0.002, 0.001))
+    _write_obj(str(d / "mesh.obj"), v, t)
+    tv, tt = meshes.torus(0.05, 0.01, 32, 12)
+    write_contact_aware_surface(str(d / "surface.txt"), tv, tt, _face_normals(tv, tt))
+    rng = np.random.default_rng(3)
+    pts = np.concatenate([rng.uniform(-0.07, 0.07, (400, 3)), rng.uniform(-0.07, 0.07, (100, 3)) * [1, 1, 0.2]])
+    with open(d / "points.bin", "wb") as f:
+        f.write(struct.pack("<q", len(pts)))
+        f.write(np.ascontiguousarray(pts, np.float64).tobytes())
+    print(f"[gpu] C++ API program in {d}", flush=True)
+    p = subprocess.run([BIN, str(d)], capture_output=True, text=True, timeout=120)
+    print(p.stdout, p.stderr, flush=True)
+    assert p.returncode == 0, p.stderr
+    return dict(dir=d, v=v, t=t, tv=tv, tt=tt, pts=pts)
+
+
+def test_files_byte_identical_to_c_abi_builds(sdfo_gpu, run, tmp_path):
+    d, v, t = run["dir"], run["v"], run["t"]
+    # the contact-aware builds start from the surface file, as the program's do
+    tv, tt, cn = load_contact_aware_surface(str(d / "surface.txt"))
+    cases = [
+        ("dense.sdf", lambda: sdfo_gpu.SdfGrid.build_dense(v, t, target_resolution=32, max_resolution=32)),
+        ("sparse.sdf", lambda: sdfo_gpu.SdfGrid.build_sparse(v, t, target_resolution=32, max_resolution=32,
+                                                             band_voxels=3)),
+        ("ca.sdf", lambda: sdfo_gpu.SdfGrid.build_contact_aware(tv, tt, cn, target_resolution=24,
+                                                                max_resolution=24)),
+        ("sca.sdf", lambda: sdfo_gpu.SdfGrid.build_contact_aware(tv, tt, cn, sparse=True,
+                                                                 target_resolution=24, max_resolution=24,
+                                                                 band_voxels=2)),
+    ]
+    for name, mk in cases:
+        g = mk()
+        ref = str(tmp_path / name)
+        g.save(ref)
+        assert open(d / name, "rb").read() == open(ref, "rb").read(), name
+
+
+@pytest.mark.parametrize("kind", ["dense", "sparse"])
+def test_queries_match_c_abi(sdfo_gpu, run, kind):
+    d, v, t, pts = run["dir"], run["v"], run["t"], run["pts"]
+    if kind == "dense":
+        g = sdfo_gpu.SdfGrid.build_dense(v, t, target_resolution=32, max_resolution=32)
+    else:
+        g = sdfo_gpu.SdfGrid.build_sparse(v, t, target_resolution=32, max_resolution=32, band_voxels=3)
+    P = torch.from_numpy(pts).cuda()
+    for m in MODES:
+        r = _read(d / f"{kind}_{m}.bin", len(pts))
+        gap, grad, valid, normal, wit, br = g.query(P, m, full=True)
+        vv = valid.cpu().numpy()
+        assert np.array_equal(r[:, 7].astype(bool), vv), m
+        assert np.array_equal(r[vv, 0], gap.cpu().numpy()[vv]), m
+        assert np.array_equal(r[vv, 1:4], normal.cpu().numpy()[vv]), m
+        assert np.array_equal(r[vv, 4:7], wit.cpu().numpy()[vv]), m
+        # every valid result carries its primary candidate (branch 0)
+        assert np.all(r[vv, 8] == 1) and np.all(r[~vv, 8] == 0), m
+        assert np.array_equal(r[vv, 9], r[vv, 0]) and np.all(r[vv, 16] == 0), m
+
+
+def test_contact_aware_candidates_match_oracle(O, sdfo_gpu, run):
+    d, pts = run["dir"], run["pts"]
+    tv, tt, cn = load_contact_aware_surface(str(d / "surface.txt"))
+    og = O.setup_grid(tv, 0.1, 0.0, 24, 24)
+    ref = O.build_contact_aware_dense(tv, tt, cn, og)
+    r = _read(d / "ca_ContactAware.bin", len(pts))
+    rq = O.query_dense(og, ref, "ContactAware", pts)
+    vv = r[:, 7].astype(bool)
+    assert np.array_equal(vv, rq["valid"])
+    # the primary is the candidate of least |gap| (first wins ties)
+    cnt = r[:, 8].astype(int)
+    for i in np.nonzero(vv & (cnt > 1))[0][:50]:
+        gaps = r[i, 9 + 8 * np.arange(cnt[i])]
+        assert r[i, 0] == gaps[int(np.argmin(np.abs(gaps)))]
+    assert np.abs(r[vv, 0] - rq["gap"][vv]).max() <= 1e-12
+    assert (r[vv, 9 + 7] >= -1).all()

@@ -66,3 +66,62 @@ def test_contact_aware_errors(tmp_path, body, msg):
     p.write_text(body)
     with pytest.raises(SurfaceParseError, match=msg.replace("(", r"\(").replace("'", "'")):
         load_contact_aware_surface(str(p))
+
+
+# ---- the C++ API's loaders (include/SdfOracle/SdfSurfaceMesh.h), host only --
+import os  # noqa: E402
+import subprocess  # noqa: E402
+
+_BIN = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                    "sdf-contact-module_amd", "_build", "sdf_api_check")
+
+
+def _cpp_load(path, tmp_path):
+    if not os.path.exists(_BIN):
+        pytest.skip("sdf_api_check not built")
+    out = tmp_path / "dump.bin"
+    subprocess.run([_BIN, "loaders", str(path), str(out)], check=True, timeout=60)
+    b = out.read_bytes()
+    if b.startswith(b"ERROR: "):
+        return b[7:].decode()
+    nv, nt, nn = np.frombuffer(b[:24], np.int64)
+    o = 24
+    v = np.frombuffer(b[o:o + 24 * nv], np.float64).reshape(-1, 3)
+    o += 24 * nv
+    t = np.frombuffer(b[o:o + 12 * nt], np.int32).reshape(-1, 3)
+    o += 12 * nt
+    n = np.frombuffer(b[o:o + 72 * nn], np.float64).reshape(-1, 3, 3)
+    return v, t, n
+
+
+def test_cpp_loaders_equal_python(tmp_path):
+    v, t = meshes.torus(0.05, 0.01, 16, 8)
+    cn = np.repeat(np.cross(v[t[:, 1]] - v[t[:, 0]], v[t[:, 2]] - v[t[:, 0]])[:, None], 3, 1)
+    p = str(tmp_path / "s.surf")
+    write_contact_aware_surface(p, v, t, cn)
+    text = open(p).read().replace("vertices", "transform 0.5 0.1 0 0.01  0 2 0.3 0  0.2 0 1 0  0 0 0 1\nvertices")
+    open(p, "w").write(text)
+    pv, pt, pn = load_contact_aware_surface(p)
+    cv, ct, cnn = _cpp_load(p, tmp_path)
+    assert np.array_equal(pv, cv) and np.array_equal(pt, ct) and np.array_equal(pn, cnn)
+    o = tmp_path / "m.obj"
+    o.write_text("v 0 0 0\nv 1 0 0\nv 0 1 0\nv 0 0 1\nf 1 2 3\nf 1/1 2/2 4/4\nf 2//1 3//1 4//1 1//1\n")
+    pv, pt = load_obj(str(o))
+    cv, ct, _ = _cpp_load(o, tmp_path)
+    assert np.array_equal(pv, cv) and np.array_equal(pt, ct)
+
+
+@pytest.mark.parametrize("body", [
+    "NEXDYN_CONTACT_AWARE_SURFACE 2\n",
+    "NEXDYN_CONTACT_AWARE_SURFACE 1\nsurface s\nvertices 2\nv 0 0 0 0\nv 0 1 0 0\n",
+    "NEXDYN_CONTACT_AWARE_SURFACE 1\nsurface s\nvertices 3\ntriangles 1\nv 0 0 0 0\nv 1 1 0 0\n"
+    "v 2 2 0 0\ntri 0 0 1 2 0 0 1 0 0 1 0 0 1\n",
+    "NEXDYN_CONTACT_AWARE_SURFACE 1\nsurface s\nbogus\n",
+    "# only a comment\n",
+])
+def test_cpp_loader_errors_equal_python(tmp_path, body):
+    p = tmp_path / "e.surf"
+    p.write_text(body)
+    with pytest.raises(SurfaceParseError) as ei:
+        load_contact_aware_surface(str(p))
+    assert _cpp_load(p, tmp_path) == str(ei.value)
