@@ -40,6 +40,45 @@ __device__ __forceinline__ void pushReplay(const Replay& r, int64_t id)
     r.ids[atomicAdd(r.count, 1)] = id;
 }
 
+// Work units whose wave search ran over its step budget, with the bound each
+// lane had reached; the frontier pass finishes them one voxel per wave.
+struct Heavy
+{
+    int* count;
+    int64_t* ids;  // unit ids
+    double* seeds; // 64 per unit
+    unsigned long long* queue;
+};
+
+// Step budgets of one wave's search (node visits + triangle tests) the
+// tuner chooses from; the mean brick of a 32k-triangle ring takes ~300.
+constexpr int kBudgetCount = 3;
+constexpr int kBudgets[kBudgetCount] = {1024, 512, 2048};
+const char* const kBudgetNames[kBudgetCount] = {"1024", "512", "2048"};
+
+__device__ __forceinline__ void pushHeavy(const Heavy& h, int64_t unit, bool active, double bound)
+{
+    int slot = 0;
+    if ((threadIdx.x & 63) == 0) slot = atomicAdd(h.count, 1);
+    slot = __shfl(slot, 0);
+    if ((threadIdx.x & 63) == 0) h.ids[slot] = unit;
+    h.seeds[int64_t(slot) * 64 + (threadIdx.x & 63)] = active ? bound : DBL_MAX;
+}
+
+// Frontier sizes: the first frontier pass runs 4 waves per block with 1000
+// entries per level each; voxels that outgrow it go to a pass with one wave
+// of 4000 entries per block.
+constexpr int kFrontierSmall = 1000, kFrontierLarge = 4000;
+
+// Voxels whose frontier overflowed, with the bound reached.
+struct Spill
+{
+    int* count;
+    int64_t* ids; // voxel ids (dense linear index / sparse pool index)
+    double* seeds;
+    unsigned long long* queue;
+};
+
 __device__ __forceinline__ D3 voxelCentre(const GridSpec& g, int i, int j, int k)
 {
     // GridSpec.h:30-34
@@ -128,7 +167,7 @@ __device__ __forceinline__ int64_t nextUnit(unsigned long long* queue)
 __global__ void __launch_bounds__(kFillBlock)
 denseFillKernel(GridSpec g, const BvhNode* __restrict__ nodes, const double* __restrict__ tri9,
                 const double* __restrict__ box6, D3 dir, double mu, BrickShape sh, DenseFillOut o,
-                Replay replay, unsigned long long* queue, int force)
+                Replay replay, Heavy heavy, unsigned long long* queue, int budget, int force)
 {
     __shared__ __align__(16) unsigned char lds[kFillWaves * dev::kWaveStackBytes];
     const int lane = threadIdx.x & 63;
@@ -140,13 +179,83 @@ denseFillKernel(GridSpec g, const BvhNode* __restrict__ nodes, const double* __r
         sh.voxel(brick, lane, i, j, k);
         const bool active = i < g.n[0] && j < g.n[1] && k < g.n[2];
         const D3 c = voxelCentre(g, min(i, g.n[0] - 1), min(j, g.n[1] - 1), min(k, g.n[2] - 1));
-        const dev::WaveSigned w = dev::waveSignedNearest(c, active, nodes, tri9, box6, dir, mu, ws);
+        const unsigned long long act = __ballot(active);
+        const dev::WaveNearest wn =
+            dev::waveNearest(c, active, act, nodes, tri9, box6, mu, ws, lane, 0, budget);
+        if (wn.aborted)
+        {
+            pushHeavy(heavy, brick, active, wn.bestSq);
+            continue;
+        }
+        const int hits = dev::waveRayHits(c, dir, act, nodes, tri9, ws, lane, 0);
         if (!active) continue;
         const int64_t idx = i + int64_t(g.n[0]) * (j + int64_t(g.n[1]) * k);
-        if (!w.exact || force)
+        if (!wn.exact || force)
             pushReplay(replay, idx);
         else
-            writeDenseVoxel(o, c, w.s, idx);
+            writeDenseVoxel(o, c, dev::finishSigned(c, tri9, wn.bestSq, wn.bestSlot, hits), idx);
+    }
+}
+
+// Voxels of the bricks denseFillKernel handed over: one voxel per wave.
+template <int Cap, int Waves>
+__global__ void __launch_bounds__(64 * Waves)
+denseHeavyKernel(GridSpec g, const BvhNode* __restrict__ nodes, const double* __restrict__ tri9,
+                 const double* __restrict__ box6, D3 dir, double mu, BrickShape sh, DenseFillOut o,
+                 Replay replay, Heavy heavy, Spill spill, int force)
+{
+    __shared__ __align__(16) unsigned char lds[Waves * dev::frontierBytes(Cap)];
+    const dev::FrontierLds f = dev::carveFrontier(lds + (threadIdx.x >> 6) * dev::frontierBytes(Cap), Cap);
+    const int64_t units = int64_t(*heavy.count) * 64;
+    for (int64_t u = nextUnit(heavy.queue); u < units; u = nextUnit(heavy.queue))
+    {
+        int i, j, k;
+        sh.voxel(heavy.ids[u >> 6], int(u & 63), i, j, k);
+        if (i >= g.n[0] || j >= g.n[1] || k >= g.n[2]) continue; // wave-uniform
+        const D3 c = voxelCentre(g, i, j, k);
+        double seed = heavy.seeds[u];
+        int status = 0;
+        const dev::SignedNearest s =
+            dev::frontierSignedNearest(c, nodes, tri9, box6, dir, mu, 0, seed, f, status);
+        if ((threadIdx.x & 63) != 0) continue;
+        const int64_t idx = i + int64_t(g.n[0]) * (j + int64_t(g.n[1]) * k);
+        if (status == dev::kFrontierOverflow)
+        {
+            const int q = atomicAdd(spill.count, 1);
+            spill.ids[q] = idx;
+            spill.seeds[q] = seed;
+        }
+        else if (status != dev::kFrontierExact || force)
+            pushReplay(replay, idx);
+        else
+            writeDenseVoxel(o, c, s, idx);
+    }
+}
+
+// Voxels whose first frontier overflowed: one wave with a 4000-entry frontier.
+__global__ void __launch_bounds__(64)
+denseSpillKernel(GridSpec g, const BvhNode* __restrict__ nodes, const double* __restrict__ tri9,
+                 const double* __restrict__ box6, D3 dir, double mu, DenseFillOut o, Replay replay,
+                 Spill spill, int force)
+{
+    __shared__ __align__(16) unsigned char lds[dev::frontierBytes(kFrontierLarge)];
+    const dev::FrontierLds f = dev::carveFrontier(lds, kFrontierLarge);
+    const int64_t n = *spill.count;
+    for (int64_t q = nextUnit(spill.queue); q < n; q = nextUnit(spill.queue))
+    {
+        const int64_t idx = spill.ids[q];
+        const int i = int(idx % g.n[0]), j = int((idx / g.n[0]) % g.n[1]),
+                  k = int(idx / (int64_t(g.n[0]) * g.n[1]));
+        const D3 c = voxelCentre(g, i, j, k);
+        double seed = spill.seeds[q];
+        int status = 0;
+        const dev::SignedNearest s =
+            dev::frontierSignedNearest(c, nodes, tri9, box6, dir, mu, 0, seed, f, status);
+        if ((threadIdx.x & 63) != 0) continue;
+        if (status != dev::kFrontierExact || force)
+            pushReplay(replay, idx);
+        else
+            writeDenseVoxel(o, c, s, idx);
     }
 }
 
@@ -392,7 +501,8 @@ __global__ void __launch_bounds__(kFillBlock)
 sparseFillKernel(GridSpec g, const BvhNode* __restrict__ nodes, const double* __restrict__ tri9,
                  const double* __restrict__ box6, D3 dir, double mu, int bd, int64_t total,
                  const int32_t* __restrict__ bijk, float* __restrict__ sphi,
-                 float* __restrict__ sn3, Replay replay, unsigned long long* queue, int force)
+                 float* __restrict__ sn3, Replay replay, Heavy heavy, unsigned long long* queue,
+                 int budget, int force)
 {
     __shared__ __align__(16) unsigned char lds[kFillWaves * dev::kWaveStackBytes];
     const int64_t nunits = (total + 63) / 64;
@@ -406,21 +516,92 @@ sparseFillKernel(GridSpec g, const BvhNode* __restrict__ nodes, const double* __
         brickVoxel(bijk, bd, t, i, j, k);
         const bool inGrid = i >= 0 && i < g.n[0] && j >= 0 && j < g.n[1] && k >= 0 && k < g.n[2];
         const bool active = t0 < total && inGrid;
-        const D3 c = voxelCentre(g, min(max(i, 0), g.n[0] - 1), min(max(j, 0), g.n[1] - 1),
-                                 min(max(k, 0), g.n[2] - 1));
-        const dev::WaveSigned w = dev::waveSignedNearest(c, active, nodes, tri9, box6, dir, mu, ws);
-        if (t0 >= total) continue;
-        if (!inGrid)
+        if (t0 < total && !inGrid)
         {
             sphi[t] = 0.0f;
             sn3[3 * t + 0] = 1.0f;
             sn3[3 * t + 1] = 0.0f;
             sn3[3 * t + 2] = 0.0f;
         }
-        else if (!w.exact || force)
+        const D3 c = voxelCentre(g, min(max(i, 0), g.n[0] - 1), min(max(j, 0), g.n[1] - 1),
+                                 min(max(k, 0), g.n[2] - 1));
+        const unsigned long long act = __ballot(active);
+        const dev::WaveNearest wn =
+            dev::waveNearest(c, active, act, nodes, tri9, box6, mu, ws, lane, 0, budget);
+        if (wn.aborted)
+        {
+            pushHeavy(heavy, u, active, wn.bestSq);
+            continue;
+        }
+        const int hits = dev::waveRayHits(c, dir, act, nodes, tri9, ws, lane, 0);
+        if (!active) continue;
+        if (!wn.exact || force)
             pushReplay(replay, t);
         else
-            writeSparseVoxel(c, w.s, t, sphi, sn3);
+            writeSparseVoxel(c, dev::finishSigned(c, tri9, wn.bestSq, wn.bestSlot, hits), t, sphi, sn3);
+    }
+}
+
+template <int Cap, int Waves>
+__global__ void __launch_bounds__(64 * Waves)
+sparseHeavyKernel(GridSpec g, const BvhNode* __restrict__ nodes, const double* __restrict__ tri9,
+                  const double* __restrict__ box6, D3 dir, double mu, int bd, int64_t total,
+                  const int32_t* __restrict__ bijk, float* __restrict__ sphi,
+                  float* __restrict__ sn3, Replay replay, Heavy heavy, Spill spill, int force)
+{
+    __shared__ __align__(16) unsigned char lds[Waves * dev::frontierBytes(Cap)];
+    const dev::FrontierLds f = dev::carveFrontier(lds + (threadIdx.x >> 6) * dev::frontierBytes(Cap), Cap);
+    const int64_t units = int64_t(*heavy.count) * 64;
+    for (int64_t u = nextUnit(heavy.queue); u < units; u = nextUnit(heavy.queue))
+    {
+        const int64_t t = heavy.ids[u >> 6] * 64 + (u & 63);
+        if (t >= total) continue; // wave-uniform
+        int i, j, k;
+        brickVoxel(bijk, bd, t, i, j, k);
+        if (i < 0 || i >= g.n[0] || j < 0 || j >= g.n[1] || k < 0 || k >= g.n[2]) continue;
+        const D3 c = voxelCentre(g, i, j, k);
+        double seed = heavy.seeds[u];
+        int status = 0;
+        const dev::SignedNearest s =
+            dev::frontierSignedNearest(c, nodes, tri9, box6, dir, mu, 0, seed, f, status);
+        if ((threadIdx.x & 63) != 0) continue;
+        if (status == dev::kFrontierOverflow)
+        {
+            const int q = atomicAdd(spill.count, 1);
+            spill.ids[q] = t;
+            spill.seeds[q] = seed;
+        }
+        else if (status != dev::kFrontierExact || force)
+            pushReplay(replay, t);
+        else
+            writeSparseVoxel(c, s, t, sphi, sn3);
+    }
+}
+
+__global__ void __launch_bounds__(64)
+sparseSpillKernel(GridSpec g, const BvhNode* __restrict__ nodes, const double* __restrict__ tri9,
+                  const double* __restrict__ box6, D3 dir, double mu, int bd,
+                  const int32_t* __restrict__ bijk, float* __restrict__ sphi,
+                  float* __restrict__ sn3, Replay replay, Spill spill, int force)
+{
+    __shared__ __align__(16) unsigned char lds[dev::frontierBytes(kFrontierLarge)];
+    const dev::FrontierLds f = dev::carveFrontier(lds, kFrontierLarge);
+    const int64_t n = *spill.count;
+    for (int64_t q = nextUnit(spill.queue); q < n; q = nextUnit(spill.queue))
+    {
+        const int64_t t = spill.ids[q];
+        int i, j, k;
+        brickVoxel(bijk, bd, t, i, j, k);
+        const D3 c = voxelCentre(g, i, j, k);
+        double seed = spill.seeds[q];
+        int status = 0;
+        const dev::SignedNearest s =
+            dev::frontierSignedNearest(c, nodes, tri9, box6, dir, mu, 0, seed, f, status);
+        if ((threadIdx.x & 63) != 0) continue;
+        if (status != dev::kFrontierExact || force)
+            pushReplay(replay, t);
+        else
+            writeSparseVoxel(c, s, t, sphi, sn3);
     }
 }
 
@@ -507,15 +688,29 @@ namespace {
 struct ReplayBuf
 {
     Replay r{nullptr, nullptr};
+    Heavy h{nullptr, nullptr, nullptr, nullptr};
+    Spill sp{nullptr, nullptr, nullptr, nullptr};
     unsigned long long* queue = nullptr; // work-queue counter of the fill launch
     hipStream_t s;
-    ReplayBuf(int64_t cap, hipStream_t st) : s(st)
+    // cap: work items (replay ids); units: 64-lane work units (heavy list)
+    ReplayBuf(int64_t cap, hipStream_t st, int64_t units = 0) : s(st)
     {
-        SDFO_HIP(hipMallocAsync(reinterpret_cast<void**>(&queue), 16, s));
-        SDFO_HIP(hipMemsetAsync(queue, 0, 16, s));
+        SDFO_HIP(hipMallocAsync(reinterpret_cast<void**>(&queue), 48, s));
+        SDFO_HIP(hipMemsetAsync(queue, 0, 48, s));
         r.count = reinterpret_cast<int*>(queue + 1);
+        h.queue = queue + 2;
+        h.count = reinterpret_cast<int*>(queue + 3);
+        sp.queue = queue + 4;
+        sp.count = reinterpret_cast<int*>(queue + 5);
         SDFO_HIP(hipMallocAsync(reinterpret_cast<void**>(&r.ids),
                                 sizeof(int64_t) * size_t(std::max<int64_t>(cap, 1)), s));
+        if (units > 0)
+        {
+            SDFO_HIP(hipMallocAsync(reinterpret_cast<void**>(&h.ids), sizeof(int64_t) * size_t(units), s));
+            SDFO_HIP(hipMallocAsync(reinterpret_cast<void**>(&h.seeds), sizeof(double) * 64 * size_t(units), s));
+            SDFO_HIP(hipMallocAsync(reinterpret_cast<void**>(&sp.ids), sizeof(int64_t) * 64 * size_t(units), s));
+            SDFO_HIP(hipMallocAsync(reinterpret_cast<void**>(&sp.seeds), sizeof(double) * 64 * size_t(units), s));
+        }
     }
     // Copies the list length to the calling thread's pinned counter (read it
     // after synchronising the stream).
@@ -527,6 +722,10 @@ struct ReplayBuf
     ~ReplayBuf()
     {
         (void)hipFreeAsync(r.ids, s);
+        if (h.ids) (void)hipFreeAsync(h.ids, s);
+        if (h.seeds) (void)hipFreeAsync(h.seeds, s);
+        if (sp.ids) (void)hipFreeAsync(sp.ids, s);
+        if (sp.seeds) (void)hipFreeAsync(sp.seeds, s);
         (void)hipFreeAsync(queue, s);
     }
 };
@@ -556,6 +755,10 @@ unsigned queueGrid(int64_t units)
     return unsigned(std::max<int64_t>(1, std::min(want, need)));
 }
 
+// Blocks of the frontier (heavy) pass: its unit count is only known on the
+// device, so a fixed grid of queue-driven waves.
+unsigned heavyGrid() { return queueGrid(int64_t(1) << 40); }
+
 } // namespace
 
 int* replayCounterHost()
@@ -579,15 +782,35 @@ bool& forceReplay()
 
 namespace {
 
-void runDenseFill(const GridSpec& g, const DeviceBvh& bvh, const DenseFillOut& o, int shape,
+// Fill variants the tuner chooses from: brick shape x step budget.
+constexpr int kFillVariants = kFillShapeCount * kBudgetCount;
+
+std::vector<std::string> fillVariantNames()
+{
+    std::vector<std::string> n;
+    for (int v = 0; v < kFillVariants; ++v)
+        n.push_back(std::string(kFillShapeNames[v / kBudgetCount]) + "/budget" + kBudgetNames[v % kBudgetCount]);
+    return n;
+}
+
+void runDenseFill(const GridSpec& g, const DeviceBvh& bvh, const DenseFillOut& o, int variant,
                   bool report, hipStream_t s)
 {
+    const int shape = variant / kBudgetCount, budget = kBudgets[variant % kBudgetCount];
     const BrickShape sh = BrickShape::make(g, kFillShapes[shape][0], kFillShapes[shape][1]);
     const D3 dir = dev::d3(bvh.rayDir[0], bvh.rayDir[1], bvh.rayDir[2]);
-    ReplayBuf rb(g.total(), s);
+    ReplayBuf rb(g.total(), s, sh.count());
+    const double mu = tieMargin(g);
     hipLaunchKernelGGL(denseFillKernel, dim3(queueGrid(sh.count())), dim3(kFillBlock), 0, s, g,
-                       bvh.nodes, bvh.tri9, bvh.box6, dir, tieMargin(g), sh, o, rb.r, rb.queue,
+                       bvh.nodes, bvh.tri9, bvh.box6, dir, mu, sh, o, rb.r, rb.h, rb.queue, budget,
                        forceReplay() ? 1 : 0);
+    SDFO_HIP(hipGetLastError());
+    hipLaunchKernelGGL((denseHeavyKernel<kFrontierSmall, 4>), dim3(heavyGrid()), dim3(256), 0, s, g,
+                       bvh.nodes, bvh.tri9, bvh.box6, dir, mu, sh, o, rb.r, rb.h, rb.sp,
+                       forceReplay() ? 1 : 0);
+    SDFO_HIP(hipGetLastError());
+    hipLaunchKernelGGL(denseSpillKernel, dim3(heavyGrid()), dim3(64), 0, s, g, bvh.nodes, bvh.tri9,
+                       bvh.box6, dir, mu, o, rb.r, rb.sp, forceReplay() ? 1 : 0);
     SDFO_HIP(hipGetLastError());
     hipLaunchKernelGGL(denseReplayKernel, dim3(kReplayGrid), dim3(kReplayBlock), 0, s, g, bvh.nodes,
                        bvh.tri9, dir, rb.r, o);
@@ -603,12 +826,12 @@ std::string fillKey(const GridSpec& g, int64_t ntris)
 
 } // namespace
 
-// Brick shape for a dense fill: the tuner's choice for this size class, else
-// 4 x 4 x 4 (the most compact, so the lanes of a wave share the most nodes).
+// Fill variant (brick shape x step budget) for a size class: the tuner's
+// choice, else 4 x 4 x 4 bricks with a budget of 1024 steps.
 int fillShapeFor(const GridSpec& g, int64_t ntris)
 {
     int v = 0;
-    if (tunerLookup(fillKey(g, ntris), v) && v >= 0 && v < kFillShapeCount) return v;
+    if (tunerLookup(fillKey(g, ntris), v) && v >= 0 && v < kFillVariants) return v;
     return 0;
 }
 
@@ -623,11 +846,15 @@ void launchDenseFill(const GridSpec& g, const DeviceBvh& bvh, double* phi, doubl
         // time every brick shape on this build (each run writes the same,
         // exact fields) and keep the fastest for the size class
         std::vector<std::pair<int, double>> t;
-        for (int v = 0; v < kFillShapeCount; ++v)
+        for (int v = 0; v < kFillVariants; ++v)
             t.emplace_back(v, timeOnStream(s, [&] { runDenseFill(g, bvh, o, v, false, s); }));
-        tunerRecord(key, t, kFillShapeNames);
+        const std::vector<std::string> names = fillVariantNames();
+        std::vector<const char*> cn;
+        for (const std::string& n : names) cn.push_back(n.c_str());
+        tunerRecord(key, t, cn.data());
         tunerLookup(key, shape);
     }
+    if (shape < 0 || shape >= kFillVariants) shape = 0;
     runDenseFill(g, bvh, o, shape, true, s);
 }
 
@@ -643,7 +870,7 @@ void launchDenseFillBatched(const std::vector<BatchMesh>& meshes, const DeviceBv
         J.g = b.g;
         J.mu = tieMargin(b.g);
         J.root = b.root;
-        const int shape = fillShapeFor(b.g, b.ntris);
+        const int shape = fillShapeFor(b.g, b.ntris) / kBudgetCount;
         J.sh = BrickShape::make(b.g, kFillShapes[shape][0], kFillShapes[shape][1]);
         J.brickBegin = bricks;
         J.voxelBegin = voxels;
@@ -727,10 +954,20 @@ void launchSparseFill(const GridSpec& g, const DeviceBvh& bvh, int blockDim, int
     const int64_t total = int64_t(nb) * blockDim * blockDim * blockDim;
     if (total <= 0) return;
     const D3 dir = dev::d3(bvh.rayDir[0], bvh.rayDir[1], bvh.rayDir[2]);
-    ReplayBuf rb(total, s);
-    hipLaunchKernelGGL(sparseFillKernel, dim3(queueGrid((total + 63) / 64)), dim3(kFillBlock), 0,
-                       s, g, bvh.nodes, bvh.tri9, bvh.box6, dir, tieMargin(g), blockDim, total,
-                       bijk, sphi, sn3, rb.r, rb.queue, forceReplay() ? 1 : 0);
+    const int64_t units = (total + 63) / 64;
+    ReplayBuf rb(total, s, units);
+    const double mu = tieMargin(g);
+    hipLaunchKernelGGL(sparseFillKernel, dim3(queueGrid(units)), dim3(kFillBlock), 0, s, g, bvh.nodes,
+                       bvh.tri9, bvh.box6, dir, mu, blockDim, total, bijk, sphi, sn3, rb.r, rb.h,
+                       rb.queue, kBudgets[0], forceReplay() ? 1 : 0);
+    SDFO_HIP(hipGetLastError());
+    hipLaunchKernelGGL((sparseHeavyKernel<kFrontierSmall, 4>), dim3(heavyGrid()), dim3(256), 0, s, g,
+                       bvh.nodes, bvh.tri9, bvh.box6, dir, mu, blockDim, total, bijk, sphi, sn3,
+                       rb.r, rb.h, rb.sp, forceReplay() ? 1 : 0);
+    SDFO_HIP(hipGetLastError());
+    hipLaunchKernelGGL(sparseSpillKernel, dim3(heavyGrid()), dim3(64), 0, s, g, bvh.nodes, bvh.tri9,
+                       bvh.box6, dir, mu, blockDim, bijk, sphi, sn3, rb.r, rb.sp,
+                       forceReplay() ? 1 : 0);
     SDFO_HIP(hipGetLastError());
     hipLaunchKernelGGL(sparseFillReplayKernel, dim3(kReplayGrid), dim3(kReplayBlock), 0, s, g,
                        bvh.nodes, bvh.tri9, dir, blockDim, bijk, rb.r, sphi, sn3);

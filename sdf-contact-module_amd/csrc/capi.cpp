@@ -4,9 +4,11 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <exception>
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -21,6 +23,24 @@ struct sdfo_query_plan
 };
 
 namespace sdfo {
+
+// Scratch buffers of the build kernels come from the device's stream-ordered
+// pool; keeping the pool's memory across synchronisations makes the next
+// build's allocations free instead of remapping pages every time.
+void keepStreamPoolMemory(int device)
+{
+    static std::mutex lock;
+    static std::vector<int> done;
+    std::lock_guard<std::mutex> lk(lock);
+    if (std::find(done.begin(), done.end(), device) != done.end()) return;
+    hipMemPool_t pool = nullptr;
+    if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess && pool)
+    {
+        uint64_t keep = ~uint64_t(0);
+        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+    }
+    done.push_back(device);
+}
 
 namespace {
 thread_local std::string g_lastError;
@@ -201,6 +221,7 @@ std::unique_ptr<sdfo_grid> buildDense(const double* verts, int64_t nv, const int
     try
     {
         const clk::time_point t2 = clk::now();
+        keepStreamPoolMemory(g->device);
         db.upload(hb, s);
         g->phi = g->alloc<double>(size_t(N));
         g->n3 = g->alloc<double>(size_t(N) * 3);
@@ -340,8 +361,9 @@ std::vector<std::unique_ptr<sdfo_grid>> buildDenseBatched(int32_t nm, const doub
     DeviceBvh db;
     try
     {
-        db.upload(all, s);
         const int dev = currentDevice();
+        keepStreamPoolMemory(dev);
+        db.upload(all, s);
         for (int32_t m = 0; m < nm; ++m)
         {
             auto g = std::make_unique<sdfo_grid>();
@@ -402,6 +424,7 @@ std::unique_ptr<sdfo_grid> buildSparse(const double* verts, int64_t nv, const in
     DeviceBvh db;
     try
     {
+        keepStreamPoolMemory(g->device);
         db.upload(hb, s);
         g->activeMask = g->alloc<uint8_t>(size_t(cells));
         launchSparseActivate(g->spec, db, bd, g->nbx, g->nby, g->nbz, g->activation, g->activeMask, s);

@@ -797,6 +797,7 @@ sdfo_grid* buildContactAware(const double* verts, int64_t nv, const int32_t* tri
     };
     try
     {
+        keepStreamPoolMemory(g->device);
         db.upload(hb, s);
         D.t21 = buf.up(ca.t21);
         if (P.useTiles)

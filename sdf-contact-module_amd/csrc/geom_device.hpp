@@ -362,7 +362,8 @@ struct WaveNearest
 {
     double bestSq;
     int bestSlot;
-    bool exact; // false: the per-lane search must be re-run for this lane
+    bool exact;   // false: the per-lane search must be re-run for this lane
+    bool aborted; // wave-uniform: the step budget ran out (bestSq is an upper bound)
 };
 
 // Distance^2 from p to a box given as bmin3, bmax3 (same arithmetic as boxSq).
@@ -394,9 +395,10 @@ __device__ inline WaveNearest waveNearest(D3 p, bool active, unsigned long long 
                                           const BvhNode* __restrict__ nodes,
                                           const double* __restrict__ tri9,
                                           const double* __restrict__ box6, double mu,
-                                          WaveStacks ws, int lane, int root)
+                                          WaveStacks ws, int lane, int root, int budget = 0)
 {
     double bestSq = DBL_MAX, thr = DBL_MAX, bestB = 0.0;
+    int steps = 0; // node visits + triangle tests (wave-uniform)
     uint32_t bestRank = 0xffffffffu;
     int bestSlot = -1;
     uint32_t curKey = 0u; // key of the last node popped
@@ -409,6 +411,7 @@ __device__ inline WaveNearest waveNearest(D3 p, bool active, unsigned long long 
     while (sp > 0)
     {
         --sp;
+        if (budget > 0 && ++steps > budget) return WaveNearest{bestSq, bestSlot, false, true};
         const BvhNode nd = ws.rec[sp];
         const int depth = uniformInt(ws.node[sp]);
         const unsigned long long bits = uniformMask(ws.mask[sp]);
@@ -443,6 +446,7 @@ __device__ inline WaveNearest waveNearest(D3 p, bool active, unsigned long long 
             {
                 const int q = __builtin_ctz(need);
                 need &= need - 1u;
+                ++steps;
                 const double* t = ws.leaf + 6 * kLeafMax + 9 * q;
                 const D3 ta = d3(t[0], t[1], t[2]), tb = d3(t[3], t[4], t[5]), tc = d3(t[6], t[7], t[8]);
                 const double sq = sqnorm(sub(p, closestPoint(p, ta, tb, tc)));
@@ -510,7 +514,7 @@ __device__ inline WaveNearest waveNearest(D3 p, bool active, unsigned long long 
             }
         }
     }
-    return WaveNearest{bestSq, bestSlot, bestSlot >= 0 && bestB <= bestSq};
+    return WaveNearest{bestSq, bestSlot, bestSlot >= 0 && bestB <= bestSq, false};
 }
 
 __device__ inline int waveRayHits(D3 p, D3 dir, unsigned long long act,
@@ -568,6 +572,199 @@ __device__ inline WaveSigned waveSignedNearest(D3 p, bool active, const BvhNode*
     const WaveNearest wn = waveNearest(p, active, act, nodes, tri9, box6, mu, ws, lane, root);
     const int hits = waveRayHits(p, dir, act, nodes, tri9, ws, lane, root);
     return WaveSigned{finishSigned(p, tri9, wn.bestSq, wn.bestSlot, hits), wn.exact};
+}
+
+// ---------------------------------------------------------------------------
+// Frontier search: one voxel per wave, the lanes expanding the BVH level by
+// level in parallel (64 nodes per step), for voxels whose wave search ran
+// over its step budget (points equidistant from many triangles).  The
+// exactness argument is the wave search's: ranks are the voxel's own
+// depth-first path keys, pruning keeps every triangle with sq <= m (margin
+// mu), the winner is the lexicographic minimum of (sq, rank, slot) over the
+// wave, and the per-lane search reaches it when its leaf box distance is
+// <= m.  A full frontier or a failed check reports exact = false.
+// ---------------------------------------------------------------------------
+struct FrontierLds
+{
+    int* node[2];     // cap entries each: node << 5 | depth
+    uint32_t* key[2]; // cap entries each
+    int* counter;     // [0] next-level count, [1] overflow flag
+    int cap;
+};
+
+// LDS bytes of a frontier of `cap` entries per level.
+__host__ __device__ constexpr int frontierBytes(int cap) { return 2 * cap * (4 + 4) + 16; }
+
+__device__ __forceinline__ FrontierLds carveFrontier(unsigned char* base, int cap)
+{
+    FrontierLds f;
+    for (int b = 0; b < 2; ++b)
+    {
+        f.node[b] = reinterpret_cast<int*>(base + b * cap * 8);
+        f.key[b] = reinterpret_cast<uint32_t*>(base + b * cap * 8 + cap * 4);
+    }
+    f.counter = reinterpret_cast<int*>(base + 2 * cap * 8);
+    f.cap = cap;
+    return f;
+}
+
+// Outcome of a frontier search.
+enum FrontierStatus
+{
+    kFrontierExact = 0,    // result certified
+    kFrontierUnsure = 1,   // re-run with the per-lane search
+    kFrontierOverflow = 2, // a level outgrew the frontier: retry with a larger one
+};
+
+__device__ __forceinline__ double waveMinD(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+    {
+        const double w = __shfl_xor(v, o);
+        v = w < v ? w : v;
+    }
+    return v;
+}
+
+// Lexicographic (sq, rank, slot) minimum over the wave; every lane gets it.
+__device__ __forceinline__ void waveLexMin(double& sq, uint32_t& rank, int& slot, double& b)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+    {
+        const double s2 = __shfl_xor(sq, o);
+        const uint32_t r2 = uint32_t(__shfl_xor(int(rank), o));
+        const int t2 = __shfl_xor(slot, o);
+        const double b2 = __shfl_xor(b, o);
+        const bool take = s2 < sq || (s2 == sq && (r2 < rank || (r2 == rank && unsigned(t2) < unsigned(slot))));
+        if (take)
+        {
+            sq = s2;
+            rank = r2;
+            slot = t2;
+            b = b2;
+        }
+    }
+}
+
+__device__ inline SignedNearest frontierSignedNearest(D3 p, const BvhNode* __restrict__ nodes,
+                                                      const double* __restrict__ tri9,
+                                                      const double* __restrict__ box6, D3 dir,
+                                                      double mu, int root, double& seed,
+                                                      FrontierLds f, int& status)
+{
+    const int lane = int(threadIdx.x) & (kWave - 1);
+    double bestSq = DBL_MAX, bestB = 0.0;
+    uint32_t bestRank = 0xffffffffu;
+    int bestSlot = -1;
+    double m = seed; // wave-uniform bound: a triangle at this squared distance exists
+    double thr = DBL_MAX;
+    if (m < DBL_MAX)
+    {
+        const double r = sqrt(m) + mu;
+        thr = r * r;
+    }
+    if (lane == 0)
+    {
+        f.node[0][0] = root << 5;
+        f.key[0][0] = 0u;
+        f.counter[1] = 0;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    int cur = 0, n = 1;
+    bool overflow = false;
+    while (n > 0 && !overflow)
+    {
+        if (lane == 0) f.counter[0] = 0;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        for (int base = 0; base < n; base += kWave)
+        {
+            const int idx = base + lane;
+            if (idx < n)
+            {
+                const int e = f.node[cur][idx];
+                const int ni = e >> 5, depth = e & 31;
+                const uint32_t key = f.key[cur][idx];
+                const BvhNode nd = nodes[ni];
+                const double bs = boxSq(p, nd);
+                if (bs <= thr)
+                {
+                    if (nd.triCount > 0)
+                    {
+                        const uint32_t rank = key << (31 - depth);
+                        for (int s = nd.triStart; s < nd.triStart + nd.triCount; ++s)
+                        {
+                            if (!(boxSq6(p, box6 + 6 * size_t(s)) <= thr)) continue;
+                            const Tri t = loadTri(tri9, s);
+                            const double sq = sqnorm(sub(p, closestPoint(p, t.a, t.b, t.c)));
+                            if (sq < bestSq || (sq == bestSq && (rank < bestRank || (rank == bestRank && s < bestSlot))))
+                            {
+                                bestSq = sq;
+                                bestRank = rank;
+                                bestSlot = s;
+                                bestB = bs;
+                            }
+                        }
+                    }
+                    else
+                    {
+                        const double dl = boxSq(p, nodes[nd.left]);
+                        const double dr = boxSq(p, nodes[nd.right]);
+                        const uint32_t fr = dr < dl ? 1u : 0u; // this voxel's per-lane order
+                        if (dl <= thr)
+                        {
+                            const int q = atomicAdd(&f.counter[0], 1);
+                            if (q < f.cap)
+                            {
+                                f.node[cur ^ 1][q] = (nd.left << 5) | (depth + 1);
+                                f.key[cur ^ 1][q] = (key << 1) | fr;
+                            }
+                            else
+                                f.counter[1] = 1;
+                        }
+                        if (dr <= thr)
+                        {
+                            const int q = atomicAdd(&f.counter[0], 1);
+                            if (q < f.cap)
+                            {
+                                f.node[cur ^ 1][q] = (nd.right << 5) | (depth + 1);
+                                f.key[cur ^ 1][q] = (key << 1) | (fr ^ 1u);
+                            }
+                            else
+                                f.counter[1] = 1;
+                        }
+                    }
+                }
+            }
+            const double wm = waveMinD(bestSq);
+            if (wm < m)
+            {
+                m = wm;
+                const double r = sqrt(m) + mu;
+                thr = r * r;
+            }
+        }
+        // the next level's entries were written by other lanes
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        overflow = uniformInt(f.counter[1]) != 0;
+        n = uniformInt(f.counter[0]);
+        n = n < f.cap ? n : f.cap;
+        cur ^= 1;
+    }
+    waveLexMin(bestSq, bestRank, bestSlot, bestB);
+    seed = m; // the bound reached (for a retry)
+    status = overflow ? kFrontierOverflow
+                      : ((bestSlot >= 0 && bestB <= bestSq) ? kFrontierExact : kFrontierUnsure);
+    if (overflow) return SignedNearest{};
+    // ray parity by lane 0 with a per-lane stack in the frontier buffer
+    int hits = 0;
+    if (lane == 0) hits = refRayHits(p, dir, nodes, tri9, LaneStack{f.node[0], 1}, root);
+    hits = __shfl(hits, 0);
+    return finishSigned(p, tri9, bestSq, bestSlot, hits);
 }
 
 // SdfBuilder.cpp:1070-1085: sign * (c - w) / |c - w|, or sign * face normal.

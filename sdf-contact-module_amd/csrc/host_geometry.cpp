@@ -65,65 +65,100 @@ namespace {
 // n / 2, so the tree shape depends on n only.
 int64_t subtreeNodes(int64_t n) { return n <= 8 ? 1 : 1 + subtreeNodes(n / 2) + subtreeNodes(n - n / 2); }
 
+// Runs f(begin, end) over [0, n) in chunks on up to 16 threads (small
+// ranges on the calling thread).
+template <typename F>
+void parallelFor(int n, F f)
+{
+    const int threads = std::max(1, std::min<int>(16, int(std::thread::hardware_concurrency())));
+    const int chunks = n < 65536 ? 1 : threads;
+    if (chunks <= 1)
+    {
+        f(0, n);
+        return;
+    }
+    std::vector<std::thread> pool;
+    const int step = (n + chunks - 1) / chunks;
+    for (int c = 1; c < chunks; ++c)
+    {
+        const int a = c * step, b = std::min(n, a + step);
+        if (a < b) pool.emplace_back([=, &f] { f(a, b); });
+    }
+    f(0, std::min(n, step));
+    for (std::thread& t : pool) t.join();
+}
+
+// A triangle in BVH slot order with its centroid, so that every pass over a
+// node's range reads memory sequentially.
+struct Item
+{
+    double c[3];
+    int idx;
+    int pad;
+};
+
 struct BvhBuilder
 {
     std::vector<BvhNode>& nodes;
-    std::vector<int>& idx;
+    std::vector<Item>& items;
     const std::vector<double>& box; // bmin3, bmax3 per triangle
-    const std::vector<double>& cen; // centroid component per axis: cen[axis * n + t]
-    int64_t n;
 
-    // Builds the subtree over idx[lo, hi) whose root has pre-order index me;
+    // Builds the subtree over items[lo, hi) whose root has pre-order index me;
     // returns the subtree depth.  Subtrees above `par` triangles are built
     // concurrently (their node ranges are disjoint and their index ranges too).
+    // A node's box is the union of its children's (min / max are exact, so it
+    // equals the box over the node's triangles).
     int build(int lo, int hi, int64_t me, int level, int64_t par)
     {
         const double big = std::numeric_limits<double>::max();
-        double bmn[3] = {big, big, big}, bmx[3] = {-big, -big, -big};
-        double cmn[3] = {big, big, big}, cmx[3] = {-big, -big, -big};
-        for (int i = lo; i < hi; ++i)
-        {
-            const int t = idx[i];
-            const double* r = &box[6 * size_t(t)];
-            for (int k = 0; k < 3; ++k)
-            {
-                bmn[k] = minOf(bmn[k], r[k]);
-                bmx[k] = maxOf(bmx[k], r[3 + k]);
-                const double c = cen[size_t(k) * n + t];
-                cmn[k] = minOf(cmn[k], c);
-                cmx[k] = maxOf(cmx[k], c);
-            }
-        }
         BvhNode nd{};
-        for (int k = 0; k < 3; ++k)
-        {
-            nd.bmin[k] = bmn[k];
-            nd.bmax[k] = bmx[k];
-        }
         nd.left = nd.right = -1;
         nd.triStart = -1;
         nd.triCount = 0;
         const int count = hi - lo;
         if (count <= 8)
         {
+            double bmn[3] = {big, big, big}, bmx[3] = {-big, -big, -big};
+            for (int i = lo; i < hi; ++i)
+            {
+                const double* r = &box[6 * size_t(items[size_t(i)].idx)];
+                for (int k = 0; k < 3; ++k)
+                {
+                    bmn[k] = minOf(bmn[k], r[k]);
+                    bmx[k] = maxOf(bmx[k], r[3 + k]);
+                }
+            }
+            for (int k = 0; k < 3; ++k)
+            {
+                nd.bmin[k] = bmn[k];
+                nd.bmax[k] = bmx[k];
+            }
             nd.triStart = lo;
             nd.triCount = count;
-            nodes[me] = nd;
+            nodes[size_t(me)] = nd;
             return level;
+        }
+        double cmn[3] = {big, big, big}, cmx[3] = {-big, -big, -big};
+        for (int i = lo; i < hi; ++i)
+        {
+            const double* c = items[size_t(i)].c;
+            for (int k = 0; k < 3; ++k)
+            {
+                cmn[k] = minOf(cmn[k], c[k]);
+                cmx[k] = maxOf(cmx[k], c[k]);
+            }
         }
         const double ext[3] = {cmx[0] - cmn[0], cmx[1] - cmn[1], cmx[2] - cmn[2]};
         int axis = 0;
         if (ext[1] > ext[0]) axis = 1;
         if (ext[2] > ext[axis]) axis = 2;
         const int mid = lo + count / 2;
-        const double* ca = &cen[size_t(axis) * n];
-        std::nth_element(idx.begin() + lo, idx.begin() + mid, idx.begin() + hi,
-                         [ca](int x, int y) { return ca[x] < ca[y]; });
+        // the same comparisons, hence the same permutation, as nth_element over
+        // triangle indices compared by centroid (KdTree.h:226-229)
+        std::nth_element(items.begin() + lo, items.begin() + mid, items.begin() + hi,
+                         [axis](const Item& x, const Item& y) { return x.c[axis] < y.c[axis]; });
         const int64_t left = me + 1;
         const int64_t right = me + 1 + subtreeNodes(mid - lo);
-        nd.left = int(left);
-        nd.right = int(right);
-        nodes[me] = nd;
         int dl, dr;
         if (count > par)
         {
@@ -136,6 +171,16 @@ struct BvhBuilder
             dl = build(lo, mid, left, level + 1, par);
             dr = build(mid, hi, right, level + 1, par);
         }
+        const BvhNode& L = nodes[size_t(left)];
+        const BvhNode& R = nodes[size_t(right)];
+        for (int k = 0; k < 3; ++k)
+        {
+            nd.bmin[k] = minOf(L.bmin[k], R.bmin[k]);
+            nd.bmax[k] = maxOf(L.bmax[k], R.bmax[k]);
+        }
+        nd.left = int(left);
+        nd.right = int(right);
+        nodes[size_t(me)] = nd;
         return dl > dr ? dl : dr;
     }
 };
@@ -151,44 +196,53 @@ HostBvh buildHostBvh(const double* verts, int64_t nv, const int32_t* tris, int64
 {
     HostBvh h;
     const int n = int(nt);
-    std::vector<double> box(size_t(n) * 6), cen(size_t(n) * 3);
-    for (int t = 0; t < n; ++t)
-    {
-        const int32_t* ix = tris + 3 * size_t(t);
-        for (int c = 0; c < 3; ++c)
-            if (ix[c] < 0 || ix[c] >= nv)
-                throw Error(SDFO_ERR_INVALID_ARGUMENT, "triangle references invalid vertex");
-        const double* a = verts + 3 * size_t(ix[0]);
-        const double* b = verts + 3 * size_t(ix[1]);
-        const double* c = verts + 3 * size_t(ix[2]);
-        for (int k = 0; k < 3; ++k)
+    for (int64_t q = 0; q < 3 * nt; ++q)
+        if (tris[q] < 0 || tris[q] >= nv)
+            throw Error(SDFO_ERR_INVALID_ARGUMENT, "triangle references invalid vertex");
+    // per-triangle boxes and centroids (in slot-order items), in parallel chunks
+    std::vector<double> box(size_t(n) * 6);
+    std::vector<Item> items(static_cast<size_t>(n));
+    parallelFor(n, [&](int t0, int t1) {
+        for (int t = t0; t < t1; ++t)
         {
-            box[6 * size_t(t) + k] = minOf(minOf(a[k], b[k]), c[k]);
-            box[6 * size_t(t) + 3 + k] = maxOf(maxOf(a[k], b[k]), c[k]);
-            cen[size_t(k) * n + t] = ((a[k] + b[k]) + c[k]) / 3.0;
+            const int32_t* ix = tris + 3 * size_t(t);
+            const double* a = verts + 3 * size_t(ix[0]);
+            const double* b = verts + 3 * size_t(ix[1]);
+            const double* c = verts + 3 * size_t(ix[2]);
+            Item& it = items[size_t(t)];
+            for (int k = 0; k < 3; ++k)
+            {
+                box[6 * size_t(t) + k] = minOf(minOf(a[k], b[k]), c[k]);
+                box[6 * size_t(t) + 3 + k] = maxOf(maxOf(a[k], b[k]), c[k]);
+                it.c[k] = ((a[k] + b[k]) + c[k]) / 3.0;
+            }
+            it.idx = t;
+            it.pad = 0;
         }
-    }
-    std::vector<int> idx(n);
-    std::iota(idx.begin(), idx.end(), 0);
+    });
     if (n > 0)
     {
         h.nodes.resize(size_t(subtreeNodes(n)));
         // Subtrees of more than n/16 triangles (and at least 4096) run on their own thread.
         const int64_t par = std::max<int64_t>(4096, n / 16);
-        BvhBuilder bb{h.nodes, idx, box, cen, n};
+        BvhBuilder bb{h.nodes, items, box};
         h.depth = bb.build(0, n, 0, 1, par);
     }
-    h.slotTri.assign(idx.begin(), idx.end());
+    h.slotTri.resize(size_t(n));
     h.slotTri9.resize(size_t(n) * 9);
     h.slotBox6.resize(size_t(n) * 6);
-    for (int s = 0; s < n; ++s)
-    {
-        for (int q = 0; q < 6; ++q) h.slotBox6[6 * size_t(s) + q] = box[6 * size_t(idx[s]) + q];
-        const int32_t* ix = tris + 3 * size_t(idx[s]);
-        for (int c = 0; c < 3; ++c)
-            for (int k = 0; k < 3; ++k)
-                h.slotTri9[9 * size_t(s) + 3 * c + k] = verts[3 * size_t(ix[c]) + k];
-    }
+    parallelFor(n, [&](int s0, int s1) {
+        for (int s = s0; s < s1; ++s)
+        {
+            const int t = items[size_t(s)].idx;
+            h.slotTri[size_t(s)] = t;
+            for (int q = 0; q < 6; ++q) h.slotBox6[6 * size_t(s) + q] = box[6 * size_t(t) + q];
+            const int32_t* ix = tris + 3 * size_t(t);
+            for (int c = 0; c < 3; ++c)
+                for (int k = 0; k < 3; ++k)
+                    h.slotTri9[9 * size_t(s) + 3 * c + k] = verts[3 * size_t(ix[c]) + k];
+        }
+    });
     // KdTree.h:330-333: fixed irrational ray direction, normalised once on the host.
     const double d0[3] = {1.0, 0.3713906763541037, 0.52999894000318};
     const double len = std::sqrt(d0[0] * d0[0] + d0[1] * d0[1] + d0[2] * d0[2]);

@@ -194,6 +194,9 @@ int fillShapeFor(const GridSpec& g, int64_t ntris);
 int* replayCounterHost();
 bool& forceReplay();
 
+// Keeps the device's stream-ordered pool memory across synchronisations.
+void keepStreamPoolMemory(int device);
+
 // Grid completion helpers (capi.cpp): FP32 mirrors and the brick table.
 void finishDense(sdfo_grid& g, hipStream_t s);
 void finishSparse(sdfo_grid& g, const std::vector<int32_t>& bijk, hipStream_t s);
