@@ -46,6 +46,8 @@ extern "C" {
 #define SDFO_CONTACT_AWARE 4
 
 typedef struct sdfo_grid sdfo_grid;
+/* Broad-phase AABB tree over a body's quadrature points (geometry/bvh.py). */
+typedef struct sdfo_point_bvh sdfo_point_bvh;
 typedef void* sdfo_stream; /* hipStream_t */
 
 /* BuilderParams / SparseBuilderParams / ContactAwareBuilderParams
@@ -286,6 +288,63 @@ int64_t sdfo_tuner_report(char* buf, int64_t cap);
 int sdfo_contact_forward(const sdfo_grid* grid, const double* X, const double* w, int64_t n,
                          const double* poses, int64_t n_poses, const sdfo_contact_params* params,
                          double* out, int32_t* active, sdfo_stream stream);
+
+/* Linearisation cache of SDFContactEngine.evaluate (contacts_sdf.py:307-327),
+ * used by the Newton iterations of implicit_be.py:474-500.
+ * sdfo_contact_linearize fills it at one reference pose (a device record of
+ * 30 doubles, only R and t read): g0[n], grad0[3n] (raw gradient), valid0[n]
+ * (0 where the narrow filter or the field rejects the point), with the same
+ * transform and interpolation as sdfo_contact_forward. */
+int sdfo_contact_linearize(const sdfo_grid* grid, const double* X, int64_t n, const double* pose,
+                           const sdfo_contact_params* params, double* g0, double* grad0,
+                           uint8_t* valid0, sdfo_stream stream);
+
+/* Linearised contact forward: the SDF is not queried; point i's gap under a
+ * pose is g0[i] + grad0[i] . (t - t_ref) with gradient grad0[i], and it takes
+ * part only where valid0[i] and the narrow filter at the current pose pass;
+ * t_ref3 (host, 3 doubles) is the reference pose's t.  With bvh != NULL the
+ * broad-phase cull of sdfo_contact_forward_culled applies too.  The rest of
+ * the law and the output are sdfo_contact_forward's. */
+int sdfo_contact_forward_linearized(const double* g0, const double* grad0, const uint8_t* valid0,
+                                    const double* t_ref3, const double* X, const double* w, int64_t n,
+                                    const double* poses, int64_t n_poses,
+                                    const sdfo_contact_params* params, const sdfo_point_bvh* bvh,
+                                    const double* boxes6, double* out, int32_t* active,
+                                    sdfo_stream stream);
+
+/* ---- contact broad phase --------------------------------------------------- */
+/* AABB_BVH(points, leaf_size) (geometry/bvh.py:19-54) over a body's host FP64
+ * quadrature points (n x 3): longest-axis midpoint splits, a median split
+ * where one side would be empty, leaves of at most leaf_size points (the
+ * engine uses 16, contacts_sdf.py:63).  Host only: the leaf tables go to the
+ * device of the first call that culls with the tree, and later culling calls
+ * must run on that device. */
+int sdfo_point_bvh_build(const double* points, int64_t n, int32_t leaf_size, sdfo_point_bvh** out);
+void sdfo_point_bvh_free(sdfo_point_bvh* bvh);
+int sdfo_point_bvh_get_info(const sdfo_point_bvh* bvh, int64_t* n_points, int64_t* n_nodes,
+                            int64_t* n_leaves);
+
+/* AABB_BVH.query (bvh.py:56-75) on the host: indices (capacity n_points) of
+ * the points in every leaf whose box, grown by margin, overlaps
+ * [qmin3, qmax3], in the reference's order; *count receives their number. */
+int sdfo_point_bvh_query(const sdfo_point_bvh* bvh, const double* qmin3, const double* qmax3,
+                         double margin, int32_t* indices, int64_t* count);
+
+/* The same test for many boxes on the device: boxes6 = n_boxes x (qmin3,
+ * qmax3), mask[b * n_points + i] = 1 where point i is in a reported leaf. */
+int sdfo_point_bvh_cull(const sdfo_point_bvh* bvh, const double* boxes6, int64_t n_boxes,
+                        double margin, uint8_t* mask, sdfo_stream stream);
+
+/* sdfo_contact_forward behind the broad phase of SDFContactEngine.evaluate
+ * (contacts_sdf.py:245-272): for pose b only the points of the leaves of
+ * bvh (built over the same X) that overlap boxes6[6b..6b+5] (device; the
+ * engine's bmin_local, bmax_local, margin included) take part, or every
+ * point when no leaf overlaps (contacts_sdf.py:270-271). */
+int sdfo_contact_forward_culled(const sdfo_grid* grid, const sdfo_point_bvh* bvh,
+                                const double* boxes6, const double* X, const double* w, int64_t n,
+                                const double* poses, int64_t n_poses,
+                                const sdfo_contact_params* params, double* out, int32_t* active,
+                                sdfo_stream stream);
 
 /* Hand-written backward of sdfo_contact_forward (no reference counterpart).
  * g_out: B x 6 upstream gradient.  Outputs (device, each optional):

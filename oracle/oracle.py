@@ -361,6 +361,115 @@ def contact_backward(grid: Grid, phi, n3, X, w, poses, g_out, kn=1e11, c=1e4, mu
     return gt, gp[:3], gx, gw
 
 
+def contact_linearize(grid: Grid, phi, n3, X, pose, kn=1e11, c=1e4, mu=0.0, reg=1e-6, margin=0.0,
+                      order=2, mode="Tricubic"):
+    """Linearisation cache at one reference pose (contacts_sdf.py:319-327):
+    (g0 (N,), grad0 (N,3), valid0 (N,) uint8)."""
+    X = _f64(X, (-1, 3))
+    P = _f64(pose, (30,))
+    prm, iprm = _contact_params(kn, c, mu, reg, margin, order, mode)
+    n = len(X)
+    g0, grad0, valid0 = np.zeros(n), np.zeros((n, 3)), np.zeros(n, np.uint8)
+    lib().oracle_contact_linearize(_p(grid.grid9), _p(grid.shape3, _ip), _p(_f64(phi)),
+                                   _p(_f64(n3, (-1, 3))), _p(X), C.c_int64(n), _p(P), _p(prm),
+                                   _p(iprm, _ip), _p(g0), _p(grad0), _p(valid0, _u8p))
+    return g0, grad0, valid0
+
+
+def contact_forward_linearized(g0, grad0, valid0, t_ref, X, w, poses, kn=1e11, c=1e4, mu=0.0,
+                               reg=1e-6, margin=0.0, order=2):
+    """Forward on the cached prediction g0 + grad0 . (t - t_ref)
+    (contacts_sdf.py:311-318)."""
+    X = _f64(X, (-1, 3))
+    w = _f64(w)
+    P = _f64(poses, (-1, 30))
+    prm, iprm = _contact_params(kn, c, mu, reg, margin, order, 3)
+    out = np.zeros((len(P), 6))
+    act = np.zeros(len(P), np.int32)
+    lib().oracle_contact_forward_linearized(
+        _p(_f64(g0)), _p(_f64(grad0, (-1, 3))), _p(np.ascontiguousarray(valid0, np.uint8), _u8p),
+        _p(_f64(t_ref, (3,))), _p(X), _p(w), C.c_int64(len(X)), _p(P), C.c_int(len(P)), _p(prm),
+        _p(iprm, _ip), _p(out), _p(act, _ip))
+    return out, act
+
+
+# --------------------------------------------------------------- broad phase
+def point_bvh_build(points, leaf_size=16):
+    """AABB_BVH._build (prototype/src/geometry/bvh.py:27-54) as a node list:
+    dict(lo, hi, left, right, idx); node 0 is the root, children after their
+    parent, left subtree first.  The degenerate-split sort is stable."""
+    P = _f64(points, (-1, 3))
+    nodes = []
+
+    def build(idx):
+        me = len(nodes)
+        nodes.append(None)
+        pts = P[idx]
+        lo, hi = pts.min(axis=0), pts.max(axis=0)
+        if len(idx) <= leaf_size:
+            nodes[me] = dict(lo=lo, hi=hi, left=-1, right=-1, idx=idx.copy())
+            return me
+        axis = int(np.argmax(hi - lo))
+        mid = (lo[axis] + hi[axis]) * 0.5
+        left, right = idx[pts[:, axis] <= mid], idx[pts[:, axis] > mid]
+        if len(left) == 0 or len(right) == 0:
+            s = np.argsort(pts[:, axis], kind="stable")
+            half = len(idx) // 2
+            left, right = idx[s[:half]], idx[s[half:]]
+        li = build(left)
+        ri = build(right)
+        nodes[me] = dict(lo=lo, hi=hi, left=li, right=ri, idx=None)
+        return me
+
+    if len(P):
+        build(np.arange(len(P)))
+    return nodes
+
+
+def point_bvh_query(nodes, qmin, qmax, margin=0.0):
+    """AABB_BVH.query (bvh.py:56-75): point indices in the reference's order."""
+    qmin, qmax = _f64(qmin, (3,)), _f64(qmax, (3,))
+    if not nodes:
+        return np.zeros(0, np.int32)
+    r = nodes[0]
+    n = sum(len(nd["idx"]) for nd in nodes if nd["left"] < 0)
+    if (qmin <= r["lo"] - margin).all() and (qmax >= r["hi"] + margin).all():
+        return np.arange(n, dtype=np.int32)
+    stack, parts = [0], []
+    while stack:
+        nd = nodes[stack.pop()]
+        if np.any(nd["hi"] + margin < qmin) or np.any(nd["lo"] - margin > qmax):
+            continue
+        if nd["left"] < 0:
+            parts.append(nd["idx"])
+        else:
+            stack.append(nd["left"])
+            stack.append(nd["right"])
+    return np.concatenate(parts).astype(np.int32) if parts else np.zeros(0, np.int32)
+
+
+def broad_phase_boxes(poses, bmin, bmax, margin):
+    """The cull box of SDFContactEngine.evaluate with markers
+    (contacts_sdf.py:247-258): the SDF box corners mapped into A's frame by
+    the inverse of Y = R X + t, grown by margin; (B,6) = (qmin, qmax)."""
+    P = _f64(poses, (-1, 30))
+    out = np.zeros((len(P), 6))
+    for b in range(len(P)):
+        R_ab, t_ab = P[b, :9].reshape(3, 3), P[b, 9:12]
+        R_ba = R_ab.T
+        t_ba = -R_ab.T @ t_ab
+        out[b, :3] = R_ba @ _f64(bmin, (3,)) + t_ba - margin
+        out[b, 3:] = R_ba @ _f64(bmax, (3,)) + t_ba + margin
+    return out
+
+
+def culled_points(nodes, box6, n):
+    """bvh_indices of evaluate (contacts_sdf.py:266-271): the query's points,
+    or every point when the query returns none."""
+    idx = point_bvh_query(nodes, box6[:3], box6[3:])
+    return idx if len(idx) else np.arange(n, dtype=np.int32)
+
+
 def contact_forward_plane(z_top, X, w, poses, kn=1e6, c=0.0, mu=0.5, reg=0.0, margin=0.0,
                           order=1):
     X = _f64(X, (-1, 3))

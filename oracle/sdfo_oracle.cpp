@@ -2195,6 +2195,77 @@ void oracle_contact_backward(const double* grid9, const int* shape3, const doubl
     contactBackward(X, w, n, poses30, nb, cp, q, gOut6, gTwist6, gParams4, gX, gW);
 }
 
+// Linearisation cache of SDFContactEngine.evaluate (contacts_sdf.py:307-327):
+// fill at one reference pose with the dense query of the forward ...
+void oracle_contact_linearize(const double* grid9, const int* shape3, const double* phi,
+                              const double* n3, const double* X, int64_t n, const double* pose30,
+                              const double* params6, const int* iparams2, double* g0,
+                              double* grad0, unsigned char* valid0)
+{
+    DenseView d{};
+    d.g = gridFromArrays(grid9, shape3);
+    d.phi = phi;
+    d.nrm = n3;
+    const ContactParams cp{params6[0], params6[1], params6[2], params6[3], params6[4],
+                           iparams2[0], iparams2[1]};
+    const Pose P = loadPose(pose30);
+    for (int64_t i = 0; i < n; ++i)
+    {
+        const V3 Y = matvec(P.R, load3(X, i)) + v3(P.t[0], P.t[1], P.t[2]);
+        QueryOut q{};
+        q.valid = false;
+        if (!(cp.margin > 0.0 && !(std::abs(Y.z) < cp.margin))) q = denseQuery(d, Y, cp.mode, true);
+        valid0[i] = q.valid ? 1 : 0;
+        g0[i] = q.valid ? q.gap : 0.0;
+        grad0[3 * i] = q.valid ? q.grad.x : 0.0;
+        grad0[3 * i + 1] = q.valid ? q.grad.y : 0.0;
+        grad0[3 * i + 2] = q.valid ? q.grad.z : 0.0;
+    }
+}
+
+// ... and the forward on the prediction g = g0 + grad0 . (t - t_ref) with
+// the cached gradient (contacts_sdf.py:311-318), gated by the cached validity
+// and the narrow filter at the current pose.
+void oracle_contact_forward_linearized(const double* g0, const double* grad0,
+                                       const unsigned char* valid0, const double* tref3,
+                                       const double* X, const double* w, int64_t n,
+                                       const double* poses30, int nb, const double* params6,
+                                       const int* iparams2, double* out6, int* activeCount)
+{
+    const ContactParams cp{params6[0], params6[1], params6[2], params6[3], params6[4],
+                           iparams2[0], iparams2[1]};
+    for (int b = 0; b < nb; ++b)
+    {
+        const Pose P = loadPose(poses30 + 30 * int64_t(b));
+        const V3 dt = v3(P.t[0] - tref3[0], P.t[1] - tref3[1], P.t[2] - tref3[2]);
+        double F[3] = {0, 0, 0}, T[3] = {0, 0, 0};
+        int act = 0;
+        for (int64_t i = 0; i < n; ++i)
+        {
+            auto q = [&](V3) {
+                QueryOut r{};
+                r.valid = valid0[i] != 0;
+                r.grad = load3(grad0, i);
+                r.gap = g0[i] + dot(r.grad, dt);
+                return r;
+            };
+            const PointTerms t = pointForward(P, load3(X, i), cp, q);
+            if (!t.active) continue;
+            ++act;
+            const V3 wf = w[i] * t.fq;
+            const V3 tq = w[i] * cross(t.off, t.fq);
+            F[0] += wf.x; F[1] += wf.y; F[2] += wf.z;
+            T[0] += tq.x; T[1] += tq.y; T[2] += tq.z;
+        }
+        for (int c = 0; c < 3; ++c)
+        {
+            out6[6 * b + c] = F[c];
+            out6[6 * b + 3 + c] = T[c];
+        }
+        if (activeCount) activeCount[b] = act;
+    }
+}
+
 // Analytic-plane SDF contact (sdf_grid.py:4-22): phi = z - zTop, grad = e_z.
 void oracle_contact_forward_plane(double zTop, const double* X, const double* w, int64_t n,
                                   const double* poses30, int nb, const double* params6,

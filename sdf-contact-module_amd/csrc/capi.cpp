@@ -820,6 +820,140 @@ int sdfo_contact_forward(const sdfo_grid* grid, const double* X, const double* w
     });
 }
 
+int sdfo_contact_linearize(const sdfo_grid* grid, const double* X, int64_t n, const double* pose,
+                           const sdfo_contact_params* params, double* g0, double* grad0,
+                           uint8_t* valid0, sdfo_stream stream)
+{
+    return guarded([&] {
+        checkContact(grid, X, X, n, pose, 1, params);
+        if (n > 0 && (!g0 || !grad0 || !valid0))
+            throw Error(SDFO_ERR_INVALID_ARGUMENT, "the cache arrays are required");
+        DeviceGuard dg(grid->device);
+        launchContactLinearize(*grid, X, n, pose, *params, g0, grad0, valid0, hipStream_t(stream));
+    });
+}
+
+static CullArgs cullArgs(const sdfo_point_bvh* bvh, const double* boxes6, int64_t n,
+                         double margin)
+{
+    if (!boxes6) throw Error(SDFO_ERR_INVALID_ARGUMENT, "boxes6 is NULL");
+    if (bvh->n != n)
+        throw Error(SDFO_ERR_INVALID_ARGUMENT, "the point BVH was built over " +
+                                                   std::to_string(bvh->n) + " points, not " +
+                                                   std::to_string(n));
+    CullArgs c;
+    c.leafBox = bvh->leafBox;
+    c.leafOf = bvh->leafOf;
+    c.nLeaves = bvh->nLeaves;
+    c.boxes = boxes6;
+    c.margin = margin;
+    return c;
+}
+
+int sdfo_point_bvh_build(const double* points, int64_t n, int32_t leaf_size, sdfo_point_bvh** out)
+{
+    return guarded([&] {
+        if (!out) throw Error(SDFO_ERR_INVALID_ARGUMENT, "out is NULL");
+        *out = nullptr;
+        if (n < 0 || n > INT32_MAX) throw Error(SDFO_ERR_INVALID_ARGUMENT, "n must be in [0, 2^31)");
+        if (n > 0 && !points) throw Error(SDFO_ERR_INVALID_ARGUMENT, "points is NULL");
+        if (leaf_size < 1) throw Error(SDFO_ERR_INVALID_ARGUMENT, "leaf_size must be positive");
+        auto b = std::make_unique<sdfo_point_bvh>();
+        buildPointBvh(*b, points, n, leaf_size);
+        *out = b.release();
+    });
+}
+
+void sdfo_point_bvh_free(sdfo_point_bvh* bvh) { delete bvh; }
+
+int sdfo_point_bvh_get_info(const sdfo_point_bvh* bvh, int64_t* n_points, int64_t* n_nodes,
+                            int64_t* n_leaves)
+{
+    return guarded([&] {
+        if (!bvh) throw Error(SDFO_ERR_INVALID_ARGUMENT, "bvh is NULL");
+        if (n_points) *n_points = bvh->n;
+        if (n_nodes) *n_nodes = int64_t(bvh->nodes.size());
+        if (n_leaves) *n_leaves = bvh->nLeaves;
+    });
+}
+
+int sdfo_point_bvh_query(const sdfo_point_bvh* bvh, const double* qmin3, const double* qmax3,
+                         double margin, int32_t* indices, int64_t* count)
+{
+    return guarded([&] {
+        if (!bvh || !qmin3 || !qmax3 || !count)
+            throw Error(SDFO_ERR_INVALID_ARGUMENT, "NULL argument");
+        const std::vector<int32_t> r = queryPointBvh(*bvh, qmin3, qmax3, margin);
+        *count = int64_t(r.size());
+        if (!r.empty())
+        {
+            if (!indices) throw Error(SDFO_ERR_INVALID_ARGUMENT, "indices is NULL");
+            std::memcpy(indices, r.data(), r.size() * sizeof(int32_t));
+        }
+    });
+}
+
+int sdfo_point_bvh_cull(const sdfo_point_bvh* bvh, const double* boxes6, int64_t n_boxes,
+                        double margin, uint8_t* mask, sdfo_stream stream)
+{
+    return guarded([&] {
+        if (!bvh) throw Error(SDFO_ERR_INVALID_ARGUMENT, "bvh is NULL");
+        if (n_boxes < 0 || n_boxes > 65535)
+            throw Error(SDFO_ERR_INVALID_ARGUMENT, "n_boxes must be in [0, 65535]");
+        if (n_boxes == 0 || bvh->n == 0) return;
+        if (!mask) throw Error(SDFO_ERR_INVALID_ARGUMENT, "mask is NULL");
+        pointBvhToDevice(*bvh, currentDevice());
+        launchCullMask(cullArgs(bvh, boxes6, bvh->n, margin), bvh->n, n_boxes, mask,
+                       hipStream_t(stream));
+    });
+}
+
+int sdfo_contact_forward_culled(const sdfo_grid* grid, const sdfo_point_bvh* bvh,
+                                const double* boxes6, const double* X, const double* w, int64_t n,
+                                const double* poses, int64_t n_poses,
+                                const sdfo_contact_params* params, double* out, int32_t* active,
+                                sdfo_stream stream)
+{
+    return guarded([&] {
+        checkContact(grid, X, w, n, poses, n_poses, params);
+        if (n_poses > 0 && !out) throw Error(SDFO_ERR_INVALID_ARGUMENT, "out is NULL");
+        if (!bvh) throw Error(SDFO_ERR_INVALID_ARGUMENT, "bvh is NULL");
+        DeviceGuard dg(grid->device);
+        pointBvhToDevice(*bvh, grid->device);
+        const CullArgs c = cullArgs(bvh, boxes6, n, 0.0);
+        launchContactForward(*grid, X, w, n, poses, n_poses, *params, out, active,
+                             hipStream_t(stream), &c);
+    });
+}
+
+int sdfo_contact_forward_linearized(const double* g0, const double* grad0, const uint8_t* valid0,
+                                    const double* t_ref3, const double* X, const double* w, int64_t n,
+                                    const double* poses, int64_t n_poses,
+                                    const sdfo_contact_params* params, const sdfo_point_bvh* bvh,
+                                    const double* boxes6, double* out, int32_t* active,
+                                    sdfo_stream stream)
+{
+    return guarded([&] {
+        if (!params) throw Error(SDFO_ERR_INVALID_ARGUMENT, "params is NULL");
+        if (n < 0 || n_poses < 0) throw Error(SDFO_ERR_INVALID_ARGUMENT, "negative size");
+        if (n > 0 && (!g0 || !grad0 || !valid0 || !X || !w))
+            throw Error(SDFO_ERR_INVALID_ARGUMENT, "cache and quadrature arrays are required");
+        if (n_poses > 0 && (!poses || !out || !t_ref3))
+            throw Error(SDFO_ERR_INVALID_ARGUMENT, "poses, t_ref and out are required");
+        if (params->order < 0 || params->order > 8)
+            throw Error(SDFO_ERR_INVALID_ARGUMENT, "order must be in [0, 8]");
+        if (n_poses > 65535) throw Error(SDFO_ERR_INVALID_ARGUMENT, "at most 65535 poses per call");
+        CullArgs c;
+        if (bvh)
+        {
+            pointBvhToDevice(*bvh, currentDevice());
+            c = cullArgs(bvh, boxes6, n, 0.0);
+        }
+        launchContactForwardLinearized(g0, grad0, valid0, t_ref3, X, w, n, poses, n_poses, *params,
+                                       out, active, hipStream_t(stream), bvh ? &c : nullptr);
+    });
+}
+
 int sdfo_contact_backward(const sdfo_grid* grid, const double* X, const double* w, int64_t n,
                           const double* poses, int64_t n_poses, const sdfo_contact_params* params,
                           const double* g_out, double* g_twist, double* g_params, double* g_X,

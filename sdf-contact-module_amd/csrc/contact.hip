@@ -93,21 +93,51 @@ __device__ __forceinline__ double powi(double x, int k)
     return r;
 }
 
-// Forward terms of one (pose, point) pair (contacts_sdf.py:329-379).
+// Gap and raw gradient predicted from a cached query at a reference
+// translation (contacts_sdf.py:307-327): g = g0 + grad0 . (t - t_ref).
+struct LinCache
+{
+    const double* g0;
+    const double* grad0;
+    const uint8_t* valid0;
+    double tref[3];
+};
+
+// The field at Y = R X + t after the narrow filter (contacts_sdf.py:91-98,
+// 319): false where the point is filtered out or outside the field.
 template <bool WithHess>
-__device__ __forceinline__ Terms pointTerms(const QGrid& g, const Pose& P, V X, const Law& L)
+__device__ __forceinline__ bool sampleField(const QGrid& g, const Law& L, V Y, Interp<double>& it)
+{
+    if (L.margin > 0.0 && !(fabs(Y.z) < L.margin)) return false;
+    const Cell c = dev::locateCell(g, Y.x, Y.y, Y.z);
+    if (!c.valid) return false;
+    return L.mode == SDFO_TRILINEAR ? dev::trilinear<double, WithHess>(g, c, it)
+                                    : dev::tricubic<double, WithHess>(g, c, it);
+}
+
+// Forward terms of one (pose, point) pair (contacts_sdf.py:329-379); with
+// Cached, the field comes from the linearisation cache instead of the grid.
+template <bool WithHess, bool Cached = false>
+__device__ __forceinline__ Terms pointTerms(const QGrid& g, const Pose& P, V X, const Law& L,
+                                            const LinCache* lc = nullptr, int64_t i = 0)
 {
     Terms t;
     t.active = false;
     t.Y = vadd(matvec(P.R, X), V{P.t[0], P.t[1], P.t[2]});
     t.off = vadd(matvec(P.M, X), V{P.m[0], P.m[1], P.m[2]});
-    if (L.margin > 0.0 && !(fabs(t.Y.z) < L.margin)) return t; // contacts_sdf.py:91-98
-    const Cell c = dev::locateCell(g, t.Y.x, t.Y.y, t.Y.z);
-    if (!c.valid) return t;
     Interp<double> it;
-    const bool ok = L.mode == SDFO_TRILINEAR ? dev::trilinear<double, WithHess>(g, c, it)
-                                             : dev::tricubic<double, WithHess>(g, c, it);
-    if (!ok) return t;
+    if (Cached)
+    {
+        if (L.margin > 0.0 && !(fabs(t.Y.z) < L.margin)) return t;
+        if (!lc->valid0[i]) return t;
+        it.grad = dev::V3T<double>{lc->grad0[3 * i], lc->grad0[3 * i + 1], lc->grad0[3 * i + 2]};
+        const double dt[3] = {P.t[0] - lc->tref[0], P.t[1] - lc->tref[1], P.t[2] - lc->tref[2]};
+        it.gap = lc->g0[i] + (it.grad.x * dt[0] + it.grad.y * dt[1] + it.grad.z * dt[2]);
+#pragma unroll
+        for (int e = 0; e < 6; ++e) it.hess[e] = 0.0;
+    }
+    else if (!sampleField<WithHess>(g, L, t.Y, it))
+        return t;
     t.g = it.gap;
     const double delta = t.g < 0.0 ? -t.g : 0.0;
     t.p = L.kn * powi(delta, L.order);
@@ -163,11 +193,75 @@ __device__ __forceinline__ void blockReduce(double (&v)[NV], double* sh)
     for (int k = 0; k < NV; ++k) v[k] = sh[k * kBlock];
 }
 
+// Cache fill at one reference pose (contacts_sdf.py:319-327): the same
+// point transform, narrow filter and interpolation as the forward.
+__global__ void __launch_bounds__(kBlock)
+contactLinearizeKernel(QGrid g, Law L, const double* __restrict__ X, int64_t n,
+                       const double* __restrict__ pose, double* __restrict__ g0,
+                       double* __restrict__ grad0, uint8_t* __restrict__ valid0)
+{
+    const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const Pose P = loadPose(pose);
+    const V x = V{X[3 * i], X[3 * i + 1], X[3 * i + 2]};
+    const V Y = vadd(matvec(P.R, x), V{P.t[0], P.t[1], P.t[2]});
+    Interp<double> it;
+    const bool ok = sampleField<false>(g, L, Y, it);
+    g0[i] = ok ? it.gap : 0.0;
+    grad0[3 * i] = ok ? it.grad.x : 0.0;
+    grad0[3 * i + 1] = ok ? it.grad.y : 0.0;
+    grad0[3 * i + 2] = ok ? it.grad.z : 0.0;
+    valid0[i] = ok ? 1 : 0;
+}
+
+// The overlap test of AABB_BVH.query (bvh.py:66) on one leaf box.
+__device__ __forceinline__ bool leafOverlaps(const double* __restrict__ lb, const double (&q)[6],
+                                             double margin)
+{
+    bool hit = true;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) hit = hit && !(lb[3 + k] + margin < q[k]) && !(lb[k] - margin > q[3 + k]);
+    return hit;
+}
+
+__device__ __forceinline__ void loadBox(const double* __restrict__ b, double (&q)[6])
+{
+#pragma unroll
+    for (int k = 0; k < 6; ++k) q[k] = b[k];
+}
+
+// grid = poses.  anyLeaf[pose] = 1 when some leaf overlaps the pose's box.
+__global__ void __launch_bounds__(kBlock) cullAnyKernel(CullArgs cu, uint8_t* __restrict__ anyLeaf)
+{
+    double q[6];
+    loadBox(cu.boxes + 6 * int64_t(blockIdx.x), q);
+    int hit = 0;
+    for (int64_t l = threadIdx.x; l < cu.nLeaves && !hit; l += kBlock)
+        hit = leafOverlaps(cu.leafBox + 6 * l, q, cu.margin) ? 1 : 0;
+    hit = __syncthreads_or(hit);
+    if (threadIdx.x == 0) anyLeaf[blockIdx.x] = uint8_t(hit ? 1 : 0);
+}
+
+// grid = (point blocks, boxes).  mask[box * n + i]: point i's leaf overlaps.
+__global__ void __launch_bounds__(kBlock)
+cullMaskKernel(CullArgs cu, int64_t n, uint8_t* __restrict__ mask)
+{
+    const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= n) return;
+    double q[6];
+    loadBox(cu.boxes + 6 * int64_t(blockIdx.y), q);
+    mask[int64_t(blockIdx.y) * n + i] = leafOverlaps(cu.leafBox + 6 * int64_t(cu.leafOf[i]), q, cu.margin) ? 1 : 0;
+}
+
 // grid = (chunks, poses).  partial[(pose*chunks + chunk)*7 + k]: F, tau, active.
+// With Culled, points outside the pose's broad-phase leaves are skipped
+// unless no leaf overlaps (then every point takes part, contacts_sdf.py:270-271).
+template <bool Cached, bool Culled>
 __global__ void __launch_bounds__(kBlock)
 contactForwardKernel(QGrid g, Law L, const double* __restrict__ X, const double* __restrict__ w,
                      int64_t n, const double* __restrict__ poses, int64_t chunkLen,
-                     double* __restrict__ partial)
+                     double* __restrict__ partial, LinCache lc, CullArgs cu,
+                     const uint8_t* __restrict__ anyLeaf)
 {
     __shared__ double sh[7 * kBlock];
     const int64_t pose = blockIdx.y;
@@ -175,11 +269,20 @@ contactForwardKernel(QGrid g, Law L, const double* __restrict__ X, const double*
     const Pose P = loadPose(poses + 30 * pose);
     const int64_t beg = int64_t(chunk) * chunkLen;
     const int64_t end = beg + chunkLen < n ? beg + chunkLen : n;
+    double q[6];
+    bool cull = false;
+    if (Culled)
+    {
+        loadBox(cu.boxes + 6 * pose, q);
+        cull = anyLeaf[pose] != 0;
+    }
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
     for (int64_t i = beg + threadIdx.x; i < end; i += kBlock)
     {
+        if (Culled && cull && !leafOverlaps(cu.leafBox + 6 * int64_t(cu.leafOf[i]), q, cu.margin))
+            continue;
         const V x = V{X[3 * i], X[3 * i + 1], X[3 * i + 2]};
-        const Terms t = pointTerms<false>(g, P, x, L);
+        const Terms t = pointTerms<false, Cached>(g, P, x, L, &lc, i);
         if (!t.active) continue;
         const double wi = w[i];
         const V tq = vcross(t.off, t.fq);
@@ -436,22 +539,77 @@ struct Scratch
 
 } // namespace
 
-void launchContactForward(const sdfo_grid& g, const double* X, const double* w, int64_t n,
-                          const double* poses, int64_t nb, const sdfo_contact_params& p,
-                          double* out, int32_t* active, hipStream_t s)
+namespace {
+
+// Forward launch shared by the plain, linearised and culled entry points.
+template <bool Cached>
+void forwardLaunch(const QGrid& q, const Law& L, const double* X, const double* w, int64_t n,
+                   const double* poses, int64_t nb, const LinCache& lc, const CullArgs* cull,
+                   double* out, int32_t* active, hipStream_t s)
 {
-    if (nb <= 0) return;
-    const QGrid q = dev::makeQGrid(g);
-    const Law L = makeLaw(p);
     const int chunks = n > 0 ? chunksFor(n, nb) : 1;
     const int64_t chunkLen = n > 0 ? (n + chunks - 1) / chunks : 0;
-    Scratch part(sizeof(double) * 7 * size_t(nb) * chunks, s);
-    hipLaunchKernelGGL(contactForwardKernel, dim3(chunks, unsigned(nb)), dim3(kBlock), 0, s, q, L,
-                       X, w, n, poses, chunkLen, static_cast<double*>(part.p));
+    Scratch part(sizeof(double) * 7 * size_t(nb) * chunks + (cull ? size_t(nb) : 0), s);
+    double* partial = static_cast<double*>(part.p);
+    uint8_t* anyLeaf = reinterpret_cast<uint8_t*>(partial + 7 * size_t(nb) * chunks);
+    if (cull)
+    {
+        hipLaunchKernelGGL(cullAnyKernel, dim3(unsigned(nb)), dim3(kBlock), 0, s, *cull, anyLeaf);
+        SDFO_HIP(hipGetLastError());
+        hipLaunchKernelGGL((contactForwardKernel<Cached, true>), dim3(chunks, unsigned(nb)),
+                           dim3(kBlock), 0, s, q, L, X, w, n, poses, chunkLen, partial, lc, *cull,
+                           anyLeaf);
+    }
+    else
+        hipLaunchKernelGGL((contactForwardKernel<Cached, false>), dim3(chunks, unsigned(nb)),
+                           dim3(kBlock), 0, s, q, L, X, w, n, poses, chunkLen, partial, lc,
+                           CullArgs{}, nullptr);
     SDFO_HIP(hipGetLastError());
     hipLaunchKernelGGL(contactForwardFinalize, dim3(unsigned((nb + 255) / 256)), dim3(256), 0, s,
-                       nb, chunks, static_cast<const double*>(part.p), out, active);
+                       nb, chunks, static_cast<const double*>(partial), out, active);
     SDFO_HIP(hipGetLastError());
+}
+
+} // namespace
+
+void launchContactForward(const sdfo_grid& g, const double* X, const double* w, int64_t n,
+                          const double* poses, int64_t nb, const sdfo_contact_params& p,
+                          double* out, int32_t* active, hipStream_t s, const CullArgs* cull)
+{
+    if (nb <= 0) return;
+    forwardLaunch<false>(dev::makeQGrid(g), makeLaw(p), X, w, n, poses, nb, LinCache{}, cull, out,
+                         active, s);
+}
+
+void launchCullMask(const CullArgs& cu, int64_t n, int64_t nBoxes, uint8_t* mask, hipStream_t s)
+{
+    if (n <= 0 || nBoxes <= 0) return;
+    hipLaunchKernelGGL(cullMaskKernel, dim3(unsigned((n + kBlock - 1) / kBlock), unsigned(nBoxes)),
+                       dim3(kBlock), 0, s, cu, n, mask);
+    SDFO_HIP(hipGetLastError());
+}
+
+void launchContactLinearize(const sdfo_grid& g, const double* X, int64_t n, const double* pose,
+                            const sdfo_contact_params& p, double* g0, double* grad0,
+                            uint8_t* valid0, hipStream_t s)
+{
+    if (n <= 0) return;
+    const QGrid q = dev::makeQGrid(g);
+    const Law L = makeLaw(p);
+    hipLaunchKernelGGL(contactLinearizeKernel, dim3(unsigned((n + kBlock - 1) / kBlock)),
+                       dim3(kBlock), 0, s, q, L, X, n, pose, g0, grad0, valid0);
+    SDFO_HIP(hipGetLastError());
+}
+
+void launchContactForwardLinearized(const double* g0, const double* grad0, const uint8_t* valid0,
+                                    const double* tref, const double* X, const double* w, int64_t n,
+                                    const double* poses, int64_t nb, const sdfo_contact_params& p,
+                                    double* out, int32_t* active, hipStream_t s,
+                                    const CullArgs* cull)
+{
+    if (nb <= 0) return;
+    const LinCache lc{g0, grad0, valid0, {tref[0], tref[1], tref[2]}};
+    forwardLaunch<true>(QGrid{}, makeLaw(p), X, w, n, poses, nb, lc, cull, out, active, s);
 }
 
 void launchContactBackward(const sdfo_grid& g, const double* X, const double* w, int64_t n,

@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <functional>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -163,7 +164,51 @@ struct sdfo_grid
     }
 };
 
+// Quadrature-point AABB tree of the contact broad phase (geometry/bvh.py).
+struct PointBvhNode
+{
+    double lo[3], hi[3];
+    int32_t left = -1, right = -1; // -1: leaf
+    int64_t start = 0, count = 0;  // leaf: its points are order[start, start + count)
+};
+
+struct sdfo_point_bvh
+{
+    int device = -1; // set by the first call that culls with the tree
+    int64_t n = 0;
+    int leafSize = 16;
+    std::vector<PointBvhNode> nodes; // nodes[0] is the root
+    std::vector<int32_t> order;      // point indices, leaf by leaf
+    int64_t nLeaves = 0;
+    std::vector<double> hostLeafBox; // lo[3], hi[3] per leaf (leaves in node order)
+    std::vector<int32_t> hostLeafOf; // leaf of each point
+    double* leafBox = nullptr;       // device copies of the two tables
+    int32_t* leafOf = nullptr;
+    std::mutex uploadMutex;
+    sdfo_point_bvh() = default;
+    sdfo_point_bvh(const sdfo_point_bvh&) = delete;
+    sdfo_point_bvh& operator=(const sdfo_point_bvh&) = delete;
+    ~sdfo_point_bvh();
+};
+
 namespace sdfo {
+
+void buildPointBvh(sdfo_point_bvh& b, const double* points, int64_t n, int leafSize);
+void pointBvhToDevice(const sdfo_point_bvh& b, int device);
+std::vector<int32_t> queryPointBvh(const sdfo_point_bvh& b, const double* qmin, const double* qmax,
+                                   double margin);
+
+// Per-pose broad-phase cull of the contact kernels: point i of pose b takes
+// part when its leaf's box overlaps boxes[6b..6b+5] (qmin, qmax) grown by
+// margin, or when no leaf does (contacts_sdf.py:270-271).
+struct CullArgs
+{
+    const double* leafBox = nullptr;
+    const int32_t* leafOf = nullptr;
+    int64_t nLeaves = 0;
+    const double* boxes = nullptr;
+    double margin = 0.0;
+};
 
 // Phase timings of the calling thread's last dense build, in ms:
 // grid set-up, host BVH, BVH upload + allocations (host clock), fill kernel
@@ -280,7 +325,17 @@ void launchCandidatesF64(const sdfo_grid& g, const double* pts, int64_t n, uint8
 // Contact launchers (contact.hip).
 void launchContactForward(const sdfo_grid& g, const double* X, const double* w, int64_t n,
                           const double* poses, int64_t nb, const sdfo_contact_params& p,
-                          double* out, int32_t* active, hipStream_t s);
+                          double* out, int32_t* active, hipStream_t s,
+                          const CullArgs* cull = nullptr);
+void launchCullMask(const CullArgs& cu, int64_t n, int64_t nBoxes, uint8_t* mask, hipStream_t s);
+void launchContactLinearize(const sdfo_grid& g, const double* X, int64_t n, const double* pose,
+                            const sdfo_contact_params& p, double* g0, double* grad0,
+                            uint8_t* valid0, hipStream_t s);
+void launchContactForwardLinearized(const double* g0, const double* grad0, const uint8_t* valid0,
+                                    const double* tref, const double* X, const double* w, int64_t n,
+                                    const double* poses, int64_t nb, const sdfo_contact_params& p,
+                                    double* out, int32_t* active, hipStream_t s,
+                                    const CullArgs* cull = nullptr);
 void launchContactBackward(const sdfo_grid& g, const double* X, const double* w, int64_t n,
                            const double* poses, int64_t nb, const sdfo_contact_params& p,
                            const double* gOut, double* gTwist, double* gParams, double* gX,

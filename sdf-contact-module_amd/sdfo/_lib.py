@@ -138,6 +138,16 @@ _sig("sdfo_tuner_set_mode", [_i32])
 _sig("sdfo_tuner_reset", [])
 _sig("sdfo_tuner_report", [_vp, _i64], C.c_int64)
 _sig("sdfo_contact_forward", [_vp, _vp, _vp, _i64, _vp, _i64, _P(ContactParams), _vp, _vp, _vp])
+_sig("sdfo_contact_linearize", [_vp, _vp, _i64, _vp, _P(ContactParams), _vp, _vp, _vp, _vp])
+_sig("sdfo_contact_forward_linearized", [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _i64,
+                                         _P(ContactParams), _vp, _vp, _vp, _vp, _vp])
+_sig("sdfo_point_bvh_build", [_vp, _i64, _i32, _P(_vp)])
+_sig("sdfo_point_bvh_free", [_vp], None)
+_sig("sdfo_point_bvh_get_info", [_vp, _P(_i64), _P(_i64), _P(_i64)])
+_sig("sdfo_point_bvh_query", [_vp, _vp, _vp, C.c_double, _vp, _P(_i64)])
+_sig("sdfo_point_bvh_cull", [_vp, _vp, _i64, C.c_double, _vp, _vp])
+_sig("sdfo_contact_forward_culled", [_vp, _vp, _vp, _vp, _vp, _i64, _vp, _i64, _P(ContactParams),
+                                     _vp, _vp, _vp])
 _sig("sdfo_contact_backward", [_vp, _vp, _vp, _i64, _vp, _i64, _P(ContactParams), _vp, _vp, _vp,
                                _vp, _vp, _vp])
 

@@ -84,6 +84,161 @@ def contact_forward(grid: SdfGrid, X, w, poses, law: ContactLaw = ContactLaw(), 
     return (out, act) if active else out
 
 
+class PointBvh:
+    """Broad-phase AABB tree over a body's quadrature points (AABB_BVH,
+    prototype/src/geometry/bvh.py:16-75), built on the host.
+
+    ``query`` is the reference's host query (indices in its order); ``cull``
+    evaluates the same leaf test for many boxes on the device."""
+
+    def __init__(self, points, leaf_size=16):
+        import numpy as np
+        P = np.ascontiguousarray(points, np.float64).reshape(-1, 3)
+        h = C.c_void_p()
+        check(lib.sdfo_point_bvh_build(P.ctypes.data_as(C.c_void_p), len(P), int(leaf_size),
+                                       C.byref(h)))
+        self.handle = h
+        self.n = len(P)
+        nn, nl = C.c_int64(), C.c_int64()
+        check(lib.sdfo_point_bvh_get_info(h, None, C.byref(nn), C.byref(nl)))
+        self.n_nodes, self.n_leaves = nn.value, nl.value
+
+    def close(self):
+        if self.handle:
+            lib.sdfo_point_bvh_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001  (interpreter shutdown)
+            pass
+
+    def query(self, query_min, query_max, margin=0.0):
+        """AABB_BVH.query: int32 indices of the points of every overlapping leaf."""
+        import numpy as np
+        qmin = np.ascontiguousarray(query_min, np.float64).reshape(3)
+        qmax = np.ascontiguousarray(query_max, np.float64).reshape(3)
+        out = np.empty(self.n, np.int32)
+        cnt = C.c_int64()
+        check(lib.sdfo_point_bvh_query(self.handle, qmin.ctypes.data_as(C.c_void_p),
+                                       qmax.ctypes.data_as(C.c_void_p), float(margin),
+                                       out.ctypes.data_as(C.c_void_p), C.byref(cnt)))
+        return out[:cnt.value].copy()
+
+    def cull(self, boxes, margin=0.0, stream=None):
+        """(B,N) uint8 device mask for (B,6) float64 device boxes (qmin, qmax)."""
+        if not boxes.is_cuda or boxes.dtype != torch.float64 or boxes.dim() != 2 \
+                or boxes.shape[1] != 6:
+            raise ValueError("boxes must be a (B,6) float64 GPU tensor")
+        boxes = boxes.contiguous()
+        mask = torch.empty((boxes.shape[0], self.n), dtype=torch.uint8, device=boxes.device)
+        check(lib.sdfo_point_bvh_cull(self.handle, _ptr(boxes), boxes.shape[0], float(margin),
+                                      _ptr(mask), _stream_handle(stream)))
+        return mask
+
+
+def broad_phase_boxes(poses, bmin, bmax, margin):
+    """Cull boxes of SDFContactEngine.evaluate (contacts_sdf.py:247-258): the
+    SDF box corners ``bmin``, ``bmax`` mapped into A's frame by the inverse of
+    ``Y = R X + t``, grown by ``margin`` (the engine passes
+    ``max(narrow_margin, 0.05)``).  (B,30) poses -> (B,6) (qmin, qmax)."""
+    R = poses[:, :9].reshape(-1, 3, 3)
+    t = poses[:, 9:12]
+    Rt = R.transpose(1, 2)
+    t_ba = -(Rt @ t[:, :, None])[..., 0]
+    lo = torch.as_tensor(bmin, dtype=torch.float64, device=poses.device).reshape(1, 3, 1)
+    hi = torch.as_tensor(bmax, dtype=torch.float64, device=poses.device).reshape(1, 3, 1)
+    qmin = (Rt @ lo)[..., 0] + t_ba - margin
+    qmax = (Rt @ hi)[..., 0] + t_ba + margin
+    return torch.cat([qmin, qmax], dim=1).contiguous()
+
+
+def contact_forward_culled(grid: SdfGrid, bvh: PointBvh, boxes, X, w, poses,
+                           law: ContactLaw = ContactLaw(), stream=None, active=False):
+    """contact_forward behind the broad phase (contacts_sdf.py:245-272): for
+    pose b only the points of ``bvh``'s leaves that overlap ``boxes[b]`` take
+    part, or all points when none does."""
+    _check_inputs(X, w, poses)
+    boxes = boxes.contiguous()
+    if boxes.shape != (poses.shape[0], 6) or boxes.dtype != torch.float64 or not boxes.is_cuda:
+        raise ValueError("boxes must be a (B,6) float64 GPU tensor")
+    B = poses.shape[0]
+    out = torch.empty((B, 6), dtype=torch.float64, device=X.device)
+    act = torch.empty(B, dtype=torch.int32, device=X.device) if active else None
+    p = law.ctypes()
+    check(lib.sdfo_contact_forward_culled(grid.handle, bvh.handle, _ptr(boxes), _ptr(X), _ptr(w),
+                                          X.shape[0], _ptr(poses), B, C.byref(p), _ptr(out),
+                                          _ptr(act), _stream_handle(stream)))
+    return (out, act) if active else out
+
+
+class LinearizationCache:
+    """SDF linearisation cache for Newton iterations (contacts_sdf.py:40-50,
+    307-327; enabled around each implicit step by implicit_be.py:474-500).
+
+    ``fill(pose)`` queries the field once at a reference pose and keeps the
+    gaps, raw gradients and validity on the device; ``forward(poses)`` then
+    evaluates the contact law on ``g0 + grad0 . (t - t_ref)`` for every pose
+    without touching the SDF.  Entries are kept per quadrature point; a point
+    takes part only where it was valid at the reference pose and passes the
+    narrow filter at the current one.  ``clear()`` drops the cache (the
+    reference's ``_cache_cleared``); ``forward`` on an empty cache fills it at
+    the first pose, as the reference's first evaluation of a step does."""
+
+    def __init__(self, grid: SdfGrid, X, w, law: ContactLaw = ContactLaw()):
+        _check_inputs(X, w, torch.zeros((0, 30), dtype=torch.float64, device=X.device))
+        self.grid, self.X, self.w, self.law = grid, X, w, law
+        n = X.shape[0]
+        self.g0 = torch.empty(n, dtype=torch.float64, device=X.device)
+        self.grad0 = torch.empty((n, 3), dtype=torch.float64, device=X.device)
+        self.valid0 = torch.empty(n, dtype=torch.uint8, device=X.device)
+        self.t_ref = None
+
+    @property
+    def filled(self) -> bool:
+        return self.t_ref is not None
+
+    def clear(self):
+        self.t_ref = None
+
+    def fill(self, pose, stream=None):
+        """Caches the field at one (30,) or (1,30) pose record."""
+        pose = pose.reshape(30).contiguous()
+        if not pose.is_cuda or pose.dtype != torch.float64:
+            raise ValueError("pose must be a float64 GPU tensor")
+        p = self.law.ctypes()
+        check(lib.sdfo_contact_linearize(self.grid.handle, _ptr(self.X), self.X.shape[0],
+                                         _ptr(pose), C.byref(p), _ptr(self.g0), _ptr(self.grad0),
+                                         _ptr(self.valid0), _stream_handle(stream)))
+        self.t_ref = (C.c_double * 3)(*pose[9:12].tolist())
+        return self
+
+    def forward(self, poses, stream=None, active=False, bvh: PointBvh = None, boxes=None):
+        """(F_A, tau_A) per pose as a (B,6) float64 tensor, on the cache;
+        with ``bvh`` and ``boxes``, behind the broad phase of
+        ``contact_forward_culled``."""
+        _check_inputs(self.X, self.w, poses)
+        if (bvh is None) != (boxes is None):
+            raise ValueError("bvh and boxes go together")
+        if boxes is not None:
+            boxes = boxes.contiguous()
+            if boxes.shape != (poses.shape[0], 6) or boxes.dtype != torch.float64:
+                raise ValueError("boxes must be a (B,6) float64 GPU tensor")
+        if not self.filled:
+            self.fill(poses[0], stream)
+        B = poses.shape[0]
+        out = torch.empty((B, 6), dtype=torch.float64, device=self.X.device)
+        act = torch.empty(B, dtype=torch.int32, device=self.X.device) if active else None
+        p = self.law.ctypes()
+        check(lib.sdfo_contact_forward_linearized(
+            _ptr(self.g0), _ptr(self.grad0), _ptr(self.valid0), self.t_ref, _ptr(self.X),
+            _ptr(self.w), self.X.shape[0], _ptr(poses), B, C.byref(p),
+            None if bvh is None else bvh.handle, _ptr(boxes), _ptr(out), _ptr(act),
+            _stream_handle(stream)))
+        return (out, act) if active else out
+
+
 def contact_backward(grid: SdfGrid, X, w, poses, g_out, law: ContactLaw = ContactLaw(),
                      want_twist=True, want_params=True, want_X=True, want_w=True, stream=None):
     """Gradients of sum(g_out * forward): (g_twist (B,6) [omega, dt] left twist,

@@ -186,3 +186,36 @@ def test_candidate_order_matches_std_sort(O):
         else:
             vals = np.where(rng.random(n) < 0.5, 1.0, -1.0) * rng.integers(1, 3, n)
         assert np.array_equal(_lib.debug_sort_by_abs(vals), O.sort_by_abs(vals)), (n, vals)
+
+
+@pytest.mark.parametrize("seed,leaf", [(0, 16), (1, 3), (2, 1), (3, 64)])
+def test_point_bvh_matches_oracle(O, seed, leaf):
+    """The library's broad-phase tree (host only) returns the oracle's
+    indices in the oracle's order (bvh.py:56-75)."""
+    from sdfo import contact as SC
+    rng = np.random.default_rng(seed)
+    P = np.concatenate([rng.normal(size=(700, 3)) * [0.05, 0.02, 0.01],
+                        rng.uniform(-0.05, 0.05, (300, 3))])
+    P[::37] = P[5]
+    P[1::41, 0] = P[2, 0]
+    bvh = SC.PointBvh(P, leaf_size=leaf)
+    nodes = O.point_bvh_build(P, leaf_size=leaf)
+    assert bvh.n_nodes == len(nodes)
+    assert bvh.n_leaves == sum(nd["left"] < 0 for nd in nodes)
+    for k in range(40):
+        c = rng.uniform(-0.06, 0.06, 3)
+        h = rng.uniform(0.0, 0.05, 3)
+        lo, hi = (c - h, c + h) if k % 5 else (c + h, c - h)     # inverted boxes too
+        m = 0.0 if k % 3 else 0.004
+        assert np.array_equal(bvh.query(lo, hi, m), O.point_bvh_query(nodes, lo, hi, m))
+    assert np.array_equal(bvh.query([-1] * 3, [1] * 3), np.arange(len(P)))
+    bvh.close()
+
+
+def test_point_bvh_empty_and_errors():
+    from sdfo import contact as SC
+    b = SC.PointBvh(np.zeros((0, 3)))
+    assert (b.n_nodes, b.n_leaves) == (0, 0)
+    assert b.query([0, 0, 0], [1, 1, 1]).size == 0
+    with pytest.raises(sdfo.SdfoError, match="leaf_size"):
+        SC.PointBvh(np.zeros((4, 3)), leaf_size=0)

@@ -100,3 +100,144 @@ def test_contact_autograd_function(O, cube):
     law0 = SC.ContactLaw(kn=1e11, damping=0.0, mu=0.3)
     base = SC.contact_forward(g, Xt.detach(), wt, SC.pose_records(R0, t0, v0=v0, Om=Om), law0)
     assert torch.allclose(kn.grad * 1e11, (base[:, 2] + 10.0 * base[:, 5]).sum(), rtol=1e-9)
+
+
+# ------------------------------------------------------ linearisation cache
+@pytest.mark.parametrize("margin", [0.0, 0.065])
+def test_linearize_fill_matches_oracle(O, cube, margin):
+    g, ref = cube
+    print(f"[gpu] linearize fill margin={margin}", flush=True)
+    X, w, poses = _scene(4, seed=21)
+    law = SC.ContactLaw(kn=1e11, damping=1e4, mu=0.3, regularizer=1e-6, narrow_margin=margin)
+    Xd, wd = _cuda(X, w)
+    for b in range(len(poses)):
+        cache = SC.LinearizationCache(g, Xd, wd, law).fill(_cuda(poses[b])[0])
+        g0, grad0, valid0 = O.contact_linearize(ograd(O, g.spec), f32(ref["phi"]), f32(ref["n"]),
+                                                X, poses[b], kn=1e11, c=1e4, mu=0.3, reg=1e-6,
+                                                margin=margin)
+        assert np.array_equal(cache.valid0.cpu().numpy(), valid0)
+        assert valid0.sum() > 8
+        assert np.abs(cache.g0.cpu().numpy() - g0).max() <= 1e-12
+        assert np.abs(cache.grad0.cpu().numpy() - grad0).max() <= 1e-10
+
+
+def test_linearized_forward_at_reference_pose_is_bitwise_forward(cube):
+    g, _ = cube
+    X, w, poses = _scene(8, seed=22)
+    law = SC.ContactLaw(kn=1e11, damping=1e4, mu=0.3, regularizer=1e-6, order=2)
+    Xd, wd, Pd = _cuda(X, w, poses)
+    direct, da = SC.contact_forward(g, Xd, wd, Pd, law, active=True)
+    for b in range(len(poses)):
+        cache = SC.LinearizationCache(g, Xd, wd, law).fill(Pd[b])
+        lin, la = cache.forward(Pd[b:b + 1].contiguous(), active=True)
+        assert torch.equal(la[0], da[b])
+        assert torch.equal(lin[0], direct[b])
+
+
+@pytest.mark.parametrize("mu,order,margin", [(0.3, 2, 0.0), (0.0, 1, 0.0), (0.5, 2, 0.065)])
+def test_linearized_forward_matches_oracle(O, cube, mu, order, margin):
+    """Perturbed poses (translation and rotation) around one reference pose,
+    the cache filled on the GPU and handed to the oracle's linearised forward."""
+    g, _ = cube
+    print(f"[gpu] linearized forward mu={mu} order={order} margin={margin}", flush=True)
+    X, w, base = _scene(1, seed=23)
+    rng = np.random.default_rng(5)
+    from sdfo.scenes import rot_axis
+    poses = np.repeat(base, 64, axis=0)
+    for b in range(1, 64):
+        Rr = rot_axis(rng.normal(size=3), rng.uniform(0, 0.01))
+        poses[b, :9] = (Rr @ base[0, :9].reshape(3, 3)).ravel()
+        poses[b, 12:21] = poses[b, :9]
+        poses[b, 9:12] += rng.uniform(-2e-3, 2e-3, 3)
+        poses[b, 11] -= 4e-3                          # deeper than the reference pose
+    law = SC.ContactLaw(kn=1e11, damping=1e4, mu=mu, regularizer=1e-6, narrow_margin=margin,
+                        order=order)
+    Xd, wd, Pd = _cuda(X, w, poses)
+    cache = SC.LinearizationCache(g, Xd, wd, law)
+    out, act = cache.forward(Pd, active=True)        # fills at the first pose
+    assert cache.filled
+    o_ref, a_ref = O.contact_forward_linearized(
+        cache.g0.cpu().numpy(), cache.grad0.cpu().numpy(), cache.valid0.cpu().numpy(),
+        poses[0, 9:12], X, w, poses, kn=1e11, c=1e4, mu=mu, reg=1e-6, margin=margin, order=order)
+    assert np.array_equal(act.cpu().numpy(), a_ref)
+    assert (a_ref > 0).sum() > 40 and a_ref.max() > 100
+    out = out.cpu().numpy()
+    scale = np.abs(o_ref).max(axis=0) + 1e-300
+    assert np.all(np.abs(out - o_ref) <= 1e-9 * scale)
+    cache.clear()
+    assert not cache.filled
+    again = cache.forward(Pd[5:].contiguous())      # refills at pose 5
+    assert torch.allclose(again[0], SC.contact_forward(g, Xd, wd, Pd[5:6].contiguous(), law)[0],
+                          rtol=0, atol=0)
+
+
+# --------------------------------------------------------------- broad phase
+def _cull_scene(g, n_poses=48, seed=31):
+    X, w, poses = _scene(n_poses, seed=seed)
+    rng = np.random.default_rng(seed)
+    poses[:, 9:11] += rng.uniform(-0.09, 0.09, (n_poses, 2))   # ring partly off the SDF box
+    # no yaw on even poses: their corner-mapped boxes stay upright
+    poses[::2, :9] = np.eye(3).ravel()
+    poses[::2, 12:21] = np.eye(3).ravel()
+    poses[-1, 9] = 1.0                                          # no leaf overlaps
+    return X, w, poses
+
+
+@pytest.mark.parametrize("margin", [0.0, 0.01])
+def test_cull_mask_matches_oracle(O, cube, margin):
+    g, _ = cube
+    print(f"[gpu] broad-phase cull margin={margin}", flush=True)
+    X, _, poses = _cull_scene(g)
+    bvh = SC.PointBvh(X, leaf_size=16)
+    nodes = O.point_bvh_build(X, leaf_size=16)
+    Pd = _cuda(poses)[0]
+    boxes = SC.broad_phase_boxes(Pd, g.spec.bmin, g.spec.bmax, 0.02)
+    ob = O.broad_phase_boxes(poses, g.spec.bmin, g.spec.bmax, 0.02)
+    assert np.abs(boxes.cpu().numpy() - ob).max() <= 1e-15
+    mask = bvh.cull(boxes, margin).cpu().numpy()
+    B = boxes.cpu().numpy()
+    sizes = []
+    for b in range(len(poses)):
+        want = np.zeros(len(X), np.uint8)
+        want[O.point_bvh_query(nodes, B[b, :3], B[b, 3:], margin)] = 1
+        assert np.array_equal(mask[b], want), b
+        sizes.append(int(want.sum()))
+    assert min(sizes) == 0 and max(sizes) == len(X) and len(set(sizes)) > 10
+
+
+def test_culled_forward_matches_oracle(O, cube):
+    g, ref = cube
+    print("[gpu] culled contact forward", flush=True)
+    X, w, poses = _cull_scene(g)
+    nodes = O.point_bvh_build(X, leaf_size=16)
+    bvh = SC.PointBvh(X)
+    Xd, wd, Pd = _cuda(X, w, poses)
+    boxes = SC.broad_phase_boxes(Pd, g.spec.bmin, g.spec.bmax, 0.0)
+    law = SC.ContactLaw(kn=1e11, damping=1e4, mu=0.3, regularizer=1e-6, order=2)
+    out, act = SC.contact_forward_culled(g, bvh, boxes, Xd, wd, Pd, law, active=True)
+    full, fact = SC.contact_forward(g, Xd, wd, Pd, law, active=True)
+    B = boxes.cpu().numpy()
+    out, act = out.cpu().numpy(), act.cpu().numpy()
+    for b in range(len(poses)):
+        idx = O.culled_points(nodes, B[b], len(X))
+        o, a = O.contact_forward(ograd(O, g.spec), f32(ref["phi"]), f32(ref["n"]), X[idx], w[idx],
+                                 poses[b:b + 1], kn=1e11, c=1e4, mu=0.3, reg=1e-6, order=2)
+        assert act[b] == a[0], b
+        scale = np.abs(o[0]).max() + 1e-300
+        assert np.abs(out[b] - o[0]).max() <= 1e-9 * scale, b
+    # contacts lie inside the SDF box, so upright boxes cull none of them
+    assert np.array_equal(act[::2], fact.cpu().numpy()[::2]) and act.max() > 100
+    assert torch.equal(torch.from_numpy(out[-1]), full[-1].cpu())   # empty cull -> all points
+
+    # the linearised forward behind the same cull
+    cache = SC.LinearizationCache(g, Xd, wd, law).fill(Pd[0])
+    lo, la = cache.forward(Pd, active=True, bvh=bvh, boxes=boxes)
+    g0, gr, v0 = (t.cpu().numpy() for t in (cache.g0, cache.grad0, cache.valid0))
+    for b in range(len(poses)):
+        idx = O.culled_points(nodes, B[b], len(X))
+        o, a = O.contact_forward_linearized(g0[idx], gr[idx], v0[idx], poses[0, 9:12], X[idx],
+                                            w[idx], poses[b:b + 1], kn=1e11, c=1e4, mu=0.3,
+                                            reg=1e-6, order=2)
+        assert la[b].item() == a[0], b
+        scale = np.abs(o[0]).max() + 1e-300
+        assert np.abs(lo[b].cpu().numpy() - o[0]).max() <= 1e-9 * scale, b

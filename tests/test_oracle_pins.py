@@ -435,3 +435,162 @@ def test_contact_backward_matches_central_differences(O):
     wp[i] += 1e-6
     wm[i] -= 1e-6
     assert np.isclose(gw[i], (L(P, X, wp) - L(P, X, wm)) / 2e-6, rtol=1e-6)
+
+
+# ------------------------------------------------------ linearisation cache
+def _cube_contact_scene(O, n_poses=3, seed=11):
+    from sdfo import scenes
+    v, t = scenes.cube_mesh()
+    g = O.setup_grid(v, 0.1, 0.0, 32, 32)
+    d = O.build_dense(v, t, g)
+    X, w = scenes.ring_quadrature(30, 8)
+    R, tt = scenes.ring_poses(n_poses, seed=seed, z_range=(0.058, 0.060))
+    v0, Om = scenes.pose_velocities(n_poses, seed=seed + 1)
+    return g, d, X, w, scenes.pose_array(R, tt, v0=v0, Om=Om)
+
+
+@pytest.mark.parametrize("margin", [0.0, 0.059])
+def test_linearization_cache_at_reference_pose_is_the_forward(O, margin):
+    """contacts_sdf.py:311-327: at the pose the cache was filled at, the
+    prediction g0 + grad0 . 0 is the query itself, so the linearised forward
+    equals the direct one bit for bit; the cache holds the dense query at
+    Y = R X + t behind the narrow filter."""
+    g, d, X, w, P = _cube_contact_scene(O)
+    law = dict(kn=1e11, c=1e4, mu=0.3, reg=1e-3, order=2, margin=margin)
+    for b in range(len(P)):
+        g0, grad0, valid0 = O.contact_linearize(g, d["phi"], d["n"], X, P[b], mode="Tricubic",
+                                                **law)
+        lin, la = O.contact_forward_linearized(g0, grad0, valid0, P[b, 9:12], X, w, P[b:b + 1],
+                                               **law)
+        ref, ra = O.contact_forward(g, d["phi"], d["n"], X, w, P[b:b + 1], mode="Tricubic", **law)
+        assert ra[0] > 0 and np.array_equal(la, ra)
+        assert np.array_equal(lin, ref)
+        Y = X @ P[b, :9].reshape(3, 3).T + P[b, 9:12]
+        q = O.query_dense(g, d, "Tricubic", Y)
+        keep = q["valid"] & ((np.abs(Y[:, 2]) < margin) if margin > 0 else True)
+        assert np.array_equal(valid0.astype(bool), keep)
+        assert np.allclose(g0[keep], q["gap"][keep], rtol=0, atol=1e-13)
+        assert np.allclose(grad0[keep], q["grad"][keep], rtol=0, atol=1e-11)
+        assert not g0[~keep].any() and not grad0[~keep].any()
+
+
+def test_linearization_exact_on_linear_field(O):
+    """On a linear field the first-order prediction is exact: translated poses
+    give the direct forward up to rounding (a wrong sign or a dropped
+    component of grad0 . (t - t_ref) moves the gaps by 2e-3)."""
+    a = np.array([0.1, -0.2, 1.0])
+    f = lambda c: c @ a - 0.1  # noqa: E731
+    df = lambda c: np.tile(a, (len(c), 1))  # noqa: E731
+    g, d = _field_grid(O, f, df, n=16)
+    from sdfo import scenes
+    X, w = meshes.annular_quadrature(0.05, 0.15, 6, 24)
+    R0 = scenes.rot_axis(np.array([1.0, 0.5, 0.0]), 0.05)[None]
+    t0 = np.array([[0.1, 0.15, 0.095]])
+    P0 = scenes.pose_array(R0, t0, Om=np.array([[0.1, -0.2, 1.5]]))
+    law = dict(kn=1e8, c=1e3, mu=0.3, reg=1e-4, order=2)
+    g0, grad0, valid0 = O.contact_linearize(g, d["phi"], d["n"], X, P0[0], mode="Tricubic", **law)
+    assert valid0.all()
+    rng = np.random.default_rng(3)
+    P = np.repeat(P0, 6, axis=0)
+    P[:, 9:12] += rng.uniform(-2e-3, 2e-3, (6, 3))
+    lin, la = O.contact_forward_linearized(g0, grad0, valid0, P0[0, 9:12], X, w, P, **law)
+    ref, ra = O.contact_forward(g, d["phi"], d["n"], X, w, P, mode="Tricubic", **law)
+    assert np.array_equal(la, ra) and (ra > 0).all() and (ra < len(X)).any()
+    assert np.allclose(lin, ref, rtol=1e-9, atol=1e-9 * np.abs(ref).max())
+
+
+def test_linearization_ignores_rotation_change(O):
+    """The cache predicts from the translation only (contacts_sdf.py:312):
+    a pose that differs from the reference by a rotation keeps every cached
+    gap, so the contact set is the reference pose's."""
+    g, d, X, w, P = _cube_contact_scene(O, n_poses=1)
+    law = dict(kn=1e11, c=0.0, mu=0.0, reg=1e-3, order=1)
+    g0, grad0, valid0 = O.contact_linearize(g, d["phi"], d["n"], X, P[0], mode="Tricubic", **law)
+    from sdfo.scenes import rot_axis
+    Q = P.copy()
+    Q[0, :9] = (rot_axis(np.array([1.0, 0.0, 0.0]), 0.05) @ P[0, :9].reshape(3, 3)).ravel()
+    lin, la = O.contact_forward_linearized(g0, grad0, valid0, P[0, 9:12], X, w, Q, **law)
+    _, ra = O.contact_forward(g, d["phi"], d["n"], X, w, P, mode="Tricubic", **law)
+    _, rq = O.contact_forward(g, d["phi"], d["n"], X, w, Q, mode="Tricubic", **law)
+    assert la[0] == ra[0] and rq[0] != ra[0]
+    # order 1, no damping or friction: F = sum w kn max(-g0, 0) grad0
+    pen = np.maximum(-g0, 0.0) * valid0
+    assert np.allclose(lin[0, :3], (1e11 * w * pen) @ grad0, rtol=1e-12)
+
+
+# --------------------------------------------------------------- broad phase
+def test_point_bvh_known_answer(O):
+    """Four points on a line, leaf size 1 (bvh.py:27-75 by hand): splits at
+    x = 1.5, 0.5, 2.5; a box from x = 0.5 rejects the leaf {0}; leaves come
+    out right subtree first; an enclosing box returns arange."""
+    P = np.array([[0.0, 0, 0], [1, 0, 0], [2, 0, 0], [3, 0, 0]])
+    nodes = O.point_bvh_build(P, leaf_size=1)
+    assert len(nodes) == 7
+    assert O.point_bvh_query(nodes, [0.5, -1, -1], [3.5, 1, 1]).tolist() == [3, 2, 1]
+    assert O.point_bvh_query(nodes, [-0.5, -1, -1], [2.2, 1, 1]).tolist() == [2, 1, 0]
+    assert O.point_bvh_query(nodes, [-1, -1, -1], [4, 1, 1]).tolist() == [0, 1, 2, 3]
+    assert O.point_bvh_query(nodes, [0.5, -1, -1], [3.5, 1, 1], margin=0.6).tolist() == [3, 2, 1, 0]
+    assert O.point_bvh_query(nodes, [5, 5, 5], [6, 6, 6]).tolist() == []
+    # identical points: every split is the median split of the stable order
+    same = O.point_bvh_build(np.zeros((5, 3)), leaf_size=2)
+    leaves = [nd["idx"].tolist() for nd in same if nd["left"] < 0]
+    assert leaves == [[0, 1], [2], [3, 4]]
+
+
+@pytest.mark.parametrize("seed,leaf", [(0, 16), (1, 4), (2, 1)])
+def test_point_bvh_invariants(O, seed, leaf):
+    """Brute force over the tree: leaves partition the points, node boxes are
+    the min/max of their points, midpoint splits separate the children, and a
+    query returns every point inside the box and only points of leaves whose
+    box overlaps it."""
+    rng = np.random.default_rng(seed)
+    P = np.concatenate([rng.normal(size=(300, 3)) * [1, 0.3, 0.1],
+                        rng.uniform(-1, 1, (200, 3))])
+    P[::50] = P[1]                                     # duplicates
+    nodes = O.point_bvh_build(P, leaf_size=leaf)
+
+    def pts(i):
+        nd = nodes[i]
+        return nd["idx"] if nd["left"] < 0 else np.concatenate([pts(nd["left"]), pts(nd["right"])])
+    allp = np.sort(np.concatenate([nd["idx"] for nd in nodes if nd["left"] < 0]))
+    assert np.array_equal(allp, np.arange(len(P)))
+    for i, nd in enumerate(nodes):
+        idx = pts(i)
+        assert np.array_equal(nd["lo"], P[idx].min(0)) and np.array_equal(nd["hi"], P[idx].max(0))
+        if nd["left"] < 0:
+            assert len(idx) <= leaf
+            continue
+        a = int(np.argmax(nd["hi"] - nd["lo"]))
+        mid = (nd["lo"][a] + nd["hi"][a]) / 2
+        L, R = pts(nd["left"]), pts(nd["right"])
+        assert len(L) + len(R) == len(idx)
+        if (P[idx, a] <= mid).all() or (P[idx, a] > mid).all():
+            assert abs(len(L) - len(R)) <= 1           # median split
+        else:
+            assert (P[L, a] <= mid).all() and (P[R, a] > mid).all()
+    leafs = [nd for nd in nodes if nd["left"] < 0]
+    for _ in range(20):
+        c = rng.uniform(-1, 1, 3)
+        h = rng.uniform(0.05, 0.6, 3)
+        q = O.point_bvh_query(nodes, c - h, c + h)
+        assert len(set(q.tolist())) == len(q)
+        inside = np.nonzero(((P >= c - h) & (P <= c + h)).all(1))[0]
+        assert set(inside.tolist()) <= set(q.tolist())
+        hit = [nd for nd in leafs if (nd["hi"] >= c - h).all() and (nd["lo"] <= c + h).all()]
+        want = np.concatenate([nd["idx"] for nd in hit]) if hit else np.zeros(0)
+        assert sorted(q.tolist()) == sorted(want.tolist()) or len(q) == len(P)
+
+
+def test_broad_phase_boxes_closed_form(O):
+    """bmin_local = R^T (bmin - t) - margin per corner (contacts_sdf.py:252-258),
+    a quarter turn about z swapping x and y with a sign."""
+    c, s = 0.0, 1.0
+    R = np.array([[c, -s, 0], [s, c, 0], [0, 0, 1]])
+    from sdfo import scenes
+    P = scenes.pose_array(np.stack([np.eye(3), R]), np.array([[0.1, 0.2, 0.3], [0.1, 0.2, 0.3]]))
+    b = O.broad_phase_boxes(P, [-1, -2, -3], [1, 2, 3], 0.05)
+    assert np.allclose(b[0], [-1.15, -2.25, -3.35, 0.95, 1.85, 2.75], atol=1e-15)
+    # R^T (x, y, z) = (y, -x, z): corner-wise, so qmin.y > qmax.y here
+    assert np.allclose(b[1], [-2.25, 1.05, -3.35, 1.85, -0.85, 2.75], atol=1e-15)
+    nodes = O.point_bvh_build(np.zeros((3, 3)) + 10.0)
+    assert O.culled_points(nodes, b[0], 3).tolist() == [0, 1, 2]     # empty cull -> all
