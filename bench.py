@@ -355,6 +355,7 @@ def main():
     n_vox_torus = int(np.prod(sdfo.setup_grid(torus[0], target_resolution=128,
                                               max_resolution=128).shape))
 
+    lin_cache = SC.LinearizationCache(cube_grid, X, w, law).fill(poses[0])
     q_gap = torch.empty(args.query_points, dtype=torch.float32, device=dev)
     q_grad = torch.empty((args.query_points, 3), dtype=torch.float32, device=dev)
     q_valid = torch.empty(args.query_points, dtype=torch.uint8, device=dev)
@@ -383,7 +384,7 @@ def main():
             rec["build_s"] += time.perf_counter() - t0
             rec["phases"].append(_lib.debug_build_phases())
             g.close()
-        e = [ev() for _ in range(8)]
+        e = [ev() for _ in range(10)]
         # (2) 1M-point tricubic query; each timed segment starts behind a short
         # device spin so the host's launch latency is not inside the events
         busy()
@@ -412,14 +413,22 @@ def main():
                 SC.contact_forward(cube_grid, X, w, pb, law_f)
                 SC.contact_backward(cube_grid, X, w, pb, g_out[b:b + 1], law_f)
         e[7].record(stream)
+        # (5) the same 1000 poses on the linearisation cache (a Newton iterate)
+        busy()
+        e[8].record(stream)
+        if "contact" in parts:
+            lin_cache.forward(poses)
+        e[9].record(stream)
         torch.cuda.synchronize()
         rec["query_ms"].append(e[0].elapsed_time(e[1]))
         rec["contact_ms"].append(e[2].elapsed_time(e[3]))
         rec["fwdbwd_ms"].append(e[4].elapsed_time(e[5]))
         rec["plain_ms"].append(e[6].elapsed_time(e[7]))
+        rec["lin_ms"].append(e[8].elapsed_time(e[9]))
 
     def new_rec():
-        return dict(build_s=0.0, query_ms=[], contact_ms=[], fwdbwd_ms=[], plain_ms=[], phases=[])
+        return dict(build_s=0.0, query_ms=[], contact_ms=[], fwdbwd_ms=[], plain_ms=[], lin_ms=[],
+                    phases=[])
 
     warm = new_rec()
     for _ in range(args.warmup):
@@ -437,6 +446,7 @@ def main():
     c_ms = max(float(np.mean(rec["contact_ms"])), 1e-9)
     fb_ms = max(float(np.mean(rec["fwdbwd_ms"])), 1e-9)
     pl_ms = max(float(np.mean(rec["plain_ms"])), 1e-9)
+    lin_ms = max(float(np.mean(rec["lin_ms"])), 1e-9)
     build_s = max(rec["build_s"] / K, 1e-12)
     nq = args.query_points
     value = nq / (q_ms * 1e-3) / 1e6
@@ -472,6 +482,8 @@ def main():
         "build_ms": round(build_s * 1e3, 3),
         "query_kernel_ms": round(q_ms, 4),
         "contact_fwd_evals_per_s": round(n_eval / (c_ms * 1e-3), 1),
+        "contact_linearized_evals_per_s": round(n_eval / (lin_ms * 1e-3), 1),
+        "linearized_vs_direct": round(c_ms / lin_ms, 3),
         "contact_fwdbwd_evals_per_s": round(n_eval_b / (fb_ms * 1e-3), 1),
         "plain_fwdbwd_evals_per_s": round(n_eval_b / (pl_ms * 1e-3), 1),
         "fwdbwd_vs_plain": round(pl_ms / fb_ms, 3),
