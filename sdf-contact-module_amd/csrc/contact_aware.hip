@@ -42,6 +42,24 @@ struct CTri
     D3 a, b, c, n0, n1, n2, face;
 };
 
+// Vertices only (the distance test needs no normals).
+__device__ __forceinline__ void loadCTriVerts(const double* __restrict__ t21, int t, CTri& r)
+{
+    const double* p = t21 + 21 * size_t(t);
+    r.a = dev::d3(p[0], p[1], p[2]);
+    r.b = dev::d3(p[3], p[4], p[5]);
+    r.c = dev::d3(p[6], p[7], p[8]);
+}
+
+__device__ __forceinline__ void loadCTriNormals(const double* __restrict__ t21, int t, CTri& r)
+{
+    const double* p = t21 + 21 * size_t(t);
+    r.n0 = dev::d3(p[9], p[10], p[11]);
+    r.n1 = dev::d3(p[12], p[13], p[14]);
+    r.n2 = dev::d3(p[15], p[16], p[17]);
+    r.face = dev::d3(p[18], p[19], p[20]);
+}
+
 __device__ __forceinline__ CTri loadCTri(const double* __restrict__ t21, int t)
 {
     const double* p = t21 + 21 * size_t(t);
@@ -121,6 +139,18 @@ struct Cand
     D3 normal;
     D3 witness;
 };
+
+// The rest of makeContactCandidate once the closest point cp is known.
+__device__ __forceinline__ void finishCandidate(D3 p, const CTri& t, D3 cp, Cand& c)
+{
+    const D3 bc = barycentric(cp, t.a, t.b, t.c);
+    const D3 blend = dev::add(dev::add(dev::scale(bc.x, t.n0), dev::scale(bc.y, t.n1)),
+                              dev::scale(bc.z, t.n2));
+    normalizedOrDefault(blend.x, blend.y, blend.z, t.face.x, t.face.y, t.face.z, c.normal.x,
+                        c.normal.y, c.normal.z);
+    c.phi = dev::dot(dev::sub(p, cp), c.normal);
+    c.witness = cp;
+}
 
 // makeContactCandidate (SdfBuilder.cpp:260-279).
 __device__ __forceinline__ Cand makeCandidate(D3 p, const CTri& t)
@@ -229,29 +259,43 @@ __device__ inline void sortByAbsPhi(MergeView& m)
     sortPerm(m.order, m.count, AbsPhiOrder{m.phi});
 }
 
-// buildContactCandidatesForPoint (SdfBuilder.cpp:281-359).  Returns the
-// merged count (> m.cap on overflow); candidates sorted by |phi|.
-__device__ inline int candidatesForPoint(D3 p, const CAParams& P, const double* __restrict__ t21,
-                                         const int32_t* __restrict__ list, int listLen,
-                                         const BvhNode* __restrict__ nodes,
-                                         const double* __restrict__ tri9,
-                                         const int32_t* __restrict__ slotTri, dev::LaneStack st,
-                                         MergeView& m)
+// buildContactCandidatesForPoint (SdfBuilder.cpp:281-359) over the voxel's
+// triangle list, visited by each(f) in list order (f returns false to stop).
+// Returns the merged count (> m.cap on overflow); candidates sorted by |phi|.
+// haveMin: minDistIn is already the list's minimum distance.  Lanes with
+// !active take part in each() (which may be wave-collective) and nothing else.
+template <class Each>
+__device__ inline int candidatesFor(D3 p, const CAParams& P, const double* __restrict__ t21,
+                                    const Each& each, bool haveMin, double minDistIn, bool active,
+                                    const BvhNode* __restrict__ nodes,
+                                    const double* __restrict__ tri9,
+                                    const int32_t* __restrict__ slotTri, dev::LaneStack st,
+                                    MergeView& m)
 {
-    double minDist = DBL_MAX;
-    for (int q = 0; q < listLen; ++q)
+    double minDist = minDistIn;
+    if (!haveMin)
     {
-        const int t = P.useTiles ? list[q] : q;
-        const Cand c = makeCandidate(p, loadCTri(t21, t));
-        minDist = (c.dist < minDist) ? c.dist : minDist;
+        minDist = DBL_MAX;
+        each([&](int t) {
+            const Cand c = makeCandidate(p, loadCTri(t21, t));
+            minDist = (c.dist < minDist) ? c.dist : minDist;
+            return true;
+        });
     }
     // DBL_MAX is finite, so the reference's isfinite() fallback never fires for
     // finite input; an empty list reaches the nearest-triangle fallback below.
     const double allowed = minDist + P.tol;
     m.count = 0;
     auto tryTri = [&](int t) -> bool {
-        const Cand c = makeCandidate(p, loadCTri(t21, t));
+        // makeCandidate's distance first: the rest only for triangles in reach
+        CTri tr;
+        loadCTriVerts(t21, t, tr);
+        const D3 cp = dev::closestPoint(p, tr.a, tr.b, tr.c);
+        Cand c;
+        c.dist = sqrt(dev::sqnorm(dev::sub(p, cp)));
         if (!(c.dist <= allowed)) return true;
+        loadCTriNormals(t21, t, tr);
+        finishCandidate(p, tr, cp, c);
         for (int e = 0; e < m.count; ++e)
         {
             if (m.nx[e] * c.normal.x + m.ny[e] * c.normal.y + m.nz[e] * c.normal.z >= P.cosMerge)
@@ -280,8 +324,8 @@ __device__ inline int candidatesForPoint(D3 p, const CAParams& P, const double* 
         m.tri[e] = t;
         return true;
     };
-    for (int q = 0; q < listLen; ++q)
-        if (!tryTri(P.useTiles ? list[q] : q)) return m.count;
+    each(tryTri);
+    if (m.count > m.cap || !active) return m.count;
     if (m.count == 0)
     {
         double d;
@@ -295,30 +339,82 @@ __device__ inline int candidatesForPoint(D3 p, const CAParams& P, const double* 
 struct CAData
 {
     const double* t21;
-    const int32_t* tileStart;
+    const double* tbox6;      // 6 per triangle: bmin3, bmax3
+    const int64_t* tileStart; // with tiles: CSR lists, triangle order within each tile
     const int32_t* tileTris;
+    const int32_t* triTile;   // with tiles: 6 per triangle, the tile range whose lists hold it
+    const int32_t* nodeTile;  // with tiles: 6 per BVH node, the union of its triangles' ranges
     const BvhNode* nodes;
     const double* tri9;
     const int32_t* slotTri;
+    double mu; // search margin, far above the rounding of box and triangle distances
 };
 
-__device__ __forceinline__ void tileList(const CAParams& P, const CAData& D, int i, int j, int k,
-                                         const int32_t*& list, int& len)
+// The tile of a voxel (ContactTileIndex::candidatesForVoxel, SdfBuilder.cpp:102-113).
+__device__ __forceinline__ void voxelTile(const CAParams& P, int i, int j, int k, int& ti, int& tj,
+                                          int& tk)
 {
-    if (!P.useTiles)
-    {
-        list = nullptr;
-        len = P.ntris;
-        return;
-    }
-    // ContactTileIndex::candidatesForVoxel (SdfBuilder.cpp:102-113)
-    int ti = i / P.tileDim, tj = j / P.tileDim, tk = k / P.tileDim;
+    ti = i / P.tileDim;
+    tj = j / P.tileDim;
+    tk = k / P.tileDim;
     ti = ti < 0 ? 0 : (ti > P.tnx - 1 ? P.tnx - 1 : ti);
     tj = tj < 0 ? 0 : (tj > P.tny - 1 ? P.tny - 1 : tj);
     tk = tk < 0 ? 0 : (tk > P.tnz - 1 ? P.tnz - 1 : tk);
-    const int64_t tile = ti + int64_t(P.tnx) * (tj + int64_t(P.tny) * tk);
-    list = D.tileTris + D.tileStart[tile];
-    len = D.tileStart[tile + 1] - D.tileStart[tile];
+}
+
+__device__ __forceinline__ bool inRange(const int32_t* __restrict__ r, int ti, int tj, int tk)
+{
+    return r[0] <= ti && ti <= r[3] && r[1] <= tj && tj <= r[4] && r[2] <= tk && tk <= r[5];
+}
+
+// Triangle t is on the list of tile (ti, tj, tk) exactly when its tile range
+// holds the tile (buildContactTileIndex, SdfBuilder.cpp:202-258); without
+// tiles every triangle is on every list.  Lists are in triangle order.
+__device__ __forceinline__ bool onList(const CAParams& P, const CAData& D, int t, int ti, int tj,
+                                       int tk)
+{
+    return !P.useTiles || inRange(D.triTile + 6 * int64_t(t), ti, tj, tk);
+}
+
+// Ascending sort of distinct triangle ids: insertion sort for short rows,
+// heap sort for long ones.
+__device__ inline void sortIds(int* a, int n)
+{
+    if (n <= 32)
+    {
+        for (int i = 1; i < n; ++i)
+        {
+            const int v = a[i];
+            int j = i - 1;
+            while (j >= 0 && a[j] > v)
+            {
+                a[j + 1] = a[j];
+                --j;
+            }
+            a[j + 1] = v;
+        }
+        return;
+    }
+    auto sift = [&](int r, int end) {
+        while (2 * r + 1 < end)
+        {
+            int c = 2 * r + 1;
+            if (c + 1 < end && a[c] < a[c + 1]) ++c;
+            if (!(a[r] < a[c])) return;
+            const int t = a[r];
+            a[r] = a[c];
+            a[c] = t;
+            r = c;
+        }
+    };
+    for (int r = n / 2 - 1; r >= 0; --r) sift(r, n);
+    for (int end = n - 1; end > 0; --end)
+    {
+        const int t = a[0];
+        a[0] = a[end];
+        a[end] = t;
+        sift(0, end);
+    }
 }
 
 struct DenseOut
@@ -429,29 +525,262 @@ __device__ __forceinline__ D3 voxelCentre(const GridSpec& g, int i, int j, int k
                    g.bmin[2] + g.voxel[2] * (k + 0.5));
 }
 
+__device__ __forceinline__ bool waveAny(bool v) { return __ballot(v) != 0ull; }
+
+// Smallest squared distance from p to a triangle on the lane's list
+// (DBL_MAX when the list is empty): the minimum the list scan of
+// buildContactCandidatesForPoint computes first (SdfBuilder.cpp:287-297),
+// found by a wave-cooperative walk of the triangle BVH instead.  One shared
+// stack (entry e in lane e), a node visited if any lane needs it; a lane
+// prunes a node when its box lies farther than sqrt(best) + mu (mu is far
+// above the rounding of box and triangle distances) or when its triangles'
+// tile ranges miss the lane's tile.  Every lane of the wave must call it.
+__device__ double waveListMinSq(D3 p, bool active, int ti, int tj, int tk, const CAParams& P,
+                                const CAData& D)
+{
+    const int lane = threadIdx.x & 63;
+    double bestSq = DBL_MAX, bound = DBL_MAX;
+    int stk = 0;
+    int sp = 1;
+    while (sp > 0)
+    {
+        --sp;
+        const int ni = __builtin_amdgcn_readlane(stk, sp);
+        const BvhNode nd = D.nodes[ni];
+        bool need = active && dev::boxSq(p, nd) <= bound;
+        if (P.useTiles) need = need && inRange(D.nodeTile + 6 * int64_t(ni), ti, tj, tk);
+        if (!waveAny(need)) continue;
+        if (nd.triCount > 0)
+        {
+            for (int s = nd.triStart; s < nd.triStart + nd.triCount; ++s)
+            {
+                const bool mem = need && onList(P, D, D.slotTri[s], ti, tj, tk);
+                if (!waveAny(mem)) continue;
+                const dev::Tri tr = dev::loadTri(D.tri9, s);
+                if (!mem) continue;
+                const double sq = dev::sqnorm(dev::sub(p, dev::closestPoint(p, tr.a, tr.b, tr.c)));
+                if (sq < bestSq)
+                {
+                    bestSq = sq;
+                    const double r = sqrt(sq) + D.mu;
+                    bound = r * r;
+                }
+            }
+            continue;
+        }
+        const bool rf = need && dev::boxSq(p, D.nodes[nd.right]) < dev::boxSq(p, D.nodes[nd.left]);
+        const bool rightFirst = 2 * __popcll(__ballot(rf)) > __popcll(__ballot(need));
+        if (lane == sp) stk = rightFirst ? nd.left : nd.right;
+        if (lane == sp + 1) stk = rightFirst ? nd.right : nd.left;
+        sp += 2;
+    }
+    return bestSq;
+}
+
+// Distance^2 from p to a triangle's box (bmin3, bmax3), as boxSq.
+__device__ __forceinline__ double triBoxSq(D3 p, const double* __restrict__ b)
+{
+    double sq = 0.0;
+    const double pc[3] = {p.x, p.y, p.z};
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+    {
+        if (pc[a] < b[a])
+        {
+            const double d = b[a] - pc[a];
+            sq += d * d;
+        }
+        else if (pc[a] > b[3 + a])
+        {
+            const double d = pc[a] - b[3 + a];
+            sq += d * d;
+        }
+    }
+    return sq;
+}
+
+// The voxel's triangle list (ContactTileIndex::candidatesForVoxel,
+// SdfBuilder.cpp:102-113); every triangle without tiles.
+struct VoxelList
+{
+    const int32_t* list;
+    int64_t len;
+};
+
+__device__ __forceinline__ VoxelList tileList(const CAParams& P, const CAData& D, int64_t tile)
+{
+    if (!P.useTiles) return VoxelList{nullptr, P.ntris};
+    return VoxelList{D.tileTris + D.tileStart[tile], D.tileStart[tile + 1] - D.tileStart[tile]};
+}
+
+__device__ __forceinline__ VoxelList voxelList(const CAParams& P, const CAData& D, int ti, int tj,
+                                               int tk)
+{
+    return tileList(P, D, ti + int64_t(P.tnx) * (tj + int64_t(P.tny) * tk));
+}
+
+// Visits the list in order, skipping triangles whose box lies farther than
+// reach (= min distance + tol + mu): those cannot be candidates, so the merge
+// over the rest is the merge over the whole list.
+struct ListScan
+{
+    VoxelList L;
+    const double* tbox6;
+    D3 p;
+    double reach2;
+    template <class F>
+    __device__ __forceinline__ void operator()(F&& f) const
+    {
+        for (int64_t q = 0; q < L.len; ++q)
+        {
+            const int t = L.list ? L.list[q] : int(q);
+            if (!(triBoxSq(p, tbox6 + 6 * int64_t(t)) <= reach2)) continue;
+            if (!f(t)) return;
+        }
+    }
+};
+
+__device__ __forceinline__ double reachOf(double minDist, const CAParams& P, const CAData& D)
+{
+    return minDist + P.tol + D.mu;
+}
+
+constexpr int kScanBuf = 256; // LDS list entries per wave
+
+// The same visit for a wave whose lanes share one list (every lane calls it):
+// the wave tests 64 list entries at a time against the union of its lanes'
+// reach boxes, appends the survivors in list order to an LDS buffer, and each
+// lane then runs the per-lane test over the buffer.  Entries no lane can
+// reach are the only ones dropped, so every lane sees what ListScan gives it.
+struct WaveListScan
+{
+    VoxelList L;
+    const double* tbox6;
+    D3 p;
+    double reach2;       // this lane's (negative: takes nothing)
+    double lo[3], hi[3]; // union of the lanes' reach boxes
+    int* buf;            // kScanBuf entries of LDS for this wave
+    template <class F>
+    __device__ void operator()(F&& f) const
+    {
+        const int lane = threadIdx.x & 63;
+        const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+        bool stopped = false;
+        int n = 0;
+        for (int64_t base = 0; base < L.len; base += 64)
+        {
+            const int64_t q = base + lane;
+            int t = 0;
+            bool pass = false;
+            if (q < L.len)
+            {
+                t = L.list ? L.list[q] : int(q);
+                const double* b = tbox6 + 6 * int64_t(t);
+                pass = b[3] >= lo[0] && b[0] <= hi[0] && b[4] >= lo[1] && b[1] <= hi[1] &&
+                       b[5] >= lo[2] && b[2] <= hi[2];
+            }
+            const uint64_t m = __ballot(pass);
+            if (pass) buf[n + __popcll(m & below)] = t;
+            n += __popcll(m);
+            if (n > kScanBuf - 64 || base + 64 >= L.len)
+            {
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                for (int e = 0; e < n; ++e)
+                {
+                    const int tt = buf[e];
+                    if (stopped || !(triBoxSq(p, tbox6 + 6 * int64_t(tt)) <= reach2)) continue;
+                    if (!f(tt)) stopped = true;
+                }
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                n = 0;
+            }
+        }
+    }
+};
+
+__device__ __forceinline__ double waveMinD(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+    {
+        const double w = __shfl_xor(v, o);
+        v = w < v ? w : v;
+    }
+    return v;
+}
+__device__ __forceinline__ double waveMaxD(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+    {
+        const double w = __shfl_xor(v, o);
+        v = w > v ? w : v;
+    }
+    return v;
+}
+
+// Runs candidatesFor for one voxel per lane (every lane of the wave calls
+// it): the wave-shared scan when all active lanes share a tile list, the
+// per-lane scan otherwise.
+__device__ inline int waveCandidates(D3 c, bool active, int ti, int tj, int tk, double minDist,
+                                     const CAParams& P, const CAData& D, int* buf,
+                                     dev::LaneStack st, MergeView& m)
+{
+    const double r = reachOf(minDist, P, D);
+    const double r2 = active ? r * r : -1.0;
+    const int64_t tile = ti + int64_t(P.tnx) * (tj + int64_t(P.tny) * tk);
+    const uint64_t act = __ballot(active);
+    if (act == 0ull) return 0;
+    const int first = __ffsll((unsigned long long)act) - 1;
+    const int64_t t0 = __shfl(tile, first);
+    const bool shared = !P.useTiles || __ballot(active && tile != t0) == 0ull;
+    if (shared)
+    {
+        // every lane walks the active lanes' common list
+        WaveListScan each{tileList(P, D, t0), D.tbox6, c, r2, {0, 0, 0}, {0, 0, 0}, buf};
+        each.lo[0] = waveMinD(active ? c.x - r : DBL_MAX);
+        each.lo[1] = waveMinD(active ? c.y - r : DBL_MAX);
+        each.lo[2] = waveMinD(active ? c.z - r : DBL_MAX);
+        each.hi[0] = waveMaxD(active ? c.x + r : -DBL_MAX);
+        each.hi[1] = waveMaxD(active ? c.y + r : -DBL_MAX);
+        each.hi[2] = waveMaxD(active ? c.z + r : -DBL_MAX);
+        return candidatesFor(c, P, D.t21, each, true, minDist, active, D.nodes, D.tri9, D.slotTri,
+                             st, m);
+    }
+    const ListScan each{voxelList(P, D, ti, tj, tk), D.tbox6, c, r2};
+    return candidatesFor(c, P, D.t21, each, true, minDist, active, D.nodes, D.tri9, D.slotTri, st,
+                         m);
+}
+
 // Dense candidate fill (SdfBuilder.cpp:477-534): one lane per voxel, a 4x4x4
-// voxel brick per wave (all lanes of a wave share one tile's triangle list).
-// Voxels whose merged list outgrows the private arrays go to the overflow list.
+// voxel brick per wave.  Voxels whose merged list outgrows the private arrays
+// go to the overflow list.
 __global__ void __launch_bounds__(kBlock)
 denseCandidateKernel(GridSpec g, CAParams P, CAData D, int bx, int by, int bz, DenseOut o,
                      Overflow ov)
 {
     __shared__ int stack[kStackDepth * kBlock];
+    __shared__ int scanBuf[(kBlock / 64) * kScanBuf];
     const int lane = threadIdx.x & 63;
     const int64_t brick = int64_t(blockIdx.x) * (kBlock / 64) + (threadIdx.x >> 6);
-    if (brick >= int64_t(bx) * by * bz) return;
+    if (brick >= int64_t(bx) * by * bz) return; // whole waves
     const int i = int(brick % bx) * 4 + (lane & 3);
     const int j = int((brick / bx) % by) * 4 + ((lane >> 2) & 3);
     const int k = int(brick / (int64_t(bx) * by)) * 4 + (lane >> 4);
-    if (i >= g.n[0] || j >= g.n[1] || k >= g.n[2]) return;
+    const bool inGrid = i < g.n[0] && j < g.n[1] && k < g.n[2];
     const D3 c = voxelCentre(g, i, j, k);
-    const int32_t* list;
-    int len;
-    tileList(P, D, i, j, k, list, len);
+    int ti, tj, tk;
+    voxelTile(P, i, j, k, ti, tj, tk);
+    const double minSq = waveListMinSq(c, inGrid, ti, tj, tk, P, D);
+    const double minDist = minSq == DBL_MAX ? DBL_MAX : sqrt(minSq);
     LocalMerge lm;
     MergeView m = lm.view();
-    dev::LaneStack st{stack + threadIdx.x, kBlock};
-    const int cnt = candidatesForPoint(c, P, D.t21, list, len, D.nodes, D.tri9, D.slotTri, st, m);
+    const int cnt = waveCandidates(c, inGrid, ti, tj, tk, minDist, P, D,
+                                   scanBuf + (threadIdx.x >> 6) * kScanBuf,
+                                   dev::LaneStack{stack + threadIdx.x, kBlock}, m);
+    if (!inGrid) return;
     const int64_t idx = i + int64_t(g.n[0]) * (j + int64_t(g.n[1]) * k);
     if (cnt > m.cap)
     {
@@ -459,6 +788,20 @@ denseCandidateKernel(GridSpec g, CAParams P, CAData D, int bx, int by, int bz, D
         return;
     }
     writeDense(o, idx, c, m, P, D.t21);
+}
+
+// Minimum distance over a voxel's whole list by a plain scan (overflow pass).
+__device__ inline double listMinDist(D3 p, const VoxelList& L, const double* __restrict__ t21)
+{
+    double best = DBL_MAX;
+    for (int64_t q = 0; q < L.len; ++q)
+    {
+        const int t = L.list ? L.list[q] : int(q);
+        const CTri tr = loadCTri(t21, t);
+        const double d = sqrt(dev::sqnorm(dev::sub(p, dev::closestPoint(p, tr.a, tr.b, tr.c))));
+        best = d < best ? d : best;
+    }
+    return best;
 }
 
 // Overflow voxels of the dense fill, one lane each, merged lists in device memory.
@@ -473,13 +816,27 @@ denseOverflowKernel(GridSpec g, CAParams P, CAData D, const int64_t* __restrict_
     const int i = int(idx % g.n[0]), j = int((idx / g.n[0]) % g.n[1]),
               k = int(idx / (int64_t(g.n[0]) * g.n[1]));
     const D3 c = voxelCentre(g, i, j, k);
-    const int32_t* list;
-    int len;
-    tileList(P, D, i, j, k, list, len);
+    int ti, tj, tk;
+    voxelTile(P, i, j, k, ti, tj, tk);
+    const VoxelList L = voxelList(P, D, ti, tj, tk);
+    const double minDist = listMinDist(c, L, D.t21);
+    const double r = reachOf(minDist, P, D);
+    const ListScan each{L, D.tbox6, c, r * r};
     MergeView m = pool.view(q);
-    dev::LaneStack st{stack + threadIdx.x, kBlock};
-    candidatesForPoint(c, P, D.t21, list, len, D.nodes, D.tri9, D.slotTri, st, m);
+    candidatesFor(c, P, D.t21, each, true, minDist, true, D.nodes, D.tri9, D.slotTri,
+                  dev::LaneStack{stack + threadIdx.x, kBlock}, m);
     writeDense(o, idx, c, m, P, D.t21);
+}
+
+__device__ __forceinline__ void sparseVoxel(const SparseOut& o, int bd, int64_t t, int& i, int& j,
+                                            int& k)
+{
+    const int bvc = bd * bd * bd;
+    const int64_t b = t / bvc;
+    const int local = int(t % bvc);
+    i = o.bijk[3 * b] * bd + local % bd;
+    j = o.bijk[3 * b + 1] * bd + (local / bd) % bd;
+    k = o.bijk[3 * b + 2] * bd + local / (bd * bd);
 }
 
 // Sparse candidate fill (SdfBuilder.cpp:731-803): one lane per brick voxel.
@@ -488,41 +845,45 @@ sparseCandidateKernel(GridSpec g, CAParams P, CAData D, int bd, int64_t total, S
                       Overflow ov)
 {
     __shared__ int stack[kStackDepth * kBlock];
+    __shared__ int scanBuf[(kBlock / 64) * kScanBuf];
     const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (t >= total) return;
-    const int bvc = bd * bd * bd;
-    const int64_t b = t / bvc;
-    const int local = int(t % bvc);
-    const int i = o.bijk[3 * b] * bd + local % bd;
-    const int j = o.bijk[3 * b + 1] * bd + (local / bd) % bd;
-    const int k = o.bijk[3 * b + 2] * bd + local / (bd * bd);
-    // block allocation defaults (SdfBuilder.cpp:697-714)
-    for (int q = 0; q < P.stride; ++q)
+    if ((t & ~int64_t(63)) >= total) return; // whole waves only
+    const bool live = t < total;
+    int i = -1, j = -1, k = -1;
+    if (live)
     {
-        const int64_t e = t * P.stride + q;
-        o.scphi[e] = 0.0f;
-        o.scn3[3 * e] = 0.0f;
-        o.scn3[3 * e + 1] = 0.0f;
-        o.scn3[3 * e + 2] = 0.0f;
-        o.scbranch[e] = -1;
+        sparseVoxel(o, bd, t, i, j, k);
+        // block allocation defaults (SdfBuilder.cpp:697-714)
+        for (int q = 0; q < P.stride; ++q)
+        {
+            const int64_t e = t * P.stride + q;
+            o.scphi[e] = 0.0f;
+            o.scn3[3 * e] = 0.0f;
+            o.scn3[3 * e + 1] = 0.0f;
+            o.scn3[3 * e + 2] = 0.0f;
+            o.scbranch[e] = -1;
+        }
+        o.sccount[t] = 0;
     }
-    o.sccount[t] = 0;
-    if (i < 0 || i >= g.n[0] || j < 0 || j >= g.n[1] || k < 0 || k >= g.n[2])
+    const bool inGrid = live && i >= 0 && i < g.n[0] && j >= 0 && j < g.n[1] && k >= 0 && k < g.n[2];
+    if (live && !inGrid)
     {
         o.sphi[t] = 0.0f;
         o.sn3[3 * t] = 1.0f;
         o.sn3[3 * t + 1] = 0.0f;
         o.sn3[3 * t + 2] = 0.0f;
-        return;
     }
-    const int32_t* list;
-    int len;
-    tileList(P, D, i, j, k, list, len);
+    const D3 c = voxelCentre(g, i, j, k);
+    int ti, tj, tk;
+    voxelTile(P, i, j, k, ti, tj, tk);
+    const double minSq = waveListMinSq(c, inGrid, ti, tj, tk, P, D);
+    const double minDist = minSq == DBL_MAX ? DBL_MAX : sqrt(minSq);
     LocalMerge lm;
     MergeView m = lm.view();
-    dev::LaneStack st{stack + threadIdx.x, kBlock};
-    const int cnt = candidatesForPoint(voxelCentre(g, i, j, k), P, D.t21, list, len, D.nodes,
-                                       D.tri9, D.slotTri, st, m);
+    const int cnt = waveCandidates(c, inGrid, ti, tj, tk, minDist, P, D,
+                                   scanBuf + (threadIdx.x >> 6) * kScanBuf,
+                                   dev::LaneStack{stack + threadIdx.x, kBlock}, m);
+    if (!inGrid) return;
     if (cnt > m.cap)
     {
         ov.ids[atomicAdd(ov.count, 1)] = t;
@@ -539,20 +900,163 @@ sparseOverflowKernel(GridSpec g, CAParams P, CAData D, int bd, const int64_t* __
     const int q = int(blockIdx.x) * kBlock + threadIdx.x;
     if (q >= n) return;
     const int64_t t = ids[q];
-    const int bvc = bd * bd * bd;
-    const int64_t b = t / bvc;
-    const int local = int(t % bvc);
-    const int i = o.bijk[3 * b] * bd + local % bd;
-    const int j = o.bijk[3 * b + 1] * bd + (local / bd) % bd;
-    const int k = o.bijk[3 * b + 2] * bd + local / (bd * bd);
-    const int32_t* list;
-    int len;
-    tileList(P, D, i, j, k, list, len);
+    int i, j, k;
+    sparseVoxel(o, bd, t, i, j, k);
+    const D3 c = voxelCentre(g, i, j, k);
+    int ti, tj, tk;
+    voxelTile(P, i, j, k, ti, tj, tk);
+    const VoxelList L = voxelList(P, D, ti, tj, tk);
+    const double minDist = listMinDist(c, L, D.t21);
+    const double r = reachOf(minDist, P, D);
+    const ListScan each{L, D.tbox6, c, r * r};
     MergeView m = pool.view(q);
-    dev::LaneStack st{stack + threadIdx.x, kBlock};
-    candidatesForPoint(voxelCentre(g, i, j, k), P, D.t21, list, len, D.nodes, D.tri9, D.slotTri, st,
-                       m);
+    candidatesFor(c, P, D.t21, each, true, minDist, true, D.nodes, D.tri9, D.slotTri,
+                  dev::LaneStack{stack + threadIdx.x, kBlock}, m);
     writeSparse(o, t, m, P);
+}
+
+// Tile lists in triangle order (buildContactTileIndex, SdfBuilder.cpp:202-258),
+// built on the device: per-tile counts, an exclusive scan, an unordered fill,
+// then each list sorted.
+__global__ void __launch_bounds__(256)
+tileCountKernel(const int32_t* __restrict__ triTile, int64_t nt, int tnx, int tny,
+                int32_t* __restrict__ count)
+{
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    const int32_t* r = triTile + 6 * t;
+    for (int tk = r[2]; tk <= r[5]; ++tk)
+        for (int tj = r[1]; tj <= r[4]; ++tj)
+            for (int ti = r[0]; ti <= r[3]; ++ti)
+                atomicAdd(&count[ti + int64_t(tnx) * (tj + int64_t(tny) * tk)], 1);
+}
+
+// One block: start[q] = sum of count[0..q), start[n] = the total; *maxCount = max count.
+constexpr int kScanBlock = 1024;
+__global__ void __launch_bounds__(kScanBlock)
+tileScanKernel(const int32_t* __restrict__ count, int64_t n, int64_t* __restrict__ start,
+               int32_t* __restrict__ maxCount)
+{
+    __shared__ int64_t part[kScanBlock];
+    __shared__ int32_t mx[kScanBlock];
+    const int tid = threadIdx.x;
+    const int64_t chunk = (n + kScanBlock - 1) / kScanBlock;
+    const int64_t b = tid * chunk, e = b + chunk < n ? b + chunk : n;
+    int64_t s = 0;
+    int32_t m = 0;
+    for (int64_t q = b; q < e; ++q)
+    {
+        s += count[q];
+        m = count[q] > m ? count[q] : m;
+    }
+    part[tid] = s;
+    mx[tid] = m;
+    __syncthreads();
+    for (int off = 1; off < kScanBlock; off <<= 1)
+    {
+        const int64_t v = tid >= off ? part[tid - off] : 0;
+        const int32_t w = tid >= off ? mx[tid - off] : 0;
+        __syncthreads();
+        part[tid] += v;
+        mx[tid] = w > mx[tid] ? w : mx[tid];
+        __syncthreads();
+    }
+    int64_t run = part[tid] - s; // exclusive prefix of this chunk
+    for (int64_t q = b; q < e; ++q)
+    {
+        start[q] = run;
+        run += count[q];
+    }
+    if (tid == kScanBlock - 1)
+    {
+        start[n] = part[tid];
+        *maxCount = mx[tid];
+    }
+}
+
+__global__ void __launch_bounds__(256)
+tileFillKernel(const int32_t* __restrict__ triTile, int64_t nt, int tnx, int tny,
+               const int64_t* __restrict__ start, int32_t* __restrict__ cursor,
+               int32_t* __restrict__ tris)
+{
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    const int32_t* r = triTile + 6 * t;
+    for (int tk = r[2]; tk <= r[5]; ++tk)
+        for (int tj = r[1]; tj <= r[4]; ++tj)
+            for (int ti = r[0]; ti <= r[3]; ++ti)
+            {
+                const int64_t tile = ti + int64_t(tnx) * (tj + int64_t(tny) * tk);
+                tris[start[tile] + atomicAdd(&cursor[tile], 1)] = int32_t(t);
+            }
+}
+
+// Sorts every tile's list ascending: a bitonic sort in LDS for lists of up
+// to kSortMax entries, a heap sort by one lane for longer ones.
+constexpr int kSortMax = 4096;
+__global__ void __launch_bounds__(256)
+tileSortKernel(const int64_t* __restrict__ start, int64_t ntiles, int32_t* __restrict__ tris)
+{
+    __shared__ int32_t v[kSortMax];
+    for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x)
+    {
+        const int64_t b = start[tile];
+        const int len = int(start[tile + 1] - b);
+        if (len <= 1) continue;
+        int32_t* a = tris + b;
+        if (len > kSortMax)
+        {
+            if (threadIdx.x == 0) sortIds(a, len);
+            __syncthreads();
+            continue;
+        }
+        int n2 = 1;
+        while (n2 < len) n2 <<= 1;
+        for (int q = threadIdx.x; q < n2; q += blockDim.x) v[q] = q < len ? a[q] : INT32_MAX;
+        __syncthreads();
+        for (int k = 2; k <= n2; k <<= 1)
+            for (int j = k >> 1; j > 0; j >>= 1)
+            {
+                for (int q = threadIdx.x; q < n2; q += blockDim.x)
+                {
+                    const int r = q ^ j;
+                    if (r > q)
+                    {
+                        const bool up = (q & k) == 0;
+                        const int32_t x = v[q], y = v[r];
+                        if ((x > y) == up)
+                        {
+                            v[q] = y;
+                            v[r] = x;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        for (int q = threadIdx.x; q < len; q += blockDim.x) a[q] = v[q];
+        __syncthreads();
+    }
+}
+
+// Sparse activation with tiles (SdfBuilder.cpp:638-664): a brick is active
+// when any tile over its voxel range has a non-empty list.
+__global__ void __launch_bounds__(256)
+sparseTileActivateKernel(GridSpec g, int bd, int nbx, int nby, int nbz, int tileDim, int tnx,
+                         int tny, const int32_t* __restrict__ count, uint8_t* __restrict__ mask)
+{
+    const int64_t bl = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (bl >= int64_t(nbx) * nby * nbz) return;
+    const int bi = int(bl % nbx), bj = int((bl / nbx) % nby), bk = int(bl / (int64_t(nbx) * nby));
+    const int i0 = bi * bd, j0 = bj * bd, k0 = bk * bd;
+    const int i1 = min(i0 + bd - 1, g.n[0] - 1);
+    const int j1 = min(j0 + bd - 1, g.n[1] - 1);
+    const int k1 = min(k0 + bd - 1, g.n[2] - 1);
+    bool act = false;
+    for (int tk = k0 / tileDim; tk <= k1 / tileDim && !act; ++tk)
+        for (int tj = j0 / tileDim; tj <= j1 / tileDim && !act; ++tj)
+            for (int ti = i0 / tileDim; ti <= i1 / tileDim && !act; ++ti)
+                act = count[ti + int64_t(tnx) * (tj + int64_t(tny) * tk)] != 0;
+    mask[bl] = act ? 1 : 0;
 }
 
 // Sparse activation without tiles: unsigned nearest distance at the brick
@@ -628,55 +1132,76 @@ HostCA makeTriangleInfos(const double* verts, int64_t nv, const int32_t* tris, i
 struct HostTiles
 {
     int tileDim = 1, nx = 1, ny = 1, nz = 1;
-    std::vector<int32_t> start, tris;
-    bool empty(int64_t tile) const { return start[tile + 1] == start[tile]; }
+    std::vector<int32_t> triTile;  // 6 per triangle: tile range (lo xyz, hi xyz)
+    std::vector<int32_t> nodeTile; // 6 per BVH node: union of its triangles' ranges
 };
 
-// buildContactTileIndex (SdfBuilder.cpp:202-258), as CSR lists in triangle order.
-HostTiles buildTiles(const GridSpec& g, const HostCA& ca, int64_t nt, int reqTileDim, double infl)
+// buildContactTileIndex (SdfBuilder.cpp:202-258) without the lists: each
+// triangle's tile range (a tile's list is its triangles in triangle order), and
+// per BVH node the union of the ranges below it, for the device search.
+HostTiles buildTiles(const GridSpec& g, const HostCA& ca, int64_t nt, int reqTileDim, double infl,
+                     const HostBvh& hb)
 {
     HostTiles T;
     T.tileDim = std::max(1, reqTileDim);
     T.nx = (g.n[0] + T.tileDim - 1) / T.tileDim;
     T.ny = (g.n[1] + T.tileDim - 1) / T.tileDim;
     T.nz = (g.n[2] + T.tileDim - 1) / T.tileDim;
-    const int64_t ntiles = int64_t(T.nx) * T.ny * T.nz;
     const double h = g.hmax();
     const double tileRadius = 0.5 * std::sqrt(3.0) * double(T.tileDim) * h;
     const double expansion = (0.0 < infl ? infl : 0.0) + tileRadius + h;
-    std::vector<int> ranges(size_t(nt) * 6);
-    std::vector<int32_t> count(size_t(ntiles) + 1, 0);
     auto clampi = [](int v, int hi) { return std::max(0, std::min(hi, v)); };
+    T.triTile.resize(size_t(nt) * 6);
     for (int64_t t = 0; t < nt; ++t)
-    {
-        int r[6];
         for (int a = 0; a < 3; ++a)
         {
             const double lo = ca.bmin[3 * size_t(t) + a] - expansion;
             const double hi = ca.bmax[3 * size_t(t) + a] + expansion;
             const int v0 = int(std::floor((lo - g.bmin[a]) / g.voxel[a] - 0.5));
             const int v1 = int(std::ceil((hi - g.bmin[a]) / g.voxel[a] - 0.5));
-            r[a] = clampi(v0, g.n[a] - 1) / T.tileDim;
-            r[3 + a] = clampi(v1, g.n[a] - 1) / T.tileDim;
+            T.triTile[6 * size_t(t) + a] = clampi(v0, g.n[a] - 1) / T.tileDim;
+            T.triTile[6 * size_t(t) + 3 + a] = clampi(v1, g.n[a] - 1) / T.tileDim;
         }
-        for (int q = 0; q < 6; ++q) ranges[6 * size_t(t) + q] = r[q];
-        for (int tk = r[2]; tk <= r[5]; ++tk)
-            for (int tj = r[1]; tj <= r[4]; ++tj)
-                for (int ti = r[0]; ti <= r[3]; ++ti) ++count[ti + int64_t(T.nx) * (tj + int64_t(T.ny) * tk)];
-    }
-    T.start.assign(size_t(ntiles) + 1, 0);
-    for (int64_t q = 0; q < ntiles; ++q) T.start[q + 1] = T.start[q] + count[q];
-    T.tris.resize(size_t(T.start[ntiles]));
-    std::vector<int32_t> fill(T.start.begin(), T.start.end() - 1);
-    for (int64_t t = 0; t < nt; ++t)
+    // nodes are in pre-order: children after their parent
+    const size_t nn = hb.nodes.size();
+    T.nodeTile.assign(nn * 6, 0);
+    for (size_t q = nn; q-- > 0;)
     {
-        const int* r = &ranges[6 * size_t(t)];
-        for (int tk = r[2]; tk <= r[5]; ++tk)
-            for (int tj = r[1]; tj <= r[4]; ++tj)
-                for (int ti = r[0]; ti <= r[3]; ++ti)
-                    T.tris[fill[ti + int64_t(T.nx) * (tj + int64_t(T.ny) * tk)]++] = int32_t(t);
+        const BvhNode& nd = hb.nodes[q];
+        int32_t* u = &T.nodeTile[6 * q];
+        for (int a = 0; a < 3; ++a)
+        {
+            u[a] = INT32_MAX;
+            u[3 + a] = -1;
+        }
+        auto merge = [&](const int32_t* r) {
+            for (int a = 0; a < 3; ++a)
+            {
+                u[a] = std::min(u[a], r[a]);
+                u[3 + a] = std::max(u[3 + a], r[3 + a]);
+            }
+        };
+        if (nd.triCount > 0)
+            for (int sl = nd.triStart; sl < nd.triStart + nd.triCount; ++sl)
+                merge(&T.triTile[6 * size_t(hb.slotTri[size_t(sl)])]);
+        else
+        {
+            merge(&T.nodeTile[6 * size_t(nd.left)]);
+            merge(&T.nodeTile[6 * size_t(nd.right)]);
+        }
     }
     return T;
+}
+
+// Search margin of the device search: 2^-30 of the largest coordinate
+// magnitude of the mesh and the grid, far above the rounding of box and
+// triangle distances at that scale.
+double searchMargin(const double* verts, int64_t nv, const GridSpec& g)
+{
+    double m = 0.0;
+    for (int64_t q = 0; q < 3 * nv; ++q) m = std::max(m, std::fabs(verts[q]));
+    for (int a = 0; a < 3; ++a) m = std::max({m, std::fabs(g.bmin[a]), std::fabs(g.bmax[a])});
+    return std::ldexp(m, -30);
 }
 
 struct DevBuf
@@ -741,7 +1266,7 @@ sdfo_grid* buildContactAware(const double* verts, int64_t nv, const int32_t* tri
     }
     const double infl = sparse ? std::max(activation, tol + 2.0 * h) : tol + 2.0 * h;
     HostTiles tiles;
-    if (p.use_tile_acceleration) tiles = buildTiles(G, ca, nt, p.tile_dim, infl);
+    if (p.use_tile_acceleration) tiles = buildTiles(G, ca, nt, p.tile_dim, infl, hb);
 
     DevBuf buf;
     DeviceBvh db;
@@ -756,18 +1281,11 @@ sdfo_grid* buildContactAware(const double* verts, int64_t nv, const int32_t* tri
     P.tny = tiles.ny;
     P.tnz = tiles.nz;
     P.ntris = int(nt);
-    // The merged list of a voxel holds at most one entry per triangle it tests
-    // (its tile list, or every triangle without tiles), plus the nearest-triangle
-    // fallback when that list yields nothing.
-    int maxList = int(nt);
-    if (P.useTiles)
-    {
-        maxList = 0;
-        for (size_t q = 0; q + 1 < tiles.start.size(); ++q)
-            maxList = std::max(maxList, tiles.start[q + 1] - tiles.start[q]);
-    }
-    const int poolCap = std::max(maxList, 1);
     Overflow ov{nullptr, nullptr};
+    // The merged list of a voxel holds at most one entry per triangle on its
+    // list (the longest tile list, or every triangle without tiles), plus the
+    // nearest-triangle fallback when that list is empty.
+    int poolCap = int(nt);
     // Re-runs the voxels the first pass listed, with merged lists in device
     // memory, in batches of at most ~1 GiB of list storage.
     auto resolveOverflow = [&](auto launch) {
@@ -800,14 +1318,55 @@ sdfo_grid* buildContactAware(const double* verts, int64_t nv, const int32_t* tri
         keepStreamPoolMemory(g->device);
         db.upload(hb, s);
         D.t21 = buf.up(ca.t21);
+        {
+            std::vector<double> tb(size_t(nt) * 6);
+            for (int64_t t = 0; t < nt; ++t)
+                for (int a = 0; a < 3; ++a)
+                {
+                    tb[6 * size_t(t) + a] = ca.bmin[3 * size_t(t) + a];
+                    tb[6 * size_t(t) + 3 + a] = ca.bmax[3 * size_t(t) + a];
+                }
+            D.tbox6 = buf.up(tb);
+        }
+        int32_t* tileCount = nullptr;
         if (P.useTiles)
         {
-            D.tileStart = buf.up(tiles.start);
-            D.tileTris = buf.up(tiles.tris);
+            D.triTile = buf.up(tiles.triTile);
+            D.nodeTile = buf.up(tiles.nodeTile);
+            // tile lists on the device (count, scan, fill, sort)
+            const int64_t ntiles = int64_t(tiles.nx) * tiles.ny * tiles.nz;
+            tileCount = static_cast<int32_t*>(buf.raw(size_t(ntiles) * sizeof(int32_t)));
+            int32_t* cursor = static_cast<int32_t*>(buf.raw(size_t(ntiles) * sizeof(int32_t)));
+            int64_t* start = static_cast<int64_t*>(buf.raw(size_t(ntiles + 1) * sizeof(int64_t)));
+            int32_t* maxCount = static_cast<int32_t*>(buf.raw(sizeof(int32_t)));
+            SDFO_HIP(hipMemsetAsync(tileCount, 0, size_t(ntiles) * sizeof(int32_t), s));
+            SDFO_HIP(hipMemsetAsync(cursor, 0, size_t(ntiles) * sizeof(int32_t), s));
+            hipLaunchKernelGGL(tileCountKernel, dim3(blocksFor(nt, 256)), dim3(256), 0, s, D.triTile,
+                               nt, tiles.nx, tiles.ny, tileCount);
+            SDFO_HIP(hipGetLastError());
+            hipLaunchKernelGGL(tileScanKernel, dim3(1), dim3(kScanBlock), 0, s, tileCount, ntiles,
+                               start, maxCount);
+            SDFO_HIP(hipGetLastError());
+            int64_t total = 0;
+            int32_t longest = 0;
+            SDFO_HIP(hipMemcpyAsync(&total, start + ntiles, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+            SDFO_HIP(hipMemcpyAsync(&longest, maxCount, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+            SDFO_HIP(hipStreamSynchronize(s));
+            int32_t* tris = static_cast<int32_t*>(buf.raw(size_t(std::max<int64_t>(total, 1)) * sizeof(int32_t)));
+            hipLaunchKernelGGL(tileFillKernel, dim3(blocksFor(nt, 256)), dim3(256), 0, s, D.triTile,
+                               nt, tiles.nx, tiles.ny, start, cursor, tris);
+            SDFO_HIP(hipGetLastError());
+            hipLaunchKernelGGL(tileSortKernel, dim3(unsigned(std::min<int64_t>(ntiles, 65536))),
+                               dim3(256), 0, s, start, ntiles, tris);
+            SDFO_HIP(hipGetLastError());
+            D.tileStart = start;
+            D.tileTris = tris;
+            poolCap = std::max(int(longest), 1);
         }
         D.nodes = db.nodes;
         D.tri9 = db.tri9;
         D.slotTri = db.triIdx;
+        D.mu = searchMargin(verts, nv, G);
         ov.count = buf.up(std::vector<int>{0});
         if (!sparse)
         {
@@ -853,23 +1412,12 @@ sdfo_grid* buildContactAware(const double* verts, int64_t nv, const int32_t* tri
             g->activeMask = g->alloc<uint8_t>(size_t(cells));
             if (P.useTiles)
             {
-                // SdfBuilder.cpp:638-664: any non-empty tile over the brick's voxel range
-                for (int64_t bl = 0; bl < cells; ++bl)
-                {
-                    const int bi = int(bl % g->nbx), bj = int((bl / g->nbx) % g->nby),
-                              bk = int(bl / (int64_t(g->nbx) * g->nby));
-                    const int i0 = bi * bd, j0 = bj * bd, k0 = bk * bd;
-                    const int i1 = std::min(i0 + bd - 1, G.n[0] - 1);
-                    const int j1 = std::min(j0 + bd - 1, G.n[1] - 1);
-                    const int k1 = std::min(k0 + bd - 1, G.n[2] - 1);
-                    bool act = false;
-                    for (int tk = k0 / tiles.tileDim; tk <= k1 / tiles.tileDim && !act; ++tk)
-                        for (int tj = j0 / tiles.tileDim; tj <= j1 / tiles.tileDim && !act; ++tj)
-                            for (int ti = i0 / tiles.tileDim; ti <= i1 / tiles.tileDim && !act; ++ti)
-                                act = !tiles.empty(ti + int64_t(tiles.nx) * (tj + int64_t(tiles.ny) * tk));
-                    mask[size_t(bl)] = act ? 1 : 0;
-                }
-                SDFO_HIP(hipMemcpy(g->activeMask, mask.data(), mask.size(), hipMemcpyHostToDevice));
+                hipLaunchKernelGGL(sparseTileActivateKernel, dim3(blocksFor(cells, 256)), dim3(256),
+                                   0, s, G, bd, g->nbx, g->nby, g->nbz, tiles.tileDim, tiles.nx,
+                                   tiles.ny, tileCount, g->activeMask);
+                SDFO_HIP(hipGetLastError());
+                SDFO_HIP(hipMemcpyAsync(mask.data(), g->activeMask, mask.size(), hipMemcpyDeviceToHost, s));
+                SDFO_HIP(hipStreamSynchronize(s));
             }
             else
             {

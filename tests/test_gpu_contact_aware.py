@@ -34,6 +34,10 @@ CASES = [
      dict(use_tile_acceleration=0, max_candidates=3)),
     ("sphere_tol_16", lambda: meshes.uv_sphere(0.05, 8, 16), vertex_corner_normals, 16,
      dict(feature_distance_tolerance=0.004, normal_merge_angle_degrees=20.0, tile_dim=4)),
+    # a wide tolerance puts more than 32 triangles within reach of many voxels:
+    # those take the second, whole-list pass
+    ("torus_wide_tol_16", lambda: meshes.torus(0.05, 0.01, 48, 16), vertex_corner_normals, 16,
+     dict(feature_distance_tolerance=0.02, normal_merge_angle_degrees=2.0, max_candidates=8)),
 ]
 
 
@@ -87,6 +91,9 @@ SPARSE = [
      dict(band_voxels=2, halo_voxels=1)),
     ("cube_no_tiles_32", lambda: meshes.cube(0.1), face_corner_normals, 32,
      dict(band_voxels=2, use_tile_acceleration=0)),
+    ("torus_wide_tol_32", lambda: meshes.torus(0.05, 0.01, 48, 16), vertex_corner_normals, 32,
+     dict(band_voxels=2, feature_distance_tolerance=0.02, normal_merge_angle_degrees=2.0,
+          max_candidates=8)),
 ]
 
 
