@@ -20,6 +20,8 @@
 #include <cmath>
 #include <memory>
 #include <numeric>
+#include <exception>
+#include <thread>
 #include <vector>
 
 #include "absphi_sort.hpp"
@@ -346,6 +348,7 @@ struct CAData
     const int32_t* nodeTile;  // with tiles: 6 per BVH node, the union of its triangles' ranges
     const BvhNode* nodes;
     const double* tri9;
+    const double* slotBox6; // 6 per BVH slot: the slot triangle's box
     const int32_t* slotTri;
     double mu; // search margin, far above the rounding of box and triangle distances
 };
@@ -525,6 +528,28 @@ __device__ __forceinline__ D3 voxelCentre(const GridSpec& g, int i, int j, int k
                    g.bmin[2] + g.voxel[2] * (k + 0.5));
 }
 
+// Distance^2 from p to a triangle's box (bmin3, bmax3), as boxSq.
+__device__ __forceinline__ double triBoxSq(D3 p, const double* __restrict__ b)
+{
+    double sq = 0.0;
+    const double pc[3] = {p.x, p.y, p.z};
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+    {
+        if (pc[a] < b[a])
+        {
+            const double d = b[a] - pc[a];
+            sq += d * d;
+        }
+        else if (pc[a] > b[3 + a])
+        {
+            const double d = pc[a] - b[3 + a];
+            sq += d * d;
+        }
+    }
+    return sq;
+}
+
 __device__ __forceinline__ bool waveAny(bool v) { return __ballot(v) != 0ull; }
 
 // Smallest squared distance from p to a triangle on the lane's list
@@ -554,7 +579,9 @@ __device__ double waveListMinSq(D3 p, bool active, int ti, int tj, int tk, const
         {
             for (int s = nd.triStart; s < nd.triStart + nd.triCount; ++s)
             {
-                const bool mem = need && onList(P, D, D.slotTri[s], ti, tj, tk);
+                // the triangle's own box bounds its distance like a node's
+                const bool mem = need && triBoxSq(p, D.slotBox6 + 6 * int64_t(s)) <= bound &&
+                                 onList(P, D, D.slotTri[s], ti, tj, tk);
                 if (!waveAny(mem)) continue;
                 const dev::Tri tr = dev::loadTri(D.tri9, s);
                 if (!mem) continue;
@@ -575,28 +602,6 @@ __device__ double waveListMinSq(D3 p, bool active, int ti, int tj, int tk, const
         sp += 2;
     }
     return bestSq;
-}
-
-// Distance^2 from p to a triangle's box (bmin3, bmax3), as boxSq.
-__device__ __forceinline__ double triBoxSq(D3 p, const double* __restrict__ b)
-{
-    double sq = 0.0;
-    const double pc[3] = {p.x, p.y, p.z};
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-    {
-        if (pc[a] < b[a])
-        {
-            const double d = b[a] - pc[a];
-            sq += d * d;
-        }
-        else if (pc[a] > b[3 + a])
-        {
-            const double d = pc[a] - b[3 + a];
-            sq += d * d;
-        }
-    }
-    return sq;
 }
 
 // The voxel's triangle list (ContactTileIndex::candidatesForVoxel,
@@ -1137,10 +1142,8 @@ struct HostTiles
 };
 
 // buildContactTileIndex (SdfBuilder.cpp:202-258) without the lists: each
-// triangle's tile range (a tile's list is its triangles in triangle order), and
-// per BVH node the union of the ranges below it, for the device search.
-HostTiles buildTiles(const GridSpec& g, const HostCA& ca, int64_t nt, int reqTileDim, double infl,
-                     const HostBvh& hb)
+// triangle's tile range (a tile's list is its triangles in triangle order).
+HostTiles buildTiles(const GridSpec& g, const HostCA& ca, int64_t nt, int reqTileDim, double infl)
 {
     HostTiles T;
     T.tileDim = std::max(1, reqTileDim);
@@ -1162,6 +1165,12 @@ HostTiles buildTiles(const GridSpec& g, const HostCA& ca, int64_t nt, int reqTil
             T.triTile[6 * size_t(t) + a] = clampi(v0, g.n[a] - 1) / T.tileDim;
             T.triTile[6 * size_t(t) + 3 + a] = clampi(v1, g.n[a] - 1) / T.tileDim;
         }
+    return T;
+}
+
+// Per BVH node, the union of the tile ranges of the triangles below it.
+void nodeTileUnion(HostTiles& T, const HostBvh& hb)
+{
     // nodes are in pre-order: children after their parent
     const size_t nn = hb.nodes.size();
     T.nodeTile.assign(nn * 6, 0);
@@ -1190,7 +1199,6 @@ HostTiles buildTiles(const GridSpec& g, const HostCA& ca, int64_t nt, int reqTil
             merge(&T.nodeTile[6 * size_t(nd.right)]);
         }
     }
-    return T;
 }
 
 // Search margin of the device search: 2^-30 of the largest coordinate
@@ -1256,8 +1264,29 @@ sdfo_grid* buildContactAware(const double* verts, int64_t nv, const int32_t* tri
     const double h = G.hmax();
     const double tol = p.feature_distance_tolerance > 0.0 ? p.feature_distance_tolerance : 1.5 * h;
     const double cosMerge = std::cos(p.normal_merge_angle_degrees * std::acos(-1.0) / 180.0);
-    const HostCA ca = makeTriangleInfos(verts, nv, tris, nt, cornerN);
-    const HostBvh hb = buildHostBvh(verts, nv, tris, nt);
+    const HostCA ca = makeTriangleInfos(verts, nv, tris, nt, cornerN); // validates indices
+    // The triangle BVH is built on a host thread while the tile lists, the
+    // activation and the allocations proceed; ensureBvh() joins it.
+    HostBvh hb;
+    std::exception_ptr bvhError;
+    std::thread bvhThread([&] {
+        try
+        {
+            hb = buildHostBvh(verts, nv, tris, nt);
+        }
+        catch (...)
+        {
+            bvhError = std::current_exception();
+        }
+    });
+    struct Joiner
+    {
+        std::thread& t;
+        ~Joiner()
+        {
+            if (t.joinable()) t.join();
+        }
+    } joiner{bvhThread};
     double band = 0.0, activation = 0.0;
     if (sparse)
     {
@@ -1266,7 +1295,7 @@ sdfo_grid* buildContactAware(const double* verts, int64_t nv, const int32_t* tri
     }
     const double infl = sparse ? std::max(activation, tol + 2.0 * h) : tol + 2.0 * h;
     HostTiles tiles;
-    if (p.use_tile_acceleration) tiles = buildTiles(G, ca, nt, p.tile_dim, infl, hb);
+    if (p.use_tile_acceleration) tiles = buildTiles(G, ca, nt, p.tile_dim, infl);
 
     DevBuf buf;
     DeviceBvh db;
@@ -1313,10 +1342,26 @@ sdfo_grid* buildContactAware(const double* verts, int64_t nv, const int32_t* tri
         }
         SDFO_HIP(hipStreamSynchronize(s));
     };
+    bool bvhReady = false;
+    auto ensureBvh = [&] {
+        if (bvhReady) return;
+        bvhThread.join();
+        if (bvhError) std::rethrow_exception(bvhError);
+        db.upload(hb, s);
+        D.nodes = db.nodes;
+        D.tri9 = db.tri9;
+        D.slotBox6 = db.box6;
+        D.slotTri = db.triIdx;
+        if (P.useTiles)
+        {
+            nodeTileUnion(tiles, hb);
+            D.nodeTile = buf.up(tiles.nodeTile);
+        }
+        bvhReady = true;
+    };
     try
     {
         keepStreamPoolMemory(g->device);
-        db.upload(hb, s);
         D.t21 = buf.up(ca.t21);
         {
             std::vector<double> tb(size_t(nt) * 6);
@@ -1332,7 +1377,6 @@ sdfo_grid* buildContactAware(const double* verts, int64_t nv, const int32_t* tri
         if (P.useTiles)
         {
             D.triTile = buf.up(tiles.triTile);
-            D.nodeTile = buf.up(tiles.nodeTile);
             // tile lists on the device (count, scan, fill, sort)
             const int64_t ntiles = int64_t(tiles.nx) * tiles.ny * tiles.nz;
             tileCount = static_cast<int32_t*>(buf.raw(size_t(ntiles) * sizeof(int32_t)));
@@ -1363,9 +1407,6 @@ sdfo_grid* buildContactAware(const double* verts, int64_t nv, const int32_t* tri
             D.tileTris = tris;
             poolCap = std::max(int(longest), 1);
         }
-        D.nodes = db.nodes;
-        D.tri9 = db.tri9;
-        D.slotTri = db.triIdx;
         D.mu = searchMargin(verts, nv, G);
         ov.count = buf.up(std::vector<int>{0});
         if (!sparse)
@@ -1382,6 +1423,7 @@ sdfo_grid* buildContactAware(const double* verts, int64_t nv, const int32_t* tri
             const int bx = (G.n[0] + 3) / 4, by = (G.n[1] + 3) / 4, bz = (G.n[2] + 3) / 4;
             const DenseOut o{g->phi, g->n3, g->witness3, g->ccount, g->cphi, g->cn3, g->cbranch};
             ov.ids = static_cast<int64_t*>(buf.raw(size_t(N) * sizeof(int64_t)));
+            ensureBvh();
             hipLaunchKernelGGL(denseCandidateKernel, dim3(blocksFor(int64_t(bx) * by * bz, kBlock / 64)),
                                dim3(kBlock), 0, s, G, P, D, bx, by, bz, o, ov);
             SDFO_HIP(hipGetLastError());
@@ -1421,6 +1463,7 @@ sdfo_grid* buildContactAware(const double* verts, int64_t nv, const int32_t* tri
             }
             else
             {
+                ensureBvh();
                 const double br = 0.5 * std::sqrt(3.0) * double(bd) * h;
                 hipLaunchKernelGGL(sparseCAActivateKernel, dim3(blocksFor(cells, kBlock)), dim3(kBlock),
                                    0, s, G, db.nodes, db.tri9, bd, g->nbx, g->nby, g->nbz,
@@ -1452,6 +1495,7 @@ sdfo_grid* buildContactAware(const double* verts, int64_t nv, const int32_t* tri
                 g->scbranch = g->alloc<int32_t>(size_t(total) * stride);
                 const SparseOut o{g->bijk, g->sphi, g->sn3, g->sccount, g->scphi, g->scn3, g->scbranch};
                 ov.ids = static_cast<int64_t*>(buf.raw(size_t(total) * sizeof(int64_t)));
+                ensureBvh();
                 hipLaunchKernelGGL(sparseCandidateKernel, dim3(blocksFor(total, kBlock)), dim3(kBlock), 0,
                                    s, G, P, D, bd, total, o, ov);
                 SDFO_HIP(hipGetLastError());
@@ -1468,6 +1512,12 @@ sdfo_grid* buildContactAware(const double* verts, int64_t nv, const int32_t* tri
                 launchSparseHessian(G, bd, g->nb, g->nbx, g->nby, g->nbz, g->bijk, g->table, g->sn3,
                                     g->sh6, s);
             }
+        }
+        if (!bvhReady)
+        {
+            // no voxel needed the BVH; its errors still stand
+            bvhThread.join();
+            if (bvhError) std::rethrow_exception(bvhError);
         }
         SDFO_HIP(hipStreamSynchronize(s));
     }
