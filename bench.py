@@ -140,12 +140,13 @@ def bench_sparse_cylinder(args):
     g = sdfo.SdfGrid.build_sparse(v, t, **prm)  # warm-up
     g.close()
     K = max(1, args.steps)
-    t0 = time.perf_counter()
-    for _ in range(K):
+    build_s = 0.0
+    for k in range(K):
+        t0 = time.perf_counter()
         g = sdfo.SdfGrid.build_sparse(v, t, **prm)
-        if _ < K - 1:
-            g.close()
-    build_s = (time.perf_counter() - t0) / K
+        build_s += (time.perf_counter() - t0) / K
+        if k < K - 1:
+            g.close()  # outside the timed build
     info = g.info
     h = g.spec.voxel.max()
     pts_np = scenes.near_surface_points(v, t, args.query_points, 2.0 * h, seed=3)
@@ -167,6 +168,7 @@ def bench_sparse_cylinder(args):
         qs.append(a.elapsed_time(b))
     q_ms = float(np.mean(qs))
     frac_valid = float(valid.float().mean().item())
+    sdfo.SdfGrid.build_contact_aware(v, t, cn, sparse=True, **prm).close()  # warm-up
     t0 = time.perf_counter()
     gca = sdfo.SdfGrid.build_contact_aware(v, t, cn, sparse=True, **prm)
     ca_build_s = time.perf_counter() - t0
@@ -231,7 +233,7 @@ def bench_batched_gears(args):
     import torch
 
     import sdfo
-    from sdfo import scenes
+    from sdfo import _lib, scenes
 
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream()
@@ -242,13 +244,15 @@ def bench_batched_gears(args):
     for g in grids:
         g.close()
     K = max(1, args.steps)
-    t0 = time.perf_counter()
+    build_s = 0.0
     for k in range(K):
+        t0 = time.perf_counter()
         grids = sdfo.SdfGrid.build_dense_batched(meshes_, prm)
+        build_s += (time.perf_counter() - t0) / K
         if k < K - 1:
-            for g in grids:
+            for g in grids:  # outside the timed build
                 g.close()
-    build_s = (time.perf_counter() - t0) / K
+    phases = _lib.debug_build_phases()
     vox = sum(int(np.prod(g.spec.shape)) for g in grids)
     rng = np.random.default_rng(9)
     counts = rng.integers(10_000, 500_001, size=len(grids))
@@ -290,6 +294,7 @@ def bench_batched_gears(args):
                    "meshes": len(grids), "triangles": int(sum(len(t) for _, t in meshes_)),
                    "voxels": vox, "query_points": n},
         "batched_build_ms": round(build_s * 1e3, 3),
+        "batched_build_phases_ms": {k: round(float(v), 3) for k, v in phases.items()},
         "batched_query_graph_ms": round(q_ms, 4),
         "batched_query_mpts_per_s": round(n / (q_ms * 1e-3) / 1e6, 3),
         "tuner": sdfo.tuner_report(),

@@ -995,4 +995,35 @@ void launchSparseMirror(int64_t count, const float* sphi, const float* sn3, floa
     SDFO_HIP(hipGetLastError());
 }
 
+namespace {
+
+// The box of each slot triangle: the same min / max as the host computes.
+__global__ void __launch_bounds__(256)
+slotBoxKernel(const double* __restrict__ tri9, int64_t n, double* __restrict__ box6)
+{
+    const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const double* t = tri9 + 9 * s;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+    {
+        const double a = t[k], b = t[3 + k], c = t[6 + k];
+        double lo = (b < a) ? b : a;
+        lo = (c < lo) ? c : lo;
+        double hi = (a < b) ? b : a;
+        hi = (hi < c) ? c : hi;
+        box6[6 * s + k] = lo;
+        box6[6 * s + 3 + k] = hi;
+    }
+}
+
+} // namespace
+
+void launchSlotBoxes(const double* tri9, int64_t n, double* box6, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(slotBoxKernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, tri9, n, box6);
+    SDFO_HIP(hipGetLastError());
+}
+
 } // namespace sdfo

@@ -270,131 +270,164 @@ std::vector<std::unique_ptr<sdfo_grid>> buildDenseBatched(int32_t nm, const doub
                                                           const sdfo_build_params* params,
                                                           hipStream_t s)
 {
+    using clk = std::chrono::steady_clock;
+    auto msSince = [](clk::time_point a) {
+        return std::chrono::duration<double, std::milli>(clk::now() - a).count();
+    };
+    double* ph = lastBuildPhases();
+    for (int q = 0; q < kBuildPhases; ++q) ph[q] = 0.0;
+    const clk::time_point t0 = clk::now();
     for (int32_t m = 0; m < nm; ++m)
     {
         if (vo[m + 1] < vo[m] || to[m + 1] < to[m])
             throw Error(SDFO_ERR_INVALID_ARGUMENT, "batched build: offsets must be non-decreasing");
         checkMesh(verts + 3 * vo[m], vo[m + 1] - vo[m], tris + 3 * to[m], to[m + 1] - to[m]);
     }
+    // The concatenated BVH: each mesh's node and slot ranges are known from
+    // its triangle count, so the meshes build straight into their slices.
     std::vector<GridSpec> specs(static_cast<size_t>(nm));
-    std::vector<HostBvh> bvhs(static_cast<size_t>(nm));
-    {
-        // meshes are independent: a small pool of host threads, largest first
-        std::vector<int32_t> order(static_cast<size_t>(nm));
-        for (int32_t m = 0; m < nm; ++m) order[size_t(m)] = m;
-        std::sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
-            return (to[a + 1] - to[a]) > (to[b + 1] - to[b]);
-        });
-        std::atomic<int32_t> next{0};
-        std::vector<std::string> errs(static_cast<size_t>(nm));
-        std::vector<int> codes(static_cast<size_t>(nm), SDFO_OK);
-        auto work = [&] {
-            for (int32_t q = next++; q < nm; q = next++)
-            {
-                const int32_t m = order[size_t(q)];
-                try
-                {
-                    const double* v = verts + 3 * vo[m];
-                    specs[size_t(m)] = setupGrid(v, vo[m + 1] - vo[m], params[m]);
-                    bvhs[size_t(m)] = buildHostBvh(v, vo[m + 1] - vo[m], tris + 3 * to[m],
-                                                   to[m + 1] - to[m]);
-                }
-                catch (const Error& e)
-                {
-                    codes[size_t(m)] = e.code;
-                    errs[size_t(m)] = e.what();
-                }
-                catch (const std::exception& e)
-                {
-                    codes[size_t(m)] = SDFO_ERR_INVALID_ARGUMENT;
-                    errs[size_t(m)] = e.what();
-                }
-            }
-        };
-        const int nth = std::max(1, std::min<int>(nm, int(std::thread::hardware_concurrency())));
-        std::vector<std::thread> pool;
-        for (int t = 1; t < nth; ++t) pool.emplace_back(work);
-        work();
-        for (std::thread& t : pool) t.join();
-        for (int32_t m = 0; m < nm; ++m)
-            if (codes[size_t(m)] != SDFO_OK)
-                throw Error(codes[size_t(m)], "mesh " + std::to_string(m) + ": " + errs[size_t(m)]);
-    }
-    // concatenate the BVHs; node links and leaf slot ranges are rebased
-    HostBvh all;
-    std::vector<int> roots(static_cast<size_t>(nm));
-    size_t nodes = 0, slots = 0;
-    for (const HostBvh& h : bvhs)
-    {
-        nodes += h.nodes.size();
-        slots += h.slotTri.size();
-    }
-    all.nodes.reserve(nodes);
-    all.slotTri9.reserve(slots * 9);
-    all.slotBox6.reserve(slots * 6);
-    all.slotTri.reserve(slots);
+    std::vector<int64_t> nodeBase(static_cast<size_t>(nm) + 1, 0), slotBase(static_cast<size_t>(nm) + 1, 0);
     for (int32_t m = 0; m < nm; ++m)
     {
-        const HostBvh& h = bvhs[size_t(m)];
-        const int nodeBase = int(all.nodes.size());
-        const int slotBase = int(all.slotTri.size());
-        roots[size_t(m)] = nodeBase;
-        for (BvhNode nd : h.nodes)
-        {
-            if (nd.triCount > 0)
-                nd.triStart += slotBase;
-            else
-            {
-                nd.left += nodeBase;
-                nd.right += nodeBase;
-            }
-            all.nodes.push_back(nd);
-        }
-        all.slotTri9.insert(all.slotTri9.end(), h.slotTri9.begin(), h.slotTri9.end());
-        all.slotBox6.insert(all.slotBox6.end(), h.slotBox6.begin(), h.slotBox6.end());
-        all.slotTri.insert(all.slotTri.end(), h.slotTri.begin(), h.slotTri.end());
-        all.depth = std::max(all.depth, h.depth);
-        for (int k = 0; k < 3; ++k) all.rayDir[k] = h.rayDir[k];
+        specs[size_t(m)] = setupGrid(verts + 3 * vo[m], vo[m + 1] - vo[m], params[m]);
+        nodeBase[size_t(m) + 1] = nodeBase[size_t(m)] + bvhNodeCount(to[m + 1] - to[m]);
+        slotBase[size_t(m) + 1] = slotBase[size_t(m)] + (to[m + 1] - to[m]);
     }
+    if (nodeBase[size_t(nm)] >= (int64_t(1) << 26) || slotBase[size_t(nm)] > INT32_MAX)
+        throw Error(SDFO_ERR_UNSUPPORTED, "batched build: the concatenated BVH is too large");
+    // raw arrays: every element is written by the mesh builds (no zero fill)
+    const int64_t nAllNodes = nodeBase[size_t(nm)], nAllSlots = slotBase[size_t(nm)];
+    std::unique_ptr<BvhNode[]> allNodes(new BvhNode[size_t(std::max<int64_t>(1, nAllNodes))]);
+    std::unique_ptr<double[]> allTri9(new double[size_t(std::max<int64_t>(1, nAllSlots)) * 9]);
+    std::unique_ptr<int32_t[]> allSlotTri(new int32_t[size_t(std::max<int64_t>(1, nAllSlots))]);
+    std::vector<int> depths(static_cast<size_t>(nm), 0);
+    // meshes are independent: a pool of host threads, largest first, each
+    // mesh built sequentially (no nested threads)
+    std::vector<int32_t> order(static_cast<size_t>(nm));
+    for (int32_t m = 0; m < nm; ++m) order[size_t(m)] = m;
+    std::sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+        return (to[a + 1] - to[a]) > (to[b + 1] - to[b]);
+    });
+    std::atomic<int32_t> next{0};
+    std::vector<std::string> errs(static_cast<size_t>(nm));
+    std::vector<int> codes(static_cast<size_t>(nm), SDFO_OK);
+    auto work = [&] {
+        for (int32_t q = next++; q < nm; q = next++)
+        {
+            const int32_t m = order[size_t(q)];
+            try
+            {
+                const size_t nb = size_t(nodeBase[size_t(m)]), sb = size_t(slotBase[size_t(m)]);
+                depths[size_t(m)] = buildHostBvhInto(
+                    verts + 3 * vo[m], vo[m + 1] - vo[m], tris + 3 * to[m], to[m + 1] - to[m],
+                    allNodes.get() + nb, allTri9.get() + 9 * sb, allSlotTri.get() + sb, int(nb),
+                    int(sb), false);
+            }
+            catch (const Error& e)
+            {
+                codes[size_t(m)] = e.code;
+                errs[size_t(m)] = e.what();
+            }
+            catch (const std::exception& e)
+            {
+                codes[size_t(m)] = SDFO_ERR_INVALID_ARGUMENT;
+                errs[size_t(m)] = e.what();
+            }
+        }
+    };
+    std::vector<std::thread> pool;
+    struct PoolJoin
+    {
+        std::vector<std::thread>& p;
+        ~PoolJoin()
+        {
+            for (std::thread& t : p)
+                if (t.joinable()) t.join();
+        }
+    } poolJoin{pool};
+    ph[0] = msSince(t0); // validation and grid set-ups
+    const clk::time_point tPool = clk::now();
+    const int nth = std::max(1, std::min<int>(nm, hostThreads()));
+    for (int t = 1; t < nth; ++t) pool.emplace_back(work);
+
+    // meanwhile this thread allocates every grid's device arrays
     std::vector<std::unique_ptr<sdfo_grid>> grids(static_cast<size_t>(nm));
     std::vector<BatchMesh> jobs(static_cast<size_t>(nm));
+    const int dev = currentDevice();
+    keepStreamPoolMemory(dev);
+    for (int32_t m = 0; m < nm; ++m)
+    {
+        auto g = std::make_unique<sdfo_grid>();
+        g->device = dev;
+        g->spec = specs[size_t(m)];
+        const int64_t N = g->spec.total();
+        g->phi = g->alloc<double>(size_t(N));
+        g->n3 = g->alloc<double>(size_t(N) * 3);
+        g->h6 = g->alloc<double>(size_t(N) * 6);
+        g->witness3 = g->alloc<double>(size_t(N) * 3);
+        g->tri = g->alloc<int32_t>(size_t(N));
+        // the FP32 mirrors and cell records of finishDense
+        g->pn = g->alloc<float4>(size_t(N));
+        g->h6f = g->alloc<float>(size_t(N) * 6);
+        const int64_t cells = int64_t(g->spec.n[0] - 1) * (g->spec.n[1] - 1) * (g->spec.n[2] - 1);
+        if (cells > 0 && cells * 8 * int64_t(sizeof(float4)) <= kCellRecordLimit)
+            g->cellRec = g->alloc<float4>(size_t(cells) * 8);
+        grids[size_t(m)] = std::move(g);
+    }
+    ph[2] = msSince(tPool); // allocations, overlapping the BVHs
+    work();
+    for (std::thread& t : pool) t.join();
+    for (int32_t m = 0; m < nm; ++m)
+        if (codes[size_t(m)] != SDFO_OK)
+            throw Error(codes[size_t(m)], "mesh " + std::to_string(m) + ": " + errs[size_t(m)]);
+    int depth = 0;
+    for (int32_t m = 0; m < nm; ++m)
+    {
+        sdfo_grid& g = *grids[size_t(m)];
+        jobs[size_t(m)] = BatchMesh{g.spec, int(nodeBase[size_t(m)]), to[m + 1] - to[m], g.phi, g.n3,
+                                    g.witness3, g.tri};
+        depth = std::max(depth, depths[size_t(m)]);
+    }
+    HostBvh dirOnly;
+    setRayDir(dirOnly);
+    ph[1] = msSince(tPool); // BVHs (with the allocations beside them)
     DeviceBvh db;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     try
     {
-        const int dev = currentDevice();
-        keepStreamPoolMemory(dev);
-        db.upload(all, s);
-        for (int32_t m = 0; m < nm; ++m)
-        {
-            auto g = std::make_unique<sdfo_grid>();
-            g->device = dev;
-            g->spec = specs[size_t(m)];
-            const int64_t N = g->spec.total();
-            g->phi = g->alloc<double>(size_t(N));
-            g->n3 = g->alloc<double>(size_t(N) * 3);
-            g->h6 = g->alloc<double>(size_t(N) * 6);
-            g->witness3 = g->alloc<double>(size_t(N) * 3);
-            g->tri = g->alloc<int32_t>(size_t(N));
-            jobs[size_t(m)] = BatchMesh{g->spec, roots[size_t(m)], to[m + 1] - to[m], g->phi, g->n3,
-                                        g->witness3, g->tri};
-            grids[size_t(m)] = std::move(g);
-        }
+        for (hipEvent_t& e : ev) SDFO_HIP(hipEventCreate(&e));
+        db.upload(allNodes.get(), nAllNodes, allTri9.get(), allSlotTri.get(), nAllSlots, depth,
+                  dirOnly.rayDir, s);
+        SDFO_HIP(hipEventRecord(ev[0], s));
         launchDenseFillBatched(jobs, db, s);
+        SDFO_HIP(hipEventRecord(ev[1], s));
         for (int32_t m = 0; m < nm; ++m)
         {
             sdfo_grid& g = *grids[size_t(m)];
             launchDenseHessian(g.spec, g.n3, g.h6, s);
             finishDense(g, s);
         }
+        SDFO_HIP(hipEventRecord(ev[2], s));
         SDFO_HIP(hipStreamSynchronize(s));
+        float f = 0.0f;
+        SDFO_HIP(hipEventElapsedTime(&f, ev[0], ev[1]));
+        ph[3] = f;
+        SDFO_HIP(hipEventElapsedTime(&f, ev[1], ev[2]));
+        ph[5] = f; // Hessians, mirrors and their allocations
     }
     catch (...)
     {
         (void)hipStreamSynchronize(s);
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
         db.release();
         throw;
     }
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     db.release();
+    ph[6] = msSince(t0);
+    const int* rc = replayCounterHost();
+    ph[7] = rc ? double(*rc) : -1.0;
     return grids;
 }
 

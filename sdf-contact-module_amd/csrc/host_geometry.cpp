@@ -7,6 +7,9 @@
 #include <numeric>
 #include <thread>
 
+#include <sched.h>
+#include <cstdlib>
+
 #include "sdfo_internal.hpp"
 
 namespace sdfo {
@@ -66,12 +69,12 @@ namespace {
 int64_t subtreeNodes(int64_t n) { return n <= 8 ? 1 : 1 + subtreeNodes(n / 2) + subtreeNodes(n - n / 2); }
 
 // Runs f(begin, end) over [0, n) in chunks on up to 16 threads (small
-// ranges on the calling thread).
+// ranges, and sequential builds, on the calling thread).
 template <typename F>
-void parallelFor(int n, F f)
+void parallelFor(int n, F f, bool parallel = true)
 {
-    const int threads = std::max(1, std::min<int>(16, int(std::thread::hardware_concurrency())));
-    const int chunks = n < 65536 ? 1 : threads;
+    const int threads = std::min(16, hostThreads());
+    const int chunks = (n < 65536 || !parallel) ? 1 : threads;
     if (chunks <= 1)
     {
         f(0, n);
@@ -99,7 +102,7 @@ struct Item
 
 struct BvhBuilder
 {
-    std::vector<BvhNode>& nodes;
+    BvhNode* nodes;                 // this mesh's nodes (indices local to the mesh)
     std::vector<Item>& items;
     const std::vector<double>& box; // bmin3, bmax3 per triangle
 
@@ -135,7 +138,7 @@ struct BvhBuilder
             }
             nd.triStart = lo;
             nd.triCount = count;
-            nodes[size_t(me)] = nd;
+            nodes[me] = nd;
             return level;
         }
         double cmn[3] = {big, big, big}, cmx[3] = {-big, -big, -big};
@@ -171,8 +174,8 @@ struct BvhBuilder
             dl = build(lo, mid, left, level + 1, par);
             dr = build(mid, hi, right, level + 1, par);
         }
-        const BvhNode& L = nodes[size_t(left)];
-        const BvhNode& R = nodes[size_t(right)];
+        const BvhNode& L = nodes[left];
+        const BvhNode& R = nodes[right];
         for (int k = 0; k < 3; ++k)
         {
             nd.bmin[k] = minOf(L.bmin[k], R.bmin[k]);
@@ -180,21 +183,43 @@ struct BvhBuilder
         }
         nd.left = int(left);
         nd.right = int(right);
-        nodes[size_t(me)] = nd;
+        nodes[me] = nd;
         return dl > dr ? dl : dr;
     }
 };
 
 } // namespace
 
+int hostThreads()
+{
+    static const int n = [] {
+        int c = int(std::thread::hardware_concurrency());
+        cpu_set_t set;
+        CPU_ZERO(&set);
+        if (sched_getaffinity(0, sizeof(set), &set) == 0) c = std::min(c, int(CPU_COUNT(&set)));
+        if (const char* e = std::getenv("OMP_NUM_THREADS"))
+        {
+            const int o = std::atoi(e);
+            if (o > 0) c = std::min(c, o);
+        }
+        return std::max(1, std::min(c, 64));
+    }();
+    return n;
+}
+
+int64_t bvhNodeCount(int64_t nt) { return nt > 0 ? subtreeNodes(nt) : 0; }
+
 // KdTree::build / buildRecursive (KdTree.h:91-121, 188-242): object-median
 // split on the largest centroid extent via std::nth_element, leaves of <= 8.
 // Node indices are the recursion's pre-order creation order, so the tree is
 // node-for-node the reference tree; the traversal order (near child first)
-// is reproduced by the kernels.
-HostBvh buildHostBvh(const double* verts, int64_t nv, const int32_t* tris, int64_t nt)
+// is reproduced by the kernels.  Writes bvhNodeCount(nt) nodes and nt slots
+// into the given arrays with node links and slot ranges offset by nodeBase
+// and slotBase (one mesh of a concatenated BVH); returns the depth.
+int buildHostBvhInto(const double* verts, int64_t nv, const int32_t* tris, int64_t nt,
+                     BvhNode* nodes, double* slotTri9, int32_t* slotTri, int nodeBase, int slotBase,
+                     bool parallel)
 {
-    HostBvh h;
     const int n = int(nt);
     for (int64_t q = 0; q < 3 * nt; ++q)
         if (tris[q] < 0 || tris[q] >= nv)
@@ -202,79 +227,117 @@ HostBvh buildHostBvh(const double* verts, int64_t nv, const int32_t* tris, int64
     // per-triangle boxes and centroids (in slot-order items), in parallel chunks
     std::vector<double> box(size_t(n) * 6);
     std::vector<Item> items(static_cast<size_t>(n));
-    parallelFor(n, [&](int t0, int t1) {
-        for (int t = t0; t < t1; ++t)
-        {
-            const int32_t* ix = tris + 3 * size_t(t);
-            const double* a = verts + 3 * size_t(ix[0]);
-            const double* b = verts + 3 * size_t(ix[1]);
-            const double* c = verts + 3 * size_t(ix[2]);
-            Item& it = items[size_t(t)];
-            for (int k = 0; k < 3; ++k)
+    parallelFor(
+        n,
+        [&](int t0, int t1) {
+            for (int t = t0; t < t1; ++t)
             {
-                box[6 * size_t(t) + k] = minOf(minOf(a[k], b[k]), c[k]);
-                box[6 * size_t(t) + 3 + k] = maxOf(maxOf(a[k], b[k]), c[k]);
-                it.c[k] = ((a[k] + b[k]) + c[k]) / 3.0;
+                const int32_t* ix = tris + 3 * size_t(t);
+                const double* a = verts + 3 * size_t(ix[0]);
+                const double* b = verts + 3 * size_t(ix[1]);
+                const double* c = verts + 3 * size_t(ix[2]);
+                Item& it = items[size_t(t)];
+                for (int k = 0; k < 3; ++k)
+                {
+                    box[6 * size_t(t) + k] = minOf(minOf(a[k], b[k]), c[k]);
+                    box[6 * size_t(t) + 3 + k] = maxOf(maxOf(a[k], b[k]), c[k]);
+                    it.c[k] = ((a[k] + b[k]) + c[k]) / 3.0;
+                }
+                it.idx = t;
+                it.pad = 0;
             }
-            it.idx = t;
-            it.pad = 0;
-        }
-    });
+        },
+        parallel);
+    int depth = 0;
     if (n > 0)
     {
-        h.nodes.resize(size_t(subtreeNodes(n)));
         // Subtrees of more than n/16 triangles (and at least 4096) run on their own thread.
-        const int64_t par = std::max<int64_t>(4096, n / 16);
-        BvhBuilder bb{h.nodes, items, box};
-        h.depth = bb.build(0, n, 0, 1, par);
+        const int64_t par = parallel ? std::max<int64_t>(4096, n / 16) : INT64_MAX;
+        BvhBuilder bb{nodes, items, box};
+        depth = bb.build(0, n, 0, 1, par);
+        const int64_t nn = bvhNodeCount(n);
+        if (nodeBase != 0 || slotBase != 0)
+            for (int64_t q = 0; q < nn; ++q)
+            {
+                BvhNode& nd = nodes[q];
+                if (nd.triCount > 0)
+                    nd.triStart += slotBase;
+                else
+                {
+                    nd.left += nodeBase;
+                    nd.right += nodeBase;
+                }
+            }
     }
-    h.slotTri.resize(size_t(n));
-    h.slotTri9.resize(size_t(n) * 9);
-    h.slotBox6.resize(size_t(n) * 6);
-    parallelFor(n, [&](int s0, int s1) {
-        for (int s = s0; s < s1; ++s)
-        {
-            const int t = items[size_t(s)].idx;
-            h.slotTri[size_t(s)] = t;
-            for (int q = 0; q < 6; ++q) h.slotBox6[6 * size_t(s) + q] = box[6 * size_t(t) + q];
-            const int32_t* ix = tris + 3 * size_t(t);
-            for (int c = 0; c < 3; ++c)
-                for (int k = 0; k < 3; ++k)
-                    h.slotTri9[9 * size_t(s) + 3 * c + k] = verts[3 * size_t(ix[c]) + k];
-        }
-    });
+    parallelFor(
+        n,
+        [&](int s0, int s1) {
+            for (int s = s0; s < s1; ++s)
+            {
+                const int t = items[size_t(s)].idx;
+                slotTri[s] = t;
+                const int32_t* ix = tris + 3 * size_t(t);
+                for (int c = 0; c < 3; ++c)
+                    for (int k = 0; k < 3; ++k)
+                        slotTri9[9 * size_t(s) + 3 * c + k] = verts[3 * size_t(ix[c]) + k];
+            }
+        },
+        parallel);
+    return depth;
+}
+
+void setRayDir(HostBvh& h)
+{
     // KdTree.h:330-333: fixed irrational ray direction, normalised once on the host.
     const double d0[3] = {1.0, 0.3713906763541037, 0.52999894000318};
     const double len = std::sqrt(d0[0] * d0[0] + d0[1] * d0[1] + d0[2] * d0[2]);
     for (int k = 0; k < 3; ++k) h.rayDir[k] = d0[k] / len;
+}
+
+HostBvh buildHostBvh(const double* verts, int64_t nv, const int32_t* tris, int64_t nt)
+{
+    HostBvh h;
+    h.nodes.resize(size_t(bvhNodeCount(nt)));
+    h.slotTri.resize(size_t(nt));
+    h.slotTri9.resize(size_t(nt) * 9);
+    h.depth = buildHostBvhInto(verts, nv, tris, nt, h.nodes.data(), h.slotTri9.data(),
+                               h.slotTri.data(), 0, 0, true);
+    setRayDir(h);
     return h;
 }
 
 void DeviceBvh::upload(const HostBvh& h, hipStream_t s)
 {
+    upload(h.nodes.data(), int64_t(h.nodes.size()), h.slotTri9.data(), h.slotTri.data(),
+           int64_t(h.slotTri.size()), h.depth, h.rayDir, s);
+}
+
+void DeviceBvh::upload(const BvhNode* hostNodes, int64_t nNodes, const double* slotTri9,
+                       const int32_t* slotTri, int64_t nSlots, int treeDepth, const double* dir,
+                       hipStream_t s)
+{
     release();
-    nnodes = int(h.nodes.size());
-    ntris = int64_t(h.slotTri.size());
-    depth = h.depth;
-    for (int k = 0; k < 3; ++k) rayDir[k] = h.rayDir[k];
+    depth = treeDepth;
+    for (int k = 0; k < 3; ++k) rayDir[k] = dir[k];
     if (depth + 2 > kStackDepth)
         throw Error(SDFO_ERR_UNSUPPORTED, "BVH depth " + std::to_string(depth) +
                                               " exceeds the traversal stack");
-    if (nnodes >= (1 << 26))
-        throw Error(SDFO_ERR_UNSUPPORTED, "BVH with " + std::to_string(nnodes) +
+    if (nNodes >= (1 << 26))
+        throw Error(SDFO_ERR_UNSUPPORTED, "BVH with " + std::to_string(nNodes) +
                                               " nodes exceeds the traversal stack encoding");
-    SDFO_HIP(hipMalloc(&nodes, sizeof(BvhNode) * std::max<size_t>(1, h.nodes.size())));
-    SDFO_HIP(hipMalloc(&tri9, sizeof(double) * std::max<size_t>(1, h.slotTri9.size())));
-    SDFO_HIP(hipMalloc(&box6, sizeof(double) * std::max<size_t>(1, h.slotBox6.size())));
-    SDFO_HIP(hipMalloc(&triIdx, sizeof(int32_t) * std::max<size_t>(1, h.slotTri.size())));
-    SDFO_HIP(hipMemcpyAsync(nodes, h.nodes.data(), sizeof(BvhNode) * h.nodes.size(),
+    nnodes = int(nNodes);
+    ntris = nSlots;
+    SDFO_HIP(hipMalloc(&nodes, sizeof(BvhNode) * size_t(std::max<int64_t>(1, nNodes))));
+    SDFO_HIP(hipMalloc(&tri9, sizeof(double) * 9 * size_t(std::max<int64_t>(1, nSlots))));
+    SDFO_HIP(hipMalloc(&box6, sizeof(double) * 6 * size_t(std::max<int64_t>(1, nSlots))));
+    SDFO_HIP(hipMalloc(&triIdx, sizeof(int32_t) * size_t(std::max<int64_t>(1, nSlots))));
+    SDFO_HIP(hipMemcpyAsync(nodes, hostNodes, sizeof(BvhNode) * size_t(nNodes),
                             hipMemcpyHostToDevice, s));
-    SDFO_HIP(hipMemcpyAsync(tri9, h.slotTri9.data(), sizeof(double) * h.slotTri9.size(),
+    SDFO_HIP(hipMemcpyAsync(tri9, slotTri9, sizeof(double) * 9 * size_t(nSlots),
                             hipMemcpyHostToDevice, s));
-    SDFO_HIP(hipMemcpyAsync(box6, h.slotBox6.data(), sizeof(double) * h.slotBox6.size(),
+    SDFO_HIP(hipMemcpyAsync(triIdx, slotTri, sizeof(int32_t) * size_t(nSlots),
                             hipMemcpyHostToDevice, s));
-    SDFO_HIP(hipMemcpyAsync(triIdx, h.slotTri.data(), sizeof(int32_t) * h.slotTri.size(),
-                            hipMemcpyHostToDevice, s));
+    launchSlotBoxes(tri9, nSlots, box6, s);
 }
 
 void DeviceBvh::release()

@@ -68,13 +68,22 @@ struct HostBvh
 {
     std::vector<BvhNode> nodes;
     std::vector<double> slotTri9; // 9 doubles (a, b, c) per slot
-    std::vector<double> slotBox6; // triangle box per slot: bmin3, bmax3
     std::vector<int32_t> slotTri; // original triangle index per slot
     int depth = 0;
     double rayDir[3] = {0, 0, 0};
 };
 
 HostBvh buildHostBvh(const double* verts, int64_t nv, const int32_t* tris, int64_t nt);
+// The same tree written into caller arrays (bvhNodeCount(nt) nodes, nt slots)
+// with node links and slot ranges offset by the bases; returns the depth.
+int buildHostBvhInto(const double* verts, int64_t nv, const int32_t* tris, int64_t nt,
+                     BvhNode* nodes, double* slotTri9, int32_t* slotTri, int nodeBase, int slotBase,
+                     bool parallel);
+int64_t bvhNodeCount(int64_t nt);
+void setRayDir(HostBvh& h);
+// Host worker threads: the CPUs this process may run on, capped by
+// OMP_NUM_THREADS when set and by 64.
+int hostThreads();
 
 // Device copy of a BVH.
 struct DeviceBvh
@@ -88,8 +97,15 @@ struct DeviceBvh
     int depth = 0;
     double rayDir[3] = {0, 0, 0};
     void upload(const HostBvh& h, hipStream_t s);
+    // The same from raw host arrays; the slot boxes are derived on the device.
+    void upload(const BvhNode* hostNodes, int64_t nNodes, const double* slotTri9,
+                const int32_t* slotTri, int64_t nSlots, int treeDepth, const double* dir,
+                hipStream_t s);
     void release();
 };
+
+// Slot boxes (bmin3, bmax3 of each slot triangle's vertices) on the device (build.hip).
+void launchSlotBoxes(const double* tri9, int64_t n, double* box6, hipStream_t s);
 
 // Maximal BVH traversal stack per lane (LDS).  Host rejects deeper trees.
 constexpr int kStackDepth = 32;
