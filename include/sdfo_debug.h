@@ -25,7 +25,9 @@ int sdfo_debug_bvh_export(const double* verts, int64_t n_verts, const int32_t* t
  * order[i] = index of the i-th value.  Host only. */
 int sdfo_debug_sort_by_abs(const double* values, int32_t n, int32_t* order);
 
-/* Phase timings (ms) of the calling thread's last sdfo_build_dense, in the
+/* Phase timings (ms) of the calling thread's last sdfo_build_dense (or
+ * sdfo_build_dense_batched: set-ups, BVH pool, allocations beside it, fill,
+ * -, Hessians + mirrors, total), in the
  * spirit of the builder's phase log (SdfBuilder.cpp:364-407): grid set-up,
  * host BVH, BVH upload + allocations, fill kernel, Hessian kernel, FP32
  * mirrors + cell records, total; then the number of voxels the fill re-ran
@@ -36,6 +38,13 @@ int sdfo_debug_build_phases(double* ms, int32_t cap);
  * every voxel (and the sparse activation every brick) to the per-lane replay
  * pass, so tests can check that pass against the CPU oracle. */
 int sdfo_debug_force_replay(int32_t on);
+
+/* Test switch for the calling thread: the step budget of the fills' wave
+ * search (0: the library's choice) and the capacity of the batched fill's
+ * list of bricks handed to the frontier pass (0: 2^18).  Small values send
+ * many bricks through the frontier pass and, past the capacity, the replay
+ * pass, so tests can check both against the CPU oracle. */
+int sdfo_debug_fill_budget(int32_t budget, int32_t heavy_cap);
 
 #ifdef __cplusplus
 }

@@ -48,6 +48,7 @@ struct Heavy
     int64_t* ids;  // unit ids
     double* seeds; // 64 per unit
     unsigned long long* queue;
+    int cap; // units the list holds
 };
 
 // Step budgets of one wave's search (node visits + triangle tests) the
@@ -55,14 +56,24 @@ struct Heavy
 constexpr int kBudgetCount = 3;
 constexpr int kBudgets[kBudgetCount] = {1024, 512, 2048};
 const char* const kBudgetNames[kBudgetCount] = {"1024", "512", "2048"};
+constexpr int kBatchedBudget = 8192;
 
-__device__ __forceinline__ void pushHeavy(const Heavy& h, int64_t unit, bool active, double bound)
+// Returns false when the list is full: the caller replays the unit's voxels.
+__device__ __forceinline__ bool pushHeavy(const Heavy& h, int64_t unit, bool active, double bound)
 {
     int slot = 0;
     if ((threadIdx.x & 63) == 0) slot = atomicAdd(h.count, 1);
     slot = __shfl(slot, 0);
+    if (slot >= h.cap) return false;
     if ((threadIdx.x & 63) == 0) h.ids[slot] = unit;
     h.seeds[int64_t(slot) * 64 + (threadIdx.x & 63)] = active ? bound : DBL_MAX;
+    return true;
+}
+
+__device__ __forceinline__ int64_t heavyUnits(const Heavy& h)
+{
+    const int n = *h.count;
+    return int64_t(n < h.cap ? n : h.cap) * 64;
 }
 
 // Frontier sizes: the first frontier pass runs 4 waves per block with 1000
@@ -164,7 +175,7 @@ __device__ __forceinline__ int64_t nextUnit(unsigned long long* queue)
 // walks the BVH cooperatively (SdfBuilder.cpp:1052-1100).  Waves take bricks
 // from a queue until it is drained, so clusters of expensive bricks (points
 // equidistant from many triangles) spread over the whole chip.
-__global__ void __launch_bounds__(kFillBlock)
+__global__ void __launch_bounds__(kFillBlock) __attribute__((amdgpu_waves_per_eu(5)))
 denseFillKernel(GridSpec g, const BvhNode* __restrict__ nodes, const double* __restrict__ tri9,
                 const double* __restrict__ box6, D3 dir, double mu, BrickShape sh, DenseFillOut o,
                 Replay replay, Heavy heavy, unsigned long long* queue, int budget, int force)
@@ -182,14 +193,14 @@ denseFillKernel(GridSpec g, const BvhNode* __restrict__ nodes, const double* __r
         const unsigned long long act = __ballot(active);
         const dev::WaveNearest wn =
             dev::waveNearest(c, active, act, nodes, tri9, box6, mu, ws, lane, 0, budget);
+        const int64_t idx = i + int64_t(g.n[0]) * (j + int64_t(g.n[1]) * k);
         if (wn.aborted)
         {
-            pushHeavy(heavy, brick, active, wn.bestSq);
+            if (!pushHeavy(heavy, brick, active, wn.bestSq) && active) pushReplay(replay, idx);
             continue;
         }
         const int hits = dev::waveRayHits(c, dir, act, nodes, tri9, ws, lane, 0);
         if (!active) continue;
-        const int64_t idx = i + int64_t(g.n[0]) * (j + int64_t(g.n[1]) * k);
         if (!wn.exact || force)
             pushReplay(replay, idx);
         else
@@ -206,7 +217,7 @@ denseHeavyKernel(GridSpec g, const BvhNode* __restrict__ nodes, const double* __
 {
     __shared__ __align__(16) unsigned char lds[Waves * dev::frontierBytes(Cap)];
     const dev::FrontierLds f = dev::carveFrontier(lds + (threadIdx.x >> 6) * dev::frontierBytes(Cap), Cap);
-    const int64_t units = int64_t(*heavy.count) * 64;
+    const int64_t units = heavyUnits(heavy);
     for (int64_t u = nextUnit(heavy.queue); u < units; u = nextUnit(heavy.queue))
     {
         int i, j, k;
@@ -277,11 +288,11 @@ __device__ __forceinline__ int findJob(const MeshJob* __restrict__ jobs, int nJo
     return lo;
 }
 
-__global__ void __launch_bounds__(kFillBlock)
+__global__ void __launch_bounds__(kFillBlock) __attribute__((amdgpu_waves_per_eu(5)))
 denseFillBatchedKernel(const MeshJob* __restrict__ jobs, int nJobs, int64_t totalBricks,
                        const BvhNode* __restrict__ nodes, const double* __restrict__ tri9,
-                       const double* __restrict__ box6, D3 dir, Replay replay,
-                       unsigned long long* queue, int force)
+                       const double* __restrict__ box6, D3 dir, Replay replay, Heavy heavy,
+                       unsigned long long* queue, int budget, int force)
 {
     __shared__ __align__(16) unsigned char lds[kFillWaves * dev::kWaveStackBytes];
     const int lane = threadIdx.x & 63;
@@ -296,14 +307,92 @@ denseFillBatchedKernel(const MeshJob* __restrict__ jobs, int nJobs, int64_t tota
         sh.voxel(unit - J.brickBegin, lane, i, j, k);
         const bool active = i < g.n[0] && j < g.n[1] && k < g.n[2];
         const D3 c = voxelCentre(g, min(i, g.n[0] - 1), min(j, g.n[1] - 1), min(k, g.n[2] - 1));
-        const dev::WaveSigned w =
-            dev::waveSignedNearest(c, active, nodes, tri9, box6, dir, J.mu, ws, J.root);
-        if (!active) continue;
+        const unsigned long long act = __ballot(active);
+        const dev::WaveNearest wn =
+            dev::waveNearest(c, active, act, nodes, tri9, box6, J.mu, ws, lane, J.root, budget);
         const int64_t idx = i + int64_t(g.n[0]) * (j + int64_t(g.n[1]) * k);
-        if (!w.exact || force)
+        if (wn.aborted)
+        {
+            if (!pushHeavy(heavy, unit, active, wn.bestSq) && active)
+                pushReplay(replay, J.voxelBegin + idx);
+            continue;
+        }
+        const int hits = dev::waveRayHits(c, dir, act, nodes, tri9, ws, lane, J.root);
+        if (!active) continue;
+        if (!wn.exact || force)
             pushReplay(replay, J.voxelBegin + idx);
         else
-            writeDenseVoxel(J.o, c, w.s, idx);
+            writeDenseVoxel(J.o, c, dev::finishSigned(c, tri9, wn.bestSq, wn.bestSlot, hits), idx);
+    }
+}
+
+// Voxels of the bricks the batched fill handed over: one voxel per wave.
+template <int Cap, int Waves>
+__global__ void __launch_bounds__(64 * Waves)
+denseHeavyBatchedKernel(const MeshJob* __restrict__ jobs, int nJobs,
+                        const BvhNode* __restrict__ nodes, const double* __restrict__ tri9,
+                        const double* __restrict__ box6, D3 dir, Replay replay, Heavy heavy,
+                        Spill spill, int force)
+{
+    __shared__ __align__(16) unsigned char lds[Waves * dev::frontierBytes(Cap)];
+    const dev::FrontierLds f = dev::carveFrontier(lds + (threadIdx.x >> 6) * dev::frontierBytes(Cap), Cap);
+    const int64_t units = heavyUnits(heavy);
+    for (int64_t u = nextUnit(heavy.queue); u < units; u = nextUnit(heavy.queue))
+    {
+        const int64_t brick = heavy.ids[u >> 6];
+        const MeshJob& J = jobs[dev::uniformInt(findJob(jobs, nJobs, brick, false))];
+        const GridSpec& g = J.g;
+        int i, j, k;
+        J.sh.voxel(brick - J.brickBegin, int(u & 63), i, j, k);
+        if (i >= g.n[0] || j >= g.n[1] || k >= g.n[2]) continue; // wave-uniform
+        const D3 c = voxelCentre(g, i, j, k);
+        double seed = heavy.seeds[u];
+        int status = 0;
+        const dev::SignedNearest s =
+            dev::frontierSignedNearest(c, nodes, tri9, box6, dir, J.mu, J.root, seed, f, status);
+        if ((threadIdx.x & 63) != 0) continue;
+        const int64_t idx = i + int64_t(g.n[0]) * (j + int64_t(g.n[1]) * k);
+        if (status == dev::kFrontierOverflow)
+        {
+            const int q = atomicAdd(spill.count, 1);
+            spill.ids[q] = J.voxelBegin + idx;
+            spill.seeds[q] = seed;
+        }
+        else if (status != dev::kFrontierExact || force)
+            pushReplay(replay, J.voxelBegin + idx);
+        else
+            writeDenseVoxel(J.o, c, s, idx);
+    }
+}
+
+// Batched voxels whose first frontier overflowed: one wave, 4000 entries.
+__global__ void __launch_bounds__(64)
+denseSpillBatchedKernel(const MeshJob* __restrict__ jobs, int nJobs,
+                        const BvhNode* __restrict__ nodes, const double* __restrict__ tri9,
+                        const double* __restrict__ box6, D3 dir, Replay replay, Spill spill,
+                        int force)
+{
+    __shared__ __align__(16) unsigned char lds[dev::frontierBytes(kFrontierLarge)];
+    const dev::FrontierLds f = dev::carveFrontier(lds, kFrontierLarge);
+    const int64_t n = *spill.count;
+    for (int64_t q = nextUnit(spill.queue); q < n; q = nextUnit(spill.queue))
+    {
+        const int64_t id = spill.ids[q];
+        const MeshJob& J = jobs[dev::uniformInt(findJob(jobs, nJobs, id, true))];
+        const GridSpec& g = J.g;
+        const int64_t idx = id - J.voxelBegin;
+        const int i = int(idx % g.n[0]), j = int((idx / g.n[0]) % g.n[1]),
+                  k = int(idx / (int64_t(g.n[0]) * g.n[1]));
+        const D3 c = voxelCentre(g, i, j, k);
+        double seed = spill.seeds[q];
+        int status = 0;
+        const dev::SignedNearest s =
+            dev::frontierSignedNearest(c, nodes, tri9, box6, dir, J.mu, J.root, seed, f, status);
+        if ((threadIdx.x & 63) != 0) continue;
+        if (status != dev::kFrontierExact || force)
+            pushReplay(replay, id);
+        else
+            writeDenseVoxel(J.o, c, s, idx);
     }
 }
 
@@ -497,7 +586,7 @@ __device__ __forceinline__ void writeSparseVoxel(D3 c, const dev::SignedNearest&
 
 // Sparse fill: one lane per brick voxel, brick-major (SdfBuilder.cpp:925-965);
 // 64 consecutive brick voxels per queue unit.
-__global__ void __launch_bounds__(kFillBlock)
+__global__ void __launch_bounds__(kFillBlock) __attribute__((amdgpu_waves_per_eu(5)))
 sparseFillKernel(GridSpec g, const BvhNode* __restrict__ nodes, const double* __restrict__ tri9,
                  const double* __restrict__ box6, D3 dir, double mu, int bd, int64_t total,
                  const int32_t* __restrict__ bijk, float* __restrict__ sphi,
@@ -530,7 +619,7 @@ sparseFillKernel(GridSpec g, const BvhNode* __restrict__ nodes, const double* __
             dev::waveNearest(c, active, act, nodes, tri9, box6, mu, ws, lane, 0, budget);
         if (wn.aborted)
         {
-            pushHeavy(heavy, u, active, wn.bestSq);
+            if (!pushHeavy(heavy, u, active, wn.bestSq) && active) pushReplay(replay, t);
             continue;
         }
         const int hits = dev::waveRayHits(c, dir, act, nodes, tri9, ws, lane, 0);
@@ -551,7 +640,7 @@ sparseHeavyKernel(GridSpec g, const BvhNode* __restrict__ nodes, const double* _
 {
     __shared__ __align__(16) unsigned char lds[Waves * dev::frontierBytes(Cap)];
     const dev::FrontierLds f = dev::carveFrontier(lds + (threadIdx.x >> 6) * dev::frontierBytes(Cap), Cap);
-    const int64_t units = int64_t(*heavy.count) * 64;
+    const int64_t units = heavyUnits(heavy);
     for (int64_t u = nextUnit(heavy.queue); u < units; u = nextUnit(heavy.queue))
     {
         const int64_t t = heavy.ids[u >> 6] * 64 + (u & 63);
@@ -688,7 +777,7 @@ namespace {
 struct ReplayBuf
 {
     Replay r{nullptr, nullptr};
-    Heavy h{nullptr, nullptr, nullptr, nullptr};
+    Heavy h{nullptr, nullptr, nullptr, nullptr, 0};
     Spill sp{nullptr, nullptr, nullptr, nullptr};
     unsigned long long* queue = nullptr; // work-queue counter of the fill launch
     hipStream_t s;
@@ -704,6 +793,7 @@ struct ReplayBuf
         sp.count = reinterpret_cast<int*>(queue + 5);
         SDFO_HIP(hipMallocAsync(reinterpret_cast<void**>(&r.ids),
                                 sizeof(int64_t) * size_t(std::max<int64_t>(cap, 1)), s));
+        h.cap = int(std::min<int64_t>(units, INT32_MAX));
         if (units > 0)
         {
             SDFO_HIP(hipMallocAsync(reinterpret_cast<void**>(&h.ids), sizeof(int64_t) * size_t(units), s));
@@ -780,6 +870,18 @@ bool& forceReplay()
     return f;
 }
 
+int& debugFillBudget()
+{
+    thread_local int b = 0;
+    return b;
+}
+
+int& debugHeavyCap()
+{
+    thread_local int c = 0;
+    return c;
+}
+
 namespace {
 
 // Fill variants the tuner chooses from: brick shape x step budget.
@@ -796,7 +898,8 @@ std::vector<std::string> fillVariantNames()
 void runDenseFill(const GridSpec& g, const DeviceBvh& bvh, const DenseFillOut& o, int variant,
                   bool report, hipStream_t s)
 {
-    const int shape = variant / kBudgetCount, budget = kBudgets[variant % kBudgetCount];
+    const int shape = variant / kBudgetCount;
+    const int budget = debugFillBudget() > 0 ? debugFillBudget() : kBudgets[variant % kBudgetCount];
     const BrickShape sh = BrickShape::make(g, kFillShapes[shape][0], kFillShapes[shape][1]);
     const D3 dir = dev::d3(bvh.rayDir[0], bvh.rayDir[1], bvh.rayDir[2]);
     ReplayBuf rb(g.total(), s, sh.count());
@@ -884,10 +987,25 @@ void launchDenseFillBatched(const std::vector<BatchMesh>& meshes, const DeviceBv
     SDFO_HIP(hipMemcpyAsync(dj, jobs.data(), sizeof(MeshJob) * jobs.size(), hipMemcpyHostToDevice, s));
     const D3 dir = dev::d3(bvh.rayDir[0], bvh.rayDir[1], bvh.rayDir[2]);
     {
-        ReplayBuf rb(voxels, s);
+        // bricks over the step budget go to the frontier pass; a list of at
+        // most 2^18 bricks, the rest (if any) to the per-lane replay
+        const int64_t cap = debugHeavyCap() > 0 ? debugHeavyCap() : int64_t(1) << 18;
+        ReplayBuf rb(voxels, s, std::min<int64_t>(bricks, cap));
+        const int force = forceReplay() ? 1 : 0;
+        // a batch has enough bricks to hide ordinary slow ones: only the
+        // extreme ones (points equidistant from thousands of triangles) go to
+        // the frontier pass
+        const int budget = debugFillBudget() > 0 ? debugFillBudget() : kBatchedBudget;
         hipLaunchKernelGGL(denseFillBatchedKernel, dim3(queueGrid(bricks)), dim3(kFillBlock), 0, s,
                            dj, int(jobs.size()), bricks, bvh.nodes, bvh.tri9, bvh.box6, dir, rb.r,
-                           rb.queue, forceReplay() ? 1 : 0);
+                           rb.h, rb.queue, budget, force);
+        SDFO_HIP(hipGetLastError());
+        hipLaunchKernelGGL((denseHeavyBatchedKernel<kFrontierSmall, 4>), dim3(heavyGrid()), dim3(256),
+                           0, s, dj, int(jobs.size()), bvh.nodes, bvh.tri9, bvh.box6, dir, rb.r, rb.h,
+                           rb.sp, force);
+        SDFO_HIP(hipGetLastError());
+        hipLaunchKernelGGL(denseSpillBatchedKernel, dim3(heavyGrid()), dim3(64), 0, s, dj,
+                           int(jobs.size()), bvh.nodes, bvh.tri9, bvh.box6, dir, rb.r, rb.sp, force);
         SDFO_HIP(hipGetLastError());
         hipLaunchKernelGGL(denseReplayBatchedKernel, dim3(kReplayGrid), dim3(kReplayBlock), 0, s, dj,
                            int(jobs.size()), bvh.nodes, bvh.tri9, dir, rb.r);
@@ -959,7 +1077,8 @@ void launchSparseFill(const GridSpec& g, const DeviceBvh& bvh, int blockDim, int
     const double mu = tieMargin(g);
     hipLaunchKernelGGL(sparseFillKernel, dim3(queueGrid(units)), dim3(kFillBlock), 0, s, g, bvh.nodes,
                        bvh.tri9, bvh.box6, dir, mu, blockDim, total, bijk, sphi, sn3, rb.r, rb.h,
-                       rb.queue, kBudgets[0], forceReplay() ? 1 : 0);
+                       rb.queue, debugFillBudget() > 0 ? debugFillBudget() : kBudgets[0],
+                       forceReplay() ? 1 : 0);
     SDFO_HIP(hipGetLastError());
     hipLaunchKernelGGL((sparseHeavyKernel<kFrontierSmall, 4>), dim3(heavyGrid()), dim3(256), 0, s, g,
                        bvh.nodes, bvh.tri9, bvh.box6, dir, mu, blockDim, total, bijk, sphi, sn3,

@@ -26,3 +26,11 @@ extern "C" int sdfo_debug_force_replay(int32_t on)
     sdfo::forceReplay() = on != 0;
     return SDFO_OK;
 }
+
+extern "C" int sdfo_debug_fill_budget(int32_t budget, int32_t heavy_cap)
+{
+    if (budget < 0 || heavy_cap < 0) return SDFO_ERR_INVALID_ARGUMENT;
+    sdfo::debugFillBudget() = budget;
+    sdfo::debugHeavyCap() = heavy_cap;
+    return SDFO_OK;
+}

@@ -254,6 +254,10 @@ int fillShapeFor(const GridSpec& g, int64_t ntris);
 // is synchronised), and a test switch that sends every voxel to the replay pass.
 int* replayCounterHost();
 bool& forceReplay();
+// Test overrides (calling thread): fill step budget and batched heavy-list
+// capacity, 0 = the library's choice.
+int& debugFillBudget();
+int& debugHeavyCap();
 
 // Keeps the device's stream-ordered pool memory across synchronisations.
 void keepStreamPoolMemory(int device);

@@ -155,9 +155,15 @@ _sig("sdfo_debug_bvh_export", [_vp, _i64, _vp, _i64, _vp, _vp, _vp, _i32, _vp])
 _sig("sdfo_debug_sort_by_abs", [_vp, _i32, _vp])
 _sig("sdfo_debug_build_phases", [_vp, _i32])
 _sig("sdfo_debug_force_replay", [_i32])
+_sig("sdfo_debug_fill_budget", [_i32, _i32])
 
 BUILD_PHASES = ("setup_grid", "host_bvh", "upload_alloc", "fill_kernel", "hessian_kernel",
                 "mirror_kernels", "total", "replayed_voxels")
+
+
+def debug_fill_budget(budget: int = 0, heavy_cap: int = 0):
+    """Test switch: fill step budget and batched heavy-list capacity (0 = default)."""
+    check(lib.sdfo_debug_fill_budget(int(budget), int(heavy_cap)))
 
 
 def debug_force_replay(on: bool):

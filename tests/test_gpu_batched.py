@@ -63,6 +63,30 @@ def test_batched_build_replay_pass(O, sdfo_gpu):
             assert np.array_equal(got[key], ref[key]), key
 
 
+@pytest.mark.parametrize("budget,cap", [(16, 0), (16, 3)])
+def test_batched_build_frontier_pass(O, sdfo_gpu, budget, cap):
+    """A tiny step budget hands most bricks to the frontier pass (and, past a
+    heavy-list capacity of 3 bricks, to the replay pass); results stay exact."""
+    from sdfo import _lib
+
+    batch = _small_batch()[2:]
+    print(f"[gpu] batched build, budget {budget}, heavy cap {cap}", flush=True)
+    _lib.debug_fill_budget(budget, cap)
+    try:
+        grids = sdfo_gpu.SdfGrid.build_dense_batched([(v, t) for v, t, _ in batch],
+                                                     [_params(r) for _, _, r in batch])
+        replayed = _lib.debug_build_phases()["replayed_voxels"]
+    finally:
+        _lib.debug_fill_budget(0, 0)
+    if cap:
+        assert replayed > 0
+    for (v, t, r), g in zip(batch, grids):
+        ref = O.build_dense(v, t, O.setup_grid(v, 0.1, 0.0, r, r))
+        got = g.download_dense()
+        for key in ("phi", "n", "witness", "tri"):
+            assert np.array_equal(got[key], ref[key]), key
+
+
 def test_batched_build_errors(sdfo_gpu):
     v, t = meshes.cube(0.1)
     with pytest.raises(sdfo_gpu.SdfoError, match="Empty mesh"):

@@ -115,6 +115,43 @@ def test_dense_replay_pass_bit_exact(O, sdfo_gpu, name, mk, res):
         assert np.array_equal(got[key], ref[key]), key
 
 
+@pytest.mark.parametrize("name,mk,res", [DENSE_CASES[2], DENSE_CASES[4]], ids=["torus_48", "gear_64"])
+def test_dense_frontier_pass_bit_exact(O, sdfo_gpu, name, mk, res):
+    """A step budget of 16 hands nearly every brick to the frontier pass."""
+    from sdfo import _lib
+
+    v, t = mk()
+    print(f"[gpu] dense build, budget 16, {name}", flush=True)
+    _lib.debug_fill_budget(16, 0)
+    try:
+        g = sdfo_gpu.SdfGrid.build_dense(v, t, target_resolution=res, max_resolution=res)
+    finally:
+        _lib.debug_fill_budget(0, 0)
+    ref = O.build_dense(v, t, O.setup_grid(v, 0.1, 0.0, res, res))
+    got = g.download_dense()
+    for key in ("tri", "phi", "n", "witness", "H"):
+        assert np.array_equal(got[key], ref[key]), key
+
+
+def test_sparse_frontier_pass_bit_exact(O, sdfo_gpu):
+    from sdfo import _lib
+
+    v, t = meshes.torus(0.05, 0.01, 64, 16)
+    print("[gpu] sparse build, budget 16, torus_64_b8", flush=True)
+    _lib.debug_fill_budget(16, 0)
+    try:
+        g = sdfo_gpu.SdfGrid.build_sparse(v, t, target_resolution=64, max_resolution=64,
+                                          block_dim=8, band_voxels=2, halo_voxels=1)
+    finally:
+        _lib.debug_fill_budget(0, 0)
+    ref = O.build_sparse(v, t, O.setup_grid(v, 0.1, 0.0, 64, 64), block_dim=8, halo=1,
+                         band_voxels=2)
+    got = g.download_sparse()
+    assert np.array_equal(got["active"], ref["active"])
+    for key in ("phi", "n", "H"):
+        assert np.array_equal(got[key], ref[key]), key
+
+
 def test_sparse_replay_pass_bit_exact(O, sdfo_gpu):
     from sdfo import _lib
 
