@@ -193,10 +193,23 @@ __device__ __forceinline__ bool cellCorners(const QGrid& g, const Cell& c, float
 // (phi, h * n) and no cross-derivative terms (SdfOracle.cpp:932-1092).
 // Returns false if a corner sample is missing (sparse).
 template <typename T, bool WithHess>
+__device__ __forceinline__ void tricubicCorners(const QGrid& g, const Cell& c, const float4 (&f)[8],
+                                                Interp<T>& r);
+
+template <typename T, bool WithHess>
 __device__ __forceinline__ bool tricubic(const QGrid& g, const Cell& c, Interp<T>& r)
 {
     float4 f[8];
     if (!cellCorners(g, c, f)) return false;
+    tricubicCorners<T, WithHess>(g, c, f, r);
+    return true;
+}
+
+// The interpolation itself, on corner records already loaded.
+template <typename T, bool WithHess>
+__device__ __forceinline__ void tricubicCorners(const QGrid& g, const Cell& c, const float4 (&f)[8],
+                                                Interp<T>& r)
+{
     const Basis<T> A = hermite<T, WithHess>(T(c.u));
     const Basis<T> B = hermite<T, WithHess>(T(c.v));
     const Basis<T> Cz = hermite<T, WithHess>(T(c.w));
@@ -250,16 +263,27 @@ __device__ __forceinline__ bool tricubic(const QGrid& g, const Cell& c, Interp<T
         r.hess[4] = hvw / (vy * vz);
         r.hess[5] = hww / (vz * vz);
     }
-    return true;
 }
 
 // Trilinear value and analytic gradient (SdfOracle.cpp:551-583); its only
 // non-zero second derivatives are the mixed ones.
 template <typename T, bool WithHess>
+__device__ __forceinline__ void trilinearCorners(const QGrid& g, const Cell& c,
+                                                 const float4 (&f)[8], Interp<T>& r);
+
+template <typename T, bool WithHess>
 __device__ __forceinline__ bool trilinear(const QGrid& g, const Cell& c, Interp<T>& r)
 {
     float4 f[8];
     if (!cellCorners(g, c, f)) return false;
+    trilinearCorners<T, WithHess>(g, c, f, r);
+    return true;
+}
+
+template <typename T, bool WithHess>
+__device__ __forceinline__ void trilinearCorners(const QGrid& g, const Cell& c,
+                                                 const float4 (&f)[8], Interp<T>& r)
+{
     T p[2][2][2];
 #pragma unroll
     for (int q = 0; q < 8; ++q) p[q & 1][(q >> 1) & 1][q >> 2] = T(f[q].x);
@@ -291,7 +315,6 @@ __device__ __forceinline__ bool trilinear(const QGrid& g, const Cell& c, Interp<
         r.hess[4] = ((ey01 - ey00) * iu + (ey11 - ey10) * u) / (vy * vz);
         r.hess[5] = T(0);
     }
-    return true;
 }
 
 } // namespace dev

@@ -226,17 +226,13 @@ struct P3
     TP x, y, z;
 };
 
-template <typename TP, typename T, int Mode>
-__global__ void __launch_bounds__(256)
-queryKernel(QGrid g, const P3<TP>* __restrict__ pts, int64_t n, TP* __restrict__ gap,
-            P3<TP>* __restrict__ grad3, uint8_t* __restrict__ valid, P3<TP>* __restrict__ normal3,
-            P3<TP>* __restrict__ witness3, int32_t* __restrict__ branch)
+template <typename TP, typename T>
+__device__ __forceinline__ void writeOut(int64_t i, const QOut<T>& o, double px, double py,
+                                         double pz, TP* __restrict__ gap,
+                                         P3<TP>* __restrict__ grad3, uint8_t* __restrict__ valid,
+                                         P3<TP>* __restrict__ normal3,
+                                         P3<TP>* __restrict__ witness3, int32_t* __restrict__ branch)
 {
-    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const P3<TP> p = pts[i];
-    const double px = double(p.x), py = double(p.y), pz = double(p.z);
-    const QOut<T> o = queryPoint<T, Mode>(g, px, py, pz);
     gap[i] = TP(o.gap);
     grad3[i] = P3<TP>{TP(o.grad.x), TP(o.grad.y), TP(o.grad.z)};
     valid[i] = o.valid ? 1 : 0;
@@ -251,8 +247,101 @@ queryKernel(QGrid g, const P3<TP>* __restrict__ pts, int64_t n, TP* __restrict__
     if (branch) branch[i] = o.branch;
 }
 
+template <typename TP, typename T, int Mode>
+__global__ void __launch_bounds__(256)
+queryKernel(QGrid g, const P3<TP>* __restrict__ pts, int64_t n, TP* __restrict__ gap,
+            P3<TP>* __restrict__ grad3, uint8_t* __restrict__ valid, P3<TP>* __restrict__ normal3,
+            P3<TP>* __restrict__ witness3, int32_t* __restrict__ branch)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const P3<TP> p = pts[i];
+    const double px = double(p.x), py = double(p.y), pz = double(p.z);
+    writeOut<TP, T>(i, queryPoint<T, Mode>(g, px, py, pz), px, py, pz, gap, grad3, valid, normal3,
+                    witness3, branch);
+}
+
+// Trilinear / Tricubic on a dense grid with cell records, two points per
+// thread: both points' 128-byte records are requested before either is
+// interpolated, doubling the loads in flight per lane (the gather is
+// latency-bound at one point per lane).  Same arithmetic as queryKernel.
+constexpr int kPairBlock = 128;
+
+// Points ia and ib (each processed when below its end) of one grid.
+template <typename TP, typename T, int Mode>
+__device__ __forceinline__ void pairQuery(const QGrid& g, const P3<TP>* __restrict__ pts,
+                                          int64_t ia, int64_t ib, int64_t end,
+                                          TP* __restrict__ gap, P3<TP>* __restrict__ grad3,
+                                          uint8_t* __restrict__ valid, P3<TP>* __restrict__ normal3,
+                                          P3<TP>* __restrict__ witness3,
+                                          int32_t* __restrict__ branch)
+{
+    double pc[2][3];
+    dev::Cell c[2];
+    float4 f[2][8];
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+    {
+        const int64_t i = h ? ib : ia;
+        P3<TP> p{TP(0), TP(0), TP(0)};
+        if (i < end) p = pts[i];
+        pc[h][0] = double(p.x);
+        pc[h][1] = double(p.y);
+        pc[h][2] = double(p.z);
+        c[h] = dev::locateCell(g, pc[h][0], pc[h][1], pc[h][2]);
+        c[h].valid = c[h].valid && i < end;
+        if (c[h].valid)
+        {
+            const float4* r = g.cellRec + 8 * (int64_t(c[h].i0) +
+                                               int64_t(g.n[0] - 1) * (c[h].j0 + int64_t(g.n[1] - 1) * c[h].k0));
+#pragma unroll
+            for (int q = 0; q < 8; ++q) f[h][q] = r[q];
+        }
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+    {
+        const int64_t i = h ? ib : ia;
+        if (i >= end) continue;
+        QOut<T> o;
+        o.gap = T(0);
+        o.grad = V3T<T>{T(0), T(0), T(0)};
+        o.normal = V3T<T>{T(1), T(0), T(0)};
+        o.valid = false;
+        o.branch = -1;
+        if (c[h].valid)
+        {
+            Interp<T> it;
+            if (Mode == SDFO_TRICUBIC)
+                dev::tricubicCorners<T, false>(g, c[h], f[h], it);
+            else
+                dev::trilinearCorners<T, false>(g, c[h], f[h], it);
+            finishInterp<T>(it, o);
+        }
+        writeOut<TP, T>(i, o, pc[h][0], pc[h][1], pc[h][2], gap, grad3, valid, normal3, witness3,
+                        branch);
+    }
+}
+
+// Trilinear / Tricubic on a dense grid with cell records, two points per
+// thread: both points' 128-byte records are requested before either is
+// interpolated, doubling the loads in flight per lane (the gather is
+// latency-bound at one point per lane).  Same arithmetic as queryKernel.
+template <typename TP, typename T, int Mode>
+__global__ void __launch_bounds__(kPairBlock)
+queryPairKernel(QGrid g, const P3<TP>* __restrict__ pts, int64_t n, TP* __restrict__ gap,
+                P3<TP>* __restrict__ grad3, uint8_t* __restrict__ valid,
+                P3<TP>* __restrict__ normal3, P3<TP>* __restrict__ witness3,
+                int32_t* __restrict__ branch)
+{
+    const int64_t i0 = int64_t(blockIdx.x) * (2 * kPairBlock) + threadIdx.x;
+    pairQuery<TP, T, Mode>(g, pts, i0, i0 + kPairBlock, n, gap, grad3, valid, normal3, witness3,
+                           branch);
+}
+
 // One point stream of a batched query: its grid, its points and its range of
-// 64-point work units (one wave each) in the launch.
+// 128-point work units (one wave each, two points per lane) in the launch.
+constexpr int kUnitPoints = 128;
 struct StreamDesc
 {
     QGrid g;
@@ -279,13 +368,29 @@ queryBatchedKernel(const StreamDesc* __restrict__ sd, int ns, int64_t units,
             hi = mid - 1;
     }
     const StreamDesc& D = sd[lo];
-    const int64_t i = D.pointBegin + (u - D.unitBegin) * 64 + (threadIdx.x & 63);
-    if (i >= D.pointEnd) return;
-    const P3<float> p = pts[i];
-    const QOut<float> o = queryPoint<float, Mode>(D.g, double(p.x), double(p.y), double(p.z));
-    gap[i] = o.gap;
-    grad3[i] = P3<float>{o.grad.x, o.grad.y, o.grad.z};
-    valid[i] = o.valid ? 1 : 0;
+    const int64_t i = D.pointBegin + (u - D.unitBegin) * kUnitPoints + (threadIdx.x & 63);
+    const bool cubic = Mode == SDFO_TRICUBIC || (Mode == SDFO_CONTACT_AWARE && D.g.stride <= 0);
+    if (!D.g.sparse && D.g.cellRec && (cubic || Mode == SDFO_TRILINEAR))
+    {
+        if (cubic)
+            pairQuery<float, float, SDFO_TRICUBIC>(D.g, pts, i, i + 64, D.pointEnd, gap, grad3, valid,
+                                                   nullptr, nullptr, nullptr);
+        else
+            pairQuery<float, float, SDFO_TRILINEAR>(D.g, pts, i, i + 64, D.pointEnd, gap, grad3,
+                                                    valid, nullptr, nullptr, nullptr);
+        return;
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+    {
+        const int64_t k = i + 64 * h;
+        if (k >= D.pointEnd) continue;
+        const P3<float> p = pts[k];
+        const QOut<float> o = queryPoint<float, Mode>(D.g, double(p.x), double(p.y), double(p.z));
+        gap[k] = o.gap;
+        grad3[k] = P3<float>{o.grad.x, o.grad.y, o.grad.z};
+        valid[k] = o.valid ? 1 : 0;
+    }
 }
 
 template <int Block>
@@ -334,6 +439,19 @@ void launchQuery(const sdfo_grid& g, int mode, const TP* pts, int64_t n, TP* gap
     V* gr = reinterpret_cast<V*>(grad3);
     V* nm = reinterpret_cast<V*>(normal3);
     V* wt = reinterpret_cast<V*>(witness3);
+    const bool cubic = mode == SDFO_TRICUBIC || (mode == SDFO_CONTACT_AWARE && q.stride <= 0);
+    if (!q.sparse && q.cellRec && (cubic || mode == SDFO_TRILINEAR))
+    {
+        const dim3 pg(unsigned((n + 2 * kPairBlock - 1) / (2 * kPairBlock))), pb(kPairBlock);
+        if (cubic)
+            hipLaunchKernelGGL((queryPairKernel<TP, T, SDFO_TRICUBIC>), pg, pb, 0, s, q, p, n, gap,
+                               gr, valid, nm, wt, branch);
+        else
+            hipLaunchKernelGGL((queryPairKernel<TP, T, SDFO_TRILINEAR>), pg, pb, 0, s, q, p, n, gap,
+                               gr, valid, nm, wt, branch);
+        SDFO_HIP(hipGetLastError());
+        return;
+    }
     switch (mode)
     {
     case SDFO_FIRST_ORDER:
@@ -395,7 +513,7 @@ std::unique_ptr<QueryPlan> makeQueryPlan(int ns, const sdfo_grid* const* grids,
         t[size_t(q)].pointBegin = offsets[q];
         t[size_t(q)].pointEnd = offsets[q + 1];
         t[size_t(q)].unitBegin = units;
-        units += (offsets[q + 1] - offsets[q] + 63) / 64;
+        units += (offsets[q + 1] - offsets[q] + kUnitPoints - 1) / kUnitPoints;
     }
     plan->points = ns > 0 ? offsets[ns] : 0;
     plan->units = units;
