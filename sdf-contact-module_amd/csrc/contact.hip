@@ -117,6 +117,10 @@ __device__ __forceinline__ bool sampleField(const QGrid& g, const Law& L, V Y, I
 
 // Forward terms of one (pose, point) pair (contacts_sdf.py:329-379); with
 // Cached, the field comes from the linearisation cache instead of the grid.
+template <bool WithHess>
+__device__ __forceinline__ void lawTerms(Terms& t, const Pose& P, const Law& L,
+                                         const Interp<double>& it);
+
 template <bool WithHess, bool Cached = false>
 __device__ __forceinline__ Terms pointTerms(const QGrid& g, const Pose& P, V X, const Law& L,
                                             const LinCache* lc = nullptr, int64_t i = 0)
@@ -138,10 +142,19 @@ __device__ __forceinline__ Terms pointTerms(const QGrid& g, const Pose& P, V X, 
     }
     else if (!sampleField<WithHess>(g, L, t.Y, it))
         return t;
+    lawTerms<WithHess>(t, P, L, it);
+    return t;
+}
+
+// The law of one point once the field (gap, raw gradient, Hessian) is known.
+template <bool WithHess>
+__device__ __forceinline__ void lawTerms(Terms& t, const Pose& P, const Law& L,
+                                         const Interp<double>& it)
+{
     t.g = it.gap;
     const double delta = t.g < 0.0 ? -t.g : 0.0;
     t.p = L.kn * powi(delta, L.order);
-    if (!(t.p > 1e-30)) return t; // contacts_sdf.py:329
+    if (!(t.p > 1e-30)) return; // contacts_sdf.py:329
     t.active = true;
     t.dp = (L.order >= 1 && t.g < 0.0) ? -L.kn * double(L.order) * powi(delta, L.order - 1) : 0.0;
     if (WithHess)
@@ -169,7 +182,6 @@ __device__ __forceinline__ Terms pointTerms(const QGrid& g, const Pose& P, V X, 
         t.ft = vmul(s, t.ut);
     }
     t.fq = vadd(vadd(t.fp, t.fd), t.ft);
-    return t;
 }
 
 // Fixed-order LDS tree reduction of NV doubles per lane.
@@ -277,13 +289,7 @@ contactForwardKernel(QGrid g, Law L, const double* __restrict__ X, const double*
         cull = anyLeaf[pose] != 0;
     }
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};
-    for (int64_t i = beg + threadIdx.x; i < end; i += kBlock)
-    {
-        if (Culled && cull && !leafOverlaps(cu.leafBox + 6 * int64_t(cu.leafOf[i]), q, cu.margin))
-            continue;
-        const V x = V{X[3 * i], X[3 * i + 1], X[3 * i + 2]};
-        const Terms t = pointTerms<false, Cached>(g, P, x, L, &lc, i);
-        if (!t.active) continue;
+    auto add = [&](const Terms& t, int64_t i) {
         const double wi = w[i];
         const V tq = vcross(t.off, t.fq);
         acc[0] += wi * t.fq.x;
@@ -293,6 +299,60 @@ contactForwardKernel(QGrid g, Law L, const double* __restrict__ X, const double*
         acc[4] += wi * tq.y;
         acc[5] += wi * tq.z;
         acc[6] += 1.0;
+    };
+    if (!Cached && !g.sparse && g.cellRec)
+    {
+        // two points per step: both cell records are requested before either
+        // is interpolated; points are added in the same order as below
+        for (int64_t i0 = beg + threadIdx.x; i0 < end; i0 += 2 * kBlock)
+        {
+            Terms t[2];
+            dev::Cell c[2];
+            float4 f[2][8];
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+            {
+                const int64_t i = i0 + h * kBlock;
+                t[h].active = false;
+                c[h].valid = false;
+                if (i >= end) continue;
+                if (Culled && cull &&
+                    !leafOverlaps(cu.leafBox + 6 * int64_t(cu.leafOf[i]), q, cu.margin))
+                    continue;
+                const V x = V{X[3 * i], X[3 * i + 1], X[3 * i + 2]};
+                t[h].Y = vadd(matvec(P.R, x), V{P.t[0], P.t[1], P.t[2]});
+                t[h].off = vadd(matvec(P.M, x), V{P.m[0], P.m[1], P.m[2]});
+                if (L.margin > 0.0 && !(fabs(t[h].Y.z) < L.margin)) continue;
+                c[h] = dev::locateCell(g, t[h].Y.x, t[h].Y.y, t[h].Y.z);
+                if (!c[h].valid) continue;
+                const float4* r = g.cellRec + 8 * (int64_t(c[h].i0) +
+                                                   int64_t(g.n[0] - 1) * (c[h].j0 + int64_t(g.n[1] - 1) * c[h].k0));
+#pragma unroll
+                for (int e = 0; e < 8; ++e) f[h][e] = r[e];
+            }
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+            {
+                if (!c[h].valid) continue;
+                Interp<double> it;
+                if (L.mode == SDFO_TRILINEAR)
+                    dev::trilinearCorners<double, false>(g, c[h], f[h], it);
+                else
+                    dev::tricubicCorners<double, false>(g, c[h], f[h], it);
+                lawTerms<false>(t[h], P, L, it);
+                if (t[h].active) add(t[h], i0 + h * kBlock);
+            }
+        }
+    }
+    else
+    for (int64_t i = beg + threadIdx.x; i < end; i += kBlock)
+    {
+        if (Culled && cull && !leafOverlaps(cu.leafBox + 6 * int64_t(cu.leafOf[i]), q, cu.margin))
+            continue;
+        const V x = V{X[3 * i], X[3 * i + 1], X[3 * i + 2]};
+        const Terms t = pointTerms<false, Cached>(g, P, x, L, &lc, i);
+        if (!t.active) continue;
+        add(t, i);
     }
     blockReduce<7>(acc, sh);
     if (threadIdx.x == 0)
