@@ -499,6 +499,63 @@ __global__ void cellPackKernel(GridSpec g, const float4* __restrict__ pn, float4
     rec[t] = pn[i + int64_t(g.n[0]) * (j + int64_t(g.n[1]) * k)];
 }
 
+// Sparse cell records: for the cell whose lowest corner is brick-pool voxel
+// t, the 8 corner records (located through the brick table across brick
+// faces) and one validity bit (all 8 corners exist, the rule of
+// SparseSdfData::sample, SdfData.h:216-247).  One thread per cell; the
+// bits of a wave's 64 cells form one mask word when a brick holds a multiple
+// of 64 voxels, else they are or-ed in.
+__global__ void __launch_bounds__(256)
+sparseCellPackKernel(GridSpec g, int bd, int nbx, int nby, int nbz,
+                     const int32_t* __restrict__ bijk, const int32_t* __restrict__ table,
+                     const float4* __restrict__ spn, int64_t count, int words,
+                     float4* __restrict__ rec, unsigned long long* __restrict__ mask)
+{
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if ((t & ~int64_t(63)) >= count) return; // whole waves
+    const int bvc = bd * bd * bd;
+    bool ok = false;
+    int64_t b = 0;
+    int local = 0;
+    if (t < count)
+    {
+        b = t / bvc;
+        local = int(t % bvc);
+        const int i0 = bijk[3 * b] * bd + local % bd;
+        const int j0 = bijk[3 * b + 1] * bd + (local / bd) % bd;
+        const int k0 = bijk[3 * b + 2] * bd + local / (bd * bd);
+        int64_t r[8];
+        ok = true;
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+        {
+            const int i = i0 + (q & 1), j = j0 + ((q >> 1) & 1), k = k0 + (q >> 2);
+            r[q] = -1;
+            if (i >= 0 && i < g.n[0] && j >= 0 && j < g.n[1] && k >= 0 && k < g.n[2])
+            {
+                const int bi = i / bd, bj = j / bd, bk = k / bd;
+                const int32_t nb = table[bi + int64_t(nbx) * (bj + int64_t(nby) * bk)];
+                if (nb >= 0)
+                    r[q] = int64_t(nb) * bvc + (i - bi * bd) + bd * ((j - bj * bd) + bd * (k - bk * bd));
+            }
+            ok = ok && r[q] >= 0;
+        }
+        if (ok)
+        {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) rec[8 * t + q] = spn[r[q]];
+        }
+    }
+    (void)nbz;
+    if (bvc % 64 == 0)
+    {
+        const unsigned long long m = __ballot(ok);
+        if ((threadIdx.x & 63) == 0 && t < count) mask[t >> 6] = m;
+    }
+    else if (ok)
+        atomicOr(&mask[b * words + local / 64], 1ull << (local % 64));
+}
+
 __global__ void candidateMirrorKernel(int64_t count, const double* __restrict__ cphi,
                                       const double* __restrict__ cn3, float4* __restrict__ cpn)
 {
@@ -1039,6 +1096,19 @@ void launchCellPack(const GridSpec& g, const float4* pn, float4* cellRec, hipStr
     if (cells <= 0) return;
     hipLaunchKernelGGL(cellPackKernel, dim3(blocksFor(cells * 8, 256)), dim3(256), 0, s, g, pn,
                        cellRec);
+    SDFO_HIP(hipGetLastError());
+}
+
+void launchSparseCellPack(const GridSpec& g, int bd, int nbx, int nby, int nbz,
+                          const int32_t* bijk, const int32_t* table, const float4* spn,
+                          int64_t count, int words, float4* rec, unsigned long long* mask,
+                          hipStream_t s)
+{
+    if (count <= 0) return;
+    if ((bd * bd * bd) % 64 != 0)
+        SDFO_HIP(hipMemsetAsync(mask, 0, sizeof(unsigned long long) * size_t(count / (bd * bd * bd)) * words, s));
+    hipLaunchKernelGGL(sparseCellPackKernel, dim3(blocksFor(count, 256)), dim3(256), 0, s, g, bd, nbx,
+                       nby, nbz, bijk, table, spn, count, words, rec, mask);
     SDFO_HIP(hipGetLastError());
 }
 

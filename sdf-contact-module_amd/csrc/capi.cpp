@@ -104,6 +104,14 @@ void finishSparse(sdfo_grid& g, const std::vector<int32_t>& bijk, hipStream_t s)
     {
         if (!g.spn) g.spn = g.alloc<float4>(size_t(count));
         launchSparseMirror(count, g.sphi, g.sn3, g.spn, s);
+        if (count * 8 * int64_t(sizeof(float4)) <= kCellRecordLimit && g.bijk)
+        {
+            const int words = int((bvc + 63) / 64);
+            if (!g.scellRec) g.scellRec = g.alloc<float4>(size_t(count) * 8);
+            if (!g.scellMask) g.scellMask = g.alloc<unsigned long long>(size_t(g.nb) * words);
+            launchSparseCellPack(g.spec, g.blockDim, g.nbx, g.nby, g.nbz, g.bijk, g.table, g.spn,
+                                 count, words, g.scellRec, g.scellMask, s);
+        }
         if (g.stride > 0)
         {
             if (!g.scpn) g.scpn = g.alloc<float4>(size_t(count) * g.stride);

@@ -300,7 +300,7 @@ contactForwardKernel(QGrid g, Law L, const double* __restrict__ X, const double*
         acc[5] += wi * tq.z;
         acc[6] += 1.0;
     };
-    if (!Cached && !g.sparse && g.cellRec)
+    if (!Cached && (g.sparse ? g.scellRec != nullptr : g.cellRec != nullptr))
     {
         // two points per step: both cell records are requested before either
         // is interpolated; points are added in the same order as below
@@ -325,8 +325,12 @@ contactForwardKernel(QGrid g, Law L, const double* __restrict__ X, const double*
                 if (L.margin > 0.0 && !(fabs(t[h].Y.z) < L.margin)) continue;
                 c[h] = dev::locateCell(g, t[h].Y.x, t[h].Y.y, t[h].Y.z);
                 if (!c[h].valid) continue;
-                const float4* r = g.cellRec + 8 * (int64_t(c[h].i0) +
-                                                   int64_t(g.n[0] - 1) * (c[h].j0 + int64_t(g.n[1] - 1) * c[h].k0));
+                const float4* r = dev::cellRecord(g, c[h].i0, c[h].j0, c[h].k0);
+                if (!r)
+                {
+                    c[h].valid = false;
+                    continue;
+                }
 #pragma unroll
                 for (int e = 0; e < 8; ++e) f[h][e] = r[e];
             }

@@ -42,7 +42,29 @@ struct QGrid
     const uint8_t* sccount;
     const float4* scpn;
     const int32_t* scbranch;
+    // sparse, optional: cell records at each brick-pool voxel and their
+    // validity bits (maskWords 64-bit words per brick)
+    const float4* scellRec;
+    const unsigned long long* scellMask;
+    int maskWords;
 };
+
+// The 8 packed corner records of cell c (dense or sparse cell records), or
+// nullptr when the cell has no record because a corner sample is missing.
+// Requires g.cellRec or g.scellRec and a cell inside the grid.
+__device__ __forceinline__ const float4* cellRecord(const QGrid& g, int i0, int j0, int k0)
+{
+    if (!g.sparse)
+        return g.cellRec + 8 * (int64_t(i0) + int64_t(g.n[0] - 1) * (j0 + int64_t(g.n[1] - 1) * k0));
+    const int bd = g.bd;
+    const int bi = i0 / bd, bj = j0 / bd, bk = k0 / bd;
+    const int32_t b = __ldg(&g.table[bi + int64_t(g.nbx) * (bj + int64_t(g.nby) * bk)]);
+    if (b < 0) return nullptr;
+    const int local = (i0 - bi * bd) + bd * ((j0 - bj * bd) + bd * (k0 - bk * bd));
+    const unsigned long long m = __ldg(&g.scellMask[int64_t(b) * g.maskWords + local / 64]);
+    if (!((m >> (local % 64)) & 1ull)) return nullptr;
+    return g.scellRec + 8 * (int64_t(b) * (bd * bd * bd) + local);
+}
 
 template <typename T>
 struct V3T
@@ -171,10 +193,10 @@ __device__ __forceinline__ Basis<T> hermite(T t)
 // The 8 corner records (phi, n) of a cell; false if a sample is missing.
 __device__ __forceinline__ bool cellCorners(const QGrid& g, const Cell& c, float4 (&f)[8])
 {
-    if (g.cellRec)
+    if (g.sparse ? g.scellRec != nullptr : g.cellRec != nullptr)
     {
-        const float4* r =
-            g.cellRec + 8 * (int64_t(c.i0) + int64_t(g.n[0] - 1) * (c.j0 + int64_t(g.n[1] - 1) * c.k0));
+        const float4* r = cellRecord(g, c.i0, c.j0, c.k0);
+        if (!r) return false;
 #pragma unroll
         for (int q = 0; q < 8; ++q) f[q] = r[q];
         return true;

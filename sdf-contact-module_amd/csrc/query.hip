@@ -48,6 +48,9 @@ QGrid makeQGrid(const sdfo_grid& g)
     q.sccount = g.sccount;
     q.scpn = g.scpn;
     q.scbranch = g.scbranch;
+    q.scellRec = g.sparse ? g.scellRec : nullptr;
+    q.scellMask = g.scellMask;
+    q.maskWords = int((int64_t(g.blockDim) * g.blockDim * g.blockDim + 63) / 64);
     return q;
 }
 
@@ -292,10 +295,14 @@ __device__ __forceinline__ void pairQuery(const QGrid& g, const P3<TP>* __restri
         c[h].valid = c[h].valid && i < end;
         if (c[h].valid)
         {
-            const float4* r = g.cellRec + 8 * (int64_t(c[h].i0) +
-                                               int64_t(g.n[0] - 1) * (c[h].j0 + int64_t(g.n[1] - 1) * c[h].k0));
+            const float4* r = dev::cellRecord(g, c[h].i0, c[h].j0, c[h].k0);
+            if (r)
+            {
 #pragma unroll
-            for (int q = 0; q < 8; ++q) f[h][q] = r[q];
+                for (int q = 0; q < 8; ++q) f[h][q] = r[q];
+            }
+            else
+                c[h].valid = false; // a missing corner sample: an invalid result
         }
     }
 #pragma unroll
@@ -370,7 +377,8 @@ queryBatchedKernel(const StreamDesc* __restrict__ sd, int ns, int64_t units,
     const StreamDesc& D = sd[lo];
     const int64_t i = D.pointBegin + (u - D.unitBegin) * kUnitPoints + (threadIdx.x & 63);
     const bool cubic = Mode == SDFO_TRICUBIC || (Mode == SDFO_CONTACT_AWARE && D.g.stride <= 0);
-    if (!D.g.sparse && D.g.cellRec && (cubic || Mode == SDFO_TRILINEAR))
+    if ((D.g.sparse ? D.g.scellRec != nullptr : D.g.cellRec != nullptr) &&
+        (cubic || Mode == SDFO_TRILINEAR))
     {
         if (cubic)
             pairQuery<float, float, SDFO_TRICUBIC>(D.g, pts, i, i + 64, D.pointEnd, gap, grad3, valid,
@@ -440,7 +448,7 @@ void launchQuery(const sdfo_grid& g, int mode, const TP* pts, int64_t n, TP* gap
     V* nm = reinterpret_cast<V*>(normal3);
     V* wt = reinterpret_cast<V*>(witness3);
     const bool cubic = mode == SDFO_TRICUBIC || (mode == SDFO_CONTACT_AWARE && q.stride <= 0);
-    if (!q.sparse && q.cellRec && (cubic || mode == SDFO_TRILINEAR))
+    if ((q.sparse ? q.scellRec != nullptr : q.cellRec != nullptr) && (cubic || mode == SDFO_TRILINEAR))
     {
         const dim3 pg(unsigned((n + 2 * kPairBlock - 1) / (2 * kPairBlock))), pb(kPairBlock);
         if (cubic)

@@ -155,6 +155,10 @@ struct sdfo_grid
     float* scn3 = nullptr;      // nb*B^3*S*3
     int32_t* scbranch = nullptr; // nb*B^3*S
     float4* scpn = nullptr;     // nb*B^3*S (phi, nx, ny, nz) query mirror
+    // sparse cell records (optional, see kCellRecordLimit): the 8 corner
+    // records of the cell at each brick-pool voxel, and a validity bit per cell
+    float4* scellRec = nullptr;           // nb*B^3*8
+    unsigned long long* scellMask = nullptr; // nb * ceil(B^3 / 64) words
 
     int64_t deviceBytes = 0;
     std::vector<void*> allocations;
@@ -291,6 +295,10 @@ struct BatchMesh
 void launchDenseFillBatched(const std::vector<BatchMesh>& meshes, const DeviceBvh& bvh,
                             hipStream_t s);
 void launchCellPack(const GridSpec& g, const float4* pn, float4* cellRec, hipStream_t s);
+void launchSparseCellPack(const GridSpec& g, int bd, int nbx, int nby, int nbz,
+                          const int32_t* bijk, const int32_t* table, const float4* spn,
+                          int64_t count, int words, float4* rec, unsigned long long* mask,
+                          hipStream_t s);
 void launchDenseFill(const GridSpec& g, const DeviceBvh& bvh, double* phi, double* n3,
                      double* witness3, int32_t* tri, hipStream_t s);
 void launchDenseHessian(const GridSpec& g, const double* n3, double* h6, hipStream_t s);
