@@ -183,7 +183,19 @@ def bench_sparse_cylinder(args):
     torch.cuda.synchronize()
     ca_q_ms = a.elapsed_time(b)
     vox = info.block_count * 512
-    alg = 29.0 * n + 16.0 * vox  # points + the touched band read once (upper bound)
+    # Algorithmic bytes: 29 B per point plus every distinct voxel the points'
+    # cells touch, read once (16 B each): the corner voxels of the valid
+    # points' cells, counted on the host with the query's own cell rule.
+    spec = g.spec
+    f = (pts_np.astype(np.float64) - spec.bmin) / spec.voxel - 0.5
+    c0 = np.floor(f).astype(np.int64)
+    nn = np.array(spec.shape, np.int64)
+    ok = np.all((c0 >= 0) & (c0 + 1 < nn), axis=1)
+    c0 = c0[ok]
+    corners = [(c0 + np.array([dx, dy, dz])) for dz in (0, 1) for dy in (0, 1) for dx in (0, 1)]
+    ids = np.concatenate([c[:, 0] + nn[0] * (c[:, 1] + nn[1] * c[:, 2]) for c in corners])
+    touched = int(np.unique(ids).size)
+    alg = 29.0 * n + 16.0 * touched
     line = {
         "metric": "sparse narrow-band build Mvoxel/s; ContactAware value+grad Mpts/s",
         "value": round(n / (q_ms * 1e-3) / 1e6, 3), "unit": "Mpts/s", "n_gpus": 1,
@@ -192,7 +204,8 @@ def bench_sparse_cylinder(args):
         "dtype": "f32 query / f64 build", "data": "synthetic (seeded mesh and points)",
         "config": {"workload": "sparse hollow cylinder 1024x64 (266240 tris) at 512^3, 8^3 bricks, "
                                "band 4, halo 2; 1e6 near-surface ContactAware points",
-                   "active_bricks": int(info.block_count), "query_points": n},
+                   "active_bricks": int(info.block_count), "query_points": n,
+                   "touched_voxels": touched},
         "sparse_build_ms": round(build_s * 1e3, 3),
         "sparse_build_mvoxel_per_s": round(vox / build_s / 1e6, 3),
         "query_kernel_ms": round(q_ms, 4), "valid_fraction": round(frac_valid, 4),
@@ -201,7 +214,8 @@ def bench_sparse_cylinder(args):
         "roofline": {"bound": "hbm", "achieved": round(alg / (q_ms * 1e-3) / 1e9, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(alg / (q_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "queryKernel<float, float, 4> (ContactAware -> sparse Tricubic)",
+                     "kernel": "queryPairKernel<float, float, 3> (ContactAware -> sparse Tricubic "
+                               "on sparse cell records)",
                      "algorithmic_bytes": alg},
     }
     if not args.no_cpu:
