@@ -87,11 +87,13 @@ def test_query_exact_on_separable_cubic(O, sdfo_gpu):
     assert np.abs(grad.cpu().numpy() - df(p[:, 0], p[:, 1], p[:, 2])).max() < 1e-5
 
 
-@pytest.fixture(scope="module")
-def sparse_grid(O, sdfo_gpu):
+# brick sizes 8 (cell-record validity bits one 64-bit word per wave) and 3
+# (27 voxels per brick: bits or-ed into shared words)
+@pytest.fixture(scope="module", params=[8, 3], ids=["b8", "b3"])
+def sparse_grid(O, sdfo_gpu, request):
     v, t = meshes.hollow_cylinder(0.04, 0.03, 0.05, 64, 8)
-    g = sdfo_gpu.SdfGrid.build_sparse(v, t, target_resolution=64, max_resolution=64, block_dim=8,
-                                      band_voxels=4, halo_voxels=2)
+    g = sdfo_gpu.SdfGrid.build_sparse(v, t, target_resolution=64, max_resolution=64,
+                                      block_dim=request.param, band_voxels=4, halo_voxels=2)
     sp = g.download_sparse()
     yield g, sp, v, t
     g.close()
