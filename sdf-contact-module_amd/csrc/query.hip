@@ -4,6 +4,8 @@
 // are gathered from the FP32 mirror (float4 = phi + normal per voxel).
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include <algorithm>
 #include <memory>
 #include <vector>
@@ -83,10 +85,12 @@ __device__ __forceinline__ double tsqrt<double>(double x)
 
 // Unit normal of trilinear / tricubic (SdfOracle.cpp:585-590, 1083-1087).
 template <typename T>
-__device__ __forceinline__ void finishInterp(const Interp<T>& it, QOut<T>& o)
+__device__ __forceinline__ void finishInterp(const Interp<T>& it, QOut<T>& o, bool withNormal = true)
 {
     o.gap = it.gap;
     o.grad = it.grad;
+    o.valid = true;
+    if (!withNormal) return; // the caller writes no normal and no witness
     const T len = tsqrt<T>(it.grad.x * it.grad.x + it.grad.y * it.grad.y + it.grad.z * it.grad.z);
     if (len > T(1e-15))
         o.normal = V3T<T>{it.grad.x / len, it.grad.y / len, it.grad.z / len};
@@ -271,7 +275,7 @@ queryKernel(QGrid g, const P3<TP>* __restrict__ pts, int64_t n, TP* __restrict__
 constexpr int kPairBlock = 128;
 
 // Points ia and ib (each processed when below its end) of one grid.
-template <typename TP, typename T, int Mode>
+template <typename TP, typename T, int Mode, bool Sparse>
 __device__ __forceinline__ void pairQuery(const QGrid& g, const P3<TP>* __restrict__ pts,
                                           int64_t ia, int64_t ib, int64_t end,
                                           TP* __restrict__ gap, P3<TP>* __restrict__ grad3,
@@ -295,8 +299,14 @@ __device__ __forceinline__ void pairQuery(const QGrid& g, const P3<TP>* __restri
         c[h].valid = c[h].valid && i < end;
         if (c[h].valid)
         {
-            const float4* r = dev::cellRecord(g, c[h].i0, c[h].j0, c[h].k0);
-            if (r)
+            if (!Sparse)
+            {
+                const float4* r = g.cellRec + 8 * (int64_t(c[h].i0) +
+                                                   int64_t(g.n[0] - 1) * (c[h].j0 + int64_t(g.n[1] - 1) * c[h].k0));
+#pragma unroll
+                for (int q = 0; q < 8; ++q) f[h][q] = r[q];
+            }
+            else if (const float4* r = dev::cellRecord(g, c[h].i0, c[h].j0, c[h].k0))
             {
 #pragma unroll
                 for (int q = 0; q < 8; ++q) f[h][q] = r[q];
@@ -323,7 +333,7 @@ __device__ __forceinline__ void pairQuery(const QGrid& g, const P3<TP>* __restri
                 dev::tricubicCorners<T, false>(g, c[h], f[h], it);
             else
                 dev::trilinearCorners<T, false>(g, c[h], f[h], it);
-            finishInterp<T>(it, o);
+            finishInterp<T>(it, o, normal3 != nullptr || witness3 != nullptr);
         }
         writeOut<TP, T>(i, o, pc[h][0], pc[h][1], pc[h][2], gap, grad3, valid, normal3, witness3,
                         branch);
@@ -334,7 +344,7 @@ __device__ __forceinline__ void pairQuery(const QGrid& g, const P3<TP>* __restri
 // thread: both points' 128-byte records are requested before either is
 // interpolated, doubling the loads in flight per lane (the gather is
 // latency-bound at one point per lane).  Same arithmetic as queryKernel.
-template <typename TP, typename T, int Mode>
+template <typename TP, typename T, int Mode, bool Sparse>
 __global__ void __launch_bounds__(kPairBlock)
 queryPairKernel(QGrid g, const P3<TP>* __restrict__ pts, int64_t n, TP* __restrict__ gap,
                 P3<TP>* __restrict__ grad3, uint8_t* __restrict__ valid,
@@ -342,7 +352,7 @@ queryPairKernel(QGrid g, const P3<TP>* __restrict__ pts, int64_t n, TP* __restri
                 int32_t* __restrict__ branch)
 {
     const int64_t i0 = int64_t(blockIdx.x) * (2 * kPairBlock) + threadIdx.x;
-    pairQuery<TP, T, Mode>(g, pts, i0, i0 + kPairBlock, n, gap, grad3, valid, normal3, witness3,
+    pairQuery<TP, T, Mode, Sparse>(g, pts, i0, i0 + kPairBlock, n, gap, grad3, valid, normal3, witness3,
                            branch);
 }
 
@@ -381,11 +391,11 @@ queryBatchedKernel(const StreamDesc* __restrict__ sd, int ns, int64_t units,
         (cubic || Mode == SDFO_TRILINEAR))
     {
         if (cubic)
-            pairQuery<float, float, SDFO_TRICUBIC>(D.g, pts, i, i + 64, D.pointEnd, gap, grad3, valid,
-                                                   nullptr, nullptr, nullptr);
+            pairQuery<float, float, SDFO_TRICUBIC, true>(D.g, pts, i, i + 64, D.pointEnd, gap, grad3,
+                                                         valid, nullptr, nullptr, nullptr);
         else
-            pairQuery<float, float, SDFO_TRILINEAR>(D.g, pts, i, i + 64, D.pointEnd, gap, grad3,
-                                                    valid, nullptr, nullptr, nullptr);
+            pairQuery<float, float, SDFO_TRILINEAR, true>(D.g, pts, i, i + 64, D.pointEnd, gap,
+                                                          grad3, valid, nullptr, nullptr, nullptr);
         return;
     }
 #pragma unroll
@@ -451,12 +461,19 @@ void launchQuery(const sdfo_grid& g, int mode, const TP* pts, int64_t n, TP* gap
     if ((q.sparse ? q.scellRec != nullptr : q.cellRec != nullptr) && (cubic || mode == SDFO_TRILINEAR))
     {
         const dim3 pg(unsigned((n + 2 * kPairBlock - 1) / (2 * kPairBlock))), pb(kPairBlock);
-        if (cubic)
-            hipLaunchKernelGGL((queryPairKernel<TP, T, SDFO_TRICUBIC>), pg, pb, 0, s, q, p, n, gap,
-                               gr, valid, nm, wt, branch);
+        auto launch = [&](auto sparseTag) {
+            constexpr bool Sp = decltype(sparseTag)::value;
+            if (cubic)
+                hipLaunchKernelGGL((queryPairKernel<TP, T, SDFO_TRICUBIC, Sp>), pg, pb, 0, s, q, p, n,
+                                   gap, gr, valid, nm, wt, branch);
+            else
+                hipLaunchKernelGGL((queryPairKernel<TP, T, SDFO_TRILINEAR, Sp>), pg, pb, 0, s, q, p, n,
+                                   gap, gr, valid, nm, wt, branch);
+        };
+        if (q.sparse)
+            launch(std::true_type{});
         else
-            hipLaunchKernelGGL((queryPairKernel<TP, T, SDFO_TRILINEAR>), pg, pb, 0, s, q, p, n, gap,
-                               gr, valid, nm, wt, branch);
+            launch(std::false_type{});
         SDFO_HIP(hipGetLastError());
         return;
     }
