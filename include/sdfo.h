@@ -48,6 +48,8 @@ extern "C" {
 typedef struct sdfo_grid sdfo_grid;
 /* Broad-phase AABB tree over a body's quadrature points (geometry/bvh.py). */
 typedef struct sdfo_point_bvh sdfo_point_bvh;
+/* A triangle mesh's BVH on the device: the KdTree of KdTree.h. */
+typedef struct sdfo_mesh sdfo_mesh;
 typedef void* sdfo_stream; /* hipStream_t */
 
 /* BuilderParams / SparseBuilderParams / ContactAwareBuilderParams
@@ -311,6 +313,33 @@ int sdfo_contact_forward_linearized(const double* g0, const double* grad0, const
                                     const sdfo_contact_params* params, const sdfo_point_bvh* bvh,
                                     const double* boxes6, double* out, int32_t* active,
                                     sdfo_stream stream);
+
+/* ---- triangle-mesh distance queries ----------------------------------------- */
+/* KdTree::build (KdTree.h:91-121, 188-242) over a host mesh; the tree is
+ * built on the host exactly as the builders build it and kept on the current
+ * device.  An empty mesh is allowed (every query then reports distance
+ * DBL_MAX and triangle -1, KdTree.h:123-133). */
+int sdfo_mesh_build(const double* verts, int64_t n_verts, const int32_t* tris, int64_t n_tris,
+                    sdfo_mesh** out);
+void sdfo_mesh_free(sdfo_mesh* mesh);
+/* KdTree::triangleCount (KdTree.h:167-170). */
+int sdfo_mesh_triangle_count(const sdfo_mesh* mesh, int64_t* count);
+
+/* KdTree::findNearest (signed_distance = 0, KdTree.h:123-133) or
+ * findSignedNearest (signed_distance != 0, ray-parity sign, KdTree.h:143-165)
+ * for n FP64 device points (n x 3): distance[n] (signed when asked),
+ * closest3[3n], normal3[3n] (the nearest triangle's face normal, (0, 0, 1)
+ * when degenerate) and triangle[n] (device; any may be NULL).  Bit-identical
+ * to the reference's search for each point. */
+int sdfo_mesh_query(const sdfo_mesh* mesh, const double* points, int64_t n,
+                    int32_t signed_distance, double* distance, double* closest3, double* normal3,
+                    int32_t* triangle, sdfo_stream stream);
+
+/* The same on host arrays (copies in and out, synchronises the stream); for
+ * per-point callers such as the C++ KdTree mirror. */
+int sdfo_mesh_query_host(const sdfo_mesh* mesh, const double* points, int64_t n,
+                         int32_t signed_distance, double* distance, double* closest3,
+                         double* normal3, int32_t* triangle, sdfo_stream stream);
 
 /* ---- contact broad phase --------------------------------------------------- */
 /* AABB_BVH(points, leaf_size) (geometry/bvh.py:19-54) over a body's host FP64

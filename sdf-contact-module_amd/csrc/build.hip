@@ -1215,4 +1215,57 @@ void launchSlotBoxes(const double* tri9, int64_t n, double* box6, hipStream_t s)
     SDFO_HIP(hipGetLastError());
 }
 
+namespace {
+
+// KdTree::findNearest / findSignedNearest (KdTree.h:123-165) for a stream of
+// points: one lane per point, the per-lane search of the reference.
+constexpr int kMeshBlock = 128;
+__global__ void __launch_bounds__(kMeshBlock)
+meshQueryKernel(const BvhNode* __restrict__ nodes, int nnodes, const double* __restrict__ tri9,
+                const int32_t* __restrict__ slotTri, D3 dir, int signedDist,
+                const double* __restrict__ pts, int64_t n, double* __restrict__ dist,
+                double* __restrict__ cp3, double* __restrict__ n3, int32_t* __restrict__ tri)
+{
+    __shared__ int stack[kStackDepth * kMeshBlock];
+    const int64_t i = int64_t(blockIdx.x) * kMeshBlock + threadIdx.x;
+    if (i >= n) return;
+    const D3 p = dev::d3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
+    const dev::LaneStack st{stack + threadIdx.x, kMeshBlock};
+    double bestSq = DBL_MAX;
+    int bestSlot = -1, hits = 0;
+    if (nnodes > 0) // an empty tree compares nothing (KdTree.h:123-133)
+    {
+        dev::refNearest(p, nodes, tri9, st, bestSq, bestSlot);
+        if (signedDist) hits = dev::refRayHits(p, dir, nodes, tri9, st);
+    }
+    const dev::SignedNearest r = dev::finishSigned(p, tri9, bestSq, bestSlot, hits);
+    if (dist) dist[i] = signedDist ? r.sd : (bestSlot >= 0 ? sqrt(bestSq) : DBL_MAX);
+    if (cp3)
+    {
+        cp3[3 * i] = r.cp.x;
+        cp3[3 * i + 1] = r.cp.y;
+        cp3[3 * i + 2] = r.cp.z;
+    }
+    if (n3)
+    {
+        n3[3 * i] = r.face.x;
+        n3[3 * i + 1] = r.face.y;
+        n3[3 * i + 2] = r.face.z;
+    }
+    if (tri) tri[i] = bestSlot >= 0 ? slotTri[bestSlot] : -1;
+}
+
+} // namespace
+
+void launchMeshQuery(const DeviceBvh& bvh, bool signedDist, const double* pts, int64_t n,
+                     double* dist, double* cp3, double* n3, int32_t* tri, hipStream_t s)
+{
+    if (n <= 0) return;
+    const D3 dir = dev::d3(bvh.rayDir[0], bvh.rayDir[1], bvh.rayDir[2]);
+    hipLaunchKernelGGL(meshQueryKernel, dim3(unsigned((n + kMeshBlock - 1) / kMeshBlock)),
+                       dim3(kMeshBlock), 0, s, bvh.nodes, bvh.nnodes, bvh.tri9, bvh.triIdx, dir,
+                       signedDist ? 1 : 0, pts, n, dist, cp3, n3, tri);
+    SDFO_HIP(hipGetLastError());
+}
+
 } // namespace sdfo

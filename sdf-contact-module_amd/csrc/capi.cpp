@@ -874,6 +874,90 @@ int sdfo_contact_linearize(const sdfo_grid* grid, const double* X, int64_t n, co
     });
 }
 
+int sdfo_mesh_build(const double* verts, int64_t n_verts, const int32_t* tris, int64_t n_tris,
+                    sdfo_mesh** out)
+{
+    return guarded([&] {
+        if (!out) throw Error(SDFO_ERR_INVALID_ARGUMENT, "out is NULL");
+        *out = nullptr;
+        if (n_verts < 0 || n_tris < 0) throw Error(SDFO_ERR_INVALID_ARGUMENT, "negative size");
+        if ((n_verts > 0 && !verts) || (n_tris > 0 && !tris))
+            throw Error(SDFO_ERR_INVALID_ARGUMENT, "NULL mesh array");
+        if (n_tris > (int64_t(1) << 30)) throw Error(SDFO_ERR_INVALID_ARGUMENT, "too many triangles");
+        auto m = std::make_unique<sdfo_mesh>();
+        m->device = currentDevice();
+        m->ntris = n_tris;
+        const HostBvh hb = buildHostBvh(verts, n_verts, tris, n_tris);
+        m->bvh.upload(hb, nullptr);
+        SDFO_HIP(hipStreamSynchronize(nullptr));
+        *out = m.release();
+    });
+}
+
+void sdfo_mesh_free(sdfo_mesh* mesh) { delete mesh; }
+
+int sdfo_mesh_triangle_count(const sdfo_mesh* mesh, int64_t* count)
+{
+    return guarded([&] {
+        if (!mesh || !count) throw Error(SDFO_ERR_INVALID_ARGUMENT, "NULL argument");
+        *count = mesh->ntris;
+    });
+}
+
+int sdfo_mesh_query(const sdfo_mesh* mesh, const double* points, int64_t n,
+                    int32_t signed_distance, double* distance, double* closest3, double* normal3,
+                    int32_t* triangle, sdfo_stream stream)
+{
+    return guarded([&] {
+        if (!mesh) throw Error(SDFO_ERR_INVALID_ARGUMENT, "mesh is NULL");
+        if (n < 0) throw Error(SDFO_ERR_INVALID_ARGUMENT, "negative size");
+        if (n > 0 && !points) throw Error(SDFO_ERR_INVALID_ARGUMENT, "points is NULL");
+        DeviceGuard dg(mesh->device);
+        launchMeshQuery(mesh->bvh, signed_distance != 0, points, n, distance, closest3, normal3,
+                        triangle, hipStream_t(stream));
+    });
+}
+
+int sdfo_mesh_query_host(const sdfo_mesh* mesh, const double* points, int64_t n,
+                         int32_t signed_distance, double* distance, double* closest3,
+                         double* normal3, int32_t* triangle, sdfo_stream stream)
+{
+    return guarded([&] {
+        if (!mesh) throw Error(SDFO_ERR_INVALID_ARGUMENT, "mesh is NULL");
+        if (n < 0) throw Error(SDFO_ERR_INVALID_ARGUMENT, "negative size");
+        if (n == 0) return;
+        if (!points) throw Error(SDFO_ERR_INVALID_ARGUMENT, "points is NULL");
+        DeviceGuard dg(mesh->device);
+        const hipStream_t s = hipStream_t(stream);
+        const size_t N = size_t(n);
+        // points, distance, closest point, normal, triangle in one device block
+        const size_t oP = 0, oD = oP + 24 * N, oC = oD + 8 * N, oN = oC + 24 * N, oT = oN + 24 * N;
+        char* base = nullptr;
+        SDFO_HIP(hipMallocAsync(reinterpret_cast<void**>(&base), oT + 4 * N, s));
+        try
+        {
+            SDFO_HIP(hipMemcpyAsync(base + oP, points, 24 * N, hipMemcpyHostToDevice, s));
+            launchMeshQuery(mesh->bvh, signed_distance != 0, reinterpret_cast<double*>(base + oP),
+                            n, reinterpret_cast<double*>(base + oD),
+                            reinterpret_cast<double*>(base + oC),
+                            reinterpret_cast<double*>(base + oN),
+                            reinterpret_cast<int32_t*>(base + oT), s);
+            if (distance) SDFO_HIP(hipMemcpyAsync(distance, base + oD, 8 * N, hipMemcpyDeviceToHost, s));
+            if (closest3) SDFO_HIP(hipMemcpyAsync(closest3, base + oC, 24 * N, hipMemcpyDeviceToHost, s));
+            if (normal3) SDFO_HIP(hipMemcpyAsync(normal3, base + oN, 24 * N, hipMemcpyDeviceToHost, s));
+            if (triangle) SDFO_HIP(hipMemcpyAsync(triangle, base + oT, 4 * N, hipMemcpyDeviceToHost, s));
+            SDFO_HIP(hipFreeAsync(base, s));
+            SDFO_HIP(hipStreamSynchronize(s));
+        }
+        catch (...)
+        {
+            (void)hipStreamSynchronize(s);
+            (void)hipFreeAsync(base, s);
+            throw;
+        }
+    });
+}
+
 static CullArgs cullArgs(const sdfo_point_bvh* bvh, const double* boxes6, int64_t n,
                          double margin)
 {

@@ -107,10 +107,26 @@ struct DeviceBvh
 // Slot boxes (bmin3, bmax3 of each slot triangle's vertices) on the device (build.hip).
 void launchSlotBoxes(const double* tri9, int64_t n, double* box6, hipStream_t s);
 
+// Nearest / signed nearest triangle of a point stream (build.hip).
+void launchMeshQuery(const DeviceBvh& bvh, bool signedDist, const double* pts, int64_t n,
+                     double* dist, double* cp3, double* n3, int32_t* tri, hipStream_t s);
+
 // Maximal BVH traversal stack per lane (LDS).  Host rejects deeper trees.
 constexpr int kStackDepth = 32;
 
 } // namespace sdfo
+
+// A triangle mesh's BVH on a device (the KdTree of the C ABI).
+struct sdfo_mesh
+{
+    int device = 0;
+    int64_t ntris = 0;
+    sdfo::DeviceBvh bvh;
+    sdfo_mesh() = default;
+    sdfo_mesh(const sdfo_mesh&) = delete;
+    sdfo_mesh& operator=(const sdfo_mesh&) = delete;
+    ~sdfo_mesh() { bvh.release(); }
+};
 
 // The opaque handle of the C ABI.
 struct sdfo_grid

@@ -7,5 +7,6 @@ fallback: import fails if the library is not built).
 from ._lib import MODES, SdfoError, default_build_params, version  # noqa: F401
 from ._lib import tuner_report, tuner_reset, tuner_set_mode  # noqa: F401
 from .grid import GridSpec, QueryPlan, SdfGrid, setup_grid  # noqa: F401
+from .kdtree import KdTree  # noqa: F401
 
 QueryMode = MODES

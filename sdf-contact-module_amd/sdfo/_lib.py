@@ -141,6 +141,10 @@ _sig("sdfo_contact_forward", [_vp, _vp, _vp, _i64, _vp, _i64, _P(ContactParams),
 _sig("sdfo_contact_linearize", [_vp, _vp, _i64, _vp, _P(ContactParams), _vp, _vp, _vp, _vp])
 _sig("sdfo_contact_forward_linearized", [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _i64,
                                          _P(ContactParams), _vp, _vp, _vp, _vp, _vp])
+_sig("sdfo_mesh_build", [_vp, _i64, _vp, _i64, _P(_vp)])
+_sig("sdfo_mesh_free", [_vp], None)
+_sig("sdfo_mesh_triangle_count", [_vp, _P(_i64)])
+_sig("sdfo_mesh_query", [_vp, _vp, _i64, _i32, _vp, _vp, _vp, _vp, _vp])
 _sig("sdfo_point_bvh_build", [_vp, _i64, _i32, _P(_vp)])
 _sig("sdfo_point_bvh_free", [_vp], None)
 _sig("sdfo_point_bvh_get_info", [_vp, _P(_i64), _P(_i64), _P(_i64)])

@@ -3,6 +3,7 @@
 // <dir>/mesh.obj and <dir>/surface.txt written by tests/test_gpu_cpp_api.py;
 // writes SDFO files and query results there for that test to compare with
 // the Python/C-ABI path.  Exit code 0 when its own checks pass.
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -11,6 +12,7 @@
 #include <string>
 #include <vector>
 
+#include "SdfOracle/KdTree.h"
 #include "SdfOracle/NexDynAdapter.h"
 #include "SdfOracle/SdfBuilder.h"
 #include "SdfOracle/SdfIO.h"
@@ -163,6 +165,33 @@ int main(int argc, char** argv)
         expect(loaded.queryBatch(pts, sdf::QueryMode::Tricubic)[3].gap ==
                    oracle.queryBatch(pts, sdf::QueryMode::Tricubic)[3].gap,
                "loadFromFile queries");
+
+        // ---- exact mesh queries through sdf::KdTree ---------------------------------
+        sdf::KdTree tree;
+        expect(tree.triangleCount() == 0 && tree.findNearest(pts[0]).triangleIndex == -1, "empty KdTree");
+        tree.build(mesh.vertices, mesh.triangles);
+        expect(tree.triangleCount() == int(mesh.triangles.size()), "KdTree triangleCount");
+        const std::vector<sdf::KdTree::SignedResult> sr = tree.findSignedNearestBatch(pts);
+        const std::vector<sdf::ClosestPointResult> nr = tree.findNearestBatch(pts);
+        {
+            std::ofstream out(dir + "/kdtree.bin", std::ios::binary);
+            for (const auto& r : sr)
+            {
+                const double rec[8] = {r.signedDistance, r.closestPoint[0], r.closestPoint[1],
+                                       r.closestPoint[2], r.normal[0], r.normal[1], r.normal[2],
+                                       double(r.triangleIndex)};
+                out.write(reinterpret_cast<const char*>(rec), sizeof(rec));
+            }
+        }
+        bool same = true;
+        for (size_t i = 0; i < pts.size(); ++i)
+            same = same && nr[i].distance == std::abs(sr[i].signedDistance) &&
+                   nr[i].triangleIndex == sr[i].triangleIndex && nr[i].point[0] == sr[i].closestPoint[0] &&
+                   nr[i].faceNormal[2] == sr[i].normal[2];
+        expect(same, "findNearest == |findSignedNearest|");
+        const sdf::KdTree::SignedResult one = tree.findSignedNearest(pts[1]);
+        expect(one.signedDistance == sr[1].signedDistance && one.triangleIndex == sr[1].triangleIndex,
+               "findSignedNearest == batch");
 
         // ---- sparse build, save, reload, query -----------------------------------
         sdf::SparseBuilderParams sp;

@@ -118,3 +118,13 @@ def test_contact_aware_candidates_match_oracle(O, sdfo_gpu, run):
         assert r[i, 0] == gaps[int(np.argmin(np.abs(gaps)))]
     assert np.abs(r[vv, 0] - rq["gap"][vv]).max() <= 1e-12
     assert (r[vv, 9 + 7] >= -1).all()
+
+
+def test_kdtree_matches_oracle(O, run):
+    d, v, t, pts = run["dir"], run["v"], run["t"], run["pts"]
+    r = np.fromfile(d / "kdtree.bin", np.float64).reshape(len(pts), 8)
+    ref = O.signed_nearest(v, t, pts)
+    assert np.array_equal(r[:, 7].astype(np.int32), ref["tri"])
+    assert np.array_equal(r[:, 0], ref["sd"])
+    assert np.array_equal(r[:, 1:4], ref["cp"])
+    assert np.array_equal(r[:, 4:7], ref["face"])
