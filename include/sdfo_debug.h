@@ -46,6 +46,15 @@ int sdfo_debug_force_replay(int32_t on);
  * pass, so tests can check both against the CPU oracle. */
 int sdfo_debug_fill_budget(int32_t budget, int32_t heavy_cap);
 
+/* Test switch for the calling thread: how many merged candidates the
+ * contact-aware builds' first pass keeps per voxel in registers (first: 1, 4
+ * or 8; 0: the library's choice, 4), and how many per voxel the second pass,
+ * whose lists live in device memory, has room for in its first launch
+ * (second: 0 for the library's choice, 64) before a last launch with room for
+ * the longest triangle list.  Small values send many voxels through the later
+ * passes, so tests can check them. */
+int sdfo_debug_ca_merge_cap(int32_t first, int32_t second);
+
 #ifdef __cplusplus
 }
 #endif

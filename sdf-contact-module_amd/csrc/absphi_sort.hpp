@@ -5,23 +5,39 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdint>
 
 namespace sdfo {
 
+// Explicit-stack size of sortPerm (its stack never holds more than
+// 2 floor(log2 n) + 1 ranges) and the largest n of the debug sort entry point.
 constexpr int kMergeCap = 64;
+
+// Array view with a stride between elements (lists interleaved across lanes).
+template <class T>
+struct Strided
+{
+    T* p;
+    int64_t s;
+    __host__ __device__ __forceinline__ T& operator[](int i) const { return p[int64_t(i) * s]; }
+    __host__ __device__ __forceinline__ Strided operator+(int k) const { return Strided{p + int64_t(k) * s, s}; }
+};
 
 // --- std::sort(merged, |a.phi| < |b.phi|) as GNU libstdc++ performs it, on a
 // permutation of the merged entries (SdfBuilder.cpp:348-353).
-struct AbsPhiOrder
+template <class Phi>
+struct AbsPhiOrderT
 {
-    const double* phi;
+    Phi phi;
     __host__ __device__ __forceinline__ bool operator()(int a, int b) const
     {
         return std::fabs(phi[a]) < std::fabs(phi[b]);
     }
 };
+using AbsPhiOrder = AbsPhiOrderT<const double*>;
 
-__host__ __device__ inline void pushHeap(int* v, int hole, int top, int value, AbsPhiOrder lt)
+template <class V, class Lt>
+__host__ __device__ inline void pushHeap(V v, int hole, int top, int value, Lt lt)
 {
     int parent = (hole - 1) / 2;
     while (hole > top && lt(v[parent], value))
@@ -33,7 +49,8 @@ __host__ __device__ inline void pushHeap(int* v, int hole, int top, int value, A
     v[hole] = value;
 }
 
-__host__ __device__ inline void adjustHeap(int* v, int hole, int len, int value, AbsPhiOrder lt)
+template <class V, class Lt>
+__host__ __device__ inline void adjustHeap(V v, int hole, int len, int value, Lt lt)
 {
     const int top = hole;
     int child = hole;
@@ -53,7 +70,8 @@ __host__ __device__ inline void adjustHeap(int* v, int hole, int len, int value,
     pushHeap(v, hole, top, value, lt);
 }
 
-__host__ __device__ inline void heapSortRange(int* v, int n, AbsPhiOrder lt)
+template <class V, class Lt>
+__host__ __device__ inline void heapSortRange(V v, int n, Lt lt)
 {
     if (n >= 2)
     {
@@ -72,14 +90,16 @@ __host__ __device__ inline void heapSortRange(int* v, int n, AbsPhiOrder lt)
     }
 }
 
-__host__ __device__ inline void swapi(int* v, int a, int b)
+template <class V>
+__host__ __device__ inline void swapi(V v, int a, int b)
 {
     const int t = v[a];
     v[a] = v[b];
     v[b] = t;
 }
 
-__host__ __device__ inline void sortPerm(int* v, int n, AbsPhiOrder lt)
+template <class V, class Lt>
+__host__ __device__ inline void sortPerm(V v, int n, Lt lt)
 {
     if (n <= 1) return;
     // introsort loop; sub-ranges are disjoint, so an explicit stack replaces

@@ -6,14 +6,17 @@
 // order by |phi| and keep the first maxCandidates.  FP64 without FMA
 // contraction, so every candidate is the CPU path's bit for bit.
 //
-// The merged list is held per lane with room for kMergeCap entries; a voxel
+// The first pass holds each voxel's merged list in registers, with room for a
+// few entries (fully unrolled, so nothing goes to scratch memory); a voxel
 // that needs more (smooth corner normals over many small triangles) is listed
 // and re-run by a second pass whose lists live in device memory, sized to the
-// longest triangle list, so no voxel is ever truncated.  The |phi| ordering repeats the steps of the
-// std::sort the CPU path uses (GNU libstdc++ introsort: median-of-three
-// partitions above 16 elements, heap sort when the depth budget runs out,
-// then a final insertion sort), so candidates with equal |phi| come out in
-// the same order as on the CPU.
+// longest triangle list, so no voxel is ever truncated.  The |phi| ordering
+// gives what the std::sort of the CPU path gives: up to 16 entries that sort
+// is GNU libstdc++'s insertion sort, a stable sort, which the register pass
+// computes as ranks; the second pass repeats the steps of the whole introsort
+// (median-of-three partitions above 16 elements, heap sort when the depth
+// budget runs out, then a final insertion sort), so candidates with equal
+// |phi| come out in the same order as on the CPU.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -34,9 +37,22 @@ using dev::D3;
 void finishDense(sdfo_grid& g, hipStream_t s);
 void finishSparse(sdfo_grid& g, const std::vector<int32_t>& bijk, hipStream_t s);
 
+int& debugMergeCap()
+{
+    thread_local int cap = 0;
+    return cap;
+}
+
+int& debugSecondCap()
+{
+    thread_local int cap = 0;
+    return cap;
+}
+
 namespace {
 
 constexpr int kBlock = 128;
+constexpr int kSecondCap = 64; // merged entries per voxel in the second pass's first launch
 
 // Triangle record: a, b, c, corner normals n0, n1, n2, face normal (21 doubles).
 struct CTri
@@ -224,41 +240,46 @@ struct CAParams
     int ntris;
 };
 
-// Merged candidate list: entry e is (phi[e], n[e], tri[e]); order[i] is the
-// entry of the i-th candidate after sorting.  Per-lane private arrays on the
-// fast path, a device-memory slice sized to the longest triangle list on the
-// overflow path.
+// Merged candidate list of the second pass: entry e is (phi[e], n[e], tri[e]);
+// order[i] is the entry of the i-th candidate after sorting.  The lists of a
+// launch live in device memory interleaved across lanes (entry e of lane q at
+// e * lanes + q), so a wave's accesses to its lanes' entry e coalesce.
 struct MergeView
 {
-    double *phi, *nx, *ny, *nz;
-    int *tri, *order;
+    Strided<double> phi, nx, ny, nz;
+    Strided<int> tri, order;
     int cap;
     int count;
 };
 
-struct LocalMerge
+// Merged list of the first pass: K entries in registers (every index below is
+// a compile-time constant after unrolling); rank[e] is entry e's place in the
+// |phi| order.
+template <int K>
+struct RegMerge
 {
-    double phi[kMergeCap], nx[kMergeCap], ny[kMergeCap], nz[kMergeCap];
-    int tri[kMergeCap], order[kMergeCap];
-    __device__ __forceinline__ MergeView view() { return MergeView{phi, nx, ny, nz, tri, order, kMergeCap, 0}; }
+    double phi[K], nx[K], ny[K], nz[K];
+    int tri[K], rank[K];
+    int count;
 };
 
 struct MergePool
 {
     double *phi, *nx, *ny, *nz;
     int *tri, *order;
-    int cap;
-    __device__ __forceinline__ MergeView view(int64_t k) const
+    int cap;       // entries per list
+    int64_t lanes; // lists per launch
+    __device__ __forceinline__ MergeView view(int64_t q) const
     {
-        const int64_t o = k * cap;
-        return MergeView{phi + o, nx + o, ny + o, nz + o, tri + o, order + o, cap, 0};
+        return MergeView{{phi + q, lanes}, {nx + q, lanes}, {ny + q, lanes}, {nz + q, lanes},
+                         {tri + q, lanes}, {order + q, lanes}, cap, 0};
     }
 };
 
 __device__ inline void sortByAbsPhi(MergeView& m)
 {
     for (int i = 0; i < m.count; ++i) m.order[i] = i;
-    sortPerm(m.order, m.count, AbsPhiOrder{m.phi});
+    sortPerm(m.order, m.count, AbsPhiOrderT<Strided<double>>{m.phi});
 }
 
 // buildContactCandidatesForPoint (SdfBuilder.cpp:281-359) over the voxel's
@@ -335,6 +356,98 @@ __device__ inline int candidatesFor(D3 p, const CAParams& P, const double* __res
         if (slot >= 0) tryTri(slotTri[slot]);
     }
     sortByAbsPhi(m);
+    return m.count;
+}
+
+// candidatesFor on a register list of K entries.  Returns the merged count, or
+// K + 1 when the list would outgrow K entries or an |phi| is NaN (the ranks
+// below assume an ordered |phi|): such voxels go to the second pass.
+template <int K, class Each>
+__device__ inline int fastCandidates(D3 p, const CAParams& P, const double* __restrict__ t21,
+                                     const Each& each, double minDist, bool active,
+                                     const BvhNode* __restrict__ nodes,
+                                     const double* __restrict__ tri9,
+                                     const int32_t* __restrict__ slotTri, dev::LaneStack st,
+                                     RegMerge<K>& m)
+{
+    const double allowed = minDist + P.tol;
+    m.count = 0;
+    auto tryTri = [&](int t) -> bool {
+        CTri tr;
+        loadCTriVerts(t21, t, tr);
+        const D3 cp = dev::closestPoint(p, tr.a, tr.b, tr.c);
+        const double dist = sqrt(dev::sqnorm(dev::sub(p, cp)));
+        if (!(dist <= allowed)) return true;
+        loadCTriNormals(t21, t, tr);
+        Cand c;
+        finishCandidate(p, tr, cp, c);
+        // mergeContactCandidate: the first entry within the merge angle takes
+        // the candidate if its |phi| is smaller
+        bool merged = false;
+#pragma unroll
+        for (int e = 0; e < K; ++e)
+        {
+            if (!merged && e < m.count &&
+                m.nx[e] * c.normal.x + m.ny[e] * c.normal.y + m.nz[e] * c.normal.z >= P.cosMerge)
+            {
+                merged = true;
+                if (fabs(c.phi) < fabs(m.phi[e]))
+                {
+                    m.phi[e] = c.phi;
+                    m.nx[e] = c.normal.x;
+                    m.ny[e] = c.normal.y;
+                    m.nz[e] = c.normal.z;
+                    m.tri[e] = t;
+                }
+            }
+        }
+        if (merged) return true;
+        if (m.count >= K)
+        {
+            m.count = K + 1;
+            return false;
+        }
+#pragma unroll
+        for (int e = 0; e < K; ++e)
+        {
+            if (e == m.count)
+            {
+                m.phi[e] = c.phi;
+                m.nx[e] = c.normal.x;
+                m.ny[e] = c.normal.y;
+                m.nz[e] = c.normal.z;
+                m.tri[e] = t;
+            }
+        }
+        ++m.count;
+        return true;
+    };
+    each(tryTri);
+    if (m.count > K || !active) return m.count;
+    if (m.count == 0)
+    {
+        double d;
+        const int slot = nearestSlot(p, nodes, tri9, st, d);
+        if (slot >= 0) tryTri(slotTri[slot]);
+    }
+    bool nan = false;
+#pragma unroll
+    for (int e = 0; e < K; ++e) nan = nan || (e < m.count && m.phi[e] != m.phi[e]);
+    if (nan) return K + 1;
+    // place of entry e in a stable sort by |phi|
+#pragma unroll
+    for (int e = 0; e < K; ++e)
+    {
+        int r = 0;
+#pragma unroll
+        for (int f = 0; f < K; ++f)
+        {
+            if (f == e || f >= m.count) continue;
+            const double af = fabs(m.phi[f]), ae = fabs(m.phi[e]);
+            r += (af < ae || (af == ae && f < e)) ? 1 : 0;
+        }
+        m.rank[e] = e < m.count ? r : K;
+    }
     return m.count;
 }
 
@@ -439,9 +552,17 @@ struct SparseOut
 
 struct Overflow
 {
-    int* count;   // voxels whose merged list exceeded the private capacity
-    int64_t* ids; // their ids (dense linear index or brick-pool slot)
+    int* count;       // voxels whose merged list outgrew the pass's capacity
+    int64_t* ids;     // their ids (dense linear index or brick-pool slot)
+    double* minDist;  // their lists' minimum distance (DBL_MAX: empty list)
 };
+
+__device__ __forceinline__ void pushOverflow(const Overflow& ov, int64_t id, double minDist)
+{
+    const int slot = atomicAdd(ov.count, 1);
+    ov.ids[slot] = id;
+    ov.minDist[slot] = minDist;
+}
 
 // Writes one dense voxel (SdfBuilder.cpp:498-519; defaults SdfData.h:52-74).
 __device__ inline void writeDense(const DenseOut& o, int64_t idx, D3 c, const MergeView& m,
@@ -520,6 +641,94 @@ __device__ inline void writeSparse(const SparseOut& o, int64_t t, const MergeVie
         o.scn3[3 * k + 2] = float(m.nz[e]);
         o.scbranch[k] = branchId(dev::d3(m.nx[e], m.ny[e], m.nz[e]));
     }
+}
+
+// writeDense for a register list: entry e goes to slot rank[e].
+template <int K>
+__device__ inline void writeDenseReg(const DenseOut& o, int64_t idx, D3 c, const RegMerge<K>& m,
+                                     const CAParams& P, const double* __restrict__ t21)
+{
+    const int keep = m.count < P.stride ? m.count : P.stride;
+    o.ccount[idx] = uint8_t(keep);
+    double ph0 = 0.0;
+    D3 n0 = dev::d3(1.0, 0.0, 0.0), w0 = dev::d3(0.0, 0.0, 0.0);
+    int t0 = -1;
+#pragma unroll
+    for (int e = 0; e < K; ++e)
+    {
+        if (m.rank[e] == 0 && keep > 0)
+        {
+            ph0 = m.phi[e];
+            n0 = dev::d3(m.nx[e], m.ny[e], m.nz[e]);
+            t0 = m.tri[e];
+        }
+    }
+    if (t0 >= 0)
+    {
+        const CTri tr = loadCTri(t21, t0);
+        w0 = dev::closestPoint(c, tr.a, tr.b, tr.c);
+    }
+    o.phi[idx] = ph0;
+    o.n3[3 * idx] = n0.x;
+    o.n3[3 * idx + 1] = n0.y;
+    o.n3[3 * idx + 2] = n0.z;
+    o.wit3[3 * idx] = w0.x;
+    o.wit3[3 * idx + 1] = w0.y;
+    o.wit3[3 * idx + 2] = w0.z;
+#pragma unroll
+    for (int e = 0; e < K; ++e)
+    {
+        if (m.rank[e] >= keep) continue;
+        const int64_t k = idx * P.stride + m.rank[e];
+        o.cphi[k] = m.phi[e];
+        o.cn3[3 * k] = m.nx[e];
+        o.cn3[3 * k + 1] = m.ny[e];
+        o.cn3[3 * k + 2] = m.nz[e];
+        o.cbranch[k] = branchId(dev::d3(m.nx[e], m.ny[e], m.nz[e]));
+    }
+    for (int q = keep; q < P.stride; ++q)
+    {
+        const int64_t k = idx * P.stride + q;
+        o.cphi[k] = 0.0;
+        o.cn3[3 * k] = 1.0;
+        o.cn3[3 * k + 1] = 0.0;
+        o.cn3[3 * k + 2] = 0.0;
+        o.cbranch[k] = -1;
+    }
+}
+
+// writeSparse for a register list: entry e goes to slot rank[e].
+template <int K>
+__device__ inline void writeSparseReg(const SparseOut& o, int64_t t, const RegMerge<K>& m,
+                                      const CAParams& P)
+{
+    const int keep = m.count < P.stride ? m.count : P.stride;
+    o.sccount[t] = uint8_t(keep);
+    float ph0 = 0.0f, nx0 = 1.0f, ny0 = 0.0f, nz0 = 0.0f;
+#pragma unroll
+    for (int e = 0; e < K; ++e)
+    {
+        if (m.rank[e] >= keep) continue;
+        const int64_t k = t * P.stride + m.rank[e];
+        const float ph = float(m.phi[e]), nx = float(m.nx[e]), ny = float(m.ny[e]),
+                    nz = float(m.nz[e]);
+        o.scphi[k] = ph;
+        o.scn3[3 * k] = nx;
+        o.scn3[3 * k + 1] = ny;
+        o.scn3[3 * k + 2] = nz;
+        o.scbranch[k] = branchId(dev::d3(m.nx[e], m.ny[e], m.nz[e]));
+        if (m.rank[e] == 0)
+        {
+            ph0 = ph;
+            nx0 = nx;
+            ny0 = ny;
+            nz0 = nz;
+        }
+    }
+    o.sphi[t] = ph0;
+    o.sn3[3 * t] = nx0;
+    o.sn3[3 * t + 1] = ny0;
+    o.sn3[3 * t + 2] = nz0;
 }
 
 __device__ __forceinline__ D3 voxelCentre(const GridSpec& g, int i, int j, int k)
@@ -729,9 +938,9 @@ __device__ __forceinline__ double waveMaxD(double v)
 // Runs candidatesFor for one voxel per lane (every lane of the wave calls
 // it): the wave-shared scan when all active lanes share a tile list, the
 // per-lane scan otherwise.
+template <class Run>
 __device__ inline int waveCandidates(D3 c, bool active, int ti, int tj, int tk, double minDist,
-                                     const CAParams& P, const CAData& D, int* buf,
-                                     dev::LaneStack st, MergeView& m)
+                                     const CAParams& P, const CAData& D, int* buf, const Run& run)
 {
     const double r = reachOf(minDist, P, D);
     const double r2 = active ? r * r : -1.0;
@@ -751,18 +960,17 @@ __device__ inline int waveCandidates(D3 c, bool active, int ti, int tj, int tk, 
         each.hi[0] = waveMaxD(active ? c.x + r : -DBL_MAX);
         each.hi[1] = waveMaxD(active ? c.y + r : -DBL_MAX);
         each.hi[2] = waveMaxD(active ? c.z + r : -DBL_MAX);
-        return candidatesFor(c, P, D.t21, each, true, minDist, active, D.nodes, D.tri9, D.slotTri,
-                             st, m);
+        return run(each);
     }
     const ListScan each{voxelList(P, D, ti, tj, tk), D.tbox6, c, r2};
-    return candidatesFor(c, P, D.t21, each, true, minDist, active, D.nodes, D.tri9, D.slotTri, st,
-                         m);
+    return run(each);
 }
 
 // Dense candidate fill (SdfBuilder.cpp:477-534): one lane per voxel, a 4x4x4
-// voxel brick per wave.  Voxels whose merged list outgrows the private arrays
-// go to the overflow list.
-__global__ void __launch_bounds__(kBlock)
+// voxel brick per wave.  Voxels whose merged list outgrows the K register
+// entries go to the overflow list.
+template <int K>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4)))
 denseCandidateKernel(GridSpec g, CAParams P, CAData D, int bx, int by, int bz, DenseOut o,
                      Overflow ov)
 {
@@ -780,39 +988,31 @@ denseCandidateKernel(GridSpec g, CAParams P, CAData D, int bx, int by, int bz, D
     voxelTile(P, i, j, k, ti, tj, tk);
     const double minSq = waveListMinSq(c, inGrid, ti, tj, tk, P, D);
     const double minDist = minSq == DBL_MAX ? DBL_MAX : sqrt(minSq);
-    LocalMerge lm;
-    MergeView m = lm.view();
+    RegMerge<K> m;
     const int cnt = waveCandidates(c, inGrid, ti, tj, tk, minDist, P, D,
-                                   scanBuf + (threadIdx.x >> 6) * kScanBuf,
-                                   dev::LaneStack{stack + threadIdx.x, kBlock}, m);
+                                   scanBuf + (threadIdx.x >> 6) * kScanBuf, [&](const auto& each) {
+                                       return fastCandidates<K>(c, P, D.t21, each, minDist, inGrid,
+                                                                D.nodes, D.tri9, D.slotTri,
+                                                                dev::LaneStack{stack + threadIdx.x, kBlock}, m);
+                                   });
     if (!inGrid) return;
     const int64_t idx = i + int64_t(g.n[0]) * (j + int64_t(g.n[1]) * k);
-    if (cnt > m.cap)
+    if (cnt > K)
     {
-        ov.ids[atomicAdd(ov.count, 1)] = idx;
+        pushOverflow(ov, idx, minDist);
         return;
     }
-    writeDense(o, idx, c, m, P, D.t21);
+    writeDenseReg(o, idx, c, m, P, D.t21);
 }
 
-// Minimum distance over a voxel's whole list by a plain scan (overflow pass).
-__device__ inline double listMinDist(D3 p, const VoxelList& L, const double* __restrict__ t21)
-{
-    double best = DBL_MAX;
-    for (int64_t q = 0; q < L.len; ++q)
-    {
-        const int t = L.list ? L.list[q] : int(q);
-        const CTri tr = loadCTri(t21, t);
-        const double d = sqrt(dev::sqnorm(dev::sub(p, dev::closestPoint(p, tr.a, tr.b, tr.c))));
-        best = d < best ? d : best;
-    }
-    return best;
-}
-
-// Overflow voxels of the dense fill, one lane each, merged lists in device memory.
+// Voxels of the first pass with more merged candidates than it holds, one lane
+// each, merged lists in device memory (pool.cap entries); the minimum distance
+// comes from the first pass.  A voxel outgrowing pool.cap goes on to esc (a
+// later launch with room for the longest list); esc.count is NULL there.
 __global__ void __launch_bounds__(kBlock)
-denseOverflowKernel(GridSpec g, CAParams P, CAData D, const int64_t* __restrict__ ids, int n,
-                    MergePool pool, DenseOut o)
+denseOverflowKernel(GridSpec g, CAParams P, CAData D, const int64_t* __restrict__ ids,
+                    const double* __restrict__ minDists, int n, MergePool pool, DenseOut o,
+                    Overflow esc)
 {
     __shared__ int stack[kStackDepth * kBlock];
     const int q = int(blockIdx.x) * kBlock + threadIdx.x;
@@ -824,12 +1024,17 @@ denseOverflowKernel(GridSpec g, CAParams P, CAData D, const int64_t* __restrict_
     int ti, tj, tk;
     voxelTile(P, i, j, k, ti, tj, tk);
     const VoxelList L = voxelList(P, D, ti, tj, tk);
-    const double minDist = listMinDist(c, L, D.t21);
+    const double minDist = minDists[q];
     const double r = reachOf(minDist, P, D);
     const ListScan each{L, D.tbox6, c, r * r};
     MergeView m = pool.view(q);
-    candidatesFor(c, P, D.t21, each, true, minDist, true, D.nodes, D.tri9, D.slotTri,
-                  dev::LaneStack{stack + threadIdx.x, kBlock}, m);
+    const int cnt = candidatesFor(c, P, D.t21, each, true, minDist, true, D.nodes, D.tri9,
+                                  D.slotTri, dev::LaneStack{stack + threadIdx.x, kBlock}, m);
+    if (cnt > m.cap) // never with room for the longest list (esc.count NULL)
+    {
+        if (esc.count) pushOverflow(esc, idx, minDist);
+        return;
+    }
     writeDense(o, idx, c, m, P, D.t21);
 }
 
@@ -845,7 +1050,8 @@ __device__ __forceinline__ void sparseVoxel(const SparseOut& o, int bd, int64_t 
 }
 
 // Sparse candidate fill (SdfBuilder.cpp:731-803): one lane per brick voxel.
-__global__ void __launch_bounds__(kBlock)
+template <int K>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4)))
 sparseCandidateKernel(GridSpec g, CAParams P, CAData D, int bd, int64_t total, SparseOut o,
                       Overflow ov)
 {
@@ -883,23 +1089,26 @@ sparseCandidateKernel(GridSpec g, CAParams P, CAData D, int bd, int64_t total, S
     voxelTile(P, i, j, k, ti, tj, tk);
     const double minSq = waveListMinSq(c, inGrid, ti, tj, tk, P, D);
     const double minDist = minSq == DBL_MAX ? DBL_MAX : sqrt(minSq);
-    LocalMerge lm;
-    MergeView m = lm.view();
+    RegMerge<K> m;
     const int cnt = waveCandidates(c, inGrid, ti, tj, tk, minDist, P, D,
-                                   scanBuf + (threadIdx.x >> 6) * kScanBuf,
-                                   dev::LaneStack{stack + threadIdx.x, kBlock}, m);
+                                   scanBuf + (threadIdx.x >> 6) * kScanBuf, [&](const auto& each) {
+                                       return fastCandidates<K>(c, P, D.t21, each, minDist, inGrid,
+                                                                D.nodes, D.tri9, D.slotTri,
+                                                                dev::LaneStack{stack + threadIdx.x, kBlock}, m);
+                                   });
     if (!inGrid) return;
-    if (cnt > m.cap)
+    if (cnt > K)
     {
-        ov.ids[atomicAdd(ov.count, 1)] = t;
+        pushOverflow(ov, t, minDist);
         return;
     }
-    writeSparse(o, t, m, P);
+    writeSparseReg(o, t, m, P);
 }
 
 __global__ void __launch_bounds__(kBlock)
 sparseOverflowKernel(GridSpec g, CAParams P, CAData D, int bd, const int64_t* __restrict__ ids,
-                     int n, MergePool pool, SparseOut o)
+                     const double* __restrict__ minDists, int n, MergePool pool, SparseOut o,
+                     Overflow esc)
 {
     __shared__ int stack[kStackDepth * kBlock];
     const int q = int(blockIdx.x) * kBlock + threadIdx.x;
@@ -911,12 +1120,17 @@ sparseOverflowKernel(GridSpec g, CAParams P, CAData D, int bd, const int64_t* __
     int ti, tj, tk;
     voxelTile(P, i, j, k, ti, tj, tk);
     const VoxelList L = voxelList(P, D, ti, tj, tk);
-    const double minDist = listMinDist(c, L, D.t21);
+    const double minDist = minDists[q];
     const double r = reachOf(minDist, P, D);
     const ListScan each{L, D.tbox6, c, r * r};
     MergeView m = pool.view(q);
-    candidatesFor(c, P, D.t21, each, true, minDist, true, D.nodes, D.tri9, D.slotTri,
-                  dev::LaneStack{stack + threadIdx.x, kBlock}, m);
+    const int cnt = candidatesFor(c, P, D.t21, each, true, minDist, true, D.nodes, D.tri9,
+                                  D.slotTri, dev::LaneStack{stack + threadIdx.x, kBlock}, m);
+    if (cnt > m.cap) // never with room for the longest list (esc.count NULL)
+    {
+        if (esc.count) pushOverflow(esc, t, minDist);
+        return;
+    }
     writeSparse(o, t, m, P);
 }
 
@@ -1310,25 +1524,27 @@ sdfo_grid* buildContactAware(const double* verts, int64_t nv, const int32_t* tri
     P.tny = tiles.ny;
     P.tnz = tiles.nz;
     P.ntris = int(nt);
-    Overflow ov{nullptr, nullptr};
+    Overflow ov{nullptr, nullptr, nullptr};
+    // register entries of the first pass (4 unless a test asks for 1 or 8)
+    const int mergeCap = debugMergeCap();
     // The merged list of a voxel holds at most one entry per triangle on its
     // list (the longest tile list, or every triangle without tiles), plus the
     // nearest-triangle fallback when that list is empty.
     int poolCap = int(nt);
     // Re-runs the voxels the first pass listed, with merged lists in device
-    // memory, in batches of at most ~1 GiB of list storage.
-    auto resolveOverflow = [&](auto launch) {
-        int cnt = 0;
-        SDFO_HIP(hipMemcpyAsync(&cnt, ov.count, sizeof(int), hipMemcpyDeviceToHost, s));
-        SDFO_HIP(hipStreamSynchronize(s));
-        if (cnt <= 0) return;
+    // memory: first kSecondCap entries per voxel, then, for the few voxels
+    // that need more, room for the longest list.  Each launch holds at most
+    // ~1 GiB of list storage.
+    auto runLevel = [&](auto launch, const int64_t* ids, const double* md, int cnt, int cap,
+                        const Overflow& esc) {
         const size_t entry = 4 * sizeof(double) + 2 * sizeof(int);
-        const int64_t per = std::max<int64_t>(1, int64_t((size_t(1) << 30) / (entry * size_t(poolCap))));
+        const int64_t per = std::max<int64_t>(1, int64_t((size_t(1) << 30) / (entry * size_t(cap))));
         const int batch = int(std::min<int64_t>(cnt, per));
         DevBuf pb;
-        const size_t e = size_t(batch) * size_t(poolCap);
+        const size_t e = size_t(batch) * size_t(cap);
         MergePool pool{};
-        pool.cap = poolCap;
+        pool.cap = cap;
+        pool.lanes = batch;
         pool.phi = static_cast<double*>(pb.raw(e * sizeof(double)));
         pool.nx = static_cast<double*>(pb.raw(e * sizeof(double)));
         pool.ny = static_cast<double*>(pb.raw(e * sizeof(double)));
@@ -1337,10 +1553,33 @@ sdfo_grid* buildContactAware(const double* verts, int64_t nv, const int32_t* tri
         pool.order = static_cast<int*>(pb.raw(e * sizeof(int)));
         for (int off = 0; off < cnt; off += batch)
         {
-            launch(ov.ids + off, std::min(batch, cnt - off), pool);
+            launch(ids + off, md + off, std::min(batch, cnt - off), pool, esc);
             SDFO_HIP(hipGetLastError());
         }
         SDFO_HIP(hipStreamSynchronize(s));
+    };
+    auto readCount = [&](const int* c) {
+        int n = 0;
+        SDFO_HIP(hipMemcpyAsync(&n, c, sizeof(int), hipMemcpyDeviceToHost, s));
+        SDFO_HIP(hipStreamSynchronize(s));
+        return n;
+    };
+    const int secondCap = debugSecondCap() > 0 ? debugSecondCap() : kSecondCap;
+    auto resolveOverflow = [&](auto launch) {
+        const int cnt = readCount(ov.count);
+        if (cnt <= 0) return;
+        if (poolCap <= secondCap)
+        {
+            runLevel(launch, ov.ids, ov.minDist, cnt, poolCap, Overflow{nullptr, nullptr, nullptr});
+            return;
+        }
+        Overflow esc{buf.up(std::vector<int>{0}),
+                     static_cast<int64_t*>(buf.raw(size_t(cnt) * sizeof(int64_t))),
+                     static_cast<double*>(buf.raw(size_t(cnt) * sizeof(double)))};
+        runLevel(launch, ov.ids, ov.minDist, cnt, secondCap, esc);
+        const int cnt2 = readCount(esc.count);
+        if (cnt2 > 0)
+            runLevel(launch, esc.ids, esc.minDist, cnt2, poolCap, Overflow{nullptr, nullptr, nullptr});
     };
     bool bvhReady = false;
     auto ensureBvh = [&] {
@@ -1423,13 +1662,20 @@ sdfo_grid* buildContactAware(const double* verts, int64_t nv, const int32_t* tri
             const int bx = (G.n[0] + 3) / 4, by = (G.n[1] + 3) / 4, bz = (G.n[2] + 3) / 4;
             const DenseOut o{g->phi, g->n3, g->witness3, g->ccount, g->cphi, g->cn3, g->cbranch};
             ov.ids = static_cast<int64_t*>(buf.raw(size_t(N) * sizeof(int64_t)));
+            ov.minDist = static_cast<double*>(buf.raw(size_t(N) * sizeof(double)));
             ensureBvh();
-            hipLaunchKernelGGL(denseCandidateKernel, dim3(blocksFor(int64_t(bx) * by * bz, kBlock / 64)),
-                               dim3(kBlock), 0, s, G, P, D, bx, by, bz, o, ov);
+            const dim3 grid(blocksFor(int64_t(bx) * by * bz, kBlock / 64));
+            if (mergeCap == 1)
+                hipLaunchKernelGGL(denseCandidateKernel<1>, grid, dim3(kBlock), 0, s, G, P, D, bx, by, bz, o, ov);
+            else if (mergeCap == 8)
+                hipLaunchKernelGGL(denseCandidateKernel<8>, grid, dim3(kBlock), 0, s, G, P, D, bx, by, bz, o, ov);
+            else
+                hipLaunchKernelGGL(denseCandidateKernel<4>, grid, dim3(kBlock), 0, s, G, P, D, bx, by, bz, o, ov);
             SDFO_HIP(hipGetLastError());
-            resolveOverflow([&](const int64_t* ids, int n, const MergePool& pool) {
+            resolveOverflow([&](const int64_t* ids, const double* md, int n, const MergePool& pool,
+                                const Overflow& esc) {
                 hipLaunchKernelGGL(denseOverflowKernel, dim3(blocksFor(n, kBlock)), dim3(kBlock), 0, s,
-                                   G, P, D, ids, n, pool, o);
+                                   G, P, D, ids, md, n, pool, o, esc);
             });
             if (p.store_hessian)
                 launchDenseHessian(G, g->n3, g->h6, s);
@@ -1495,13 +1741,20 @@ sdfo_grid* buildContactAware(const double* verts, int64_t nv, const int32_t* tri
                 g->scbranch = g->alloc<int32_t>(size_t(total) * stride);
                 const SparseOut o{g->bijk, g->sphi, g->sn3, g->sccount, g->scphi, g->scn3, g->scbranch};
                 ov.ids = static_cast<int64_t*>(buf.raw(size_t(total) * sizeof(int64_t)));
+                ov.minDist = static_cast<double*>(buf.raw(size_t(total) * sizeof(double)));
                 ensureBvh();
-                hipLaunchKernelGGL(sparseCandidateKernel, dim3(blocksFor(total, kBlock)), dim3(kBlock), 0,
-                                   s, G, P, D, bd, total, o, ov);
+                const dim3 grid(blocksFor(total, kBlock));
+                if (mergeCap == 1)
+                    hipLaunchKernelGGL(sparseCandidateKernel<1>, grid, dim3(kBlock), 0, s, G, P, D, bd, total, o, ov);
+                else if (mergeCap == 8)
+                    hipLaunchKernelGGL(sparseCandidateKernel<8>, grid, dim3(kBlock), 0, s, G, P, D, bd, total, o, ov);
+                else
+                    hipLaunchKernelGGL(sparseCandidateKernel<4>, grid, dim3(kBlock), 0, s, G, P, D, bd, total, o, ov);
                 SDFO_HIP(hipGetLastError());
-                resolveOverflow([&](const int64_t* ids, int n, const MergePool& pool) {
+                resolveOverflow([&](const int64_t* ids, const double* md, int n, const MergePool& pool,
+                                    const Overflow& esc) {
                     hipLaunchKernelGGL(sparseOverflowKernel, dim3(blocksFor(n, kBlock)), dim3(kBlock), 0,
-                                       s, G, P, D, bd, ids, n, pool, o);
+                                       s, G, P, D, bd, ids, md, n, pool, o, esc);
                 });
             }
             finishSparse(*g, bijk, s);

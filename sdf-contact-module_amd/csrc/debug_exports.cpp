@@ -34,3 +34,12 @@ extern "C" int sdfo_debug_fill_budget(int32_t budget, int32_t heavy_cap)
     sdfo::debugHeavyCap() = heavy_cap;
     return SDFO_OK;
 }
+
+extern "C" int sdfo_debug_ca_merge_cap(int32_t first, int32_t second)
+{
+    if ((first != 0 && first != 1 && first != 4 && first != 8) || second < 0)
+        return SDFO_ERR_INVALID_ARGUMENT;
+    sdfo::debugMergeCap() = first;
+    sdfo::debugSecondCap() = second;
+    return SDFO_OK;
+}

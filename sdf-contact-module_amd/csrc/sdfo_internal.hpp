@@ -277,6 +277,8 @@ bool& forceReplay();
 // Test overrides (calling thread): fill step budget and batched heavy-list
 // capacity, 0 = the library's choice.
 int& debugFillBudget();
+int& debugMergeCap();  // register entries of the contact-aware first pass (0: 4)
+int& debugSecondCap(); // entries per voxel of its second pass's first launch (0: 64)
 int& debugHeavyCap();
 
 // Keeps the device's stream-ordered pool memory across synchronisations.

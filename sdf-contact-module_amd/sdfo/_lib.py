@@ -160,6 +160,7 @@ _sig("sdfo_debug_sort_by_abs", [_vp, _i32, _vp])
 _sig("sdfo_debug_build_phases", [_vp, _i32])
 _sig("sdfo_debug_force_replay", [_i32])
 _sig("sdfo_debug_fill_budget", [_i32, _i32])
+_sig("sdfo_debug_ca_merge_cap", [_i32, _i32])
 
 BUILD_PHASES = ("setup_grid", "host_bvh", "upload_alloc", "fill_kernel", "hessian_kernel",
                 "mirror_kernels", "total", "replayed_voxels")
@@ -168,6 +169,12 @@ BUILD_PHASES = ("setup_grid", "host_bvh", "upload_alloc", "fill_kernel", "hessia
 def debug_fill_budget(budget: int = 0, heavy_cap: int = 0):
     """Test switch: fill step budget and batched heavy-list capacity (0 = default)."""
     check(lib.sdfo_debug_fill_budget(int(budget), int(heavy_cap)))
+
+
+def debug_ca_merge_cap(first: int = 0, second: int = 0):
+    """Test switch: merged entries per voxel in the contact-aware first pass (1, 4, 8)
+    and in the second pass's first launch (0 = defaults)."""
+    check(lib.sdfo_debug_ca_merge_cap(int(first), int(second)))
 
 
 def debug_force_replay(on: bool):

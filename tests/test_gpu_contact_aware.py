@@ -34,8 +34,8 @@ CASES = [
      dict(use_tile_acceleration=0, max_candidates=3)),
     ("sphere_tol_16", lambda: meshes.uv_sphere(0.05, 8, 16), vertex_corner_normals, 16,
      dict(feature_distance_tolerance=0.004, normal_merge_angle_degrees=20.0, tile_dim=4)),
-    # a wide tolerance puts more than 32 triangles within reach of many voxels:
-    # those take the second, whole-list pass
+    # a wide tolerance and a narrow merge angle leave more merged candidates
+    # than the first pass keeps in registers: those voxels take the second pass
     ("torus_wide_tol_16", lambda: meshes.torus(0.05, 0.01, 48, 16), vertex_corner_normals, 16,
      dict(feature_distance_tolerance=0.02, normal_merge_angle_degrees=2.0, max_candidates=8)),
 ]
@@ -119,6 +119,44 @@ def test_sparse_contact_aware_bit_exact(O, sdfo_gpu, name, mk, nf, res, params):
     rq = O.query_sparse(ograd(O, g.spec), ref, "ContactAware", pts.astype(np.float64))
     assert np.array_equal(valid.cpu().numpy(), rq["valid"])
     assert np.abs(gap.cpu().numpy() - rq["gap"]).max() <= 1e-6 * g.spec.diagonal
+
+
+@pytest.mark.parametrize("cap", [(1, 0), (8, 0), (1, 2)], ids=["1-64", "8-64", "1-2"])
+@pytest.mark.parametrize("idx", [1, 2, 5])
+def test_contact_aware_merge_capacity(O, sdfo_gpu, cap, idx):
+    """Other capacities of the passes that hold merged candidates (first pass in
+    registers, second pass in device memory, then room for the longest list):
+    voxels move between the passes, results stay exact."""
+    from sdfo import _lib
+
+    name, mk, nf, res, params = CASES[idx]
+    v, t = mk()
+    cn = nf(v, t)
+    print(f"[gpu] contact-aware merge capacity {cap} {name}", flush=True)
+    _lib.debug_ca_merge_cap(*cap)
+    try:
+        g = sdfo_gpu.SdfGrid.build_contact_aware(v, t, cn, target_resolution=res,
+                                                 max_resolution=res, **params)
+        sp = dict(band_voxels=2, **params)
+        gs = sdfo_gpu.SdfGrid.build_contact_aware(v, t, cn, sparse=True, target_resolution=res,
+                                                  max_resolution=res, **sp)
+    finally:
+        _lib.debug_ca_merge_cap(0, 0)
+    og = O.setup_grid(v, 0.1, 0.0, res, res)
+    ref = O.build_contact_aware_dense(v, t, cn, og, **_oracle_kw(params))
+    got = g.download_dense()
+    S = ref["stride"]
+    assert np.array_equal(got["candidates"]["count"], ref["ccount"])
+    assert np.array_equal(got["candidates"]["phi"], ref["cphi"][:, :S])
+    assert np.array_equal(got["candidates"]["n"], ref["cn"][:, :S])
+    assert np.array_equal(got["candidates"]["branch"], ref["cbranch"][:, :S])
+    for k in ("phi", "n", "witness"):
+        assert np.array_equal(got[k], ref[k]), k
+    refs = O.build_contact_aware_sparse(v, t, cn, og, block_dim=8, halo=2, band_voxels=2,
+                                        **_oracle_kw(params))
+    gots = gs.download_sparse()
+    for k in ("bijk", "phi", "n", "ccount", "cphi", "cn", "cbranch"):
+        assert np.array_equal(gots[k], refs[k]), k
 
 
 def test_contact_aware_errors(sdfo_gpu):
