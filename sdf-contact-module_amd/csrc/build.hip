@@ -577,30 +577,71 @@ __device__ __forceinline__ D3 brickCentre(const GridSpec& g, int bd, int nbx, in
                    g.bmin[2] + g.voxel[2] * (0.5 * double(k0 + k1)));
 }
 
+// Whether some triangle lies within distance a of p, for each lane of a wave
+// (every lane calls it).  The brick test |findSignedNearest(c)| <= a
+// (SdfBuilder.cpp:887-889) compares the nearest distance sqrt(bestSq), the
+// smallest sqrt(sq) over the triangles, so it holds exactly when one
+// triangle's sqrt(sq) <= a: neither the sign (ray parity) nor which of several
+// nearest triangles wins matters.  One shared stack (entry e in lane e), a
+// node visited while some lane without a hit may reach it; a lane prunes boxes
+// farther than a + mu (mu is far above the rounding of box distances).
+__device__ bool waveWithin(D3 p, bool active, double a, double mu,
+                           const BvhNode* __restrict__ nodes, const double* __restrict__ tri9,
+                           const double* __restrict__ box6)
+{
+    const int lane = threadIdx.x & 63;
+    const double reach = a + mu, reach2 = reach * reach;
+    bool found = false;
+    int stk = 0;
+    int sp = 1;
+    while (sp > 0 && __ballot(active && !found) != 0ull)
+    {
+        --sp;
+        const int ni = __builtin_amdgcn_readlane(stk, sp);
+        const BvhNode nd = nodes[ni];
+        const bool need = active && !found && dev::boxSq(p, nd) <= reach2;
+        if (__ballot(need) == 0ull) continue;
+        if (nd.triCount > 0)
+        {
+            for (int q = nd.triStart; q < nd.triStart + nd.triCount; ++q)
+            {
+                const bool mem = need && !found && dev::boxSq6(p, box6 + 6 * size_t(q)) <= reach2;
+                if (__ballot(mem) == 0ull) continue;
+                const dev::Tri tr = dev::loadTri(tri9, q);
+                if (mem && sqrt(dev::sqnorm(dev::sub(p, dev::closestPoint(p, tr.a, tr.b, tr.c)))) <= a)
+                    found = true;
+            }
+            continue;
+        }
+        // the child most lanes find nearer is popped first
+        const bool rf = need && dev::boxSq(p, nodes[nd.right]) < dev::boxSq(p, nodes[nd.left]);
+        const bool rightFirst = 2 * __popcll(__ballot(rf)) > __popcll(__ballot(need));
+        if (lane == sp) stk = rightFirst ? nd.left : nd.right;
+        if (lane == sp + 1) stk = rightFirst ? nd.right : nd.left;
+        sp += 2;
+    }
+    return found;
+}
+
 __global__ void __launch_bounds__(kFillBlock)
 sparseActivateKernel(GridSpec g, const BvhNode* __restrict__ nodes,
-                     const double* __restrict__ tri9, const double* __restrict__ box6, D3 dir,
-                     double mu, int bd, int nbx, int nby, int nbz, double activation,
-                     uint8_t* __restrict__ mask, Replay replay, unsigned long long* queue, int force)
+                     const double* __restrict__ tri9, const double* __restrict__ box6, double mu,
+                     int bd, int nbx, int nby, int nbz, double activation,
+                     uint8_t* __restrict__ mask, Replay replay, int force)
 {
-    __shared__ __align__(16) unsigned char lds[kFillWaves * dev::kWaveStackBytes];
     const int64_t ncell = int64_t(nbx) * nby * nbz;
-    const int64_t nunits = (ncell + 63) / 64;
-    const int lane = threadIdx.x & 63;
-    const dev::WaveStacks ws = waveStacks(lds);
-    for (int64_t u = nextUnit(queue); u < nunits; u = nextUnit(queue))
+    const int64_t bl0 = int64_t(blockIdx.x) * kFillBlock + threadIdx.x;
+    if ((bl0 & ~int64_t(63)) >= ncell) return; // whole waves only
+    const bool active = bl0 < ncell;
+    const int64_t bl = active ? bl0 : ncell - 1;
+    const D3 c = brickCentre(g, bd, nbx, nby, bl);
+    if (force)
     {
-        const int64_t bl0 = u * 64 + lane;
-        const bool active = bl0 < ncell;
-        const int64_t bl = active ? bl0 : ncell - 1;
-        const D3 c = brickCentre(g, bd, nbx, nby, bl);
-        const dev::WaveSigned w = dev::waveSignedNearest(c, active, nodes, tri9, box6, dir, mu, ws);
-        if (!active) continue;
-        if (!w.exact || force)
-            pushReplay(replay, bl);
-        else
-            mask[bl] = fabs(w.s.sd) <= activation ? 1 : 0;
+        if (active) pushReplay(replay, bl);
+        return;
     }
+    const bool hit = waveWithin(c, active, activation, mu, nodes, tri9, box6);
+    if (active) mask[bl] = hit ? 1 : 0;
 }
 
 __global__ void __launch_bounds__(kReplayBlock)
@@ -1127,9 +1168,9 @@ void launchSparseActivate(const GridSpec& g, const DeviceBvh& bvh, int blockDim,
     const int64_t nb = int64_t(nbx) * nby * nbz;
     const D3 dir = dev::d3(bvh.rayDir[0], bvh.rayDir[1], bvh.rayDir[2]);
     ReplayBuf rb(nb, s);
-    hipLaunchKernelGGL(sparseActivateKernel, dim3(queueGrid((nb + 63) / 64)), dim3(kFillBlock), 0,
-                       s, g, bvh.nodes, bvh.tri9, bvh.box6, dir, tieMargin(g), blockDim, nbx, nby,
-                       nbz, activation, mask, rb.r, rb.queue, forceReplay() ? 1 : 0);
+    hipLaunchKernelGGL(sparseActivateKernel, dim3(blocksFor(nb, kFillBlock)), dim3(kFillBlock), 0,
+                       s, g, bvh.nodes, bvh.tri9, bvh.box6, tieMargin(g), blockDim, nbx, nby, nbz,
+                       activation, mask, rb.r, forceReplay() ? 1 : 0);
     SDFO_HIP(hipGetLastError());
     hipLaunchKernelGGL(sparseActivateReplayKernel, dim3(kReplayGrid), dim3(kReplayBlock), 0, s, g,
                        bvh.nodes, bvh.tri9, dir, blockDim, nbx, nby, activation, rb.r, mask);
