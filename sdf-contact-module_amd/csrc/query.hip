@@ -142,7 +142,7 @@ __device__ __forceinline__ void taylorQuery(const QGrid& g, double px, double py
 // arrays -> tricubic; zero candidates at the voxel -> first order.
 template <typename T>
 __device__ __forceinline__ void contactAwareQuery(const QGrid& g, double px, double py,
-                                                  double pz, QOut<T>& o)
+                                                  double pz, QOut<T>& o, bool wantBranch = true)
 {
     int i, j, k;
     if (!dev::nearestVoxel(g, px, py, pz, i, j, k)) return;
@@ -163,6 +163,7 @@ __device__ __forceinline__ void contactAwareQuery(const QGrid& g, double px, dou
     const T dy = T(py - (g.bmin[1] + g.voxel[1] * (j + 0.5)));
     const T dz = T(pz - (g.bmin[2] + g.voxel[2] * (k + 0.5)));
     T best = T(0);
+    int bestC = 0;
     bool have = false;
     for (int c = 0; c < count; ++c)
     {
@@ -189,24 +190,31 @@ __device__ __forceinline__ void contactAwareQuery(const QGrid& g, double px, dou
             best = ag;
             o.gap = gap;
             o.normal = V3T<T>{nx, ny, nz};
-            o.branch = cb[rec * g.stride + c];
+            bestC = c;
         }
     }
+    if (wantBranch) o.branch = cb[rec * g.stride + bestC];
     o.grad = o.normal;
     o.valid = true;
 }
 
+// ContactAware on a grid known to hold candidate arrays (no tricubic branch to
+// carry); a kernel specialisation, not a query mode of the interface.
+constexpr int kContactAwareCand = 100;
+
 template <typename T, int Mode>
-__device__ __forceinline__ QOut<T> queryPoint(const QGrid& g, double px, double py, double pz)
+__device__ __forceinline__ QOut<T> queryPoint(const QGrid& g, double px, double py, double pz,
+                                              bool wantBranch = true)
 {
-    constexpr int mode = Mode;
+    constexpr int mode = Mode == kContactAwareCand ? SDFO_CONTACT_AWARE : Mode;
     QOut<T> o;
     o.gap = T(0);
     o.grad = V3T<T>{T(0), T(0), T(0)};
     o.normal = V3T<T>{T(1), T(0), T(0)};
     o.valid = false;
     o.branch = -1;
-    const bool cubic = mode == SDFO_TRICUBIC || (mode == SDFO_CONTACT_AWARE && g.stride <= 0);
+    const bool cubic = mode == SDFO_TRICUBIC ||
+                       (Mode == SDFO_CONTACT_AWARE && mode == SDFO_CONTACT_AWARE && g.stride <= 0);
     if (mode == SDFO_TRILINEAR || cubic)
     {
         const Cell c = dev::locateCell(g, px, py, pz);
@@ -218,7 +226,7 @@ __device__ __forceinline__ QOut<T> queryPoint(const QGrid& g, double px, double 
     }
     if (mode == SDFO_CONTACT_AWARE)
     {
-        contactAwareQuery<T>(g, px, py, pz, o);
+        contactAwareQuery<T>(g, px, py, pz, o, wantBranch);
         return o;
     }
     taylorQuery<T>(g, px, py, pz, mode != SDFO_FIRST_ORDER, o);
@@ -264,8 +272,8 @@ queryKernel(QGrid g, const P3<TP>* __restrict__ pts, int64_t n, TP* __restrict__
     if (i >= n) return;
     const P3<TP> p = pts[i];
     const double px = double(p.x), py = double(p.y), pz = double(p.z);
-    writeOut<TP, T>(i, queryPoint<T, Mode>(g, px, py, pz), px, py, pz, gap, grad3, valid, normal3,
-                    witness3, branch);
+    writeOut<TP, T>(i, queryPoint<T, Mode>(g, px, py, pz, branch != nullptr), px, py, pz, gap, grad3,
+                    valid, normal3, witness3, branch);
 }
 
 // Trilinear / Tricubic on a dense grid with cell records, two points per
@@ -496,8 +504,12 @@ void launchQuery(const sdfo_grid& g, int mode, const TP* pts, int64_t n, TP* gap
                            valid, nm, wt, branch);
         break;
     default:
-        hipLaunchKernelGGL((queryKernel<TP, T, SDFO_CONTACT_AWARE>), grid, block, 0, s, q, p, n,
-                           gap, gr, valid, nm, wt, branch);
+        if (q.stride > 0)
+            hipLaunchKernelGGL((queryKernel<TP, T, kContactAwareCand>), grid, block, 0, s, q, p, n,
+                               gap, gr, valid, nm, wt, branch);
+        else
+            hipLaunchKernelGGL((queryKernel<TP, T, SDFO_CONTACT_AWARE>), grid, block, 0, s, q, p, n,
+                               gap, gr, valid, nm, wt, branch);
         break;
     }
     SDFO_HIP(hipGetLastError());
