@@ -309,8 +309,8 @@ std::vector<std::unique_ptr<sdfo_grid>> buildDenseBatched(int32_t nm, const doub
     std::unique_ptr<double[]> allTri9(new double[size_t(std::max<int64_t>(1, nAllSlots)) * 9]);
     std::unique_ptr<int32_t[]> allSlotTri(new int32_t[size_t(std::max<int64_t>(1, nAllSlots))]);
     std::vector<int> depths(static_cast<size_t>(nm), 0);
-    // meshes are independent: a pool of host threads, largest first, each
-    // mesh built sequentially (no nested threads)
+    // meshes are independent: the host worker threads take them largest
+    // first, and a mesh's large subtrees go to idle workers
     std::vector<int32_t> order(static_cast<size_t>(nm));
     for (int32_t m = 0; m < nm; ++m) order[size_t(m)] = m;
     std::sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
@@ -329,7 +329,7 @@ std::vector<std::unique_ptr<sdfo_grid>> buildDenseBatched(int32_t nm, const doub
                 depths[size_t(m)] = buildHostBvhInto(
                     verts + 3 * vo[m], vo[m + 1] - vo[m], tris + 3 * to[m], to[m + 1] - to[m],
                     allNodes.get() + nb, allTri9.get() + 9 * sb, allSlotTri.get() + sb, int(nb),
-                    int(sb), false);
+                    int(sb), true);
             }
             catch (const Error& e)
             {
@@ -343,20 +343,32 @@ std::vector<std::unique_ptr<sdfo_grid>> buildDenseBatched(int32_t nm, const doub
             }
         }
     };
-    std::vector<std::thread> pool;
+    const int nth = std::max(1, std::min<int>(nm, hostThreads()));
+    std::vector<PoolTask> pool(size_t(nth - 1));
     struct PoolJoin
     {
-        std::vector<std::thread>& p;
+        std::vector<PoolTask>& p;
         ~PoolJoin()
         {
-            for (std::thread& t : p)
-                if (t.joinable()) t.join();
+            for (PoolTask& t : p)
+            {
+                try
+                {
+                    if (t.fn) poolWait(&t);
+                }
+                catch (...)
+                {
+                }
+            }
         }
     } poolJoin{pool};
     ph[0] = msSince(t0); // validation and grid set-ups
     const clk::time_point tPool = clk::now();
-    const int nth = std::max(1, std::min<int>(nm, hostThreads()));
-    for (int t = 1; t < nth; ++t) pool.emplace_back(work);
+    for (PoolTask& t : pool)
+    {
+        t.fn = work;
+        poolSubmit(&t);
+    }
 
     // meanwhile this thread allocates every grid's device arrays
     std::vector<std::unique_ptr<sdfo_grid>> grids(static_cast<size_t>(nm));
@@ -384,7 +396,7 @@ std::vector<std::unique_ptr<sdfo_grid>> buildDenseBatched(int32_t nm, const doub
     }
     ph[2] = msSince(tPool); // allocations, overlapping the BVHs
     work();
-    for (std::thread& t : pool) t.join();
+    for (PoolTask& t : pool) poolWait(&t);
     for (int32_t m = 0; m < nm; ++m)
         if (codes[size_t(m)] != SDFO_OK)
             throw Error(codes[size_t(m)], "mesh " + std::to_string(m) + ": " + errs[size_t(m)]);

@@ -3,7 +3,10 @@
 // per-voxel work is on the GPU (build.hip).
 #include <algorithm>
 #include <cmath>
+#include <condition_variable>
+#include <deque>
 #include <limits>
+#include <mutex>
 #include <numeric>
 #include <thread>
 
@@ -68,6 +71,83 @@ namespace {
 // n / 2, so the tree shape depends on n only.
 int64_t subtreeNodes(int64_t n) { return n <= 8 ? 1 : 1 + subtreeNodes(n / 2) + subtreeNodes(n - n / 2); }
 
+// Persistent worker threads (created on first use, kept for the process):
+// tasks wait in one queue; a thread waiting for a task runs queued ones
+// meanwhile, so nested submissions never deadlock.
+class WorkPool
+{
+public:
+    static WorkPool& get()
+    {
+        // never destroyed: the workers sleep on the queue until the process ends
+        static WorkPool* p = new WorkPool(std::max(0, std::min(16, hostThreads()) - 1));
+        return *p;
+    }
+    void submit(PoolTask* t)
+    {
+        {
+            std::lock_guard<std::mutex> l(m_);
+            q_.push_back(t);
+        }
+        cv_.notify_one();
+    }
+    void wait(PoolTask* t)
+    {
+        while (!t->done.load(std::memory_order_acquire))
+        {
+            PoolTask* o = tryPop();
+            if (o)
+                run(o);
+            else
+                std::this_thread::yield();
+        }
+        if (t->error) std::rethrow_exception(t->error);
+    }
+
+private:
+    explicit WorkPool(int n)
+    {
+        for (int i = 0; i < n; ++i) std::thread([this] { loop(); }).detach();
+    }
+    PoolTask* tryPop()
+    {
+        std::lock_guard<std::mutex> l(m_);
+        if (q_.empty()) return nullptr;
+        PoolTask* t = q_.front();
+        q_.pop_front();
+        return t;
+    }
+    static void run(PoolTask* t)
+    {
+        try
+        {
+            t->fn();
+        }
+        catch (...)
+        {
+            t->error = std::current_exception();
+        }
+        t->done.store(true, std::memory_order_release);
+    }
+    void loop()
+    {
+        for (;;)
+        {
+            PoolTask* t;
+            {
+                std::unique_lock<std::mutex> l(m_);
+                cv_.wait(l, [&] { return !q_.empty(); });
+                t = q_.front();
+                q_.pop_front();
+            }
+            run(t);
+        }
+    }
+    std::mutex m_;
+    std::condition_variable cv_;
+    std::deque<PoolTask*> q_;
+};
+
 // Runs f(begin, end) over [0, n) in chunks on up to 16 threads (small
 // ranges, and sequential builds, on the calling thread).
 template <typename F>
@@ -80,15 +160,18 @@ void parallelFor(int n, F f, bool parallel = true)
         f(0, n);
         return;
     }
-    std::vector<std::thread> pool;
     const int step = (n + chunks - 1) / chunks;
+    std::vector<PoolTask> tasks(size_t(chunks - 1));
     for (int c = 1; c < chunks; ++c)
     {
         const int a = c * step, b = std::min(n, a + step);
-        if (a < b) pool.emplace_back([=, &f] { f(a, b); });
+        tasks[size_t(c - 1)].fn = [=, &f] {
+            if (a < b) f(a, b);
+        };
+        poolSubmit(&tasks[size_t(c - 1)]);
     }
     f(0, std::min(n, step));
-    for (std::thread& t : pool) t.join();
+    for (PoolTask& t : tasks) poolWait(&t);
 }
 
 // A triangle in BVH slot order with its centroid, so that every pass over a
@@ -165,9 +248,11 @@ struct BvhBuilder
         int dl, dr;
         if (count > par)
         {
-            std::thread th([&] { dl = build(lo, mid, left, level + 1, par); });
+            PoolTask t;
+            t.fn = [&] { dl = build(lo, mid, left, level + 1, par); };
+            poolSubmit(&t);
             dr = build(mid, hi, right, level + 1, par);
-            th.join();
+            poolWait(&t);
         }
         else
         {
@@ -208,6 +293,9 @@ int hostThreads()
 }
 
 int64_t bvhNodeCount(int64_t nt) { return nt > 0 ? subtreeNodes(nt) : 0; }
+
+void poolSubmit(PoolTask* t) { WorkPool::get().submit(t); }
+void poolWait(PoolTask* t) { WorkPool::get().wait(t); }
 
 // KdTree::build / buildRecursive (KdTree.h:91-121, 188-242): object-median
 // split on the largest centroid extent via std::nth_element, leaves of <= 8.
@@ -251,8 +339,9 @@ int buildHostBvhInto(const double* verts, int64_t nv, const int32_t* tris, int64
     int depth = 0;
     if (n > 0)
     {
-        // Subtrees of more than n/16 triangles (and at least 4096) run on their own thread.
-        const int64_t par = parallel ? std::max<int64_t>(4096, n / 16) : INT64_MAX;
+        // Subtrees of more than n/64 triangles (and at least 1024) are handed
+        // to the worker threads.
+        const int64_t par = parallel ? std::max<int64_t>(1024, n / 64) : INT64_MAX;
         BvhBuilder bb{nodes, items, box};
         depth = bb.build(0, n, 0, 1, par);
         const int64_t nn = bvhNodeCount(n);

@@ -3,7 +3,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
+#include <exception>
 #include <functional>
 #include <memory>
 #include <mutex>
@@ -83,6 +85,17 @@ int64_t bvhNodeCount(int64_t nt);
 void setRayDir(HostBvh& h);
 // Host worker threads: the CPUs this process may run on, capped by
 // OMP_NUM_THREADS when set and by 64.
+// A task for the library's persistent host worker threads.  poolWait runs
+// queued tasks until t is done, then rethrows what t threw.
+struct PoolTask
+{
+    std::function<void()> fn;
+    std::atomic<bool> done{false};
+    std::exception_ptr error;
+};
+void poolSubmit(PoolTask* t);
+void poolWait(PoolTask* t);
+
 int hostThreads();
 
 // Device copy of a BVH.

@@ -59,7 +59,8 @@ def test_setup_grid_matches_oracle(O, scale, target, maxr, voxel):
 
 @pytest.mark.parametrize("mk", [lambda: meshes.cube(0.1), lambda: meshes.torus(0.05, 0.01, 64, 32),
                                 lambda: meshes.gear(40, 12, 0.02, 0.002, 12),
-                                lambda: meshes.hollow_cylinder(0.04, 0.03, 0.05, 256, 32)])
+                                lambda: meshes.hollow_cylinder(0.04, 0.03, 0.05, 256, 32),
+                                lambda: meshes.hollow_cylinder(0.04, 0.03, 0.05, 1024, 40)])
 def test_host_bvh_matches_oracle_tree(O, mk):
     """The library's (multi-threaded) host BVH is node-for-node the KdTree of
     the reference as restated by the oracle, including the nth_element leaf order."""
