@@ -218,6 +218,7 @@ std::unique_ptr<sdfo_grid> buildDense(const double* verts, int64_t nv, const int
     checkMesh(verts, nv, tris, nt);
     auto g = std::make_unique<sdfo_grid>();
     g->device = currentDevice();
+    g->allocStream = s;
     g->spec = setupGrid(verts, nv, p);
     const int64_t N = g->spec.total();
     ph[0] = msSince(t0);
@@ -379,6 +380,7 @@ std::vector<std::unique_ptr<sdfo_grid>> buildDenseBatched(int32_t nm, const doub
     {
         auto g = std::make_unique<sdfo_grid>();
         g->device = dev;
+        g->allocStream = s;
         g->spec = specs[size_t(m)];
         const int64_t N = g->spec.total();
         g->phi = g->alloc<double>(size_t(N));
@@ -458,6 +460,7 @@ std::unique_ptr<sdfo_grid> buildSparse(const double* verts, int64_t nv, const in
     if (p.block_dim <= 0) throw Error(SDFO_ERR_INVALID_ARGUMENT, "Sparse SDF blockDim must be positive");
     auto g = std::make_unique<sdfo_grid>();
     g->device = currentDevice();
+    g->allocStream = s;
     g->sparse = true;
     g->spec = setupGrid(verts, nv, p);
     g->blockDim = p.block_dim;

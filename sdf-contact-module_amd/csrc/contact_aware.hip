@@ -1471,6 +1471,7 @@ sdfo_grid* buildContactAware(const double* verts, int64_t nv, const int32_t* tri
         throw Error(SDFO_ERR_INVALID_ARGUMENT, "Sparse contact-aware SDF blockDim must be positive");
     auto g = std::make_unique<sdfo_grid>();
     SDFO_HIP(hipGetDevice(&g->device));
+    g->allocStream = s;
     g->spec = setupGrid(verts, nv, p);
     const GridSpec& G = g->spec;
     const int stride = std::max(0, std::min(int(p.max_candidates), 8));

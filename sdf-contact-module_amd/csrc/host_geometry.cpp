@@ -416,10 +416,15 @@ void DeviceBvh::upload(const BvhNode* hostNodes, int64_t nNodes, const double* s
                                               " nodes exceeds the traversal stack encoding");
     nnodes = int(nNodes);
     ntris = nSlots;
-    SDFO_HIP(hipMalloc(&nodes, sizeof(BvhNode) * size_t(std::max<int64_t>(1, nNodes))));
-    SDFO_HIP(hipMalloc(&tri9, sizeof(double) * 9 * size_t(std::max<int64_t>(1, nSlots))));
-    SDFO_HIP(hipMalloc(&box6, sizeof(double) * 6 * size_t(std::max<int64_t>(1, nSlots))));
-    SDFO_HIP(hipMalloc(&triIdx, sizeof(int32_t) * size_t(std::max<int64_t>(1, nSlots))));
+    stream = s;
+    SDFO_HIP(hipMallocAsync(reinterpret_cast<void**>(&nodes),
+                            sizeof(BvhNode) * size_t(std::max<int64_t>(1, nNodes)), s));
+    SDFO_HIP(hipMallocAsync(reinterpret_cast<void**>(&tri9),
+                            sizeof(double) * 9 * size_t(std::max<int64_t>(1, nSlots)), s));
+    SDFO_HIP(hipMallocAsync(reinterpret_cast<void**>(&box6),
+                            sizeof(double) * 6 * size_t(std::max<int64_t>(1, nSlots)), s));
+    SDFO_HIP(hipMallocAsync(reinterpret_cast<void**>(&triIdx),
+                            sizeof(int32_t) * size_t(std::max<int64_t>(1, nSlots)), s));
     SDFO_HIP(hipMemcpyAsync(nodes, hostNodes, sizeof(BvhNode) * size_t(nNodes),
                             hipMemcpyHostToDevice, s));
     SDFO_HIP(hipMemcpyAsync(tri9, slotTri9, sizeof(double) * 9 * size_t(nSlots),
@@ -431,10 +436,10 @@ void DeviceBvh::upload(const BvhNode* hostNodes, int64_t nNodes, const double* s
 
 void DeviceBvh::release()
 {
-    if (nodes) (void)hipFree(nodes);
-    if (tri9) (void)hipFree(tri9);
-    if (box6) (void)hipFree(box6);
-    if (triIdx) (void)hipFree(triIdx);
+    if (nodes) (void)hipFreeAsync(nodes, stream);
+    if (tri9) (void)hipFreeAsync(tri9, stream);
+    if (box6) (void)hipFreeAsync(box6, stream);
+    if (triIdx) (void)hipFreeAsync(triIdx, stream);
     nodes = nullptr;
     tri9 = nullptr;
     box6 = nullptr;

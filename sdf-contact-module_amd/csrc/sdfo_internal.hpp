@@ -83,8 +83,6 @@ int buildHostBvhInto(const double* verts, int64_t nv, const int32_t* tris, int64
                      bool parallel);
 int64_t bvhNodeCount(int64_t nt);
 void setRayDir(HostBvh& h);
-// Host worker threads: the CPUs this process may run on, capped by
-// OMP_NUM_THREADS when set and by 64.
 // A task for the library's persistent host worker threads.  poolWait runs
 // queued tasks until t is done, then rethrows what t threw.
 struct PoolTask
@@ -96,6 +94,8 @@ struct PoolTask
 void poolSubmit(PoolTask* t);
 void poolWait(PoolTask* t);
 
+// Host worker threads: the CPUs this process may run on, capped by
+// OMP_NUM_THREADS when set and by 64.
 int hostThreads();
 
 // Device copy of a BVH.
@@ -109,6 +109,7 @@ struct DeviceBvh
     int64_t ntris = 0;
     int depth = 0;
     double rayDir[3] = {0, 0, 0};
+    hipStream_t stream = nullptr; // the arrays are stream-ordered allocations on it
     void upload(const HostBvh& h, hipStream_t s);
     // The same from raw host arrays; the slot boxes are derived on the device.
     void upload(const BvhNode* hostNodes, int64_t nNodes, const double* slotTri9,
@@ -191,6 +192,10 @@ struct sdfo_grid
 
     int64_t deviceBytes = 0;
     std::vector<void*> allocations;
+    // Arrays come from the device's stream-ordered pool, ordered on this stream
+    // (a builder sets its stream before allocating; loads use the null stream
+    // with synchronous copies).
+    hipStream_t allocStream = nullptr;
 
     template <typename T>
     T* alloc(size_t count)
@@ -198,10 +203,10 @@ struct sdfo_grid
         if (count == 0) return nullptr;
         void* p = nullptr;
         const size_t bytes = count * sizeof(T);
-        hipError_t e = hipMalloc(&p, bytes);
+        hipError_t e = hipMallocAsync(&p, bytes, allocStream);
         if (e != hipSuccess)
             throw sdfo::Error(SDFO_ERR_OUT_OF_MEMORY,
-                              "hipMalloc of " + std::to_string(bytes) + " bytes failed: " +
+                              "hipMallocAsync of " + std::to_string(bytes) + " bytes failed: " +
                                   hipGetErrorString(e));
         allocations.push_back(p);
         deviceBytes += int64_t(bytes);
@@ -209,7 +214,10 @@ struct sdfo_grid
     }
     ~sdfo_grid()
     {
-        for (void* p : allocations) (void)hipFree(p);
+        if (allocations.empty()) return;
+        // no work on any stream may still use the arrays when they return to the pool
+        (void)hipDeviceSynchronize();
+        for (void* p : allocations) (void)hipFreeAsync(p, nullptr);
     }
 };
 
