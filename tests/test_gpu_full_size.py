@@ -14,6 +14,8 @@
 * Sparse tube (1024 x 64 segments, 266 240 triangles) at 512^3 in 8^3 bricks:
   every brick's activation against the oracle's |signed distance| of the
   brick centre, and the voxels of 300 seeded active bricks bit for bit.
+* Differentiation: 256 poses with twists x 7680 points, friction on: forward
+  and every gradient output of the backward against the oracle.
 * Batched gears: 64 seeded meshes (2.88 M triangles) in one batched build,
   1500 seeded voxels of every grid bit for bit.
 Plus size-independent properties: the dense field is finite with unit
@@ -167,3 +169,28 @@ def test_batched_gears_full_size_sampled(O, sdfo_gpu):
     finally:
         for g in grids:
             g.close()
+
+
+def test_fwd_bwd_256_poses_full_size(O, cube_field):
+    """The differentiation workload: 256 seeded poses with twists x 7680 points,
+    friction on; every gradient output against the oracle's FP64 backward."""
+    g, ref = cube_field
+    print("[gpu] full-size fwd+bwd, 256 poses", flush=True)
+    X, w = scenes.ring_quadrature(120, 32)
+    R, t = scenes.ring_poses(256, seed=2)
+    v0, Om = scenes.pose_velocities(256, seed=3)
+    poses = scenes.pose_array(R, t, v0=v0, Om=Om)
+    gout = np.random.default_rng(17).normal(size=(256, 6))
+    law = SC.ContactLaw(kn=1e11, damping=1e4, mu=0.3, regularizer=1e-6, order=2)
+    Xd, wd, Pd = (torch.from_numpy(a).cuda() for a in (X, w, poses))
+    out, _ = SC.contact_forward(g, Xd, wd, Pd, law, active=True)
+    o_ref, _ = O.contact_forward(ograd(O, g.spec), f32(ref["phi"]), f32(ref["n"]), X, w, poses,
+                                 kn=1e11, c=1e4, mu=0.3, reg=1e-6, order=2)
+    scale = np.abs(o_ref).max(axis=0) + 1e-300
+    assert np.all(np.abs(out.cpu().numpy() - o_ref) <= 1e-9 * scale)
+    grads = SC.contact_backward(g, Xd, wd, Pd, torch.from_numpy(gout).cuda(), law)
+    refs = O.contact_backward(ograd(O, g.spec), f32(ref["phi"]), f32(ref["n"]), X, w, poses, gout,
+                              kn=1e11, c=1e4, mu=0.3, reg=1e-6, order=2)
+    for got, exp in zip(grads, refs):
+        got = got.cpu().numpy()
+        assert np.abs(got - exp).max() <= 1e-7 * (np.abs(exp).max() + 1e-300)
