@@ -270,6 +270,34 @@ std::unique_ptr<sdfo_grid> buildDense(const double* verts, int64_t nv, const int
     return g;
 }
 
+// Page-locked host staging for the batched build's concatenated BVH, kept
+// between builds and grown on demand, so its upload is a DMA instead of a
+// staged copy of pageable memory.  One batched build holds it at a time.
+struct PinnedStaging
+{
+    std::mutex lock;
+    char* p = nullptr;
+    size_t bytes = 0;
+    char* get(size_t n)
+    {
+        if (n > bytes)
+        {
+            if (p) (void)hipHostFree(p);
+            p = nullptr;
+            bytes = 0;
+            SDFO_HIP(hipHostMalloc(reinterpret_cast<void**>(&p), n, hipHostMallocDefault));
+            bytes = n;
+        }
+        return p;
+    }
+};
+
+PinnedStaging& batchStaging()
+{
+    static PinnedStaging* s = new PinnedStaging; // kept for the process
+    return *s;
+}
+
 // Many meshes: host set-up and BVHs in parallel, one concatenated BVH on the
 // device, one queue-driven fill launch for all bricks, then the per-mesh
 // Hessian and mirror kernels.
@@ -306,9 +334,16 @@ std::vector<std::unique_ptr<sdfo_grid>> buildDenseBatched(int32_t nm, const doub
         throw Error(SDFO_ERR_UNSUPPORTED, "batched build: the concatenated BVH is too large");
     // raw arrays: every element is written by the mesh builds (no zero fill)
     const int64_t nAllNodes = nodeBase[size_t(nm)], nAllSlots = slotBase[size_t(nm)];
-    std::unique_ptr<BvhNode[]> allNodes(new BvhNode[size_t(std::max<int64_t>(1, nAllNodes))]);
-    std::unique_ptr<double[]> allTri9(new double[size_t(std::max<int64_t>(1, nAllSlots)) * 9]);
-    std::unique_ptr<int32_t[]> allSlotTri(new int32_t[size_t(std::max<int64_t>(1, nAllSlots))]);
+    // (the stream is synchronised before this function returns, so the
+    // staging is free again when the lock is released)
+    std::lock_guard<std::mutex> stagingLock(batchStaging().lock);
+    const size_t nodeBytes = sizeof(BvhNode) * size_t(std::max<int64_t>(1, nAllNodes));
+    const size_t tri9Bytes = sizeof(double) * 9 * size_t(std::max<int64_t>(1, nAllSlots));
+    const size_t slotBytes = sizeof(int32_t) * size_t(std::max<int64_t>(1, nAllSlots));
+    char* staging = batchStaging().get(nodeBytes + tri9Bytes + slotBytes);
+    BvhNode* const allNodes = reinterpret_cast<BvhNode*>(staging);
+    double* const allTri9 = reinterpret_cast<double*>(staging + nodeBytes);
+    int32_t* const allSlotTri = reinterpret_cast<int32_t*>(staging + nodeBytes + tri9Bytes);
     std::vector<int> depths(static_cast<size_t>(nm), 0);
     // meshes are independent: the host worker threads take them largest
     // first, and a mesh's large subtrees go to idle workers
@@ -329,7 +364,7 @@ std::vector<std::unique_ptr<sdfo_grid>> buildDenseBatched(int32_t nm, const doub
                 const size_t nb = size_t(nodeBase[size_t(m)]), sb = size_t(slotBase[size_t(m)]);
                 depths[size_t(m)] = buildHostBvhInto(
                     verts + 3 * vo[m], vo[m + 1] - vo[m], tris + 3 * to[m], to[m + 1] - to[m],
-                    allNodes.get() + nb, allTri9.get() + 9 * sb, allSlotTri.get() + sb, int(nb),
+                    allNodes + nb, allTri9 + 9 * sb, allSlotTri + sb, int(nb),
                     int(sb), true);
             }
             catch (const Error& e)
@@ -371,17 +406,45 @@ std::vector<std::unique_ptr<sdfo_grid>> buildDenseBatched(int32_t nm, const doub
         poolSubmit(&t);
     }
 
-    // meanwhile this thread allocates every grid's device arrays
+    // meanwhile this thread allocates every grid's device arrays, all from
+    // one device block the grids share
     std::vector<std::unique_ptr<sdfo_grid>> grids(static_cast<size_t>(nm));
     std::vector<BatchMesh> jobs(static_cast<size_t>(nm));
     const int dev = currentDevice();
     keepStreamPoolMemory(dev);
+    auto cellsOf = [](const GridSpec& sp) {
+        return int64_t(sp.n[0] - 1) * (sp.n[1] - 1) * (sp.n[2] - 1);
+    };
+    auto gridBytes = [&](const GridSpec& sp) {
+        const size_t N = size_t(sp.total());
+        const int64_t cells = cellsOf(sp);
+        size_t b = 0;
+        for (size_t per : {sizeof(double), 3 * sizeof(double), 6 * sizeof(double), 3 * sizeof(double),
+                           sizeof(int32_t), sizeof(float4), 6 * sizeof(float)})
+            b += sdfo_grid::arenaBytes(per * N);
+        if (cells > 0 && cells * 8 * int64_t(sizeof(float4)) <= kCellRecordLimit)
+            b += sdfo_grid::arenaBytes(size_t(cells) * 8 * sizeof(float4));
+        return b;
+    };
+    size_t arenaTotal = 0;
+    for (int32_t m = 0; m < nm; ++m) arenaTotal += gridBytes(specs[size_t(m)]);
+    char* arenaBase = nullptr;
+    SDFO_HIP(hipMallocAsync(reinterpret_cast<void**>(&arenaBase), std::max<size_t>(arenaTotal, 256), s));
+    const std::shared_ptr<char> arena(arenaBase, [](char* p) {
+        (void)hipDeviceSynchronize(); // no work may still use the grids' arrays
+        (void)hipFreeAsync(p, nullptr);
+    });
+    size_t arenaOff = 0;
     for (int32_t m = 0; m < nm; ++m)
     {
         auto g = std::make_unique<sdfo_grid>();
         g->device = dev;
         g->allocStream = s;
         g->spec = specs[size_t(m)];
+        g->arena = arena;
+        g->arenaNext = arenaBase + arenaOff;
+        g->arenaLeft = gridBytes(g->spec);
+        arenaOff += g->arenaLeft;
         const int64_t N = g->spec.total();
         g->phi = g->alloc<double>(size_t(N));
         g->n3 = g->alloc<double>(size_t(N) * 3);
@@ -391,7 +454,7 @@ std::vector<std::unique_ptr<sdfo_grid>> buildDenseBatched(int32_t nm, const doub
         // the FP32 mirrors and cell records of finishDense
         g->pn = g->alloc<float4>(size_t(N));
         g->h6f = g->alloc<float>(size_t(N) * 6);
-        const int64_t cells = int64_t(g->spec.n[0] - 1) * (g->spec.n[1] - 1) * (g->spec.n[2] - 1);
+        const int64_t cells = cellsOf(g->spec);
         if (cells > 0 && cells * 8 * int64_t(sizeof(float4)) <= kCellRecordLimit)
             g->cellRec = g->alloc<float4>(size_t(cells) * 8);
         grids[size_t(m)] = std::move(g);
@@ -418,7 +481,7 @@ std::vector<std::unique_ptr<sdfo_grid>> buildDenseBatched(int32_t nm, const doub
     try
     {
         for (hipEvent_t& e : ev) SDFO_HIP(hipEventCreate(&e));
-        db.upload(allNodes.get(), nAllNodes, allTri9.get(), allSlotTri.get(), nAllSlots, depth,
+        db.upload(allNodes, nAllNodes, allTri9, allSlotTri, nAllSlots, depth,
                   dirOnly.rayDir, s);
         SDFO_HIP(hipEventRecord(ev[0], s));
         launchDenseFillBatched(jobs, db, s);

@@ -196,6 +196,14 @@ struct sdfo_grid
     // (a builder sets its stream before allocating; loads use the null stream
     // with synchronous copies).
     hipStream_t allocStream = nullptr;
+    // Optional slice of a shared device block: alloc() carves 256-byte aligned
+    // arrays from it while it has room (the grids of one batched build share
+    // one block, returned to the pool with the last of them).
+    std::shared_ptr<char> arena;
+    char* arenaNext = nullptr;
+    size_t arenaLeft = 0;
+
+    static size_t arenaBytes(size_t bytes) { return (bytes + 255) & ~size_t(255); }
 
     template <typename T>
     T* alloc(size_t count)
@@ -203,6 +211,14 @@ struct sdfo_grid
         if (count == 0) return nullptr;
         void* p = nullptr;
         const size_t bytes = count * sizeof(T);
+        if (arenaBytes(bytes) <= arenaLeft)
+        {
+            p = arenaNext;
+            arenaNext += arenaBytes(bytes);
+            arenaLeft -= arenaBytes(bytes);
+            deviceBytes += int64_t(bytes);
+            return static_cast<T*>(p);
+        }
         hipError_t e = hipMallocAsync(&p, bytes, allocStream);
         if (e != hipSuccess)
             throw sdfo::Error(SDFO_ERR_OUT_OF_MEMORY,
