@@ -438,9 +438,11 @@ denseReplayKernel(GridSpec g, const BvhNode* __restrict__ nodes, const double* _
 }
 
 // Central differences of the normal field, clamped at the border, then
-// symmetrised (SdfBuilder.cpp:1105-1152).
+// symmetrised (SdfBuilder.cpp:1105-1152).  With pn set it also writes the FP32
+// query mirror of the voxel (what denseMirrorKernel writes), saving a pass.
 __global__ void denseHessianKernel(GridSpec g, const double* __restrict__ n3,
-                                   double* __restrict__ h6)
+                                   double* __restrict__ h6, const double* __restrict__ phi,
+                                   float4* __restrict__ pn, float* __restrict__ h6f)
 {
     const int64_t N = g.total();
     const int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -459,13 +461,18 @@ __global__ void denseHessianKernel(GridSpec g, const double* __restrict__ n3,
     const D3 dx = dev::divs(dev::sub(at(i + 1, j, k), at(i - 1, j, k)), 2.0 * g.voxel[0]);
     const D3 dy = dev::divs(dev::sub(at(i, j + 1, k), at(i, j - 1, k)), 2.0 * g.voxel[1]);
     const D3 dz = dev::divs(dev::sub(at(i, j, k + 1), at(i, j, k - 1)), 2.0 * g.voxel[2]);
+    const double H[6] = {0.5 * (dx.x + dx.x), 0.5 * (dy.x + dx.y), 0.5 * (dz.x + dx.z),
+                         0.5 * (dy.y + dy.y), 0.5 * (dz.y + dy.z), 0.5 * (dz.z + dz.z)};
     double* o = h6 + 6 * idx;
-    o[0] = 0.5 * (dx.x + dx.x);
-    o[1] = 0.5 * (dy.x + dx.y);
-    o[2] = 0.5 * (dz.x + dx.z);
-    o[3] = 0.5 * (dy.y + dy.y);
-    o[4] = 0.5 * (dz.y + dy.z);
-    o[5] = 0.5 * (dz.z + dz.z);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) o[c] = H[c];
+    if (pn)
+    {
+        pn[idx] = make_float4(float(phi[idx]), float(n3[3 * idx]), float(n3[3 * idx + 1]),
+                              float(n3[3 * idx + 2]));
+#pragma unroll
+        for (int c = 0; c < 6; ++c) h6f[6 * idx + c] = float(H[c]);
+    }
 }
 
 // FP32 query mirror: float4 (phi, n) per voxel, 6 floats of Hessian.
@@ -1116,10 +1123,11 @@ void launchDenseFillBatched(const std::vector<BatchMesh>& meshes, const DeviceBv
     SDFO_HIP(hipFreeAsync(dj, s));
 }
 
-void launchDenseHessian(const GridSpec& g, const double* n3, double* h6, hipStream_t s)
+void launchDenseHessian(const GridSpec& g, const double* n3, double* h6, hipStream_t s,
+                        const double* phi, float4* pn, float* h6f)
 {
     hipLaunchKernelGGL(denseHessianKernel, dim3(blocksFor(g.total(), 256)), dim3(256), 0, s, g,
-                       n3, h6);
+                       n3, h6, phi, pn, h6f);
     SDFO_HIP(hipGetLastError());
 }
 

@@ -55,12 +55,12 @@ double* lastBuildPhases()
     return phases;
 }
 
-void finishDense(sdfo_grid& g, hipStream_t s)
+void finishDense(sdfo_grid& g, hipStream_t s, bool mirrored)
 {
     const int64_t N = g.spec.total();
     if (!g.pn) g.pn = g.alloc<float4>(size_t(N));
     if (!g.h6f) g.h6f = g.alloc<float>(size_t(N) * 6);
-    launchDenseMirror(N, g.phi, g.n3, g.h6, g.pn, g.h6f, s);
+    if (!mirrored) launchDenseMirror(N, g.phi, g.n3, g.h6, g.pn, g.h6f, s);
     const int64_t cells = int64_t(g.spec.n[0] - 1) * (g.spec.n[1] - 1) * (g.spec.n[2] - 1);
     if (cells > 0 && cells * 8 * int64_t(sizeof(float4)) <= kCellRecordLimit)
     {
@@ -242,9 +242,11 @@ std::unique_ptr<sdfo_grid> buildDense(const double* verts, int64_t nv, const int
         SDFO_HIP(hipEventRecord(ev[0], s));
         launchDenseFill(g->spec, db, g->phi, g->n3, g->witness3, g->tri, s);
         SDFO_HIP(hipEventRecord(ev[1], s));
-        launchDenseHessian(g->spec, g->n3, g->h6, s);
+        g->pn = g->alloc<float4>(size_t(N));
+        g->h6f = g->alloc<float>(size_t(N) * 6);
+        launchDenseHessian(g->spec, g->n3, g->h6, s, g->phi, g->pn, g->h6f);
         SDFO_HIP(hipEventRecord(ev[2], s));
-        finishDense(*g, s);
+        finishDense(*g, s, true);
         SDFO_HIP(hipEventRecord(ev[3], s));
         SDFO_HIP(hipStreamSynchronize(s));
         float f = 0.0f;
@@ -489,8 +491,8 @@ std::vector<std::unique_ptr<sdfo_grid>> buildDenseBatched(int32_t nm, const doub
         for (int32_t m = 0; m < nm; ++m)
         {
             sdfo_grid& g = *grids[size_t(m)];
-            launchDenseHessian(g.spec, g.n3, g.h6, s);
-            finishDense(g, s);
+            launchDenseHessian(g.spec, g.n3, g.h6, s, g.phi, g.pn, g.h6f);
+            finishDense(g, s, true);
         }
         SDFO_HIP(hipEventRecord(ev[2], s));
         SDFO_HIP(hipStreamSynchronize(s));

@@ -34,9 +34,6 @@ namespace sdfo {
 
 using dev::D3;
 
-void finishDense(sdfo_grid& g, hipStream_t s);
-void finishSparse(sdfo_grid& g, const std::vector<int32_t>& bijk, hipStream_t s);
-
 int& debugMergeCap()
 {
     thread_local int cap = 0;

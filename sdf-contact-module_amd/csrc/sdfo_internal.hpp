@@ -322,7 +322,9 @@ int& debugHeavyCap();
 void keepStreamPoolMemory(int device);
 
 // Grid completion helpers (capi.cpp): FP32 mirrors and the brick table.
-void finishDense(sdfo_grid& g, hipStream_t s);
+// FP32 mirrors and cell records of a dense grid (mirrored: pn and h6f were
+// already written by the Hessian pass).
+void finishDense(sdfo_grid& g, hipStream_t s, bool mirrored = false);
 void finishSparse(sdfo_grid& g, const std::vector<int32_t>& bijk, hipStream_t s);
 std::vector<int32_t> brickTable(const std::vector<int32_t>& bijk, int nbx, int nby, int nbz);
 
@@ -356,7 +358,9 @@ void launchSparseCellPack(const GridSpec& g, int bd, int nbx, int nby, int nbz,
                           hipStream_t s);
 void launchDenseFill(const GridSpec& g, const DeviceBvh& bvh, double* phi, double* n3,
                      double* witness3, int32_t* tri, hipStream_t s);
-void launchDenseHessian(const GridSpec& g, const double* n3, double* h6, hipStream_t s);
+// With pn (and h6f) set, also writes the FP32 query mirror of every voxel.
+void launchDenseHessian(const GridSpec& g, const double* n3, double* h6, hipStream_t s,
+                        const double* phi = nullptr, float4* pn = nullptr, float* h6f = nullptr);
 void launchDenseMirror(int64_t N, const double* phi, const double* n3, const double* h6,
                        float4* pn, float* h6f, hipStream_t s);
 void launchCandidateMirror(int64_t count, const double* cphi, const double* cn3, float4* cpn,
