@@ -2,8 +2,8 @@
 
 The build kernels run FP64 without FMA contraction and traverse the same BVH
 in the same near-first order as SdfBuilder / KdTree, so every field is
-compared bit for bit (north-star rule (a): nearest-triangle indices and the
-sparse brick mask bit-exact; here the values are bit-exact too).
+compared bit for bit (the accuracy rule asks for nearest-triangle indices and
+the sparse brick mask bit-exact; here the values are bit-exact too).
 """
 import numpy as np
 import pytest

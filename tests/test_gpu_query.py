@@ -2,8 +2,9 @@
 
 The HIP query reads the FP32 field mirror and computes in FP32 (FP64 for the
 cell location); the oracle is run on the same FP32-rounded fields in FP64.
-Bar: valid mask identical; gap within 1e-6 * bbox diagonal (north-star rule
-(b)); raw gradient within 2e-4 absolute (an FP32 interpolant of unit normals).
+Bar: valid mask identical; gap within 1e-6 * bbox diagonal (the accuracy
+rule for field values); raw gradient within 2e-4 absolute (an FP32
+interpolant of unit normals).
 """
 import numpy as np
 import pytest
