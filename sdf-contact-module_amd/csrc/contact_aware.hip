@@ -1303,53 +1303,58 @@ inline unsigned blocksFor(int64_t n, int bs) { return unsigned((n + bs - 1) / bs
 
 struct HostCA
 {
-    std::vector<double> t21;
-    std::vector<double> bmin, bmax; // 3 per triangle
+    std::unique_ptr<double[]> t21;  // 21 per triangle (see CTri)
+    std::unique_ptr<double[]> box6; // bmin3, bmax3 per triangle
 };
 
-// makeContactTriangleInfos (SdfBuilder.cpp:155-184)
+// makeContactTriangleInfos (SdfBuilder.cpp:155-184), triangles in parallel;
+// arrays left uninitialised until written (every element is).
 HostCA makeTriangleInfos(const double* verts, int64_t nv, const int32_t* tris, int64_t nt,
                          const double* cn)
 {
     HostCA h;
-    h.t21.resize(size_t(nt) * 21);
-    h.bmin.resize(size_t(nt) * 3);
-    h.bmax.resize(size_t(nt) * 3);
-    for (int64_t t = 0; t < nt; ++t)
-    {
-        const int32_t* ix = tris + 3 * t;
-        for (int c = 0; c < 3; ++c)
-            if (ix[c] < 0 || ix[c] >= nv)
-                throw Error(SDFO_ERR_INVALID_ARGUMENT,
-                            "Contact-aware mesh triangle references invalid vertex");
-        double* o = &h.t21[21 * size_t(t)];
-        for (int c = 0; c < 3; ++c)
-            for (int k = 0; k < 3; ++k) o[3 * c + k] = verts[3 * size_t(ix[c]) + k];
-        for (int q = 0; q < 9; ++q) o[9 + q] = cn[9 * size_t(t) + q];
-        const double e1[3] = {o[3] - o[0], o[4] - o[1], o[5] - o[2]};
-        const double e2[3] = {o[6] - o[0], o[7] - o[1], o[8] - o[2]};
-        const double cr[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2],
-                              e1[0] * e2[1] - e1[1] * e2[0]};
-        normalizedOrDefault(cr[0], cr[1], cr[2], 1.0, 0.0, 0.0, o[18], o[19], o[20]);
-        for (int k = 0; k < 3; ++k)
+    h.t21.reset(new double[size_t(std::max<int64_t>(1, nt)) * 21]);
+    h.box6.reset(new double[size_t(std::max<int64_t>(1, nt)) * 6]);
+    std::atomic<bool> bad{false};
+    parallelRange(nt, 8192, [&](int64_t t0, int64_t t1) {
+        for (int64_t t = t0; t < t1; ++t)
         {
-            const double a = o[k], b = o[3 + k], c = o[6 + k];
-            double lo = (b < a) ? b : a;
-            lo = (c < lo) ? c : lo;
-            double hi = (a < b) ? b : a;
-            hi = (hi < c) ? c : hi;
-            h.bmin[3 * size_t(t) + k] = lo;
-            h.bmax[3 * size_t(t) + k] = hi;
+            const int32_t* ix = tris + 3 * t;
+            if (ix[0] < 0 || ix[0] >= nv || ix[1] < 0 || ix[1] >= nv || ix[2] < 0 || ix[2] >= nv)
+            {
+                bad = true;
+                continue;
+            }
+            double* o = &h.t21[21 * size_t(t)];
+            for (int c = 0; c < 3; ++c)
+                for (int k = 0; k < 3; ++k) o[3 * c + k] = verts[3 * size_t(ix[c]) + k];
+            for (int q = 0; q < 9; ++q) o[9 + q] = cn[9 * size_t(t) + q];
+            const double e1[3] = {o[3] - o[0], o[4] - o[1], o[5] - o[2]};
+            const double e2[3] = {o[6] - o[0], o[7] - o[1], o[8] - o[2]};
+            const double cr[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2],
+                                  e1[0] * e2[1] - e1[1] * e2[0]};
+            normalizedOrDefault(cr[0], cr[1], cr[2], 1.0, 0.0, 0.0, o[18], o[19], o[20]);
+            for (int k = 0; k < 3; ++k)
+            {
+                const double a = o[k], b = o[3 + k], c = o[6 + k];
+                double lo = (b < a) ? b : a;
+                lo = (c < lo) ? c : lo;
+                double hi = (a < b) ? b : a;
+                hi = (hi < c) ? c : hi;
+                h.box6[6 * size_t(t) + k] = lo;
+                h.box6[6 * size_t(t) + 3 + k] = hi;
+            }
         }
-    }
+    });
+    if (bad) throw Error(SDFO_ERR_INVALID_ARGUMENT, "Contact-aware mesh triangle references invalid vertex");
     return h;
 }
 
 struct HostTiles
 {
     int tileDim = 1, nx = 1, ny = 1, nz = 1;
-    std::vector<int32_t> triTile;  // 6 per triangle: tile range (lo xyz, hi xyz)
-    std::vector<int32_t> nodeTile; // 6 per BVH node: union of its triangles' ranges
+    std::unique_ptr<int32_t[]> triTile; // 6 per triangle: tile range (lo xyz, hi xyz)
+    std::vector<int32_t> nodeTile;      // 6 per BVH node: union of its triangles' ranges
 };
 
 // buildContactTileIndex (SdfBuilder.cpp:202-258) without the lists: each
@@ -1365,17 +1370,19 @@ HostTiles buildTiles(const GridSpec& g, const HostCA& ca, int64_t nt, int reqTil
     const double tileRadius = 0.5 * std::sqrt(3.0) * double(T.tileDim) * h;
     const double expansion = (0.0 < infl ? infl : 0.0) + tileRadius + h;
     auto clampi = [](int v, int hi) { return std::max(0, std::min(hi, v)); };
-    T.triTile.resize(size_t(nt) * 6);
-    for (int64_t t = 0; t < nt; ++t)
-        for (int a = 0; a < 3; ++a)
-        {
-            const double lo = ca.bmin[3 * size_t(t) + a] - expansion;
-            const double hi = ca.bmax[3 * size_t(t) + a] + expansion;
-            const int v0 = int(std::floor((lo - g.bmin[a]) / g.voxel[a] - 0.5));
-            const int v1 = int(std::ceil((hi - g.bmin[a]) / g.voxel[a] - 0.5));
-            T.triTile[6 * size_t(t) + a] = clampi(v0, g.n[a] - 1) / T.tileDim;
-            T.triTile[6 * size_t(t) + 3 + a] = clampi(v1, g.n[a] - 1) / T.tileDim;
-        }
+    T.triTile.reset(new int32_t[size_t(std::max<int64_t>(1, nt)) * 6]);
+    parallelRange(nt, 16384, [&](int64_t t0, int64_t t1) {
+        for (int64_t t = t0; t < t1; ++t)
+            for (int a = 0; a < 3; ++a)
+            {
+                const double lo = ca.box6[6 * size_t(t) + a] - expansion;
+                const double hi = ca.box6[6 * size_t(t) + 3 + a] + expansion;
+                const int v0 = int(std::floor((lo - g.bmin[a]) / g.voxel[a] - 0.5));
+                const int v1 = int(std::ceil((hi - g.bmin[a]) / g.voxel[a] - 0.5));
+                T.triTile[6 * size_t(t) + a] = clampi(v0, g.n[a] - 1) / T.tileDim;
+                T.triTile[6 * size_t(t) + 3 + a] = clampi(v1, g.n[a] - 1) / T.tileDim;
+            }
+    });
     return T;
 }
 
@@ -1423,28 +1430,43 @@ double searchMargin(const double* verts, int64_t nv, const GridSpec& g)
     return std::ldexp(m, -30);
 }
 
+// Device buffers of one build: stream-ordered allocations on the build's
+// stream, returned to the pool on that stream when the build ends.
 struct DevBuf
 {
+    hipStream_t s;
     std::vector<void*> ptrs;
-    template <typename T>
-    T* up(const std::vector<T>& h)
-    {
-        void* p = nullptr;
-        SDFO_HIP(hipMalloc(&p, std::max<size_t>(1, h.size()) * sizeof(T)));
-        ptrs.push_back(p);
-        if (!h.empty()) SDFO_HIP(hipMemcpy(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
-        return static_cast<T*>(p);
-    }
+    explicit DevBuf(hipStream_t st) : s(st) {}
     void* raw(size_t bytes)
     {
         void* p = nullptr;
-        SDFO_HIP(hipMalloc(&p, std::max<size_t>(1, bytes)));
+        SDFO_HIP(hipMallocAsync(&p, std::max<size_t>(1, bytes), s));
         ptrs.push_back(p);
+        return p;
+    }
+    // a copy of n host elements; the host array must stay valid until the
+    // stream has passed the copy
+    template <typename T>
+    T* up(const T* h, size_t n)
+    {
+        T* p = static_cast<T*>(raw(n * sizeof(T)));
+        if (n) SDFO_HIP(hipMemcpyAsync(p, h, n * sizeof(T), hipMemcpyHostToDevice, s));
+        return p;
+    }
+    template <typename T>
+    T* up(const std::vector<T>& h)
+    {
+        return up(h.data(), h.size());
+    }
+    int* zeroInt()
+    {
+        int* p = static_cast<int*>(raw(sizeof(int)));
+        SDFO_HIP(hipMemsetAsync(p, 0, sizeof(int), s));
         return p;
     }
     ~DevBuf()
     {
-        for (void* p : ptrs) (void)hipFree(p);
+        for (void* p : ptrs) (void)hipFreeAsync(p, s);
     }
 };
 
@@ -1509,7 +1531,7 @@ sdfo_grid* buildContactAware(const double* verts, int64_t nv, const int32_t* tri
     HostTiles tiles;
     if (p.use_tile_acceleration) tiles = buildTiles(G, ca, nt, p.tile_dim, infl);
 
-    DevBuf buf;
+    DevBuf buf(s);
     DeviceBvh db;
     CAData D{};
     CAParams P{};
@@ -1538,7 +1560,7 @@ sdfo_grid* buildContactAware(const double* verts, int64_t nv, const int32_t* tri
         const size_t entry = 4 * sizeof(double) + 2 * sizeof(int);
         const int64_t per = std::max<int64_t>(1, int64_t((size_t(1) << 30) / (entry * size_t(cap))));
         const int batch = int(std::min<int64_t>(cnt, per));
-        DevBuf pb;
+        DevBuf pb(s);
         const size_t e = size_t(batch) * size_t(cap);
         MergePool pool{};
         pool.cap = cap;
@@ -1571,7 +1593,7 @@ sdfo_grid* buildContactAware(const double* verts, int64_t nv, const int32_t* tri
             runLevel(launch, ov.ids, ov.minDist, cnt, poolCap, Overflow{nullptr, nullptr, nullptr});
             return;
         }
-        Overflow esc{buf.up(std::vector<int>{0}),
+        Overflow esc{buf.zeroInt(),
                      static_cast<int64_t*>(buf.raw(size_t(cnt) * sizeof(int64_t))),
                      static_cast<double*>(buf.raw(size_t(cnt) * sizeof(double)))};
         runLevel(launch, ov.ids, ov.minDist, cnt, secondCap, esc);
@@ -1599,21 +1621,12 @@ sdfo_grid* buildContactAware(const double* verts, int64_t nv, const int32_t* tri
     try
     {
         keepStreamPoolMemory(g->device);
-        D.t21 = buf.up(ca.t21);
-        {
-            std::vector<double> tb(size_t(nt) * 6);
-            for (int64_t t = 0; t < nt; ++t)
-                for (int a = 0; a < 3; ++a)
-                {
-                    tb[6 * size_t(t) + a] = ca.bmin[3 * size_t(t) + a];
-                    tb[6 * size_t(t) + 3 + a] = ca.bmax[3 * size_t(t) + a];
-                }
-            D.tbox6 = buf.up(tb);
-        }
+        D.t21 = buf.up(ca.t21.get(), size_t(nt) * 21);
+        D.tbox6 = buf.up(ca.box6.get(), size_t(nt) * 6);
         int32_t* tileCount = nullptr;
         if (P.useTiles)
         {
-            D.triTile = buf.up(tiles.triTile);
+            D.triTile = buf.up(tiles.triTile.get(), size_t(nt) * 6);
             // tile lists on the device (count, scan, fill, sort)
             const int64_t ntiles = int64_t(tiles.nx) * tiles.ny * tiles.nz;
             tileCount = static_cast<int32_t*>(buf.raw(size_t(ntiles) * sizeof(int32_t)));
@@ -1645,7 +1658,7 @@ sdfo_grid* buildContactAware(const double* verts, int64_t nv, const int32_t* tri
             poolCap = std::max(int(longest), 1);
         }
         D.mu = searchMargin(verts, nv, G);
-        ov.count = buf.up(std::vector<int>{0});
+        ov.count = buf.zeroInt();
         if (!sparse)
         {
             const int64_t N = G.total();

@@ -297,6 +297,44 @@ int64_t bvhNodeCount(int64_t nt) { return nt > 0 ? subtreeNodes(nt) : 0; }
 void poolSubmit(PoolTask* t) { WorkPool::get().submit(t); }
 void poolWait(PoolTask* t) { WorkPool::get().wait(t); }
 
+void parallelRange(int64_t n, int64_t grain, const std::function<void(int64_t, int64_t)>& f)
+{
+    if (n <= 0) return;
+    const int64_t chunks = std::max<int64_t>(
+        1, std::min<int64_t>(std::min(16, hostThreads()), n / std::max<int64_t>(1, grain)));
+    const int64_t step = (n + chunks - 1) / chunks;
+    std::vector<PoolTask> tasks(size_t(chunks - 1));
+    for (int64_t c = 1; c < chunks; ++c)
+    {
+        const int64_t a = c * step, b = std::min(n, a + step);
+        tasks[size_t(c - 1)].fn = [=, &f] {
+            if (a < b) f(a, b);
+        };
+        poolSubmit(&tasks[size_t(c - 1)]);
+    }
+    std::exception_ptr first;
+    try
+    {
+        f(0, std::min(n, step));
+    }
+    catch (...)
+    {
+        first = std::current_exception();
+    }
+    for (PoolTask& t : tasks)
+    {
+        try
+        {
+            poolWait(&t);
+        }
+        catch (...)
+        {
+            if (!first) first = std::current_exception();
+        }
+    }
+    if (first) std::rethrow_exception(first);
+}
+
 // KdTree::build / buildRecursive (KdTree.h:91-121, 188-242): object-median
 // split on the largest centroid extent via std::nth_element, leaves of <= 8.
 // Node indices are the recursion's pre-order creation order, so the tree is

@@ -98,6 +98,10 @@ void poolWait(PoolTask* t);
 // OMP_NUM_THREADS when set and by 64.
 int hostThreads();
 
+// Runs f(begin, end) over [0, n) in up to 16 chunks of at least `grain`
+// items on the worker threads (the calling thread takes the first chunk).
+void parallelRange(int64_t n, int64_t grain, const std::function<void(int64_t, int64_t)>& f);
+
 // Device copy of a BVH.
 struct DeviceBvh
 {
