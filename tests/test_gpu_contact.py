@@ -10,7 +10,7 @@ import torch
 
 from conftest import f32, ograd
 from sdfo import contact as SC
-from sdfo import scenes
+from sdfo import meshes, scenes
 
 pytestmark = pytest.mark.gpu
 
@@ -241,3 +241,32 @@ def test_culled_forward_matches_oracle(O, cube):
         assert la[b].item() == a[0], b
         scale = np.abs(o[0]).max() + 1e-300
         assert np.abs(lo[b].cpu().numpy() - o[0]).max() <= 1e-9 * scale, b
+
+
+@pytest.mark.parametrize("n_radial", [20, 40])
+def test_torsion_on_sdf_face_matches_closed_form(sdfo_gpu, n_radial):
+    """The annular-patch torsion check (ring_on_cube_torsion.py:55-60) through the
+    GPU path: an annulus (Ri 0.1, Ro 0.2, midpoint rings x 128 sectors) pressed
+    0.01 into the top face of a cube SDF and spinning about the face normal,
+    mu = 0.3, linear penalty.  Normal force = kn * depth * area and torque =
+    -mu N (2/3)(Ro^3 - Ri^3)/(Ro^2 - Ri^2) times the quadrature's exact relative
+    error, both to 1e-6; the symmetric patch's tangential residual to 1e-12 mu N."""
+    v, t = meshes.cube(1.0)
+    g = sdfo_gpu.SdfGrid.build_dense(v, t, target_resolution=64, max_resolution=64)
+    try:
+        Ri, Ro, mu, kn, pen = 0.1, 0.2, 0.3, 1e6, 0.01
+        X, w = meshes.annular_quadrature(Ri, Ro, n_radial, 128)
+        P = scenes.pose_array(np.eye(3)[None], np.array([[0.0, 0.0, 0.5 - pen]]),
+                              Om=np.array([[0.0, 0.0, 1.0]]))
+        law = SC.ContactLaw(kn=kn, damping=0.0, mu=mu, regularizer=0.0, order=1, mode="Tricubic")
+        out, act = SC.contact_forward(g, *_cuda(X, w, P), law, active=True)
+        out = out.cpu().numpy()[0]
+        N = kn * pen * np.pi * (Ro ** 2 - Ri ** 2)
+        Tz = -mu * N * (2.0 / 3.0) * (Ro ** 3 - Ri ** 3) / (Ro ** 2 - Ri ** 2)
+        quad = -(Ro - Ri) ** 3 / (4 * n_radial ** 2 * (Ro ** 3 - Ri ** 3))
+        assert int(act.cpu()[0]) == len(X)
+        assert np.isclose(out[2], N, rtol=1e-6)
+        assert np.isclose(out[5], Tz * (1 + quad), rtol=1e-6)
+        assert np.hypot(out[0], out[1]) <= 1e-12 * mu * N
+    finally:
+        g.close()
