@@ -270,3 +270,51 @@ def test_torsion_on_sdf_face_matches_closed_form(sdfo_gpu, n_radial):
         assert np.hypot(out[0], out[1]) <= 1e-12 * mu * N
     finally:
         g.close()
+
+
+def test_contact_gradients_match_central_differences(cube):
+    """The hand-written backward through `sdfo.contact.contact` against FP64
+    central differences of the same GPU forward (h = 1e-7 m / rad), friction
+    on: pose twist, kn and a few quadrature points, to 1e-4 relative."""
+    g, _ = cube
+    X, w, poses = _scene(4, seed=21)
+    R0 = torch.from_numpy(poses[:, :9].reshape(-1, 3, 3).copy()).cuda()
+    t0 = torch.from_numpy(poses[:, 9:12].copy()).cuda()
+    v0 = torch.from_numpy(poses[:, 24:27].copy()).cuda()
+    Om = torch.from_numpy(poses[:, 27:30].copy()).cuda()
+    Xt, wt = _cuda(X, w)
+    wsum = torch.from_numpy(np.random.default_rng(4).normal(size=(4, 6))).cuda()
+
+    def loss(twist, kn, Xq):
+        out = SC.contact(g, twist, kn, 1e4, 0.3, Xq, wt, R0, t0, v0, Om)
+        return (out * wsum).sum()
+
+    twist = torch.zeros((4, 6), dtype=torch.float64, device="cuda", requires_grad=True)
+    kn = torch.tensor(1e11, dtype=torch.float64, requires_grad=True)
+    Xq = Xt.clone().requires_grad_(True)
+    L = loss(twist, kn, Xq)
+    L.backward()
+    assert float(L.detach().abs()) > 0
+    h = 1e-7
+    with torch.no_grad():
+        fd = torch.zeros_like(twist)
+        for b in range(4):
+            for k in range(6):
+                tp, tm = twist.detach().clone(), twist.detach().clone()
+                tp[b, k] += h
+                tm[b, k] -= h
+                fd[b, k] = (loss(tp, kn.detach(), Xt) - loss(tm, kn.detach(), Xt)) / (2 * h)
+        gt = twist.grad
+        assert (gt - fd).abs().max() <= 1e-4 * fd.abs().max()
+        dk = 1e11 * 1e-6
+        fdk = (loss(twist.detach(), kn.detach() + dk, Xt) - loss(twist.detach(), kn.detach() - dk, Xt)) / (2 * dk)
+        assert abs(float(kn.grad) - float(fdk)) <= 1e-4 * abs(float(fdk))
+        gx = Xq.grad
+        rows = torch.argsort(gx.abs().sum(1), descending=True)[:5]
+        for r in rows.tolist():
+            for k in range(3):
+                xp, xm = Xt.clone(), Xt.clone()
+                xp[r, k] += h
+                xm[r, k] -= h
+                fdx = (loss(twist.detach(), kn.detach(), xp) - loss(twist.detach(), kn.detach(), xm)) / (2 * h)
+                assert abs(float(gx[r, k]) - float(fdx)) <= 1e-4 * float(gx[rows[0]].abs().max())
