@@ -18,6 +18,8 @@
   and every gradient output of the backward against the oracle.
 * Batched gears: 64 seeded meshes (2.88 M triangles) in one batched build,
   1500 seeded voxels of every grid bit for bit.
+* The reference's own CPU-runnable case: the 0.1 m cube at 64^3, every
+  voxel, and the demo's 7680 LCG points in Trilinear mode.
 Plus size-independent properties: the dense field is finite with unit
 normals, and a batch of queries equals the same points queried in pieces.
 """
@@ -194,3 +196,32 @@ def test_fwd_bwd_256_poses_full_size(O, cube_field):
     for got, exp in zip(grads, refs):
         got = got.cpu().numpy()
         assert np.abs(got - exp).max() <= 1e-7 * (np.abs(exp).max() + 1e-300)
+
+
+def test_cube64_demo_trilinear(O, sdfo_gpu):
+    """The reference's CPU-runnable case: the 0.1 m cube (12 triangles) at 64^3
+    with 10% padding, every voxel bit for bit, and 7680 points from the demo's
+    LCG (SdfOracleDemo.cpp:202-274), Trilinear value + gradient: FP64 points
+    against the oracle on the same FP32 field to 1e-12, and against the
+    oracle's FP64 field (the reference's own numbers) to 1e-6 x bbox diagonal."""
+    v, t = scenes.cube_mesh()
+    g = sdfo_gpu.SdfGrid.build_dense(v, t, target_resolution=64, max_resolution=64)
+    try:
+        assert tuple(g.spec.shape) == (64, 64, 64)
+        og = ograd(O, g.spec)
+        ref = O.build_dense(v, t, og)
+        got = g.download_dense()
+        for k in ("phi", "n", "witness", "H"):
+            assert np.array_equal(got[k], ref[k]), k
+        pts = scenes.demo_benchmark_points(g.spec, 7680)
+        gap, grad, valid = g.query(torch.from_numpy(pts).cuda(), "Trilinear")
+        vv = valid.cpu().numpy()
+        assert vv.all()
+        r32 = O.query_dense(og, dict(phi=f32(ref["phi"]), n=f32(ref["n"]), H=f32(ref["H"])),
+                            "Trilinear", pts)
+        assert np.abs(gap.cpu().numpy() - r32["gap"]).max() <= 1e-12
+        assert np.abs(grad.cpu().numpy() - r32["grad"]).max() <= 1e-9
+        r64 = O.query_dense(og, ref, "Trilinear", pts)
+        assert np.abs(gap.cpu().numpy() - r64["gap"]).max() <= 1e-6 * g.spec.diagonal
+    finally:
+        g.close()
