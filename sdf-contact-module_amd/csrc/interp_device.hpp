@@ -9,6 +9,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include <cfloat>
 #include <cstdint>
 
@@ -47,6 +49,10 @@ struct QGrid
     const float4* scellRec;
     const unsigned long long* scellMask;
     int maskWords;
+    // dense FP64 fields (the authoritative SdfData arrays), read by the FP64
+    // query kernels: phi, n (3 per voxel), H (6 per voxel), candidate phi
+    // (S per voxel) and normals (3S per voxel)
+    const double *phi64, *n64, *h64, *cphi64, *cn64;
 };
 
 // The 8 packed corner records of cell c (dense or sparse cell records), or
@@ -85,9 +91,38 @@ __device__ __forceinline__ int64_t recordOf(const QGrid& g, int i, int j, int k)
     return int64_t(b) * (bd * bd * bd) + (i - bi * bd) + bd * ((j - bj * bd) + bd * (k - bk * bd));
 }
 
-__device__ __forceinline__ float4 fieldAt(const QGrid& g, int64_t rec)
+// (phi, n) of a voxel record: the FP32 mirror, or with F64 a dense grid's
+// FP64 fields.
+template <bool F64 = false>
+__device__ __forceinline__ std::conditional_t<F64, double4, float4> fieldAt(const QGrid& g,
+                                                                             int64_t rec)
 {
-    return g.sparse ? g.spn[rec] : g.pn[rec];
+    if constexpr (F64)
+        return make_double4(g.phi64[rec], g.n64[3 * rec], g.n64[3 * rec + 1], g.n64[3 * rec + 2]);
+    else
+        return g.sparse ? g.spn[rec] : g.pn[rec];
+}
+
+// The Hessian array (6 per record) of the grid: FP32, or with F64 the dense FP64 one.
+template <bool F64 = false>
+__device__ __forceinline__ std::conditional_t<F64, const double*, const float*>
+hessianArray(const QGrid& g)
+{
+    if constexpr (F64)
+        return g.h64;
+    else
+        return g.sparse ? g.sh6 : g.h6f;
+}
+
+// (phi, n) of contact-aware candidate slot q (record * stride + c).
+template <bool F64 = false>
+__device__ __forceinline__ std::conditional_t<F64, double4, float4> candidateAt(const QGrid& g,
+                                                                                int64_t q)
+{
+    if constexpr (F64)
+        return make_double4(g.cphi64[q], g.cn64[3 * q], g.cn64[3 * q + 1], g.cn64[3 * q + 2]);
+    else
+        return g.sparse ? g.scpn[q] : g.cpn[q];
 }
 
 // Continuous cell location (SdfOracle.cpp:530-549).
@@ -191,22 +226,27 @@ __device__ __forceinline__ Basis<T> hermite(T t)
 }
 
 // The 8 corner records (phi, n) of a cell; false if a sample is missing.
-__device__ __forceinline__ bool cellCorners(const QGrid& g, const Cell& c, float4 (&f)[8])
+template <bool F64 = false>
+__device__ __forceinline__ bool cellCorners(const QGrid& g, const Cell& c,
+                                            std::conditional_t<F64, double4, float4> (&f)[8])
 {
-    if (g.sparse ? g.scellRec != nullptr : g.cellRec != nullptr)
+    if (!F64 && (g.sparse ? g.scellRec != nullptr : g.cellRec != nullptr))
     {
-        const float4* r = cellRecord(g, c.i0, c.j0, c.k0);
-        if (!r) return false;
+        if constexpr (!F64)
+        {
+            const float4* r = cellRecord(g, c.i0, c.j0, c.k0);
+            if (!r) return false;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) f[q] = r[q];
-        return true;
+            for (int q = 0; q < 8; ++q) f[q] = r[q];
+            return true;
+        }
     }
 #pragma unroll
     for (int q = 0; q < 8; ++q)
     {
         const int64_t rec = recordOf(g, c.i0 + (q & 1), c.j0 + ((q >> 1) & 1), c.k0 + (q >> 2));
         if (rec < 0) return false;
-        f[q] = fieldAt(g, rec);
+        f[q] = fieldAt<F64>(g, rec);
     }
     return true;
 }
@@ -214,24 +254,25 @@ __device__ __forceinline__ bool cellCorners(const QGrid& g, const Cell& c, float
 // Tensor-product cubic Hermite on the 2x2x2 cell corners with corner data
 // (phi, h * n) and no cross-derivative terms (SdfOracle.cpp:932-1092).
 // Returns false if a corner sample is missing (sparse).
-template <typename T, bool WithHess>
-__device__ __forceinline__ void tricubicCorners(const QGrid& g, const Cell& c, const float4 (&f)[8],
+template <typename T, bool WithHess, typename F4>
+__device__ __forceinline__ void tricubicCorners(const QGrid& g, const Cell& c, const F4 (&f)[8],
                                                 Interp<T>& r);
 
-template <typename T, bool WithHess>
+template <typename T, bool WithHess, bool F64 = false>
 __device__ __forceinline__ bool tricubic(const QGrid& g, const Cell& c, Interp<T>& r)
 {
-    float4 f[8];
-    if (!cellCorners(g, c, f)) return false;
+    std::conditional_t<F64, double4, float4> f[8];
+    if (!cellCorners<F64>(g, c, f)) return false;
     tricubicCorners<T, WithHess>(g, c, f, r);
     return true;
 }
 
 // The interpolation itself, on corner records already loaded.
-template <typename T, bool WithHess>
-__device__ __forceinline__ void tricubicCorners(const QGrid& g, const Cell& c, const float4 (&f)[8],
+template <typename T, bool WithHess, typename F4>
+__device__ __forceinline__ void tricubicCorners(const QGrid& g, const Cell& c, const F4 (&f)[8],
                                                 Interp<T>& r)
 {
+    constexpr bool kSeq = std::is_same_v<F4, double4>;
     const Basis<T> A = hermite<T, WithHess>(T(c.u));
     const Basis<T> B = hermite<T, WithHess>(T(c.v));
     const Basis<T> Cz = hermite<T, WithHess>(T(c.w));
@@ -247,31 +288,47 @@ __device__ __forceinline__ void tricubicCorners(const QGrid& g, const Cell& c, c
         const T P0 = A.v(di, 0), P1 = A.v(di, 1), dP0 = A.d(di, 0), dP1 = A.d(di, 1);
         const T Q0 = B.v(dj, 0), Q1 = B.v(dj, 1), dQ0 = B.d(dj, 0), dQ1 = B.d(dj, 1);
         const T R0 = Cz.v(dk, 0), R1 = Cz.v(dk, 1), dR0 = Cz.d(dk, 0), dR1 = Cz.d(dk, 1);
-        // s(A,B,C) = A0 B0 C0 phi + A1 B0 C0 gx + A0 B1 C0 gy + A0 B0 C1 gz
-        gap += (P0 * Q0 * R0) * phi + (P1 * Q0 * R0) * gx + (P0 * Q1 * R0) * gy + (P0 * Q0 * R1) * gz;
-        fu += (dP0 * Q0 * R0) * phi + (dP1 * Q0 * R0) * gx + (dP0 * Q1 * R0) * gy +
-              (dP0 * Q0 * R1) * gz;
-        fv += (P0 * dQ0 * R0) * phi + (P1 * dQ0 * R0) * gx + (P0 * dQ1 * R0) * gy +
-              (P0 * dQ0 * R1) * gz;
-        fw += (P0 * Q0 * dR0) * phi + (P1 * Q0 * dR0) * gx + (P0 * Q1 * dR0) * gy +
-              (P0 * Q0 * dR1) * gz;
+        // s(A,B,C) = A0 B0 C0 phi + A1 B0 C0 gx + A0 B1 C0 gy + A0 B0 C1 gz.  FP64
+        // corners add the four terms one at a time into the running sum, the
+        // CPU path's order (SdfOracle.cpp:999-1078), so its results are its bits;
+        // FP32 corners sum each corner's terms first (shorter dependency chain).
+        auto acc = [&](T& s, T a, T b, T c, T d) {
+            if constexpr (kSeq)
+            {
+                s += a;
+                s += b;
+                s += c;
+                s += d;
+            }
+            else
+            {
+                s += a + b + c + d;
+            }
+        };
+        acc(gap, (P0 * Q0 * R0) * phi, (P1 * Q0 * R0) * gx, (P0 * Q1 * R0) * gy, (P0 * Q0 * R1) * gz);
+        acc(fu, (dP0 * Q0 * R0) * phi, (dP1 * Q0 * R0) * gx, (dP0 * Q1 * R0) * gy,
+            (dP0 * Q0 * R1) * gz);
+        acc(fv, (P0 * dQ0 * R0) * phi, (P1 * dQ0 * R0) * gx, (P0 * dQ1 * R0) * gy,
+            (P0 * dQ0 * R1) * gz);
+        acc(fw, (P0 * Q0 * dR0) * phi, (P1 * Q0 * dR0) * gx, (P0 * Q1 * dR0) * gy,
+            (P0 * Q0 * dR1) * gz);
         if (WithHess)
         {
             const T eP0 = A.e(di, 0), eP1 = A.e(di, 1);
             const T eQ0 = B.e(dj, 0), eQ1 = B.e(dj, 1);
             const T eR0 = Cz.e(dk, 0), eR1 = Cz.e(dk, 1);
-            huu += (eP0 * Q0 * R0) * phi + (eP1 * Q0 * R0) * gx + (eP0 * Q1 * R0) * gy +
-                   (eP0 * Q0 * R1) * gz;
-            hvv += (P0 * eQ0 * R0) * phi + (P1 * eQ0 * R0) * gx + (P0 * eQ1 * R0) * gy +
-                   (P0 * eQ0 * R1) * gz;
-            hww += (P0 * Q0 * eR0) * phi + (P1 * Q0 * eR0) * gx + (P0 * Q1 * eR0) * gy +
-                   (P0 * Q0 * eR1) * gz;
-            huv += (dP0 * dQ0 * R0) * phi + (dP1 * dQ0 * R0) * gx + (dP0 * dQ1 * R0) * gy +
-                   (dP0 * dQ0 * R1) * gz;
-            huw += (dP0 * Q0 * dR0) * phi + (dP1 * Q0 * dR0) * gx + (dP0 * Q1 * dR0) * gy +
-                   (dP0 * Q0 * dR1) * gz;
-            hvw += (P0 * dQ0 * dR0) * phi + (P1 * dQ0 * dR0) * gx + (P0 * dQ1 * dR0) * gy +
-                   (P0 * dQ0 * dR1) * gz;
+            acc(huu, (eP0 * Q0 * R0) * phi, (eP1 * Q0 * R0) * gx, (eP0 * Q1 * R0) * gy,
+                (eP0 * Q0 * R1) * gz);
+            acc(hvv, (P0 * eQ0 * R0) * phi, (P1 * eQ0 * R0) * gx, (P0 * eQ1 * R0) * gy,
+                (P0 * eQ0 * R1) * gz);
+            acc(hww, (P0 * Q0 * eR0) * phi, (P1 * Q0 * eR0) * gx, (P0 * Q1 * eR0) * gy,
+                (P0 * Q0 * eR1) * gz);
+            acc(huv, (dP0 * dQ0 * R0) * phi, (dP1 * dQ0 * R0) * gx, (dP0 * dQ1 * R0) * gy,
+                (dP0 * dQ0 * R1) * gz);
+            acc(huw, (dP0 * Q0 * dR0) * phi, (dP1 * Q0 * dR0) * gx, (dP0 * Q1 * dR0) * gy,
+                (dP0 * Q0 * dR1) * gz);
+            acc(hvw, (P0 * dQ0 * dR0) * phi, (P1 * dQ0 * dR0) * gx, (P0 * dQ1 * dR0) * gy,
+                (P0 * dQ0 * dR1) * gz);
         }
     }
     r.gap = gap;
@@ -289,22 +346,22 @@ __device__ __forceinline__ void tricubicCorners(const QGrid& g, const Cell& c, c
 
 // Trilinear value and analytic gradient (SdfOracle.cpp:551-583); its only
 // non-zero second derivatives are the mixed ones.
-template <typename T, bool WithHess>
-__device__ __forceinline__ void trilinearCorners(const QGrid& g, const Cell& c,
-                                                 const float4 (&f)[8], Interp<T>& r);
+template <typename T, bool WithHess, typename F4>
+__device__ __forceinline__ void trilinearCorners(const QGrid& g, const Cell& c, const F4 (&f)[8],
+                                                 Interp<T>& r);
 
-template <typename T, bool WithHess>
+template <typename T, bool WithHess, bool F64 = false>
 __device__ __forceinline__ bool trilinear(const QGrid& g, const Cell& c, Interp<T>& r)
 {
-    float4 f[8];
-    if (!cellCorners(g, c, f)) return false;
+    std::conditional_t<F64, double4, float4> f[8];
+    if (!cellCorners<F64>(g, c, f)) return false;
     trilinearCorners<T, WithHess>(g, c, f, r);
     return true;
 }
 
-template <typename T, bool WithHess>
-__device__ __forceinline__ void trilinearCorners(const QGrid& g, const Cell& c,
-                                                 const float4 (&f)[8], Interp<T>& r)
+template <typename T, bool WithHess, typename F4>
+__device__ __forceinline__ void trilinearCorners(const QGrid& g, const Cell& c, const F4 (&f)[8],
+                                                 Interp<T>& r)
 {
     T p[2][2][2];
 #pragma unroll

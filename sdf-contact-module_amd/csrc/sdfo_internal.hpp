@@ -383,6 +383,11 @@ void launchSparseMirror(int64_t count, const float* sphi, const float* sn3, floa
 void launchQueryF32(const sdfo_grid& g, int mode, const float* pts, int64_t n, float* gap,
                     float* grad3, uint8_t* valid, float* normal3, float* witness3,
                     int32_t* branch, hipStream_t s);
+// FP64 points on a dense grid's FP64 fields, without FMA contraction: the
+// CPU path's results bit for bit (query_f64.hip).
+void launchQueryDenseF64(const sdfo_grid& g, int mode, const double* pts, int64_t n, double* gap,
+                         double* grad3, uint8_t* valid, double* normal3, double* witness3,
+                         int32_t* branch, hipStream_t s);
 void launchQueryF64(const sdfo_grid& g, int mode, const double* pts, int64_t n, double* gap,
                     double* grad3, uint8_t* valid, double* normal3, double* witness3,
                     int32_t* branch, hipStream_t s);

@@ -84,6 +84,13 @@ def test_dense_contact_aware_bit_exact(O, sdfo_gpu, name, mk, nf, res, params):
     # branch ids are exact unless two candidates tie on |gap| within FP32 rounding
     assert (bb[vv] == rq["branch"][vv]).mean() > 0.999
 
+    # FP64 points read the FP64 candidates: the oracle's answer bit for bit
+    p64 = scenes.box_points(g.spec, 8000, seed=5)
+    out = g.query(torch.from_numpy(p64).cuda(), "ContactAware", full=True)
+    r64 = O.query_dense(ograd(O, g.spec), ref, "ContactAware", p64)
+    for k, a in zip(("gap", "grad", "valid", "normal", "witness", "branch"), out):
+        assert np.array_equal(a.cpu().numpy(), r64[k]), k
+
 
 SPARSE = [
     ("cube_faces_32", lambda: meshes.cube(0.1), face_corner_normals, 32, dict(band_voxels=2)),

@@ -57,16 +57,24 @@ def test_dense_query_modes(O, torus_grid, mode):
     assert np.all(gg[~v] == 0.0)
 
 
-@pytest.mark.parametrize("mode", ["Trilinear", "Tricubic", "SecondOrder"])
+@pytest.mark.parametrize("mode", MODES)
 def test_dense_query_f64_points(O, torus_grid, mode):
+    """FP64 points on a dense grid read its FP64 fields in FP64 arithmetic
+    without FMA contraction: every output is the oracle's on the same FP64
+    SdfData, bit for bit."""
     g, ref = torus_grid
+    print(f"[gpu] dense FP64-point query {mode}", flush=True)
     og = ograd(O, g.spec)
-    pts = scenes.box_points(g.spec, 5000, seed=12)
-    gap, grad, valid = g.query(torch.from_numpy(pts).cuda(), mode)
-    rq = O.query_dense(og, dict(phi=f32(ref["phi"]), n=f32(ref["n"]), H=f32(ref["H"])), mode, pts)
+    pts = _points(g.spec, 5000, seed=12).astype(np.float64)
+    pts[:4000] = scenes.box_points(g.spec, 4000, seed=13)  # off the FP32 lattice
+    gap, grad, valid, normal, wit, br = g.query(torch.from_numpy(pts).cuda(), mode, full=True)
+    rq = O.query_dense(og, ref, mode, pts)
     assert np.array_equal(valid.cpu().numpy(), rq["valid"])
-    assert np.abs(gap.cpu().numpy() - rq["gap"]).max() <= 1e-12
-    assert np.abs(grad.cpu().numpy() - rq["grad"]).max() <= 1e-9
+    assert np.array_equal(gap.cpu().numpy(), rq["gap"])
+    assert np.array_equal(grad.cpu().numpy(), rq["grad"])
+    assert np.array_equal(normal.cpu().numpy(), rq["normal"])
+    assert np.array_equal(wit.cpu().numpy(), rq["witness"])
+    assert np.array_equal(br.cpu().numpy(), rq["branch"])
 
 
 def test_query_exact_on_separable_cubic(O, sdfo_gpu):
