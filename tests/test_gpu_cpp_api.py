@@ -1,7 +1,8 @@
 """The source-compatible C++ API (include/SdfOracle, the reference's
 SdfBuilder / SdfOracle / SdfIO / mesh loaders) driven by a compiled program,
 checked against the Python / C-ABI path: same files byte for byte, same query
-results, ContactAware candidates equal to the FP64 oracle's."""
+results; on dense grids every query result, ContactAware candidates included,
+equal to the FP64 oracle's bit for bit."""
 import os
 import struct
 import subprocess
@@ -82,7 +83,7 @@ def test_files_byte_identical_to_c_abi_builds(sdfo_gpu, run, tmp_path):
 
 
 @pytest.mark.parametrize("kind", ["dense", "sparse"])
-def test_queries_match_c_abi(sdfo_gpu, run, kind):
+def test_queries_match_c_abi(O, sdfo_gpu, run, kind):
     d, v, t, pts = run["dir"], run["v"], run["t"], run["pts"]
     if kind == "dense":
         g = sdfo_gpu.SdfGrid.build_dense(v, t, target_resolution=32, max_resolution=32)
@@ -100,6 +101,13 @@ def test_queries_match_c_abi(sdfo_gpu, run, kind):
         # every valid result carries its primary candidate (branch 0)
         assert np.all(r[vv, 8] == 1) and np.all(r[~vv, 8] == 0), m
         assert np.array_equal(r[vv, 9], r[vv, 0]) and np.all(r[vv, 16] == 0), m
+        if kind == "dense":
+            # SdfOracle on a dense grid: the CPU path's numbers bit for bit
+            rq = O.query_dense(ograd(O, g.spec), g.download_dense(), m, pts)
+            assert np.array_equal(vv, rq["valid"]), m
+            assert np.array_equal(r[vv, 0], rq["gap"][vv]), m
+            assert np.array_equal(r[vv, 1:4], rq["normal"][vv]), m
+            assert np.array_equal(r[vv, 4:7], rq["witness"][vv]), m
 
 
 def test_contact_aware_candidates_match_oracle(O, sdfo_gpu, run):
@@ -116,7 +124,9 @@ def test_contact_aware_candidates_match_oracle(O, sdfo_gpu, run):
     for i in np.nonzero(vv & (cnt > 1))[0][:50]:
         gaps = r[i, 9 + 8 * np.arange(cnt[i])]
         assert r[i, 0] == gaps[int(np.argmin(np.abs(gaps)))]
-    assert np.abs(r[vv, 0] - rq["gap"][vv]).max() <= 1e-12
+    assert np.array_equal(r[vv, 0], rq["gap"][vv])
+    assert np.array_equal(r[vv, 1:4], rq["normal"][vv])
+    assert np.array_equal(r[vv, 4:7], rq["witness"][vv])
     assert (r[vv, 9 + 7] >= -1).all()
 
 
