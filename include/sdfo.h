@@ -221,11 +221,11 @@ int sdfo_query(const sdfo_grid* grid, int32_t mode, const float* points, int64_t
                float* grad3, uint8_t* valid, float* normal3, float* witness3, int32_t* branch,
                sdfo_stream stream);
 
-/* Same for FP64 point streams with FP64 outputs, in FP64 arithmetic.  On a
- * dense grid the kernels read the FP64 fields (phi, n, H, candidates) without
- * FMA contraction, so every output equals the CPU path's (SdfOracle.cpp:
- * 176-1092) bit for bit; a sparse grid's fields are FP32, as in the
- * reference.  Used by the source-compatible C++ SdfOracle mirror. */
+/* Same for FP64 point streams with FP64 outputs, in FP64 arithmetic without
+ * FMA contraction, on a dense grid's FP64 fields (phi, n, H, candidates) or a
+ * sparse grid's FP32 fields widened (sparse data is FP32, as in the
+ * reference): every output equals the CPU path's (SdfOracle.cpp:176-1092) bit
+ * for bit.  Used by the source-compatible C++ SdfOracle mirror. */
 int sdfo_query_f64(const sdfo_grid* grid, int32_t mode, const double* points, int64_t n,
                    double* gap, double* grad3, uint8_t* valid, double* normal3,
                    double* witness3, int32_t* branch, sdfo_stream stream);

@@ -91,27 +91,46 @@ __device__ __forceinline__ int64_t recordOf(const QGrid& g, int i, int j, int k)
     return int64_t(b) * (bd * bd * bd) + (i - bi * bd) + bd * ((j - bj * bd) + bd * (k - bk * bd));
 }
 
-// (phi, n) of a voxel record: the FP32 mirror, or with F64 a dense grid's
-// FP64 fields.
+// (phi, n) of a voxel record: the FP32 mirror; with F64, a dense grid's FP64
+// fields or a sparse grid's FP32 fields widened (the reference's sparse data
+// is FP32).
 template <bool F64 = false>
 __device__ __forceinline__ std::conditional_t<F64, double4, float4> fieldAt(const QGrid& g,
                                                                              int64_t rec)
 {
     if constexpr (F64)
+    {
+        if (g.sparse)
+        {
+            const float4 f = g.spn[rec];
+            return make_double4(f.x, f.y, f.z, f.w);
+        }
         return make_double4(g.phi64[rec], g.n64[3 * rec], g.n64[3 * rec + 1], g.n64[3 * rec + 2]);
+    }
     else
+    {
         return g.sparse ? g.spn[rec] : g.pn[rec];
+    }
 }
 
-// The Hessian array (6 per record) of the grid: FP32, or with F64 the dense FP64 one.
+// The 6 Hessian entries of a voxel record (FP64 for a dense grid with F64,
+// else FP32 widened); false if the grid stores none.
 template <bool F64 = false>
-__device__ __forceinline__ std::conditional_t<F64, const double*, const float*>
-hessianArray(const QGrid& g)
+__device__ __forceinline__ bool hessianAt(const QGrid& g, int64_t rec,
+                                          std::conditional_t<F64, double, float> (&h)[6])
 {
-    if constexpr (F64)
-        return g.h64;
-    else
-        return g.sparse ? g.sh6 : g.h6f;
+    if (F64 && !g.sparse)
+    {
+        if (g.h64 == nullptr) return false;
+#pragma unroll
+        for (int e = 0; e < 6; ++e) h[e] = g.h64[6 * rec + e];
+        return true;
+    }
+    const float* p = g.sparse ? g.sh6 : g.h6f;
+    if (p == nullptr) return false;
+#pragma unroll
+    for (int e = 0; e < 6; ++e) h[e] = p[6 * rec + e];
+    return true;
 }
 
 // (phi, n) of contact-aware candidate slot q (record * stride + c).
@@ -120,9 +139,18 @@ __device__ __forceinline__ std::conditional_t<F64, double4, float4> candidateAt(
                                                                                 int64_t q)
 {
     if constexpr (F64)
+    {
+        if (g.sparse)
+        {
+            const float4 f = g.scpn[q];
+            return make_double4(f.x, f.y, f.z, f.w);
+        }
         return make_double4(g.cphi64[q], g.cn64[3 * q], g.cn64[3 * q + 1], g.cn64[3 * q + 2]);
+    }
     else
+    {
         return g.sparse ? g.scpn[q] : g.cpn[q];
+    }
 }
 
 // Continuous cell location (SdfOracle.cpp:530-549).

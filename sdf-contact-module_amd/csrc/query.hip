@@ -315,13 +315,9 @@ void launchQueryF64(const sdfo_grid& g, int mode, const double* pts, int64_t n, 
                     double* grad3, uint8_t* valid, double* normal3, double* witness3,
                     int32_t* branch, hipStream_t s)
 {
-    if (!g.sparse && g.phi && g.n3)
-    {
-        // a dense grid keeps its FP64 fields: the exact kernels read those
-        launchQueryDenseF64(g, mode, pts, n, gap, grad3, valid, normal3, witness3, branch, s);
-        return;
-    }
-    launchQuery<double, double>(g, mode, pts, n, gap, grad3, valid, normal3, witness3, branch, s);
+    // FP64 points run the exact kernels: a dense grid's FP64 fields, a sparse
+    // grid's FP32 fields widened, no FMA contraction
+    launchQueryExactF64(g, mode, pts, n, gap, grad3, valid, normal3, witness3, branch, s);
 }
 
 // ---- batched query plans ------------------------------------------------

@@ -73,8 +73,8 @@ __device__ __forceinline__ void taylorQuery(const QGrid& g, double px, double py
     const T dz = T(pz - (g.bmin[2] + g.voxel[2] * (k + 0.5)));
     const T nx = T(f.y), ny = T(f.z), nz = T(f.w);
     const T first = nx * dx + ny * dy + nz * dz;
-    const auto* hp = dev::hessianArray<F64>(g);
-    if (!second || hp == nullptr || !g.hasHessian)
+    std::conditional_t<F64, double, float> h[6];
+    if (!second || !g.hasHessian || !dev::hessianAt<F64>(g, rec, h))
     {
         o.gap = T(f.x) + first;
         o.grad = V3T<T>{nx, ny, nz};
@@ -82,7 +82,6 @@ __device__ __forceinline__ void taylorQuery(const QGrid& g, double px, double py
         o.valid = true;
         return;
     }
-    const auto* h = hp + 6 * rec;
     const T hxx = T(h[0]), hxy = T(h[1]), hxz = T(h[2]), hyy = T(h[3]), hyz = T(h[4]),
             hzz = T(h[5]);
     const T Hx = hxx * dx + hxy * dy + hxz * dz;

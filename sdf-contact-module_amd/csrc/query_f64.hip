@@ -1,7 +1,8 @@
-// FP64 point queries on a dense grid's FP64 fields (SdfOracle.cpp:176-1092):
-// the query modes of query_device.hpp on the authoritative SdfData arrays,
-// built without FMA contraction, so every result is the CPU path's bit for
-// bit (the FP32 kernels of query.hip read the FP32 mirror instead).
+// FP64 point queries (SdfOracle.cpp:176-1092): the query modes of
+// query_device.hpp in FP64 arithmetic on a dense grid's FP64 fields (SdfData)
+// or a sparse grid's FP32 fields widened (SparseSdfData holds FP32, as in the
+// reference), built without FMA contraction, so every result is the CPU
+// path's bit for bit (the FP32 kernels of query.hip read the FP32 mirror).
 #include <hip/hip_runtime.h>
 
 #include "query_device.hpp"
@@ -13,7 +14,7 @@ namespace dev {
 QGrid makeQGrid(const sdfo_grid& g);
 } // namespace dev
 
-void launchQueryDenseF64(const sdfo_grid& g, int mode, const double* pts, int64_t n, double* gap,
+void launchQueryExactF64(const sdfo_grid& g, int mode, const double* pts, int64_t n, double* gap,
                          double* grad3, uint8_t* valid, double* normal3, double* witness3,
                          int32_t* branch, hipStream_t s)
 {

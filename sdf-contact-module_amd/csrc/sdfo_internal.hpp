@@ -385,7 +385,7 @@ void launchQueryF32(const sdfo_grid& g, int mode, const float* pts, int64_t n, f
                     int32_t* branch, hipStream_t s);
 // FP64 points on a dense grid's FP64 fields, without FMA contraction: the
 // CPU path's results bit for bit (query_f64.hip).
-void launchQueryDenseF64(const sdfo_grid& g, int mode, const double* pts, int64_t n, double* gap,
+void launchQueryExactF64(const sdfo_grid& g, int mode, const double* pts, int64_t n, double* gap,
                          double* grad3, uint8_t* valid, double* normal3, double* witness3,
                          int32_t* branch, hipStream_t s);
 void launchQueryF64(const sdfo_grid& g, int mode, const double* pts, int64_t n, double* gap,

@@ -126,6 +126,12 @@ def test_sparse_contact_aware_bit_exact(O, sdfo_gpu, name, mk, nf, res, params):
     rq = O.query_sparse(ograd(O, g.spec), ref, "ContactAware", pts.astype(np.float64))
     assert np.array_equal(valid.cpu().numpy(), rq["valid"])
     assert np.abs(gap.cpu().numpy() - rq["gap"]).max() <= 1e-6 * g.spec.diagonal
+    # FP64 points: the FP32 candidates widened, as the CPU path does, bit for bit
+    p64 = pts.astype(np.float64) + 1e-9
+    out = g.query(torch.from_numpy(p64).cuda(), "ContactAware", full=True)
+    r64 = O.query_sparse(ograd(O, g.spec), ref, "ContactAware", p64)
+    for k, a in zip(("gap", "grad", "valid", "normal", "witness", "branch"), out):
+        assert np.array_equal(a.cpu().numpy(), r64[k]), k
 
 
 @pytest.mark.parametrize("cap", [(1, 0), (8, 0), (1, 2)], ids=["1-64", "8-64", "1-2"])

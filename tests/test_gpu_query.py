@@ -126,6 +126,22 @@ def test_sparse_query_modes(O, sparse_grid, mode):
     assert np.abs(grad.cpu().numpy() - rq["grad"]).max() <= 2e-4
 
 
+@pytest.mark.parametrize("mode", MODES)
+def test_sparse_query_f64_points(O, sparse_grid, mode):
+    """FP64 points on a sparse grid: FP64 arithmetic on the FP32 brick fields
+    widened, as the CPU path does, so every output is the oracle's bit for bit."""
+    g, sp, v, t = sparse_grid
+    print(f"[gpu] sparse FP64-point query {mode}", flush=True)
+    pts = np.concatenate([scenes.near_surface_points(v, t, 6000, 2 * g.spec.voxel[0], seed=8),
+                          _points(g.spec, 2000, seed=9)]).astype(np.float64)
+    pts[:3000] += 1e-9 * np.random.default_rng(10).standard_normal((3000, 3))  # off the FP32 lattice
+    out = g.query(torch.from_numpy(pts).cuda(), mode, full=True)
+    rq = O.query_sparse(ograd(O, g.spec), sp, mode, pts)
+    assert 0.3 < rq["valid"].mean() < 1.0
+    for k, a in zip(("gap", "grad", "valid", "normal", "witness", "branch"), out):
+        assert np.array_equal(a.cpu().numpy(), rq[k]), k
+
+
 def test_query_batch_contract(torus_grid):
     g, _ = torus_grid
     pts = _points(g.spec, 1000, seed=3)
