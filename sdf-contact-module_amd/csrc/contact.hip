@@ -300,55 +300,9 @@ contactForwardKernel(QGrid g, Law L, const double* __restrict__ X, const double*
         acc[5] += wi * tq.z;
         acc[6] += 1.0;
     };
-    if (!Cached && (g.sparse ? g.scellRec != nullptr : g.cellRec != nullptr))
-    {
-        // two points per step: both cell records are requested before either
-        // is interpolated; points are added in the same order as below
-        for (int64_t i0 = beg + threadIdx.x; i0 < end; i0 += 2 * kBlock)
-        {
-            Terms t[2];
-            dev::Cell c[2];
-            float4 f[2][8];
-#pragma unroll
-            for (int h = 0; h < 2; ++h)
-            {
-                const int64_t i = i0 + h * kBlock;
-                t[h].active = false;
-                c[h].valid = false;
-                if (i >= end) continue;
-                if (Culled && cull &&
-                    !leafOverlaps(cu.leafBox + 6 * int64_t(cu.leafOf[i]), q, cu.margin))
-                    continue;
-                const V x = V{X[3 * i], X[3 * i + 1], X[3 * i + 2]};
-                t[h].Y = vadd(matvec(P.R, x), V{P.t[0], P.t[1], P.t[2]});
-                t[h].off = vadd(matvec(P.M, x), V{P.m[0], P.m[1], P.m[2]});
-                if (L.margin > 0.0 && !(fabs(t[h].Y.z) < L.margin)) continue;
-                c[h] = dev::locateCell(g, t[h].Y.x, t[h].Y.y, t[h].Y.z);
-                if (!c[h].valid) continue;
-                const float4* r = dev::cellRecord(g, c[h].i0, c[h].j0, c[h].k0);
-                if (!r)
-                {
-                    c[h].valid = false;
-                    continue;
-                }
-#pragma unroll
-                for (int e = 0; e < 8; ++e) f[h][e] = r[e];
-            }
-#pragma unroll
-            for (int h = 0; h < 2; ++h)
-            {
-                if (!c[h].valid) continue;
-                Interp<double> it;
-                if (L.mode == SDFO_TRILINEAR)
-                    dev::trilinearCorners<double, false>(g, c[h], f[h], it);
-                else
-                    dev::tricubicCorners<double, false>(g, c[h], f[h], it);
-                lawTerms<false>(t[h], P, L, it);
-                if (t[h].active) add(t[h], i0 + h * kBlock);
-            }
-        }
-    }
-    else
+    // one point per step: at 136 VGPRs (three waves per SIMD) this beat
+    // requesting two points' records before interpolating either (218 VGPRs,
+    // two waves), since a pose's points hit L2-resident records
     for (int64_t i = beg + threadIdx.x; i < end; i += kBlock)
     {
         if (Culled && cull && !leafOverlaps(cu.leafBox + 6 * int64_t(cu.leafOf[i]), q, cu.margin))
