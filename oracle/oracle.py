@@ -490,3 +490,87 @@ def sort_by_abs(values):
     order = np.zeros(len(v), np.int32)
     lib().oracle_sort_by_abs(_p(v), C.c_int(len(v)), _p(order, _ip))
     return order
+
+
+# --------------------------------------------------------- evaluate (world)
+def pair_states(rA, RA, vA, wA, rB, RB, vB, wB):
+    """(B,36) records rA RA vA wA rB RB vB wB of SDFContactEngine.evaluate's
+    body states (contacts_sdf.py:231-240)."""
+    parts = [rA, RA, vA, wA, rB, RB, vB, wB]
+    B = len(np.asarray(rA).reshape(-1, 3))
+    return np.concatenate([np.asarray(a, np.float64).reshape(B, -1) for a in parts], axis=1)
+
+
+def _state_fields(s):
+    s = _f64(s, (36,))
+    return (s[0:3], s[3:12].reshape(3, 3), s[12:15], s[15:18], s[18:21], s[21:30].reshape(3, 3),
+            s[30:33], s[33:36])
+
+
+def evaluate_keep_mask(nodes, states, frames, bmin, bmax, narrow_margin, n):
+    """Broad-phase cull of evaluate (contacts_sdf.py:243-271): the SDF box
+    (bmin, bmax) mapped into the quadrature frame, grown by max(narrow, 0.05),
+    queried in the point tree; every point when nothing overlaps.  (B,N) uint8."""
+    S = _f64(states, (-1, 36))
+    margin = max(narrow_margin, 0.05)
+    bmin, bmax = _f64(bmin, (3,)), _f64(bmax, (3,))
+    keep = np.zeros((len(S), n), np.uint8)
+    for b in range(len(S)):
+        rA, RA, _, _, _, _, _, _ = _state_fields(S[b])
+        if frames is not None:
+            f = _f64(frames, (-1, 24))[b]
+            R_ab, t_ab = f[:9].reshape(3, 3), f[9:12]
+            R_ba, t_ba = R_ab.T, -R_ab.T @ t_ab
+            lo, hi = R_ba @ bmin + t_ba - margin, R_ba @ bmax + t_ba + margin
+        else:
+            lo, hi = RA.T @ (bmin - rA) - margin, RA.T @ (bmax - rA) + margin
+        idx = point_bvh_query(nodes, lo, hi)
+        if len(idx) == 0:
+            idx = np.arange(n)
+        keep[b, idx] = 1
+    return keep
+
+
+def evaluate_body_check(states, center_a, half_a, half_b):
+    """_check_broad_phase (contacts_sdf.py:68-89): 1 where A's rotated local
+    AABB around rA overlaps B's half extents around rB.  (B,) uint8."""
+    S = _f64(states, (-1, 36))
+    out = np.zeros(len(S), np.uint8)
+    for b in range(len(S)):
+        rA, RA, _, _, rB, _, _, _ = _state_fields(S[b])
+        c = rA + RA @ _f64(center_a, (3,))
+        h = np.sum(np.abs(RA) * _f64(half_a, (3,))[None, :], axis=1)
+        diff = c - rB
+        out[b] = 1 if all(abs(diff[i]) <= h[i] + half_b[i] for i in range(3)) else 0
+    return out
+
+
+def contact_evaluate(grid: Grid, phi, n3, X, w, states, frames=None, keep=None, pair_on=None,
+                     kn=1e11, c=1e4, mu=0.0, reg=1e-6, margin=0.01, order=2, mode="Tricubic"):
+    """SDFContactEngine.evaluate in the world frame: (B,12) = F_A, tau_A, F_B,
+    tau_B per pair, and the active-point counts."""
+    X = _f64(X, (-1, 3))
+    w = _f64(w)
+    S = _f64(states, (-1, 36))
+    Fr = None if frames is None else _f64(frames, (-1, 24))
+    K = None if keep is None else np.ascontiguousarray(keep, np.uint8).reshape(len(S), len(X))
+    On = None if pair_on is None else np.ascontiguousarray(pair_on, np.uint8).reshape(len(S))
+    prm, iprm = _contact_params(kn, c, mu, reg, margin, order, mode)
+    out = np.zeros((len(S), 12))
+    act = np.zeros(len(S), np.int32)
+    lib().oracle_contact_evaluate(_p(grid.grid9), _p(grid.shape3, _ip), _p(_f64(phi)),
+                                  _p(_f64(n3, (-1, 3))), _p(X), _p(w), C.c_int64(len(X)), _p(S),
+                                  _p(Fr), C.c_int(len(S)), _p(prm), _p(iprm, _ip), _p(K, _u8p),
+                                  _p(On, _u8p), _p(out), _p(act, _ip))
+    return out, act
+
+
+def sparse_hessian(grid: Grid, block_dim, bijk, n3):
+    """Sparse Hessian stencil (SdfBuilder.cpp:1154-1225) of FP32 brick normals
+    n3 (nb, B^3, 3) -> (nb, B^3, 6) FP32."""
+    bijk = _i32(bijk).reshape(-1, 3)
+    n3 = np.ascontiguousarray(n3, np.float32).reshape(len(bijk), block_dim ** 3, 3)
+    H = np.zeros((len(bijk), block_dim ** 3, 6), np.float32)
+    lib().oracle_sparse_hessian(_p(grid.grid9), _p(grid.shape3, _ip), C.c_int(block_dim),
+                                C.c_int(len(bijk)), _p(bijk, _ip), _p(n3, _fp), _p(H, _fp))
+    return H

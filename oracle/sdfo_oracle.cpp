@@ -1379,6 +1379,121 @@ void contactForward(const double* X, const double* w, int64_t n, const double* p
     }
 }
 
+// SDFContactEngine.evaluate (contacts_sdf.py:225-385) restated in the world
+// frame, step for step, for one contact pair per record.
+// state (36 doubles): rA(3) RA(9) vA(3) wA(3) rB(3) RB(9) vB(3) wB(3)
+// frames (24 doubles, optional): R_ab(9) t_ab(3) RAm(9) rAm(3) -- the action
+//   marker relative to the base marker and the action marker's world pose
+//   (contacts_sdf.py:243-249, 275-279); without them R_sdf = RB^T RA,
+//   t_sdf = RB^T (rA - rB), rA_m = rA, RA_m = RA (:280-283).
+// keep (nb x n, optional): point i of pair b survives the broad phase.
+// pairOn (nb, optional): 0 where _check_broad_phase (:68-89) rejects the pair.
+// The narrow filter |Y_z| < margin always applies (:91-98, RB is given).
+// out12 = F_A, tau_A, F_B, tau_B (world frame).
+struct PairState
+{
+    V3 rA, vA, wA, rB, vB, wB;
+    double RA[9], RB[9];
+};
+
+inline PairState loadPairState(const double* s)
+{
+    PairState p;
+    p.rA = load3(s, 0);
+    std::memcpy(p.RA, s + 3, 9 * sizeof(double));
+    p.vA = load3(s + 12, 0);
+    p.wA = load3(s + 15, 0);
+    p.rB = load3(s + 18, 0);
+    std::memcpy(p.RB, s + 21, 9 * sizeof(double));
+    p.vB = load3(s + 30, 0);
+    p.wB = load3(s + 33, 0);
+    return p;
+}
+
+// A^T B for row-major 3x3 (numpy's RB.T @ RA), sums in index order.
+inline void matTmat(const double* A, const double* B, double* C)
+{
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c)
+            C[3 * r + c] = A[r] * B[c] + A[3 + r] * B[3 + c] + A[6 + r] * B[6 + c];
+}
+
+template <typename QF>
+void contactEvaluate(const double* X, const double* w, int64_t n, const double* states36,
+                     const double* frames24, int nb, const ContactParams& cp,
+                     const uint8_t* keep, const uint8_t* pairOn, QF query, double* out12,
+                     int* activeCount)
+{
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int b = 0; b < nb; ++b)
+    {
+        const PairState S = loadPairState(states36 + 36 * int64_t(b));
+        double Rs[9], RAm[9];
+        V3 ts, rAm;
+        if (frames24)
+        {
+            const double* f = frames24 + 24 * int64_t(b);
+            std::memcpy(Rs, f, 9 * sizeof(double));
+            ts = load3(f + 9, 0);
+            std::memcpy(RAm, f + 12, 9 * sizeof(double));
+            rAm = load3(f + 21, 0);
+        }
+        else
+        {
+            matTmat(S.RB, S.RA, Rs);
+            ts = matTvec(S.RB, S.rA - S.rB);
+            std::memcpy(RAm, S.RA, 9 * sizeof(double));
+            rAm = S.rA;
+        }
+        V3 F = v3(0, 0, 0), T = v3(0, 0, 0);
+        int act = 0;
+        const bool on = pairOn == nullptr || pairOn[b] != 0;
+        for (int64_t i = 0; on && i < n; ++i)
+        {
+            if (keep && !keep[int64_t(b) * n + i]) continue;
+            const V3 x = load3(X, i);
+            const V3 Y = matvec(Rs, x) + ts;
+            if (!(std::abs(Y.z) < cp.margin)) continue; // _narrow_filter
+            const QueryOut q = query(Y);
+            if (!q.valid) continue;
+            const double p = cp.kn * powi(smax(-q.gap, 0.0), cp.order);
+            if (!(p > 1e-30)) continue; // valid_mask, :329
+            ++act;
+            const V3 rg = matvec(S.RB, q.grad); // :342
+            const double gn = norm(q.grad);      // grad_norm of query_batch
+            const V3 xw = rAm + matvec(RAm, x);  // :345
+            const V3 offA = xw - S.rA;           // :348
+            V3 fn = p * rg;                      // :351
+            const double phat = p * gn;          // :352
+            if (cp.c > 0.0)                      // :355-362
+            {
+                const V3 uAd = S.vA + cross(S.wA, offA);
+                const V3 uBd = S.vB + cross(S.wB, xw - S.rB);
+                const double vn = dot(uAd - uBd, rg);
+                if (q.gap < 0.0) fn = fn + (cp.c * (-vn)) * rg;
+            }
+            const V3 uA = S.vA + cross(S.wA, offA); // :364-367
+            const V3 uB = S.vB + cross(S.wB, xw - S.rB);
+            const V3 urel = uA - uB;
+            const V3 un = rg / smax(gn, 1e-30);     // :369
+            const double udn = dot(urel, un);
+            const V3 ut = urel - udn * un;
+            const double utn = norm(ut);
+            const double denom = std::sqrt(utn * utn + cp.reg * cp.reg);
+            V3 ft = v3(0, 0, 0);
+            if (cp.mu * phat > 0.0) ft = (-cp.mu * phat) * ut / denom; // :374-377
+            const V3 fq = fn + ft;
+            F = F + w[i] * fq;                      // :380-381
+            T = T + w[i] * cross(offA, fq);
+        }
+        const V3 FB = -1.0 * F;                     // :383-384
+        const V3 TB = -1.0 * (T + cross(S.rA - S.rB, F));
+        const double o[12] = {F.x, F.y, F.z, T.x, T.y, T.z, FB.x, FB.y, FB.z, TB.x, TB.y, TB.z};
+        for (int c = 0; c < 12; ++c) out12[12 * int64_t(b) + c] = on ? o[c] : 0.0;
+        if (activeCount) activeCount[b] = act;
+    }
+}
+
 // Analytic backward of contactForward (no counterpart in the reference; the
 // derivation is in the comments).  Upstream gradient per pose gF, gT.
 //   a = w (gF + gT x off)                    dL/dfq
@@ -1506,6 +1621,58 @@ void contactBackward(const double* X, const double* w, int64_t n, const double* 
             gW[i] = s;
         }
     }
+}
+
+// SdfBuilder.cpp:1154-1225: central differences of the stored FP32 normals
+// over the active bricks; a neighbour outside the grid or in a missing brick
+// (SparseSdfData::sample returns false) falls back to the voxel's own normal.
+void estimateSparseHessian(const Grid& g, int blockDim, int nb, const int* bijk, const float* nrm,
+                           float* hes)
+{
+    const int bvc = blockDim * blockDim * blockDim;
+    SparseView sv;
+    sv.g = g;
+    sv.blockDim = blockDim;
+    sv.nblocks = nb;
+    sv.bijk = bijk;
+    sv.phi = nullptr;
+    sv.nrm = nrm;
+    sv.hes = nullptr;
+    sv.stride = 0;
+    sv.rebuild();
+    const double vx = g.voxel.x, vy = g.voxel.y, vz = g.voxel.z;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int b = 0; b < nb; ++b)
+        for (int lz = 0; lz < blockDim; ++lz)
+            for (int ly = 0; ly < blockDim; ++ly)
+                for (int lx = 0; lx < blockDim; ++lx)
+                {
+                    const int i = bijk[3 * b] * blockDim + lx;
+                    const int j = bijk[3 * b + 1] * blockDim + ly;
+                    const int k = bijk[3 * b + 2] * blockDim + lz;
+                    if (!g.inBounds(i, j, k)) continue;
+                    const int64_t base = int64_t(b) * bvc + lx + blockDim * (ly + blockDim * lz);
+                    const V3 cur = v3(double(nrm[3 * base]), double(nrm[3 * base + 1]),
+                                      double(nrm[3 * base + 2]));
+                    auto nf = [&](int ii, int jj, int kk) {
+                        const int nbk = sv.blockForVoxel(ii, jj, kk);
+                        if (nbk < 0) return cur;
+                        const int64_t nl = int64_t(nbk) * bvc + (ii - bijk[3 * nbk] * blockDim) +
+                                           blockDim * ((jj - bijk[3 * nbk + 1] * blockDim) +
+                                                       blockDim * (kk - bijk[3 * nbk + 2] * blockDim));
+                        return v3(double(nrm[3 * nl]), double(nrm[3 * nl + 1]), double(nrm[3 * nl + 2]));
+                    };
+                    const V3 dx = (nf(i + 1, j, k) - nf(i - 1, j, k)) / (2.0 * vx);
+                    const V3 dy = (nf(i, j + 1, k) - nf(i, j - 1, k)) / (2.0 * vy);
+                    const V3 dz = (nf(i, j, k + 1) - nf(i, j, k - 1)) / (2.0 * vz);
+                    float* o = &hes[6 * base];
+                    o[0] = static_cast<float>(0.5 * (dx.x + dx.x));
+                    o[1] = static_cast<float>(0.5 * (dy.x + dx.y));
+                    o[2] = static_cast<float>(0.5 * (dz.x + dx.z));
+                    o[3] = static_cast<float>(0.5 * (dy.y + dy.y));
+                    o[4] = static_cast<float>(0.5 * (dz.y + dy.z));
+                    o[5] = static_cast<float>(0.5 * (dz.z + dz.z));
+                }
 }
 
 // Sparse build state kept for the accessors.
@@ -1773,50 +1940,7 @@ void* oracle_build_sparse(const double* verts, int64_t nv, const int* tris, int 
     if (storeHessian)
     {
         sb->hes.assign(size_t(nb) * bvc * 6, 0.0f);
-        SparseView sv;
-        sv.g = g;
-        sv.blockDim = blockDim;
-        sv.nblocks = nb;
-        sv.bijk = sb->bijk.data();
-        sv.phi = sb->phi.data();
-        sv.nrm = sb->nrm.data();
-        sv.hes = nullptr;
-        sv.stride = 0;
-        sv.rebuild();
-        const double vx = g.voxel.x, vy = g.voxel.y, vz = g.voxel.z;
-#pragma omp parallel for schedule(dynamic, 1)
-        for (int b = 0; b < nb; ++b)
-        {
-            for (int lz = 0; lz < blockDim; ++lz)
-                for (int ly = 0; ly < blockDim; ++ly)
-                    for (int lx = 0; lx < blockDim; ++lx)
-                    {
-                        const int i = sb->bijk[3 * b] * blockDim + lx;
-                        const int j = sb->bijk[3 * b + 1] * blockDim + ly;
-                        const int k = sb->bijk[3 * b + 2] * blockDim + lz;
-                        if (!g.inBounds(i, j, k)) continue;
-                        const int64_t local = lx + blockDim * (ly + blockDim * lz);
-                        const int64_t base = int64_t(b) * bvc + local;
-                        const V3 cur = v3(double(sb->nrm[3 * base]), double(sb->nrm[3 * base + 1]),
-                                          double(sb->nrm[3 * base + 2]));
-                        auto nf = [&](int ii, int jj, int kk) {
-                            double ph;
-                            V3 nn;
-                            if (!sv.sample(ii, jj, kk, ph, nn, nullptr)) return cur;
-                            return nn;
-                        };
-                        const V3 dx = (nf(i + 1, j, k) - nf(i - 1, j, k)) / (2.0 * vx);
-                        const V3 dy = (nf(i, j + 1, k) - nf(i, j - 1, k)) / (2.0 * vy);
-                        const V3 dz = (nf(i, j, k + 1) - nf(i, j, k - 1)) / (2.0 * vz);
-                        float* o = &sb->hes[6 * base];
-                        o[0] = static_cast<float>(0.5 * (dx.x + dx.x));
-                        o[1] = static_cast<float>(0.5 * (dy.x + dx.y));
-                        o[2] = static_cast<float>(0.5 * (dz.x + dx.z));
-                        o[3] = static_cast<float>(0.5 * (dy.y + dy.y));
-                        o[4] = static_cast<float>(0.5 * (dz.y + dy.z));
-                        o[5] = static_cast<float>(0.5 * (dz.z + dz.z));
-                    }
-        }
+        estimateSparseHessian(g, blockDim, nb, sb->bijk.data(), sb->nrm.data(), sb->hes.data());
     }
     return sb;
 }
@@ -2054,47 +2178,7 @@ void* oracle_build_contact_aware_sparse(const double* verts, int64_t nv, const i
     if (storeHessian)
     {
         sb->hes.assign(size_t(nb) * bvc * 6, 0.0f);
-        SparseView sv;
-        sv.g = g;
-        sv.blockDim = blockDim;
-        sv.nblocks = nb;
-        sv.bijk = sb->bijk.data();
-        sv.phi = sb->phi.data();
-        sv.nrm = sb->nrm.data();
-        sv.hes = nullptr;
-        sv.stride = 0;
-        sv.rebuild();
-        const double vx = g.voxel.x, vy = g.voxel.y, vz = g.voxel.z;
-#pragma omp parallel for schedule(dynamic, 1)
-        for (int b = 0; b < nb; ++b)
-            for (int lz = 0; lz < blockDim; ++lz)
-                for (int ly = 0; ly < blockDim; ++ly)
-                    for (int lx = 0; lx < blockDim; ++lx)
-                    {
-                        const int i = sb->bijk[3 * b] * blockDim + lx;
-                        const int j = sb->bijk[3 * b + 1] * blockDim + ly;
-                        const int k = sb->bijk[3 * b + 2] * blockDim + lz;
-                        if (!g.inBounds(i, j, k)) continue;
-                        const int64_t base = int64_t(b) * bvc + lx + blockDim * (ly + blockDim * lz);
-                        const V3 cur = v3(double(sb->nrm[3 * base]), double(sb->nrm[3 * base + 1]),
-                                          double(sb->nrm[3 * base + 2]));
-                        auto nf = [&](int ii, int jj, int kk) {
-                            double ph;
-                            V3 nn;
-                            if (!sv.sample(ii, jj, kk, ph, nn, nullptr)) return cur;
-                            return nn;
-                        };
-                        const V3 dx = (nf(i + 1, j, k) - nf(i - 1, j, k)) / (2.0 * vx);
-                        const V3 dy = (nf(i, j + 1, k) - nf(i, j - 1, k)) / (2.0 * vy);
-                        const V3 dz = (nf(i, j, k + 1) - nf(i, j, k - 1)) / (2.0 * vz);
-                        float* o = &sb->hes[6 * base];
-                        o[0] = static_cast<float>(0.5 * (dx.x + dx.x));
-                        o[1] = static_cast<float>(0.5 * (dy.x + dx.y));
-                        o[2] = static_cast<float>(0.5 * (dz.x + dx.z));
-                        o[3] = static_cast<float>(0.5 * (dy.y + dy.y));
-                        o[4] = static_cast<float>(0.5 * (dz.y + dy.z));
-                        o[5] = static_cast<float>(0.5 * (dz.z + dz.z));
-                    }
+        estimateSparseHessian(g, blockDim, nb, sb->bijk.data(), sb->nrm.data(), sb->hes.data());
     }
     return sb;
 }
@@ -2264,6 +2348,32 @@ void oracle_contact_forward_linearized(const double* g0, const double* grad0,
         }
         if (activeCount) activeCount[b] = act;
     }
+}
+
+// SDFContactEngine.evaluate in the world frame on a dense grid (see
+// contactEvaluate).  params6 = kn, c, mu, reg, narrowMargin, unused;
+// iparams2 = order, queryMode.
+void oracle_contact_evaluate(const double* grid9, const int* shape3, const double* phi,
+                             const double* n3, const double* X, const double* w, int64_t n,
+                             const double* states36, const double* frames24, int nb,
+                             const double* params6, const int* iparams2, const uint8_t* keep,
+                             const uint8_t* pairOn, double* out12, int* activeCount)
+{
+    DenseView d{};
+    d.g = gridFromArrays(grid9, shape3);
+    d.phi = phi;
+    d.nrm = n3;
+    ContactParams cp{params6[0], params6[1], params6[2], params6[3], params6[4], iparams2[0],
+                     iparams2[1]};
+    auto q = [&](V3 y) { return denseQuery(d, y, cp.mode, false); };
+    contactEvaluate(X, w, n, states36, frames24, nb, cp, keep, pairOn, q, out12, activeCount);
+}
+
+// Sparse Hessian stencil alone (SdfBuilder.cpp:1154-1225) on given bricks.
+void oracle_sparse_hessian(const double* grid9, const int* shape3, int blockDim, int nblocks,
+                           const int* bijk, const float* nrm, float* hes)
+{
+    estimateSparseHessian(gridFromArrays(grid9, shape3), blockDim, nblocks, bijk, nrm, hes);
 }
 
 // Analytic-plane SDF contact (sdf_grid.py:4-22): phi = z - zTop, grad = e_z.
