@@ -294,6 +294,35 @@ int sdfo_contact_forward(const sdfo_grid* grid, const double* X, const double* w
                          const double* poses, int64_t n_poses, const sdfo_contact_params* params,
                          double* out, int32_t* active, sdfo_stream stream);
 
+/* SDFContactEngine.evaluate (contacts_sdf.py:225-385) for many contact pairs,
+ * in the world frame, FP64 arithmetic, deterministic FP64 sums.
+ * X: n x 3 quadrature points in body A's (action marker) frame, w: n weights
+ * (device, FP64; the engine normalises them to sum 1 at registration).
+ * states: n_pairs x 36 doubles (device), the body states of each pair
+ *   (contacts_sdf.py:231-240): rA(3) RA(9, row major) vA(3) wA(3)
+ *   rB(3) RB(9) vB(3) wB(3); A carries the points, B the SDF.
+ * frames: n_pairs x 24 doubles (device) or NULL: the action marker relative
+ *   to the base marker, R_ab(9) t_ab(3) (Y = R_ab X + t_ab), and the action
+ *   marker's world pose RAm(9) rAm(3) (x_w = rAm + RAm X)
+ *   (contacts_sdf.py:243-249, 275-279); NULL uses the body frames
+ *   (R_sdf = RB^T RA, t_sdf = RB^T (rA - rB), :280-283).
+ * The narrow filter |Y_z| < params->narrow_margin always applies (the
+ * engine's default margin is 0.01; a margin <= 0 rejects every point, as in
+ * the reference).  bvh (built over the same X, or NULL for no cull): the
+ * points of the leaves overlapping the SDF box (grid bmin/bmax mapped into A's
+ * frame, grown by max(narrow_margin, 0.05)) take part, or every point when
+ * no leaf overlaps (:243-271).  body_boxes9 (host, or NULL to skip): A's local
+ * AABB centre(3) and half extents(3), then B's half extents(3): pairs failing
+ * _check_broad_phase (:68-89) return zeros.
+ * out: n_pairs x 12 (device) = F_A, tau_A, F_B, tau_B in the world frame,
+ * tau about each body's reference point; active: n_pairs int32 (optional),
+ * the number of points in the sums. */
+int sdfo_contact_evaluate(const sdfo_grid* grid, const sdfo_point_bvh* bvh, const double* X,
+                          const double* w, int64_t n, const double* states, const double* frames,
+                          int64_t n_pairs, const sdfo_contact_params* params,
+                          const double* body_boxes9, double* out, int32_t* active,
+                          sdfo_stream stream);
+
 /* Linearisation cache of SDFContactEngine.evaluate (contacts_sdf.py:307-327),
  * used by the Newton iterations of implicit_be.py:474-500.
  * sdfo_contact_linearize fills it at one reference pose (a device record of

@@ -55,6 +55,13 @@ int sdfo_debug_fill_budget(int32_t budget, int32_t heavy_cap);
  * passes, so tests can check them. */
 int sdfo_debug_ca_merge_cap(int32_t first, int32_t second);
 
+/* Test switch for the calling thread: grids built or loaded while it is on
+ * keep no cell records (the packed 8-corner copies that FP32 Trilinear /
+ * Tricubic queries and the contact kernels read), as grids whose records
+ * would exceed the library's 4 GiB limit do, so tests can check the
+ * per-corner gather path against the CPU oracle on small grids. */
+int sdfo_debug_skip_cell_records(int32_t on);
+
 #ifdef __cplusplus
 }
 #endif

@@ -44,7 +44,16 @@ void keepStreamPoolMemory(int device)
 
 namespace {
 thread_local std::string g_lastError;
+
+// Test switch of the calling thread: grids built or loaded while it is on
+// keep no cell records, so every query and contact kernel takes the
+// per-corner gather path that grids past kCellRecordLimit use.
+thread_local bool g_skipCellRecords = false;
+
+bool cellRecordsFit(int64_t bytes) { return !g_skipCellRecords && bytes <= kCellRecordLimit; }
 }
+
+void setSkipCellRecords(bool on) { g_skipCellRecords = on; }
 
 void setLastError(const std::string& msg) { g_lastError = msg; }
 
@@ -62,7 +71,7 @@ void finishDense(sdfo_grid& g, hipStream_t s, bool mirrored)
     if (!g.h6f) g.h6f = g.alloc<float>(size_t(N) * 6);
     if (!mirrored) launchDenseMirror(N, g.phi, g.n3, g.h6, g.pn, g.h6f, s);
     const int64_t cells = int64_t(g.spec.n[0] - 1) * (g.spec.n[1] - 1) * (g.spec.n[2] - 1);
-    if (cells > 0 && cells * 8 * int64_t(sizeof(float4)) <= kCellRecordLimit)
+    if (cells > 0 && cellRecordsFit(cells * 8 * int64_t(sizeof(float4))))
     {
         if (!g.cellRec) g.cellRec = g.alloc<float4>(size_t(cells) * 8);
         launchCellPack(g.spec, g.pn, g.cellRec, s);
@@ -104,7 +113,7 @@ void finishSparse(sdfo_grid& g, const std::vector<int32_t>& bijk, hipStream_t s)
     {
         if (!g.spn) g.spn = g.alloc<float4>(size_t(count));
         launchSparseMirror(count, g.sphi, g.sn3, g.spn, s);
-        if (count * 8 * int64_t(sizeof(float4)) <= kCellRecordLimit && g.bijk)
+        if (cellRecordsFit(count * 8 * int64_t(sizeof(float4))) && g.bijk)
         {
             const int words = int((bvc + 63) / 64);
             if (!g.scellRec) g.scellRec = g.alloc<float4>(size_t(count) * 8);
@@ -424,7 +433,7 @@ std::vector<std::unique_ptr<sdfo_grid>> buildDenseBatched(int32_t nm, const doub
         for (size_t per : {sizeof(double), 3 * sizeof(double), 6 * sizeof(double), 3 * sizeof(double),
                            sizeof(int32_t), sizeof(float4), 6 * sizeof(float)})
             b += sdfo_grid::arenaBytes(per * N);
-        if (cells > 0 && cells * 8 * int64_t(sizeof(float4)) <= kCellRecordLimit)
+        if (cells > 0 && cellRecordsFit(cells * 8 * int64_t(sizeof(float4))))
             b += sdfo_grid::arenaBytes(size_t(cells) * 8 * sizeof(float4));
         return b;
     };
@@ -457,7 +466,7 @@ std::vector<std::unique_ptr<sdfo_grid>> buildDenseBatched(int32_t nm, const doub
         g->pn = g->alloc<float4>(size_t(N));
         g->h6f = g->alloc<float>(size_t(N) * 6);
         const int64_t cells = cellsOf(g->spec);
-        if (cells > 0 && cells * 8 * int64_t(sizeof(float4)) <= kCellRecordLimit)
+        if (cells > 0 && cellRecordsFit(cells * 8 * int64_t(sizeof(float4))))
             g->cellRec = g->alloc<float4>(size_t(cells) * 8);
         grids[size_t(m)] = std::move(g);
     }
@@ -1128,6 +1137,28 @@ int sdfo_contact_forward_culled(const sdfo_grid* grid, const sdfo_point_bvh* bvh
         const CullArgs c = cullArgs(bvh, boxes6, n, 0.0);
         launchContactForward(*grid, X, w, n, poses, n_poses, *params, out, active,
                              hipStream_t(stream), &c);
+    });
+}
+
+int sdfo_contact_evaluate(const sdfo_grid* grid, const sdfo_point_bvh* bvh, const double* X,
+                          const double* w, int64_t n, const double* states, const double* frames,
+                          int64_t n_pairs, const sdfo_contact_params* params,
+                          const double* body_boxes9, double* out, int32_t* active,
+                          sdfo_stream stream)
+{
+    return guarded([&] {
+        checkContact(grid, X, w, n, states, n_pairs, params);
+        if (n_pairs > 0 && !out) throw Error(SDFO_ERR_INVALID_ARGUMENT, "out is NULL");
+        DeviceGuard dg(grid->device);
+        CullArgs c;
+        if (bvh)
+        {
+            pointBvhToDevice(*bvh, grid->device);
+            // the boxes are computed on the device; this only checks the point count
+            c = cullArgs(bvh, X, n, 0.0);
+        }
+        launchContactEvaluate(*grid, X, w, n, states, frames, n_pairs, *params, bvh ? &c : nullptr,
+                              body_boxes9, out, active, hipStream_t(stream));
     });
 }
 

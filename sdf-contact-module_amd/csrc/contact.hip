@@ -524,6 +524,214 @@ __global__ void paramReduceKernel(int64_t nposes, const double* __restrict__ per
     }
 }
 
+// ---- SDFContactEngine.evaluate in the world frame (contacts_sdf.py:225-385)
+// Per-pair record laid out by evaluatePrepKernel:
+//   R_sdf(9) t_sdf(3)  quadrature point X -> SDF frame, Y = R_sdf X + t_sdf
+//   RAm(9) rAm(3)      quadrature frame in the world, x_w = rAm + RAm X
+//   rA vA wA (3 each), rB(3) RB(9) vB wB (3 each), on (1: the pair passed
+//   the body-level check of _check_broad_phase)
+constexpr int kEvalRec = 52;
+
+struct EvalPrep
+{
+    const double* states; // pairs x 36: rA RA vA wA rB RB vB wB
+    const double* frames; // pairs x 24: R_ab t_ab RAm rAm, or null
+    double bmin[3], bmax[3];
+    double cullMargin;    // max(narrow margin, 0.05), contacts_sdf.py:243
+    int bodyCheck;
+    double centerA[3], halfA[3], halfB[3];
+};
+
+// One thread per pair: the transforms of contacts_sdf.py:275-283, the cull
+// box of :247-258 and the body-level check of :68-89.
+__global__ void evaluatePrepKernel(EvalPrep in, int64_t nb, double* __restrict__ rec,
+                                   double* __restrict__ boxes)
+{
+    const int64_t b = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    const double* s = in.states + 36 * b;
+    double* r = rec + kEvalRec * b;
+    const V rA = V{s[0], s[1], s[2]};
+    const double* RA = s + 3;
+    const V rB = V{s[18], s[19], s[20]};
+    const double* RB = s + 21;
+    double lo[3], hi[3];
+    if (in.frames)
+    {
+        const double* f = in.frames + 24 * b;
+#pragma unroll
+        for (int k = 0; k < 24; ++k) r[k] = f[k];
+        // R_ba = R_ab^T, t_ba = -R_ab^T t_ab
+        const V tba = vmul(-1.0, matTvec(f, V{f[9], f[10], f[11]}));
+        const V l = vadd(matTvec(f, V{in.bmin[0], in.bmin[1], in.bmin[2]}), tba);
+        const V h = vadd(matTvec(f, V{in.bmax[0], in.bmax[1], in.bmax[2]}), tba);
+        lo[0] = l.x; lo[1] = l.y; lo[2] = l.z;
+        hi[0] = h.x; hi[1] = h.y; hi[2] = h.z;
+    }
+    else
+    {
+        // R_sdf = RB^T RA, t_sdf = RB^T (rA - rB), marker = body frame
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                r[3 * i + j] = RB[i] * RA[j] + RB[3 + i] * RA[3 + j] + RB[6 + i] * RA[6 + j];
+        const V ts = matTvec(RB, vsub(rA, rB));
+        r[9] = ts.x; r[10] = ts.y; r[11] = ts.z;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) r[12 + k] = RA[k];
+        r[21] = rA.x; r[22] = rA.y; r[23] = rA.z;
+        const V l = matTvec(RA, vsub(V{in.bmin[0], in.bmin[1], in.bmin[2]}, rA));
+        const V h = matTvec(RA, vsub(V{in.bmax[0], in.bmax[1], in.bmax[2]}, rA));
+        lo[0] = l.x; lo[1] = l.y; lo[2] = l.z;
+        hi[0] = h.x; hi[1] = h.y; hi[2] = h.z;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+    {
+        boxes[6 * b + k] = lo[k] - in.cullMargin;
+        boxes[6 * b + 3 + k] = hi[k] + in.cullMargin;
+    }
+    // rA vA wA
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+    {
+        r[24 + k] = s[k];
+        r[27 + k] = s[12 + k];
+        r[30 + k] = s[15 + k];
+        r[33 + k] = s[18 + k];
+        r[45 + k] = s[30 + k];
+        r[48 + k] = s[33 + k];
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) r[36 + k] = RB[k];
+    int on = 1;
+    if (in.bodyCheck)
+    {
+        const V c = vadd(rA, matvec(RA, V{in.centerA[0], in.centerA[1], in.centerA[2]}));
+        const double d[3] = {c.x - rB.x, c.y - rB.y, c.z - rB.z};
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+        {
+            const double h = fabs(RA[3 * i]) * in.halfA[0] + fabs(RA[3 * i + 1]) * in.halfA[1] +
+                             fabs(RA[3 * i + 2]) * in.halfA[2];
+            if (!(fabs(d[i]) <= h + in.halfB[i])) on = 0;
+        }
+    }
+    r[51] = double(on);
+}
+
+// grid = (chunks, pairs).  partial[(pair*chunks + chunk)*7 + k]: F_A, tau_A
+// (world frame) and the active count.
+template <bool Culled>
+__global__ void __launch_bounds__(kBlock)
+contactEvaluateKernel(QGrid g, Law L, const double* __restrict__ X, const double* __restrict__ w,
+                      int64_t n, const double* __restrict__ rec, int64_t chunkLen,
+                      double* __restrict__ partial, CullArgs cu, const uint8_t* __restrict__ anyLeaf)
+{
+    __shared__ double sh[7 * kBlock];
+    const int64_t pair = blockIdx.y;
+    const int chunk = blockIdx.x;
+    const double* r = rec + kEvalRec * pair;
+    double Rs[9], RAm[9], RB[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+    {
+        Rs[k] = r[k];
+        RAm[k] = r[12 + k];
+        RB[k] = r[36 + k];
+    }
+    const V ts = V{r[9], r[10], r[11]}, rAm = V{r[21], r[22], r[23]};
+    const V rA = V{r[24], r[25], r[26]}, vA = V{r[27], r[28], r[29]}, wA = V{r[30], r[31], r[32]};
+    const V rB = V{r[33], r[34], r[35]}, vB = V{r[45], r[46], r[47]}, wB = V{r[48], r[49], r[50]};
+    const bool on = r[51] != 0.0;
+    const int64_t beg = int64_t(chunk) * chunkLen;
+    const int64_t end = beg + chunkLen < n ? beg + chunkLen : n;
+    double q[6];
+    bool cull = false;
+    if (Culled)
+    {
+        loadBox(cu.boxes + 6 * pair, q);
+        cull = anyLeaf[pair] != 0;
+    }
+    Law Lq = L;
+    Lq.margin = 0.0; // the narrow filter is applied below, always (RB is given)
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int64_t i = beg + threadIdx.x; on && i < end; i += kBlock)
+    {
+        if (Culled && cull && !leafOverlaps(cu.leafBox + 6 * int64_t(cu.leafOf[i]), q, cu.margin))
+            continue;
+        const V x = V{X[3 * i], X[3 * i + 1], X[3 * i + 2]};
+        const V Y = vadd(matvec(Rs, x), ts);
+        if (!(fabs(Y.z) < L.margin)) continue; // _narrow_filter, contacts_sdf.py:91-98
+        Interp<double> it;
+        if (!sampleField<false>(g, Lq, Y, it)) continue;
+        const double p = L.kn * powi(it.gap < 0.0 ? -it.gap : 0.0, L.order);
+        if (!(p > 1e-30)) continue; // contacts_sdf.py:329
+        const V gl = V{it.grad.x, it.grad.y, it.grad.z};
+        const V G = matvec(RB, gl);           // rg = RB rg_local (:342)
+        const double gn = sqrt(vdot(gl, gl)); // grad_norm of the local gradient
+        const V xw = vadd(rAm, matvec(RAm, x));
+        const V off = vsub(xw, rA);
+        const V u = vsub(vadd(vA, vcross(wA, off)), vadd(vB, vcross(wB, vsub(xw, rB))));
+        V fq = vmul(p, G);
+        if (L.c > 0.0 && it.gap < 0.0) fq = vadd(fq, vmul(L.c * (-vdot(u, G)), G)); // :355-362
+        const double gnc = gn > 1e-30 ? gn : 1e-30;
+        const V un = V{G.x / gnc, G.y / gnc, G.z / gnc};
+        const V ut = vsub(u, vmul(vdot(u, un), un));
+        const double utn = sqrt(vdot(ut, ut));
+        const double D = sqrt(utn * utn + L.reg * L.reg);
+        const double phat = p * gn;
+        if (L.mu * phat > 0.0) // :374-377
+        {
+            const double k = -L.mu * phat;
+            fq = vadd(fq, V{k * ut.x / D, k * ut.y / D, k * ut.z / D});
+        }
+        const double wi = w[i];
+        const V tq = vcross(off, fq);
+        acc[0] += wi * fq.x;
+        acc[1] += wi * fq.y;
+        acc[2] += wi * fq.z;
+        acc[3] += wi * tq.x;
+        acc[4] += wi * tq.y;
+        acc[5] += wi * tq.z;
+        acc[6] += 1.0;
+    }
+    blockReduce<7>(acc, sh);
+    if (threadIdx.x == 0)
+    {
+        double* o = partial + (pair * gridDim.x + chunk) * 7;
+#pragma unroll
+        for (int k = 0; k < 7; ++k) o[k] = acc[k];
+    }
+}
+
+// F_A, tau_A from the chunk partials in chunk order; F_B = -F_A,
+// tau_B = -(tau_A + (rA - rB) x F_A) (contacts_sdf.py:383-385).
+__global__ void contactEvaluateFinalize(int64_t npairs, int chunks, const double* __restrict__ partial,
+                                        const double* __restrict__ rec, double* __restrict__ out,
+                                        int32_t* __restrict__ active)
+{
+    const int64_t b = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (b >= npairs) return;
+    double s[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int c = 0; c < chunks; ++c)
+    {
+        const double* p = partial + (b * chunks + c) * 7;
+#pragma unroll
+        for (int k = 0; k < 7; ++k) s[k] += p[k];
+    }
+    const double* r = rec + kEvalRec * b;
+    const bool on = r[51] != 0.0;
+    const V F = V{s[0], s[1], s[2]}, T = V{s[3], s[4], s[5]};
+    const V dr = V{r[24] - r[33], r[25] - r[34], r[26] - r[35]};
+    const V TB = vadd(T, vcross(dr, F));
+    const double o[12] = {F.x, F.y, F.z, T.x, T.y, T.z, -F.x, -F.y, -F.z, -TB.x, -TB.y, -TB.z};
+#pragma unroll
+    for (int k = 0; k < 12; ++k) out[12 * b + k] = on ? o[k] : 0.0;
+    if (active) active[b] = on ? int32_t(s[6]) : 0;
+}
+
 Law makeLaw(const sdfo_contact_params& p)
 {
     return Law{p.kn, p.damping, p.mu, p.regularizer, p.narrow_margin, p.order, p.mode};
@@ -597,6 +805,62 @@ void launchContactForward(const sdfo_grid& g, const double* X, const double* w, 
     if (nb <= 0) return;
     forwardLaunch<false>(dev::makeQGrid(g), makeLaw(p), X, w, n, poses, nb, LinCache{}, cull, out,
                          active, s);
+}
+
+void launchContactEvaluate(const sdfo_grid& g, const double* X, const double* w, int64_t n,
+                           const double* states, const double* frames, int64_t nb,
+                           const sdfo_contact_params& p, const CullArgs* cull,
+                           const double* bodyBoxes9, double* out, int32_t* active, hipStream_t s)
+{
+    if (nb <= 0) return;
+    const QGrid q = dev::makeQGrid(g);
+    const Law L = makeLaw(p);
+    EvalPrep prep{};
+    prep.states = states;
+    prep.frames = frames;
+    for (int k = 0; k < 3; ++k)
+    {
+        prep.bmin[k] = g.spec.bmin[k];
+        prep.bmax[k] = g.spec.bmax[k];
+    }
+    prep.cullMargin = p.narrow_margin > 0.05 ? p.narrow_margin : 0.05;
+    prep.bodyCheck = bodyBoxes9 ? 1 : 0;
+    if (bodyBoxes9)
+        for (int k = 0; k < 3; ++k)
+        {
+            prep.centerA[k] = bodyBoxes9[k];
+            prep.halfA[k] = bodyBoxes9[3 + k];
+            prep.halfB[k] = bodyBoxes9[6 + k];
+        }
+    const int chunks = n > 0 ? chunksFor(n, nb) : 1;
+    const int64_t chunkLen = n > 0 ? (n + chunks - 1) / chunks : 0;
+    Scratch part(sizeof(double) * (size_t(kEvalRec + 6 + 7 * chunks) * size_t(nb)) + size_t(nb), s);
+    double* rec = static_cast<double*>(part.p);
+    double* boxes = rec + size_t(kEvalRec) * nb;
+    double* partial = boxes + 6 * size_t(nb);
+    uint8_t* anyLeaf = reinterpret_cast<uint8_t*>(partial + 7 * size_t(nb) * chunks);
+    hipLaunchKernelGGL(evaluatePrepKernel, dim3(unsigned((nb + 255) / 256)), dim3(256), 0, s, prep,
+                       nb, rec, boxes);
+    SDFO_HIP(hipGetLastError());
+    if (cull)
+    {
+        CullArgs cu = *cull;
+        cu.boxes = boxes;
+        hipLaunchKernelGGL(cullAnyKernel, dim3(unsigned(nb)), dim3(kBlock), 0, s, cu, anyLeaf);
+        SDFO_HIP(hipGetLastError());
+        hipLaunchKernelGGL((contactEvaluateKernel<true>), dim3(chunks, unsigned(nb)), dim3(kBlock),
+                           0, s, q, L, X, w, n, static_cast<const double*>(rec), chunkLen, partial,
+                           cu, static_cast<const uint8_t*>(anyLeaf));
+    }
+    else
+        hipLaunchKernelGGL((contactEvaluateKernel<false>), dim3(chunks, unsigned(nb)), dim3(kBlock),
+                           0, s, q, L, X, w, n, static_cast<const double*>(rec), chunkLen, partial,
+                           CullArgs{}, static_cast<const uint8_t*>(nullptr));
+    SDFO_HIP(hipGetLastError());
+    hipLaunchKernelGGL(contactEvaluateFinalize, dim3(unsigned((nb + 255) / 256)), dim3(256), 0, s,
+                       nb, chunks, static_cast<const double*>(partial),
+                       static_cast<const double*>(rec), out, active);
+    SDFO_HIP(hipGetLastError());
 }
 
 void launchCullMask(const CullArgs& cu, int64_t n, int64_t nBoxes, uint8_t* mask, hipStream_t s)

@@ -43,3 +43,13 @@ extern "C" int sdfo_debug_ca_merge_cap(int32_t first, int32_t second)
     sdfo::debugSecondCap() = second;
     return SDFO_OK;
 }
+
+namespace sdfo {
+void setSkipCellRecords(bool on);
+}
+
+extern "C" int sdfo_debug_skip_cell_records(int32_t on)
+{
+    sdfo::setSkipCellRecords(on != 0);
+    return SDFO_OK;
+}

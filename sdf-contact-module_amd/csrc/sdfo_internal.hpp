@@ -423,6 +423,10 @@ void launchContactForward(const sdfo_grid& g, const double* X, const double* w, 
                           const double* poses, int64_t nb, const sdfo_contact_params& p,
                           double* out, int32_t* active, hipStream_t s,
                           const CullArgs* cull = nullptr);
+void launchContactEvaluate(const sdfo_grid& g, const double* X, const double* w, int64_t n,
+                           const double* states, const double* frames, int64_t nb,
+                           const sdfo_contact_params& p, const CullArgs* cull,
+                           const double* bodyBoxes9, double* out, int32_t* active, hipStream_t s);
 void launchCullMask(const CullArgs& cu, int64_t n, int64_t nBoxes, uint8_t* mask, hipStream_t s);
 void launchContactLinearize(const sdfo_grid& g, const double* X, int64_t n, const double* pose,
                             const sdfo_contact_params& p, double* g0, double* grad0,

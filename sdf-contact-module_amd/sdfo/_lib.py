@@ -138,6 +138,8 @@ _sig("sdfo_tuner_set_mode", [_i32])
 _sig("sdfo_tuner_reset", [])
 _sig("sdfo_tuner_report", [_vp, _i64], C.c_int64)
 _sig("sdfo_contact_forward", [_vp, _vp, _vp, _i64, _vp, _i64, _P(ContactParams), _vp, _vp, _vp])
+_sig("sdfo_contact_evaluate", [_vp, _vp, _vp, _vp, _i64, _vp, _vp, _i64, _P(ContactParams), _vp,
+                               _vp, _vp, _vp])
 _sig("sdfo_contact_linearize", [_vp, _vp, _i64, _vp, _P(ContactParams), _vp, _vp, _vp, _vp])
 _sig("sdfo_contact_forward_linearized", [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _i64,
                                          _P(ContactParams), _vp, _vp, _vp, _vp, _vp])
@@ -161,6 +163,7 @@ _sig("sdfo_debug_build_phases", [_vp, _i32])
 _sig("sdfo_debug_force_replay", [_i32])
 _sig("sdfo_debug_fill_budget", [_i32, _i32])
 _sig("sdfo_debug_ca_merge_cap", [_i32, _i32])
+_sig("sdfo_debug_skip_cell_records", [_i32])
 
 BUILD_PHASES = ("setup_grid", "host_bvh", "upload_alloc", "fill_kernel", "hessian_kernel",
                 "mirror_kernels", "total", "replayed_voxels")
@@ -175,6 +178,12 @@ def debug_ca_merge_cap(first: int = 0, second: int = 0):
     """Test switch: merged entries per voxel in the contact-aware first pass (1, 4, 8)
     and in the second pass's first launch (0 = defaults)."""
     check(lib.sdfo_debug_ca_merge_cap(int(first), int(second)))
+
+
+def debug_skip_cell_records(on: bool):
+    """Grids built or loaded on this thread while on keep no cell records
+    (test switch: the per-corner gather path of very large grids)."""
+    check(lib.sdfo_debug_skip_cell_records(1 if on else 0))
 
 
 def debug_force_replay(on: bool):

@@ -84,6 +84,70 @@ def contact_forward(grid: SdfGrid, X, w, poses, law: ContactLaw = ContactLaw(), 
     return (out, act) if active else out
 
 
+def pair_states(rA, RA, vA, wA, rB, RB, vB, wB):
+    """Packs per-pair body states (contacts_sdf.py:231-240) into the (B,36)
+    float64 records of ``contact_evaluate``: rA RA vA wA rB RB vB wB (world
+    frame, rotations row major).  Each argument is (B,3) or (B,3,3)."""
+    parts = [rA, RA, vA, wA, rB, RB, vB, wB]
+    ref = next(p for p in parts if torch.is_tensor(p)) if any(torch.is_tensor(p) for p in parts) \
+        else None
+    dev = ref.device if ref is not None else "cpu"
+    ts = [torch.as_tensor(p, dtype=torch.float64, device=dev) for p in parts]
+    B = ts[0].reshape(-1, 3).shape[0]
+    return torch.cat([t.reshape(B, -1) for t in ts], dim=1).contiguous()
+
+
+def pair_frames(R_ab, t_ab, RAm, rAm):
+    """(B,24) marker-frame records (contacts_sdf.py:243-249, 275-279): the
+    action marker relative to the base marker (``Y = R_ab X + t_ab``) and the
+    action marker's world pose (``x_w = rAm + RAm X``)."""
+    parts = [R_ab, t_ab, RAm, rAm]
+    dev = next((p.device for p in parts if torch.is_tensor(p)), "cpu")
+    ts = [torch.as_tensor(p, dtype=torch.float64, device=dev) for p in parts]
+    B = ts[1].reshape(-1, 3).shape[0]
+    return torch.cat([t.reshape(B, -1) for t in ts], dim=1).contiguous()
+
+
+def _check_evaluate(X, w, states, frames):
+    _check_inputs(X, w, torch.zeros((0, 30), dtype=torch.float64, device=X.device))
+    if not states.is_cuda or states.dtype != torch.float64 or not states.is_contiguous() \
+            or states.dim() != 2 or states.shape[1] != 36:
+        raise ValueError("states must be a contiguous (B,36) float64 GPU tensor")
+    if frames is not None and (not frames.is_cuda or frames.dtype != torch.float64
+                               or not frames.is_contiguous()
+                               or tuple(frames.shape) != (states.shape[0], 24)):
+        raise ValueError("frames must be a contiguous (B,24) float64 GPU tensor")
+
+
+def contact_evaluate(grid: SdfGrid, X, w, states, law: "ContactLaw" = None, frames=None,
+                     bvh: "PointBvh" = None, body_boxes=None, stream=None, active=False):
+    """``SDFContactEngine.evaluate`` (contacts_sdf.py:225-385) for B pairs at
+    once: (B,12) float64 = F_A, tau_A, F_B, tau_B in the world frame.
+
+    ``states``: (B,36) from ``pair_states``; ``frames``: optional (B,24) from
+    ``pair_frames``; ``law.narrow_margin`` is the engine's narrow filter
+    (default 0.01 here, as ``register_pair``); ``bvh``: the broad-phase tree
+    over X (``PointBvh(X, 16)``) or None; ``body_boxes``: (A centre, A half,
+    B half) for ``_check_broad_phase`` or None."""
+    law = ContactLaw(narrow_margin=0.01, order=1) if law is None else law
+    _check_evaluate(X, w, states, frames)
+    B = states.shape[0]
+    out = torch.empty((B, 12), dtype=torch.float64, device=X.device)
+    act = torch.empty(B, dtype=torch.int32, device=X.device) if active else None
+    p = law.ctypes()
+    bb = None
+    if body_boxes is not None:
+        import numpy as np
+        bb = np.ascontiguousarray(np.concatenate([np.ravel(b) for b in body_boxes]), np.float64)
+        if bb.shape != (9,):
+            raise ValueError("body_boxes = (A centre 3, A half 3, B half 3)")
+    check(lib.sdfo_contact_evaluate(grid.handle, None if bvh is None else bvh.handle, _ptr(X),
+                                    _ptr(w), X.shape[0], _ptr(states), _ptr(frames), B, C.byref(p),
+                                    None if bb is None else bb.ctypes.data_as(C.c_void_p),
+                                    _ptr(out), _ptr(act), _stream_handle(stream)))
+    return (out, act) if active else out
+
+
 class PointBvh:
     """Broad-phase AABB tree over a body's quadrature points (AABB_BVH,
     prototype/src/geometry/bvh.py:16-75), built on the host.
@@ -340,3 +404,53 @@ def contact(grid, twist, kn, damping, mu, X, w, R0, t0, v0=None, Om=None, regula
     as_t = lambda v: v if torch.is_tensor(v) else torch.tensor(float(v), dtype=torch.float64)  # noqa: E731
     return ContactFunction.apply(twist, as_t(kn), as_t(damping), as_t(mu), X, w, grid, R0, t0, v0,
                                  Om, regularizer, order, mode)
+
+
+class SdfContactEngine:
+    """Pair registry and ``evaluate`` of ``SDFContactEngine``
+    (contacts_sdf.py:33-66, 68-89, 225-385) on the device, batched over the
+    body states of many evaluations of one pair.
+
+    ``register_pair`` normalises the weights to sum 1, keeps A's point AABB
+    and B's half extents for the body-level check and builds the broad-phase
+    tree (leaf size 16); like the reference, the narrow margin is one value
+    for the engine, the one given at the last registration."""
+
+    def __init__(self):
+        self._pairs = {}
+        self._narrow_margin = 0.01
+
+    def register_pair(self, cp_id, X_q, w_q, grid: SdfGrid, aabb_b_half=None,
+                      narrow_margin=0.01, device=None):
+        import numpy as np
+        X = np.ascontiguousarray(X_q, np.float64).reshape(-1, 3)
+        w = np.array(w_q, np.float64).reshape(-1)
+        total = w.sum()
+        if total > 1e-30:
+            w = w / total
+        if len(X):
+            lo, hi = X.min(axis=0), X.max(axis=0)
+            box_a = ((lo + hi) / 2, (hi - lo) / 2)
+        else:
+            box_a = (np.zeros(3), np.zeros(3))
+        dev = torch.device("cuda") if device is None else torch.device(device)
+        self._pairs[cp_id] = dict(
+            grid=grid, X=torch.from_numpy(X).to(dev), w=torch.from_numpy(w).to(dev),
+            bvh=PointBvh(X, leaf_size=16) if len(X) else None, box_a=box_a,
+            half_b=None if aabb_b_half is None else np.asarray(aabb_b_half, np.float64).reshape(3))
+        self._narrow_margin = float(narrow_margin)
+
+    def evaluate(self, cp_id, states, law: ContactLaw = ContactLaw(order=1), frames=None,
+                 stream=None):
+        """(F_A, tau_A, F_B, tau_B), each (B,3) float64, world frame."""
+        B = states.shape[0]
+        pair = self._pairs.get(cp_id)
+        if pair is None or pair["X"].shape[0] == 0:
+            z = torch.zeros((B, 3), dtype=torch.float64, device=states.device)
+            return z, z.clone(), z.clone(), z.clone()
+        lw = ContactLaw(law.kn, law.damping, law.mu, law.regularizer, self._narrow_margin,
+                        law.order, law.mode)
+        boxes = None if pair["half_b"] is None else (*pair["box_a"], pair["half_b"])
+        out = contact_evaluate(pair["grid"], pair["X"], pair["w"], states, lw, frames=frames,
+                               bvh=pair["bvh"], body_boxes=boxes, stream=stream)
+        return out[:, 0:3], out[:, 3:6], out[:, 6:9], out[:, 9:12]

@@ -286,13 +286,16 @@ class SdfGrid:
         import torch
 
         mode = MODES.get(mode, mode)
+        if points.dtype not in (torch.float32, torch.float64) or points.dim() != 2 or \
+                points.shape[1] != 3:
+            raise ValueError("points must be an (N,3) float32/float64 tensor")
         n = points.shape[0]
-        for t in (points, gap, grad, valid):
-            if not (t.is_cuda and t.is_contiguous()):
-                raise ValueError("query_into needs contiguous GPU tensors")
         if gap.shape != (n,) or grad.shape != (n, 3) or valid.shape != (n,) or \
                 valid.dtype != torch.uint8 or gap.dtype != points.dtype or grad.dtype != points.dtype:
             raise ValueError("query_into output shapes/dtypes do not match the points")
+        for t in (points, gap, grad, valid):
+            if not (t.is_cuda and t.is_contiguous()):
+                raise ValueError("query_into needs contiguous GPU tensors")
         fn = lib.sdfo_query if points.dtype == torch.float32 else lib.sdfo_query_f64
         ptr = lambda t: C.c_void_p(t.data_ptr())  # noqa: E731
         check(fn(self._h, int(mode), ptr(points), n, ptr(gap), ptr(grad), ptr(valid), None, None,
@@ -342,9 +345,16 @@ class QueryPlan:
         """points (N,3) f32, gap (N) f32, grad (N,3) f32, valid (N) u8: GPU
         tensors packed stream after stream (stream s owns rows
         offsets[s]:offsets[s+1])."""
+        import torch
+
         mode = MODES.get(mode, mode)
         n = self.total
-        for t, shape in ((points, (n, 3)), (gap, (n,)), (grad, (n, 3)), (valid, (n,))):
+        specs = ((points, (n, 3), torch.float32), (gap, (n,), torch.float32),
+                 (grad, (n, 3), torch.float32), (valid, (n,), torch.uint8))
+        for t, _, dt in specs:
+            if t.dtype != dt:
+                raise ValueError(f"plan tensors: expected {dt}, got {t.dtype}")
+        for t, shape, _ in specs:
             if tuple(t.shape) != shape or not t.is_cuda or not t.is_contiguous():
                 raise ValueError("plan tensors must be contiguous GPU tensors of the plan's size")
         ptr = lambda t: C.c_void_p(t.data_ptr())  # noqa: E731

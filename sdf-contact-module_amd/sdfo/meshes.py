@@ -167,3 +167,35 @@ def annular_quadrature(r_in: float, r_out: float, n_radial: int = 20, n_azimutha
     X = np.stack([RC * np.cos(TH), RC * np.sin(TH), np.zeros_like(RC)], -1).reshape(-1, 3)
     w = np.repeat(area, n_azimuthal)
     return X, w
+
+
+def load_obj_triangles(path: str):
+    """load_obj_triangles (quadrature.py:17-29): ``v`` lines give vertices,
+    ``f`` lines give zero-based vertex indices (the part before the first
+    ``/``), and only faces with exactly three indices are kept; quads and
+    larger polygons are skipped, not split."""
+    verts, tris = [], []
+    with open(path) as f:
+        for line in f:
+            if line.startswith("v "):
+                verts.append([float(x) for x in line.strip().split()[1:]])
+            elif line.startswith("f "):
+                parts = line.strip().split()
+                idx = [int(p.split("/")[0]) - 1 for p in parts[1:]]
+                if len(idx) == 3:
+                    tris.append(idx)
+    return np.array(verts), np.array(tris)
+
+
+def quadrature_from_obj(path: str):
+    """quadrature_mesh_from_obj (quadrature.py:45-48): triangle centroids,
+    raw triangle areas as weights (the engine normalises them when a pair is
+    registered), unit face normals and triangle ids."""
+    verts, tris = load_obj_triangles(path)
+    v = verts[tris]
+    X = v.mean(axis=1)
+    n = np.cross(v[:, 1] - v[:, 0], v[:, 2] - v[:, 0])
+    nn = np.linalg.norm(n, axis=1)
+    w = 0.5 * nn
+    normals = n / (nn[:, None] + 1e-30)
+    return X, w, normals, np.arange(len(tris))
