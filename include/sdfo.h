@@ -323,6 +323,22 @@ int sdfo_contact_evaluate(const sdfo_grid* grid, const sdfo_point_bvh* bvh, cons
                           const double* body_boxes9, double* out, int32_t* active,
                           sdfo_stream stream);
 
+/* sdfo_contact_evaluate without any device allocation.  No counterpart in
+ * the reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The pair
+ * records, cull boxes and partial sums live in a static device array of
+ * about 2.3 MB per GPU, held until the process ends; at most 4096 pairs per
+ * call (more is rejected with SDFO_ERR_INVALID_ARGUMENT).  Every call on a GPU
+ * uses that GPU's one array, so calls that overlap on one GPU (other streams,
+ * other threads) race: serialise them.  A point BVH must already be resident
+ * on the grid's device (one sdfo_contact_evaluate or sdfo_point_bvh_cull call
+ * with it uploads it).  Arguments and results are sdfo_contact_evaluate's. */
+int sdfo_contact_evaluate_noalloc(const sdfo_grid* grid, const sdfo_point_bvh* bvh,
+                                  const double* X, const double* w, int64_t n,
+                                  const double* states, const double* frames, int64_t n_pairs,
+                                  const sdfo_contact_params* params, const double* body_boxes9,
+                                  double* out, int32_t* active, sdfo_stream stream);
+
 /* Linearisation cache of SDFContactEngine.evaluate (contacts_sdf.py:307-327),
  * used by the Newton iterations of implicit_be.py:474-500.
  * sdfo_contact_linearize fills it at one reference pose (a device record of

@@ -1162,6 +1162,32 @@ int sdfo_contact_evaluate(const sdfo_grid* grid, const sdfo_point_bvh* bvh, cons
     });
 }
 
+int sdfo_contact_evaluate_noalloc(const sdfo_grid* grid, const sdfo_point_bvh* bvh,
+                                  const double* X, const double* w, int64_t n,
+                                  const double* states, const double* frames, int64_t n_pairs,
+                                  const sdfo_contact_params* params, const double* body_boxes9,
+                                  double* out, int32_t* active, sdfo_stream stream)
+{
+    return guarded([&] {
+        checkContact(grid, X, w, n, states, n_pairs, params);
+        if (n_pairs > 0 && !out) throw Error(SDFO_ERR_INVALID_ARGUMENT, "out is NULL");
+        DeviceGuard dg(grid->device);
+        CullArgs c;
+        if (bvh)
+        {
+            // uploading the tree would allocate: it must already be on this device
+            if (bvh->device != grid->device || !bvh->leafBox || !bvh->leafOf)
+                throw Error(SDFO_ERR_INVALID_ARGUMENT,
+                            "the point BVH is not on the grid's device yet; one "
+                            "sdfo_contact_evaluate or sdfo_point_bvh_cull call with it uploads it");
+            c = cullArgs(bvh, X, n, 0.0);
+        }
+        launchContactEvaluateNoAlloc(*grid, X, w, n, states, frames, n_pairs, *params,
+                                     bvh ? &c : nullptr, body_boxes9, out, active,
+                                     hipStream_t(stream));
+    });
+}
+
 int sdfo_contact_forward_linearized(const double* g0, const double* grad0, const uint8_t* valid0,
                                     const double* t_ref3, const double* X, const double* w, int64_t n,
                                     const double* poses, int64_t n_poses,

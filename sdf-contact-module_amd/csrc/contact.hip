@@ -7,6 +7,9 @@
 // recomputes the forward terms per point instead of storing them.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+#include <string>
+
 #include "interp_device.hpp"
 #include "sdfo_internal.hpp"
 
@@ -807,12 +810,22 @@ void launchContactForward(const sdfo_grid& g, const double* X, const double* w, 
                          active, s);
 }
 
-void launchContactEvaluate(const sdfo_grid& g, const double* X, const double* w, int64_t n,
-                           const double* states, const double* frames, int64_t nb,
-                           const sdfo_contact_params& p, const CullArgs* cull,
-                           const double* bodyBoxes9, double* out, int32_t* active, hipStream_t s)
+namespace {
+
+// Bytes of device scratch one evaluate call over nb pairs uses: the pair
+// records, the cull boxes, the chunk partials and the per-pair leaf flags.
+size_t evaluateScratchBytes(int64_t n, int64_t nb, int* chunksOut)
 {
-    if (nb <= 0) return;
+    const int chunks = n > 0 ? chunksFor(n, nb) : 1;
+    if (chunksOut) *chunksOut = chunks;
+    return sizeof(double) * (size_t(kEvalRec + 6 + 7 * chunks) * size_t(nb)) + size_t(nb);
+}
+
+void evaluateLaunch(const sdfo_grid& g, const double* X, const double* w, int64_t n,
+                    const double* states, const double* frames, int64_t nb,
+                    const sdfo_contact_params& p, const CullArgs* cull, const double* bodyBoxes9,
+                    double* out, int32_t* active, hipStream_t s, int chunks, void* scratchBase)
+{
     const QGrid q = dev::makeQGrid(g);
     const Law L = makeLaw(p);
     EvalPrep prep{};
@@ -832,10 +845,8 @@ void launchContactEvaluate(const sdfo_grid& g, const double* X, const double* w,
             prep.halfA[k] = bodyBoxes9[3 + k];
             prep.halfB[k] = bodyBoxes9[6 + k];
         }
-    const int chunks = n > 0 ? chunksFor(n, nb) : 1;
     const int64_t chunkLen = n > 0 ? (n + chunks - 1) / chunks : 0;
-    Scratch part(sizeof(double) * (size_t(kEvalRec + 6 + 7 * chunks) * size_t(nb)) + size_t(nb), s);
-    double* rec = static_cast<double*>(part.p);
+    double* rec = static_cast<double*>(scratchBase);
     double* boxes = rec + size_t(kEvalRec) * nb;
     double* partial = boxes + 6 * size_t(nb);
     uint8_t* anyLeaf = reinterpret_cast<uint8_t*>(partial + 7 * size_t(nb) * chunks);
@@ -861,6 +872,75 @@ void launchContactEvaluate(const sdfo_grid& g, const double* X, const double* w,
                        nb, chunks, static_cast<const double*>(partial),
                        static_cast<const double*>(rec), out, active);
     SDFO_HIP(hipGetLastError());
+}
+
+// Scratch of the allocation-free evaluate: one call of at most
+// kNoAllocPairs pairs (chunks x pairs <= 2048 + pairs - 1 by chunksFor).
+constexpr int64_t kNoAllocPairs = 4096;
+constexpr size_t kNoAllocScratchBytes =
+    sizeof(double) * size_t((kEvalRec + 6) * kNoAllocPairs + 7 * (2048 + kNoAllocPairs)) +
+    size_t(kNoAllocPairs);
+// A host-side name only: kernels get the per-device address that
+// hipGetSymbolAddress returns for it.
+static __device__ __align__(256) unsigned char scratch[(kNoAllocScratchBytes + 255) / 256 * 256];
+
+constexpr int kMaxDevices = 64;
+
+// Device address of `scratch` on the current device, looked up once per
+// device (each device holds its own copy of the array).
+void* noAllocScratch(int device)
+{
+    static std::mutex m;
+    static void* ptrs[kMaxDevices] = {};
+    if (device < 0 || device >= kMaxDevices)
+        throw Error(SDFO_ERR_DEVICE, "device index " + std::to_string(device) + " out of range");
+    std::lock_guard<std::mutex> lk(m);
+    if (!ptrs[device])
+    {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            throw Error(SDFO_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch) failed: ") +
+                                             hipGetErrorString(e));
+        ptrs[device] = ptr;
+    }
+    return ptrs[device];
+}
+
+} // namespace
+
+void launchContactEvaluate(const sdfo_grid& g, const double* X, const double* w, int64_t n,
+                           const double* states, const double* frames, int64_t nb,
+                           const sdfo_contact_params& p, const CullArgs* cull,
+                           const double* bodyBoxes9, double* out, int32_t* active, hipStream_t s)
+{
+    if (nb <= 0) return;
+    int chunks = 1;
+    const size_t bytes = evaluateScratchBytes(n, nb, &chunks);
+    Scratch part(bytes, s);
+    evaluateLaunch(g, X, w, n, states, frames, nb, p, cull, bodyBoxes9, out, active, s, chunks,
+                   part.p);
+}
+
+void launchContactEvaluateNoAlloc(const sdfo_grid& g, const double* X, const double* w, int64_t n,
+                                  const double* states, const double* frames, int64_t nb,
+                                  const sdfo_contact_params& p, const CullArgs* cull,
+                                  const double* bodyBoxes9, double* out, int32_t* active,
+                                  hipStream_t s)
+{
+    if (nb > kNoAllocPairs)
+        throw Error(SDFO_ERR_INVALID_ARGUMENT,
+                    "sdfo_contact_evaluate_noalloc takes at most " + std::to_string(kNoAllocPairs) +
+                        " pairs per call (got " + std::to_string(nb) + ")");
+    if (nb <= 0) return;
+    int chunks = 1;
+    const size_t bytes = evaluateScratchBytes(n, nb, &chunks);
+    if (bytes > sizeof(scratch))
+        throw Error(SDFO_ERR_INVALID_ARGUMENT, "evaluate scratch of " + std::to_string(bytes) +
+                                                   " bytes exceeds the static " +
+                                                   std::to_string(sizeof(scratch)));
+    evaluateLaunch(g, X, w, n, states, frames, nb, p, cull, bodyBoxes9, out, active, s, chunks,
+                   noAllocScratch(g.device));
 }
 
 void launchCullMask(const CullArgs& cu, int64_t n, int64_t nBoxes, uint8_t* mask, hipStream_t s)

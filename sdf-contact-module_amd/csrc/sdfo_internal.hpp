@@ -427,6 +427,13 @@ void launchContactEvaluate(const sdfo_grid& g, const double* X, const double* w,
                            const double* states, const double* frames, int64_t nb,
                            const sdfo_contact_params& p, const CullArgs* cull,
                            const double* bodyBoxes9, double* out, int32_t* active, hipStream_t s);
+// The same with the device scratch in a static array (no allocation);
+// at most 4096 pairs per call.
+void launchContactEvaluateNoAlloc(const sdfo_grid& g, const double* X, const double* w, int64_t n,
+                                  const double* states, const double* frames, int64_t nb,
+                                  const sdfo_contact_params& p, const CullArgs* cull,
+                                  const double* bodyBoxes9, double* out, int32_t* active,
+                                  hipStream_t s);
 void launchCullMask(const CullArgs& cu, int64_t n, int64_t nBoxes, uint8_t* mask, hipStream_t s);
 void launchContactLinearize(const sdfo_grid& g, const double* X, int64_t n, const double* pose,
                             const sdfo_contact_params& p, double* g0, double* grad0,

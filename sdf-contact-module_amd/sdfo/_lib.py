@@ -140,6 +140,8 @@ _sig("sdfo_tuner_report", [_vp, _i64], C.c_int64)
 _sig("sdfo_contact_forward", [_vp, _vp, _vp, _i64, _vp, _i64, _P(ContactParams), _vp, _vp, _vp])
 _sig("sdfo_contact_evaluate", [_vp, _vp, _vp, _vp, _i64, _vp, _vp, _i64, _P(ContactParams), _vp,
                                _vp, _vp, _vp])
+_sig("sdfo_contact_evaluate_noalloc", [_vp, _vp, _vp, _vp, _i64, _vp, _vp, _i64, _P(ContactParams),
+                                       _vp, _vp, _vp, _vp])
 _sig("sdfo_contact_linearize", [_vp, _vp, _i64, _vp, _P(ContactParams), _vp, _vp, _vp, _vp])
 _sig("sdfo_contact_forward_linearized", [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _i64,
                                          _P(ContactParams), _vp, _vp, _vp, _vp, _vp])
