@@ -574,3 +574,31 @@ def sparse_hessian(grid: Grid, block_dim, bijk, n3):
     lib().oracle_sparse_hessian(_p(grid.grid9), _p(grid.shape3, _ip), C.c_int(block_dim),
                                 C.c_int(len(bijk)), _p(bijk, _ip), _p(n3, _fp), _p(H, _fp))
     return H
+
+
+def contact_aware_probe(verts, tris, corner_normals, grid: Grid, ijk, max_candidates=4,
+                        feature_tol=0.0, merge_deg=5.0, tile_dim=8, use_tiles=True,
+                        sparse_block_dim=0, halo=2, band_voxels=8, band_distance=0.0):
+    """Contact-aware candidates at the voxels ijk (n,3) as the dense
+    (sparse_block_dim = 0) or sparse build computes them, and for the sparse
+    build its brick activation mask: dict(ccount, cphi, cn, cbranch, mask)."""
+    v, t = _f64(verts), _i32(tris)
+    cn = _f64(corner_normals, (-1, 3, 3))
+    ijk = _i32(ijk).reshape(-1, 3)
+    n = len(ijk)
+    S = max(0, min(int(max_candidates), 8))
+    out = dict(ccount=np.zeros(n, np.uint8), cphi=np.zeros((n, S)), cn=np.zeros((n, S, 3)),
+               cbranch=np.zeros((n, S), np.int32), mask=None)
+    if sparse_block_dim > 0:
+        nb = [(s + sparse_block_dim - 1) // sparse_block_dim for s in grid.shape]
+        out["mask"] = np.zeros(int(np.prod(nb)), np.uint8)
+    rc = lib().oracle_contact_aware_probe(
+        _p(v), C.c_int64(len(v)), _p(t, _ip), C.c_int(len(t)), _p(cn), _p(grid.grid9),
+        _p(grid.shape3, _ip), C.c_int(max_candidates), C.c_double(feature_tol),
+        C.c_double(merge_deg), C.c_int(tile_dim), C.c_int(1 if use_tiles else 0),
+        C.c_int(sparse_block_dim), C.c_int(halo), C.c_int(band_voxels), C.c_double(band_distance),
+        _p(ijk, _ip), C.c_int64(n), _p(out["ccount"], _u8p), _p(out["cphi"]), _p(out["cn"]),
+        _p(out["cbranch"], _ip), _p(out["mask"], _u8p))
+    if rc < 0:
+        raise RuntimeError("contact-aware probe rejected its inputs")
+    return out

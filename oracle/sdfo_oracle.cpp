@@ -2376,6 +2376,81 @@ void oracle_sparse_hessian(const double* grid9, const int* shape3, int blockDim,
     estimateSparseHessian(gridFromArrays(grid9, shape3), blockDim, nblocks, bijk, nrm, hes);
 }
 
+// The contact-aware builds' per-voxel work (SdfBuilder.cpp:281-359 behind
+// the tile index of :202-258) at selected voxels only, for checking large
+// builds by sampling.  sparseBlockDim > 0 selects the sparse build's tile
+// influence max(activation, tol + 2h) with activation = band + halo h
+// (:575-640) and fills brickMask (one byte per brick of the block grid,
+// x fastest) with its activation rule (any overlapping tile list non-empty);
+// sparseBlockDim = 0 is the dense build (influence tol + 2h).
+// Outputs per voxel: count, then stride entries of phi, normal, branch.
+int oracle_contact_aware_probe(const double* verts, int64_t nv, const int* tris, int nt,
+                               const double* cornerN, const double* grid9, const int* shape3,
+                               int maxCandidates, double featureTol, double mergeDeg, int tileDim,
+                               int useTiles, int sparseBlockDim, int halo, int bandVoxels,
+                               double bandDistance, const int* ijk, int64_t nvox, uint8_t* ccount,
+                               double* cphi, double* cn3, int* cbranch, uint8_t* brickMask)
+{
+    if (nv <= 0 || nt <= 0 || maxCandidates <= 0) return -1;
+    const Grid g = gridFromArrays(grid9, shape3);
+    const int S = std::max(0, std::min(maxCandidates, 8));
+    const std::vector<CTri> ct = makeContactTris(verts, tris, nt, cornerN);
+    Bvh tree;
+    tree.build(verts, tris, nt);
+    const double h = g.hmax();
+    const double tol = featureTol > 0.0 ? featureTol : 1.5 * h;
+    const double cosMerge = std::cos(mergeDeg * std::acos(-1.0) / 180.0);
+    double infl = tol + 2.0 * h;
+    if (sparseBlockDim > 0)
+    {
+        const double band = bandDistance > 0.0 ? bandDistance : static_cast<double>(bandVoxels) * h;
+        infl = smax(band + static_cast<double>(halo) * h, infl);
+    }
+    std::vector<int> all(nt);
+    std::iota(all.begin(), all.end(), 0);
+    TileIndex ti;
+    if (useTiles) ti = buildTileIndex(g, ct, tileDim, infl);
+    if (brickMask && sparseBlockDim > 0 && useTiles && !ti.ids.empty())
+    {
+        const int bd = sparseBlockDim;
+        const int nbx = (g.n[0] + bd - 1) / bd, nby = (g.n[1] + bd - 1) / bd,
+                  nbz = (g.n[2] + bd - 1) / bd;
+#pragma omp parallel for schedule(dynamic, 64)
+        for (int64_t bl = 0; bl < int64_t(nbx) * nby * nbz; ++bl)
+        {
+            const int bi = int(bl % nbx), bj = int((bl / nbx) % nby), bk = int(bl / (int64_t(nbx) * nby));
+            const int i0 = bi * bd, j0 = bj * bd, k0 = bk * bd;
+            const int i1 = std::min(i0 + bd - 1, g.n[0] - 1);
+            const int j1 = std::min(j0 + bd - 1, g.n[1] - 1);
+            const int k1 = std::min(k0 + bd - 1, g.n[2] - 1);
+            bool act = false;
+            for (int tk = k0 / ti.tileDim; tk <= k1 / ti.tileDim && !act; ++tk)
+                for (int tj = j0 / ti.tileDim; tj <= j1 / ti.tileDim && !act; ++tj)
+                    for (int tii = i0 / ti.tileDim; tii <= i1 / ti.tileDim && !act; ++tii)
+                        act = !ti.ids[size_t(ti.lin(tii, tj, tk))].empty();
+            brickMask[bl] = act ? 1 : 0;
+        }
+    }
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t q = 0; q < nvox; ++q)
+    {
+        const int i = ijk[3 * q], j = ijk[3 * q + 1], k = ijk[3 * q + 2];
+        const std::vector<int>& cand = useTiles ? ti.forVoxel(i, j, k) : all;
+        const std::vector<CCand> cs = candidatesForPoint(g.center(i, j, k), ct, cand, tree, tol,
+                                                         cosMerge, S);
+        const int cnt = std::min(int(cs.size()), S);
+        ccount[q] = uint8_t(cnt);
+        for (int e = 0; e < S; ++e)
+        {
+            const bool on = e < cnt;
+            cphi[q * S + e] = on ? cs[e].phi : 0.0;
+            store3(cn3, q * S + e, on ? cs[e].normal : v3(1, 0, 0));
+            cbranch[q * S + e] = on ? cs[e].branch : -1;
+        }
+    }
+    return S;
+}
+
 // Analytic-plane SDF contact (sdf_grid.py:4-22): phi = z - zTop, grad = e_z.
 void oracle_contact_forward_plane(double zTop, const double* X, const double* w, int64_t n,
                                   const double* poses30, int nb, const double* params6,
