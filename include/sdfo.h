@@ -146,7 +146,16 @@ int sdfo_build_dense_batched(int32_t n_meshes, const double* verts, const int64_
                              sdfo_stream stream);
 
 /* ---- grid handle ------------------------------------------------------- */
+/* Releases a grid.  The device is synchronised first, so work still in
+ * flight on any stream finishes before the arrays return to the pool. */
 void sdfo_grid_free(sdfo_grid* grid);
+/* Stream-ordered release (no device-wide synchronisation): the arrays return
+ * to the pool after the work already enqueued on `stream`.  The caller
+ * guarantees that no other stream still uses the grid (work on other streams
+ * is ordered before this call's point on `stream`, or complete).  The grids
+ * of one batched build share one device block, freed with the last of them
+ * after every such release point. */
+void sdfo_grid_free_async(sdfo_grid* grid, sdfo_stream stream);
 int sdfo_grid_get_info(const sdfo_grid* grid, sdfo_grid_info* info);
 
 /* Copies the FP64 dense fields to host arrays (SdfData.h:27-118 members

@@ -441,10 +441,8 @@ std::vector<std::unique_ptr<sdfo_grid>> buildDenseBatched(int32_t nm, const doub
     for (int32_t m = 0; m < nm; ++m) arenaTotal += gridBytes(specs[size_t(m)]);
     char* arenaBase = nullptr;
     SDFO_HIP(hipMallocAsync(reinterpret_cast<void**>(&arenaBase), std::max<size_t>(arenaTotal, 256), s));
-    const std::shared_ptr<char> arena(arenaBase, [](char* p) {
-        (void)hipDeviceSynchronize(); // no work may still use the grids' arrays
-        (void)hipFreeAsync(p, nullptr);
-    });
+    const auto arena = std::make_shared<ArenaBlock>();
+    arena->base = arenaBase;
     size_t arenaOff = 0;
     for (int32_t m = 0; m < nm; ++m)
     {
@@ -727,6 +725,14 @@ int sdfo_build_sparse(const double* verts, int64_t n_verts, const int32_t* tris,
         checkParams(params);
         *out = buildSparse(verts, n_verts, tris, n_tris, *params, hipStream_t(stream)).release();
     });
+}
+
+void sdfo_grid_free_async(sdfo_grid* grid, sdfo_stream stream)
+{
+    if (!grid) return;
+    grid->releaseOrdered = true;
+    grid->releaseStream = hipStream_t(stream);
+    sdfo_grid_free(grid);
 }
 
 void sdfo_grid_free(sdfo_grid* grid)

@@ -147,6 +147,53 @@ struct sdfo_mesh
 };
 
 // The opaque handle of the C ABI.
+// A device block shared by the grids of one batched build, returned to the
+// pool when the last of them is released.  A grid released stream-ordered
+// records an event on its stream; the block is then freed on the last
+// release's stream after every recorded event, without a device-wide
+// synchronisation.  Any synchronous release makes the block's release
+// synchronise the device.
+struct ArenaBlock
+{
+    char* base = nullptr;
+    std::mutex m;
+    std::vector<hipEvent_t> events;
+    hipStream_t last = nullptr;
+    bool ordered = true;
+    void noteRelease(bool streamOrdered, hipStream_t s)
+    {
+        std::lock_guard<std::mutex> lk(m);
+        if (!streamOrdered)
+        {
+            ordered = false;
+            return;
+        }
+        hipEvent_t e = nullptr;
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+        {
+            ordered = false;
+            return;
+        }
+        if (hipEventRecord(e, s) != hipSuccess) ordered = false;
+        events.push_back(e);
+        last = s;
+    }
+    ~ArenaBlock()
+    {
+        bool ok = ordered;
+        if (ok)
+            for (hipEvent_t e : events) ok = ok && hipStreamWaitEvent(last, e, 0) == hipSuccess;
+        if (ok)
+            (void)hipFreeAsync(base, last);
+        else
+        {
+            (void)hipDeviceSynchronize(); // no work may still use the grids' arrays
+            (void)hipFreeAsync(base, nullptr);
+        }
+        for (hipEvent_t e : events) (void)hipEventDestroy(e);
+    }
+};
+
 struct sdfo_grid
 {
     sdfo::GridSpec spec;
@@ -203,7 +250,7 @@ struct sdfo_grid
     // Optional slice of a shared device block: alloc() carves 256-byte aligned
     // arrays from it while it has room (the grids of one batched build share
     // one block, returned to the pool with the last of them).
-    std::shared_ptr<char> arena;
+    std::shared_ptr<ArenaBlock> arena;
     char* arenaNext = nullptr;
     size_t arenaLeft = 0;
 
@@ -232,12 +279,16 @@ struct sdfo_grid
         deviceBytes += int64_t(bytes);
         return static_cast<T*>(p);
     }
+    // Release mode: synchronous (the device is synchronised first, so no work
+    // on any stream can still read the arrays) or ordered on releaseStream.
+    bool releaseOrdered = false;
+    hipStream_t releaseStream = nullptr;
     ~sdfo_grid()
     {
+        if (arena) arena->noteRelease(releaseOrdered, releaseStream);
         if (allocations.empty()) return;
-        // no work on any stream may still use the arrays when they return to the pool
-        (void)hipDeviceSynchronize();
-        for (void* p : allocations) (void)hipFreeAsync(p, nullptr);
+        if (!releaseOrdered) (void)hipDeviceSynchronize();
+        for (void* p : allocations) (void)hipFreeAsync(p, releaseOrdered ? releaseStream : nullptr);
     }
 };
 

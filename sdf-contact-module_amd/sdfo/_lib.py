@@ -118,6 +118,7 @@ _sig("sdfo_build_sparse", [_vp, _i64, _vp, _i64, _P(BuildParams), _P(_vp), _vp])
 _sig("sdfo_build_contact_aware", [_vp, _i64, _vp, _i64, _vp, _P(BuildParams), _i32, _P(_vp), _vp])
 _sig("sdfo_build_dense_batched", [_i32, _vp, _vp, _vp, _vp, _P(BuildParams), _P(_vp), _vp])
 _sig("sdfo_grid_free", [_vp], None)
+_sig("sdfo_grid_free_async", [_vp, _vp], None)
 _sig("sdfo_grid_get_info", [_vp, _P(GridInfo)])
 _sig("sdfo_grid_download_dense", [_vp, _vp, _vp, _vp, _vp, _vp])
 _sig("sdfo_grid_download_candidates", [_vp, _vp, _vp, _vp, _vp])

@@ -181,9 +181,16 @@ class SdfGrid:
         check(lib.sdfo_grid_save(self._h, os.fsencode(path)))
 
     # ---- lifetime ----------------------------------------------------------
-    def close(self) -> None:
+    def close(self, stream=None) -> None:
+        """Releases the grid.  With ``stream`` (a torch stream) the release is
+        ordered after the work already on that stream, with no device-wide
+        synchronisation; the caller guarantees no other stream still uses the
+        grid.  Without it the device is synchronised first."""
         if getattr(self, "_h", None) is not None and self._h.value:
-            lib.sdfo_grid_free(self._h)
+            if stream is None:
+                lib.sdfo_grid_free(self._h)
+            else:
+                lib.sdfo_grid_free_async(self._h, _stream_handle(stream))
             self._h = C.c_void_p()
 
     def __del__(self):

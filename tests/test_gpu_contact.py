@@ -250,7 +250,7 @@ def test_torsion_on_sdf_face_matches_closed_form(sdfo_gpu, n_radial):
     0.01 into the top face of a cube SDF and spinning about the face normal,
     mu = 0.3, linear penalty.  Normal force = kn * depth * area and torque =
     -mu N (2/3)(Ro^3 - Ri^3)/(Ro^2 - Ri^2) times the quadrature's exact relative
-    error, both to 1e-6; the symmetric patch's tangential residual to 1e-12 mu N."""
+    error, both to 1e-6; the symmetric patch's tangential residual to 1e-15 mu N."""
     v, t = meshes.cube(1.0)
     g = sdfo_gpu.SdfGrid.build_dense(v, t, target_resolution=64, max_resolution=64)
     try:
@@ -267,7 +267,9 @@ def test_torsion_on_sdf_face_matches_closed_form(sdfo_gpu, n_radial):
         assert int(act.cpu()[0]) == len(X)
         assert np.isclose(out[2], N, rtol=1e-6)
         assert np.isclose(out[5], Tz * (1 + quad), rtol=1e-6)
-        assert np.hypot(out[0], out[1]) <= 1e-12 * mu * N
+        resid = np.hypot(out[0], out[1]) / (mu * N)
+        print(f"[gpu] torsion tangential residual / (mu N) = {resid:.3e}", flush=True)
+        assert resid <= 1e-15
     finally:
         g.close()
 
@@ -305,16 +307,20 @@ def test_contact_gradients_match_central_differences(cube):
                 tm[b, k] -= h
                 fd[b, k] = (loss(tp, kn.detach(), Xt) - loss(tm, kn.detach(), Xt)) / (2 * h)
         gt = twist.grad
-        assert (gt - fd).abs().max() <= 1e-4 * fd.abs().max()
+        # per component: 1e-4 relative, plus a floor at the difference quotient's
+        # rounding (|L| eps / h ~ 1e-9 of the largest component)
+        floor = 1e-9 * fd.abs().max()
+        assert ((gt - fd).abs() <= 1e-4 * fd.abs() + floor).all()
         dk = 1e11 * 1e-6
         fdk = (loss(twist.detach(), kn.detach() + dk, Xt) - loss(twist.detach(), kn.detach() - dk, Xt)) / (2 * dk)
         assert abs(float(kn.grad) - float(fdk)) <= 1e-4 * abs(float(fdk))
         gx = Xq.grad
         rows = torch.argsort(gx.abs().sum(1), descending=True)[:5]
+        floor_x = 1e-9 * gx.abs().max()
         for r in rows.tolist():
             for k in range(3):
                 xp, xm = Xt.clone(), Xt.clone()
                 xp[r, k] += h
                 xm[r, k] -= h
                 fdx = (loss(twist.detach(), kn.detach(), xp) - loss(twist.detach(), kn.detach(), xm)) / (2 * h)
-                assert abs(float(gx[r, k]) - float(fdx)) <= 1e-4 * float(gx[rows[0]].abs().max())
+                assert abs(float(gx[r, k]) - float(fdx)) <= 1e-4 * abs(float(fdx)) + float(floor_x)
