@@ -3,21 +3,31 @@
 One step = (1) dense FP64 build of the torus ring mesh (R 50 mm, r 10 mm,
 256 x 64 segments = 32 768 triangles) at target resolution 128;
 (2) Tricubic value + raw gradient of 10^6 FP32 points uniform in the valid
-region of the 128^3 cube SDF (side 0.1 m, top face at z = 55 mm);
+region of the 128^3 cube SDF (side 0.1 m, top face at z = 55 mm), over five
+distinct pre-generated point sets with their own outputs, so no pass reuses
+another's points or outputs in a cache; also the same points five times
+(warm) and 10^6 FP64 points (the exact path);
 (3) contact forward (penalty kn = 1e11 order 2, damping 1e4) of the 7 680
-ring-centroid quadrature points (sum w = 1) under 1 000 seeded poses;
+ring-centroid quadrature points (sum w = 1) under 1 000 seeded poses, in
+body B's frame and as the world-frame evaluate with a rotated body B;
 (4) batched contact forward + backward over 256 poses with friction (mu 0.3)
-against the same work done pose by pose (the plain per-pose GPU path).
+against the same work done pose by pose (the plain per-pose GPU path);
+(5) the 1 000 poses on the linearisation cache.
+Beside the step, the other two benchmark configurations run once each: the
+512^3 sparse tube (build + 1e6 ContactAware points, plain and contact-aware)
+and the 64-gear batched build + captured 64-stream query.
 
-``value`` is (2) in Mpts/s; the other rates are reported beside it.  Inputs
-are resident in HBM before the timed region.  The cpu_baseline leg times the
-FP64 CPU oracle (oracle/) on a bounded sample of the same workload.
+``value`` is (2) on the distinct point sets, in Mpts/s; the other rates are
+reported beside it.  Inputs are resident in HBM before the timed region.  The
+cpu_baseline leg times the FP64 CPU oracle (oracle/) on bounded samples of
+the same workloads: one warm-up, then the median of five runs.
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -29,25 +39,50 @@ sys.path.insert(0, os.path.join(ROOT, "sdf-contact-module_amd"))
 METRIC = "build Mvoxel/s; tricubic value+grad Mpts/s; fwd+bwd contact evals/s vs plain"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
-
+# Algorithmic flops per (pose, point) evaluation of the contact law: 600 for the
+# forward (tricubic value + gradient 500, transform, law and torque 100) and
+# 1300 for the backward (recomputed forward 600, Hessian 500, chain rule 200).
+FWD_FLOPS = 600.0
+BWD_FLOPS = 1300.0
 
 TRAFFIC_FILE = "profiles/query_tricubic_cube128_1M_traffic.json"
+WORK_FILE = "profiles/algorithmic_work_counts.json"
+
+
+def _json(path):
+    try:
+        with open(os.path.join(ROOT, path)) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
 
 
 def _traffic(nq, shape):
     """HBM bytes per query launch from the committed FETCH_SIZE / WRITE_SIZE
     summary (counters scaled by the copy-kernel calibration), when the
     workload is the one it was measured on."""
-    try:
-        with open(os.path.join(ROOT, TRAFFIC_FILE)) as f:
-            t = json.load(f)
-    except (OSError, ValueError):
+    t = _json(TRAFFIC_FILE)
+    if not t:
         return None
     w = t.get("workload", {})
     if w.get("query_points") != nq or list(w.get("grid", [])) != list(shape):
         return None
     return float(t["fetch_kib_reported"] * 1024 * t["fetch_factor"] +
                  t["write_kib_reported"] * 1024 * t["write_factor"])
+
+
+def _valu_roofline(flops, seconds, kernel, basis):
+    ach = flops / seconds / 1e12
+    return {"bound": "valu", "achieved": round(ach, 3), "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(ach / FP64_PEAK_TFLOPS, 4), "traffic": None,
+            "kernel": kernel, "algorithmic_flops": flops, "flops_basis": basis}
+
+
+def _hbm_roofline(alg_bytes, seconds, kernel, traffic=None):
+    ach = alg_bytes / seconds / 1e9
+    return {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kernel,
+            "algorithmic_bytes": alg_bytes}
 
 
 def parse():
@@ -57,13 +92,15 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--query-points", type=int, default=1_000_000)
     ap.add_argument("--query-reps", type=int, default=5,
-                    help="query passes per step, timed together (per-pass time reported)")
+                    help="query passes per step (distinct point sets), timed together")
     ap.add_argument("--poses", type=int, default=1000)
     ap.add_argument("--bwd-poses", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--parts", default="build,query,contact,bwd",
                     help="subset of the step to run (profiling)")
+    ap.add_argument("--legs", default="sparse_cylinder,batched_gears",
+                    help="other configurations measured once beside the step ('' for none)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 256 MiB device copy first (scales HBM counters)")
     ap.add_argument("--workload", default="ring_on_cube",
@@ -74,38 +111,17 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(args, cube_spec, torus_mesh, cube_mesh, pts_np, X, w, poses):
-    """FP64 oracle on bounded samples of the same workload (rank 0, N=1)."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O
-
+def cpu_leg(args, leg):
+    """The FP64 oracle on a bounded sample of one workload (rank 0, N=1), in a
+    child process with its OpenMP threads bound to cores (OMP_PROC_BIND=close,
+    OMP_PLACES=cores) so the binding stays out of this process's host threads:
+    one warm-up, then the median of five runs per rate."""
+    env = dict(os.environ, OMP_PROC_BIND="close", OMP_PLACES="cores")
+    cmd = [sys.executable, os.path.join(ROOT, "oracle", "cpu_baseline.py"), "--leg", leg]
     if args.cpu_threads > 0:
-        O.set_threads(args.cpu_threads)
-    threads = O.num_threads()
-    res = {}
-    # build: a 16-slice z slab of the torus grid (same per-voxel work as the full build)
-    tv, tt = torus_mesh
-    tg = O.setup_grid(tv, 0.1, 0.0, 128, 128)
-    slab = tg.centers()[: tg.shape[0] * tg.shape[1] * 16]
-    t0 = time.perf_counter()
-    O.signed_nearest(tv, tt, slab)
-    res["build_mvoxel_per_s"] = len(slab) / (time.perf_counter() - t0) / 1e6
-    # query: the same 1e6 points, FP64 tricubic, on the oracle's cube field
-    cv, ct = cube_mesh
-    cg = O.Grid(cube_spec.bmin, cube_spec.bmax, cube_spec.voxel, cube_spec.shape)
-    cd = O.build_dense(cv, ct, cg)
-    p64 = pts_np.astype(np.float64)
-    reps = 3
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        O.query_dense(cg, cd, "Tricubic", p64)
-    res["tricubic_mpts_per_s"] = reps * len(p64) / (time.perf_counter() - t0) / 1e6
-    # contact: 200 of the poses
-    sub = poses[:200]
-    t0 = time.perf_counter()
-    O.contact_forward(cg, cd["phi"], cd["n"], X, w, sub, kn=1e11, c=1e4, mu=0.0, reg=1e-6, order=2)
-    res["contact_fwd_evals_per_s"] = len(sub) * len(X) / (time.perf_counter() - t0)
-    return res, threads
+        cmd += ["--threads", str(args.cpu_threads)]
+    r = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=600, check=True)
+    return json.loads(r.stdout.strip().splitlines()[-1])
 
 
 def _events(torch, stream):
@@ -119,7 +135,11 @@ def _busy(torch):
         pass
 
 
-def bench_sparse_cylinder(args):
+def _work_counts():
+    return _json(WORK_FILE) or {}
+
+
+def sparse_cylinder_leg(args, steps):
     """Tube R_o 40 mm, R_i 30 mm, H 50 mm, 1024 x 64 segments (266 240 triangles),
     sparse 512^3, 8^3 bricks, band 4 voxels, halo 2; 1e6 FP32 points on the
     surface offset by N(0, (2h)^2) along the normal, ContactAware value + gradient
@@ -141,14 +161,15 @@ def bench_sparse_cylinder(args):
     cn = np.repeat(fn[:, None, :], 3, axis=1)
     g = sdfo.SdfGrid.build_sparse(v, t, **prm)  # warm-up
     g.close()
-    K = max(1, args.steps)
-    build_s = 0.0
+    K = max(1, steps)
+    builds = []
     for k in range(K):
         t0 = time.perf_counter()
         g = sdfo.SdfGrid.build_sparse(v, t, **prm)
-        build_s += (time.perf_counter() - t0) / K
+        builds.append(time.perf_counter() - t0)
         if k < K - 1:
             g.close()  # outside the timed build
+    build_s = float(np.median(builds))
     info = g.info
     h = g.spec.voxel.max()
     pts_np = scenes.near_surface_points(v, t, args.query_points, 2.0 * h, seed=3)
@@ -157,7 +178,7 @@ def bench_sparse_cylinder(args):
     gap = torch.empty(n, device=dev)
     grad = torch.empty(n, 3, device=dev)
     valid = torch.empty(n, dtype=torch.uint8, device=dev)
-    for _ in range(args.warmup):
+    for _ in range(2):
         g.query_into(pts, gap, grad, valid, "ContactAware")
     qs = []
     for _ in range(K):
@@ -168,20 +189,24 @@ def bench_sparse_cylinder(args):
         b.record(stream)
         torch.cuda.synchronize()
         qs.append(a.elapsed_time(b))
-    q_ms = float(np.mean(qs))
+    q_ms = float(np.median(qs))
     frac_valid = float(valid.float().mean().item())
     sdfo.SdfGrid.build_contact_aware(v, t, cn, sparse=True, **prm).close()  # warm-up
     t0 = time.perf_counter()
     gca = sdfo.SdfGrid.build_contact_aware(v, t, cn, sparse=True, **prm)
     ca_build_s = time.perf_counter() - t0
-    a, b = _events(torch, stream)
     gca.query_into(pts, gap, grad, valid, "ContactAware")
-    _busy(torch)
-    a.record(stream)
-    gca.query_into(pts, gap, grad, valid, "ContactAware")
-    b.record(stream)
-    torch.cuda.synchronize()
-    ca_q_ms = a.elapsed_time(b)
+    cqs = []
+    for _ in range(K):
+        a, b = _events(torch, stream)
+        _busy(torch)
+        a.record(stream)
+        gca.query_into(pts, gap, grad, valid, "ContactAware")
+        b.record(stream)
+        torch.cuda.synchronize()
+        cqs.append(a.elapsed_time(b))
+    ca_q_ms = float(np.median(cqs))
+    gca.close()
     vox = info.block_count * 512
     # Algorithmic bytes: 29 B per point plus every distinct voxel the points'
     # cells touch, read once (16 B each): the corner voxels of the valid
@@ -196,52 +221,54 @@ def bench_sparse_cylinder(args):
     ids = np.concatenate([c[:, 0] + nn[0] * (c[:, 1] + nn[1] * c[:, 2]) for c in corners])
     touched = int(np.unique(ids).size)
     alg = 29.0 * n + 16.0 * touched
-    line = {
-        "metric": "sparse narrow-band build Mvoxel/s; ContactAware value+grad Mpts/s",
-        "value": round(n / (q_ms * 1e-3) / 1e6, 3), "unit": "Mpts/s", "n_gpus": 1,
-        "steps": K, "warmup": args.warmup, "ms_per_step": round(build_s * 1e3 + q_ms, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 query / f64 build", "data": "synthetic (seeded mesh and points)",
-        "config": {"workload": "sparse hollow cylinder 1024x64 (266240 tris) at 512^3, 8^3 bricks, "
-                               "band 4, halo 2; 1e6 near-surface ContactAware points",
-                   "active_bricks": int(info.block_count), "query_points": n,
-                   "touched_voxels": touched},
+    work = _work_counts().get("tube_1024x64_sparse_512")
+    out = {
+        "workload": "sparse hollow cylinder 1024x64 (266240 tris) at 512^3, 8^3 bricks, band 4, "
+                    "halo 2; 1e6 near-surface ContactAware points",
+        "active_bricks": int(info.block_count), "query_points": n, "touched_voxels": touched,
         "sparse_build_ms": round(build_s * 1e3, 3),
         "sparse_build_mvoxel_per_s": round(vox / build_s / 1e6, 3),
-        "query_kernel_ms": round(q_ms, 4), "valid_fraction": round(frac_valid, 4),
+        "query_kernel_ms": round(q_ms, 4), "query_mpts_per_s": round(n / (q_ms * 1e-3) / 1e6, 3),
+        "valid_fraction": round(frac_valid, 4),
         "contact_aware_sparse_build_ms": round(ca_build_s * 1e3, 3),
         "contact_aware_query_kernel_ms": round(ca_q_ms, 4),
-        "roofline": {"bound": "hbm", "achieved": round(alg / (q_ms * 1e-3) / 1e9, 2),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(alg / (q_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "queryPairKernel<float, float, 3> (ContactAware -> sparse Tricubic "
-                               "on sparse cell records)",
-                     "algorithmic_bytes": alg},
+        "contact_aware_query_mpts_per_s": round(n / (ca_q_ms * 1e-3) / 1e6, 3),
+        "roofline": _hbm_roofline(alg, q_ms * 1e-3, "queryPairKernel<float, float, 3> "
+                                  "(ContactAware -> sparse Tricubic on sparse cell records)"),
     }
+    if work:
+        out["build_roofline"] = _valu_roofline(
+            work["flops"], build_s, "sparse build (activation + fill + Hessian), whole call",
+            "reference-order nearest-search tests: fill voxels x " +
+            str(work["fill"]["flops_per_voxel"]) + " + brick centres x " +
+            str(work["activation"]["flops_per_voxel"]) + " flops (" + WORK_FILE + ")")
     if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle as O
+        c = cpu_leg(args, "sparse_cylinder")
+        out["cpu_baseline"] = {
+            "value": round(c["sparse_fill_mvoxel_per_s"], 4), "unit": "Mvoxel/s (sparse fill)",
+            "cores": c["threads"], "kind": "port", "cpu_model": c["cpu_model"],
+            "sample": "FP64 oracle signed nearest over the voxels of 400 random active bricks "
+                      "(204 800 voxels), warm-up + median of 5"}
+    g.close()
+    return out
 
-        if args.cpu_threads > 0:
-            O.set_threads(args.cpu_threads)
-        bijk = g.download_sparse()["bijk"]
-        og = O.setup_grid(v, 0.1, 0.0, 512, 512)
-        rng = np.random.default_rng(0)
-        pick = bijk[rng.choice(len(bijk), size=min(len(bijk), 400), replace=False)]
-        L = np.arange(512)
-        ijk = (pick[:, None, :] * 8 + np.stack([L % 8, (L // 8) % 8, L // 64], 1)[None]).reshape(-1, 3)
-        c = og.bmin + og.voxel * (ijk + 0.5)
-        t0 = time.perf_counter()
-        O.signed_nearest(v, t, c)
-        cpu_rate = len(c) / (time.perf_counter() - t0) / 1e6
-        line["cpu_baseline"] = {
-            "value": round(cpu_rate, 4), "unit": "Mvoxel/s (sparse fill)", "cores": O.num_threads(),
-            "kind": "port", "sample": "FP64 oracle signed nearest over the voxels of 400 random "
-                                      "active bricks (204 800 voxels)"}
+
+def bench_sparse_cylinder(args):
+    leg = sparse_cylinder_leg(args, args.steps)
+    line = {
+        "metric": "sparse narrow-band build Mvoxel/s; ContactAware value+grad Mpts/s",
+        "value": leg["query_mpts_per_s"], "unit": "Mpts/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": 2, "ms_per_step": round(leg["sparse_build_ms"] + leg["query_kernel_ms"], 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 query / f64 build", "data": "synthetic (seeded mesh and points)",
+        "config": {"workload": leg["workload"], "active_bricks": leg["active_bricks"],
+                   "query_points": leg["query_points"]},
+    }
+    line.update({k: v for k, v in leg.items() if k not in ("workload",)})
     print(json.dumps(line))
 
 
-def bench_batched_gears(args):
+def batched_gears_leg(args, steps):
     """64 seeded gear meshes (2k-200k triangles, grids 64^3-256^3 by size)
     built by one batched call (one fill launch), then 64 Tricubic query
     streams of 10k-500k near-surface points against them in one plan launch,
@@ -259,16 +286,17 @@ def bench_batched_gears(args):
     grids = sdfo.SdfGrid.build_dense_batched(meshes_, prm)  # warm-up
     for g in grids:
         g.close()
-    K = max(1, args.steps)
-    build_s = 0.0
+    K = max(1, steps)
+    builds, phases = [], []
     for k in range(K):
         t0 = time.perf_counter()
         grids = sdfo.SdfGrid.build_dense_batched(meshes_, prm)
-        build_s += (time.perf_counter() - t0) / K
+        builds.append(time.perf_counter() - t0)
+        phases.append(_lib.debug_build_phases())
         if k < K - 1:
             for g in grids:  # outside the timed build
                 g.close()
-    phases = _lib.debug_build_phases()
+    build_s = float(np.median(builds))
     vox = sum(int(np.prod(g.spec.shape)) for g in grids)
     rng = np.random.default_rng(9)
     counts = rng.integers(10_000, 500_001, size=len(grids))
@@ -298,38 +326,59 @@ def bench_batched_gears(args):
         b.record(stream)
         torch.cuda.synchronize()
         qs.append(a.elapsed_time(b))
-    q_ms = float(np.mean(qs))
-    line = {
-        "metric": "batched build Mvoxel/s; batched tricubic Mpts/s",
-        "value": round(vox / build_s / 1e6, 3), "unit": "Mvoxel/s", "n_gpus": 1, "steps": K,
-        "warmup": 1, "ms_per_step": round(build_s * 1e3 + q_ms, 3), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64 build / f32 query",
-        "data": "synthetic (seeded gear meshes and points)",
-        "config": {"workload": "64 gear meshes (12-120 teeth, 2k-200k tris, grids 64-256 per "
-                               "axis) in one batched build; 64 query streams 10k-500k points",
-                   "meshes": len(grids), "triangles": int(sum(len(t) for _, t in meshes_)),
-                   "voxels": vox, "query_points": n},
+    q_ms = float(np.median(qs))
+    fill_ms = float(np.median([p["fill_kernel"] for p in phases]))
+    # 29 B per point plus each grid's FP32 field (16 B per voxel) read once
+    alg = 29.0 * n + 16.0 * vox
+    work = _work_counts().get("gears_64_batched")
+    out = {
+        "workload": "64 gear meshes (12-120 teeth, 2k-200k tris, grids 64-256 per axis) in one "
+                    "batched build; 64 query streams 10k-500k near-surface points, Tricubic, one "
+                    "captured graph",
+        "meshes": len(grids), "triangles": int(sum(len(t) for _, t in meshes_)), "voxels": vox,
+        "query_points": n,
         "batched_build_ms": round(build_s * 1e3, 3),
-        "batched_build_phases_ms": {k: round(float(v), 3) for k, v in phases.items()},
+        "batched_build_mvoxel_per_s": round(vox / build_s / 1e6, 3),
+        "batched_build_phases_ms": {k: round(float(np.median([p[k] for p in phases])), 3)
+                                    for k in phases[0]},
         "batched_query_graph_ms": round(q_ms, 4),
         "batched_query_mpts_per_s": round(n / (q_ms * 1e-3) / 1e6, 3),
+        "query_roofline": _hbm_roofline(alg, q_ms * 1e-3, "queryBatchedKernel (Tricubic plan, "
+                                        "one graph replay)"),
         "tuner": sdfo.tuner_report(),
     }
+    if work:
+        out["build_roofline"] = _valu_roofline(
+            work["flops"], fill_ms * 1e-3, "denseFillBatchedKernel + frontier passes (fill phase)",
+            "reference-order nearest-search tests, " + str(work["flops_per_voxel"]) +
+            " flops per voxel averaged over the 64 grids (" + WORK_FILE + ")")
     if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle as O
+        c = cpu_leg(args, "batched_gears")
+        out["cpu_baseline"] = {
+            "value": round(c["build_mvoxel_per_s"], 4), "unit": "Mvoxel/s", "cores": c["threads"],
+            "kind": "port", "cpu_model": c["cpu_model"],
+            "sample": f"FP64 oracle dense build of the smallest gear ({c['triangles']} tris, grid "
+                      f"{c['grid']}), warm-up + median of 5"}
+    for g in grids:
+        g.close()
+    plan.close()
+    return out
 
-        if args.cpu_threads > 0:
-            O.set_threads(args.cpu_threads)
-        v, t, r = min(batch, key=lambda b: len(b[1]))
-        og = O.setup_grid(v, 0.1, 0.0, r, r)
-        t0 = time.perf_counter()
-        O.build_dense(v, t, og)
-        cpu_rate = og.total / (time.perf_counter() - t0) / 1e6
-        line["cpu_baseline"] = {
-            "value": round(cpu_rate, 4), "unit": "Mvoxel/s", "cores": O.num_threads(),
-            "kind": "port", "sample": f"FP64 oracle dense build of the smallest gear "
-                                      f"({len(t)} tris, grid {list(og.shape)})"}
+
+def bench_batched_gears(args):
+    leg = batched_gears_leg(args, args.steps)
+    line = {
+        "metric": "batched build Mvoxel/s; batched tricubic Mpts/s",
+        "value": leg["batched_build_mvoxel_per_s"], "unit": "Mvoxel/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": 1,
+        "ms_per_step": round(leg["batched_build_ms"] + leg["batched_query_graph_ms"], 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 build / f32 query", "data": "synthetic (seeded gear meshes and points)",
+        "config": {"workload": leg["workload"], "meshes": leg["meshes"],
+                   "triangles": leg["triangles"], "voxels": leg["voxels"],
+                   "query_points": leg["query_points"]},
+    }
+    line.update({k: v for k, v in leg.items() if k != "workload"})
     print(json.dumps(line))
 
 
@@ -358,28 +407,47 @@ def main():
     cube = scenes.cube_mesh()
     cube_grid = sdfo.SdfGrid.build_dense(*cube, target_resolution=128, max_resolution=128)
     spec = cube_grid.spec
-    pts_np = scenes.valid_region_points(spec, args.query_points, seed=1)
-    pts = torch.from_numpy(pts_np).to(dev)
+    nq = args.query_points
+    reps = max(1, args.query_reps)
+    # distinct point sets (and outputs) for the passes of one step
+    sets_np = [scenes.valid_region_points(spec, nq, seed=1 + r) for r in range(reps)]
+    sets = [torch.from_numpy(p).to(dev) for p in sets_np]
+    outs = [(torch.empty(nq, dtype=torch.float32, device=dev),
+             torch.empty((nq, 3), dtype=torch.float32, device=dev),
+             torch.empty(nq, dtype=torch.uint8, device=dev)) for _ in range(reps)]
+    pts64 = torch.from_numpy(sets_np[0].astype(np.float64)).to(dev)
+    out64 = (torch.empty(nq, dtype=torch.float64, device=dev),
+             torch.empty((nq, 3), dtype=torch.float64, device=dev),
+             torch.empty(nq, dtype=torch.uint8, device=dev))
     X_np, w_np = scenes.ring_quadrature(120, 32)
     R, t = scenes.ring_poses(args.poses, seed=0)
     poses_np = scenes.pose_array(R, t)
     Rb, tb = scenes.ring_poses(args.bwd_poses, seed=2)
     v0, Om = scenes.pose_velocities(args.bwd_poses, seed=3)
     poses_b_np = scenes.pose_array(Rb, tb, v0=v0, Om=Om)
+    # the same 1000 poses as world-frame body states with a rotated, moving body B
+    rng = np.random.default_rng(4)
+    st = np.zeros((args.poses, 36))
+    for b in range(args.poses):
+        ax = rng.normal(size=3)
+        RB = scenes.rot_axis(ax, rng.uniform(0.0, np.pi))
+        rB = rng.normal(0, 0.3, 3)
+        vA, wA, vB, wB = rng.normal(0, 0.05, (4, 3))
+        st[b] = np.concatenate([rB + RB @ t[b], (RB @ R[b]).ravel(), vA, wA, rB, RB.ravel(), vB, wB])
     X = torch.from_numpy(X_np).to(dev)
     w = torch.from_numpy(w_np).to(dev)
     poses = torch.from_numpy(poses_np).to(dev)
     poses_b = torch.from_numpy(poses_b_np).to(dev)
+    states = torch.from_numpy(st).to(dev)
     g_out = torch.ones((args.bwd_poses, 6), dtype=torch.float64, device=dev)
     law = SC.ContactLaw(kn=1e11, damping=1e4, mu=0.0, regularizer=1e-6, order=2)
     law_f = SC.ContactLaw(kn=1e11, damping=1e4, mu=0.3, regularizer=1e-6, order=2)
+    law_e = SC.ContactLaw(kn=1e11, damping=1e4, mu=0.0, regularizer=1e-6, narrow_margin=0.07,
+                          order=2)
     n_vox_torus = int(np.prod(sdfo.setup_grid(torus[0], target_resolution=128,
                                               max_resolution=128).shape))
 
     lin_cache = SC.LinearizationCache(cube_grid, X, w, law).fill(poses[0])
-    q_gap = torch.empty(args.query_points, dtype=torch.float32, device=dev)
-    q_grad = torch.empty((args.query_points, 3), dtype=torch.float32, device=dev)
-    q_valid = torch.empty(args.query_points, dtype=torch.uint8, device=dev)
 
     def busy():
         try:
@@ -402,58 +470,77 @@ def main():
         if "build" in parts:
             t0 = time.perf_counter()
             g = sdfo.SdfGrid.build_dense(*torus, target_resolution=128, max_resolution=128)
-            rec["build_s"] += time.perf_counter() - t0
+            rec["build_s"].append(time.perf_counter() - t0)
             rec["phases"].append(_lib.debug_build_phases())
             g.close()
-        e = [ev() for _ in range(10)]
-        # (2) 1M-point tricubic query; each timed segment starts behind a short
-        # device spin so the host's launch latency is not inside the events
+        e = [ev() for _ in range(18)]
+        # (2) 1e6-point tricubic queries: each timed segment starts behind a
+        # short device spin so the host's launch latency is not in the events.
+        # (2a) distinct point sets and outputs, back to back
         busy()
         e[0].record(stream)
         if "query" in parts:
-            # back-to-back passes over the same batch: the per-pass time at
-            # steady clocks (a single 35 us launch after the build's host
-            # phases would also time the clock ramp)
-            for _ in range(args.query_reps):
-                cube_grid.query_into(pts, q_gap, q_grad, q_valid, "Tricubic")
+            for r in range(reps):
+                cube_grid.query_into(sets[r], *outs[r], "Tricubic")
         e[1].record(stream)
-        # (3) contact forward, 1000 poses
+        # (2b) the same points and outputs again and again (cache-warm)
         busy()
         e[2].record(stream)
-        if "contact" in parts:
-            SC.contact_forward(cube_grid, X, w, poses, law)
+        if "query" in parts:
+            for _ in range(reps):
+                cube_grid.query_into(sets[0], *outs[0], "Tricubic")
         e[3].record(stream)
-        # (4) batched fwd+bwd vs per-pose plain path
+        # (2c) FP64 points, the exact path
         busy()
         e[4].record(stream)
+        if "query" in parts:
+            cube_grid.query_into(pts64, *out64, "Tricubic")
+        e[5].record(stream)
+        # (3) contact forward, 1000 poses, B frame and world-frame evaluate
+        busy()
+        e[6].record(stream)
+        if "contact" in parts:
+            SC.contact_forward(cube_grid, X, w, poses, law)
+        e[7].record(stream)
+        busy()
+        e[8].record(stream)
+        if "contact" in parts:
+            SC.contact_evaluate(cube_grid, X, w, states, law_e)
+        e[9].record(stream)
+        # (4) batched fwd+bwd vs per-pose plain path
+        busy()
+        e[10].record(stream)
         if "bwd" in parts:
             SC.contact_forward(cube_grid, X, w, poses_b, law_f)
             SC.contact_backward(cube_grid, X, w, poses_b, g_out, law_f)
-        e[5].record(stream)
+        e[11].record(stream)
         busy()
-        e[6].record(stream)
+        e[12].record(stream)
         if "bwd" in parts:
             for b in range(args.bwd_poses):
                 pb = poses_b[b:b + 1]
                 SC.contact_forward(cube_grid, X, w, pb, law_f)
                 SC.contact_backward(cube_grid, X, w, pb, g_out[b:b + 1], law_f)
-        e[7].record(stream)
+        e[13].record(stream)
         # (5) the same 1000 poses on the linearisation cache (a Newton iterate)
         busy()
-        e[8].record(stream)
+        e[14].record(stream)
         if "contact" in parts:
             lin_cache.forward(poses)
-        e[9].record(stream)
+        e[15].record(stream)
         torch.cuda.synchronize()
-        rec["query_ms"].append(e[0].elapsed_time(e[1]) / args.query_reps)
-        rec["contact_ms"].append(e[2].elapsed_time(e[3]))
-        rec["fwdbwd_ms"].append(e[4].elapsed_time(e[5]))
-        rec["plain_ms"].append(e[6].elapsed_time(e[7]))
-        rec["lin_ms"].append(e[8].elapsed_time(e[9]))
+        rec["query_ms"].append(e[0].elapsed_time(e[1]) / reps)
+        rec["query_warm_ms"].append(e[2].elapsed_time(e[3]) / reps)
+        rec["query64_ms"].append(e[4].elapsed_time(e[5]))
+        rec["contact_ms"].append(e[6].elapsed_time(e[7]))
+        rec["eval_ms"].append(e[8].elapsed_time(e[9]))
+        rec["fwdbwd_ms"].append(e[10].elapsed_time(e[11]))
+        rec["plain_ms"].append(e[12].elapsed_time(e[13]))
+        rec["lin_ms"].append(e[14].elapsed_time(e[15]))
 
     def new_rec():
-        return dict(build_s=0.0, query_ms=[], contact_ms=[], fwdbwd_ms=[], plain_ms=[], lin_ms=[],
-                    phases=[])
+        return dict(build_s=[], query_ms=[], query_warm_ms=[], query64_ms=[], contact_ms=[],
+                    eval_ms=[], fwdbwd_ms=[], plain_ms=[], lin_ms=[], phases=[])
 
     warm = new_rec()
     for _ in range(args.warmup):
@@ -467,20 +554,20 @@ def main():
     wall = time.perf_counter() - t0
 
     K = args.steps
-    q_ms = max(float(np.mean(rec["query_ms"])), 1e-9)
-    c_ms = max(float(np.mean(rec["contact_ms"])), 1e-9)
-    fb_ms = max(float(np.mean(rec["fwdbwd_ms"])), 1e-9)
-    pl_ms = max(float(np.mean(rec["plain_ms"])), 1e-9)
-    lin_ms = max(float(np.mean(rec["lin_ms"])), 1e-9)
-    build_s = max(rec["build_s"] / K, 1e-12)
-    nq = args.query_points
+    med = lambda k: max(float(np.median(rec[k])), 1e-9) if rec[k] else 1e-9  # noqa: E731
+    q_ms, qw_ms, q64_ms = med("query_ms"), med("query_warm_ms"), med("query64_ms")
+    c_ms, e_ms, fb_ms = med("contact_ms"), med("eval_ms"), med("fwdbwd_ms")
+    pl_ms, lin_ms = med("plain_ms"), med("lin_ms")
+    build_s = med("build_s")
+    phase = {k: float(np.median([p[k] for p in rec["phases"]])) for k in
+             (rec["phases"][0] if rec["phases"] else {})}
     value = nq / (q_ms * 1e-3) / 1e6
     # Algorithmic HBM bytes of one query launch: 12 B in + 16 B out + 1 B valid
     # per point, plus the FP32 field (16 B per voxel) read once.
     alg_bytes = 29.0 * nq + 16.0 * spec.total
-    achieved = alg_bytes / (q_ms * 1e-3) / 1e9
     n_eval = args.poses * len(X_np)
     n_eval_b = args.bwd_poses * len(X_np)
+    work = _work_counts().get("torus_256x64_dense_128")
     line = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -496,43 +583,64 @@ def main():
         "data": "synthetic (seeded meshes, points and poses)",
         "config": {
             "workload": "ring-on-cube full size: torus 256x64 (32768 tris) dense build at 128; "
-                        "1e6-pt Tricubic value+grad on cube 128^3; 1000 poses x 7680 pts contact; "
-                        "256-pose fwd+bwd with friction",
+                        "1e6-pt Tricubic value+grad on cube 128^3 (five distinct point sets per "
+                        "step); 1000 poses x 7680 pts contact; 256-pose fwd+bwd with friction",
             "query_points": nq, "grid": list(spec.shape), "poses": args.poses,
             "quadrature_points": len(X_np), "bwd_poses": args.bwd_poses,
         },
         "build_mvoxel_per_s": round(n_vox_torus / build_s / 1e6, 3),
-        "build_phases_ms": {k: round(float(np.mean([p[k] for p in rec["phases"]])), 4)
-                            for k in (rec["phases"][0] if rec["phases"] else {})},
+        "build_phases_ms": {k: round(v, 4) for k, v in phase.items()},
         "build_ms": round(build_s * 1e3, 3),
         "query_kernel_ms": round(q_ms, 4),
+        "query_same_points_kernel_ms": round(qw_ms, 4),
+        "query_same_points_mpts_per_s": round(nq / (qw_ms * 1e-3) / 1e6, 3),
+        "tricubic_f64_points_mpts_per_s": round(nq / (q64_ms * 1e-3) / 1e6, 3),
         "contact_fwd_evals_per_s": round(n_eval / (c_ms * 1e-3), 1),
+        "contact_evaluate_world_evals_per_s": round(n_eval / (e_ms * 1e-3), 1),
         "contact_linearized_evals_per_s": round(n_eval / (lin_ms * 1e-3), 1),
         "linearized_vs_direct": round(c_ms / lin_ms, 3),
         "contact_fwdbwd_evals_per_s": round(n_eval_b / (fb_ms * 1e-3), 1),
         "plain_fwdbwd_evals_per_s": round(n_eval_b / (pl_ms * 1e-3), 1),
         "fwdbwd_vs_plain": round(pl_ms / fb_ms, 3),
-        "roofline": {
-            "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "queryPairKernel<float, float, 3> (Tricubic, two points per lane)",
-            "algorithmic_bytes": alg_bytes,
-        },
+        "roofline": _hbm_roofline(alg_bytes, q_ms * 1e-3,
+                                  "queryPairKernel<float, float, 3> (Tricubic, two points per "
+                                  "lane), distinct point sets"),
+        "contact_roofline": _valu_roofline(FWD_FLOPS * n_eval, c_ms * 1e-3,
+                                           "contactForwardKernel + finalize (1000 poses)",
+                                           "600 flops per (pose, point) evaluation"),
+        "fwdbwd_roofline": _valu_roofline((FWD_FLOPS + BWD_FLOPS) * n_eval_b, fb_ms * 1e-3,
+                                          "contact forward + backward launches (256 poses)",
+                                          "1900 flops per (pose, point) evaluation"),
     }
     tr = _traffic(nq, spec.shape)
     if tr is not None:
         line["roofline"]["traffic"] = tr
         line["roofline"]["traffic_source"] = TRAFFIC_FILE
+    if work and "fill_kernel" in phase:
+        line["build_roofline"] = _valu_roofline(
+            work["flops"], phase["fill_kernel"] * 1e-3,
+            "denseFillKernel + frontier and replay passes (fill phase, device events)",
+            "reference-order nearest-search tests, " + str(work["flops_per_voxel"]) +
+            " flops per voxel (" + WORK_FILE + ")")
     if not args.no_cpu:
-        cpu, threads = cpu_baseline(args, spec, torus, cube, pts_np, X_np, w_np, poses_np)
+        cpu = cpu_leg(args, "ring_on_cube")
         line["cpu_baseline"] = {
-            "value": round(cpu["tricubic_mpts_per_s"], 3), "unit": "Mpts/s", "cores": threads,
-            "kind": "port",
-            "sample": "FP64 oracle: Tricubic on the same 1e6 points x3; build rate on a 16-slice "
-                      "slab of the torus grid; contact on 200 of the 1000 poses",
+            "value": round(cpu["tricubic_mpts_per_s"], 3), "unit": "Mpts/s",
+            "cores": cpu["threads"], "kind": "port", "cpu_model": cpu["cpu_model"],
+            "sample": "FP64 oracle, one warm-up then the median of 5 runs each: Tricubic on the "
+                      "first 1e6-point set; build rate on 262144 uniformly drawn voxels of the "
+                      "torus grid; contact on all 1000 poses; fwd+bwd on 64 of the 256 poses; "
+                      "the 0.1 m cube at 64^3 with the demo's 7680 Trilinear points",
             "build_mvoxel_per_s": round(cpu["build_mvoxel_per_s"], 4),
             "contact_fwd_evals_per_s": round(cpu["contact_fwd_evals_per_s"], 1),
+            "fwdbwd_evals_per_s": round(cpu["fwdbwd_evals_per_s"], 1),
+            "cube64_demo": cpu["cube64_demo"],
         }
+    legs = [x for x in args.legs.split(",") if x]
+    if "sparse_cylinder" in legs:
+        line["sparse_cylinder"] = sparse_cylinder_leg(args, min(3, max(1, K)))
+    if "batched_gears" in legs:
+        line["batched_gears"] = batched_gears_leg(args, min(3, max(1, K)))
     print(json.dumps(line))
 
 
