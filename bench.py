@@ -97,7 +97,7 @@ def parse():
     ap.add_argument("--bwd-poses", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--parts", default="build,query,contact,bwd",
+    ap.add_argument("--parts", default="build,query,contact,bwd,plain",
                     help="subset of the step to run (profiling)")
     ap.add_argument("--legs", default="sparse_cylinder,batched_gears",
                     help="other configurations measured once beside the step ('' for none)")
@@ -516,7 +516,7 @@ def main():
         e[11].record(stream)
         busy()
         e[12].record(stream)
-        if "bwd" in parts:
+        if "plain" in parts:
             for b in range(args.bwd_poses):
                 pb = poses_b[b:b + 1]
                 SC.contact_forward(cube_grid, X, w, pb, law_f)

@@ -79,6 +79,9 @@ struct Law
 {
     double kn, c, mu, reg, margin;
     int order, mode;
+    // 1/kn, 1/c, 1/mu (0 where the parameter is 0), for the backward's
+    // parameter terms
+    double ikn, ic, imu;
 };
 
 struct Terms
@@ -173,7 +176,8 @@ __device__ __forceinline__ void lawTerms(Terms& t, const Pose& P, const Law& L,
     t.damp = L.c > 0.0 && t.g < 0.0; // contacts_sdf.py:355-362
     if (t.damp) t.fd = vmul(L.c * (-vdot(t.u, t.G)), t.G);
     const double gnc = t.gn > 1e-30 ? t.gn : 1e-30; // contacts_sdf.py:369
-    const V un = V{t.G.x / gnc, t.G.y / gnc, t.G.z / gnc};
+    const double ig = 1.0 / gnc;
+    const V un = V{t.G.x * ig, t.G.y * ig, t.G.z * ig};
     t.ut = vsub(t.u, vmul(vdot(t.u, un), un));
     t.D = sqrt(vdot(t.ut, t.ut) + L.reg * L.reg);
     const double phat = t.p * t.gn;
@@ -358,6 +362,7 @@ __device__ __forceinline__ Back pointBackward(const Terms& t, const Law& L, V a,
     const double aG = vdot(a, G);
     double sg = t.dp * aG;
     if (t.fric && t.p != 0.0) sg += (t.dp / t.p) * vdot(a, t.ft);
+    const double ig = 1.0 / gn, iD = 1.0 / t.D;
     V sG = vmul(t.p, a);
     V su = V{0, 0, 0};
     const double uG = vdot(t.u, G);
@@ -371,14 +376,13 @@ __device__ __forceinline__ Back pointBackward(const Terms& t, const Law& L, V a,
     {
         const double k = -L.mu * t.p;
         const V h = vmul(gn, t.ut);
-        const V ahG = vadd(vsub(vsub(vmul(vdot(a, t.u) / gn, G), vmul(aG / gn, t.u)), vmul(uG / gn, a)),
-                           vmul(uG * aG / (gn * gn * gn), G));
-        const V dD = vmul(-uG / (gn * gn * t.D), t.ut);
-        sG = vadd(sG, vmul(k, vsub(vmul(1.0 / t.D, ahG), vmul(vdot(a, h) / (t.D * t.D), dD))));
-        const V Pa = vsub(a, vmul(aG / (gn * gn), G));
-        su = vadd(su, vmul(k * gn, vsub(vmul(1.0 / t.D, Pa),
-                                        vmul(vdot(a, t.ut) / (t.D * t.D * t.D), t.ut))));
-        b.smu = vdot(a, t.ft) / L.mu;
+        const V ahG = vadd(vsub(vsub(vmul(vdot(a, t.u) * ig, G), vmul(aG * ig, t.u)), vmul(uG * ig, a)),
+                           vmul(uG * aG * (ig * ig * ig), G));
+        const V dD = vmul(-uG * (ig * ig * iD), t.ut);
+        sG = vadd(sG, vmul(k, vsub(vmul(iD, ahG), vmul(vdot(a, h) * (iD * iD), dD))));
+        const V Pa = vsub(a, vmul(aG * (ig * ig), G));
+        su = vadd(su, vmul(k * gn, vsub(vmul(iD, Pa), vmul(vdot(a, t.ut) * (iD * iD * iD), t.ut))));
+        b.smu = vdot(a, t.ft) * L.imu;
     }
     const double* h6 = t.hess;
     const V HsG = V{h6[0] * sG.x + h6[1] * sG.y + h6[2] * sG.z,
@@ -386,8 +390,8 @@ __device__ __forceinline__ Back pointBackward(const Terms& t, const Law& L, V a,
                     h6[2] * sG.x + h6[4] * sG.y + h6[5] * sG.z};
     b.gY = vadd(vmul(sg, G), HsG);
     b.gOff = vadd(vmul(wi, vcross(t.fq, gT)), vcross(su, Om));
-    b.skn = L.kn != 0.0 ? vdot(a, vadd(t.fp, t.ft)) / L.kn : 0.0;
-    b.sc = t.damp ? vdot(a, t.fd) / L.c : 0.0;
+    b.skn = L.kn != 0.0 ? vdot(a, vadd(t.fp, t.ft)) * L.ikn : 0.0;
+    b.sc = t.damp ? vdot(a, t.fd) * L.ic : 0.0;
     return b;
 }
 
@@ -395,7 +399,7 @@ __device__ __forceinline__ Back pointBackward(const Terms& t, const Law& L, V a,
 // in partial[(pose*chunks + chunk)*9 + k]; when pointPartial is given, every
 // (pose, point) pair also writes its d/dX (3) and d/dw (1) contribution to
 // pointPartial[(pose - poseBase) * n + point] (zero for inactive pairs), to be
-// summed over poses in pose order by pointReduceKernel.
+// summed over poses by pointReduceKernel.
 __global__ void __launch_bounds__(kBlock)
 contactBackwardPoseKernel(QGrid g, Law L, const double* __restrict__ X,
                           const double* __restrict__ w, int64_t n,
@@ -452,17 +456,47 @@ contactBackwardPoseKernel(QGrid g, Law L, const double* __restrict__ X,
     }
 }
 
-// Sums the per-pose point partials of one pose group in pose order and adds
-// them to (or, for the first group, stores them in) gX / gW.
-__global__ void pointReduceKernel(int64_t n, int poses, const double4* __restrict__ pointPartial,
-                                  int accumulate, double* __restrict__ gX, double* __restrict__ gW)
+// Point-gradient reduction over the poses of one pose group, in a fixed
+// order: a block holds kReducePoints points x kReduceSlices pose slices; each
+// thread sums its slice's poses for its point in pose order, then the slices
+// are added in slice order through LDS and the result is added to (or, for
+// the first pose group, stored in) gX / gW.  Consecutive lanes read
+// consecutive points of one pose, so every load is coalesced.
+constexpr int kReducePoints = 16;
+constexpr int kReduceSlices = 16;
+
+__global__ void __launch_bounds__(kReducePoints * kReduceSlices)
+pointReduceKernel(int64_t n, int poses, const double4* __restrict__ pointPartial, int accumulate,
+                  double* __restrict__ gX, double* __restrict__ gW)
 {
-    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    __shared__ double4 sh[kReduceSlices][kReducePoints];
+    const int lane = int(threadIdx.x) % kReducePoints;
+    const int slice = int(threadIdx.x) / kReducePoints;
+    const int64_t i = int64_t(blockIdx.x) * kReducePoints + lane;
+    const int per = (poses + kReduceSlices - 1) / kReduceSlices;
+    const int p0 = slice * per;
+    const int p1 = p0 + per < poses ? p0 + per : poses;
     double sx = 0.0, sy = 0.0, sz = 0.0, sw = 0.0;
-    for (int p = 0; p < poses; ++p)
+    if (i < n)
     {
-        const double4 v = pointPartial[int64_t(p) * n + i];
+#pragma unroll 4
+        for (int p = p0; p < p1; ++p)
+        {
+            const double4 v = pointPartial[int64_t(p) * n + i];
+            sx += v.x;
+            sy += v.y;
+            sz += v.z;
+            sw += v.w;
+        }
+    }
+    sh[slice][lane] = make_double4(sx, sy, sz, sw);
+    __syncthreads();
+    if (slice != 0 || i >= n) return;
+    sx = 0.0, sy = 0.0, sz = 0.0, sw = 0.0;
+#pragma unroll
+    for (int q = 0; q < kReduceSlices; ++q)
+    {
+        const double4 v = sh[q][lane];
         sx += v.x;
         sy += v.y;
         sz += v.z;
@@ -680,15 +714,16 @@ contactEvaluateKernel(QGrid g, Law L, const double* __restrict__ X, const double
         V fq = vmul(p, G);
         if (L.c > 0.0 && it.gap < 0.0) fq = vadd(fq, vmul(L.c * (-vdot(u, G)), G)); // :355-362
         const double gnc = gn > 1e-30 ? gn : 1e-30;
-        const V un = V{G.x / gnc, G.y / gnc, G.z / gnc};
+        const double ig = 1.0 / gnc;
+        const V un = V{G.x * ig, G.y * ig, G.z * ig};
         const V ut = vsub(u, vmul(vdot(u, un), un));
         const double utn = sqrt(vdot(ut, ut));
         const double D = sqrt(utn * utn + L.reg * L.reg);
         const double phat = p * gn;
         if (L.mu * phat > 0.0) // :374-377
         {
-            const double k = -L.mu * phat;
-            fq = vadd(fq, V{k * ut.x / D, k * ut.y / D, k * ut.z / D});
+            const double k = -L.mu * phat * (1.0 / D);
+            fq = vadd(fq, V{k * ut.x, k * ut.y, k * ut.z});
         }
         const double wi = w[i];
         const V tq = vcross(off, fq);
@@ -735,9 +770,50 @@ __global__ void contactEvaluateFinalize(int64_t npairs, int chunks, const double
     if (active) active[b] = on ? int32_t(s[6]) : 0;
 }
 
+// contactBackwardPoseFinalize and paramReduceKernel in one block for up to
+// kFusedFinalizePoses poses: thread t finalises poses t, t + kBlock, ... (chunk
+// sums in chunk order) and keeps their parameter terms, which the block then
+// reduces exactly as paramReduceKernel does, so the bits are the same.
+constexpr int64_t kFusedFinalizePoses = 4096;
+
+__global__ void __launch_bounds__(kBlock)
+contactBackwardFinalizeParams(int64_t nposes, int chunks, const double* __restrict__ partial,
+                              double* __restrict__ gTwist, double* __restrict__ gParams)
+{
+    __shared__ double sh[3 * kBlock];
+    double acc[3] = {0, 0, 0};
+    for (int64_t pose = threadIdx.x; pose < nposes; pose += kBlock)
+    {
+        double sum[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int c = 0; c < chunks; ++c)
+        {
+            const double* p = partial + (pose * chunks + c) * 9;
+#pragma unroll
+            for (int k = 0; k < 9; ++k) sum[k] += p[k];
+        }
+        if (gTwist)
+        {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) gTwist[6 * pose + k] = sum[k];
+        }
+        acc[0] += sum[6];
+        acc[1] += sum[7];
+        acc[2] += sum[8];
+    }
+    blockReduce<3>(acc, sh);
+    if (threadIdx.x == 0)
+    {
+        gParams[0] = acc[0];
+        gParams[1] = acc[1];
+        gParams[2] = acc[2];
+    }
+}
+
 Law makeLaw(const sdfo_contact_params& p)
 {
-    return Law{p.kn, p.damping, p.mu, p.regularizer, p.narrow_margin, p.order, p.mode};
+    return Law{p.kn, p.damping, p.mu, p.regularizer, p.narrow_margin, p.order, p.mode,
+               p.kn != 0.0 ? 1.0 / p.kn : 0.0, p.damping != 0.0 ? 1.0 / p.damping : 0.0,
+               p.mu != 0.0 ? 1.0 / p.mu : 0.0};
 }
 
 // Split each pose's points into chunks so that the launch holds at least
@@ -1005,10 +1081,20 @@ void launchContactBackward(const sdfo_grid& g, const double* X, const double* w,
         SDFO_HIP(hipGetLastError());
         if (wantPoint)
         {
-            hipLaunchKernelGGL(pointReduceKernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s,
-                               n, int(cnt), pointPartial, base > 0 ? 1 : 0, gX, gW);
+            hipLaunchKernelGGL(pointReduceKernel,
+                               dim3(unsigned((n + kReducePoints - 1) / kReducePoints)),
+                               dim3(kReducePoints * kReduceSlices), 0, s, n, int(cnt),
+                               static_cast<const double4*>(pointPartial), base > 0 ? 1 : 0, gX, gW);
             SDFO_HIP(hipGetLastError());
         }
+    }
+    if (gParams && nb <= kFusedFinalizePoses)
+    {
+        // one block: the pose finalize and the parameter sum in one launch
+        hipLaunchKernelGGL(contactBackwardFinalizeParams, dim3(1), dim3(kBlock), 0, s, nb, chunks,
+                           static_cast<const double*>(partial), gTwist, gParams);
+        SDFO_HIP(hipGetLastError());
+        return;
     }
     if (gTwist || gParams)
     {
