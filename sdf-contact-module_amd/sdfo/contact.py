@@ -310,11 +310,17 @@ def contact_backward(grid: SdfGrid, X, w, poses, g_out, law: ContactLaw = Contac
     _check_inputs(X, w, poses)
     g_out = g_out.contiguous().to(torch.float64)
     B, N = poses.shape[0], X.shape[0]
-    dev = X.device
-    gt = torch.empty((B, 6), dtype=torch.float64, device=dev) if want_twist else None
-    gp = torch.empty(3, dtype=torch.float64, device=dev) if want_params else None
-    gx = torch.empty((N, 3), dtype=torch.float64, device=dev) if want_X else None
-    gw = torch.empty(N, dtype=torch.float64, device=dev) if want_w else None
+    if g_out.shape != (B, 6) or not g_out.is_cuda:
+        raise ValueError("g_out must be a (B,6) GPU tensor")
+    # one allocation for all outputs, split into views
+    sizes = [6 * B if want_twist else 0, 3 if want_params else 0, 3 * N if want_X else 0,
+             N if want_w else 0]
+    buf = torch.empty(sum(sizes), dtype=torch.float64, device=X.device)
+    parts = torch.split(buf, sizes)
+    gt = parts[0].view(B, 6) if want_twist else None
+    gp = parts[1] if want_params else None
+    gx = parts[2].view(N, 3) if want_X else None
+    gw = parts[3] if want_w else None
     p = law.ctypes()
     check(lib.sdfo_contact_backward(grid.handle, _ptr(X), _ptr(w), N, _ptr(poses), B, C.byref(p),
                                     _ptr(g_out), _ptr(gt), _ptr(gp), _ptr(gx), _ptr(gw),
