@@ -220,5 +220,14 @@ def test_cube64_demo_trilinear(O, sdfo_gpu):
         r64 = O.query_dense(og, ref, "Trilinear", pts)
         assert np.array_equal(gap.cpu().numpy(), r64["gap"])
         assert np.array_equal(grad.cpu().numpy(), r64["grad"])
+        # the configuration's own sampling: 7680 seeded points uniform over the
+        # padded box [bmin, bmax], about (63/64)^3 = 95.4% of them interpolable
+        ub = scenes.box_points(g.spec, 7680, seed=11)
+        gap, grad, valid = g.query(torch.from_numpy(ub).cuda(), "Trilinear")
+        rb = O.query_dense(og, ref, "Trilinear", ub)
+        vv = valid.cpu().numpy()
+        assert np.array_equal(vv, rb["valid"]) and 0.93 < vv.mean() < 0.98
+        assert np.array_equal(gap.cpu().numpy(), rb["gap"])
+        assert np.array_equal(grad.cpu().numpy(), rb["grad"])
     finally:
         g.close()

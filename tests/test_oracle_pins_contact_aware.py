@@ -260,3 +260,33 @@ def test_sparse_second_order_exact_on_quadratic(O):
     assert np.array_equal(r2["gap"], r1["gap"]) and np.array_equal(r2["grad"], r1["grad"])
     # first order is off by the dropped quadratic term
     assert np.abs(r1["gap"] - f(p)).max() > 1e-4
+
+
+def test_probe_equals_the_full_builds(O):
+    """The sampled contact-aware probe (used to check large GPU builds) gives
+    the full dense and sparse builds' candidates voxel for voxel, and the
+    sparse build's brick mask."""
+    v, t = meshes.torus(0.05, 0.012, 24, 12)
+    fn = np.cross(v[t[:, 1]] - v[t[:, 0]], v[t[:, 2]] - v[t[:, 0]])
+    fn /= np.linalg.norm(fn, axis=1, keepdims=True)
+    cn = np.repeat(fn[:, None], 3, 1)
+    g = O.setup_grid(v, 0.1, 0.0, 24, 24)
+    nx, ny, nz = g.shape
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    ijk = np.stack([i.ravel(), j.ravel(), k.ravel()], 1)
+    d = O.build_contact_aware_dense(v, t, cn, g)
+    p = O.contact_aware_probe(v, t, cn, g, ijk)
+    for key in ("ccount", "cphi", "cn", "cbranch"):
+        assert np.array_equal(d[key], p[key]), key
+    s = O.build_contact_aware_sparse(v, t, cn, g, block_dim=8, halo=2, band_voxels=4)
+    L = np.arange(512)
+    loc = np.stack([L % 8, (L // 8) % 8, L // 64], 1)
+    for b in range(len(s["bijk"])):
+        vox = s["bijk"][b] * 8 + loc
+        inb = np.all(vox < np.array(g.shape), 1)
+        pp = O.contact_aware_probe(v, t, cn, g, vox[inb], sparse_block_dim=8, halo=2, band_voxels=4)
+        if b == 0:
+            assert np.array_equal(s["active"], pp["mask"])
+        assert np.array_equal(s["ccount"][b][inb], pp["ccount"])
+        assert np.array_equal(s["cphi"][b][inb], pp["cphi"].astype(np.float32))
+        assert np.array_equal(s["cbranch"][b][inb], pp["cbranch"])
