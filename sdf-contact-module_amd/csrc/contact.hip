@@ -84,12 +84,18 @@ struct Law
     double ikn, ic, imu;
 };
 
+// The penalty, damping and friction forces are kept as their scalar factors
+// (f_p = p G, f_d = cd G, f_t = s u_t) and rebuilt where the backward needs
+// them, with the same operations as the forward, to keep the live state small.
 struct Terms
 {
     bool active, damp, fric;
-    V Y, off, u, G, fp, fd, ft, fq, ut;
-    double g, p, dp, gn, D;
+    V Y, off, u, G, fq, ut;
+    double g, p, dp, gn, D, cd, s;
     double hess[6];
+    __device__ __forceinline__ V fp() const { return vmul(p, G); }
+    __device__ __forceinline__ V fd() const { return damp ? vmul(cd, G) : V{0, 0, 0}; }
+    __device__ __forceinline__ V ft() const { return fric ? vmul(s, ut) : V{0, 0, 0}; }
 };
 
 __device__ __forceinline__ double powi(double x, int k)
@@ -171,24 +177,17 @@ __device__ __forceinline__ void lawTerms(Terms& t, const Pose& P, const Law& L,
     t.G = V{it.grad.x, it.grad.y, it.grad.z};
     t.gn = sqrt(vdot(t.G, t.G));
     t.u = vadd(V{P.v0[0], P.v0[1], P.v0[2]}, vcross(V{P.Om[0], P.Om[1], P.Om[2]}, t.off));
-    t.fp = vmul(t.p, t.G);
-    t.fd = V{0, 0, 0};
     t.damp = L.c > 0.0 && t.g < 0.0; // contacts_sdf.py:355-362
-    if (t.damp) t.fd = vmul(L.c * (-vdot(t.u, t.G)), t.G);
+    t.cd = t.damp ? L.c * (-vdot(t.u, t.G)) : 0.0;
     const double gnc = t.gn > 1e-30 ? t.gn : 1e-30; // contacts_sdf.py:369
     const double ig = 1.0 / gnc;
     const V un = V{t.G.x * ig, t.G.y * ig, t.G.z * ig};
     t.ut = vsub(t.u, vmul(vdot(t.u, un), un));
     t.D = sqrt(vdot(t.ut, t.ut) + L.reg * L.reg);
     const double phat = t.p * t.gn;
-    t.ft = V{0, 0, 0};
     t.fric = L.mu * phat > 0.0; // contacts_sdf.py:375-377
-    if (t.fric)
-    {
-        const double s = -L.mu * phat / t.D;
-        t.ft = vmul(s, t.ut);
-    }
-    t.fq = vadd(vadd(t.fp, t.fd), t.ft);
+    t.s = t.fric ? -L.mu * phat / t.D : 0.0;
+    t.fq = vadd(vadd(t.fp(), t.fd()), t.ft());
 }
 
 // Fixed-order LDS tree reduction of NV doubles per lane.
@@ -361,7 +360,7 @@ __device__ __forceinline__ Back pointBackward(const Terms& t, const Law& L, V a,
     const double gn = t.gn;
     const double aG = vdot(a, G);
     double sg = t.dp * aG;
-    if (t.fric && t.p != 0.0) sg += (t.dp / t.p) * vdot(a, t.ft);
+    if (t.fric && t.p != 0.0) sg += (t.dp / t.p) * vdot(a, t.ft());
     const double ig = 1.0 / gn, iD = 1.0 / t.D;
     V sG = vmul(t.p, a);
     V su = V{0, 0, 0};
@@ -382,7 +381,7 @@ __device__ __forceinline__ Back pointBackward(const Terms& t, const Law& L, V a,
         sG = vadd(sG, vmul(k, vsub(vmul(iD, ahG), vmul(vdot(a, h) * (iD * iD), dD))));
         const V Pa = vsub(a, vmul(aG * (ig * ig), G));
         su = vadd(su, vmul(k * gn, vsub(vmul(iD, Pa), vmul(vdot(a, t.ut) * (iD * iD * iD), t.ut))));
-        b.smu = vdot(a, t.ft) * L.imu;
+        b.smu = vdot(a, t.ft()) * L.imu;
     }
     const double* h6 = t.hess;
     const V HsG = V{h6[0] * sG.x + h6[1] * sG.y + h6[2] * sG.z,
@@ -390,8 +389,8 @@ __device__ __forceinline__ Back pointBackward(const Terms& t, const Law& L, V a,
                     h6[2] * sG.x + h6[4] * sG.y + h6[5] * sG.z};
     b.gY = vadd(vmul(sg, G), HsG);
     b.gOff = vadd(vmul(wi, vcross(t.fq, gT)), vcross(su, Om));
-    b.skn = L.kn != 0.0 ? vdot(a, vadd(t.fp, t.ft)) * L.ikn : 0.0;
-    b.sc = t.damp ? vdot(a, t.fd) * L.ic : 0.0;
+    b.skn = L.kn != 0.0 ? vdot(a, vadd(t.fp(), t.ft())) * L.ikn : 0.0;
+    b.sc = t.damp ? vdot(a, t.fd()) * L.ic : 0.0;
     return b;
 }
 
