@@ -101,6 +101,9 @@ def parse():
                     help="subset of the step to run (profiling)")
     ap.add_argument("--legs", default="sparse_cylinder,batched_gears",
                     help="other configurations measured once beside the step ('' for none)")
+    ap.add_argument("--tuner", default="on", choices=["on", "off"],
+                    help="autotuner: 'on' times the launch variants on the first call of each "
+                         "size class (during the warm-up) and keeps the fastest")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 256 MiB device copy first (scales HBM counters)")
     ap.add_argument("--workload", default="ring_on_cube",
@@ -151,6 +154,7 @@ def sparse_cylinder_leg(args, steps):
     import sdfo
     from sdfo import meshes, scenes
 
+    sdfo.tuner_set_mode(1 if args.tuner == "on" else 0)
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream()
     v, t = meshes.hollow_cylinder(0.04, 0.03, 0.05, 1024, 64)
@@ -278,6 +282,7 @@ def batched_gears_leg(args, steps):
     import sdfo
     from sdfo import _lib, scenes
 
+    sdfo.tuner_set_mode(1 if args.tuner == "on" else 0)
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream()
     batch = scenes.gear_batch(64, seed=5)
@@ -402,6 +407,7 @@ def main():
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     stream = torch.cuda.current_stream()
+    sdfo.tuner_set_mode(1 if args.tuner == "on" else 0)
 
     torus = scenes.ring_mesh(256, 64)
     cube = scenes.cube_mesh()
@@ -611,6 +617,7 @@ def main():
         "fwdbwd_roofline": _valu_roofline((FWD_FLOPS + BWD_FLOPS) * n_eval_b, fb_ms * 1e-3,
                                           "contact forward + backward launches (256 poses)",
                                           "1900 flops per (pose, point) evaluation"),
+        "tuner": sdfo.tuner_report(),
     }
     tr = _traffic(nq, spec.shape)
     if tr is not None:

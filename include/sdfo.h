@@ -278,9 +278,11 @@ void sdfo_query_plan_free(sdfo_query_plan* plan);
 
 /* ---- autotuner (no reference counterpart) ------------------------------ */
 /* SDFO_TUNE_ON_FIRST_USE: the first dense build of a size class (triangle
- * and voxel counts, powers of two) times every voxel-brick shape of the fill
- * and keeps the fastest; the first run of a batched query plan of a size
- * class times every block size.  Results never depend on the choice. */
+ * and voxel counts, powers of two) times every voxel-brick shape x step
+ * budget of the fill and keeps the fastest; the first run of a batched query
+ * plan of a size class, and the first FP32 Trilinear / Tricubic query of a
+ * size class on a grid with cell records, time every block size.  A tuning
+ * call synchronises its stream.  Results never depend on the choice. */
 #define SDFO_TUNE_OFF 0
 #define SDFO_TUNE_ON_FIRST_USE 1
 int sdfo_tuner_set_mode(int32_t mode);

@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "interp_device.hpp"
@@ -143,17 +144,23 @@ __device__ __forceinline__ void pairQuery(const QGrid& g, const P3<TP>* __restri
 // thread: both points' 128-byte records are requested before either is
 // interpolated, doubling the loads in flight per lane (the gather is
 // latency-bound at one point per lane).  Same arithmetic as queryKernel.
-template <typename TP, typename T, int Mode, bool Sparse>
-__global__ void __launch_bounds__(kPairBlock)
+template <typename TP, typename T, int Mode, bool Sparse, int Block = kPairBlock>
+__global__ void __launch_bounds__(Block)
 queryPairKernel(QGrid g, const P3<TP>* __restrict__ pts, int64_t n, TP* __restrict__ gap,
                 P3<TP>* __restrict__ grad3, uint8_t* __restrict__ valid,
                 P3<TP>* __restrict__ normal3, P3<TP>* __restrict__ witness3,
                 int32_t* __restrict__ branch)
 {
-    const int64_t i0 = int64_t(blockIdx.x) * (kLanePoints * kPairBlock) + threadIdx.x;
-    pairQuery<TP, T, Mode, Sparse, kLanePoints>(g, pts, i0, kPairBlock, n, gap, grad3, valid, normal3, witness3,
+    const int64_t i0 = int64_t(blockIdx.x) * (kLanePoints * Block) + threadIdx.x;
+    pairQuery<TP, T, Mode, Sparse, kLanePoints>(g, pts, i0, Block, n, gap, grad3, valid, normal3, witness3,
                            branch);
 }
+
+// Block sizes of the record-path query the tuner chooses from (the default
+// first); every variant computes the same outputs.
+constexpr int kPairBlockCount = 4;
+constexpr int kPairBlocks[kPairBlockCount] = {128, 64, 256, 512};
+const char* const kPairBlockNames[kPairBlockCount] = {"block128", "block64", "block256", "block512"};
 
 // One point stream of a batched query: its grid, its points and its range of
 // 128-point work units (one wave each, two points per lane) in the launch.
@@ -259,21 +266,50 @@ void launchQuery(const sdfo_grid& g, int mode, const TP* pts, int64_t n, TP* gap
     const bool cubic = mode == SDFO_TRICUBIC || (mode == SDFO_CONTACT_AWARE && q.stride <= 0);
     if ((q.sparse ? q.scellRec != nullptr : q.cellRec != nullptr) && (cubic || mode == SDFO_TRILINEAR))
     {
-        const dim3 pg(unsigned((n + kLanePoints * kPairBlock - 1) / (kLanePoints * kPairBlock))), pb(kPairBlock);
-        auto launch = [&](auto sparseTag) {
+        auto launchBlock = [&](auto sparseTag, auto blockTag) {
             constexpr bool Sp = decltype(sparseTag)::value;
+            constexpr int B = decltype(blockTag)::value;
+            const dim3 pg(unsigned((n + kLanePoints * B - 1) / (kLanePoints * B))), pb(B);
             if (cubic)
-                hipLaunchKernelGGL((queryPairKernel<TP, T, SDFO_TRICUBIC, Sp>), pg, pb, 0, s, q, p, n,
-                                   gap, gr, valid, nm, wt, branch);
+                hipLaunchKernelGGL((queryPairKernel<TP, T, SDFO_TRICUBIC, Sp, B>), pg, pb, 0, s, q, p,
+                                   n, gap, gr, valid, nm, wt, branch);
             else
-                hipLaunchKernelGGL((queryPairKernel<TP, T, SDFO_TRILINEAR, Sp>), pg, pb, 0, s, q, p, n,
-                                   gap, gr, valid, nm, wt, branch);
+                hipLaunchKernelGGL((queryPairKernel<TP, T, SDFO_TRILINEAR, Sp, B>), pg, pb, 0, s, q,
+                                   p, n, gap, gr, valid, nm, wt, branch);
         };
-        if (q.sparse)
-            launch(std::true_type{});
-        else
-            launch(std::false_type{});
-        SDFO_HIP(hipGetLastError());
+        auto launch = [&](int variant) {
+            auto withSparse = [&](auto sparseTag) {
+                switch (variant)
+                {
+                case 1: launchBlock(sparseTag, std::integral_constant<int, kPairBlocks[1]>{}); break;
+                case 2: launchBlock(sparseTag, std::integral_constant<int, kPairBlocks[2]>{}); break;
+                case 3: launchBlock(sparseTag, std::integral_constant<int, kPairBlocks[3]>{}); break;
+                default: launchBlock(sparseTag, std::integral_constant<int, kPairBlocks[0]>{}); break;
+                }
+            };
+            if (q.sparse)
+                withSparse(std::true_type{});
+            else
+                withSparse(std::false_type{});
+            SDFO_HIP(hipGetLastError());
+        };
+        // the tuner times every block size on this call's own points (each
+        // run writes the same outputs) the first time a size class is seen
+        const std::string key = std::string("query ") + (q.sparse ? "sparse " : "dense ") +
+                                (cubic ? "tricubic" : "trilinear") +
+                                (sizeof(TP) == 8 ? " f64" : " f32") + " points~2^" +
+                                std::to_string(log2Bucket(n));
+        int variant = 0;
+        if (!tunerLookup(key, variant) && tunerEnabled() && !streamCapturing(s))
+        {
+            std::vector<std::pair<int, double>> t;
+            launch(0); // warm the first launch out of the timings
+            for (int v = 0; v < kPairBlockCount; ++v)
+                t.emplace_back(v, timeOnStream(s, [&] { launch(v); }));
+            tunerRecord(key, t, kPairBlockNames);
+            tunerLookup(key, variant);
+        }
+        launch(variant);
         return;
     }
     switch (mode)

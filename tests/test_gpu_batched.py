@@ -196,11 +196,19 @@ def test_tuner_chooses_and_keeps_results(sdfo_gpu):
                 torch.empty(n, dtype=torch.uint8, device="cuda")]
         print("[gpu] tuner: query plan block sizes", flush=True)
         plan.run(P, *outs, "Tricubic")
+        print("[gpu] tuner: record-path query block sizes", flush=True)
+        q_pts = P[: counts[0]].contiguous()
+        q_tuned = grids[0].query(q_pts, "Tricubic")
         rep = sdfo.tuner_report()
     finally:
         sdfo.tuner_set_mode(0)
     print(rep, flush=True)
-    assert "dense_fill" in rep and "query_plan" in rep and "*" in rep
+    assert "dense_fill" in rep and "query_plan" in rep and "query dense tricubic" in rep
+    assert "*" in rep
+    # the tuned query gives the bits of every other block size
+    q_ref = grids[0].query(q_pts, "Tricubic")
+    for a, b in zip(q_tuned, q_ref):
+        assert torch.equal(a, b)
     for key in ("phi", "n", "witness", "tri"):
         assert np.array_equal(got[key], ref[key]), key
     # a tuned choice is reused (and still exact) with the tuner off
