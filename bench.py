@@ -387,6 +387,34 @@ def bench_batched_gears(args):
     print(json.dumps(line))
 
 
+def _host_api_latency(grid, pts):
+    """Round trip of the per-point C++ API path (SdfOracle::query and
+    queryBatch call sdfo_query_host_f64: copy in, FP64 kernel, copy out,
+    synchronise): median microseconds per call for one point and for a batch
+    of 7680 points, Tricubic."""
+    import ctypes as C
+
+    from sdfo import _lib
+
+    out = {}
+    for n in (1, 7680):
+        p = np.ascontiguousarray(pts[:n], np.float64)
+        gap, nrm, wit = np.empty(n), np.empty((n, 3)), np.empty((n, 3))
+        val = np.empty(n, np.uint8)
+        ptr = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+        ts = []
+        for k in range(220):
+            t0 = time.perf_counter()
+            _lib.check(_lib.lib.sdfo_query_host_f64(grid.handle, 3, ptr(p), n, ptr(gap), ptr(nrm),
+                                                   ptr(wit), ptr(val), None, None, None, None,
+                                                   None, None, None))
+            if k >= 20:
+                ts.append(time.perf_counter() - t0)
+        out["query_1_point" if n == 1 else "query_batch_7680_points"] = round(
+            float(np.median(ts)) * 1e6, 1)
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -618,6 +646,7 @@ def main():
                                           "contact forward + backward launches (256 poses)",
                                           "1900 flops per (pose, point) evaluation"),
         "tuner": sdfo.tuner_report(),
+        "host_api_latency_us": _host_api_latency(cube_grid, sets_np[0]),
     }
     tr = _traffic(nq, spec.shape)
     if tr is not None:

@@ -58,7 +58,8 @@ int sdfo_debug_ca_merge_cap(int32_t first, int32_t second);
 /* Test switch for the calling thread: grids built or loaded while it is on
  * keep no cell records (the packed 8-corner copies that FP32 Trilinear /
  * Tricubic queries and the contact kernels read), as grids whose records
- * would exceed the library's 4 GiB limit do, so tests can check the
+ * would exceed the library's limit (an eighth of device memory, at least
+ * 4 GiB) do, so tests can check the
  * per-corner gather path against the CPU oracle on small grids. */
 int sdfo_debug_skip_cell_records(int32_t on);
 

@@ -47,10 +47,28 @@ thread_local std::string g_lastError;
 
 // Test switch of the calling thread: grids built or loaded while it is on
 // keep no cell records, so every query and contact kernel takes the
-// per-corner gather path that grids past kCellRecordLimit use.
+// per-corner gather path that grids past cellRecordLimit() use.
 thread_local bool g_skipCellRecords = false;
 
-bool cellRecordsFit(int64_t bytes) { return !g_skipCellRecords && bytes <= kCellRecordLimit; }
+bool cellRecordsFit(int64_t bytes) { return !g_skipCellRecords && bytes <= cellRecordLimit(); }
+}
+
+int64_t cellRecordLimit()
+{
+    static std::mutex m;
+    static int64_t limits[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return kCellRecordLimit;
+    std::lock_guard<std::mutex> lk(m);
+    if (limits[dev] == 0)
+    {
+        size_t freeB = 0, totalB = 0;
+        int64_t lim = kCellRecordLimit;
+        if (hipMemGetInfo(&freeB, &totalB) == hipSuccess && int64_t(totalB / 8) > lim)
+            lim = int64_t(totalB / 8);
+        limits[dev] = lim;
+    }
+    return limits[dev];
 }
 
 void setSkipCellRecords(bool on) { g_skipCellRecords = on; }
@@ -1274,6 +1292,44 @@ int sdfo_build_dense_batched(int32_t n_meshes, const double* verts, const int64_
     });
 }
 
+namespace {
+// Pinned, device-mapped, coherent host memory per calling thread for the
+// host-pointer FP64 query: calls whose arrays fit kHostStageBytes run the
+// kernels directly on it (they read the points and write the results over
+// PCIe), so a call costs one launch and one synchronisation plus two host
+// copies, instead of a device allocation and one DMA per array.  Each call
+// synchronises before it returns, so one buffer per thread is never shared
+// by two calls in flight.
+constexpr size_t kHostStageBytes = size_t(8) << 20;
+
+struct HostStage
+{
+    char* host = nullptr;
+    char* dev = nullptr;
+    size_t cap = 0;
+    ~HostStage()
+    {
+        if (host) (void)hipHostFree(host);
+    }
+    char* reserve(size_t bytes)
+    {
+        if (bytes <= cap) return dev;
+        if (host) SDFO_HIP(hipHostFree(host));
+        host = dev = nullptr;
+        cap = 0;
+        const size_t want = std::max(bytes, size_t(64) << 10);
+        SDFO_HIP(hipHostMalloc(reinterpret_cast<void**>(&host), want,
+                               hipHostMallocMapped | hipHostMallocPortable | hipHostMallocCoherent));
+        void* d = nullptr;
+        SDFO_HIP(hipHostGetDevicePointer(&d, host, 0));
+        dev = static_cast<char*>(d);
+        cap = want;
+        return dev;
+    }
+};
+thread_local HostStage g_hostStage;
+} // namespace
+
 int sdfo_query_host_f64(const sdfo_grid* grid, int32_t mode, const double* points, int64_t n,
                         double* gap, double* normal3, double* witness3, uint8_t* valid,
                         int32_t* branch, uint8_t* cand_count, double* cand_gap,
@@ -1294,7 +1350,7 @@ int sdfo_query_host_f64(const sdfo_grid* grid, int32_t mode, const double* point
         DeviceGuard dg(grid->device);
         const hipStream_t s = hipStream_t(stream);
         const size_t N = size_t(n);
-        // one device block for every array: points, gap, grad, normal, witness,
+        // one block for every array: points, gap, grad, normal, witness,
         // branch, valid, then the candidate arrays
         size_t off = 0;
         auto carve = [&](size_t bytes) {
@@ -1313,53 +1369,77 @@ int sdfo_query_host_f64(const sdfo_grid* grid, int32_t mode, const double* point
             oCW = carve(192 * N);
             oCB = carve(32 * N);
         }
+        const bool mapped = off <= kHostStageBytes;
         char* base = nullptr;
-        SDFO_HIP(hipMallocAsync(reinterpret_cast<void**>(&base), off, s));
-        try
-        {
-            auto D = [&](size_t o) { return base + o; };
-            SDFO_HIP(hipMemcpyAsync(D(oP), points, 24 * N, hipMemcpyHostToDevice, s));
+        if (mapped)
+            base = g_hostStage.reserve(off);
+        else
+            SDFO_HIP(hipMallocAsync(reinterpret_cast<void**>(&base), off, s));
+        char* const host = mapped ? g_hostStage.host : nullptr;
+        auto D = [&](size_t o) { return base + o; };
+        auto run = [&] {
+            if (mapped)
+            {
+                std::memcpy(host + oP, points, 24 * N);
+                if (cands) std::memset(host + oCC, 0, N);
+            }
+            else
+            {
+                SDFO_HIP(hipMemcpyAsync(D(oP), points, 24 * N, hipMemcpyHostToDevice, s));
+                if (cands) SDFO_HIP(hipMemsetAsync(D(oCC), 0, N, s));
+            }
             launchQueryF64(*grid, mode, reinterpret_cast<double*>(D(oP)), n,
                            reinterpret_cast<double*>(D(oG)), reinterpret_cast<double*>(D(oGr)),
                            reinterpret_cast<uint8_t*>(D(oV)), reinterpret_cast<double*>(D(oN)),
                            reinterpret_cast<double*>(D(oW)), reinterpret_cast<int32_t*>(D(oB)), s);
-            if (cands)
+            if (cands && mode == SDFO_CONTACT_AWARE)
+                launchCandidatesF64(*grid, reinterpret_cast<double*>(D(oP)), n,
+                                    reinterpret_cast<uint8_t*>(D(oCC)),
+                                    reinterpret_cast<double*>(D(oCG)),
+                                    reinterpret_cast<double*>(D(oCN)),
+                                    reinterpret_cast<double*>(D(oCW)),
+                                    reinterpret_cast<int32_t*>(D(oCB)),
+                                    reinterpret_cast<double*>(D(oG)),
+                                    reinterpret_cast<double*>(D(oN)),
+                                    reinterpret_cast<double*>(D(oW)),
+                                    reinterpret_cast<uint8_t*>(D(oV)), s);
+            struct Out
             {
-                SDFO_HIP(hipMemsetAsync(D(oCC), 0, N, s));
-                if (mode == SDFO_CONTACT_AWARE)
-                    launchCandidatesF64(*grid, reinterpret_cast<double*>(D(oP)), n,
-                                        reinterpret_cast<uint8_t*>(D(oCC)),
-                                        reinterpret_cast<double*>(D(oCG)),
-                                        reinterpret_cast<double*>(D(oCN)),
-                                        reinterpret_cast<double*>(D(oCW)),
-                                        reinterpret_cast<int32_t*>(D(oCB)),
-                                        reinterpret_cast<double*>(D(oG)),
-                                        reinterpret_cast<double*>(D(oN)),
-                                        reinterpret_cast<double*>(D(oW)),
-                                        reinterpret_cast<uint8_t*>(D(oV)), s);
-            }
-            SDFO_HIP(hipMemcpyAsync(gap, D(oG), 8 * N, hipMemcpyDeviceToHost, s));
-            SDFO_HIP(hipMemcpyAsync(normal3, D(oN), 24 * N, hipMemcpyDeviceToHost, s));
-            SDFO_HIP(hipMemcpyAsync(witness3, D(oW), 24 * N, hipMemcpyDeviceToHost, s));
-            SDFO_HIP(hipMemcpyAsync(valid, D(oV), N, hipMemcpyDeviceToHost, s));
-            if (branch) SDFO_HIP(hipMemcpyAsync(branch, D(oB), 4 * N, hipMemcpyDeviceToHost, s));
-            if (cands)
-            {
-                SDFO_HIP(hipMemcpyAsync(cand_count, D(oCC), N, hipMemcpyDeviceToHost, s));
-                SDFO_HIP(hipMemcpyAsync(cand_gap, D(oCG), 64 * N, hipMemcpyDeviceToHost, s));
-                SDFO_HIP(hipMemcpyAsync(cand_normal3, D(oCN), 192 * N, hipMemcpyDeviceToHost, s));
-                SDFO_HIP(hipMemcpyAsync(cand_witness3, D(oCW), 192 * N, hipMemcpyDeviceToHost, s));
-                SDFO_HIP(hipMemcpyAsync(cand_branch, D(oCB), 32 * N, hipMemcpyDeviceToHost, s));
-            }
+                void* dst;
+                size_t o, bytes;
+            };
+            const Out outs[] = {{gap, oG, 8 * N},
+                                {normal3, oN, 24 * N},
+                                {witness3, oW, 24 * N},
+                                {valid, oV, N},
+                                {branch, oB, 4 * N},
+                                {cand_count, oCC, N},
+                                {cand_gap, oCG, 64 * N},
+                                {cand_normal3, oCN, 192 * N},
+                                {cand_witness3, oCW, 192 * N},
+                                {cand_branch, oCB, 32 * N}};
+            const size_t nOut = cands ? 10 : 5;
+            if (!mapped)
+                for (size_t k = 0; k < nOut; ++k)
+                    if (outs[k].dst)
+                        SDFO_HIP(hipMemcpyAsync(outs[k].dst, D(outs[k].o), outs[k].bytes,
+                                                hipMemcpyDeviceToHost, s));
             SDFO_HIP(hipStreamSynchronize(s));
+            if (mapped)
+                for (size_t k = 0; k < nOut; ++k)
+                    if (outs[k].dst) std::memcpy(outs[k].dst, host + outs[k].o, outs[k].bytes);
+        };
+        try
+        {
+            run();
         }
         catch (...)
         {
             (void)hipStreamSynchronize(s);
-            (void)hipFreeAsync(base, s);
+            if (!mapped) (void)hipFreeAsync(base, s);
             throw;
         }
-        SDFO_HIP(hipFreeAsync(base, s));
+        if (!mapped) SDFO_HIP(hipFreeAsync(base, s));
     });
 }
 

@@ -383,9 +383,12 @@ void finishDense(sdfo_grid& g, hipStream_t s, bool mirrored = false);
 void finishSparse(sdfo_grid& g, const std::vector<int32_t>& bijk, hipStream_t s);
 std::vector<int32_t> brickTable(const std::vector<int32_t>& bijk, int nbx, int nby, int nbz);
 
-// Dense grids whose packed cell records fit in this many bytes get them
-// (8 x float4 per cell: a random interpolation reads one 128-byte line).
+// Grids whose packed cell records fit in cellRecordLimit() bytes get them
+// (8 x float4 per cell: a random interpolation reads one 128-byte line):
+// an eighth of the current device's memory (36 GB on a 288 GB MI355X, so a
+// dense 512^3 grid's 17.2 GB of records fit), and never less than 4 GiB.
 constexpr int64_t kCellRecordLimit = int64_t(4) << 30;
+int64_t cellRecordLimit();
 
 // Contact-aware build (contact_aware.hip); corner normals: 9 doubles per triangle.
 sdfo_grid* buildContactAware(const double* verts, int64_t nv, const int32_t* tris, int64_t nt,

@@ -131,6 +131,8 @@ _sig("sdfo_grid_from_dense", [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _vp,
                               _P(_vp)])
 _sig("sdfo_query", [_vp, _i32, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp])
 _sig("sdfo_query_f64", [_vp, _i32, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp])
+_sig("sdfo_query_host_f64", [_vp, _i32, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                             _vp, _vp])
 _sig("sdfo_query_batched", [_i32, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp])
 _sig("sdfo_query_plan_create", [_i32, _vp, _vp, _P(_vp)])
 _sig("sdfo_query_plan_run", [_vp, _i32, _vp, _vp, _vp, _vp, _vp])

@@ -148,3 +148,35 @@ def test_batched_gears_plan_full_size_captured(O, sdfo_gpu):
     finally:
         for g in grids:
             g.close()
+
+
+def test_dense_512_grid_keeps_cell_records(O, sdfo_gpu):
+    """The record limit scales with device memory (an eighth of it): a dense
+    512^3 grid (17.2 GB of records) keeps its records, and 1e6 Tricubic points
+    on them match the oracle on the FP32-rounded field."""
+    import ctypes as C
+
+    from sdfo import _lib
+    v, t = scenes.cube_mesh()
+    print("[gpu] dense 512^3 cube grid with cell records", flush=True)
+    g = sdfo_gpu.SdfGrid.build_dense(v, t, target_resolution=512, max_resolution=512)
+    try:
+        n = np.array(g.spec.shape, np.int64)
+        assert tuple(n) == (512, 512, 512)
+        cells = int(np.prod(n - 1))
+        assert g.info.device_bytes > cells * 128 + int(np.prod(n)) * 100
+        pts = scenes.valid_region_points(g.spec, 1_000_000, seed=6)
+        gap, grad, valid = g.query(torch.from_numpy(pts).cuda(), "Tricubic")
+        N = int(np.prod(n))
+        phi, n3 = np.empty(N), np.empty((N, 3))
+        _lib.check(_lib.lib.sdfo_grid_download_dense(g.handle, phi.ctypes.data_as(C.c_void_p),
+                                                     n3.ctypes.data_as(C.c_void_p), None, None,
+                                                     None))
+        rq = O.query_dense(ograd(O, g.spec), dict(phi=f32(phi), n=f32(n3)), "Tricubic",
+                           pts.astype(np.float64))
+        vv = valid.cpu().numpy()
+        assert np.array_equal(vv, rq["valid"]) and vv.mean() > 0.99
+        assert np.abs(gap.cpu().numpy()[vv] - rq["gap"][vv]).max() <= 1e-6 * g.spec.diagonal
+        assert np.abs(grad.cpu().numpy()[vv] - rq["grad"][vv]).max() <= 2e-4
+    finally:
+        g.close()

@@ -1,7 +1,8 @@
 """The per-corner gather path of grids that keep no cell records.
 
-Grids whose packed 8-corner cell records would exceed the library's 4 GiB
-limit (dense grids past about 322^3) query by gathering each corner from the
+Grids whose packed 8-corner cell records would exceed the library's limit
+(an eighth of device memory, at least 4 GiB: dense grids past about 650^3 on
+a 288 GB device) query by gathering each corner from the
 FP32 mirror instead.  The debug switch builds small grids that way, and the
 FP32 queries (all five modes, dense and sparse), the batched plan and the
 contact kernels are checked against the oracle at the same bars as the
