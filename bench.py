@@ -104,6 +104,10 @@ def parse():
     ap.add_argument("--tuner", default="on", choices=["on", "off"],
                     help="autotuner: 'on' times the launch variants on the first call of each "
                          "size class (during the warm-up) and keeps the fastest")
+    ap.add_argument("--cell-records", default="on", choices=["on", "off"],
+                    help="'off' builds the query cube without its 128-byte cell records, so the "
+                         "FP32 queries gather the 8 corners from the mirror (the path of grids "
+                         "past the record limit; profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 256 MiB device copy first (scales HBM counters)")
     ap.add_argument("--workload", default="ring_on_cube",
@@ -439,7 +443,9 @@ def main():
 
     torus = scenes.ring_mesh(256, 64)
     cube = scenes.cube_mesh()
+    _lib.debug_skip_cell_records(args.cell_records == "off")
     cube_grid = sdfo.SdfGrid.build_dense(*cube, target_resolution=128, max_resolution=128)
+    _lib.debug_skip_cell_records(False)
     spec = cube_grid.spec
     nq = args.query_points
     reps = max(1, args.query_reps)
@@ -545,9 +551,18 @@ def main():
         busy()
         e[10].record(stream)
         if "bwd" in parts:
+            # a training step: the forward keeps its pairs in contact for the
+            # backward (what ContactFunction does)
+            _, saved = SC.contact_forward(cube_grid, X, w, poses_b, law_f, save=True)
+            SC.contact_backward(cube_grid, X, w, poses_b, g_out, law_f, saved=saved)
+        e[11].record(stream)
+        # (4b) the same with a backward that screens every pair itself
+        busy()
+        e[16].record(stream)
+        if "bwd" in parts:
             SC.contact_forward(cube_grid, X, w, poses_b, law_f)
             SC.contact_backward(cube_grid, X, w, poses_b, g_out, law_f)
-        e[11].record(stream)
+        e[17].record(stream)
         busy()
         e[12].record(stream)
         if "plain" in parts:
@@ -569,12 +584,14 @@ def main():
         rec["contact_ms"].append(e[6].elapsed_time(e[7]))
         rec["eval_ms"].append(e[8].elapsed_time(e[9]))
         rec["fwdbwd_ms"].append(e[10].elapsed_time(e[11]))
+        rec["fwdbwd_screen_ms"].append(e[16].elapsed_time(e[17]))
         rec["plain_ms"].append(e[12].elapsed_time(e[13]))
         rec["lin_ms"].append(e[14].elapsed_time(e[15]))
 
     def new_rec():
         return dict(build_s=[], query_ms=[], query_warm_ms=[], query64_ms=[], contact_ms=[],
-                    eval_ms=[], fwdbwd_ms=[], plain_ms=[], lin_ms=[], phases=[])
+                    eval_ms=[], fwdbwd_ms=[], fwdbwd_screen_ms=[], plain_ms=[], lin_ms=[],
+                    phases=[])
 
     warm = new_rec()
     for _ in range(args.warmup):
@@ -591,7 +608,7 @@ def main():
     med = lambda k: max(float(np.median(rec[k])), 1e-9) if rec[k] else 1e-9  # noqa: E731
     q_ms, qw_ms, q64_ms = med("query_ms"), med("query_warm_ms"), med("query64_ms")
     c_ms, e_ms, fb_ms = med("contact_ms"), med("eval_ms"), med("fwdbwd_ms")
-    pl_ms, lin_ms = med("plain_ms"), med("lin_ms")
+    pl_ms, lin_ms, fbs_ms = med("plain_ms"), med("lin_ms"), med("fwdbwd_screen_ms")
     build_s = med("build_s")
     phase = {k: float(np.median([p[k] for p in rec["phases"]])) for k in
              (rec["phases"][0] if rec["phases"] else {})}
@@ -621,6 +638,7 @@ def main():
                         "step); 1000 poses x 7680 pts contact; 256-pose fwd+bwd with friction",
             "query_points": nq, "grid": list(spec.shape), "poses": args.poses,
             "quadrature_points": len(X_np), "bwd_poses": args.bwd_poses,
+            "cell_records": args.cell_records == "on",
         },
         "build_mvoxel_per_s": round(n_vox_torus / build_s / 1e6, 3),
         "build_phases_ms": {k: round(v, 4) for k, v in phase.items()},
@@ -634,6 +652,7 @@ def main():
         "contact_linearized_evals_per_s": round(n_eval / (lin_ms * 1e-3), 1),
         "linearized_vs_direct": round(c_ms / lin_ms, 3),
         "contact_fwdbwd_evals_per_s": round(n_eval_b / (fb_ms * 1e-3), 1),
+        "contact_fwdbwd_screening_backward_evals_per_s": round(n_eval_b / (fbs_ms * 1e-3), 1),
         "plain_fwdbwd_evals_per_s": round(n_eval_b / (pl_ms * 1e-3), 1),
         "fwdbwd_vs_plain": round(pl_ms / fb_ms, 3),
         "roofline": _hbm_roofline(alg_bytes, q_ms * 1e-3,

@@ -444,6 +444,29 @@ int sdfo_contact_backward(const sdfo_grid* grid, const double* X, const double* 
                           double* g_twist, double* g_params, double* g_X, double* g_w,
                           sdfo_stream stream);
 
+/* A forward that keeps, for its backward, which (pose, point) pairs are in
+ * contact (no reference counterpart; what a training step's forward saves).
+ * sdfo_contact_saved_bytes: the size of that record for n points and
+ * n_poses poses (-1 for negative sizes).  sdfo_contact_forward_save is
+ * sdfo_contact_forward (same outputs, bit for bit) that also writes the
+ * record into saved (device, saved_bytes = sdfo_contact_saved_bytes(n,
+ * n_poses)).  sdfo_contact_backward_saved is sdfo_contact_backward reading
+ * the record instead of screening every pair again; it must get the same
+ * grid, X, w, poses and params as the forward that wrote the record (the
+ * backward re-evaluates the recorded pairs exactly, but pairs in contact
+ * under other inputs and absent from the record would be missed).  Its
+ * results equal sdfo_contact_backward's up to the order of the sums. */
+int64_t sdfo_contact_saved_bytes(int64_t n, int64_t n_poses);
+int sdfo_contact_forward_save(const sdfo_grid* grid, const double* X, const double* w, int64_t n,
+                              const double* poses, int64_t n_poses,
+                              const sdfo_contact_params* params, double* out, int32_t* active,
+                              void* saved, int64_t saved_bytes, sdfo_stream stream);
+int sdfo_contact_backward_saved(const sdfo_grid* grid, const double* X, const double* w, int64_t n,
+                                const double* poses, int64_t n_poses,
+                                const sdfo_contact_params* params, const double* g_out,
+                                const void* saved, int64_t saved_bytes, double* g_twist,
+                                double* g_params, double* g_X, double* g_w, sdfo_stream stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1254,6 +1254,49 @@ int sdfo_contact_backward(const sdfo_grid* grid, const double* X, const double* 
     });
 }
 
+int64_t sdfo_contact_saved_bytes(int64_t n, int64_t n_poses)
+{
+    if (n < 0 || n_poses < 0) return -1;
+    return contactSavedBytes(n, n_poses);
+}
+
+int sdfo_contact_forward_save(const sdfo_grid* grid, const double* X, const double* w, int64_t n,
+                              const double* poses, int64_t n_poses,
+                              const sdfo_contact_params* params, double* out, int32_t* active,
+                              void* saved, int64_t saved_bytes, sdfo_stream stream)
+{
+    return guarded([&] {
+        checkContact(grid, X, w, n, poses, n_poses, params);
+        if (n_poses > 0 && !out) throw Error(SDFO_ERR_INVALID_ARGUMENT, "out is NULL");
+        if (saved_bytes != contactSavedBytes(n, n_poses))
+            throw Error(SDFO_ERR_INVALID_ARGUMENT,
+                        "saved_bytes must be sdfo_contact_saved_bytes(n, n_poses)");
+        if (n_poses > 0 && !saved) throw Error(SDFO_ERR_INVALID_ARGUMENT, "saved is NULL");
+        DeviceGuard dg(grid->device);
+        launchContactForward(*grid, X, w, n, poses, n_poses, *params, out, active,
+                             hipStream_t(stream), nullptr, saved);
+    });
+}
+
+int sdfo_contact_backward_saved(const sdfo_grid* grid, const double* X, const double* w, int64_t n,
+                                const double* poses, int64_t n_poses,
+                                const sdfo_contact_params* params, const double* g_out,
+                                const void* saved, int64_t saved_bytes, double* g_twist,
+                                double* g_params, double* g_X, double* g_w, sdfo_stream stream)
+{
+    return guarded([&] {
+        checkContact(grid, X, w, n, poses, n_poses, params);
+        if (n_poses > 0 && !g_out) throw Error(SDFO_ERR_INVALID_ARGUMENT, "g_out is NULL");
+        if (saved_bytes != contactSavedBytes(n, n_poses))
+            throw Error(SDFO_ERR_INVALID_ARGUMENT,
+                        "saved_bytes must be sdfo_contact_saved_bytes(n, n_poses)");
+        if (n_poses > 0 && !saved) throw Error(SDFO_ERR_INVALID_ARGUMENT, "saved is NULL");
+        DeviceGuard dg(grid->device);
+        launchContactBackward(*grid, X, w, n, poses, n_poses, *params, g_out, g_twist, g_params,
+                              g_X, g_w, hipStream_t(stream), saved);
+    });
+}
+
 } // extern "C"
 
 // Contact-aware build, batched builds and queries, autotuner.

@@ -2,9 +2,12 @@
 // reduction (contacts_sdf.py:329-385) and its hand-written backward.
 //
 // All arithmetic after the field gather is FP64 and every sum has a fixed
-// order (per-lane strided partials, LDS tree, then chunk partials summed in
-// chunk order), so results are bit-reproducible run to run.  The backward
-// recomputes the forward terms per point instead of storing them.
+// order (forward: per-lane strided partials, LDS tree, then chunk partials in
+// chunk order; backward: a fixed butterfly per piece of 64 listed pairs, then
+// the pieces in list order), so results are bit-reproducible run to run.  The
+// backward runs only on the (pose, point) pairs in contact, listed by a
+// screen or by the forward that preceded it, and recomputes their forward
+// terms instead of storing them.
 #include <hip/hip_runtime.h>
 
 #include <mutex>
@@ -271,17 +274,48 @@ cullMaskKernel(CullArgs cu, int64_t n, uint8_t* __restrict__ mask)
     mask[int64_t(blockIdx.y) * n + i] = leafOverlaps(cu.leafBox + 6 * int64_t(cu.leafOf[i]), q, cu.margin) ? 1 : 0;
 }
 
+// The (pose, point) pairs in contact, by (chunk, pose) slot: slot
+// pose * chunks + chunk lists its pairs' point indices at
+// list[slot * chunkLen ...] and their number in count[slot]; act holds one
+// flag per (pose, point) (pose-major, n per pose).  Written by the backward's
+// screen, or by a forward that keeps them for its backward.
+struct PairList
+{
+    int32_t* list = nullptr;
+    int32_t* count = nullptr;
+    uint8_t* act = nullptr;
+};
+
+// Ordered compaction of one pass of a block over kBlock points: the point
+// indices of the hit lanes, in (wave, lane) order, appended at out[base...];
+// returns the new base.  Every thread of the block calls it.
+__device__ __forceinline__ int appendHits(bool hit, int64_t i, int32_t* __restrict__ out, int base,
+                                          int* waveCount)
+{
+    const int wave = int(threadIdx.x) >> 6, lane = int(threadIdx.x) & 63;
+    const unsigned long long m = __ballot(hit);
+    if (lane == 0) waveCount[wave] = __popcll(m);
+    __syncthreads();
+    int off = base;
+    for (int q = 0; q < wave; ++q) off += waveCount[q];
+    if (hit) out[off + __popcll(m & ((1ull << lane) - 1ull))] = int32_t(i);
+    for (int q = 0; q < kBlock / 64; ++q) base += waveCount[q];
+    __syncthreads();
+    return base;
+}
+
 // grid = (chunks, poses).  partial[(pose*chunks + chunk)*7 + k]: F, tau, active.
 // With Culled, points outside the pose's broad-phase leaves are skipped
 // unless no leaf overlaps (then every point takes part, contacts_sdf.py:270-271).
-template <bool Cached, bool Culled>
-__global__ void __launch_bounds__(kBlock)
-contactForwardKernel(QGrid g, Law L, const double* __restrict__ X, const double* __restrict__ w,
-                     int64_t n, const double* __restrict__ poses, int64_t chunkLen,
-                     double* __restrict__ partial, LinCache lc, CullArgs cu,
-                     const uint8_t* __restrict__ anyLeaf)
+// With Save, the pose's pairs in contact are recorded in save too.
+template <bool Cached, bool Culled, bool Save>
+__device__ __forceinline__ void forwardBlock(const QGrid& g, const Law& L, const double* __restrict__ X,
+                                             const double* __restrict__ w, int64_t n,
+                                             const double* __restrict__ poses, int64_t chunkLen,
+                                             double* __restrict__ partial, const LinCache& lc,
+                                             const CullArgs& cu, const uint8_t* __restrict__ anyLeaf,
+                                             const PairList& save, double* sh, int* waveCount)
 {
-    __shared__ double sh[7 * kBlock];
     const int64_t pose = blockIdx.y;
     const int chunk = blockIdx.x;
     const Pose P = loadPose(poses + 30 * pose);
@@ -309,14 +343,43 @@ contactForwardKernel(QGrid g, Law L, const double* __restrict__ X, const double*
     // one point per step: at 136 VGPRs (three waves per SIMD) this beat
     // requesting two points' records before interpolating either (218 VGPRs,
     // two waves), since a pose's points hit L2-resident records
-    for (int64_t i = beg + threadIdx.x; i < end; i += kBlock)
+    if (Save)
     {
-        if (Culled && cull && !leafOverlaps(cu.leafBox + 6 * int64_t(cu.leafOf[i]), q, cu.margin))
-            continue;
-        const V x = V{X[3 * i], X[3 * i + 1], X[3 * i + 2]};
-        const Terms t = pointTerms<false, Cached>(g, P, x, L, &lc, i);
-        if (!t.active) continue;
-        add(t, i);
+        // the same points in the same order per thread, and the active
+        // pairs recorded for the backward
+        int base = 0;
+        int32_t* out = save.list + (pose * gridDim.x + chunk) * chunkLen;
+        for (int64_t i0 = beg; i0 < end; i0 += kBlock)
+        {
+            const int64_t i = i0 + threadIdx.x;
+            bool hit = false;
+            if (i < end)
+            {
+                const V x = V{X[3 * i], X[3 * i + 1], X[3 * i + 2]};
+                const Terms t = pointTerms<false, Cached>(g, P, x, L, &lc, i);
+                if (t.active)
+                {
+                    add(t, i);
+                    hit = true;
+                }
+                save.act[pose * n + i] = hit ? 1 : 0;
+            }
+            base = appendHits(hit, i, out, base, waveCount);
+        }
+        if (threadIdx.x == 0) save.count[pose * gridDim.x + chunk] = base;
+    }
+    else
+    {
+        for (int64_t i = beg + threadIdx.x; i < end; i += kBlock)
+        {
+            if (Culled && cull &&
+                !leafOverlaps(cu.leafBox + 6 * int64_t(cu.leafOf[i]), q, cu.margin))
+                continue;
+            const V x = V{X[3 * i], X[3 * i + 1], X[3 * i + 2]};
+            const Terms t = pointTerms<false, Cached>(g, P, x, L, &lc, i);
+            if (!t.active) continue;
+            add(t, i);
+        }
     }
     blockReduce<7>(acc, sh);
     if (threadIdx.x == 0)
@@ -325,6 +388,32 @@ contactForwardKernel(QGrid g, Law L, const double* __restrict__ X, const double*
 #pragma unroll
         for (int k = 0; k < 7; ++k) o[k] = acc[k];
     }
+}
+
+// The forward kernels: four waves per SIMD (128 registers) for the plain,
+// linearised and culled passes (+10% over three waves for the 1000-pose
+// forward); the recording pass keeps three, as at 128 registers it spills.
+template <bool Cached, bool Culled>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4)))
+contactForwardKernel(QGrid g, Law L, const double* __restrict__ X, const double* __restrict__ w,
+                     int64_t n, const double* __restrict__ poses, int64_t chunkLen,
+                     double* __restrict__ partial, LinCache lc, CullArgs cu,
+                     const uint8_t* __restrict__ anyLeaf)
+{
+    __shared__ double sh[7 * kBlock];
+    forwardBlock<Cached, Culled, false>(g, L, X, w, n, poses, chunkLen, partial, lc, cu, anyLeaf,
+                                        PairList{}, sh, nullptr);
+}
+
+__global__ void __launch_bounds__(kBlock)
+contactForwardSaveKernel(QGrid g, Law L, const double* __restrict__ X, const double* __restrict__ w,
+                         int64_t n, const double* __restrict__ poses, int64_t chunkLen,
+                         double* __restrict__ partial, PairList save)
+{
+    __shared__ double sh[7 * kBlock];
+    __shared__ int waveCount[kBlock / 64];
+    forwardBlock<false, false, true>(g, L, X, w, n, poses, chunkLen, partial, LinCache{},
+                                     CullArgs{}, nullptr, save, sh, waveCount);
 }
 
 __global__ void contactForwardFinalize(int64_t nposes, int chunks, const double* __restrict__ partial,
@@ -394,64 +483,194 @@ __device__ __forceinline__ Back pointBackward(const Terms& t, const Law& L, V a,
     return b;
 }
 
-// Pose-major backward: per pose d/d(omega, dt) and per-pose (kn, c, mu) terms
-// in partial[(pose*chunks + chunk)*9 + k]; when pointPartial is given, every
-// (pose, point) pair also writes its d/dX (3) and d/dw (1) contribution to
-// pointPartial[(pose - poseBase) * n + point] (zero for inactive pairs), to be
-// summed over poses by pointReduceKernel.
+// Backward, first pass: which (pose, point) pairs can be in contact.  Only a
+// small share of the pairs is (about 9% for a ring resting on a cube), and
+// in point order they are scattered over most waves, so the backward proper
+// runs on compacted lists instead of every pair.  A pair is listed when its
+// field sample exists and its gap, computed with the value part of the same
+// interpolation after the same transform, narrow filter and cell, is below
+// kScreenGap (or for every sampled pair when the law's order is 0, where
+// p = kn for any gap); the second pass re-evaluates each listed pair exactly
+// and drops the ones whose law is not active, so the screen's tolerance only
+// has to admit every active pair.  Block (chunk, pose) writes its slot of
+// the group's PairList (slots and flags counted from pose poseBase).
+constexpr double kScreenGap = 1e-9;
+
+template <bool Trilinear>
 __global__ void __launch_bounds__(kBlock)
-contactBackwardPoseKernel(QGrid g, Law L, const double* __restrict__ X,
-                          const double* __restrict__ w, int64_t n,
-                          const double* __restrict__ poses, const double* __restrict__ gOut,
-                          int64_t chunkLen, int64_t poseBase, double* __restrict__ partial,
-                          double4* __restrict__ pointPartial)
+contactScreenKernel(QGrid g, Law L, const double* __restrict__ X, int64_t n,
+                    const double* __restrict__ poses, int64_t chunkLen, int64_t poseBase,
+                    PairList pl)
 {
-    __shared__ double sh[9 * kBlock];
-    const int64_t pose = poseBase + blockIdx.y;
-    const int chunk = blockIdx.x;
-    const Pose P = loadPose(poses + 30 * pose);
-    const V gF = V{gOut[6 * pose], gOut[6 * pose + 1], gOut[6 * pose + 2]};
-    const V gT = V{gOut[6 * pose + 3], gOut[6 * pose + 4], gOut[6 * pose + 5]};
-    const V Om = V{P.Om[0], P.Om[1], P.Om[2]};
-    const V tv = V{P.t[0], P.t[1], P.t[2]};
-    const int64_t beg = int64_t(chunk) * chunkLen;
-    const int64_t end = beg + chunkLen < n ? beg + chunkLen : n;
-    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    double4* pp = pointPartial ? pointPartial + int64_t(blockIdx.y) * n : nullptr;
-    for (int64_t i = beg + threadIdx.x; i < end; i += kBlock)
-    {
-        const V x = V{X[3 * i], X[3 * i + 1], X[3 * i + 2]};
-        const Terms t = pointTerms<true>(g, P, x, L);
-        if (!t.active)
-        {
-            if (pp) pp[i] = make_double4(0.0, 0.0, 0.0, 0.0);
-            continue;
-        }
-        const double wi = w[i];
-        const V a = vmul(wi, vadd(gF, vcross(gT, t.off)));
-        const Back b = pointBackward(t, L, a, gT, wi, Om);
-        const V gw = vadd(vcross(vsub(t.Y, tv), b.gY), vcross(t.off, b.gOff));
-        acc[0] += gw.x;
-        acc[1] += gw.y;
-        acc[2] += gw.z;
-        acc[3] += b.gY.x;
-        acc[4] += b.gY.y;
-        acc[5] += b.gY.z;
-        acc[6] += b.skn;
-        acc[7] += b.sc;
-        acc[8] += b.smu;
-        if (pp)
-        {
-            const V gx = vadd(matTvec(P.R, b.gY), matTvec(P.M, b.gOff));
-            pp[i] = make_double4(gx.x, gx.y, gx.z, vdot(gF, t.fq) + vdot(gT, vcross(t.off, t.fq)));
-        }
-    }
-    blockReduce<9>(acc, sh);
-    if (threadIdx.x == 0)
-    {
-        double* o = partial + (pose * gridDim.x + chunk) * 9;
+    __shared__ int waveCount[kBlock / 64];
+    const int64_t lp = blockIdx.y;
+    const double* pp = poses + 30 * (poseBase + lp);
+    double R[9], tv[3];
 #pragma unroll
-        for (int k = 0; k < 9; ++k) o[k] = acc[k];
+    for (int k = 0; k < 9; ++k) R[k] = pp[k];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) tv[k] = pp[9 + k];
+    const int64_t slot = lp * gridDim.x + blockIdx.x;
+    const int64_t beg = int64_t(blockIdx.x) * chunkLen;
+    const int64_t end = beg + chunkLen < n ? beg + chunkLen : n;
+    int32_t* out = pl.list + slot * chunkLen;
+    int base = 0;
+    for (int64_t i0 = beg; i0 < end; i0 += kBlock)
+    {
+        const int64_t i = i0 + threadIdx.x;
+        bool hit = false;
+        if (i < end)
+        {
+            const V x = V{X[3 * i], X[3 * i + 1], X[3 * i + 2]};
+            const V Y = vadd(matvec(R, x), V{tv[0], tv[1], tv[2]});
+            if (!(L.margin > 0.0 && !(fabs(Y.z) < L.margin)))
+            {
+                const Cell c = dev::locateCell(g, Y.x, Y.y, Y.z);
+                if (c.valid)
+                {
+                    double gap = 0.0;
+                    bool ok;
+                    if (Trilinear)
+                    {
+                        Interp<double> it;
+                        ok = dev::trilinear<double, false>(g, c, it);
+                        gap = it.gap;
+                    }
+                    else
+                    {
+                        float4 f[8];
+                        ok = dev::cellCorners(g, c, f);
+                        if (ok) gap = dev::tricubicValueCorners<double>(g, c, f);
+                    }
+                    hit = ok && (L.order <= 0 ? L.kn > 1e-30 : gap < kScreenGap);
+                }
+            }
+            if (pl.act) pl.act[lp * n + i] = hit ? 1 : 0;
+        }
+        base = appendHits(hit, i, out, base, waveCount);
+    }
+    if (threadIdx.x == 0) pl.count[slot] = base;
+}
+
+// Backward, second pass, in pieces of kPiece listed pairs: piece p of slot
+// s covers entries [p kPiece, (p+1) kPiece) of the slot's list, and item code
+// s * maxPieces + p names it.  pieceTableKernel (one block) lays the items of
+// all slots out in slot order, so the pairs in contact are spread evenly over
+// the waves of the next kernel whatever their distribution over poses, and
+// the items of pose pl of the group are [poseStart[pl], poseStart[pl + 1]).
+constexpr int kPiece = 64;
+constexpr int kTableBlock = 1024;
+
+__global__ void __launch_bounds__(kTableBlock)
+pieceTableKernel(int64_t slots, int chunks, int maxPieces, const int32_t* __restrict__ count,
+                 int32_t* __restrict__ items, int32_t* __restrict__ poseStart,
+                 int32_t* __restrict__ total)
+{
+    __shared__ int sh[kTableBlock];
+    const int tid = int(threadIdx.x);
+    const int64_t per = (slots + kTableBlock - 1) / kTableBlock;
+    const int64_t s0 = tid * per < slots ? tid * per : slots;
+    const int64_t s1 = s0 + per < slots ? s0 + per : slots;
+    int mine = 0;
+    for (int64_t sl = s0; sl < s1; ++sl) mine += (count[sl] + kPiece - 1) / kPiece;
+    sh[tid] = mine;
+    __syncthreads();
+    // inclusive scan of the per-thread piece counts (Hillis-Steele)
+    for (int d = 1; d < kTableBlock; d <<= 1)
+    {
+        const int v = tid >= d ? sh[tid - d] : 0;
+        __syncthreads();
+        sh[tid] += v;
+        __syncthreads();
+    }
+    int off = sh[tid] - mine;
+    for (int64_t sl = s0; sl < s1; ++sl)
+    {
+        if (sl % chunks == 0) poseStart[sl / chunks] = off;
+        const int np = (count[sl] + kPiece - 1) / kPiece;
+        for (int p = 0; p < np; ++p) items[off++] = int32_t(sl * maxPieces + p);
+    }
+    if (tid == kTableBlock - 1)
+    {
+        *total = sh[tid];
+        poseStart[slots / chunks] = sh[tid];
+    }
+}
+
+// One wave per item, the waves looping over the items: lane l takes entry
+// p kPiece + l of the piece, with the same per-pair arithmetic as the
+// forward's terms and the derivation stated next to the CPU restatement.  A
+// listed pair whose law is inactive writes a zero point partial.  The
+// piece's nine sums are added by a fixed butterfly into pieceSum[k * 9 + q]
+// for item k, so results are bit-reproducible run to run.
+__global__ void __launch_bounds__(64)
+contactBackwardPieceKernel(QGrid g, Law L, const double* __restrict__ X,
+                           const double* __restrict__ w, int64_t n,
+                           const double* __restrict__ poses, const double* __restrict__ gOut,
+                           int chunks, int64_t chunkLen, int maxPieces, int64_t poseBase,
+                           const int32_t* __restrict__ list, const int32_t* __restrict__ count,
+                           const int32_t* __restrict__ items, const int32_t* __restrict__ total,
+                           double* __restrict__ pieceSum, double4* __restrict__ pointPartial)
+{
+    const int nItems = *total;
+    for (int k = blockIdx.x; k < nItems; k += gridDim.x)
+    {
+        const int code = items[k];
+        const int64_t slot = code / maxPieces;
+        const int piece = code - int(slot) * maxPieces;
+        const int64_t pl = slot / chunks;
+        const int64_t pose = poseBase + pl;
+        const int j = piece * kPiece + int(threadIdx.x);
+        double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        if (j < count[slot])
+        {
+            const int64_t i = list[slot * chunkLen + j];
+            const Pose P = loadPose(poses + 30 * pose);
+            const V gF = V{gOut[6 * pose], gOut[6 * pose + 1], gOut[6 * pose + 2]};
+            const V gT = V{gOut[6 * pose + 3], gOut[6 * pose + 4], gOut[6 * pose + 5]};
+            double4* pp = pointPartial ? pointPartial + pl * n : nullptr;
+            const V x = V{X[3 * i], X[3 * i + 1], X[3 * i + 2]};
+            const Terms t = pointTerms<true>(g, P, x, L);
+            if (!t.active)
+            {
+                if (pp) pp[i] = make_double4(0.0, 0.0, 0.0, 0.0);
+            }
+            else
+            {
+                const double wi = w[i];
+                const V a = vmul(wi, vadd(gF, vcross(gT, t.off)));
+                const Back b = pointBackward(t, L, a, gT, wi, V{P.Om[0], P.Om[1], P.Om[2]});
+                const V gw = vadd(vcross(vsub(t.Y, V{P.t[0], P.t[1], P.t[2]}), b.gY),
+                                  vcross(t.off, b.gOff));
+                acc[0] = gw.x;
+                acc[1] = gw.y;
+                acc[2] = gw.z;
+                acc[3] = b.gY.x;
+                acc[4] = b.gY.y;
+                acc[5] = b.gY.z;
+                acc[6] = b.skn;
+                acc[7] = b.sc;
+                acc[8] = b.smu;
+                if (pp)
+                {
+                    const V gx = vadd(matTvec(P.R, b.gY), matTvec(P.M, b.gOff));
+                    pp[i] = make_double4(gx.x, gx.y, gx.z,
+                                         vdot(gF, t.fq) + vdot(gT, vcross(t.off, t.fq)));
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 9; ++q)
+        {
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) acc[q] += __shfl_xor(acc[q], o);
+        }
+        if (threadIdx.x == 0)
+        {
+            double* o = pieceSum + int64_t(k) * 9;
+#pragma unroll
+            for (int q = 0; q < 9; ++q) o[q] = acc[q];
+        }
     }
 }
 
@@ -459,33 +678,50 @@ contactBackwardPoseKernel(QGrid g, Law L, const double* __restrict__ X,
 // order: a block holds kReducePoints points x kReduceSlices pose slices; each
 // thread sums its slice's poses for its point in pose order, then the slices
 // are added in slice order through LDS and the result is added to (or, for
-// the first pose group, stored in) gX / gW.  Consecutive lanes read
-// consecutive points of one pose, so every load is coalesced.
+// the first pose group, stored in) gX / gW.  Only the pairs the screen
+// listed (flag act) have a partial.  Consecutive lanes read consecutive
+// points of one pose, so every load is coalesced.
 constexpr int kReducePoints = 16;
 constexpr int kReduceSlices = 16;
+constexpr int kReduceBatch = 8;
 
-__global__ void __launch_bounds__(kReducePoints * kReduceSlices)
-pointReduceKernel(int64_t n, int poses, const double4* __restrict__ pointPartial, int accumulate,
-                  double* __restrict__ gX, double* __restrict__ gW)
+__device__ __forceinline__ void pointReduceBlock(int64_t blk, int64_t n, int poses,
+                                                 const double4* __restrict__ pointPartial,
+                                                 const uint8_t* __restrict__ act, int accumulate,
+                                                 double* __restrict__ gX, double* __restrict__ gW,
+                                                 double4 (&sh)[kReduceSlices][kReducePoints])
 {
-    __shared__ double4 sh[kReduceSlices][kReducePoints];
     const int lane = int(threadIdx.x) % kReducePoints;
     const int slice = int(threadIdx.x) / kReducePoints;
-    const int64_t i = int64_t(blockIdx.x) * kReducePoints + lane;
+    const int64_t i = blk * kReducePoints + lane;
     const int per = (poses + kReduceSlices - 1) / kReduceSlices;
     const int p0 = slice * per;
     const int p1 = p0 + per < poses ? p0 + per : poses;
     double sx = 0.0, sy = 0.0, sz = 0.0, sw = 0.0;
     if (i < n)
     {
-#pragma unroll 4
-        for (int p = p0; p < p1; ++p)
+        // kReduceBatch poses at a time: every flag is loaded, then every
+        // listed partial (pairs the screen did not list have none), then
+        // they are added in pose order
+        for (int pb = p0; pb < p1; pb += kReduceBatch)
         {
-            const double4 v = pointPartial[int64_t(p) * n + i];
-            sx += v.x;
-            sy += v.y;
-            sz += v.z;
-            sw += v.w;
+            bool on[kReduceBatch];
+#pragma unroll
+            for (int q = 0; q < kReduceBatch; ++q)
+                on[q] = pb + q < p1 && act[int64_t(pb + q) * n + i] != 0;
+            double4 v[kReduceBatch];
+#pragma unroll
+            for (int q = 0; q < kReduceBatch; ++q)
+                v[q] = on[q] ? pointPartial[int64_t(pb + q) * n + i] : make_double4(0.0, 0.0, 0.0, 0.0);
+#pragma unroll
+            for (int q = 0; q < kReduceBatch; ++q)
+            {
+                if (!on[q]) continue;
+                sx += v[q].x;
+                sy += v[q].y;
+                sz += v[q].z;
+                sw += v[q].w;
+            }
         }
     }
     sh[slice][lane] = make_double4(sx, sy, sz, sw);
@@ -516,27 +752,95 @@ pointReduceKernel(int64_t n, int poses, const double4* __restrict__ pointPartial
     if (gW) gW[i] = accumulate ? gW[i] + sw : sw;
 }
 
-__global__ void contactBackwardPoseFinalize(int64_t nposes, int chunks,
-                                            const double* __restrict__ partial,
-                                            double* __restrict__ gTwist, double* __restrict__ perPose3)
+// One wave finalises pose pl of a group: its items' nine sums, read seven
+// items (63 consecutive doubles) at a time and handed to lanes 0-8 by
+// shuffles, are added in item order into d/d(omega, dt) (gTwist) and the
+// (kn, c, mu) terms (perPose3).  Every lane of the wave calls it.
+__device__ __forceinline__ void poseFinalizeWave(int64_t pl, int64_t poseBase,
+                                                 const int32_t* __restrict__ poseStart,
+                                                 const double* __restrict__ pieceSum,
+                                                 double* __restrict__ gTwist,
+                                                 double* __restrict__ perPose3)
 {
-    const int64_t pose = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (pose >= nposes) return;
-    double s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int c = 0; c < chunks; ++c)
+    const int lane = int(threadIdx.x) & 63;
+    const int k0 = poseStart[pl], k1 = poseStart[pl + 1];
+    const int q = lane < 9 ? lane : 0;
+    double s = 0.0;
+    for (int kb = k0; kb < k1; kb += 7)
     {
-        const double* p = partial + (pose * chunks + c) * 9;
+        const double v = lane < 63 && kb + lane / 9 < k1 ? pieceSum[int64_t(kb) * 9 + lane] : 0.0;
+        const int m = k1 - kb < 7 ? k1 - kb : 7;
 #pragma unroll
-        for (int k = 0; k < 9; ++k) s[k] += p[k];
+        for (int j = 0; j < 7; ++j)
+        {
+            const double x = __shfl(v, j * 9 + q);
+            if (lane < 9 && j < m) s += x;
+        }
     }
-    if (gTwist)
+    const int64_t pose = poseBase + pl;
+    if (lane < 6)
     {
-#pragma unroll
-        for (int k = 0; k < 6; ++k) gTwist[6 * pose + k] = s[k];
+        if (gTwist) gTwist[6 * pose + lane] = s;
     }
-    perPose3[3 * pose + 0] = s[6];
-    perPose3[3 * pose + 1] = s[7];
-    perPose3[3 * pose + 2] = s[8];
+    else if (lane < 9 && perPose3)
+        perPose3[3 * pose + lane - 6] = s;
+}
+
+// The (kn, c, mu) gradient of a single-group call straight from the items:
+// thread t adds items t, t + blockDim, ... in order, then the block's fixed
+// LDS tree.
+__device__ __forceinline__ void paramsFromItems(const int32_t* __restrict__ total,
+                                                const double* __restrict__ pieceSum,
+                                                double* __restrict__ gParams, double* sh)
+{
+    double acc[3] = {0, 0, 0};
+    const int nItems = *total;
+    for (int k = threadIdx.x; k < nItems; k += kBlock)
+    {
+        acc[0] += pieceSum[int64_t(k) * 9 + 6];
+        acc[1] += pieceSum[int64_t(k) * 9 + 7];
+        acc[2] += pieceSum[int64_t(k) * 9 + 8];
+    }
+    blockReduce<3>(acc, sh);
+    if (threadIdx.x == 0)
+    {
+        gParams[0] = acc[0];
+        gParams[1] = acc[1];
+        gParams[2] = acc[2];
+    }
+}
+
+// The backward's tail in one launch, all on one pose group: blocks
+// [0, nReduce) reduce the point gradients, the next ceil(cnt / 4) blocks
+// finalise four poses each (one per wave), and with gParams the last block
+// sums the parameter terms.
+static_assert(kReducePoints * kReduceSlices == kBlock, "tail blocks share one size");
+
+__global__ void __launch_bounds__(kBlock)
+backwardTailKernel(int64_t nReduce, int64_t n, int cnt, const double4* __restrict__ pointPartial,
+                   const uint8_t* __restrict__ act, int accumulate, double* __restrict__ gX,
+                   double* __restrict__ gW, int64_t poseBase, const int32_t* __restrict__ poseStart,
+                   const double* __restrict__ pieceSum, const int32_t* __restrict__ total,
+                   double* __restrict__ gTwist, double* __restrict__ perPose3,
+                   double* __restrict__ gParams)
+{
+    __shared__ double4 shR[kReduceSlices][kReducePoints];
+    __shared__ double shS[3 * kBlock];
+    const int64_t b = blockIdx.x;
+    if (b < nReduce)
+    {
+        pointReduceBlock(b, n, cnt, pointPartial, act, accumulate, gX, gW, shR);
+        return;
+    }
+    const int64_t poseBlocks = (cnt + kBlock / 64 - 1) / (kBlock / 64);
+    if (b < nReduce + poseBlocks)
+    {
+        const int wave = int(threadIdx.x) >> 6;
+        const int64_t pl = (b - nReduce) * (kBlock / 64) + wave;
+        if (pl < cnt) poseFinalizeWave(pl, poseBase, poseStart, pieceSum, gTwist, perPose3);
+        return;
+    }
+    paramsFromItems(total, pieceSum, gParams, shS);
 }
 
 // Sum of the per-pose parameter terms in pose order (one block).
@@ -769,45 +1073,6 @@ __global__ void contactEvaluateFinalize(int64_t npairs, int chunks, const double
     if (active) active[b] = on ? int32_t(s[6]) : 0;
 }
 
-// contactBackwardPoseFinalize and paramReduceKernel in one block for up to
-// kFusedFinalizePoses poses: thread t finalises poses t, t + kBlock, ... (chunk
-// sums in chunk order) and keeps their parameter terms, which the block then
-// reduces exactly as paramReduceKernel does, so the bits are the same.
-constexpr int64_t kFusedFinalizePoses = 4096;
-
-__global__ void __launch_bounds__(kBlock)
-contactBackwardFinalizeParams(int64_t nposes, int chunks, const double* __restrict__ partial,
-                              double* __restrict__ gTwist, double* __restrict__ gParams)
-{
-    __shared__ double sh[3 * kBlock];
-    double acc[3] = {0, 0, 0};
-    for (int64_t pose = threadIdx.x; pose < nposes; pose += kBlock)
-    {
-        double sum[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        for (int c = 0; c < chunks; ++c)
-        {
-            const double* p = partial + (pose * chunks + c) * 9;
-#pragma unroll
-            for (int k = 0; k < 9; ++k) sum[k] += p[k];
-        }
-        if (gTwist)
-        {
-#pragma unroll
-            for (int k = 0; k < 6; ++k) gTwist[6 * pose + k] = sum[k];
-        }
-        acc[0] += sum[6];
-        acc[1] += sum[7];
-        acc[2] += sum[8];
-    }
-    blockReduce<3>(acc, sh);
-    if (threadIdx.x == 0)
-    {
-        gParams[0] = acc[0];
-        gParams[1] = acc[1];
-        gParams[2] = acc[2];
-    }
-}
-
 Law makeLaw(const sdfo_contact_params& p)
 {
     return Law{p.kn, p.damping, p.mu, p.regularizer, p.narrow_margin, p.order, p.mode,
@@ -845,29 +1110,63 @@ struct Scratch
 
 namespace {
 
-// Forward launch shared by the plain, linearised and culled entry points.
+// Layout of the pair record a forward keeps for its backward: counts
+// (nb x chunks int32), lists (nb x chunks x chunkLen int32), flags (nb x n
+// bytes), each 256-byte aligned, with the chunking both passes use.
+struct SavedLayout
+{
+    int chunks;
+    int64_t chunkLen;
+    size_t oCount, oList, oAct, bytes;
+};
+
+SavedLayout savedLayout(int64_t n, int64_t nb)
+{
+    SavedLayout o{};
+    o.chunks = n > 0 ? chunksFor(n, nb) : 1;
+    o.chunkLen = n > 0 ? (n + o.chunks - 1) / o.chunks : 0;
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    const size_t slots = size_t(nb) * size_t(o.chunks);
+    o.oCount = 0;
+    o.oList = al(sizeof(int32_t) * slots);
+    o.oAct = o.oList + al(sizeof(int32_t) * slots * size_t(o.chunkLen));
+    o.bytes = o.oAct + al(size_t(nb) * size_t(n > 0 ? n : 0));
+    return o;
+}
+
+PairList savedPairs(void* saved, const SavedLayout& o)
+{
+    char* b = static_cast<char*>(saved);
+    return PairList{reinterpret_cast<int32_t*>(b + o.oList), reinterpret_cast<int32_t*>(b + o.oCount),
+                    reinterpret_cast<uint8_t*>(b + o.oAct)};
+}
+
+// Forward launch shared by the plain, linearised and culled entry points;
+// with saved (plain forward only), the pair record is written too.
 template <bool Cached>
 void forwardLaunch(const QGrid& q, const Law& L, const double* X, const double* w, int64_t n,
                    const double* poses, int64_t nb, const LinCache& lc, const CullArgs* cull,
-                   double* out, int32_t* active, hipStream_t s)
+                   double* out, int32_t* active, hipStream_t s, void* saved = nullptr)
 {
     const int chunks = n > 0 ? chunksFor(n, nb) : 1;
     const int64_t chunkLen = n > 0 ? (n + chunks - 1) / chunks : 0;
     Scratch part(sizeof(double) * 7 * size_t(nb) * chunks + (cull ? size_t(nb) : 0), s);
     double* partial = static_cast<double*>(part.p);
     uint8_t* anyLeaf = reinterpret_cast<uint8_t*>(partial + 7 * size_t(nb) * chunks);
+    const dim3 grid(static_cast<unsigned>(chunks), static_cast<unsigned>(nb));
     if (cull)
     {
         hipLaunchKernelGGL(cullAnyKernel, dim3(unsigned(nb)), dim3(kBlock), 0, s, *cull, anyLeaf);
         SDFO_HIP(hipGetLastError());
-        hipLaunchKernelGGL((contactForwardKernel<Cached, true>), dim3(chunks, unsigned(nb)),
-                           dim3(kBlock), 0, s, q, L, X, w, n, poses, chunkLen, partial, lc, *cull,
-                           anyLeaf);
+        hipLaunchKernelGGL((contactForwardKernel<Cached, true>), grid, dim3(kBlock), 0, s, q, L, X,
+                           w, n, poses, chunkLen, partial, lc, *cull, anyLeaf);
     }
+    else if (saved && !Cached)
+        hipLaunchKernelGGL(contactForwardSaveKernel, grid, dim3(kBlock), 0, s, q, L, X, w, n, poses,
+                           chunkLen, partial, savedPairs(saved, savedLayout(n, nb)));
     else
-        hipLaunchKernelGGL((contactForwardKernel<Cached, false>), dim3(chunks, unsigned(nb)),
-                           dim3(kBlock), 0, s, q, L, X, w, n, poses, chunkLen, partial, lc,
-                           CullArgs{}, nullptr);
+        hipLaunchKernelGGL((contactForwardKernel<Cached, false>), grid, dim3(kBlock), 0, s, q, L,
+                           X, w, n, poses, chunkLen, partial, lc, CullArgs{}, nullptr);
     SDFO_HIP(hipGetLastError());
     hipLaunchKernelGGL(contactForwardFinalize, dim3(unsigned((nb + 255) / 256)), dim3(256), 0, s,
                        nb, chunks, static_cast<const double*>(partial), out, active);
@@ -876,13 +1175,19 @@ void forwardLaunch(const QGrid& q, const Law& L, const double* X, const double* 
 
 } // namespace
 
+int64_t contactSavedBytes(int64_t n, int64_t nb)
+{
+    return nb > 0 ? int64_t(savedLayout(n, nb).bytes) : 0;
+}
+
 void launchContactForward(const sdfo_grid& g, const double* X, const double* w, int64_t n,
                           const double* poses, int64_t nb, const sdfo_contact_params& p,
-                          double* out, int32_t* active, hipStream_t s, const CullArgs* cull)
+                          double* out, int32_t* active, hipStream_t s, const CullArgs* cull,
+                          void* saved)
 {
     if (nb <= 0) return;
     forwardLaunch<false>(dev::makeQGrid(g), makeLaw(p), X, w, n, poses, nb, LinCache{}, cull, out,
-                         active, s);
+                         active, s, saved);
 }
 
 namespace {
@@ -1052,7 +1357,7 @@ void launchContactForwardLinearized(const double* g0, const double* grad0, const
 void launchContactBackward(const sdfo_grid& g, const double* X, const double* w, int64_t n,
                            const double* poses, int64_t nb, const sdfo_contact_params& p,
                            const double* gOut, double* gTwist, double* gParams, double* gX,
-                           double* gW, hipStream_t s)
+                           double* gW, hipStream_t s, const void* saved)
 {
     if (nb <= 0) return;
     const QGrid q = dev::makeQGrid(g);
@@ -1060,50 +1365,106 @@ void launchContactBackward(const sdfo_grid& g, const double* X, const double* w,
     const bool wantPoint = (gX || gW) && n > 0;
     const int chunks = n > 0 ? chunksFor(n, nb) : 1;
     const int64_t chunkLen = n > 0 ? (n + chunks - 1) / chunks : 0;
-    // Poses per group bounded so the point partials stay under 256 MiB.
+    const int maxPieces = int((chunkLen + kPiece - 1) / kPiece) > 0 ? int((chunkLen + kPiece - 1) / kPiece) : 1;
+    // Poses per group bounded so the per-(pose, point) buffers (screen list
+    // entry, and with point gradients the point partial and flag) stay under
+    // 256 MiB.
     int64_t group = nb;
-    if (wantPoint)
+    if (n > 0)
     {
-        const int64_t cap = (int64_t(256) << 20) / (n * int64_t(sizeof(double4)));
+        const int64_t perPair = int64_t(sizeof(int32_t)) + (wantPoint ? int64_t(sizeof(double4)) + 1 : 0);
+        const int64_t cap = (int64_t(256) << 20) / (n * perPair);
         group = cap < 1 ? 1 : (cap < nb ? cap : nb);
     }
-    Scratch part(sizeof(double) * (9 * size_t(nb) * chunks + 3 * size_t(nb)), s);
-    double* partial = static_cast<double*>(part.p);
-    double* perPose = partial + 9 * size_t(nb) * chunks;
-    Scratch pts(wantPoint ? sizeof(double4) * size_t(group) * size_t(n) : 16, s);
-    double4* pointPartial = wantPoint ? static_cast<double4*>(pts.p) : nullptr;
+    // per group: screen lists (point indices per (pose, chunk) slot), their
+    // counts, the piece table and its length, the piece sums, and with point
+    // gradients one flag per (pose, point) and the point partials; per pose:
+    // the (kn, c, mu) terms
+    const size_t slots = size_t(group) * size_t(chunks);
+    const size_t pieces = slots * size_t(maxPieces);
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    // with a forward's pair record the lists, counts and flags are read from
+    // it and none are allocated
+    const size_t screenSlots = saved ? 0 : slots;
+    const size_t oList = 0, oCount = oList + al(sizeof(int32_t) * screenSlots * size_t(chunkLen)),
+                 oItems = oCount + al(sizeof(int32_t) * screenSlots),
+                 oStart = oItems + al(sizeof(int32_t) * pieces),
+                 oTotal = oStart + al(sizeof(int32_t) * (size_t(group) + 1)), oSum = oTotal + 256,
+                 oPer = oSum + al(sizeof(double) * 9 * pieces),
+                 oAct = oPer + al(sizeof(double) * 3 * size_t(nb)),
+                 oPts = oAct + al(wantPoint && !saved ? size_t(group) * size_t(n) : 0),
+                 bytes = oPts + (wantPoint ? sizeof(double4) * size_t(group) * size_t(n) : 0);
+    Scratch scr(bytes, s);
+    char* base8 = static_cast<char*>(scr.p);
+    int32_t* list = reinterpret_cast<int32_t*>(base8 + oList);
+    int32_t* count = reinterpret_cast<int32_t*>(base8 + oCount);
+    int32_t* items = reinterpret_cast<int32_t*>(base8 + oItems);
+    int32_t* poseStart = reinterpret_cast<int32_t*>(base8 + oStart);
+    int32_t* total = reinterpret_cast<int32_t*>(base8 + oTotal);
+    double* pieceSum = reinterpret_cast<double*>(base8 + oSum);
+    double* perPose = reinterpret_cast<double*>(base8 + oPer);
+    uint8_t* act = wantPoint && !saved ? reinterpret_cast<uint8_t*>(base8 + oAct) : nullptr;
+    const PairList rec = saved ? savedPairs(const_cast<void*>(saved), savedLayout(n, nb)) : PairList{};
+    double4* pointPartial = wantPoint ? reinterpret_cast<double4*>(base8 + oPts) : nullptr;
+    const bool fused = gParams && group == nb;
+    int cus = 256;
+    {
+        int dev = 0;
+        if (hipGetDevice(&dev) == hipSuccess)
+            (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (cus < 1) cus = 256;
+    }
     for (int64_t base = 0; base < nb; base += group)
     {
         const int64_t cnt = base + group < nb ? group : nb - base;
-        hipLaunchKernelGGL(contactBackwardPoseKernel, dim3(chunks, unsigned(cnt)), dim3(kBlock), 0,
-                           s, q, L, X, w, n, poses, gOut, chunkLen, base, partial, pointPartial);
-        SDFO_HIP(hipGetLastError());
-        if (wantPoint)
+        const dim3 grid(static_cast<unsigned>(chunks), static_cast<unsigned>(cnt));
+        // this group's slots and flags
+        PairList gl{list, count, act};
+        if (saved)
         {
-            hipLaunchKernelGGL(pointReduceKernel,
-                               dim3(unsigned((n + kReducePoints - 1) / kReducePoints)),
-                               dim3(kReducePoints * kReduceSlices), 0, s, n, int(cnt),
-                               static_cast<const double4*>(pointPartial), base > 0 ? 1 : 0, gX, gW);
+            gl.list = rec.list + base * chunks * chunkLen;
+            gl.count = rec.count + base * chunks;
+            gl.act = wantPoint ? rec.act + base * n : nullptr;
+        }
+        else if (L.mode == SDFO_TRILINEAR)
+            hipLaunchKernelGGL(contactScreenKernel<true>, grid, dim3(kBlock), 0, s, q, L, X, n,
+                               poses, chunkLen, base, gl);
+        else
+            hipLaunchKernelGGL(contactScreenKernel<false>, grid, dim3(kBlock), 0, s, q, L, X, n,
+                               poses, chunkLen, base, gl);
+        SDFO_HIP(hipGetLastError());
+        hipLaunchKernelGGL(pieceTableKernel, dim3(1), dim3(kTableBlock), 0, s,
+                           int64_t(cnt) * chunks, chunks, maxPieces,
+                           static_cast<const int32_t*>(gl.count), items, poseStart, total);
+        SDFO_HIP(hipGetLastError());
+        // two waves per SIMD (the kernel's register budget), four SIMDs per CU
+        hipLaunchKernelGGL(contactBackwardPieceKernel, dim3(unsigned(8 * cus)), dim3(64), 0, s, q,
+                           L, X, w, n, poses, gOut, chunks, chunkLen, maxPieces, base,
+                           static_cast<const int32_t*>(gl.list), static_cast<const int32_t*>(gl.count),
+                           static_cast<const int32_t*>(items), static_cast<const int32_t*>(total),
+                           pieceSum, pointPartial);
+        SDFO_HIP(hipGetLastError());
+        // the tail: point reduction, pose finalize and, for a single group,
+        // the parameter sum, as the blocks of one launch
+        const int64_t nReduce = wantPoint ? (n + kReducePoints - 1) / kReducePoints : 0;
+        const int64_t poseBlocks = (gTwist || gParams) ? (cnt + kBlock / 64 - 1) / (kBlock / 64) : 0;
+        const int64_t tailBlocks = nReduce + poseBlocks + (fused ? 1 : 0);
+        if (tailBlocks > 0)
+        {
+            hipLaunchKernelGGL(backwardTailKernel, dim3(unsigned(tailBlocks)), dim3(kBlock), 0, s,
+                               nReduce, n, int(cnt), static_cast<const double4*>(pointPartial),
+                               static_cast<const uint8_t*>(gl.act), base > 0 ? 1 : 0, gX, gW, base,
+                               static_cast<const int32_t*>(poseStart),
+                               static_cast<const double*>(pieceSum),
+                               static_cast<const int32_t*>(total), gTwist,
+                               fused ? nullptr : perPose, fused ? gParams : nullptr);
             SDFO_HIP(hipGetLastError());
         }
     }
-    if (gParams && nb <= kFusedFinalizePoses)
+    if (gParams && !fused)
     {
-        // one block: the pose finalize and the parameter sum in one launch
-        hipLaunchKernelGGL(contactBackwardFinalizeParams, dim3(1), dim3(kBlock), 0, s, nb, chunks,
-                           static_cast<const double*>(partial), gTwist, gParams);
-        SDFO_HIP(hipGetLastError());
-        return;
-    }
-    if (gTwist || gParams)
-    {
-        hipLaunchKernelGGL(contactBackwardPoseFinalize, dim3(unsigned((nb + 255) / 256)), dim3(256),
-                           0, s, nb, chunks, partial, gTwist, perPose);
-        SDFO_HIP(hipGetLastError());
-    }
-    if (gParams)
-    {
-        hipLaunchKernelGGL(paramReduceKernel, dim3(1), dim3(kBlock), 0, s, nb, perPose, gParams);
+        hipLaunchKernelGGL(paramReduceKernel, dim3(1), dim3(kBlock), 0, s, nb,
+                           static_cast<const double*>(perPose), gParams);
         SDFO_HIP(hipGetLastError());
     }
 }

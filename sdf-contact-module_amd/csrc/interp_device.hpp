@@ -296,6 +296,43 @@ template <typename T, bool WithHess, typename F4>
 __device__ __forceinline__ void tricubicCorners(const QGrid& g, const Cell& c, const F4 (&f)[8],
                                                 Interp<T>& r);
 
+// The value alone of the FP32-corner interpolant, with the operations that
+// tricubicCorners uses for its gap (the x, y and z reductions of the value
+// terms), for screens that only need the sign of the gap.
+template <typename T>
+__device__ __forceinline__ T tricubicValueCorners(const QGrid& g, const Cell& c, const float4 (&f)[8])
+{
+    const Basis<T> A = hermite<T, false>(T(c.u));
+    const Basis<T> B = hermite<T, false>(T(c.v));
+    const Basis<T> Cz = hermite<T, false>(T(c.w));
+    const T vx = T(g.voxel[0]), vy = T(g.voxel[1]), vz = T(g.voxel[2]);
+    auto dot4 = [](T a0, T x0, T a1, T x1, T a2, T x2, T a3, T x3) {
+        return fmadd(a0, x0, fmadd(a1, x1, fmadd(a2, x2, a3 * x3)));
+    };
+    auto dot2 = [](T a0, T x0, T a1, T x1) { return fmadd(a0, x0, a1 * x1); };
+    const T a0[2] = {A.h0, A.h2}, a1[2] = {A.h1 * vx, A.h3 * vx};
+    T X0[4], Xy[4], Xz[4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+    {
+        const float4 e = f[2 * p], o = f[2 * p + 1];
+        X0[p] = dot4(a0[0], T(e.x), a1[0], T(e.y), a0[1], T(o.x), a1[1], T(o.y));
+        Xy[p] = dot2(a0[0], T(e.z), a0[1], T(o.z));
+        Xz[p] = dot2(a0[0], T(e.w), a0[1], T(o.w));
+    }
+    const T b0[2] = {B.h0, B.h2}, b1[2] = {B.h1 * vy, B.h3 * vy};
+    T Y0[2], Yz[2];
+#pragma unroll
+    for (int dk = 0; dk < 2; ++dk)
+    {
+        const int p = 2 * dk, q = 2 * dk + 1;
+        Y0[dk] = dot4(b0[0], X0[p], b1[0], Xy[p], b0[1], X0[q], b1[1], Xy[q]);
+        Yz[dk] = dot2(b0[0], Xz[p], b0[1], Xz[q]);
+    }
+    const T c0[2] = {Cz.h0, Cz.h2}, c1[2] = {Cz.h1 * vz, Cz.h3 * vz};
+    return dot4(c0[0], Y0[0], c1[0], Yz[0], c0[1], Y0[1], c1[1], Yz[1]);
+}
+
 template <typename T, bool WithHess, bool F64 = false>
 __device__ __forceinline__ bool tricubic(const QGrid& g, const Cell& c, Interp<T>& r)
 {

@@ -64,18 +64,59 @@ QGrid makeQGrid(const sdfo_grid& g)
 } // namespace dev
 
 namespace {
-// Trilinear / Tricubic on a dense grid with cell records, two points per
-// thread: both points' 128-byte records are requested before either is
-// interpolated, doubling the loads in flight per lane (the gather is
-// latency-bound at one point per lane).  Same arithmetic as queryKernel.
+// Default block size of the two-points-per-lane query (queryPairKernel).
 constexpr int kPairBlock = 128;
 // Points per lane of the single-grid kernel: two (three took 32.4 us against
 // 29.8 us for 1e6 Tricubic points on a 128^3 grid).
 constexpr int kLanePoints = 2;
 
+// Where pairQuery takes a cell's 8 corner samples from: the dense record
+// array, any grid's records (dense or through the brick table), or gathered
+// corner by corner from the FP32 mirror of a grid without records (dense:
+// eight addresses from the cell index; sparse: eight brick-table lookups).
+enum CornerSrc
+{
+    kDenseRec = 0,
+    kAnyRec = 1,
+    kDenseGather = 2,
+    kSparseGather = 3
+};
+
+// The 8 corners of a cell from the mirror; false if a sample is missing.  All
+// addresses are formed before any load, so the eight loads are in flight
+// together (a missing sparse sample reads record 0 and invalidates the cell).
+template <int Src>
+__device__ __forceinline__ bool gatherCorners(const QGrid& g, const dev::Cell& c, float4 (&f)[8])
+{
+    int64_t rec[8];
+    bool ok = true;
+    if (Src == kDenseGather)
+    {
+        const int64_t nx = g.n[0], nxy = int64_t(g.n[0]) * g.n[1];
+        const int64_t b = c.i0 + nx * c.j0 + nxy * c.k0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) rec[q] = b + (q & 1) + nx * ((q >> 1) & 1) + nxy * (q >> 2);
+    }
+    else
+    {
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+        {
+            const int64_t r = dev::recordOf(g, c.i0 + (q & 1), c.j0 + ((q >> 1) & 1), c.k0 + (q >> 2));
+            ok = ok && r >= 0;
+            rec[q] = r < 0 ? 0 : r;
+        }
+    }
+    const float4* pn = Src == kDenseGather ? g.pn : g.spn;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) f[q] = pn[rec[q]];
+    return ok;
+}
+
 // Points i0, i0 + stride, ... (Pts of them, each processed when below its
-// end) of one grid.
-template <typename TP, typename T, int Mode, bool Sparse, int Pts = 2>
+// end) of one grid: every point's corner loads are issued before any point
+// is interpolated.
+template <typename TP, typename T, int Mode, int Src, int Pts = 2>
 __device__ __forceinline__ void pairQuery(const QGrid& g, const P3<TP>* __restrict__ pts,
                                           int64_t i0, int64_t stride, int64_t end,
                                           TP* __restrict__ gap, P3<TP>* __restrict__ grad3,
@@ -99,20 +140,25 @@ __device__ __forceinline__ void pairQuery(const QGrid& g, const P3<TP>* __restri
         c[h].valid = c[h].valid && i < end;
         if (c[h].valid)
         {
-            if (!Sparse)
+            if (Src == kDenseRec)
             {
                 const float4* r = g.cellRec + 8 * (int64_t(c[h].i0) +
                                                    int64_t(g.n[0] - 1) * (c[h].j0 + int64_t(g.n[1] - 1) * c[h].k0));
 #pragma unroll
                 for (int q = 0; q < 8; ++q) f[h][q] = r[q];
             }
-            else if (const float4* r = dev::cellRecord(g, c[h].i0, c[h].j0, c[h].k0))
+            else if (Src == kAnyRec)
             {
+                if (const float4* r = dev::cellRecord(g, c[h].i0, c[h].j0, c[h].k0))
+                {
 #pragma unroll
-                for (int q = 0; q < 8; ++q) f[h][q] = r[q];
+                    for (int q = 0; q < 8; ++q) f[h][q] = r[q];
+                }
+                else
+                    c[h].valid = false; // a missing corner sample: an invalid result
             }
-            else
-                c[h].valid = false; // a missing corner sample: an invalid result
+            else if (!gatherCorners<Src>(g, c[h], f[h]))
+                c[h].valid = false;
         }
     }
 #pragma unroll
@@ -140,11 +186,11 @@ __device__ __forceinline__ void pairQuery(const QGrid& g, const P3<TP>* __restri
     }
 }
 
-// Trilinear / Tricubic on a dense grid with cell records, two points per
-// thread: both points' 128-byte records are requested before either is
-// interpolated, doubling the loads in flight per lane (the gather is
-// latency-bound at one point per lane).  Same arithmetic as queryKernel.
-template <typename TP, typename T, int Mode, bool Sparse, int Block = kPairBlock>
+// Trilinear / Tricubic, two points per thread: both points' corner samples
+// (one 128-byte record each, or eight mirror loads) are requested before
+// either is interpolated, doubling the loads in flight per lane (the gather
+// is latency-bound at one point per lane).  Same arithmetic as queryKernel.
+template <typename TP, typename T, int Mode, int Src, int Block = kPairBlock>
 __global__ void __launch_bounds__(Block)
 queryPairKernel(QGrid g, const P3<TP>* __restrict__ pts, int64_t n, TP* __restrict__ gap,
                 P3<TP>* __restrict__ grad3, uint8_t* __restrict__ valid,
@@ -152,7 +198,7 @@ queryPairKernel(QGrid g, const P3<TP>* __restrict__ pts, int64_t n, TP* __restri
                 int32_t* __restrict__ branch)
 {
     const int64_t i0 = int64_t(blockIdx.x) * (kLanePoints * Block) + threadIdx.x;
-    pairQuery<TP, T, Mode, Sparse, kLanePoints>(g, pts, i0, Block, n, gap, grad3, valid, normal3, witness3,
+    pairQuery<TP, T, Mode, Src, kLanePoints>(g, pts, i0, Block, n, gap, grad3, valid, normal3, witness3,
                            branch);
 }
 
@@ -193,15 +239,25 @@ queryBatchedKernel(const StreamDesc* __restrict__ sd, int ns, int64_t units,
     const StreamDesc& D = sd[lo];
     const int64_t i = D.pointBegin + (u - D.unitBegin) * kUnitPoints + (threadIdx.x & 63);
     const bool cubic = Mode == SDFO_TRICUBIC || (Mode == SDFO_CONTACT_AWARE && D.g.stride <= 0);
-    if ((D.g.sparse ? D.g.scellRec != nullptr : D.g.cellRec != nullptr) &&
-        (cubic || Mode == SDFO_TRILINEAR))
+    if (cubic || Mode == SDFO_TRILINEAR)
     {
-        if (cubic)
-            pairQuery<float, float, SDFO_TRICUBIC, true>(D.g, pts, i, 64, D.pointEnd, gap, grad3,
+        const bool rec = D.g.sparse ? D.g.scellRec != nullptr : D.g.cellRec != nullptr;
+        auto run = [&](auto modeTag) {
+            constexpr int M = decltype(modeTag)::value;
+            if (rec)
+                pairQuery<float, float, M, kAnyRec>(D.g, pts, i, 64, D.pointEnd, gap, grad3, valid,
+                                                    nullptr, nullptr, nullptr);
+            else if (D.g.sparse)
+                pairQuery<float, float, M, kSparseGather>(D.g, pts, i, 64, D.pointEnd, gap, grad3,
+                                                          valid, nullptr, nullptr, nullptr);
+            else
+                pairQuery<float, float, M, kDenseGather>(D.g, pts, i, 64, D.pointEnd, gap, grad3,
                                                          valid, nullptr, nullptr, nullptr);
+        };
+        if (cubic)
+            run(std::integral_constant<int, SDFO_TRICUBIC>{});
         else
-            pairQuery<float, float, SDFO_TRILINEAR, true>(D.g, pts, i, 64, D.pointEnd, gap,
-                                                          grad3, valid, nullptr, nullptr, nullptr);
+            run(std::integral_constant<int, SDFO_TRILINEAR>{});
         return;
     }
 #pragma unroll
@@ -264,39 +320,44 @@ void launchQuery(const sdfo_grid& g, int mode, const TP* pts, int64_t n, TP* gap
     V* nm = reinterpret_cast<V*>(normal3);
     V* wt = reinterpret_cast<V*>(witness3);
     const bool cubic = mode == SDFO_TRICUBIC || (mode == SDFO_CONTACT_AWARE && q.stride <= 0);
-    if ((q.sparse ? q.scellRec != nullptr : q.cellRec != nullptr) && (cubic || mode == SDFO_TRILINEAR))
+    if (cubic || mode == SDFO_TRILINEAR)
     {
-        auto launchBlock = [&](auto sparseTag, auto blockTag) {
-            constexpr bool Sp = decltype(sparseTag)::value;
+        const bool rec = q.sparse ? q.scellRec != nullptr : q.cellRec != nullptr;
+        const int src = rec ? (q.sparse ? kAnyRec : kDenseRec) : (q.sparse ? kSparseGather : kDenseGather);
+        auto launchBlock = [&](auto srcTag, auto blockTag) {
+            constexpr int Src = decltype(srcTag)::value;
             constexpr int B = decltype(blockTag)::value;
             const dim3 pg(unsigned((n + kLanePoints * B - 1) / (kLanePoints * B))), pb(B);
             if (cubic)
-                hipLaunchKernelGGL((queryPairKernel<TP, T, SDFO_TRICUBIC, Sp, B>), pg, pb, 0, s, q, p,
-                                   n, gap, gr, valid, nm, wt, branch);
+                hipLaunchKernelGGL((queryPairKernel<TP, T, SDFO_TRICUBIC, Src, B>), pg, pb, 0, s, q,
+                                   p, n, gap, gr, valid, nm, wt, branch);
             else
-                hipLaunchKernelGGL((queryPairKernel<TP, T, SDFO_TRILINEAR, Sp, B>), pg, pb, 0, s, q,
+                hipLaunchKernelGGL((queryPairKernel<TP, T, SDFO_TRILINEAR, Src, B>), pg, pb, 0, s, q,
                                    p, n, gap, gr, valid, nm, wt, branch);
         };
         auto launch = [&](int variant) {
-            auto withSparse = [&](auto sparseTag) {
+            auto withSrc = [&](auto srcTag) {
                 switch (variant)
                 {
-                case 1: launchBlock(sparseTag, std::integral_constant<int, kPairBlocks[1]>{}); break;
-                case 2: launchBlock(sparseTag, std::integral_constant<int, kPairBlocks[2]>{}); break;
-                case 3: launchBlock(sparseTag, std::integral_constant<int, kPairBlocks[3]>{}); break;
-                default: launchBlock(sparseTag, std::integral_constant<int, kPairBlocks[0]>{}); break;
+                case 1: launchBlock(srcTag, std::integral_constant<int, kPairBlocks[1]>{}); break;
+                case 2: launchBlock(srcTag, std::integral_constant<int, kPairBlocks[2]>{}); break;
+                case 3: launchBlock(srcTag, std::integral_constant<int, kPairBlocks[3]>{}); break;
+                default: launchBlock(srcTag, std::integral_constant<int, kPairBlocks[0]>{}); break;
                 }
             };
-            if (q.sparse)
-                withSparse(std::true_type{});
-            else
-                withSparse(std::false_type{});
+            switch (src)
+            {
+            case kDenseRec: withSrc(std::integral_constant<int, kDenseRec>{}); break;
+            case kAnyRec: withSrc(std::integral_constant<int, kAnyRec>{}); break;
+            case kDenseGather: withSrc(std::integral_constant<int, kDenseGather>{}); break;
+            default: withSrc(std::integral_constant<int, kSparseGather>{}); break;
+            }
             SDFO_HIP(hipGetLastError());
         };
         // the tuner times every block size on this call's own points (each
         // run writes the same outputs) the first time a size class is seen
         const std::string key = std::string("query ") + (q.sparse ? "sparse " : "dense ") +
-                                (cubic ? "tricubic" : "trilinear") +
+                                (rec ? "" : "gather ") + (cubic ? "tricubic" : "trilinear") +
                                 (sizeof(TP) == 8 ? " f64" : " f32") + " points~2^" +
                                 std::to_string(log2Bucket(n));
         int variant = 0;

@@ -473,10 +473,14 @@ void launchCandidatesF64(const sdfo_grid& g, const double* pts, int64_t n, uint8
                          double* normal3, double* witness3, uint8_t* valid, hipStream_t s);
 
 // Contact launchers (contact.hip).
+// With saved (device, contactSavedBytes(n, nb) bytes; plain forward only)
+// the forward also records which (pose, point) pairs are in contact, for
+// launchContactBackward to read instead of screening every pair again.
 void launchContactForward(const sdfo_grid& g, const double* X, const double* w, int64_t n,
                           const double* poses, int64_t nb, const sdfo_contact_params& p,
                           double* out, int32_t* active, hipStream_t s,
-                          const CullArgs* cull = nullptr);
+                          const CullArgs* cull = nullptr, void* saved = nullptr);
+int64_t contactSavedBytes(int64_t n, int64_t nb);
 void launchContactEvaluate(const sdfo_grid& g, const double* X, const double* w, int64_t n,
                            const double* states, const double* frames, int64_t nb,
                            const sdfo_contact_params& p, const CullArgs* cull,
@@ -500,6 +504,6 @@ void launchContactForwardLinearized(const double* g0, const double* grad0, const
 void launchContactBackward(const sdfo_grid& g, const double* X, const double* w, int64_t n,
                            const double* poses, int64_t nb, const sdfo_contact_params& p,
                            const double* gOut, double* gTwist, double* gParams, double* gX,
-                           double* gW, hipStream_t s);
+                           double* gW, hipStream_t s, const void* saved = nullptr);
 
 } // namespace sdfo

@@ -161,6 +161,11 @@ _sig("sdfo_contact_forward_culled", [_vp, _vp, _vp, _vp, _vp, _i64, _vp, _i64, _
                                      _vp, _vp, _vp])
 _sig("sdfo_contact_backward", [_vp, _vp, _vp, _i64, _vp, _i64, _P(ContactParams), _vp, _vp, _vp,
                                _vp, _vp, _vp])
+_sig("sdfo_contact_saved_bytes", [_i64, _i64], C.c_int64)
+_sig("sdfo_contact_forward_save", [_vp, _vp, _vp, _i64, _vp, _i64, _P(ContactParams), _vp, _vp,
+                                   _vp, _i64, _vp])
+_sig("sdfo_contact_backward_saved", [_vp, _vp, _vp, _i64, _vp, _i64, _P(ContactParams), _vp, _vp,
+                                     _i64, _vp, _vp, _vp, _vp, _vp])
 
 _sig("sdfo_debug_bvh_export", [_vp, _i64, _vp, _i64, _vp, _vp, _vp, _i32, _vp])
 _sig("sdfo_debug_sort_by_abs", [_vp, _i32, _vp])

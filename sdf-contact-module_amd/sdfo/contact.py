@@ -72,14 +72,25 @@ def _check_inputs(X, w, poses):
 
 
 def contact_forward(grid: SdfGrid, X, w, poses, law: ContactLaw = ContactLaw(), stream=None,
-                    active=False):
-    """(F_A, tau_A) per pose as a (B,6) float64 tensor (SDF frame)."""
+                    active=False, save=False):
+    """(F_A, tau_A) per pose as a (B,6) float64 tensor (SDF frame).
+
+    With ``save``, also returns the record of the (pose, point) pairs in
+    contact (a uint8 GPU tensor) for ``contact_backward(..., saved=...)`` on
+    the same inputs: ``out[, act], saved``."""
     _check_inputs(X, w, poses)
-    B = poses.shape[0]
+    B, N = poses.shape[0], X.shape[0]
     out = torch.empty((B, 6), dtype=torch.float64, device=X.device)
     act = torch.empty(B, dtype=torch.int32, device=X.device) if active else None
     p = law.ctypes()
-    check(lib.sdfo_contact_forward(grid.handle, _ptr(X), _ptr(w), X.shape[0], _ptr(poses), B,
+    if save:
+        nbytes = int(lib.sdfo_contact_saved_bytes(N, B))
+        rec = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=X.device)
+        check(lib.sdfo_contact_forward_save(grid.handle, _ptr(X), _ptr(w), N, _ptr(poses), B,
+                                            C.byref(p), _ptr(out), _ptr(act), _ptr(rec), nbytes,
+                                            _stream_handle(stream)))
+        return (out, act, rec) if active else (out, rec)
+    check(lib.sdfo_contact_forward(grid.handle, _ptr(X), _ptr(w), N, _ptr(poses), B,
                                    C.byref(p), _ptr(out), _ptr(act), _stream_handle(stream)))
     return (out, act) if active else out
 
@@ -304,9 +315,14 @@ class LinearizationCache:
 
 
 def contact_backward(grid: SdfGrid, X, w, poses, g_out, law: ContactLaw = ContactLaw(),
-                     want_twist=True, want_params=True, want_X=True, want_w=True, stream=None):
+                     want_twist=True, want_params=True, want_X=True, want_w=True, stream=None,
+                     saved=None):
     """Gradients of sum(g_out * forward): (g_twist (B,6) [omega, dt] left twist,
-    g_params (3,) [kn, damping, mu], g_X (N,3), g_w (N,))."""
+    g_params (3,) [kn, damping, mu], g_X (N,3), g_w (N,)).
+
+    ``saved``: the record of ``contact_forward(..., save=True)`` on the same
+    grid, X, w, poses and law; the backward then reads which pairs are in
+    contact instead of screening every pair again."""
     _check_inputs(X, w, poses)
     g_out = g_out.contiguous().to(torch.float64)
     B, N = poses.shape[0], X.shape[0]
@@ -322,6 +338,16 @@ def contact_backward(grid: SdfGrid, X, w, poses, g_out, law: ContactLaw = Contac
     gx = parts[2].view(N, 3) if want_X else None
     gw = parts[3] if want_w else None
     p = law.ctypes()
+    if saved is not None:
+        nbytes = int(lib.sdfo_contact_saved_bytes(N, B))
+        if saved.dtype != torch.uint8 or not saved.is_cuda or saved.numel() < nbytes:
+            raise ValueError("saved must be the record of contact_forward(..., save=True) "
+                             "on the same inputs")
+        check(lib.sdfo_contact_backward_saved(grid.handle, _ptr(X), _ptr(w), N, _ptr(poses), B,
+                                              C.byref(p), _ptr(g_out), _ptr(saved), nbytes,
+                                              _ptr(gt), _ptr(gp), _ptr(gx), _ptr(gw),
+                                              _stream_handle(stream)))
+        return gt, gp, gx, gw
     check(lib.sdfo_contact_backward(grid.handle, _ptr(X), _ptr(w), N, _ptr(poses), B, C.byref(p),
                                     _ptr(g_out), _ptr(gt), _ptr(gp), _ptr(gx), _ptr(gw),
                                     _stream_handle(stream)))
@@ -375,19 +401,19 @@ class ContactFunction(torch.autograd.Function):
                          mode)
         Xc = X.detach().contiguous()
         wc = w.detach().contiguous()
-        out = contact_forward(grid, Xc, wc, poses, law)
-        ctx.save_for_backward(twist, Xc, wc, poses)
+        out, rec = contact_forward(grid, Xc, wc, poses, law, save=True)
+        ctx.save_for_backward(twist, Xc, wc, poses, rec)
         ctx.grid = grid
         ctx.law = law
         return out
 
     @staticmethod
     def backward(ctx, g_out):
-        twist, X, w, poses = ctx.saved_tensors
+        twist, X, w, poses, rec = ctx.saved_tensors
         need = ctx.needs_input_grad
         gt, gp, gx, gw = contact_backward(ctx.grid, X, w, poses, g_out, ctx.law,
                                           want_twist=need[0], want_params=any(need[1:4]),
-                                          want_X=need[4], want_w=need[5])
+                                          want_X=need[4], want_w=need[5], saved=rec)
         g_twist = None
         if need[0]:
             Jl = so3_left_jacobian(twist[:, :3])

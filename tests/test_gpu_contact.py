@@ -324,3 +324,51 @@ def test_contact_gradients_match_central_differences(cube):
                 xm[r, k] -= h
                 fdx = (loss(twist.detach(), kn.detach(), xp) - loss(twist.detach(), kn.detach(), xm)) / (2 * h)
                 assert abs(float(gx[r, k]) - float(fdx)) <= 1e-4 * abs(float(fdx)) + float(floor_x)
+
+
+@pytest.mark.parametrize("mode,mu,order,margin", [("Tricubic", 0.3, 2, 0.0),
+                                                  ("Trilinear", 0.3, 1, 0.0),
+                                                  ("Tricubic", 0.0, 1, 0.058),
+                                                  ("Tricubic", 0.2, 0, 0.0)])
+def test_saved_forward_and_backward(O, cube, mode, mu, order, margin):
+    """The forward that keeps its pairs in contact returns the plain forward's
+    bits; the backward on that record matches the oracle and the screening
+    backward (up to the order of the sums) and is run-to-run reproducible."""
+    g, ref = cube
+    print(f"[gpu] saved forward/backward {mode} mu={mu} order={order} margin={margin}",
+          flush=True)
+    X, w, poses = _scene(40, seed=21)
+    Xd, wd, Pd = _cuda(X, w, poses)
+    law = SC.ContactLaw(kn=1e11, damping=1e4, mu=mu, regularizer=1e-6, order=order, mode=mode,
+                        narrow_margin=margin)
+    plain, act0 = SC.contact_forward(g, Xd, wd, Pd, law, active=True)
+    out, act, rec = SC.contact_forward(g, Xd, wd, Pd, law, active=True, save=True)
+    assert torch.equal(out, plain) and torch.equal(act, act0)
+    assert int(act.sum()) > 0
+    gout = torch.from_numpy(np.random.default_rng(3).normal(size=(len(poses), 6))).cuda()
+    a = SC.contact_backward(g, Xd, wd, Pd, gout, law, saved=rec)
+    b = SC.contact_backward(g, Xd, wd, Pd, gout, law, saved=rec)
+    s = SC.contact_backward(g, Xd, wd, Pd, gout, law)
+    exp = O.contact_backward(ograd(O, g.spec), f32(ref["phi"]), f32(ref["n"]), X, w, poses,
+                             gout.cpu().numpy(), kn=1e11, c=1e4, mu=mu, reg=1e-6, order=order,
+                             mode=mode, margin=margin)
+    for x, y, z, e in zip(a, b, s, exp):
+        assert torch.equal(x, y)
+        x, z = x.cpu().numpy(), z.cpu().numpy()
+        scale = np.abs(e).max() + 1e-300
+        assert np.abs(x - e).max() <= 1e-7 * scale
+        assert np.abs(x - z).max() <= 1e-12 * scale
+
+
+def test_saved_record_size_checked(cube):
+    from sdfo import _lib
+    g, _ = cube
+    X, w, poses = _scene(4, seed=2)
+    Xd, wd, Pd = _cuda(X, w, poses)
+    law = SC.ContactLaw()
+    _, rec = SC.contact_forward(g, Xd, wd, Pd, law, save=True)
+    gout = torch.zeros((4, 6), dtype=torch.float64, device="cuda")
+    with pytest.raises(ValueError):
+        SC.contact_backward(g, Xd, wd, Pd, gout, law, saved=rec[:10])
+    assert _lib.lib.sdfo_contact_saved_bytes(-1, 4) == -1
+    assert _lib.lib.sdfo_contact_saved_bytes(len(X), 0) == 0
