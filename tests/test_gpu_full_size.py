@@ -15,7 +15,10 @@
   every brick's activation against the oracle's |signed distance| of the
   brick centre, and the voxels of 300 seeded active bricks bit for bit.
 * Differentiation: 256 poses with twists x 7680 points, friction on: forward
-  and every gradient output of the backward against the oracle.
+  and every gradient output of the backward against the oracle, with the
+  backward's own screen and with the forward's record of the pairs in
+  contact; 1200 poses (the backward in pose groups) the same; and poses out of
+  contact give zeros.
 * Batched gears: 64 seeded meshes (2.88 M triangles) in one batched build,
   1500 seeded voxels of every grid bit for bit.
 * The reference's own CPU-runnable case: the 0.1 m cube at 64^3, every
@@ -196,6 +199,55 @@ def test_fwd_bwd_256_poses_full_size(O, cube_field):
     for got, exp in zip(grads, refs):
         got = got.cpu().numpy()
         assert np.abs(got - exp).max() <= 1e-7 * (np.abs(exp).max() + 1e-300)
+    # the training-step form: the forward records its pairs for the backward
+    out2, rec = SC.contact_forward(g, Xd, wd, Pd, law, save=True)
+    assert torch.equal(out2, out)
+    grads2 = SC.contact_backward(g, Xd, wd, Pd, torch.from_numpy(gout).cuda(), law, saved=rec)
+    for got, exp in zip(grads2, refs):
+        got = got.cpu().numpy()
+        assert np.abs(got - exp).max() <= 1e-7 * (np.abs(exp).max() + 1e-300)
+
+
+def test_fwd_bwd_1200_poses_in_pose_groups(O, cube_field):
+    """Past 944 poses x 7680 points the backward's per-(pose, point) buffers
+    are split into pose groups (separate finalize, parameter sum over the
+    groups, point gradients accumulated group by group): every output of the
+    screening and of the recorded backward against the oracle."""
+    g, ref = cube_field
+    print("[gpu] fwd+bwd, 1200 poses in pose groups", flush=True)
+    X, w = scenes.ring_quadrature(120, 32)
+    R, t = scenes.ring_poses(1200, seed=12)
+    v0, Om = scenes.pose_velocities(1200, seed=13)
+    poses = scenes.pose_array(R, t, v0=v0, Om=Om)
+    gout = np.random.default_rng(18).normal(size=(1200, 6))
+    law = SC.ContactLaw(kn=1e11, damping=1e4, mu=0.3, regularizer=1e-6, order=2)
+    Xd, wd, Pd = (torch.from_numpy(a).cuda() for a in (X, w, poses))
+    G = torch.from_numpy(gout).cuda()
+    refs = O.contact_backward(ograd(O, g.spec), f32(ref["phi"]), f32(ref["n"]), X, w, poses, gout,
+                              kn=1e11, c=1e4, mu=0.3, reg=1e-6, order=2)
+    _, rec = SC.contact_forward(g, Xd, wd, Pd, law, save=True)
+    for grads in (SC.contact_backward(g, Xd, wd, Pd, G, law),
+                  SC.contact_backward(g, Xd, wd, Pd, G, law, saved=rec)):
+        for got, exp in zip(grads, refs):
+            got = got.cpu().numpy()
+            assert np.abs(got - exp).max() <= 1e-7 * (np.abs(exp).max() + 1e-300)
+
+
+def test_fwd_bwd_without_contact_is_zero(cube_field):
+    """Poses 50 mm above the cube: no pair is in contact, every output is 0."""
+    g, _ = cube_field
+    X, w = scenes.ring_quadrature(120, 32)
+    R, t = scenes.ring_poses(64, seed=4, z_range=(0.11, 0.12))
+    poses = scenes.pose_array(R, t)
+    law = SC.ContactLaw(kn=1e11, damping=1e4, mu=0.3, regularizer=1e-6, order=2)
+    Xd, wd, Pd = (torch.from_numpy(a).cuda() for a in (X, w, poses))
+    out, act, rec = SC.contact_forward(g, Xd, wd, Pd, law, active=True, save=True)
+    assert int(act.sum()) == 0 and float(out.abs().max()) == 0.0
+    G = torch.ones((64, 6), dtype=torch.float64, device="cuda")
+    for grads in (SC.contact_backward(g, Xd, wd, Pd, G, law),
+                  SC.contact_backward(g, Xd, wd, Pd, G, law, saved=rec)):
+        for t_ in grads:
+            assert float(t_.abs().max()) == 0.0
 
 
 def test_cube64_demo_trilinear(O, sdfo_gpu):
