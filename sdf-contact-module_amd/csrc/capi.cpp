@@ -49,6 +49,9 @@ thread_local std::string g_lastError;
 // keep no cell records, so every query and contact kernel takes the
 // per-corner gather path that grids past cellRecordLimit() use.
 thread_local bool g_skipCellRecords = false;
+// A thread keeps the host BVH scratch of meshes up to this many triangles
+// between builds (about 100 MB at the limit).
+constexpr int64_t kKeepScratchTris = int64_t(1) << 20;
 
 bool cellRecordsFit(int64_t bytes) { return !g_skipCellRecords && bytes <= cellRecordLimit(); }
 }
@@ -250,7 +253,7 @@ std::unique_ptr<sdfo_grid> buildDense(const double* verts, int64_t nv, const int
     const int64_t N = g->spec.total();
     ph[0] = msSince(t0);
     const clk::time_point t1 = clk::now();
-    const HostBvh hb = buildHostBvh(verts, nv, tris, nt);
+    const HostBvhView hb = buildHostBvhScratch(verts, nv, tris, nt);
     ph[1] = msSince(t1);
     DeviceBvh db;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -258,7 +261,7 @@ std::unique_ptr<sdfo_grid> buildDense(const double* verts, int64_t nv, const int
     {
         const clk::time_point t2 = clk::now();
         keepStreamPoolMemory(g->device);
-        db.upload(hb, s);
+        db.upload(hb.nodes, hb.nNodes, hb.slotTri9, hb.slotTri, hb.nSlots, hb.depth, hb.rayDir, s);
         g->phi = g->alloc<double>(size_t(N));
         g->n3 = g->alloc<double>(size_t(N) * 3);
         g->h6 = g->alloc<double>(size_t(N) * 6);
@@ -289,10 +292,13 @@ std::unique_ptr<sdfo_grid> buildDense(const double* verts, int64_t nv, const int
         for (hipEvent_t e : ev)
             if (e) (void)hipEventDestroy(e);
         db.release();
+        releaseHostBvhScratch(kKeepScratchTris);
         throw;
     }
     for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     db.release();
+    // the copies from the scratch arrays finished with the stream
+    releaseHostBvhScratch(kKeepScratchTris);
     ph[6] = msSince(t0);
     const int* rc = replayCounterHost();
     ph[7] = rc ? double(*rc) : -1.0;
@@ -533,10 +539,13 @@ std::vector<std::unique_ptr<sdfo_grid>> buildDenseBatched(int32_t nm, const doub
         for (hipEvent_t e : ev)
             if (e) (void)hipEventDestroy(e);
         db.release();
+        releaseHostBvhScratch(kKeepScratchTris);
         throw;
     }
     for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     db.release();
+    // the copies from the scratch arrays finished with the stream
+    releaseHostBvhScratch(kKeepScratchTris);
     ph[6] = msSince(t0);
     const int* rc = replayCounterHost();
     ph[7] = rc ? double(*rc) : -1.0;
@@ -566,12 +575,12 @@ std::unique_ptr<sdfo_grid> buildSparse(const double* verts, int64_t nv, const in
     g->nby = (g->spec.n[1] + bd - 1) / bd;
     g->nbz = (g->spec.n[2] + bd - 1) / bd;
     const int64_t cells = int64_t(g->nbx) * g->nby * g->nbz;
-    const HostBvh hb = buildHostBvh(verts, nv, tris, nt);
+    const HostBvhView hb = buildHostBvhScratch(verts, nv, tris, nt);
     DeviceBvh db;
     try
     {
         keepStreamPoolMemory(g->device);
-        db.upload(hb, s);
+        db.upload(hb.nodes, hb.nNodes, hb.slotTri9, hb.slotTri, hb.nSlots, hb.depth, hb.rayDir, s);
         g->activeMask = g->alloc<uint8_t>(size_t(cells));
         launchSparseActivate(g->spec, db, bd, g->nbx, g->nby, g->nbz, g->activation, g->activeMask, s);
         std::vector<uint8_t> mask(static_cast<size_t>(cells));
@@ -612,9 +621,11 @@ std::unique_ptr<sdfo_grid> buildSparse(const double* verts, int64_t nv, const in
     {
         (void)hipStreamSynchronize(s);
         db.release();
+        releaseHostBvhScratch(kKeepScratchTris);
         throw;
     }
     db.release();
+    releaseHostBvhScratch(kKeepScratchTris);
     return g;
 }
 

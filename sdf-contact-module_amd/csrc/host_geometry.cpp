@@ -6,6 +6,7 @@
 #include <condition_variable>
 #include <deque>
 #include <limits>
+#include <memory>
 #include <mutex>
 #include <numeric>
 #include <thread>
@@ -154,7 +155,7 @@ template <typename F>
 void parallelFor(int n, F f, bool parallel = true)
 {
     const int threads = std::min(16, hostThreads());
-    const int chunks = (n < 65536 || !parallel) ? 1 : threads;
+    const int chunks = (n < 16384 || !parallel) ? 1 : threads;
     if (chunks <= 1)
     {
         f(0, n);
@@ -183,11 +184,75 @@ struct Item
     int pad;
 };
 
+// A growable buffer of trivially constructible elements that keeps its
+// memory between uses and leaves new elements uninitialised: per-build host
+// arrays then cost neither a fresh page fault per page nor a zero fill.
+template <typename T>
+struct GrowBuffer
+{
+    std::unique_ptr<T[]> p;
+    size_t cap = 0;
+    T* get(size_t n)
+    {
+        if (n > cap)
+        {
+            p.reset(new T[n]);
+            cap = n;
+        }
+        return p.get();
+    }
+    // drops buffers far larger than typical meshes once they are not needed
+    void trim(size_t keep)
+    {
+        if (cap > keep)
+        {
+            p.reset();
+            cap = 0;
+        }
+    }
+};
+
+// Per-triangle boxes and centroid items of the calling thread's builds.  A
+// thread waiting for the worker pool runs queued tasks meanwhile, which may
+// be another mesh's build: such a nested build finds the thread's scratch in
+// use and takes buffers of its own.
+struct BvhScratch
+{
+    GrowBuffer<double> box;
+    GrowBuffer<Item> items;
+    bool busy = false;
+};
+thread_local BvhScratch t_bvhScratch;
+
+struct ScratchLease
+{
+    BvhScratch* s;
+    std::unique_ptr<BvhScratch> own;
+    ScratchLease()
+    {
+        if (t_bvhScratch.busy)
+        {
+            own = std::make_unique<BvhScratch>();
+            s = own.get();
+        }
+        else
+            s = &t_bvhScratch;
+        s->busy = true;
+    }
+    ~ScratchLease()
+    {
+        s->busy = false;
+        // keep the scratch of meshes up to 2^19 triangles (46 MB) per thread
+        s->box.trim(size_t(6) << 19);
+        s->items.trim(size_t(1) << 19);
+    }
+};
+
 struct BvhBuilder
 {
-    BvhNode* nodes;                 // this mesh's nodes (indices local to the mesh)
-    std::vector<Item>& items;
-    const std::vector<double>& box; // bmin3, bmax3 per triangle
+    BvhNode* nodes;    // this mesh's nodes (indices local to the mesh)
+    Item* items;
+    const double* box; // bmin3, bmax3 per triangle
 
     // Builds the subtree over items[lo, hi) whose root has pre-order index me;
     // returns the subtree depth.  Subtrees above `par` triangles are built
@@ -207,7 +272,7 @@ struct BvhBuilder
             double bmn[3] = {big, big, big}, bmx[3] = {-big, -big, -big};
             for (int i = lo; i < hi; ++i)
             {
-                const double* r = &box[6 * size_t(items[size_t(i)].idx)];
+                const double* r = box + 6 * size_t(items[size_t(i)].idx);
                 for (int k = 0; k < 3; ++k)
                 {
                     bmn[k] = minOf(bmn[k], r[k]);
@@ -241,7 +306,7 @@ struct BvhBuilder
         const int mid = lo + count / 2;
         // the same comparisons, hence the same permutation, as nth_element over
         // triangle indices compared by centroid (KdTree.h:226-229)
-        std::nth_element(items.begin() + lo, items.begin() + mid, items.begin() + hi,
+        std::nth_element(items + lo, items + mid, items + hi,
                          [axis](const Item& x, const Item& y) { return x.c[axis] < y.c[axis]; });
         const int64_t left = me + 1;
         const int64_t right = me + 1 + subtreeNodes(mid - lo);
@@ -351,8 +416,9 @@ int buildHostBvhInto(const double* verts, int64_t nv, const int32_t* tris, int64
         if (tris[q] < 0 || tris[q] >= nv)
             throw Error(SDFO_ERR_INVALID_ARGUMENT, "triangle references invalid vertex");
     // per-triangle boxes and centroids (in slot-order items), in parallel chunks
-    std::vector<double> box(size_t(n) * 6);
-    std::vector<Item> items(static_cast<size_t>(n));
+    ScratchLease lease;
+    double* box = lease.s->box.get(size_t(n) * 6);
+    Item* items = lease.s->items.get(size_t(n));
     parallelFor(
         n,
         [&](int t0, int t1) {
@@ -419,6 +485,45 @@ void setRayDir(HostBvh& h)
     const double d0[3] = {1.0, 0.3713906763541037, 0.52999894000318};
     const double len = std::sqrt(d0[0] * d0[0] + d0[1] * d0[1] + d0[2] * d0[2]);
     for (int k = 0; k < 3; ++k) h.rayDir[k] = d0[k] / len;
+}
+
+namespace {
+// The output arrays of the calling thread's scratch builds.
+// Only the top-level dense and sparse builds of a user thread use it, and
+// those never run as pool tasks, so it is never in use twice on one thread.
+struct BvhOut
+{
+    GrowBuffer<BvhNode> nodes;
+    GrowBuffer<double> slotTri9;
+    GrowBuffer<int32_t> slotTri;
+};
+thread_local BvhOut t_bvhOut;
+} // namespace
+
+HostBvhView buildHostBvhScratch(const double* verts, int64_t nv, const int32_t* tris, int64_t nt)
+{
+    HostBvhView v{};
+    const size_t nn = size_t(bvhNodeCount(nt));
+    BvhNode* nodes = t_bvhOut.nodes.get(std::max<size_t>(1, nn));
+    double* tri9 = t_bvhOut.slotTri9.get(std::max<size_t>(1, size_t(nt) * 9));
+    int32_t* slot = t_bvhOut.slotTri.get(std::max<size_t>(1, size_t(nt)));
+    v.depth = buildHostBvhInto(verts, nv, tris, nt, nodes, tri9, slot, 0, 0, true);
+    v.nodes = nodes;
+    v.nNodes = int64_t(nn);
+    v.slotTri9 = tri9;
+    v.slotTri = slot;
+    v.nSlots = nt;
+    HostBvh dir;
+    setRayDir(dir);
+    for (int k = 0; k < 3; ++k) v.rayDir[k] = dir.rayDir[k];
+    return v;
+}
+
+void releaseHostBvhScratch(int64_t keepTris)
+{
+    t_bvhOut.nodes.trim(size_t(bvhNodeCount(keepTris)));
+    t_bvhOut.slotTri9.trim(size_t(keepTris) * 9);
+    t_bvhOut.slotTri.trim(size_t(keepTris));
 }
 
 HostBvh buildHostBvh(const double* verts, int64_t nv, const int32_t* tris, int64_t nt)

@@ -83,6 +83,22 @@ int buildHostBvhInto(const double* verts, int64_t nv, const int32_t* tris, int64
                      bool parallel);
 int64_t bvhNodeCount(int64_t nt);
 void setRayDir(HostBvh& h);
+// The same tree in the calling thread's reusable host buffers (valid until
+// that thread's next scratch build): no zero fill, no fresh pages per build.
+struct HostBvhView
+{
+    const BvhNode* nodes;
+    int64_t nNodes;
+    const double* slotTri9;
+    const int32_t* slotTri;
+    int64_t nSlots;
+    int depth;
+    double rayDir[3];
+};
+HostBvhView buildHostBvhScratch(const double* verts, int64_t nv, const int32_t* tris, int64_t nt);
+// Frees the calling thread's scratch arrays if they hold more than keepTris
+// triangles' worth.
+void releaseHostBvhScratch(int64_t keepTris);
 // A task for the library's persistent host worker threads.  poolWait runs
 // queued tasks until t is done, then rethrows what t threw.
 struct PoolTask

@@ -73,6 +73,32 @@ def test_host_bvh_matches_oracle_tree(O, mk):
     assert depth + 2 <= 32
 
 
+def test_host_bvh_concurrent_builds_match():
+    """Builds on several threads at once (each thread reuses its own host
+    scratch between builds, and the worker pool interleaves their subtree
+    tasks) give the single-threaded trees, build after build."""
+    import threading
+
+    ms = [meshes.torus(0.05, 0.01, 64, 32), meshes.gear(40, 12, 0.02, 0.002, 12),
+          meshes.hollow_cylinder(0.04, 0.03, 0.05, 256, 32), meshes.cube(0.1)]
+    ref = [_lib.debug_bvh_export(v, t) for v, t in ms]
+    bad = []
+
+    def run(k):
+        for r in range(6):
+            m = (k + r) % len(ms)
+            got = _lib.debug_bvh_export(*ms[m])
+            if not all(np.array_equal(a, b) for a, b in zip(got[:3], ref[m][:3])):
+                bad.append((k, r, m))
+
+    th = [threading.Thread(target=run, args=(k,)) for k in range(4)]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join()
+    assert not bad
+
+
 def test_empty_mesh_is_rejected_without_device():
     with pytest.raises(sdfo.SdfoError, match="Empty mesh"):
         sdfo.setup_grid(np.zeros((0, 3)))
