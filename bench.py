@@ -391,6 +391,21 @@ def bench_batched_gears(args):
     print(json.dumps(line))
 
 
+def _random_read_floor(nq, shape, q_ms):
+    """The memory system's floor for the record-path query: nq uniformly
+    random 128-byte reads from an array the size of the grid's cell records
+    (and the same count of 64-byte reads), best of 10 launches with fresh
+    random records each."""
+    from sdfo import _lib
+
+    cells = int(np.prod(np.asarray(shape, np.int64) - 1))
+    t128 = _lib.debug_random_read_floor(128, nq, cells * 128, 10)
+    t64 = _lib.debug_random_read_floor(64, nq, cells * 64, 10)
+    return {"reads": nq, "array_mb": round(cells * 128 / 1e6, 1), "read_128B_us": round(t128, 2),
+            "read_64B_us": round(t64, 2), "query_us": round(q_ms * 1e3, 2),
+            "query_over_128B_floor": round(q_ms * 1e3 / t128, 3)}
+
+
 def _host_api_latency(grid, pts):
     """Round trip of the per-point C++ API path (SdfOracle::query and
     queryBatch call sdfo_query_host_f64: copy in, FP64 kernel, copy out,
@@ -666,6 +681,7 @@ def main():
                                           "1900 flops per (pose, point) evaluation"),
         "tuner": sdfo.tuner_report(),
         "host_api_latency_us": _host_api_latency(cube_grid, sets_np[0]),
+        "query_random_read_floor": _random_read_floor(nq, spec.shape, q_ms),
     }
     tr = _traffic(nq, spec.shape)
     if tr is not None:

@@ -63,6 +63,15 @@ int sdfo_debug_ca_merge_cap(int32_t first, int32_t second);
  * per-corner gather path against the CPU oracle on small grids. */
 int sdfo_debug_skip_cell_records(int32_t on);
 
+/* Measurement support: the best time in microseconds (best_us) over reps
+ * launches (after one warm-up) of n_reads uniformly random reads of
+ * record_bytes-byte records (64 or 128) from a device array of array_bytes,
+ * two per lane in 128-lane blocks like the record-path query, each launch
+ * with its own random records.  It allocates and frees the array; the
+ * current device, synchronous. */
+int sdfo_debug_random_read_floor(int32_t record_bytes, int64_t n_reads, int64_t array_bytes,
+                                 int32_t reps, double* best_us);
+
 #ifdef __cplusplus
 }
 #endif

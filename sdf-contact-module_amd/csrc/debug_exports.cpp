@@ -53,3 +53,22 @@ extern "C" int sdfo_debug_skip_cell_records(int32_t on)
     sdfo::setSkipCellRecords(on != 0);
     return SDFO_OK;
 }
+
+extern "C" int sdfo_debug_random_read_floor(int32_t record_bytes, int64_t n_reads,
+                                            int64_t array_bytes, int32_t reps, double* best_us)
+{
+    try
+    {
+        if (!best_us) return SDFO_ERR_INVALID_ARGUMENT;
+        *best_us = sdfo::randomRecordReadUs(record_bytes, n_reads, array_bytes, reps);
+        return SDFO_OK;
+    }
+    catch (const sdfo::Error& e)
+    {
+        return e.code;
+    }
+    catch (...)
+    {
+        return SDFO_ERR_INVALID_ARGUMENT;
+    }
+}

@@ -405,6 +405,97 @@ void launchQuery(const sdfo_grid& g, int mode, const TP* pts, int64_t n, TP* gap
 
 } // namespace
 
+namespace {
+// The memory system's floor for the record-path query: every lane reads two
+// uniformly random records of F4 float4 (the query's 128-byte cell record:
+// F4 = 8) from an array, in 128-lane blocks as queryPairKernel, and keeps
+// only a sum of their words.
+__device__ __forceinline__ uint32_t mixBits(uint32_t x)
+{
+    x ^= x >> 16;
+    x *= 0x7feb352dU;
+    x ^= x >> 15;
+    x *= 0x846ca68bU;
+    x ^= x >> 16;
+    return x;
+}
+
+template <int F4>
+__global__ void __launch_bounds__(kPairBlock)
+randomRecordReadKernel(const float4* __restrict__ a, uint32_t nrec, uint32_t seed, int64_t n,
+                       float* __restrict__ out)
+{
+    const int64_t i0 = int64_t(blockIdx.x) * (2 * kPairBlock) + threadIdx.x;
+    float4 v[2][F4];
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+    {
+        const uint32_t r = mixBits(uint32_t(i0 + h * kPairBlock) * 2654435761u + seed) % nrec;
+#pragma unroll
+        for (int q = 0; q < F4; ++q) v[h][q] = a[size_t(r) * F4 + q];
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+    {
+        float sum = 0.f;
+#pragma unroll
+        for (int q = 0; q < F4; ++q) sum += (v[h][q].x + v[h][q].y) + (v[h][q].z + v[h][q].w);
+        const int64_t i = i0 + h * kPairBlock;
+        if (i < n) out[i] = sum;
+    }
+}
+} // namespace
+
+double randomRecordReadUs(int recordBytes, int64_t nReads, int64_t arrayBytes, int reps)
+{
+    if ((recordBytes != 64 && recordBytes != 128) || nReads <= 0 || reps <= 0 ||
+        arrayBytes < recordBytes || arrayBytes / recordBytes > int64_t(0xffffffffu))
+        throw Error(SDFO_ERR_INVALID_ARGUMENT, "random read floor: bad sizes");
+    const uint32_t nrec = uint32_t(arrayBytes / recordBytes);
+    float4* a = nullptr;
+    float* out = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    double best = 1e30;
+    try
+    {
+        SDFO_HIP(hipMalloc(&a, size_t(nrec) * size_t(recordBytes)));
+        SDFO_HIP(hipMalloc(&out, sizeof(float) * size_t(nReads)));
+        SDFO_HIP(hipMemset(a, 0, size_t(nrec) * size_t(recordBytes)));
+        SDFO_HIP(hipEventCreate(&e0));
+        SDFO_HIP(hipEventCreate(&e1));
+        const dim3 grid(unsigned((nReads + 2 * kPairBlock - 1) / (2 * kPairBlock))), block(kPairBlock);
+        for (int r = 0; r <= reps; ++r) // the first launch warms up
+        {
+            SDFO_HIP(hipEventRecord(e0, nullptr));
+            if (recordBytes == 128)
+                hipLaunchKernelGGL(randomRecordReadKernel<8>, grid, block, 0, nullptr, a, nrec,
+                                   1234u + 7919u * uint32_t(r), nReads, out);
+            else
+                hipLaunchKernelGGL(randomRecordReadKernel<4>, grid, block, 0, nullptr, a, nrec,
+                                   1234u + 7919u * uint32_t(r), nReads, out);
+            SDFO_HIP(hipGetLastError());
+            SDFO_HIP(hipEventRecord(e1, nullptr));
+            SDFO_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            SDFO_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (r > 0 && ms * 1e3 < best) best = ms * 1e3;
+        }
+    }
+    catch (...)
+    {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (a) (void)hipFree(a);
+        if (out) (void)hipFree(out);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(out);
+    return best;
+}
+
 void launchQueryF32(const sdfo_grid& g, int mode, const float* pts, int64_t n, float* gap,
                     float* grad3, uint8_t* valid, float* normal3, float* witness3,
                     int32_t* branch, hipStream_t s)

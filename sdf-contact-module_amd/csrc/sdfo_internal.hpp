@@ -488,6 +488,12 @@ void launchCandidatesF64(const sdfo_grid& g, const double* pts, int64_t n, uint8
                          double* cgap, double* cn3, double* cw3, int32_t* cb, double* gap,
                          double* normal3, double* witness3, uint8_t* valid, hipStream_t s);
 
+// Best time (us) over reps launches of nReads uniformly random reads of
+// recordBytes-byte records (64 or 128) from an arrayBytes array, two per
+// lane in 128-lane blocks (query.hip): the memory system's floor for the
+// record-path query.  Synchronous, on the null stream.
+double randomRecordReadUs(int recordBytes, int64_t nReads, int64_t arrayBytes, int reps);
+
 // Contact launchers (contact.hip).
 // With saved (device, contactSavedBytes(n, nb) bytes; plain forward only)
 // the forward also records which (pose, point) pairs are in contact, for

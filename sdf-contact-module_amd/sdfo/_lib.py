@@ -168,6 +168,7 @@ _sig("sdfo_contact_backward_saved", [_vp, _vp, _vp, _i64, _vp, _i64, _P(ContactP
                                      _i64, _vp, _vp, _vp, _vp, _vp])
 
 _sig("sdfo_debug_bvh_export", [_vp, _i64, _vp, _i64, _vp, _vp, _vp, _i32, _vp])
+_sig("sdfo_debug_random_read_floor", [_i32, _i64, _i64, _i32, _P(C.c_double)])
 _sig("sdfo_debug_sort_by_abs", [_vp, _i32, _vp])
 _sig("sdfo_debug_build_phases", [_vp, _i32])
 _sig("sdfo_debug_force_replay", [_i32])
@@ -188,6 +189,15 @@ def debug_ca_merge_cap(first: int = 0, second: int = 0):
     """Test switch: merged entries per voxel in the contact-aware first pass (1, 4, 8)
     and in the second pass's first launch (0 = defaults)."""
     check(lib.sdfo_debug_ca_merge_cap(int(first), int(second)))
+
+
+def debug_random_read_floor(record_bytes: int, n_reads: int, array_bytes: int, reps: int = 10):
+    """Best microseconds for n_reads uniformly random record reads (the
+    record-path query's memory floor; measurement support)."""
+    out = C.c_double()
+    check(lib.sdfo_debug_random_read_floor(int(record_bytes), int(n_reads), int(array_bytes),
+                                           int(reps), C.byref(out)))
+    return out.value
 
 
 def debug_skip_cell_records(on: bool):
