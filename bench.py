@@ -71,6 +71,15 @@ def _traffic(nq, shape):
                  t["write_kib_reported"] * 1024 * t["write_factor"])
 
 
+def _leg_traffic(name, nq):
+    """HBM bytes per launch of a leg's query kernel from its committed
+    counter summary, when the point count is the one it was measured on."""
+    t = _json(os.path.join("profiles", name))
+    if not t or t.get("workload", {}).get("query_points") != nq:
+        return None
+    return float(t["bytes_per_launch"])
+
+
 def _valu_roofline(flops, seconds, kernel, basis):
     ach = flops / seconds / 1e12
     return {"bound": "valu", "achieved": round(ach, 3), "peak": FP64_PEAK_TFLOPS,
@@ -242,7 +251,8 @@ def sparse_cylinder_leg(args, steps):
         "contact_aware_query_kernel_ms": round(ca_q_ms, 4),
         "contact_aware_query_mpts_per_s": round(n / (ca_q_ms * 1e-3) / 1e6, 3),
         "roofline": _hbm_roofline(alg, q_ms * 1e-3, "queryPairKernel<float, float, 3> "
-                                  "(ContactAware -> sparse Tricubic on sparse cell records)"),
+                                  "(ContactAware -> sparse Tricubic on sparse cell records)",
+                                  _leg_traffic("query_sparse_tube512_1M_traffic.json", n)),
     }
     if work:
         out["build_roofline"] = _valu_roofline(
@@ -353,7 +363,8 @@ def batched_gears_leg(args, steps):
         "batched_query_graph_ms": round(q_ms, 4),
         "batched_query_mpts_per_s": round(n / (q_ms * 1e-3) / 1e6, 3),
         "query_roofline": _hbm_roofline(alg, q_ms * 1e-3, "queryBatchedKernel (Tricubic plan, "
-                                        "one graph replay)"),
+                                        "one graph replay)",
+                                        _leg_traffic("query_batched_gears_traffic.json", n)),
         "tuner": sdfo.tuner_report(),
     }
     if work:
