@@ -451,7 +451,10 @@ int sdfo_contact_backward(const sdfo_grid* grid, const double* X, const double* 
  * sdfo_contact_forward (same outputs, bit for bit) that also writes the
  * record into saved (device, saved_bytes = sdfo_contact_saved_bytes(n,
  * n_poses)).  sdfo_contact_backward_saved is sdfo_contact_backward reading
- * the record instead of screening every pair again; it must get the same
+ * the record instead of screening every pair again.  The record also holds
+ * every buffer of the two calls (about 37 bytes per (pose, point), the point
+ * gradients' partials for up to 64 MiB of poses at a time), so neither call
+ * allocates device memory; it must get the same
  * grid, X, w, poses and params as the forward that wrote the record (the
  * backward re-evaluates the recorded pairs exactly, but pairs in contact
  * under other inputs and absent from the record would be missed).  Its

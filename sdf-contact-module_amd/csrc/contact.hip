@@ -10,6 +10,7 @@
 // terms instead of storing them.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <mutex>
 #include <string>
 
@@ -1073,6 +1074,20 @@ __global__ void contactEvaluateFinalize(int64_t npairs, int chunks, const double
     if (active) active[b] = on ? int32_t(s[6]) : 0;
 }
 
+// Compute units of the current device, looked up once per device.
+int computeUnits()
+{
+    static std::atomic<int> cached[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    int c = cached[dev].load(std::memory_order_relaxed);
+    if (c > 0) return c;
+    if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || c < 1)
+        c = 256;
+    cached[dev].store(c, std::memory_order_relaxed);
+    return c;
+}
+
 Law makeLaw(const sdfo_contact_params& p)
 {
     return Law{p.kn, p.damping, p.mu, p.regularizer, p.narrow_margin, p.order, p.mode,
@@ -1098,7 +1113,7 @@ struct Scratch
     hipStream_t s;
     explicit Scratch(size_t bytes, hipStream_t st) : s(st)
     {
-        SDFO_HIP(hipMallocAsync(&p, bytes, s));
+        if (bytes > 0) SDFO_HIP(hipMallocAsync(&p, bytes, s));
     }
     ~Scratch()
     {
@@ -1110,31 +1125,70 @@ struct Scratch
 
 namespace {
 
-// Layout of the pair record a forward keeps for its backward: counts
-// (nb x chunks int32), lists (nb x chunks x chunkLen int32), flags (nb x n
-// bytes), each 256-byte aligned, with the chunking both passes use.
-struct SavedLayout
+// The backward's buffers for n points and nb poses.  Point gradients need
+// one partial per (pose, point) of a pose group; groups are bounded so that
+// the per-(pose, point) arrays stay under groupBytes.  lists / counts / flags
+// cover the group (screening backward) or every pose (a forward's record);
+// the piece table, piece sums, pose starts and per-pose terms cover one
+// group, the forward partials every pose.
+struct ContactLayout
 {
-    int chunks;
-    int64_t chunkLen;
-    size_t oCount, oList, oAct, bytes;
+    int chunks = 1, maxPieces = 1;
+    int64_t chunkLen = 0, group = 0;
+    size_t oCount = 0, oList = 0, oAct = 0, oPart = 0, oItems = 0, oStart = 0, oTotal = 0,
+           oSum = 0, oPer = 0, oPts = 0, bytes = 0;
 };
 
-SavedLayout savedLayout(int64_t n, int64_t nb)
+ContactLayout contactLayout(int64_t n, int64_t nb, bool wantPoint, bool record)
 {
-    SavedLayout o{};
+    ContactLayout o;
     o.chunks = n > 0 ? chunksFor(n, nb) : 1;
     o.chunkLen = n > 0 ? (n + o.chunks - 1) / o.chunks : 0;
+    o.maxPieces = int(std::max<int64_t>(1, (o.chunkLen + kPiece - 1) / kPiece));
+    // a record keeps its point partials until the backward: 64 MiB of them
+    // per group (its lists and flags cover every pose anyway); a screening
+    // backward allocates lists, flags and partials per group for the call:
+    // 256 MiB
+    const int64_t groupBytes = int64_t(record ? 64 : 256) << 20;
+    o.group = nb;
+    if (n > 0)
+    {
+        const int64_t perPair =
+            record ? int64_t(sizeof(double4))
+                   : int64_t(sizeof(int32_t)) + (wantPoint ? int64_t(sizeof(double4)) + 1 : 0);
+        const int64_t cap = groupBytes / (n * perPair);
+        o.group = cap < 1 ? 1 : (cap < nb ? cap : nb);
+    }
     auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
-    const size_t slots = size_t(nb) * size_t(o.chunks);
-    o.oCount = 0;
-    o.oList = al(sizeof(int32_t) * slots);
-    o.oAct = o.oList + al(sizeof(int32_t) * slots * size_t(o.chunkLen));
-    o.bytes = o.oAct + al(size_t(nb) * size_t(n > 0 ? n : 0));
+    const int64_t listPoses = record ? nb : o.group;
+    const size_t listSlots = size_t(listPoses) * size_t(o.chunks);
+    const size_t pieces = size_t(o.group) * size_t(o.chunks) * size_t(o.maxPieces);
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t r = off;
+        off += al(bytes);
+        return r;
+    };
+    o.oCount = take(sizeof(int32_t) * listSlots);
+    o.oList = take(sizeof(int32_t) * listSlots * size_t(o.chunkLen));
+    o.oAct = take(wantPoint ? size_t(listPoses) * size_t(n > 0 ? n : 0) : 0);
+    o.oPart = take(record ? sizeof(double) * 7 * size_t(nb) * size_t(o.chunks) : 0);
+    o.oItems = take(sizeof(int32_t) * pieces);
+    o.oStart = take(sizeof(int32_t) * (size_t(o.group) + 1));
+    o.oTotal = take(sizeof(int32_t));
+    o.oSum = take(sizeof(double) * 9 * pieces);
+    o.oPer = take(sizeof(double) * 3 * size_t(nb));
+    o.oPts = take(wantPoint ? sizeof(double4) * size_t(o.group) * size_t(n) : 0);
+    o.bytes = off;
     return o;
 }
 
-PairList savedPairs(void* saved, const SavedLayout& o)
+// The record of contactForwardSaveKernel for its backward: the pairs in
+// contact of every pose plus every buffer the backward uses, so that neither
+// pass allocates.
+ContactLayout savedLayout(int64_t n, int64_t nb) { return contactLayout(n, nb, n > 0, true); }
+
+PairList savedPairs(void* saved, const ContactLayout& o)
 {
     char* b = static_cast<char*>(saved);
     return PairList{reinterpret_cast<int32_t*>(b + o.oList), reinterpret_cast<int32_t*>(b + o.oCount),
@@ -1150,8 +1204,12 @@ void forwardLaunch(const QGrid& q, const Law& L, const double* X, const double* 
 {
     const int chunks = n > 0 ? chunksFor(n, nb) : 1;
     const int64_t chunkLen = n > 0 ? (n + chunks - 1) / chunks : 0;
-    Scratch part(sizeof(double) * 7 * size_t(nb) * chunks + (cull ? size_t(nb) : 0), s);
-    double* partial = static_cast<double*>(part.p);
+    // a recording forward keeps its chunk partials in the record (no allocation)
+    const bool rec = saved && !Cached && !cull;
+    const ContactLayout lay = rec ? savedLayout(n, nb) : ContactLayout{};
+    Scratch part(rec ? 0 : sizeof(double) * 7 * size_t(nb) * chunks + (cull ? size_t(nb) : 0), s);
+    double* partial = rec ? reinterpret_cast<double*>(static_cast<char*>(saved) + lay.oPart)
+                          : static_cast<double*>(part.p);
     uint8_t* anyLeaf = reinterpret_cast<uint8_t*>(partial + 7 * size_t(nb) * chunks);
     const dim3 grid(static_cast<unsigned>(chunks), static_cast<unsigned>(nb));
     if (cull)
@@ -1161,9 +1219,9 @@ void forwardLaunch(const QGrid& q, const Law& L, const double* X, const double* 
         hipLaunchKernelGGL((contactForwardKernel<Cached, true>), grid, dim3(kBlock), 0, s, q, L, X,
                            w, n, poses, chunkLen, partial, lc, *cull, anyLeaf);
     }
-    else if (saved && !Cached)
+    else if (rec)
         hipLaunchKernelGGL(contactForwardSaveKernel, grid, dim3(kBlock), 0, s, q, L, X, w, n, poses,
-                           chunkLen, partial, savedPairs(saved, savedLayout(n, nb)));
+                           chunkLen, partial, savedPairs(saved, lay));
     else
         hipLaunchKernelGGL((contactForwardKernel<Cached, false>), grid, dim3(kBlock), 0, s, q, L,
                            X, w, n, poses, chunkLen, partial, lc, CullArgs{}, nullptr);
@@ -1363,68 +1421,36 @@ void launchContactBackward(const sdfo_grid& g, const double* X, const double* w,
     const QGrid q = dev::makeQGrid(g);
     const Law L = makeLaw(p);
     const bool wantPoint = (gX || gW) && n > 0;
-    const int chunks = n > 0 ? chunksFor(n, nb) : 1;
-    const int64_t chunkLen = n > 0 ? (n + chunks - 1) / chunks : 0;
-    const int maxPieces = int((chunkLen + kPiece - 1) / kPiece) > 0 ? int((chunkLen + kPiece - 1) / kPiece) : 1;
-    // Poses per group bounded so the per-(pose, point) buffers (screen list
-    // entry, and with point gradients the point partial and flag) stay under
-    // 256 MiB.
-    int64_t group = nb;
-    if (n > 0)
-    {
-        const int64_t perPair = int64_t(sizeof(int32_t)) + (wantPoint ? int64_t(sizeof(double4)) + 1 : 0);
-        const int64_t cap = (int64_t(256) << 20) / (n * perPair);
-        group = cap < 1 ? 1 : (cap < nb ? cap : nb);
-    }
-    // per group: screen lists (point indices per (pose, chunk) slot), their
-    // counts, the piece table and its length, the piece sums, and with point
-    // gradients one flag per (pose, point) and the point partials; per pose:
-    // the (kn, c, mu) terms
-    const size_t slots = size_t(group) * size_t(chunks);
-    const size_t pieces = slots * size_t(maxPieces);
-    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
-    // with a forward's pair record the lists, counts and flags are read from
-    // it and none are allocated
-    const size_t screenSlots = saved ? 0 : slots;
-    const size_t oList = 0, oCount = oList + al(sizeof(int32_t) * screenSlots * size_t(chunkLen)),
-                 oItems = oCount + al(sizeof(int32_t) * screenSlots),
-                 oStart = oItems + al(sizeof(int32_t) * pieces),
-                 oTotal = oStart + al(sizeof(int32_t) * (size_t(group) + 1)), oSum = oTotal + 256,
-                 oPer = oSum + al(sizeof(double) * 9 * pieces),
-                 oAct = oPer + al(sizeof(double) * 3 * size_t(nb)),
-                 oPts = oAct + al(wantPoint && !saved ? size_t(group) * size_t(n) : 0),
-                 bytes = oPts + (wantPoint ? sizeof(double4) * size_t(group) * size_t(n) : 0);
-    Scratch scr(bytes, s);
-    char* base8 = static_cast<char*>(scr.p);
-    int32_t* list = reinterpret_cast<int32_t*>(base8 + oList);
-    int32_t* count = reinterpret_cast<int32_t*>(base8 + oCount);
-    int32_t* items = reinterpret_cast<int32_t*>(base8 + oItems);
-    int32_t* poseStart = reinterpret_cast<int32_t*>(base8 + oStart);
-    int32_t* total = reinterpret_cast<int32_t*>(base8 + oTotal);
-    double* pieceSum = reinterpret_cast<double*>(base8 + oSum);
-    double* perPose = reinterpret_cast<double*>(base8 + oPer);
-    uint8_t* act = wantPoint && !saved ? reinterpret_cast<uint8_t*>(base8 + oAct) : nullptr;
-    const PairList rec = saved ? savedPairs(const_cast<void*>(saved), savedLayout(n, nb)) : PairList{};
-    double4* pointPartial = wantPoint ? reinterpret_cast<double4*>(base8 + oPts) : nullptr;
+    // with a forward's record every buffer is in it (the lists, counts and
+    // flags of all poses included) and nothing is allocated; otherwise one
+    // scratch block holds them for the call
+    const ContactLayout lay = saved ? savedLayout(n, nb) : contactLayout(n, nb, wantPoint, false);
+    const int chunks = lay.chunks, maxPieces = lay.maxPieces;
+    const int64_t chunkLen = lay.chunkLen, group = lay.group;
+    Scratch scr(saved ? 0 : lay.bytes, s);
+    char* base8 = saved ? static_cast<char*>(const_cast<void*>(saved)) : static_cast<char*>(scr.p);
+    int32_t* list = reinterpret_cast<int32_t*>(base8 + lay.oList);
+    int32_t* count = reinterpret_cast<int32_t*>(base8 + lay.oCount);
+    int32_t* items = reinterpret_cast<int32_t*>(base8 + lay.oItems);
+    int32_t* poseStart = reinterpret_cast<int32_t*>(base8 + lay.oStart);
+    int32_t* total = reinterpret_cast<int32_t*>(base8 + lay.oTotal);
+    double* pieceSum = reinterpret_cast<double*>(base8 + lay.oSum);
+    double* perPose = reinterpret_cast<double*>(base8 + lay.oPer);
+    uint8_t* act = wantPoint ? reinterpret_cast<uint8_t*>(base8 + lay.oAct) : nullptr;
+    double4* pointPartial = wantPoint ? reinterpret_cast<double4*>(base8 + lay.oPts) : nullptr;
     const bool fused = gParams && group == nb;
-    int cus = 256;
-    {
-        int dev = 0;
-        if (hipGetDevice(&dev) == hipSuccess)
-            (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        if (cus < 1) cus = 256;
-    }
+    const int cus = computeUnits();
     for (int64_t base = 0; base < nb; base += group)
     {
         const int64_t cnt = base + group < nb ? group : nb - base;
         const dim3 grid(static_cast<unsigned>(chunks), static_cast<unsigned>(cnt));
-        // this group's slots and flags
+        // this group's slots and flags (a record holds every pose's)
         PairList gl{list, count, act};
         if (saved)
         {
-            gl.list = rec.list + base * chunks * chunkLen;
-            gl.count = rec.count + base * chunks;
-            gl.act = wantPoint ? rec.act + base * n : nullptr;
+            gl.list = list + base * chunks * chunkLen;
+            gl.count = count + base * chunks;
+            gl.act = wantPoint ? act + base * n : nullptr;
         }
         else if (L.mode == SDFO_TRILINEAR)
             hipLaunchKernelGGL(contactScreenKernel<true>, grid, dim3(kBlock), 0, s, q, L, X, n,

@@ -35,9 +35,15 @@ class ContactLaw:
     mode: str = "Tricubic"
 
     def ctypes(self) -> _lib.ContactParams:
-        return _lib.ContactParams(float(self.kn), float(self.damping), float(self.mu),
-                                  float(self.regularizer), float(self.narrow_margin),
-                                  int(self.order), int(MODES.get(self.mode, self.mode)))
+        key = (self.kn, self.damping, self.mu, self.regularizer, self.narrow_margin, self.order,
+               self.mode)
+        c = self.__dict__.get("_ctypes")
+        if c is None or c[0] != key:   # built once per set of values
+            c = (key, _lib.ContactParams(float(self.kn), float(self.damping), float(self.mu),
+                                         float(self.regularizer), float(self.narrow_margin),
+                                         int(self.order), int(MODES.get(self.mode, self.mode))))
+            self.__dict__["_ctypes"] = c
+        return c[1]
 
 
 def pose_records(R, t, M=None, m=None, v0=None, Om=None):
@@ -328,15 +334,15 @@ def contact_backward(grid: SdfGrid, X, w, poses, g_out, law: ContactLaw = Contac
     B, N = poses.shape[0], X.shape[0]
     if g_out.shape != (B, 6) or not g_out.is_cuda:
         raise ValueError("g_out must be a (B,6) GPU tensor")
-    # one allocation for all outputs, split into views
-    sizes = [6 * B if want_twist else 0, 3 if want_params else 0, 3 * N if want_X else 0,
-             N if want_w else 0]
-    buf = torch.empty(sum(sizes), dtype=torch.float64, device=X.device)
-    parts = torch.split(buf, sizes)
-    gt = parts[0].view(B, 6) if want_twist else None
-    gp = parts[1] if want_params else None
-    gx = parts[2].view(N, 3) if want_X else None
-    gw = parts[3] if want_w else None
+    # one allocation for all outputs, sliced into views
+    o1 = 6 * B if want_twist else 0
+    o2 = o1 + (3 if want_params else 0)
+    o3 = o2 + (3 * N if want_X else 0)
+    buf = torch.empty(o3 + (N if want_w else 0), dtype=torch.float64, device=X.device)
+    gt = buf[:o1].view(B, 6) if want_twist else None
+    gp = buf[o1:o2] if want_params else None
+    gx = buf[o2:o3].view(N, 3) if want_X else None
+    gw = buf[o3:] if want_w else None
     p = law.ctypes()
     if saved is not None:
         nbytes = int(lib.sdfo_contact_saved_bytes(N, B))
