@@ -1053,9 +1053,8 @@ void launchDenseFill(const GridSpec& g, const DeviceBvh& bvh, double* phi, doubl
     {
         // time every brick shape on this build (each run writes the same,
         // exact fields) and keep the fastest for the size class
-        std::vector<std::pair<int, double>> t;
-        for (int v = 0; v < kFillVariants; ++v)
-            t.emplace_back(v, timeOnStream(s, [&] { runDenseFill(g, bvh, o, v, false, s); }));
+        const std::vector<std::pair<int, double>> t = timeVariants(
+            s, kFillVariants, [&](int v) { runDenseFill(g, bvh, o, v, false, s); });
         const std::vector<std::string> names = fillVariantNames();
         std::vector<const char*> cn;
         for (const std::string& n : names) cn.push_back(n.c_str());

@@ -194,14 +194,10 @@ void runPlanTuned(const QueryPlan& plan, int mode, const float* pts, float* gap,
     int v = 2;
     if (!tunerLookup(key, v) && tunerEnabled() && !streamCapturing(s) && plan.points > 0)
     {
-        std::vector<std::pair<int, double>> t;
-        for (int q = 0; q < 4; ++q)
-        {
-            runQueryPlan(plan, mode, kPlanBlocks[q], pts, gap, grad3, valid, s); // warm
-            t.emplace_back(q, timeOnStream(s, [&] {
-                               runQueryPlan(plan, mode, kPlanBlocks[q], pts, gap, grad3, valid, s);
-                           }));
-        }
+        runQueryPlan(plan, mode, kPlanBlocks[0], pts, gap, grad3, valid, s); // warm
+        const std::vector<std::pair<int, double>> t = timeVariants(s, 4, [&](int q) {
+            runQueryPlan(plan, mode, kPlanBlocks[q], pts, gap, grad3, valid, s);
+        });
         tunerRecord(key, t, kPlanBlockNames);
         tunerLookup(key, v);
     }

@@ -363,10 +363,9 @@ void launchQuery(const sdfo_grid& g, int mode, const TP* pts, int64_t n, TP* gap
         int variant = 0;
         if (!tunerLookup(key, variant) && tunerEnabled() && !streamCapturing(s))
         {
-            std::vector<std::pair<int, double>> t;
             launch(0); // warm the first launch out of the timings
-            for (int v = 0; v < kPairBlockCount; ++v)
-                t.emplace_back(v, timeOnStream(s, [&] { launch(v); }));
+            const std::vector<std::pair<int, double>> t =
+                timeVariants(s, kPairBlockCount, [&](int v) { launch(v); });
             tunerRecord(key, t, kPairBlockNames);
             tunerLookup(key, variant);
         }

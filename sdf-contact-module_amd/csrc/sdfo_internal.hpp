@@ -375,6 +375,12 @@ int log2Bucket(int64_t n);
 bool streamCapturing(hipStream_t s);
 // Runs f (which enqueues work on s) and returns its device time in ms.
 double timeOnStream(hipStream_t s, const std::function<void()>& f);
+// Times run(v) for every variant v < n in `rounds` interleaved rounds (v = 0,
+// 1, ..., n-1, then again) and returns each variant's fastest time in ms, so
+// that clock and cache drift during the timing favours no variant.
+std::vector<std::pair<int, double>> timeVariants(hipStream_t s, int n,
+                                                 const std::function<void(int)>& run,
+                                                 int rounds = 3);
 int fillShapeFor(const GridSpec& g, int64_t ntris);
 
 // Build kernels (build.hip): the calling thread's pinned counter that the last

@@ -116,6 +116,17 @@ double timeOnStream(hipStream_t s, const std::function<void()>& f)
     return double(ms);
 }
 
+std::vector<std::pair<int, double>> timeVariants(hipStream_t s, int n,
+                                                 const std::function<void(int)>& run, int rounds)
+{
+    std::vector<std::pair<int, double>> t;
+    for (int v = 0; v < n; ++v) t.emplace_back(v, 1e30);
+    for (int r = 0; r < rounds; ++r)
+        for (int v = 0; v < n; ++v)
+            t[size_t(v)].second = std::min(t[size_t(v)].second, timeOnStream(s, [&] { run(v); }));
+    return t;
+}
+
 void tunerSetMode(int mode) { gMode.store(mode); }
 
 void tunerReset()
