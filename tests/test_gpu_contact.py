@@ -372,3 +372,26 @@ def test_saved_record_size_checked(cube):
         SC.contact_backward(g, Xd, wd, Pd, gout, law, saved=rec[:10])
     assert _lib.lib.sdfo_contact_saved_bytes(-1, 4) == -1
     assert _lib.lib.sdfo_contact_saved_bytes(len(X), 0) == 0
+
+
+def test_saved_path_edge_sizes(O, cube):
+    """One pose (30 chunks of one pose), and no points at all."""
+    g, ref = cube
+    X, w, poses = _scene(1, seed=31)
+    Xd, wd, Pd = _cuda(X, w, poses)
+    law = SC.ContactLaw(kn=1e11, damping=1e4, mu=0.3, regularizer=1e-6, order=2)
+    out, rec = SC.contact_forward(g, Xd, wd, Pd, law, save=True)
+    assert torch.equal(out, SC.contact_forward(g, Xd, wd, Pd, law))
+    gout = torch.ones((1, 6), dtype=torch.float64, device="cuda")
+    a = SC.contact_backward(g, Xd, wd, Pd, gout, law, saved=rec)
+    exp = O.contact_backward(ograd(O, g.spec), f32(ref["phi"]), f32(ref["n"]), X, w, poses,
+                             gout.cpu().numpy(), kn=1e11, c=1e4, mu=0.3, reg=1e-6, order=2)
+    for x, e in zip(a, exp):
+        assert np.abs(x.cpu().numpy() - e).max() <= 1e-7 * (np.abs(e).max() + 1e-300)
+    X0 = torch.zeros((0, 3), dtype=torch.float64, device="cuda")
+    w0 = torch.zeros(0, dtype=torch.float64, device="cuda")
+    out0, rec0 = SC.contact_forward(g, X0, w0, Pd, law, save=True)
+    assert float(out0.abs().max()) == 0.0
+    gt, gp, gx, gw = SC.contact_backward(g, X0, w0, Pd, gout, law, saved=rec0)
+    assert float(gt.abs().max()) == 0.0 and float(gp.abs().max()) == 0.0
+    assert gx.shape == (0, 3) and gw.shape == (0,)
