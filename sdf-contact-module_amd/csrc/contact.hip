@@ -341,46 +341,20 @@ __device__ __forceinline__ void forwardBlock(const QGrid& g, const Law& L, const
         acc[5] += wi * tq.z;
         acc[6] += 1.0;
     };
-    // one point per step: at 136 VGPRs (three waves per SIMD) this beat
-    // requesting two points' records before interpolating either (218 VGPRs,
-    // two waves), since a pose's points hit L2-resident records
-    if (Save)
+    // one point per step: at 128-136 VGPRs (three or four waves per SIMD)
+    // this beat requesting two points' records before interpolating either
+    // (218 VGPRs, two waves), since a pose's points hit L2-resident records
+    // the same points in the same order per thread; with Save each pair's
+    // contact flag is recorded too (pairListKernel lists them)
+    for (int64_t i = beg + threadIdx.x; i < end; i += kBlock)
     {
-        // the same points in the same order per thread, and the active
-        // pairs recorded for the backward
-        int base = 0;
-        int32_t* out = save.list + (pose * gridDim.x + chunk) * chunkLen;
-        for (int64_t i0 = beg; i0 < end; i0 += kBlock)
-        {
-            const int64_t i = i0 + threadIdx.x;
-            bool hit = false;
-            if (i < end)
-            {
-                const V x = V{X[3 * i], X[3 * i + 1], X[3 * i + 2]};
-                const Terms t = pointTerms<false, Cached>(g, P, x, L, &lc, i);
-                if (t.active)
-                {
-                    add(t, i);
-                    hit = true;
-                }
-                save.act[pose * n + i] = hit ? 1 : 0;
-            }
-            base = appendHits(hit, i, out, base, waveCount);
-        }
-        if (threadIdx.x == 0) save.count[pose * gridDim.x + chunk] = base;
-    }
-    else
-    {
-        for (int64_t i = beg + threadIdx.x; i < end; i += kBlock)
-        {
-            if (Culled && cull &&
-                !leafOverlaps(cu.leafBox + 6 * int64_t(cu.leafOf[i]), q, cu.margin))
-                continue;
-            const V x = V{X[3 * i], X[3 * i + 1], X[3 * i + 2]};
-            const Terms t = pointTerms<false, Cached>(g, P, x, L, &lc, i);
-            if (!t.active) continue;
-            add(t, i);
-        }
+        if (Culled && cull && !leafOverlaps(cu.leafBox + 6 * int64_t(cu.leafOf[i]), q, cu.margin))
+            continue;
+        const V x = V{X[3 * i], X[3 * i + 1], X[3 * i + 2]};
+        const Terms t = pointTerms<false, Cached>(g, P, x, L, &lc, i);
+        if (Save) save.act[pose * n + i] = t.active ? 1 : 0;
+        if (!t.active) continue;
+        add(t, i);
     }
     blockReduce<7>(acc, sh);
     if (threadIdx.x == 0)
@@ -391,9 +365,8 @@ __device__ __forceinline__ void forwardBlock(const QGrid& g, const Law& L, const
     }
 }
 
-// The forward kernels: four waves per SIMD (128 registers) for the plain,
-// linearised and culled passes (+10% over three waves for the 1000-pose
-// forward); the recording pass keeps three, as at 128 registers it spills.
+// The forward kernels: four waves per SIMD (128 registers; +10% over three
+// waves for the 1000-pose forward).
 template <bool Cached, bool Culled>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4)))
 contactForwardKernel(QGrid g, Law L, const double* __restrict__ X, const double* __restrict__ w,
@@ -406,22 +379,56 @@ contactForwardKernel(QGrid g, Law L, const double* __restrict__ X, const double*
                                         PairList{}, sh, nullptr);
 }
 
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4)))
 contactForwardSaveKernel(QGrid g, Law L, const double* __restrict__ X, const double* __restrict__ w,
                          int64_t n, const double* __restrict__ poses, int64_t chunkLen,
                          double* __restrict__ partial, PairList save)
 {
     __shared__ double sh[7 * kBlock];
-    __shared__ int waveCount[kBlock / 64];
     forwardBlock<false, false, true>(g, L, X, w, n, poses, chunkLen, partial, LinCache{},
-                                     CullArgs{}, nullptr, save, sh, waveCount);
+                                     CullArgs{}, nullptr, save, sh, nullptr);
 }
 
-__global__ void contactForwardFinalize(int64_t nposes, int chunks, const double* __restrict__ partial,
-                                       double* __restrict__ out, int32_t* __restrict__ active)
+__device__ __forceinline__ void finalizePose(int64_t pose, int chunks,
+                                             const double* __restrict__ partial,
+                                             double* __restrict__ out, int32_t* __restrict__ active);
+
+// After a recording forward, one launch: blocks [0, chunks * nb) compact
+// slot b's contact flags, in point order, into list[b * chunkLen ...] and
+// count[b] (slot b = pose * chunks + chunk); the rest finalise the forces of
+// kBlock poses each, as contactForwardFinalize.
+__global__ void __launch_bounds__(kBlock)
+pairListKernel(int64_t n, int64_t nb, int chunks, int64_t chunkLen, PairList pl,
+               const double* __restrict__ partial, double* __restrict__ outF,
+               int32_t* __restrict__ active)
 {
-    const int64_t pose = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (pose >= nposes) return;
+    __shared__ int waveCount[kBlock / 64];
+    const int64_t slot = blockIdx.x;
+    if (slot >= nb * chunks)
+    {
+        const int64_t pose = (slot - nb * chunks) * kBlock + threadIdx.x;
+        if (pose < nb) finalizePose(pose, chunks, partial, outF, active);
+        return;
+    }
+    const int64_t pose = slot / chunks;
+    const int64_t beg = int64_t(slot - pose * chunks) * chunkLen;
+    const int64_t end = beg + chunkLen < n ? beg + chunkLen : n;
+    int32_t* out = pl.list + slot * chunkLen;
+    int base = 0;
+    for (int64_t i0 = beg; i0 < end; i0 += kBlock)
+    {
+        const int64_t i = i0 + threadIdx.x;
+        const bool hit = i < end && pl.act[pose * n + i] != 0;
+        base = appendHits(hit, i, out, base, waveCount);
+    }
+    if (threadIdx.x == 0) pl.count[slot] = base;
+}
+
+// Pose `pose`'s forces: its chunk partials summed in chunk order.
+__device__ __forceinline__ void finalizePose(int64_t pose, int chunks,
+                                             const double* __restrict__ partial,
+                                             double* __restrict__ out, int32_t* __restrict__ active)
+{
     double s[7] = {0, 0, 0, 0, 0, 0, 0};
     for (int c = 0; c < chunks; ++c)
     {
@@ -432,6 +439,13 @@ __global__ void contactForwardFinalize(int64_t nposes, int chunks, const double*
 #pragma unroll
     for (int k = 0; k < 6; ++k) out[6 * pose + k] = s[k];
     if (active) active[pose] = int32_t(s[6]);
+}
+
+__global__ void contactForwardFinalize(int64_t nposes, int chunks, const double* __restrict__ partial,
+                                       double* __restrict__ out, int32_t* __restrict__ active)
+{
+    const int64_t pose = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (pose < nposes) finalizePose(pose, chunks, partial, out, active);
 }
 
 // Backward per (pose, point), given a = w (gF + gT x off) -- the derivation
@@ -1220,8 +1234,18 @@ void forwardLaunch(const QGrid& q, const Law& L, const double* X, const double* 
                            w, n, poses, chunkLen, partial, lc, *cull, anyLeaf);
     }
     else if (rec)
+    {
         hipLaunchKernelGGL(contactForwardSaveKernel, grid, dim3(kBlock), 0, s, q, L, X, w, n, poses,
                            chunkLen, partial, savedPairs(saved, lay));
+        SDFO_HIP(hipGetLastError());
+        // the lists and the force finalize in one launch
+        const int64_t blocks = nb * chunks + (nb + kBlock - 1) / kBlock;
+        hipLaunchKernelGGL(pairListKernel, dim3(unsigned(blocks)), dim3(kBlock), 0, s, n, nb, chunks,
+                           chunkLen, savedPairs(saved, lay), static_cast<const double*>(partial),
+                           out, active);
+        SDFO_HIP(hipGetLastError());
+        return;
+    }
     else
         hipLaunchKernelGGL((contactForwardKernel<Cached, false>), grid, dim3(kBlock), 0, s, q, L,
                            X, w, n, poses, chunkLen, partial, lc, CullArgs{}, nullptr);
