@@ -119,16 +119,40 @@ struct LinCache
     double tref[3];
 };
 
+// How the contact kernels sample the field: kFieldAny handles every grid and
+// both modes at run time; kFieldDenseCubic is Tricubic on a dense grid with
+// cell records (one record read, no other path compiled in, which leaves the
+// kernels fewer registers).  Both give the same bits for the same grid.
+constexpr int kFieldAny = 0;
+constexpr int kFieldDenseCubic = 1;
+
+__host__ __device__ __forceinline__ int fieldKind(const QGrid& g, const Law& L)
+{
+    return (!g.sparse && g.cellRec != nullptr && L.mode == SDFO_TRICUBIC) ? kFieldDenseCubic
+                                                                          : kFieldAny;
+}
+
 // The field at Y = R X + t after the narrow filter (contacts_sdf.py:91-98,
 // 319): false where the point is filtered out or outside the field.
-template <bool WithHess>
+template <bool WithHess, int Kind = kFieldAny>
 __device__ __forceinline__ bool sampleField(const QGrid& g, const Law& L, V Y, Interp<double>& it)
 {
     if (L.margin > 0.0 && !(fabs(Y.z) < L.margin)) return false;
     const Cell c = dev::locateCell(g, Y.x, Y.y, Y.z);
     if (!c.valid) return false;
-    return L.mode == SDFO_TRILINEAR ? dev::trilinear<double, WithHess>(g, c, it)
-                                    : dev::tricubic<double, WithHess>(g, c, it);
+    if constexpr (Kind == kFieldDenseCubic)
+    {
+        const float4* r = g.cellRec + 8 * (int64_t(c.i0) + int64_t(g.n[0] - 1) *
+                                                               (c.j0 + int64_t(g.n[1] - 1) * c.k0));
+        float4 f[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) f[q] = r[q];
+        dev::tricubicCorners<double, WithHess>(g, c, f, it);
+        return true;
+    }
+    else
+        return L.mode == SDFO_TRILINEAR ? dev::trilinear<double, WithHess>(g, c, it)
+                                        : dev::tricubic<double, WithHess>(g, c, it);
 }
 
 // Forward terms of one (pose, point) pair (contacts_sdf.py:329-379); with
@@ -137,7 +161,7 @@ template <bool WithHess>
 __device__ __forceinline__ void lawTerms(Terms& t, const Pose& P, const Law& L,
                                          const Interp<double>& it);
 
-template <bool WithHess, bool Cached = false>
+template <bool WithHess, bool Cached = false, int Kind = kFieldAny>
 __device__ __forceinline__ Terms pointTerms(const QGrid& g, const Pose& P, V X, const Law& L,
                                             const LinCache* lc = nullptr, int64_t i = 0)
 {
@@ -156,7 +180,7 @@ __device__ __forceinline__ Terms pointTerms(const QGrid& g, const Pose& P, V X, 
 #pragma unroll
         for (int e = 0; e < 6; ++e) it.hess[e] = 0.0;
     }
-    else if (!sampleField<WithHess>(g, L, t.Y, it))
+    else if (!sampleField<WithHess, Kind>(g, L, t.Y, it))
         return t;
     lawTerms<WithHess>(t, P, L, it);
     return t;
@@ -309,7 +333,7 @@ __device__ __forceinline__ int appendHits(bool hit, int64_t i, int32_t* __restri
 // With Culled, points outside the pose's broad-phase leaves are skipped
 // unless no leaf overlaps (then every point takes part, contacts_sdf.py:270-271).
 // With Save, the pose's pairs in contact are recorded in save too.
-template <bool Cached, bool Culled, bool Save>
+template <bool Cached, bool Culled, bool Save, int Kind = kFieldAny>
 __device__ __forceinline__ void forwardBlock(const QGrid& g, const Law& L, const double* __restrict__ X,
                                              const double* __restrict__ w, int64_t n,
                                              const double* __restrict__ poses, int64_t chunkLen,
@@ -351,7 +375,7 @@ __device__ __forceinline__ void forwardBlock(const QGrid& g, const Law& L, const
         if (Culled && cull && !leafOverlaps(cu.leafBox + 6 * int64_t(cu.leafOf[i]), q, cu.margin))
             continue;
         const V x = V{X[3 * i], X[3 * i + 1], X[3 * i + 2]};
-        const Terms t = pointTerms<false, Cached>(g, P, x, L, &lc, i);
+        const Terms t = pointTerms<false, Cached, Kind>(g, P, x, L, &lc, i);
         if (Save) save.act[pose * n + i] = t.active ? 1 : 0;
         if (!t.active) continue;
         add(t, i);
@@ -367,7 +391,7 @@ __device__ __forceinline__ void forwardBlock(const QGrid& g, const Law& L, const
 
 // The forward kernels: four waves per SIMD (128 registers; +10% over three
 // waves for the 1000-pose forward).
-template <bool Cached, bool Culled>
+template <bool Cached, bool Culled, int Kind = kFieldAny>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4)))
 contactForwardKernel(QGrid g, Law L, const double* __restrict__ X, const double* __restrict__ w,
                      int64_t n, const double* __restrict__ poses, int64_t chunkLen,
@@ -375,18 +399,19 @@ contactForwardKernel(QGrid g, Law L, const double* __restrict__ X, const double*
                      const uint8_t* __restrict__ anyLeaf)
 {
     __shared__ double sh[7 * kBlock];
-    forwardBlock<Cached, Culled, false>(g, L, X, w, n, poses, chunkLen, partial, lc, cu, anyLeaf,
-                                        PairList{}, sh, nullptr);
+    forwardBlock<Cached, Culled, false, Kind>(g, L, X, w, n, poses, chunkLen, partial, lc, cu,
+                                              anyLeaf, PairList{}, sh, nullptr);
 }
 
+template <int Kind>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4)))
 contactForwardSaveKernel(QGrid g, Law L, const double* __restrict__ X, const double* __restrict__ w,
                          int64_t n, const double* __restrict__ poses, int64_t chunkLen,
                          double* __restrict__ partial, PairList save)
 {
     __shared__ double sh[7 * kBlock];
-    forwardBlock<false, false, true>(g, L, X, w, n, poses, chunkLen, partial, LinCache{},
-                                     CullArgs{}, nullptr, save, sh, nullptr);
+    forwardBlock<false, false, true, Kind>(g, L, X, w, n, poses, chunkLen, partial, LinCache{},
+                                           CullArgs{}, nullptr, save, sh, nullptr);
 }
 
 __device__ __forceinline__ void finalizePose(int64_t pose, int chunks,
@@ -511,7 +536,7 @@ __device__ __forceinline__ Back pointBackward(const Terms& t, const Law& L, V a,
 // the group's PairList (slots and flags counted from pose poseBase).
 constexpr double kScreenGap = 1e-9;
 
-template <bool Trilinear>
+template <bool Trilinear, int Kind = kFieldAny>
 __global__ void __launch_bounds__(kBlock)
 contactScreenKernel(QGrid g, Law L, const double* __restrict__ X, int64_t n,
                     const double* __restrict__ poses, int64_t chunkLen, int64_t poseBase,
@@ -550,6 +575,17 @@ contactScreenKernel(QGrid g, Law L, const double* __restrict__ X, int64_t n,
                         Interp<double> it;
                         ok = dev::trilinear<double, false>(g, c, it);
                         gap = it.gap;
+                    }
+                    else if constexpr (Kind == kFieldDenseCubic)
+                    {
+                        const float4* r =
+                            g.cellRec + 8 * (int64_t(c.i0) +
+                                             int64_t(g.n[0] - 1) * (c.j0 + int64_t(g.n[1] - 1) * c.k0));
+                        float4 f[8];
+#pragma unroll
+                        for (int q = 0; q < 8; ++q) f[q] = r[q];
+                        ok = true;
+                        gap = dev::tricubicValueCorners<double>(g, c, f);
                     }
                     else
                     {
@@ -618,6 +654,7 @@ pieceTableKernel(int64_t slots, int chunks, int maxPieces, const int32_t* __rest
 // listed pair whose law is inactive writes a zero point partial.  The
 // piece's nine sums are added by a fixed butterfly into pieceSum[k * 9 + q]
 // for item k, so results are bit-reproducible run to run.
+template <int Kind>
 __global__ void __launch_bounds__(64)
 contactBackwardPieceKernel(QGrid g, Law L, const double* __restrict__ X,
                            const double* __restrict__ w, int64_t n,
@@ -645,7 +682,7 @@ contactBackwardPieceKernel(QGrid g, Law L, const double* __restrict__ X,
             const V gT = V{gOut[6 * pose + 3], gOut[6 * pose + 4], gOut[6 * pose + 5]};
             double4* pp = pointPartial ? pointPartial + pl * n : nullptr;
             const V x = V{X[3 * i], X[3 * i + 1], X[3 * i + 2]};
-            const Terms t = pointTerms<true>(g, P, x, L);
+            const Terms t = pointTerms<true, false, Kind>(g, P, x, L);
             if (!t.active)
             {
                 if (pp) pp[i] = make_double4(0.0, 0.0, 0.0, 0.0);
@@ -978,7 +1015,7 @@ __global__ void evaluatePrepKernel(EvalPrep in, int64_t nb, double* __restrict__
 
 // grid = (chunks, pairs).  partial[(pair*chunks + chunk)*7 + k]: F_A, tau_A
 // (world frame) and the active count.
-template <bool Culled>
+template <bool Culled, int Kind = kFieldAny>
 __global__ void __launch_bounds__(kBlock)
 contactEvaluateKernel(QGrid g, Law L, const double* __restrict__ X, const double* __restrict__ w,
                       int64_t n, const double* __restrict__ rec, int64_t chunkLen,
@@ -1020,7 +1057,7 @@ contactEvaluateKernel(QGrid g, Law L, const double* __restrict__ X, const double
         const V Y = vadd(matvec(Rs, x), ts);
         if (!(fabs(Y.z) < L.margin)) continue; // _narrow_filter, contacts_sdf.py:91-98
         Interp<double> it;
-        if (!sampleField<false>(g, Lq, Y, it)) continue;
+        if (!sampleField<false, Kind>(g, Lq, Y, it)) continue;
         const double p = L.kn * powi(it.gap < 0.0 ? -it.gap : 0.0, L.order);
         if (!(p > 1e-30)) continue; // contacts_sdf.py:329
         const V gl = V{it.grad.x, it.grad.y, it.grad.z};
@@ -1235,8 +1272,8 @@ void forwardLaunch(const QGrid& q, const Law& L, const double* X, const double* 
     }
     else if (rec)
     {
-        hipLaunchKernelGGL(contactForwardSaveKernel, grid, dim3(kBlock), 0, s, q, L, X, w, n, poses,
-                           chunkLen, partial, savedPairs(saved, lay));
+        hipLaunchKernelGGL(contactForwardSaveKernel<kFieldAny>, grid, dim3(kBlock), 0, s, q, L, X, w,
+                           n, poses, chunkLen, partial, savedPairs(saved, lay));
         SDFO_HIP(hipGetLastError());
         // the lists and the force finalize in one launch
         const int64_t blocks = nb * chunks + (nb + kBlock - 1) / kBlock;
@@ -1321,10 +1358,21 @@ void evaluateLaunch(const sdfo_grid& g, const double* X, const double* w, int64_
         cu.boxes = boxes;
         hipLaunchKernelGGL(cullAnyKernel, dim3(unsigned(nb)), dim3(kBlock), 0, s, cu, anyLeaf);
         SDFO_HIP(hipGetLastError());
-        hipLaunchKernelGGL((contactEvaluateKernel<true>), dim3(chunks, unsigned(nb)), dim3(kBlock),
-                           0, s, q, L, X, w, n, static_cast<const double*>(rec), chunkLen, partial,
-                           cu, static_cast<const uint8_t*>(anyLeaf));
+        if (fieldKind(q, L) == kFieldDenseCubic)
+            hipLaunchKernelGGL((contactEvaluateKernel<true, kFieldDenseCubic>),
+                               dim3(chunks, unsigned(nb)), dim3(kBlock), 0, s, q, L, X, w, n,
+                               static_cast<const double*>(rec), chunkLen, partial, cu,
+                               static_cast<const uint8_t*>(anyLeaf));
+        else
+            hipLaunchKernelGGL((contactEvaluateKernel<true>), dim3(chunks, unsigned(nb)),
+                               dim3(kBlock), 0, s, q, L, X, w, n, static_cast<const double*>(rec),
+                               chunkLen, partial, cu, static_cast<const uint8_t*>(anyLeaf));
     }
+    else if (fieldKind(q, L) == kFieldDenseCubic)
+        hipLaunchKernelGGL((contactEvaluateKernel<false, kFieldDenseCubic>),
+                           dim3(chunks, unsigned(nb)), dim3(kBlock), 0, s, q, L, X, w, n,
+                           static_cast<const double*>(rec), chunkLen, partial, CullArgs{},
+                           static_cast<const uint8_t*>(nullptr));
     else
         hipLaunchKernelGGL((contactEvaluateKernel<false>), dim3(chunks, unsigned(nb)), dim3(kBlock),
                            0, s, q, L, X, w, n, static_cast<const double*>(rec), chunkLen, partial,
@@ -1464,6 +1512,7 @@ void launchContactBackward(const sdfo_grid& g, const double* X, const double* w,
     double4* pointPartial = wantPoint ? reinterpret_cast<double4*>(base8 + lay.oPts) : nullptr;
     const bool fused = gParams && group == nb;
     const int cus = computeUnits();
+    const bool dc = fieldKind(q, L) == kFieldDenseCubic;
     for (int64_t base = 0; base < nb; base += group)
     {
         const int64_t cnt = base + group < nb ? group : nb - base;
@@ -1479,6 +1528,9 @@ void launchContactBackward(const sdfo_grid& g, const double* X, const double* w,
         else if (L.mode == SDFO_TRILINEAR)
             hipLaunchKernelGGL(contactScreenKernel<true>, grid, dim3(kBlock), 0, s, q, L, X, n,
                                poses, chunkLen, base, gl);
+        else if (dc)
+            hipLaunchKernelGGL((contactScreenKernel<false, kFieldDenseCubic>), grid, dim3(kBlock), 0,
+                               s, q, L, X, n, poses, chunkLen, base, gl);
         else
             hipLaunchKernelGGL(contactScreenKernel<false>, grid, dim3(kBlock), 0, s, q, L, X, n,
                                poses, chunkLen, base, gl);
@@ -1488,11 +1540,16 @@ void launchContactBackward(const sdfo_grid& g, const double* X, const double* w,
                            static_cast<const int32_t*>(gl.count), items, poseStart, total);
         SDFO_HIP(hipGetLastError());
         // two waves per SIMD (the kernel's register budget), four SIMDs per CU
-        hipLaunchKernelGGL(contactBackwardPieceKernel, dim3(unsigned(8 * cus)), dim3(64), 0, s, q,
-                           L, X, w, n, poses, gOut, chunks, chunkLen, maxPieces, base,
-                           static_cast<const int32_t*>(gl.list), static_cast<const int32_t*>(gl.count),
-                           static_cast<const int32_t*>(items), static_cast<const int32_t*>(total),
-                           pieceSum, pointPartial);
+        auto pieces = [&](auto kindTag) {
+            hipLaunchKernelGGL(contactBackwardPieceKernel<decltype(kindTag)::value>,
+                               dim3(unsigned(8 * cus)), dim3(64), 0, s, q, L, X, w, n, poses, gOut,
+                               chunks, chunkLen, maxPieces, base,
+                               static_cast<const int32_t*>(gl.list),
+                               static_cast<const int32_t*>(gl.count),
+                               static_cast<const int32_t*>(items),
+                               static_cast<const int32_t*>(total), pieceSum, pointPartial);
+        };
+        pieces(std::integral_constant<int, kFieldAny>{});
         SDFO_HIP(hipGetLastError());
         // the tail: point reduction, pose finalize and, for a single group,
         // the parameter sum, as the blocks of one launch
