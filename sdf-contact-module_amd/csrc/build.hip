@@ -690,7 +690,7 @@ __device__ __forceinline__ void writeSparseVoxel(D3 c, const dev::SignedNearest&
 }
 
 // Sparse fill: one lane per brick voxel, brick-major (SdfBuilder.cpp:925-965);
-// 64 consecutive brick voxels per queue unit.
+// 64 brick voxels per queue unit, laid out by brickUnitVoxel.
 __global__ void __launch_bounds__(kFillBlock) __attribute__((amdgpu_waves_per_eu(5)))
 sparseFillKernel(GridSpec g, const BvhNode* __restrict__ nodes, const double* __restrict__ tri9,
                  const double* __restrict__ box6, D3 dir, double mu, int bd, int64_t total,
@@ -704,7 +704,7 @@ sparseFillKernel(GridSpec g, const BvhNode* __restrict__ nodes, const double* __
     const dev::WaveStacks ws = waveStacks(lds);
     for (int64_t u = nextUnit(queue); u < nunits; u = nextUnit(queue))
     {
-        const int64_t t0 = u * 64 + lane;
+        const int64_t t0 = dev::brickUnitVoxel(bd, u, lane);
         const int64_t t = t0 < total ? t0 : total - 1;
         int i, j, k;
         brickVoxel(bijk, bd, t, i, j, k);
@@ -748,7 +748,7 @@ sparseHeavyKernel(GridSpec g, const BvhNode* __restrict__ nodes, const double* _
     const int64_t units = heavyUnits(heavy);
     for (int64_t u = nextUnit(heavy.queue); u < units; u = nextUnit(heavy.queue))
     {
-        const int64_t t = heavy.ids[u >> 6] * 64 + (u & 63);
+        const int64_t t = dev::brickUnitVoxel(bd, heavy.ids[u >> 6], int(u & 63));
         if (t >= total) continue; // wave-uniform
         int i, j, k;
         brickVoxel(bijk, bd, t, i, j, k);

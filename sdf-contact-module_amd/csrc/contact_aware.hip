@@ -1046,7 +1046,8 @@ __device__ __forceinline__ void sparseVoxel(const SparseOut& o, int bd, int64_t 
     k = o.bijk[3 * b + 2] * bd + local / (bd * bd);
 }
 
-// Sparse candidate fill (SdfBuilder.cpp:731-803): one lane per brick voxel.
+// Sparse candidate fill (SdfBuilder.cpp:731-803): one lane per brick voxel,
+// 64 brick voxels per wave laid out by brickUnitVoxel.
 template <int K>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4)))
 sparseCandidateKernel(GridSpec g, CAParams P, CAData D, int bd, int64_t total, SparseOut o,
@@ -1054,8 +1055,9 @@ sparseCandidateKernel(GridSpec g, CAParams P, CAData D, int bd, int64_t total, S
 {
     __shared__ int stack[kStackDepth * kBlock];
     __shared__ int scanBuf[(kBlock / 64) * kScanBuf];
-    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if ((t & ~int64_t(63)) >= total) return; // whole waves only
+    const int64_t w = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if ((w & ~int64_t(63)) >= total) return; // whole waves only
+    const int64_t t = dev::brickUnitVoxel(bd, w >> 6, int(w & 63));
     const bool live = t < total;
     int i = -1, j = -1, k = -1;
     if (live)

@@ -349,6 +349,23 @@ __device__ __forceinline__ WaveStacks carveWaveStacks(unsigned char* base)
     return ws;
 }
 
+// Brick-pool index (brick-major, x fastest inside a brick of edge bd) of
+// lane `lane` of wave unit `u`.  When bd is a multiple of 4 a unit is a
+// 4x4x4 sub-brick, whose 64 lanes share more of their nearest-search work
+// than the 8x8x1 slab of 64 consecutive voxels; other edges keep 64
+// consecutive voxels per unit.
+__device__ __forceinline__ int64_t brickUnitVoxel(int bd, int64_t u, int lane)
+{
+    if ((bd & 3) != 0) return u * 64 + lane;
+    const int q = bd >> 2, subs = q * q * q;
+    const int64_t b = u / subs;
+    const int s = int(u - b * subs);
+    const int x = 4 * (s % q) + (lane & 3);
+    const int y = 4 * ((s / q) % q) + ((lane >> 2) & 3);
+    const int z = 4 * (s / (q * q)) + (lane >> 4);
+    return b * int64_t(bd) * bd * bd + x + bd * (y + bd * z);
+}
+
 __device__ __forceinline__ int uniformInt(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 __device__ __forceinline__ unsigned long long uniformMask(unsigned long long v)
