@@ -718,6 +718,24 @@ def main():
             "fwdbwd_evals_per_s": round(cpu["fwdbwd_evals_per_s"], 1),
             "cube64_demo": cpu["cube64_demo"],
         }
+    # a part left out of --parts was not timed: its figures are dropped, not
+    # reported from empty timings
+    skipped = {
+        "build": ("build_mvoxel_per_s", "build_phases_ms", "build_ms", "build_roofline"),
+        "query": ("query_kernel_ms", "query_same_points_kernel_ms", "query_same_points_mpts_per_s",
+                  "tricubic_f64_points_mpts_per_s", "roofline", "query_random_read_floor"),
+        "contact": ("contact_fwd_evals_per_s", "contact_evaluate_world_evals_per_s",
+                    "contact_linearized_evals_per_s", "linearized_vs_direct", "contact_roofline"),
+        "bwd": ("contact_fwdbwd_evals_per_s", "contact_fwdbwd_screening_backward_evals_per_s",
+                "fwdbwd_vs_plain", "fwdbwd_roofline"),
+        "plain": ("plain_fwdbwd_evals_per_s", "fwdbwd_vs_plain"),
+    }
+    for part, keys in skipped.items():
+        if part not in parts:
+            for k in keys:
+                line.pop(k, None)
+    if "query" not in parts:
+        line["value"] = None
     legs = [x for x in args.legs.split(",") if x]
     if "sparse_cylinder" in legs:
         line["sparse_cylinder"] = sparse_cylinder_leg(args, min(3, max(1, K)))

@@ -737,23 +737,9 @@ __device__ __forceinline__ D3 voxelCentre(const GridSpec& g, int i, int j, int k
 // Distance^2 from p to a triangle's box (bmin3, bmax3), as boxSq.
 __device__ __forceinline__ double triBoxSq(D3 p, const double* __restrict__ b)
 {
-    double sq = 0.0;
-    const double pc[3] = {p.x, p.y, p.z};
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-    {
-        if (pc[a] < b[a])
-        {
-            const double d = b[a] - pc[a];
-            sq += d * d;
-        }
-        else if (pc[a] > b[3 + a])
-        {
-            const double d = pc[a] - b[3 + a];
-            sq += d * d;
-        }
-    }
-    return sq;
+    double sq = dev::addAxisGap(0.0, p.x, b[0], b[3]);
+    sq = dev::addAxisGap(sq, p.y, b[1], b[4]);
+    return dev::addAxisGap(sq, p.z, b[2], b[5]);
 }
 
 __device__ __forceinline__ bool waveAny(bool v) { return __ballot(v) != 0ull; }

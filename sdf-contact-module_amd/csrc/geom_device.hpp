@@ -76,52 +76,47 @@ __device__ inline D3 closestPoint(D3 p, D3 a, D3 b, D3 c)
     return add(add(a, D3{ab.x * v, ab.y * v, ab.z * v}), D3{ac.x * w, ac.y * w, ac.z * w});
 }
 
+// One axis of the point-box distance of KdTree.h:244-264, its two branches
+// taken as selects: inside [lo, hi] the gap is 0 and sq + 0 * 0 is sq, so the
+// sum is the branchy one bit for bit, without divergent control flow.
+__device__ __forceinline__ double addAxisGap(double sq, double p, double lo, double hi)
+{
+    const double below = lo - p, above = p - hi;
+    const double d = p < lo ? below : (p > hi ? above : 0.0);
+    return sq + d * d;
+}
+
 // KdTree.h:244-264.
 __device__ __forceinline__ double boxSq(D3 p, const BvhNode& n)
 {
-    double sq = 0.0;
-    const double pc[3] = {p.x, p.y, p.z};
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-    {
-        if (pc[a] < n.bmin[a])
-        {
-            const double d = n.bmin[a] - pc[a];
-            sq += d * d;
-        }
-        else if (pc[a] > n.bmax[a])
-        {
-            const double d = pc[a] - n.bmax[a];
-            sq += d * d;
-        }
-    }
-    return sq;
+    double sq = addAxisGap(0.0, p.x, n.bmin[0], n.bmax[0]);
+    sq = addAxisGap(sq, p.y, n.bmin[1], n.bmax[1]);
+    return addAxisGap(sq, p.z, n.bmin[2], n.bmax[2]);
 }
 
-// KdTree.h:335-360 (slab test; std::max / std::min argument order).
+// KdTree.h:335-360 (slab test; std::max / std::min argument order).  The
+// swap and the early exit are selects: once an axis empties the interval the
+// result is false whatever the later axes give, as with the early return.
 __device__ __forceinline__ bool rayBox(D3 o, D3 d, const BvhNode& n)
 {
     double tmin = 0.0;
     double tmax = DBL_MAX;
+    bool hit = true;
     const double oc[3] = {o.x, o.y, o.z};
     const double dc[3] = {d.x, d.y, d.z};
 #pragma unroll
     for (int a = 0; a < 3; ++a)
     {
         const double invD = 1.0 / dc[a];
-        double t0 = (n.bmin[a] - oc[a]) * invD;
-        double t1 = (n.bmax[a] - oc[a]) * invD;
-        if (t0 > t1)
-        {
-            const double tt = t0;
-            t0 = t1;
-            t1 = tt;
-        }
-        tmin = (tmin < t0) ? t0 : tmin;
-        tmax = (t1 < tmax) ? t1 : tmax;
-        if (tmax < tmin) return false;
+        const double t0 = (n.bmin[a] - oc[a]) * invD;
+        const double t1 = (n.bmax[a] - oc[a]) * invD;
+        const bool swap = t0 > t1;
+        const double lo = swap ? t1 : t0, hi = swap ? t0 : t1;
+        tmin = (tmin < lo) ? lo : tmin;
+        tmax = (hi < tmax) ? hi : tmax;
+        hit = hit && !(tmax < tmin);
     }
-    return true;
+    return hit;
 }
 
 // KdTree.h:362-396 (Moller-Trumbore 1997, eps 1e-12).
@@ -386,23 +381,9 @@ struct WaveNearest
 // Distance^2 from p to a box given as bmin3, bmax3 (same arithmetic as boxSq).
 __device__ __forceinline__ double boxSq6(D3 p, const double* b)
 {
-    double sq = 0.0;
-    const double pc[3] = {p.x, p.y, p.z};
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-    {
-        if (pc[a] < b[a])
-        {
-            const double d = b[a] - pc[a];
-            sq += d * d;
-        }
-        else if (pc[a] > b[3 + a])
-        {
-            const double d = pc[a] - b[3 + a];
-            sq += d * d;
-        }
-    }
-    return sq;
+    double sq = addAxisGap(0.0, p.x, b[0], b[3]);
+    sq = addAxisGap(sq, p.y, b[1], b[4]);
+    return addAxisGap(sq, p.z, b[2], b[5]);
 }
 
 // The stack holds whole node records (copied from the parent's child loads),
