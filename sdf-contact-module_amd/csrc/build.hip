@@ -886,42 +886,61 @@ struct ReplayBuf
     Spill sp{nullptr, nullptr, nullptr, nullptr};
     unsigned long long* queue = nullptr; // work-queue counter of the fill launch
     hipStream_t s;
-    // cap: work items (replay ids); units: 64-lane work units (heavy list)
-    ReplayBuf(int64_t cap, hipStream_t st, int64_t units = 0) : s(st)
+    void* owned = nullptr; // the block, when this object allocated it
+    // Bytes of the one block a ReplayBuf carves: the 48-byte counter header,
+    // then the replay ids (cap items) and, for 64-lane work units, the heavy
+    // list, its seeds and the spill list (units * 64 voxels).
+    static size_t bytes(int64_t cap, int64_t units)
     {
-        SDFO_HIP(hipMallocAsync(reinterpret_cast<void**>(&queue), 48, s));
+        auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+        size_t b = up(48) + up(sizeof(int64_t) * size_t(std::max<int64_t>(cap, 1)));
+        if (units > 0)
+            b += up(sizeof(int64_t) * size_t(units)) + 3 * up(sizeof(double) * 64 * size_t(units));
+        return b;
+    }
+    // cap: work items (replay ids); units: 64-lane work units (heavy list).
+    // The lists come from `mem` (at least bytes(cap, units), 256-byte aligned,
+    // not in use by other work on the stream) or from one allocation.
+    ReplayBuf(int64_t cap, hipStream_t st, int64_t units = 0, void* mem = nullptr) : s(st)
+    {
+        if (!mem)
+        {
+            SDFO_HIP(hipMallocAsync(&owned, bytes(cap, units), s));
+            mem = owned;
+        }
+        auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+        char* p = static_cast<char*>(mem);
+        queue = reinterpret_cast<unsigned long long*>(p);
         SDFO_HIP(hipMemsetAsync(queue, 0, 48, s));
         r.count = reinterpret_cast<int*>(queue + 1);
         h.queue = queue + 2;
         h.count = reinterpret_cast<int*>(queue + 3);
         sp.queue = queue + 4;
         sp.count = reinterpret_cast<int*>(queue + 5);
-        SDFO_HIP(hipMallocAsync(reinterpret_cast<void**>(&r.ids),
-                                sizeof(int64_t) * size_t(std::max<int64_t>(cap, 1)), s));
+        p += up(48);
+        r.ids = reinterpret_cast<int64_t*>(p);
+        p += up(sizeof(int64_t) * size_t(std::max<int64_t>(cap, 1)));
         h.cap = int(std::min<int64_t>(units, INT32_MAX));
         if (units > 0)
         {
-            SDFO_HIP(hipMallocAsync(reinterpret_cast<void**>(&h.ids), sizeof(int64_t) * size_t(units), s));
-            SDFO_HIP(hipMallocAsync(reinterpret_cast<void**>(&h.seeds), sizeof(double) * 64 * size_t(units), s));
-            SDFO_HIP(hipMallocAsync(reinterpret_cast<void**>(&sp.ids), sizeof(int64_t) * 64 * size_t(units), s));
-            SDFO_HIP(hipMallocAsync(reinterpret_cast<void**>(&sp.seeds), sizeof(double) * 64 * size_t(units), s));
+            const size_t lane = up(sizeof(double) * 64 * size_t(units));
+            h.ids = reinterpret_cast<int64_t*>(p);
+            p += up(sizeof(int64_t) * size_t(units));
+            h.seeds = reinterpret_cast<double*>(p);
+            sp.ids = reinterpret_cast<int64_t*>(p + lane);
+            sp.seeds = reinterpret_cast<double*>(p + 2 * lane);
         }
     }
     // Copies the list length to the calling thread's pinned counter (read it
     // after synchronising the stream).
     void report()
     {
-        int* h = replayCounterHost();
-        if (h) SDFO_HIP(hipMemcpyAsync(h, r.count, sizeof(int), hipMemcpyDeviceToHost, s));
+        int* hc = replayCounterHost();
+        if (hc) SDFO_HIP(hipMemcpyAsync(hc, r.count, sizeof(int), hipMemcpyDeviceToHost, s));
     }
     ~ReplayBuf()
     {
-        (void)hipFreeAsync(r.ids, s);
-        if (h.ids) (void)hipFreeAsync(h.ids, s);
-        if (h.seeds) (void)hipFreeAsync(h.seeds, s);
-        if (sp.ids) (void)hipFreeAsync(sp.ids, s);
-        if (sp.seeds) (void)hipFreeAsync(sp.seeds, s);
-        (void)hipFreeAsync(queue, s);
+        if (owned) (void)hipFreeAsync(owned, s);
     }
 };
 
@@ -1001,13 +1020,13 @@ std::vector<std::string> fillVariantNames()
 }
 
 void runDenseFill(const GridSpec& g, const DeviceBvh& bvh, const DenseFillOut& o, int variant,
-                  bool report, hipStream_t s)
+                  bool report, hipStream_t s, void* scratch)
 {
     const int shape = variant / kBudgetCount;
     const int budget = debugFillBudget() > 0 ? debugFillBudget() : kBudgets[variant % kBudgetCount];
     const BrickShape sh = BrickShape::make(g, kFillShapes[shape][0], kFillShapes[shape][1]);
     const D3 dir = dev::d3(bvh.rayDir[0], bvh.rayDir[1], bvh.rayDir[2]);
-    ReplayBuf rb(g.total(), s, sh.count());
+    ReplayBuf rb(g.total(), s, sh.count(), scratch);
     const double mu = tieMargin(g);
     hipLaunchKernelGGL(denseFillKernel, dim3(queueGrid(sh.count())), dim3(kFillBlock), 0, s, g,
                        bvh.nodes, bvh.tri9, bvh.box6, dir, mu, sh, o, rb.r, rb.h, rb.queue, budget,
@@ -1043,8 +1062,17 @@ int fillShapeFor(const GridSpec& g, int64_t ntris)
     return 0;
 }
 
+size_t denseFillScratchBytes(const GridSpec& g)
+{
+    size_t b = 0;
+    for (int q = 0; q < kFillShapeCount; ++q)
+        b = std::max(b, ReplayBuf::bytes(g.total(),
+                                         BrickShape::make(g, kFillShapes[q][0], kFillShapes[q][1]).count()));
+    return b;
+}
+
 void launchDenseFill(const GridSpec& g, const DeviceBvh& bvh, double* phi, double* n3,
-                     double* witness3, int32_t* tri, hipStream_t s)
+                     double* witness3, int32_t* tri, hipStream_t s, void* scratch)
 {
     const DenseFillOut o{bvh.triIdx, phi, n3, witness3, tri};
     const std::string key = fillKey(g, bvh.ntris);
@@ -1054,7 +1082,7 @@ void launchDenseFill(const GridSpec& g, const DeviceBvh& bvh, double* phi, doubl
         // time every brick shape on this build (each run writes the same,
         // exact fields) and keep the fastest for the size class
         const std::vector<std::pair<int, double>> t = timeVariants(
-            s, kFillVariants, [&](int v) { runDenseFill(g, bvh, o, v, false, s); });
+            s, kFillVariants, [&](int v) { runDenseFill(g, bvh, o, v, false, s, scratch); });
         const std::vector<std::string> names = fillVariantNames();
         std::vector<const char*> cn;
         for (const std::string& n : names) cn.push_back(n.c_str());
@@ -1062,7 +1090,7 @@ void launchDenseFill(const GridSpec& g, const DeviceBvh& bvh, double* phi, doubl
         tunerLookup(key, shape);
     }
     if (shape < 0 || shape >= kFillVariants) shape = 0;
-    runDenseFill(g, bvh, o, shape, true, s);
+    runDenseFill(g, bvh, o, shape, true, s, scratch);
 }
 
 void launchDenseFillBatched(const std::vector<BatchMesh>& meshes, const DeviceBvh& bvh,

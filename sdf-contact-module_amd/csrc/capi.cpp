@@ -104,6 +104,57 @@ void finishDense(sdfo_grid& g, hipStream_t s, bool mirrored)
     }
 }
 
+int64_t denseCells(const GridSpec& sp)
+{
+    return int64_t(sp.n[0] - 1) * (sp.n[1] - 1) * (sp.n[2] - 1);
+}
+
+// Whether a dense grid of this shape keeps cell records (the calling
+// thread's switch and the device's record limit).
+bool denseRecords(const GridSpec& sp)
+{
+    const int64_t cells = denseCells(sp);
+    return cells > 0 && cellRecordsFit(cells * 8 * int64_t(sizeof(float4)));
+}
+
+// Device bytes of a dense grid's arrays carved from one block: the FP64
+// fields, the FP32 mirrors and, with `records`, the cell records.
+size_t denseGridBytes(const GridSpec& sp, bool records)
+{
+    const size_t N = size_t(sp.total());
+    size_t b = 0;
+    for (size_t per : {sizeof(double), 3 * sizeof(double), 6 * sizeof(double), 3 * sizeof(double),
+                       sizeof(int32_t), sizeof(float4), 6 * sizeof(float)})
+        b += sdfo_grid::arenaBytes(per * N);
+    if (records) b += sdfo_grid::arenaBytes(size_t(denseCells(sp)) * 8 * sizeof(float4));
+    return b;
+}
+
+// Carves the arrays denseGridBytes counts from g's block, in the same order.
+void carveDenseGrid(sdfo_grid& g, bool records)
+{
+    const int64_t N = g.spec.total();
+    g.phi = g.alloc<double>(size_t(N));
+    g.n3 = g.alloc<double>(size_t(N) * 3);
+    g.h6 = g.alloc<double>(size_t(N) * 6);
+    g.witness3 = g.alloc<double>(size_t(N) * 3);
+    g.tri = g.alloc<int32_t>(size_t(N));
+    // the FP32 mirrors and cell records of finishDense
+    g.pn = g.alloc<float4>(size_t(N));
+    g.h6f = g.alloc<float>(size_t(N) * 6);
+    if (records) g.cellRec = g.alloc<float4>(size_t(denseCells(g.spec)) * 8);
+}
+
+// Gives g one device block of `bytes` (ordered on s) to carve its arrays from.
+void giveArena(sdfo_grid& g, size_t bytes, hipStream_t s)
+{
+    auto arena = std::make_shared<ArenaBlock>();
+    SDFO_HIP(hipMallocAsync(reinterpret_cast<void**>(&arena->base), std::max<size_t>(bytes, 256), s));
+    g.arena = arena;
+    g.arenaNext = arena->base;
+    g.arenaLeft = bytes;
+}
+
 // Brick table from the brick list: first brick wins on duplicate keys
 // (std::unordered_map::emplace in SparseSdfData::rebuildIndex, SdfData.h:172-182);
 // bricks outside the brick grid are unreachable.
@@ -246,30 +297,53 @@ std::unique_ptr<sdfo_grid> buildDense(const double* verts, int64_t nv, const int
     g->device = currentDevice();
     g->allocStream = s;
     g->spec = setupGrid(verts, nv, p);
-    const int64_t N = g->spec.total();
     ph[0] = msSince(t0);
-    const clk::time_point t1 = clk::now();
-    const HostBvhView hb = buildHostBvhScratch(verts, nv, tris, nt);
-    ph[1] = msSince(t1);
+    // A worker thread requests the device memory while this thread builds the
+    // host BVH: the grid's arrays in one block, the BVH's block and the fill
+    // passes' lists (the allocations cost host time comparable to the tree).
+    const bool records = denseRecords(g->spec);
     DeviceBvh db;
+    void* fillScratch = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    PoolTask alloc;
+    alloc.fn = [&] {
+        DeviceGuard dg(g->device);
+        keepStreamPoolMemory(g->device);
+        const size_t bytes = denseGridBytes(g->spec, records);
+        giveArena(*g, bytes, s);
+        carveDenseGrid(*g, records);
+        db.reserve(bvhNodeCount(nt), nt, s);
+        SDFO_HIP(hipMallocAsync(&fillScratch, denseFillScratchBytes(g->spec), s));
+        for (hipEvent_t& e : ev) SDFO_HIP(hipEventCreate(&e));
+    };
+    bool allocJoined = false;
+    auto joinAlloc = [&] {
+        if (allocJoined) return;
+        allocJoined = true;
+        poolWait(&alloc);
+    };
+    auto cleanUp = [&] {
+        (void)hipStreamSynchronize(s);
+        if (fillScratch) (void)hipFreeAsync(fillScratch, s);
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+        db.release();
+        // the copies from the scratch arrays finished with the stream
+        releaseHostBvhScratch(kKeepScratchTris);
+    };
     try
     {
+        poolSubmit(&alloc);
+        const clk::time_point t1 = clk::now();
+        const HostBvhView hb = buildHostBvhScratch(verts, nv, tris, nt);
+        ph[1] = msSince(t1);
         const clk::time_point t2 = clk::now();
-        keepStreamPoolMemory(g->device);
+        joinAlloc();
         db.upload(hb.nodes, hb.nNodes, hb.slotTri9, hb.slotTri, hb.nSlots, hb.depth, hb.rayDir, s);
-        g->phi = g->alloc<double>(size_t(N));
-        g->n3 = g->alloc<double>(size_t(N) * 3);
-        g->h6 = g->alloc<double>(size_t(N) * 6);
-        g->witness3 = g->alloc<double>(size_t(N) * 3);
-        g->tri = g->alloc<int32_t>(size_t(N));
-        for (hipEvent_t& e : ev) SDFO_HIP(hipEventCreate(&e));
         ph[2] = msSince(t2);
         SDFO_HIP(hipEventRecord(ev[0], s));
-        launchDenseFill(g->spec, db, g->phi, g->n3, g->witness3, g->tri, s);
+        launchDenseFill(g->spec, db, g->phi, g->n3, g->witness3, g->tri, s, fillScratch);
         SDFO_HIP(hipEventRecord(ev[1], s));
-        g->pn = g->alloc<float4>(size_t(N));
-        g->h6f = g->alloc<float>(size_t(N) * 6);
         launchDenseHessian(g->spec, g->n3, g->h6, s, g->phi, g->pn, g->h6f);
         SDFO_HIP(hipEventRecord(ev[2], s));
         finishDense(*g, s, true);
@@ -284,17 +358,18 @@ std::unique_ptr<sdfo_grid> buildDense(const double* verts, int64_t nv, const int
     }
     catch (...)
     {
-        (void)hipStreamSynchronize(s);
-        for (hipEvent_t e : ev)
-            if (e) (void)hipEventDestroy(e);
-        db.release();
-        releaseHostBvhScratch(kKeepScratchTris);
+        // the worker's allocations must be finished before they are freed
+        try
+        {
+            joinAlloc();
+        }
+        catch (...)
+        {
+        }
+        cleanUp();
         throw;
     }
-    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
-    db.release();
-    // the copies from the scratch arrays finished with the stream
-    releaseHostBvhScratch(kKeepScratchTris);
+    cleanUp();
     ph[6] = msSince(t0);
     const int* rc = replayCounterHost();
     ph[7] = rc ? double(*rc) : -1.0;
@@ -443,20 +518,7 @@ std::vector<std::unique_ptr<sdfo_grid>> buildDenseBatched(int32_t nm, const doub
     std::vector<BatchMesh> jobs(static_cast<size_t>(nm));
     const int dev = currentDevice();
     keepStreamPoolMemory(dev);
-    auto cellsOf = [](const GridSpec& sp) {
-        return int64_t(sp.n[0] - 1) * (sp.n[1] - 1) * (sp.n[2] - 1);
-    };
-    auto gridBytes = [&](const GridSpec& sp) {
-        const size_t N = size_t(sp.total());
-        const int64_t cells = cellsOf(sp);
-        size_t b = 0;
-        for (size_t per : {sizeof(double), 3 * sizeof(double), 6 * sizeof(double), 3 * sizeof(double),
-                           sizeof(int32_t), sizeof(float4), 6 * sizeof(float)})
-            b += sdfo_grid::arenaBytes(per * N);
-        if (cells > 0 && cellRecordsFit(cells * 8 * int64_t(sizeof(float4))))
-            b += sdfo_grid::arenaBytes(size_t(cells) * 8 * sizeof(float4));
-        return b;
-    };
+    auto gridBytes = [&](const GridSpec& sp) { return denseGridBytes(sp, denseRecords(sp)); };
     size_t arenaTotal = 0;
     for (int32_t m = 0; m < nm; ++m) arenaTotal += gridBytes(specs[size_t(m)]);
     char* arenaBase = nullptr;
@@ -474,18 +536,7 @@ std::vector<std::unique_ptr<sdfo_grid>> buildDenseBatched(int32_t nm, const doub
         g->arenaNext = arenaBase + arenaOff;
         g->arenaLeft = gridBytes(g->spec);
         arenaOff += g->arenaLeft;
-        const int64_t N = g->spec.total();
-        g->phi = g->alloc<double>(size_t(N));
-        g->n3 = g->alloc<double>(size_t(N) * 3);
-        g->h6 = g->alloc<double>(size_t(N) * 6);
-        g->witness3 = g->alloc<double>(size_t(N) * 3);
-        g->tri = g->alloc<int32_t>(size_t(N));
-        // the FP32 mirrors and cell records of finishDense
-        g->pn = g->alloc<float4>(size_t(N));
-        g->h6f = g->alloc<float>(size_t(N) * 6);
-        const int64_t cells = cellsOf(g->spec);
-        if (cells > 0 && cellRecordsFit(cells * 8 * int64_t(sizeof(float4))))
-            g->cellRec = g->alloc<float4>(size_t(cells) * 8);
+        carveDenseGrid(*g, denseRecords(g->spec));
         grids[size_t(m)] = std::move(g);
     }
     ph[2] = msSince(tPool); // allocations, overlapping the BVHs

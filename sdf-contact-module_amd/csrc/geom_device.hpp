@@ -119,7 +119,10 @@ __device__ __forceinline__ bool rayBox(D3 o, D3 d, const BvhNode& n)
     return hit;
 }
 
-// KdTree.h:362-396 (Moller-Trumbore 1997, eps 1e-12).
+// KdTree.h:362-396 (Moller-Trumbore 1997, eps 1e-12).  The early exits are
+// folded into one condition: every quantity is computed from the same
+// operands as in the branchy form, and a test that would have returned false
+// early makes the conjunction false whatever the later ones give.
 __device__ __forceinline__ bool rayTri(D3 o, D3 d, D3 a, D3 b, D3 c)
 {
     const double eps = 1.0e-12;
@@ -127,16 +130,14 @@ __device__ __forceinline__ bool rayTri(D3 o, D3 d, D3 a, D3 b, D3 c)
     const D3 e2 = sub(c, a);
     const D3 h = cross(d, e2);
     const double det = dot(e1, h);
-    if (fabs(det) < eps) return false;
     const double invDet = 1.0 / det;
     const D3 s = sub(o, a);
     const double u = invDet * dot(s, h);
-    if (u < -eps || u > 1.0 + eps) return false;
     const D3 q = cross(s, e1);
     const double v = invDet * dot(d, q);
-    if (v < -eps || u + v > 1.0 + eps) return false;
     const double t = invDet * dot(e2, q);
-    return t > eps;
+    return !(fabs(det) < eps) && !(u < -eps || u > 1.0 + eps) && !(v < -eps || u + v > 1.0 + eps) &&
+           t > eps;
 }
 
 struct Tri

@@ -544,11 +544,30 @@ void DeviceBvh::upload(const HostBvh& h, hipStream_t s)
            int64_t(h.slotTri.size()), h.depth, h.rayDir, s);
 }
 
+namespace {
+size_t alignUp256(size_t b) { return (b + 255) & ~size_t(255); }
+size_t bvhBlockBytes(int64_t nNodes, int64_t nSlots)
+{
+    const size_t n = size_t(std::max<int64_t>(1, nNodes)), t = size_t(std::max<int64_t>(1, nSlots));
+    return alignUp256(sizeof(BvhNode) * n) + alignUp256(sizeof(double) * 9 * t) +
+           alignUp256(sizeof(double) * 6 * t) + alignUp256(sizeof(int32_t) * t);
+}
+} // namespace
+
+void DeviceBvh::reserve(int64_t nNodes, int64_t nSlots, hipStream_t s)
+{
+    release();
+    stream = s;
+    SDFO_HIP(hipMallocAsync(&block, bvhBlockBytes(nNodes, nSlots), s));
+    capNodes = std::max<int64_t>(1, nNodes);
+    capSlots = std::max<int64_t>(1, nSlots);
+}
+
 void DeviceBvh::upload(const BvhNode* hostNodes, int64_t nNodes, const double* slotTri9,
                        const int32_t* slotTri, int64_t nSlots, int treeDepth, const double* dir,
                        hipStream_t s)
 {
-    release();
+    if (!block || stream != s || nNodes > capNodes || nSlots > capSlots) reserve(nNodes, nSlots, s);
     depth = treeDepth;
     for (int k = 0; k < 3; ++k) rayDir[k] = dir[k];
     if (depth + 2 > kStackDepth)
@@ -559,15 +578,14 @@ void DeviceBvh::upload(const BvhNode* hostNodes, int64_t nNodes, const double* s
                                               " nodes exceeds the traversal stack encoding");
     nnodes = int(nNodes);
     ntris = nSlots;
-    stream = s;
-    SDFO_HIP(hipMallocAsync(reinterpret_cast<void**>(&nodes),
-                            sizeof(BvhNode) * size_t(std::max<int64_t>(1, nNodes)), s));
-    SDFO_HIP(hipMallocAsync(reinterpret_cast<void**>(&tri9),
-                            sizeof(double) * 9 * size_t(std::max<int64_t>(1, nSlots)), s));
-    SDFO_HIP(hipMallocAsync(reinterpret_cast<void**>(&box6),
-                            sizeof(double) * 6 * size_t(std::max<int64_t>(1, nSlots)), s));
-    SDFO_HIP(hipMallocAsync(reinterpret_cast<void**>(&triIdx),
-                            sizeof(int32_t) * size_t(std::max<int64_t>(1, nSlots)), s));
+    char* b = static_cast<char*>(block);
+    nodes = reinterpret_cast<BvhNode*>(b);
+    b += alignUp256(sizeof(BvhNode) * size_t(capNodes));
+    tri9 = reinterpret_cast<double*>(b);
+    b += alignUp256(sizeof(double) * 9 * size_t(capSlots));
+    box6 = reinterpret_cast<double*>(b);
+    b += alignUp256(sizeof(double) * 6 * size_t(capSlots));
+    triIdx = reinterpret_cast<int32_t*>(b);
     SDFO_HIP(hipMemcpyAsync(nodes, hostNodes, sizeof(BvhNode) * size_t(nNodes),
                             hipMemcpyHostToDevice, s));
     SDFO_HIP(hipMemcpyAsync(tri9, slotTri9, sizeof(double) * 9 * size_t(nSlots),
@@ -579,10 +597,9 @@ void DeviceBvh::upload(const BvhNode* hostNodes, int64_t nNodes, const double* s
 
 void DeviceBvh::release()
 {
-    if (nodes) (void)hipFreeAsync(nodes, stream);
-    if (tri9) (void)hipFreeAsync(tri9, stream);
-    if (box6) (void)hipFreeAsync(box6, stream);
-    if (triIdx) (void)hipFreeAsync(triIdx, stream);
+    if (block) (void)hipFreeAsync(block, stream);
+    block = nullptr;
+    capNodes = capSlots = 0;
     nodes = nullptr;
     tri9 = nullptr;
     box6 = nullptr;

@@ -129,7 +129,13 @@ struct DeviceBvh
     int64_t ntris = 0;
     int depth = 0;
     double rayDir[3] = {0, 0, 0};
-    hipStream_t stream = nullptr; // the arrays are stream-ordered allocations on it
+    hipStream_t stream = nullptr; // the block is a stream-ordered allocation on it
+    // The four arrays share one device block; reserve() allocates it ahead of
+    // the upload (for a tree of nNodes nodes over nSlots triangle slots), so
+    // the allocation can overlap the host build of the tree.
+    void* block = nullptr;
+    int64_t capNodes = 0, capSlots = 0;
+    void reserve(int64_t nNodes, int64_t nSlots, hipStream_t s);
     void upload(const HostBvh& h, hipStream_t s);
     // The same from raw host arrays; the slot boxes are derived on the device.
     void upload(const BvhNode* hostNodes, int64_t nNodes, const double* slotTri9,
@@ -436,8 +442,12 @@ void launchSparseCellPack(const GridSpec& g, int bd, int nbx, int nby, int nbz,
                           const int32_t* bijk, const int32_t* table, const float4* spn,
                           int64_t count, int words, float4* rec, unsigned long long* mask,
                           hipStream_t s);
+// `scratch`, when given, holds denseFillScratchBytes(g) bytes (256-byte
+// aligned) that no other work on s uses meanwhile; otherwise the lists of the
+// fill passes are allocated on s.
 void launchDenseFill(const GridSpec& g, const DeviceBvh& bvh, double* phi, double* n3,
-                     double* witness3, int32_t* tri, hipStream_t s);
+                     double* witness3, int32_t* tri, hipStream_t s, void* scratch = nullptr);
+size_t denseFillScratchBytes(const GridSpec& g);
 // With pn (and h6f) set, also writes the FP32 query mirror of every voxel.
 void launchDenseHessian(const GridSpec& g, const double* n3, double* h6, hipStream_t s,
                         const double* phi = nullptr, float4* pn = nullptr, float* h6f = nullptr);
