@@ -47,6 +47,7 @@ struct Heavy
     int* count;
     int64_t* ids;  // unit ids
     double* seeds; // 64 per unit
+    int* hits;     // 64 per unit: each lane's ray parity count, made by the wave
     unsigned long long* queue;
     int cap; // units the list holds
 };
@@ -59,7 +60,8 @@ const char* const kBudgetNames[kBudgetCount] = {"1024", "512", "2048"};
 constexpr int kBatchedBudget = 8192;
 
 // Returns false when the list is full: the caller replays the unit's voxels.
-__device__ __forceinline__ bool pushHeavy(const Heavy& h, int64_t unit, bool active, double bound)
+__device__ __forceinline__ bool pushHeavy(const Heavy& h, int64_t unit, bool active, double bound,
+                                          int hits)
 {
     int slot = 0;
     if ((threadIdx.x & 63) == 0) slot = atomicAdd(h.count, 1);
@@ -67,6 +69,7 @@ __device__ __forceinline__ bool pushHeavy(const Heavy& h, int64_t unit, bool act
     if (slot >= h.cap) return false;
     if ((threadIdx.x & 63) == 0) h.ids[slot] = unit;
     h.seeds[int64_t(slot) * 64 + (threadIdx.x & 63)] = active ? bound : DBL_MAX;
+    h.hits[int64_t(slot) * 64 + (threadIdx.x & 63)] = hits;
     return true;
 }
 
@@ -76,10 +79,10 @@ __device__ __forceinline__ int64_t heavyUnits(const Heavy& h)
     return int64_t(n < h.cap ? n : h.cap) * 64;
 }
 
-// Frontier sizes: the first frontier pass runs 4 waves per block with 1000
-// entries per level each; voxels that outgrow it go to a pass with one wave
-// of 4000 entries per block.
-constexpr int kFrontierSmall = 1000, kFrontierLarge = 4000;
+// Frontier sizes: the first frontier pass runs 2 waves per block with 640
+// entries per level each (12 waves per CU fit its LDS); voxels that outgrow
+// it go to a pass with one wave of 4000 entries per block.
+constexpr int kFrontierSmall = 640, kFrontierLarge = 4000;
 
 // Voxels whose frontier overflowed, with the bound reached.
 struct Spill
@@ -87,6 +90,7 @@ struct Spill
     int* count;
     int64_t* ids; // voxel ids (dense linear index / sparse pool index)
     double* seeds;
+    int* hits; // ray parity counts
     unsigned long long* queue;
 };
 
@@ -194,12 +198,12 @@ denseFillKernel(GridSpec g, const BvhNode* __restrict__ nodes, const double* __r
         const dev::WaveNearest wn =
             dev::waveNearest(c, active, act, nodes, tri9, box6, mu, ws, lane, 0, budget);
         const int64_t idx = i + int64_t(g.n[0]) * (j + int64_t(g.n[1]) * k);
+        const int hits = dev::waveRayHits(c, dir, act, nodes, tri9, ws, lane, 0);
         if (wn.aborted)
         {
-            if (!pushHeavy(heavy, brick, active, wn.bestSq) && active) pushReplay(replay, idx);
+            if (!pushHeavy(heavy, brick, active, wn.bestSq, hits) && active) pushReplay(replay, idx);
             continue;
         }
-        const int hits = dev::waveRayHits(c, dir, act, nodes, tri9, ws, lane, 0);
         if (!active) continue;
         if (!wn.exact || force)
             pushReplay(replay, idx);
@@ -225,9 +229,10 @@ denseHeavyKernel(GridSpec g, const BvhNode* __restrict__ nodes, const double* __
         if (i >= g.n[0] || j >= g.n[1] || k >= g.n[2]) continue; // wave-uniform
         const D3 c = voxelCentre(g, i, j, k);
         double seed = heavy.seeds[u];
+        const int hits = heavy.hits[u];
         int status = 0;
         const dev::SignedNearest s =
-            dev::frontierSignedNearest(c, nodes, tri9, box6, dir, mu, 0, seed, f, status);
+            dev::frontierSignedNearest(c, nodes, tri9, box6, dir, mu, 0, seed, f, status, hits);
         if ((threadIdx.x & 63) != 0) continue;
         const int64_t idx = i + int64_t(g.n[0]) * (j + int64_t(g.n[1]) * k);
         if (status == dev::kFrontierOverflow)
@@ -235,6 +240,7 @@ denseHeavyKernel(GridSpec g, const BvhNode* __restrict__ nodes, const double* __
             const int q = atomicAdd(spill.count, 1);
             spill.ids[q] = idx;
             spill.seeds[q] = seed;
+            spill.hits[q] = hits;
         }
         else if (status != dev::kFrontierExact || force)
             pushReplay(replay, idx);
@@ -259,9 +265,10 @@ denseSpillKernel(GridSpec g, const BvhNode* __restrict__ nodes, const double* __
                   k = int(idx / (int64_t(g.n[0]) * g.n[1]));
         const D3 c = voxelCentre(g, i, j, k);
         double seed = spill.seeds[q];
+        const int hits = spill.hits[q];
         int status = 0;
         const dev::SignedNearest s =
-            dev::frontierSignedNearest(c, nodes, tri9, box6, dir, mu, 0, seed, f, status);
+            dev::frontierSignedNearest(c, nodes, tri9, box6, dir, mu, 0, seed, f, status, hits);
         if ((threadIdx.x & 63) != 0) continue;
         if (status != dev::kFrontierExact || force)
             pushReplay(replay, idx);
@@ -311,13 +318,13 @@ denseFillBatchedKernel(const MeshJob* __restrict__ jobs, int nJobs, int64_t tota
         const dev::WaveNearest wn =
             dev::waveNearest(c, active, act, nodes, tri9, box6, J.mu, ws, lane, J.root, budget);
         const int64_t idx = i + int64_t(g.n[0]) * (j + int64_t(g.n[1]) * k);
+        const int hits = dev::waveRayHits(c, dir, act, nodes, tri9, ws, lane, J.root);
         if (wn.aborted)
         {
-            if (!pushHeavy(heavy, unit, active, wn.bestSq) && active)
+            if (!pushHeavy(heavy, unit, active, wn.bestSq, hits) && active)
                 pushReplay(replay, J.voxelBegin + idx);
             continue;
         }
-        const int hits = dev::waveRayHits(c, dir, act, nodes, tri9, ws, lane, J.root);
         if (!active) continue;
         if (!wn.exact || force)
             pushReplay(replay, J.voxelBegin + idx);
@@ -347,9 +354,10 @@ denseHeavyBatchedKernel(const MeshJob* __restrict__ jobs, int nJobs,
         if (i >= g.n[0] || j >= g.n[1] || k >= g.n[2]) continue; // wave-uniform
         const D3 c = voxelCentre(g, i, j, k);
         double seed = heavy.seeds[u];
+        const int hits = heavy.hits[u];
         int status = 0;
         const dev::SignedNearest s =
-            dev::frontierSignedNearest(c, nodes, tri9, box6, dir, J.mu, J.root, seed, f, status);
+            dev::frontierSignedNearest(c, nodes, tri9, box6, dir, J.mu, J.root, seed, f, status, hits);
         if ((threadIdx.x & 63) != 0) continue;
         const int64_t idx = i + int64_t(g.n[0]) * (j + int64_t(g.n[1]) * k);
         if (status == dev::kFrontierOverflow)
@@ -357,6 +365,7 @@ denseHeavyBatchedKernel(const MeshJob* __restrict__ jobs, int nJobs,
             const int q = atomicAdd(spill.count, 1);
             spill.ids[q] = J.voxelBegin + idx;
             spill.seeds[q] = seed;
+            spill.hits[q] = hits;
         }
         else if (status != dev::kFrontierExact || force)
             pushReplay(replay, J.voxelBegin + idx);
@@ -385,9 +394,10 @@ denseSpillBatchedKernel(const MeshJob* __restrict__ jobs, int nJobs,
                   k = int(idx / (int64_t(g.n[0]) * g.n[1]));
         const D3 c = voxelCentre(g, i, j, k);
         double seed = spill.seeds[q];
+        const int hits = spill.hits[q];
         int status = 0;
         const dev::SignedNearest s =
-            dev::frontierSignedNearest(c, nodes, tri9, box6, dir, J.mu, J.root, seed, f, status);
+            dev::frontierSignedNearest(c, nodes, tri9, box6, dir, J.mu, J.root, seed, f, status, hits);
         if ((threadIdx.x & 63) != 0) continue;
         if (status != dev::kFrontierExact || force)
             pushReplay(replay, id);
@@ -722,12 +732,12 @@ sparseFillKernel(GridSpec g, const BvhNode* __restrict__ nodes, const double* __
         const unsigned long long act = __ballot(active);
         const dev::WaveNearest wn =
             dev::waveNearest(c, active, act, nodes, tri9, box6, mu, ws, lane, 0, budget);
+        const int hits = dev::waveRayHits(c, dir, act, nodes, tri9, ws, lane, 0);
         if (wn.aborted)
         {
-            if (!pushHeavy(heavy, u, active, wn.bestSq) && active) pushReplay(replay, t);
+            if (!pushHeavy(heavy, u, active, wn.bestSq, hits) && active) pushReplay(replay, t);
             continue;
         }
-        const int hits = dev::waveRayHits(c, dir, act, nodes, tri9, ws, lane, 0);
         if (!active) continue;
         if (!wn.exact || force)
             pushReplay(replay, t);
@@ -755,15 +765,17 @@ sparseHeavyKernel(GridSpec g, const BvhNode* __restrict__ nodes, const double* _
         if (i < 0 || i >= g.n[0] || j < 0 || j >= g.n[1] || k < 0 || k >= g.n[2]) continue;
         const D3 c = voxelCentre(g, i, j, k);
         double seed = heavy.seeds[u];
+        const int hits = heavy.hits[u];
         int status = 0;
         const dev::SignedNearest s =
-            dev::frontierSignedNearest(c, nodes, tri9, box6, dir, mu, 0, seed, f, status);
+            dev::frontierSignedNearest(c, nodes, tri9, box6, dir, mu, 0, seed, f, status, hits);
         if ((threadIdx.x & 63) != 0) continue;
         if (status == dev::kFrontierOverflow)
         {
             const int q = atomicAdd(spill.count, 1);
             spill.ids[q] = t;
             spill.seeds[q] = seed;
+            spill.hits[q] = hits;
         }
         else if (status != dev::kFrontierExact || force)
             pushReplay(replay, t);
@@ -788,9 +800,10 @@ sparseSpillKernel(GridSpec g, const BvhNode* __restrict__ nodes, const double* _
         brickVoxel(bijk, bd, t, i, j, k);
         const D3 c = voxelCentre(g, i, j, k);
         double seed = spill.seeds[q];
+        const int hits = spill.hits[q];
         int status = 0;
         const dev::SignedNearest s =
-            dev::frontierSignedNearest(c, nodes, tri9, box6, dir, mu, 0, seed, f, status);
+            dev::frontierSignedNearest(c, nodes, tri9, box6, dir, mu, 0, seed, f, status, hits);
         if ((threadIdx.x & 63) != 0) continue;
         if (status != dev::kFrontierExact || force)
             pushReplay(replay, t);
@@ -882,8 +895,8 @@ namespace {
 struct ReplayBuf
 {
     Replay r{nullptr, nullptr};
-    Heavy h{nullptr, nullptr, nullptr, nullptr, 0};
-    Spill sp{nullptr, nullptr, nullptr, nullptr};
+    Heavy h{nullptr, nullptr, nullptr, nullptr, nullptr, 0};
+    Spill sp{nullptr, nullptr, nullptr, nullptr, nullptr};
     unsigned long long* queue = nullptr; // work-queue counter of the fill launch
     hipStream_t s;
     void* owned = nullptr; // the block, when this object allocated it
@@ -895,7 +908,8 @@ struct ReplayBuf
         auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
         size_t b = up(48) + up(sizeof(int64_t) * size_t(std::max<int64_t>(cap, 1)));
         if (units > 0)
-            b += up(sizeof(int64_t) * size_t(units)) + 3 * up(sizeof(double) * 64 * size_t(units));
+            b += up(sizeof(int64_t) * size_t(units)) + 3 * up(sizeof(double) * 64 * size_t(units)) +
+                 2 * up(sizeof(int) * 64 * size_t(units));
         return b;
     }
     // cap: work items (replay ids); units: 64-lane work units (heavy list).
@@ -929,6 +943,9 @@ struct ReplayBuf
             h.seeds = reinterpret_cast<double*>(p);
             sp.ids = reinterpret_cast<int64_t*>(p + lane);
             sp.seeds = reinterpret_cast<double*>(p + 2 * lane);
+            p += 3 * lane;
+            h.hits = reinterpret_cast<int*>(p);
+            sp.hits = reinterpret_cast<int*>(p + up(sizeof(int) * 64 * size_t(units)));
         }
     }
     // Copies the list length to the calling thread's pinned counter (read it
@@ -953,7 +970,7 @@ const char* const kFillShapeNames[kFillShapeCount] = {"4x4x4", "8x8x1", "8x4x2",
 
 // Blocks of a queue-driven fill launch: enough resident waves for the chip,
 // never more than there are work units.
-unsigned queueGrid(int64_t units)
+int deviceCUs()
 {
     static int cus = 0;
     if (cus == 0)
@@ -964,7 +981,12 @@ unsigned queueGrid(int64_t units)
             cus = prop.multiProcessorCount;
         if (cus <= 0) cus = 256;
     }
-    const int64_t want = int64_t(cus) * 8; // 8 blocks of 4 waves per CU
+    return cus;
+}
+
+unsigned queueGrid(int64_t units)
+{
+    const int64_t want = int64_t(deviceCUs()) * 8; // 8 blocks of 4 waves per CU
     const int64_t need = (units + kFillWaves - 1) / kFillWaves;
     return unsigned(std::max<int64_t>(1, std::min(want, need)));
 }
@@ -972,6 +994,10 @@ unsigned queueGrid(int64_t units)
 // Blocks of the frontier (heavy) pass: its unit count is only known on the
 // device, so a fixed grid of queue-driven waves.
 unsigned heavyGrid() { return queueGrid(int64_t(1) << 40); }
+
+// Blocks of the overflow pass (one wave each): a block per CU, since
+// overflowing voxels are few and each block holds a 32 KB frontier.
+unsigned spillGrid() { return unsigned(deviceCUs()); }
 
 } // namespace
 
@@ -1032,11 +1058,11 @@ void runDenseFill(const GridSpec& g, const DeviceBvh& bvh, const DenseFillOut& o
                        bvh.nodes, bvh.tri9, bvh.box6, dir, mu, sh, o, rb.r, rb.h, rb.queue, budget,
                        forceReplay() ? 1 : 0);
     SDFO_HIP(hipGetLastError());
-    hipLaunchKernelGGL((denseHeavyKernel<kFrontierSmall, 4>), dim3(heavyGrid()), dim3(256), 0, s, g,
+    hipLaunchKernelGGL((denseHeavyKernel<kFrontierSmall, 2>), dim3(heavyGrid()), dim3(128), 0, s, g,
                        bvh.nodes, bvh.tri9, bvh.box6, dir, mu, sh, o, rb.r, rb.h, rb.sp,
                        forceReplay() ? 1 : 0);
     SDFO_HIP(hipGetLastError());
-    hipLaunchKernelGGL(denseSpillKernel, dim3(heavyGrid()), dim3(64), 0, s, g, bvh.nodes, bvh.tri9,
+    hipLaunchKernelGGL(denseSpillKernel, dim3(spillGrid()), dim3(64), 0, s, g, bvh.nodes, bvh.tri9,
                        bvh.box6, dir, mu, o, rb.r, rb.sp, forceReplay() ? 1 : 0);
     SDFO_HIP(hipGetLastError());
     hipLaunchKernelGGL(denseReplayKernel, dim3(kReplayGrid), dim3(kReplayBlock), 0, s, g, bvh.nodes,
@@ -1132,11 +1158,11 @@ void launchDenseFillBatched(const std::vector<BatchMesh>& meshes, const DeviceBv
                            dj, int(jobs.size()), bricks, bvh.nodes, bvh.tri9, bvh.box6, dir, rb.r,
                            rb.h, rb.queue, budget, force);
         SDFO_HIP(hipGetLastError());
-        hipLaunchKernelGGL((denseHeavyBatchedKernel<kFrontierSmall, 4>), dim3(heavyGrid()), dim3(256),
+        hipLaunchKernelGGL((denseHeavyBatchedKernel<kFrontierSmall, 2>), dim3(heavyGrid()), dim3(128),
                            0, s, dj, int(jobs.size()), bvh.nodes, bvh.tri9, bvh.box6, dir, rb.r, rb.h,
                            rb.sp, force);
         SDFO_HIP(hipGetLastError());
-        hipLaunchKernelGGL(denseSpillBatchedKernel, dim3(heavyGrid()), dim3(64), 0, s, dj,
+        hipLaunchKernelGGL(denseSpillBatchedKernel, dim3(spillGrid()), dim3(64), 0, s, dj,
                            int(jobs.size()), bvh.nodes, bvh.tri9, bvh.box6, dir, rb.r, rb.sp, force);
         SDFO_HIP(hipGetLastError());
         hipLaunchKernelGGL(denseReplayBatchedKernel, dim3(kReplayGrid), dim3(kReplayBlock), 0, s, dj,
@@ -1226,11 +1252,11 @@ void launchSparseFill(const GridSpec& g, const DeviceBvh& bvh, int blockDim, int
                        rb.queue, debugFillBudget() > 0 ? debugFillBudget() : kBudgets[0],
                        forceReplay() ? 1 : 0);
     SDFO_HIP(hipGetLastError());
-    hipLaunchKernelGGL((sparseHeavyKernel<kFrontierSmall, 4>), dim3(heavyGrid()), dim3(256), 0, s, g,
+    hipLaunchKernelGGL((sparseHeavyKernel<kFrontierSmall, 2>), dim3(heavyGrid()), dim3(128), 0, s, g,
                        bvh.nodes, bvh.tri9, bvh.box6, dir, mu, blockDim, total, bijk, sphi, sn3,
                        rb.r, rb.h, rb.sp, forceReplay() ? 1 : 0);
     SDFO_HIP(hipGetLastError());
-    hipLaunchKernelGGL(sparseSpillKernel, dim3(heavyGrid()), dim3(64), 0, s, g, bvh.nodes, bvh.tri9,
+    hipLaunchKernelGGL(sparseSpillKernel, dim3(spillGrid()), dim3(64), 0, s, g, bvh.nodes, bvh.tri9,
                        bvh.box6, dir, mu, blockDim, bijk, sphi, sn3, rb.r, rb.sp,
                        forceReplay() ? 1 : 0);
     SDFO_HIP(hipGetLastError());

@@ -651,7 +651,7 @@ __device__ inline SignedNearest frontierSignedNearest(D3 p, const BvhNode* __res
                                                       const double* __restrict__ tri9,
                                                       const double* __restrict__ box6, D3 dir,
                                                       double mu, int root, double& seed,
-                                                      FrontierLds f, int& status)
+                                                      FrontierLds f, int& status, int hits)
 {
     const int lane = int(threadIdx.x) & (kWave - 1);
     double bestSq = DBL_MAX, bestB = 0.0;
@@ -759,10 +759,14 @@ __device__ inline SignedNearest frontierSignedNearest(D3 p, const BvhNode* __res
     status = overflow ? kFrontierOverflow
                       : ((bestSlot >= 0 && bestB <= bestSq) ? kFrontierExact : kFrontierUnsure);
     if (overflow) return SignedNearest{};
-    // ray parity by lane 0 with a per-lane stack in the frontier buffer
-    int hits = 0;
-    if (lane == 0) hits = refRayHits(p, dir, nodes, tri9, LaneStack{f.node[0], 1}, root);
-    hits = __shfl(hits, 0);
+    // ray parity: the count the caller has (hits >= 0), else counted by lane 0
+    // with a per-lane stack in the frontier buffer
+    if (hits < 0)
+    {
+        hits = 0;
+        if (lane == 0) hits = refRayHits(p, dir, nodes, tri9, LaneStack{f.node[0], 1}, root);
+        hits = __shfl(hits, 0);
+    }
     return finishSigned(p, tri9, bestSq, bestSlot, hits);
 }
 
