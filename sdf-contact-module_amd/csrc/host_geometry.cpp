@@ -248,6 +248,9 @@ struct ScratchLease
     }
 };
 
+// Nodes of at least this many triangles scan their centroids in parallel.
+constexpr int64_t kParallelExtent = 65536;
+
 struct BvhBuilder
 {
     BvhNode* nodes;    // this mesh's nodes (indices local to the mesh)
@@ -290,15 +293,35 @@ struct BvhBuilder
             return level;
         }
         double cmn[3] = {big, big, big}, cmx[3] = {-big, -big, -big};
-        for (int i = lo; i < hi; ++i)
-        {
-            const double* c = items[size_t(i)].c;
-            for (int k = 0; k < 3; ++k)
+        auto extents = [&](int64_t a, int64_t b, double* mn, double* mx) {
+            for (int64_t i = a; i < b; ++i)
             {
-                cmn[k] = minOf(cmn[k], c[k]);
-                cmx[k] = maxOf(cmx[k], c[k]);
+                const double* c = items[size_t(i)].c;
+                for (int k = 0; k < 3; ++k)
+                {
+                    mn[k] = minOf(mn[k], c[k]);
+                    mx[k] = maxOf(mx[k], c[k]);
+                }
             }
+        };
+        if (count >= kParallelExtent && par != INT64_MAX)
+        {
+            // the top nodes' scans are on every build's critical path: chunks
+            // on the workers, merged by min / max (exact, so in any order)
+            std::mutex m;
+            parallelRange(count, kParallelExtent / 4, [&](int64_t a, int64_t b) {
+                double mn[3] = {big, big, big}, mx[3] = {-big, -big, -big};
+                extents(lo + a, lo + b, mn, mx);
+                std::lock_guard<std::mutex> lk(m);
+                for (int k = 0; k < 3; ++k)
+                {
+                    cmn[k] = minOf(cmn[k], mn[k]);
+                    cmx[k] = maxOf(cmx[k], mx[k]);
+                }
+            });
         }
+        else
+            extents(lo, hi, cmn, cmx);
         const double ext[3] = {cmx[0] - cmn[0], cmx[1] - cmn[1], cmx[2] - cmn[2]};
         int axis = 0;
         if (ext[1] > ext[0]) axis = 1;
