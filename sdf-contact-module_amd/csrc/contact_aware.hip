@@ -885,7 +885,9 @@ struct WaveListScan
                 __builtin_amdgcn_wave_barrier();
                 for (int e = 0; e < n; ++e)
                 {
-                    const int tt = buf[e];
+                    // every lane reads the same entry: a wave-uniform index lets
+                    // the box and vertex loads below go through the scalar cache
+                    const int tt = __builtin_amdgcn_readfirstlane(buf[e]);
                     if (stopped || !(triBoxSq(p, tbox6 + 6 * int64_t(tt)) <= reach2)) continue;
                     if (!f(tt)) stopped = true;
                 }
