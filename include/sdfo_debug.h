@@ -20,6 +20,20 @@ int sdfo_debug_bvh_export(const double* verts, int64_t n_verts, const int32_t* t
                           int64_t n_tris, double* nodes6, int32_t* nodeI4, int32_t* perm,
                           int32_t max_nodes, int32_t* depth);
 
+/* The device BVH as the builds assemble it (host slot order, then nodes,
+ * slot triangles and slot boxes built on the current device) for n_meshes
+ * meshes concatenated: vertex offsets vo[n_meshes + 1], triangle offsets
+ * to[n_meshes + 1], triangle indices local to each mesh's vertices.  Per
+ * node: bmin3, bmax3 (nodes6) and left, right, triStart, triCount (nodeI4,
+ * links and slot starts into the concatenation); per slot: the mesh-local
+ * triangle (perm), its vertices a, b, c (tri9) and box bmin3, bmax3 (box6).
+ * Returns the node count, or a negative error (-(needed) when max_nodes is
+ * too small).  Synchronous on the current device. */
+int sdfo_debug_device_bvh_export(int32_t n_meshes, const double* verts, const int64_t* vo,
+                                 const int32_t* tris, const int64_t* to, double* nodes6,
+                                 int32_t* nodeI4, int32_t* perm, double* tri9, double* box6,
+                                 int32_t max_nodes, int32_t* depth);
+
 /* Order of n <= 64 values by absolute value, as the contact-aware builder
  * orders merged candidates (std::sort with |a| < |b|, SdfBuilder.cpp:348-353).
  * order[i] = index of the i-th value.  Host only. */

@@ -1308,6 +1308,202 @@ slotBoxKernel(const double* __restrict__ tri9, int64_t n, double* __restrict__ b
     }
 }
 
+// std::min / std::max as the host tree applies them (the first argument wins
+// ties, which fixes the sign of zero bounds).
+__device__ __forceinline__ double minOfDev(double a, double b) { return (b < a) ? b : a; }
+__device__ __forceinline__ double maxOfDev(double a, double b) { return (a < b) ? b : a; }
+
+// Box of a triangle's vertices, folded as the host builder folds it.
+__device__ __forceinline__ void triangleBox(const double* __restrict__ verts,
+                                            const int32_t* __restrict__ ix, double* lo,
+                                            double* hi)
+{
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+    {
+        const double a = verts[3 * int64_t(ix[0]) + k], b = verts[3 * int64_t(ix[1]) + k],
+                     c = verts[3 * int64_t(ix[2]) + k];
+        lo[k] = minOfDev(minOfDev(a, b), c);
+        hi[k] = maxOfDev(maxOfDev(a, b), c);
+    }
+}
+
+// The BVH of KdTree::build (KdTree.h:91-121, 188-242) assembled from the slot
+// order: the tree's shape follows from the triangle count (split at
+// count / 2, leaves <= 8, pre-order indices), so a node's index, slot range
+// and children follow from its path from the root.  Leaf boxes are folded
+// over the leaf's slots in order and inner boxes are the union of the
+// children's, as the host builder makes them.  The tree is cut at depth
+// P.topLevels into 2^topLevels subtrees of at most kSubtreeSlots slots: one
+// workgroup per subtree writes its slots (triangle, box) and nodes, then one
+// workgroup per mesh writes the nodes above the cut.  Inside a workgroup a
+// node is written by the second of its children to finish (counted with a
+// workgroup-scope atomic), so no lane waits and no cache traffic between
+// XCDs is needed.
+constexpr int kAssembleBlock = 256;
+
+struct TreeNodeAt
+{
+    int me, lo, hi, depth; // mesh-local node index, slot range, depth (root 0)
+};
+
+// Node count of the left child of a node at depth d holding c slots: the
+// sizes at depth d + 1 are (n >> (d + 1)) or one more, tabulated per mesh.
+__device__ __forceinline__ int leftChildNodes(const BvhMeshPart& P, int c, int d)
+{
+    const int h = c >> 1;
+    return h == int(P.nSlots >> (d + 1)) ? P.f0[d + 1] : P.f1[d + 1];
+}
+
+// The node reached from t by n steps, step e going right when bit
+// (n - 1 - e) of bits is set.
+__device__ __forceinline__ TreeNodeAt walkDown(const BvhMeshPart& P, TreeNodeAt t, uint32_t bits, int n)
+{
+    for (int e = 0; e < n; ++e)
+    {
+        const int mid = t.lo + ((t.hi - t.lo) >> 1);
+        if ((bits >> (n - 1 - e)) & 1u)
+        {
+            t.me += 1 + leftChildNodes(P, t.hi - t.lo, t.depth);
+            t.lo = mid;
+        }
+        else
+        {
+            t.me += 1;
+            t.hi = mid;
+        }
+        ++t.depth;
+    }
+    return t;
+}
+
+// Writes inner node t from its two children.
+__device__ __forceinline__ void writeInner(const BvhMeshPart& P, BvhNode* nodes, const TreeNodeAt& t)
+{
+    const int L = t.me + 1, R = t.me + 1 + leftChildNodes(P, t.hi - t.lo, t.depth);
+    const BvhNode& nl = nodes[P.nodeBase + L];
+    const BvhNode& nr = nodes[P.nodeBase + R];
+    BvhNode nd;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+    {
+        nd.bmin[k] = minOfDev(nl.bmin[k], nr.bmin[k]);
+        nd.bmax[k] = maxOfDev(nl.bmax[k], nr.bmax[k]);
+    }
+    nd.left = int(P.nodeBase + L);
+    nd.right = int(P.nodeBase + R);
+    nd.triStart = -1;
+    nd.triCount = 0;
+    nodes[P.nodeBase + t.me] = nd;
+}
+
+// The second of a node's children to finish writes it (its flag counts the
+// children; workgroup scope: the node and its children belong to one
+// workgroup's part of the tree).
+__device__ __forceinline__ bool secondChild(int32_t* flag)
+{
+    return __hip_atomic_fetch_add(flag, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) != 0;
+}
+
+__global__ void __launch_bounds__(kAssembleBlock)
+bvhSubtreeKernel(const BvhMeshPart* __restrict__ parts, int nParts,
+                 const double* __restrict__ verts, const int32_t* __restrict__ tris,
+                 const int32_t* __restrict__ slotTri, double* __restrict__ tri9,
+                 double* __restrict__ box6, BvhNode* nodes, int32_t* flags)
+{
+    int lo = 0, hi = nParts - 1;
+    while (lo < hi)
+    {
+        const int mid = (lo + hi + 1) >> 1;
+        if (parts[mid].subBase <= int64_t(blockIdx.x))
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const BvhMeshPart& P = parts[lo];
+    const double* V = verts + 3 * P.vertBase;
+    const int32_t* T = tris + 3 * P.triBase;
+    const TreeNodeAt root = walkDown(P, TreeNodeAt{0, 0, int(P.nSlots), 0},
+                                     uint32_t(int64_t(blockIdx.x) - P.subBase), P.topLevels);
+    for (int s = root.lo + int(threadIdx.x); s < root.hi; s += kAssembleBlock)
+    {
+        const int64_t gs = P.slotBase + s;
+        const int32_t* ix = T + 3 * int64_t(slotTri[gs]);
+        double* t9 = tri9 + 9 * gs;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) t9[3 * c + k] = V[3 * int64_t(ix[c]) + k];
+        double bl[3], bh[3];
+        triangleBox(V, ix, bl, bh);
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+        {
+            box6[6 * gs + k] = bl[k];
+            box6[6 * gs + 3 + k] = bh[k];
+        }
+        // down to the leaf holding s; bit (steps - 1 - e) of path = right at step e
+        TreeNodeAt t = root;
+        uint32_t path = 0;
+        int steps = 0;
+        while (t.hi - t.lo > 8)
+        {
+            const bool right = s >= t.lo + ((t.hi - t.lo) >> 1);
+            path = (path << 1) | (right ? 1u : 0u);
+            t = walkDown(P, t, right ? 1u : 0u, 1);
+            ++steps;
+        }
+        if (s != t.lo) continue;
+        double bmn[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, bmx[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+        for (int q = t.lo; q < t.hi; ++q)
+        {
+            triangleBox(V, T + 3 * int64_t(slotTri[P.slotBase + q]), bl, bh);
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+            {
+                bmn[k] = minOfDev(bmn[k], bl[k]);
+                bmx[k] = maxOfDev(bmx[k], bh[k]);
+            }
+        }
+        BvhNode nd;
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+        {
+            nd.bmin[k] = bmn[k];
+            nd.bmax[k] = bmx[k];
+        }
+        nd.left = nd.right = -1;
+        nd.triStart = int(P.slotBase + t.lo);
+        nd.triCount = t.hi - t.lo;
+        nodes[P.nodeBase + t.me] = nd;
+        for (int d = steps - 1; d >= 0; --d)
+        {
+            const TreeNodeAt up = walkDown(P, root, path >> (steps - d), d);
+            if (!secondChild(flags + P.nodeBase + up.me)) break;
+            writeInner(P, nodes, up);
+        }
+    }
+}
+
+// The nodes above the cut: one workgroup per mesh, one lane per subtree root
+// (written by bvhSubtreeKernel), climbing the same way.
+__global__ void __launch_bounds__(kAssembleBlock)
+bvhTopKernel(const BvhMeshPart* __restrict__ parts, BvhNode* nodes, int32_t* flags)
+{
+    const BvhMeshPart& P = parts[blockIdx.x];
+    const int L = P.topLevels;
+    for (int r = int(threadIdx.x); r < (1 << L); r += kAssembleBlock)
+    {
+        for (int d = L - 1; d >= 0; --d)
+        {
+            const TreeNodeAt up =
+                walkDown(P, TreeNodeAt{0, 0, int(P.nSlots), 0}, uint32_t(r) >> (L - d), d);
+            if (!secondChild(flags + P.nodeBase + up.me)) break;
+            writeInner(P, nodes, up);
+        }
+    }
+}
+
 } // namespace
 
 void launchSlotBoxes(const double* tri9, int64_t n, double* box6, hipStream_t s)
@@ -1315,6 +1511,21 @@ void launchSlotBoxes(const double* tri9, int64_t n, double* box6, hipStream_t s)
     if (n <= 0) return;
     hipLaunchKernelGGL(slotBoxKernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, tri9, n, box6);
     SDFO_HIP(hipGetLastError());
+}
+
+void launchBvhAssemble(const DeviceBvh& db, hipStream_t s)
+{
+    if (db.ntris <= 0 || db.nSubtrees <= 0) return;
+    hipLaunchKernelGGL(bvhSubtreeKernel, dim3(unsigned(db.nSubtrees)), dim3(kAssembleBlock), 0, s,
+                       db.parts, db.nParts, db.verts, db.tris, db.triIdx, db.tri9, db.box6, db.nodes,
+                       db.flags);
+    SDFO_HIP(hipGetLastError());
+    if (db.topLevelsMax > 0)
+    {
+        hipLaunchKernelGGL(bvhTopKernel, dim3(unsigned(db.nParts)), dim3(kAssembleBlock), 0, s, db.parts,
+                           db.nodes, db.flags);
+        SDFO_HIP(hipGetLastError());
+    }
 }
 
 namespace {

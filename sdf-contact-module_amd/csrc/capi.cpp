@@ -312,7 +312,10 @@ std::unique_ptr<sdfo_grid> buildDense(const double* verts, int64_t nv, const int
         const size_t bytes = denseGridBytes(g->spec, records);
         giveArena(*g, bytes, s);
         carveDenseGrid(*g, records);
-        db.reserve(bvhNodeCount(nt), nt, s);
+        // the mesh goes to the device while the host orders the tree's slots
+        db.reserveAssembly(bvhNodeCount(nt), nt, nv, 1, s);
+        const BvhMeshPart part{0, nt, 0, 0, 0};
+        db.stageMeshes(verts, nv, tris, nt, &part, 1, s);
         SDFO_HIP(hipMallocAsync(&fillScratch, denseFillScratchBytes(g->spec), s));
         for (hipEvent_t& e : ev) SDFO_HIP(hipEventCreate(&e));
     };
@@ -335,11 +338,11 @@ std::unique_ptr<sdfo_grid> buildDense(const double* verts, int64_t nv, const int
     {
         poolSubmit(&alloc);
         const clk::time_point t1 = clk::now();
-        const HostBvhView hb = buildHostBvhScratch(verts, nv, tris, nt);
+        const HostBvhOrderView hb = buildHostBvhOrderScratch(verts, nv, tris, nt);
         ph[1] = msSince(t1);
         const clk::time_point t2 = clk::now();
         joinAlloc();
-        db.upload(hb.nodes, hb.nNodes, hb.slotTri9, hb.slotTri, hb.nSlots, hb.depth, hb.rayDir, s);
+        db.assemble(hb.slotTri, hb.depth, hb.rayDir, s);
         ph[2] = msSince(t2);
         SDFO_HIP(hipEventRecord(ev[0], s));
         launchDenseFill(g->spec, db, g->phi, g->n3, g->witness3, g->tri, s, fillScratch);
@@ -443,13 +446,8 @@ std::vector<std::unique_ptr<sdfo_grid>> buildDenseBatched(int32_t nm, const doub
     // (the stream is synchronised before this function returns, so the
     // staging is free again when the lock is released)
     std::lock_guard<std::mutex> stagingLock(batchStaging().lock);
-    const size_t nodeBytes = sizeof(BvhNode) * size_t(std::max<int64_t>(1, nAllNodes));
-    const size_t tri9Bytes = sizeof(double) * 9 * size_t(std::max<int64_t>(1, nAllSlots));
     const size_t slotBytes = sizeof(int32_t) * size_t(std::max<int64_t>(1, nAllSlots));
-    char* staging = batchStaging().get(nodeBytes + tri9Bytes + slotBytes);
-    BvhNode* const allNodes = reinterpret_cast<BvhNode*>(staging);
-    double* const allTri9 = reinterpret_cast<double*>(staging + nodeBytes);
-    int32_t* const allSlotTri = reinterpret_cast<int32_t*>(staging + nodeBytes + tri9Bytes);
+    int32_t* const allSlotTri = reinterpret_cast<int32_t*>(batchStaging().get(slotBytes));
     std::vector<int> depths(static_cast<size_t>(nm), 0);
     // meshes are independent: the host worker threads take them largest
     // first, and a mesh's large subtrees go to idle workers
@@ -467,11 +465,11 @@ std::vector<std::unique_ptr<sdfo_grid>> buildDenseBatched(int32_t nm, const doub
             const int32_t m = order[size_t(q)];
             try
             {
-                const size_t nb = size_t(nodeBase[size_t(m)]), sb = size_t(slotBase[size_t(m)]);
-                depths[size_t(m)] = buildHostBvhInto(
-                    verts + 3 * vo[m], vo[m + 1] - vo[m], tris + 3 * to[m], to[m + 1] - to[m],
-                    allNodes + nb, allTri9 + 9 * sb, allSlotTri + sb, int(nb),
-                    int(sb), true);
+                // the slot order only: the tree is assembled on the device
+                const size_t sb = size_t(slotBase[size_t(m)]);
+                depths[size_t(m)] = buildHostBvhOrder(verts + 3 * vo[m], vo[m + 1] - vo[m],
+                                                      tris + 3 * to[m], to[m + 1] - to[m],
+                                                      allSlotTri + sb, true);
             }
             catch (const Error& e)
             {
@@ -526,6 +524,20 @@ std::vector<std::unique_ptr<sdfo_grid>> buildDenseBatched(int32_t nm, const doub
     const auto arena = std::make_shared<ArenaBlock>();
     arena->base = arenaBase;
     size_t arenaOff = 0;
+    // the meshes go to the device beside the BVH work too
+    DeviceBvh db;
+    struct ReleaseOnExit
+    {
+        DeviceBvh& d;
+        ~ReleaseOnExit() { d.release(); } // stream-ordered, also on an early throw
+    } dbRelease{db};
+    std::vector<BvhMeshPart> parts(static_cast<size_t>(nm));
+    for (int32_t m = 0; m < nm; ++m)
+        parts[size_t(m)] = BvhMeshPart{slotBase[size_t(m)], to[m + 1] - to[m], nodeBase[size_t(m)],
+                                       vo[m] - vo[0], to[m] - to[0]};
+    db.reserveAssembly(nAllNodes, nAllSlots, vo[nm] - vo[0], nm, s);
+    db.stageMeshes(verts + 3 * vo[0], vo[nm] - vo[0], tris + 3 * to[0], to[nm] - to[0], parts.data(),
+                   nm, s);
     for (int32_t m = 0; m < nm; ++m)
     {
         auto g = std::make_unique<sdfo_grid>();
@@ -556,13 +568,11 @@ std::vector<std::unique_ptr<sdfo_grid>> buildDenseBatched(int32_t nm, const doub
     HostBvh dirOnly;
     setRayDir(dirOnly);
     ph[1] = msSince(tPool); // BVHs (with the allocations beside them)
-    DeviceBvh db;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     try
     {
         for (hipEvent_t& e : ev) SDFO_HIP(hipEventCreate(&e));
-        db.upload(allNodes, nAllNodes, allTri9, allSlotTri, nAllSlots, depth,
-                  dirOnly.rayDir, s);
+        db.assemble(allSlotTri, depth, dirOnly.rayDir, s);
         SDFO_HIP(hipEventRecord(ev[0], s));
         launchDenseFillBatched(jobs, db, s);
         SDFO_HIP(hipEventRecord(ev[1], s));
@@ -622,12 +632,26 @@ std::unique_ptr<sdfo_grid> buildSparse(const double* verts, int64_t nv, const in
     g->nby = (g->spec.n[1] + bd - 1) / bd;
     g->nbz = (g->spec.n[2] + bd - 1) / bd;
     const int64_t cells = int64_t(g->nbx) * g->nby * g->nbz;
-    const HostBvhView hb = buildHostBvhScratch(verts, nv, tris, nt);
+    // A worker thread reserves the device BVH and uploads the mesh while this
+    // thread orders the tree's slots; the tree is then assembled on the device.
     DeviceBvh db;
+    PoolTask stage;
+    stage.fn = [&] {
+        DeviceGuard dg(g->device);
+        keepStreamPoolMemory(g->device);
+        db.reserveAssembly(bvhNodeCount(nt), nt, nv, 1, s);
+        const BvhMeshPart part{0, nt, 0, 0, 0};
+        db.stageMeshes(verts, nv, tris, nt, &part, 1, s);
+    };
+    bool submitted = false, staged = false;
     try
     {
-        keepStreamPoolMemory(g->device);
-        db.upload(hb.nodes, hb.nNodes, hb.slotTri9, hb.slotTri, hb.nSlots, hb.depth, hb.rayDir, s);
+        poolSubmit(&stage);
+        submitted = true;
+        const HostBvhOrderView hb = buildHostBvhOrderScratch(verts, nv, tris, nt);
+        staged = true;
+        poolWait(&stage);
+        db.assemble(hb.slotTri, hb.depth, hb.rayDir, s);
         g->activeMask = g->alloc<uint8_t>(size_t(cells));
         launchSparseActivate(g->spec, db, bd, g->nbx, g->nby, g->nbz, g->activation, g->activeMask, s);
         std::vector<uint8_t> mask(static_cast<size_t>(cells));
@@ -666,6 +690,16 @@ std::unique_ptr<sdfo_grid> buildSparse(const double* verts, int64_t nv, const in
     }
     catch (...)
     {
+        if (submitted && !staged)
+        {
+            try
+            {
+                poolWait(&stage);
+            }
+            catch (...)
+            {
+            }
+        }
         (void)hipStreamSynchronize(s);
         db.release();
         releaseHostBvhScratch(kKeepScratchTris);

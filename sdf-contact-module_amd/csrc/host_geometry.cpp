@@ -68,9 +68,8 @@ GridSpec setupGrid(const double* verts, int64_t nv, const sdfo_build_params& p)
 
 namespace {
 
-// Node count of the subtree built over n triangles: the split is always at
-// n / 2, so the tree shape depends on n only.
-int64_t subtreeNodes(int64_t n) { return n <= 8 ? 1 : 1 + subtreeNodes(n / 2) + subtreeNodes(n - n / 2); }
+// Node count of the subtree built over n triangles (the shape depends on n only).
+int64_t subtreeNodes(int64_t n) { return medianTreeNodes(n); }
 
 // Persistent worker threads (created on first use, kept for the process):
 // tasks wait in one queue; a thread waiting for a task runs queued ones
@@ -253,9 +252,9 @@ constexpr int64_t kParallelExtent = 65536;
 
 struct BvhBuilder
 {
-    BvhNode* nodes;    // this mesh's nodes (indices local to the mesh)
+    BvhNode* nodes;    // this mesh's nodes (indices local to the mesh); null: order only
     Item* items;
-    const double* box; // bmin3, bmax3 per triangle
+    const double* box; // bmin3, bmax3 per triangle (unused when nodes is null)
 
     // Builds the subtree over items[lo, hi) whose root has pre-order index me;
     // returns the subtree depth.  Subtrees above `par` triangles are built
@@ -270,6 +269,7 @@ struct BvhBuilder
         nd.triStart = -1;
         nd.triCount = 0;
         const int count = hi - lo;
+        if (count <= 8 && !nodes) return level;
         if (count <= 8)
         {
             double bmn[3] = {big, big, big}, bmx[3] = {-big, -big, -big};
@@ -347,6 +347,7 @@ struct BvhBuilder
             dl = build(lo, mid, left, level + 1, par);
             dr = build(mid, hi, right, level + 1, par);
         }
+        if (!nodes) return dl > dr ? dl : dr;
         const BvhNode& L = nodes[left];
         const BvhNode& R = nodes[right];
         for (int k = 0; k < 3; ++k)
@@ -438,9 +439,10 @@ int buildHostBvhInto(const double* verts, int64_t nv, const int32_t* tris, int64
     for (int64_t q = 0; q < 3 * nt; ++q)
         if (tris[q] < 0 || tris[q] >= nv)
             throw Error(SDFO_ERR_INVALID_ARGUMENT, "triangle references invalid vertex");
-    // per-triangle boxes and centroids (in slot-order items), in parallel chunks
+    // per-triangle boxes (not when only the order is asked for) and
+    // centroids (in slot-order items), in parallel chunks
     ScratchLease lease;
-    double* box = lease.s->box.get(size_t(n) * 6);
+    double* box = nodes ? lease.s->box.get(size_t(n) * 6) : nullptr;
     Item* items = lease.s->items.get(size_t(n));
     parallelFor(
         n,
@@ -454,8 +456,11 @@ int buildHostBvhInto(const double* verts, int64_t nv, const int32_t* tris, int64
                 Item& it = items[size_t(t)];
                 for (int k = 0; k < 3; ++k)
                 {
-                    box[6 * size_t(t) + k] = minOf(minOf(a[k], b[k]), c[k]);
-                    box[6 * size_t(t) + 3 + k] = maxOf(maxOf(a[k], b[k]), c[k]);
+                    if (box)
+                    {
+                        box[6 * size_t(t) + k] = minOf(minOf(a[k], b[k]), c[k]);
+                        box[6 * size_t(t) + 3 + k] = maxOf(maxOf(a[k], b[k]), c[k]);
+                    }
                     it.c[k] = ((a[k] + b[k]) + c[k]) / 3.0;
                 }
                 it.idx = t;
@@ -472,7 +477,7 @@ int buildHostBvhInto(const double* verts, int64_t nv, const int32_t* tris, int64
         BvhBuilder bb{nodes, items, box};
         depth = bb.build(0, n, 0, 1, par);
         const int64_t nn = bvhNodeCount(n);
-        if (nodeBase != 0 || slotBase != 0)
+        if (nodes && (nodeBase != 0 || slotBase != 0))
             for (int64_t q = 0; q < nn; ++q)
             {
                 BvhNode& nd = nodes[q];
@@ -492,6 +497,7 @@ int buildHostBvhInto(const double* verts, int64_t nv, const int32_t* tris, int64
             {
                 const int t = items[size_t(s)].idx;
                 slotTri[s] = t;
+                if (!slotTri9) continue;
                 const int32_t* ix = tris + 3 * size_t(t);
                 for (int c = 0; c < 3; ++c)
                     for (int k = 0; k < 3; ++k)
@@ -500,6 +506,12 @@ int buildHostBvhInto(const double* verts, int64_t nv, const int32_t* tris, int64
         },
         parallel);
     return depth;
+}
+
+int buildHostBvhOrder(const double* verts, int64_t nv, const int32_t* tris, int64_t nt,
+                      int32_t* slotTri, bool parallel)
+{
+    return buildHostBvhInto(verts, nv, tris, nt, nullptr, nullptr, slotTri, 0, 0, parallel);
 }
 
 void setRayDir(HostBvh& h)
@@ -534,6 +546,20 @@ HostBvhView buildHostBvhScratch(const double* verts, int64_t nv, const int32_t* 
     v.nodes = nodes;
     v.nNodes = int64_t(nn);
     v.slotTri9 = tri9;
+    v.slotTri = slot;
+    v.nSlots = nt;
+    HostBvh dir;
+    setRayDir(dir);
+    for (int k = 0; k < 3; ++k) v.rayDir[k] = dir.rayDir[k];
+    return v;
+}
+
+HostBvhOrderView buildHostBvhOrderScratch(const double* verts, int64_t nv, const int32_t* tris,
+                                          int64_t nt)
+{
+    HostBvhOrderView v{};
+    int32_t* slot = t_bvhOut.slotTri.get(std::max<size_t>(1, size_t(nt)));
+    v.depth = buildHostBvhOrder(verts, nv, tris, nt, slot, true);
     v.slotTri = slot;
     v.nSlots = nt;
     HostBvh dir;
@@ -618,11 +644,120 @@ void DeviceBvh::upload(const BvhNode* hostNodes, int64_t nNodes, const double* s
     launchSlotBoxes(tri9, nSlots, box6, s);
 }
 
+namespace {
+size_t assemblyExtraBytes(int64_t nNodes, int64_t nSlots, int64_t nVerts, int nParts)
+{
+    return alignUp256(sizeof(double) * 3 * size_t(std::max<int64_t>(1, nVerts))) +
+           alignUp256(sizeof(int32_t) * 3 * size_t(std::max<int64_t>(1, nSlots))) +
+           alignUp256(sizeof(int32_t) * size_t(std::max<int64_t>(1, nNodes))) +
+           alignUp256(sizeof(BvhMeshPart) * size_t(std::max(1, nParts)));
+}
+} // namespace
+
+void DeviceBvh::reserveAssembly(int64_t nNodes, int64_t nSlots, int64_t nVerts, int nPartsMax,
+                                hipStream_t s)
+{
+    release();
+    stream = s;
+    SDFO_HIP(hipMallocAsync(&block,
+                            bvhBlockBytes(nNodes, nSlots) +
+                                assemblyExtraBytes(nNodes, nSlots, nVerts, nPartsMax),
+                            s));
+    capNodes = std::max<int64_t>(1, nNodes);
+    capSlots = std::max<int64_t>(1, nSlots);
+    capVerts = std::max<int64_t>(1, nVerts);
+    capParts = std::max(1, nPartsMax);
+    char* b = static_cast<char*>(block);
+    nodes = reinterpret_cast<BvhNode*>(b);
+    b += alignUp256(sizeof(BvhNode) * size_t(capNodes));
+    tri9 = reinterpret_cast<double*>(b);
+    b += alignUp256(sizeof(double) * 9 * size_t(capSlots));
+    box6 = reinterpret_cast<double*>(b);
+    b += alignUp256(sizeof(double) * 6 * size_t(capSlots));
+    triIdx = reinterpret_cast<int32_t*>(b);
+    b += alignUp256(sizeof(int32_t) * size_t(capSlots));
+    verts = reinterpret_cast<double*>(b);
+    b += alignUp256(sizeof(double) * 3 * size_t(capVerts));
+    tris = reinterpret_cast<int32_t*>(b);
+    b += alignUp256(sizeof(int32_t) * 3 * size_t(capSlots));
+    flags = reinterpret_cast<int32_t*>(b);
+    b += alignUp256(sizeof(int32_t) * size_t(capNodes));
+    parts = reinterpret_cast<BvhMeshPart*>(b);
+}
+
+void DeviceBvh::stageMeshes(const double* hostVerts, int64_t nVerts, const int32_t* hostTris,
+                            int64_t nTris, const BvhMeshPart* hostParts, int np, hipStream_t s)
+{
+    if (!verts || stream != s || nVerts > capVerts || nTris > capSlots || np > capParts)
+        throw Error(SDFO_ERR_INVALID_ARGUMENT, "device BVH: staging exceeds the reserved block");
+    int64_t nodesTotal = 0;
+    for (int q = 0; q < np; ++q) nodesTotal = std::max(nodesTotal, hostParts[q].nodeBase + bvhNodeCount(hostParts[q].nSlots));
+    if (nodesTotal > capNodes)
+        throw Error(SDFO_ERR_INVALID_ARGUMENT, "device BVH: staging exceeds the reserved block");
+    nParts = np;
+    nnodes = int(nodesTotal);
+    ntris = nTris;
+    // cut each mesh's tree where its subtrees hold at most kSubtreeSlots slots
+    std::vector<BvhMeshPart> pp(hostParts, hostParts + np);
+    nSubtrees = 0;
+    topLevelsMax = 0;
+    for (BvhMeshPart& q : pp)
+    {
+        if (q.nSlots > INT32_MAX / 2)
+            throw Error(SDFO_ERR_UNSUPPORTED, "device BVH: mesh too large");
+        for (int d = 0; d < 32; ++d)
+        {
+            q.f0[d] = int32_t(medianTreeNodes(q.nSlots >> d));
+            q.f1[d] = int32_t(medianTreeNodes((q.nSlots >> d) + 1));
+        }
+        // every node above the cut must be inner: depth L - 1 holds more than 8
+        int L = 0;
+        while (L < kMaxTopLevels && (q.nSlots >> (L + 1)) >= kMinSubtreeSlots) ++L;
+        q.topLevels = L;
+        q.subBase = nSubtrees;
+        nSubtrees += int64_t(1) << L;
+        topLevelsMax = std::max(topLevelsMax, L);
+    }
+    if (nVerts > 0)
+        SDFO_HIP(hipMemcpyAsync(verts, hostVerts, sizeof(double) * 3 * size_t(nVerts),
+                                hipMemcpyHostToDevice, s));
+    if (nTris > 0)
+        SDFO_HIP(hipMemcpyAsync(tris, hostTris, sizeof(int32_t) * 3 * size_t(nTris),
+                                hipMemcpyHostToDevice, s));
+    SDFO_HIP(hipMemcpyAsync(parts, pp.data(), sizeof(BvhMeshPart) * size_t(np), hipMemcpyHostToDevice, s));
+    SDFO_HIP(hipMemsetAsync(flags, 0, sizeof(int32_t) * size_t(std::max<int64_t>(1, nodesTotal)), s));
+}
+
+void DeviceBvh::assemble(const int32_t* slotTri, int treeDepth, const double* dir, hipStream_t s)
+{
+    depth = treeDepth;
+    for (int k = 0; k < 3; ++k) rayDir[k] = dir[k];
+    if (depth + 2 > kStackDepth)
+        throw Error(SDFO_ERR_UNSUPPORTED, "BVH depth " + std::to_string(depth) +
+                                              " exceeds the traversal stack");
+    if (nnodes >= (1 << 26))
+        throw Error(SDFO_ERR_UNSUPPORTED, "BVH with " + std::to_string(nnodes) +
+                                              " nodes exceeds the traversal stack encoding");
+    if (ntris > 0)
+        SDFO_HIP(hipMemcpyAsync(triIdx, slotTri, sizeof(int32_t) * size_t(ntris),
+                                hipMemcpyHostToDevice, s));
+    launchBvhAssemble(*this, s);
+}
+
 void DeviceBvh::release()
 {
     if (block) (void)hipFreeAsync(block, stream);
     block = nullptr;
     capNodes = capSlots = 0;
+    capVerts = 0;
+    capParts = 0;
+    nParts = 0;
+    verts = nullptr;
+    tris = nullptr;
+    flags = nullptr;
+    parts = nullptr;
+    nSubtrees = 0;
+    topLevelsMax = 0;
     nodes = nullptr;
     tri9 = nullptr;
     box6 = nullptr;
@@ -667,6 +802,79 @@ extern "C" int sdfo_debug_bvh_export(const double* verts, int64_t n_verts, const
     }
     catch (...)
     {
+        return SDFO_ERR_INVALID_ARGUMENT;
+    }
+}
+
+extern "C" int sdfo_debug_device_bvh_export(int32_t n_meshes, const double* verts,
+                                            const int64_t* vo, const int32_t* tris,
+                                            const int64_t* to, double* nodes6, int32_t* nodeI4,
+                                            int32_t* perm, double* tri9, double* box6,
+                                            int32_t max_nodes, int32_t* depth)
+{
+    sdfo::DeviceBvh db;
+    try
+    {
+        if (n_meshes <= 0 || !verts || !vo || !tris || !to) return SDFO_ERR_EMPTY_MESH;
+        std::vector<sdfo::BvhMeshPart> parts(static_cast<size_t>(n_meshes));
+        int64_t nodes = 0, slots = 0;
+        for (int32_t m = 0; m < n_meshes; ++m)
+        {
+            const int64_t nt = to[m + 1] - to[m];
+            if (nt <= 0 || vo[m + 1] <= vo[m]) return SDFO_ERR_EMPTY_MESH;
+            parts[size_t(m)] = sdfo::BvhMeshPart{slots, nt, nodes, vo[m] - vo[0], to[m] - to[0]};
+            nodes += sdfo::bvhNodeCount(nt);
+            slots += nt;
+        }
+        if (nodes > max_nodes) return int(-nodes);
+        std::vector<int32_t> slot(static_cast<size_t>(slots));
+        int d = 0;
+        for (int32_t m = 0; m < n_meshes; ++m)
+        {
+            const sdfo::BvhMeshPart& P = parts[size_t(m)];
+            d = std::max(d, sdfo::buildHostBvhOrder(verts + 3 * vo[m], vo[m + 1] - vo[m],
+                                                    tris + 3 * to[m], P.nSlots,
+                                                    slot.data() + P.slotBase, true));
+        }
+        sdfo::HostBvh dir;
+        sdfo::setRayDir(dir);
+        db.reserveAssembly(nodes, slots, vo[n_meshes] - vo[0], n_meshes, nullptr);
+        db.stageMeshes(verts + 3 * vo[0], vo[n_meshes] - vo[0], tris + 3 * to[0],
+                       to[n_meshes] - to[0], parts.data(), n_meshes, nullptr);
+        db.assemble(slot.data(), d, dir.rayDir, nullptr);
+        std::vector<sdfo::BvhNode> nd(static_cast<size_t>(nodes));
+        SDFO_HIP(hipMemcpy(nd.data(), db.nodes, sizeof(sdfo::BvhNode) * nd.size(), hipMemcpyDeviceToHost));
+        SDFO_HIP(hipMemcpy(perm, db.triIdx, sizeof(int32_t) * size_t(slots), hipMemcpyDeviceToHost));
+        SDFO_HIP(hipMemcpy(tri9, db.tri9, sizeof(double) * 9 * size_t(slots), hipMemcpyDeviceToHost));
+        SDFO_HIP(hipMemcpy(box6, db.box6, sizeof(double) * 6 * size_t(slots), hipMemcpyDeviceToHost));
+        SDFO_HIP(hipDeviceSynchronize());
+        db.release();
+        for (size_t i = 0; i < nd.size(); ++i)
+        {
+            for (int k = 0; k < 3; ++k)
+            {
+                nodes6[6 * i + k] = nd[i].bmin[k];
+                nodes6[6 * i + 3 + k] = nd[i].bmax[k];
+            }
+            nodeI4[4 * i + 0] = nd[i].left;
+            nodeI4[4 * i + 1] = nd[i].right;
+            nodeI4[4 * i + 2] = nd[i].triStart;
+            nodeI4[4 * i + 3] = nd[i].triCount;
+        }
+        if (depth) *depth = d;
+        return int(nodes);
+    }
+    catch (const sdfo::Error& e)
+    {
+        (void)hipDeviceSynchronize();
+        db.release();
+        sdfo::setLastError(e.what());
+        return e.code;
+    }
+    catch (...)
+    {
+        (void)hipDeviceSynchronize();
+        db.release();
         return SDFO_ERR_INVALID_ARGUMENT;
     }
 }

@@ -64,6 +64,28 @@ struct alignas(16) BvhNode
 };
 static_assert(sizeof(BvhNode) == 64, "BvhNode layout");
 
+// Node count of the tree KdTree::buildRecursive makes over c triangles
+// (KdTree.h:188-242): the split is always at c / 2 and leaves hold <= 8, so
+// the shape depends on c only.  All nodes at depth d hold floor(c / 2^d) or
+// one more, so the count is carried bottom-up for that pair of sizes:
+// O(log c) instead of a walk over the whole subtree.
+__host__ __device__ inline int64_t medianTreeNodes(int64_t c)
+{
+    if (c <= 8) return 1;
+    int D = 0;
+    while ((c >> D) + 1 > 8) ++D;
+    int64_t fa = 1, fa1 = 1; // nodes over (c >> D) and (c >> D) + 1 items: single leaves
+    for (int d = D - 1; d >= 0; --d)
+    {
+        const int64_t a = c >> d, b = a >> 1; // children sizes are b or b + 1
+        const int64_t fb = fa, fb1 = fa1;
+        const int64_t ha = a >> 1, ha1 = (a + 1) >> 1;
+        fa = a <= 8 ? 1 : 1 + (ha == b ? fb : fb1) + ((a - ha) == b ? fb : fb1);
+        fa1 = a + 1 <= 8 ? 1 : 1 + (ha1 == b ? fb : fb1) + ((a + 1 - ha1) == b ? fb : fb1);
+    }
+    return fa;
+}
+
 // Host BVH, built exactly like KdTree::build (KdTree.h:91-121, 188-242) and
 // flattened: nodes in creation (pre-)order, triangles stored per BVH slot.
 struct HostBvh
@@ -82,6 +104,11 @@ int buildHostBvhInto(const double* verts, int64_t nv, const int32_t* tris, int64
                      BvhNode* nodes, double* slotTri9, int32_t* slotTri, int nodeBase, int slotBase,
                      bool parallel);
 int64_t bvhNodeCount(int64_t nt);
+// Only the slot order of that tree (slotTri[s] = triangle of slot s) and its
+// depth: the node boxes and slot triangles are assembled on the device
+// (DeviceBvh::assemble), since the tree's shape follows from nt alone.
+int buildHostBvhOrder(const double* verts, int64_t nv, const int32_t* tris, int64_t nt,
+                      int32_t* slotTri, bool parallel);
 void setRayDir(HostBvh& h);
 // The same tree in the calling thread's reusable host buffers (valid until
 // that thread's next scratch build): no zero fill, no fresh pages per build.
@@ -96,6 +123,17 @@ struct HostBvhView
     double rayDir[3];
 };
 HostBvhView buildHostBvhScratch(const double* verts, int64_t nv, const int32_t* tris, int64_t nt);
+// Only the slot order and depth (buildHostBvhOrder), in the calling thread's
+// scratch arrays (valid until its next scratch build).
+struct HostBvhOrderView
+{
+    const int32_t* slotTri;
+    int64_t nSlots;
+    int depth;
+    double rayDir[3];
+};
+HostBvhOrderView buildHostBvhOrderScratch(const double* verts, int64_t nv, const int32_t* tris,
+                                          int64_t nt);
 // Frees the calling thread's scratch arrays if they hold more than keepTris
 // triangles' worth.
 void releaseHostBvhScratch(int64_t keepTris);
@@ -118,6 +156,8 @@ int hostThreads();
 // items on the worker threads (the calling thread takes the first chunk).
 void parallelRange(int64_t n, int64_t grain, const std::function<void(int64_t, int64_t)>& f);
 
+struct BvhMeshPart;
+
 // Device copy of a BVH.
 struct DeviceBvh
 {
@@ -137,6 +177,26 @@ struct DeviceBvh
     int64_t capNodes = 0, capSlots = 0;
     void reserve(int64_t nNodes, int64_t nSlots, hipStream_t s);
     void upload(const HostBvh& h, hipStream_t s);
+    // Assembly on the device.  reserveAssembly allocates the block with room
+    // for the meshes' vertices and triangles too; stageMeshes uploads them
+    // (parts give each mesh's slot, node, vertex and triangle offsets, all
+    // meshes' slots concatenated in part order); assemble uploads the slot
+    // order (slotTri[s] = the mesh-local triangle of slot s, as
+    // buildHostBvhOrder makes it) and builds nodes, slot triangles and slot
+    // boxes in one kernel, equal to the host tree bit for bit.
+    void reserveAssembly(int64_t nNodes, int64_t nSlots, int64_t nVerts, int nParts, hipStream_t s);
+    void stageMeshes(const double* verts, int64_t nVerts, const int32_t* tris, int64_t nTris,
+                     const BvhMeshPart* parts, int nParts, hipStream_t s);
+    void assemble(const int32_t* slotTri, int treeDepth, const double* dir, hipStream_t s);
+    double* verts = nullptr;
+    int32_t* tris = nullptr;
+    int32_t* flags = nullptr;
+    BvhMeshPart* parts = nullptr;
+    int nParts = 0;
+    int64_t nSubtrees = 0; // assembly workgroups of the staged meshes
+    int topLevelsMax = 0;
+    int64_t capVerts = 0;
+    int capParts = 0;
     // The same from raw host arrays; the slot boxes are derived on the device.
     void upload(const BvhNode* hostNodes, int64_t nNodes, const double* slotTri9,
                 const int32_t* slotTri, int64_t nSlots, int treeDepth, const double* dir,
@@ -146,6 +206,29 @@ struct DeviceBvh
 
 // Slot boxes (bmin3, bmax3 of each slot triangle's vertices) on the device (build.hip).
 void launchSlotBoxes(const double* tri9, int64_t n, double* box6, hipStream_t s);
+
+// One mesh of a device BVH assembly: its slots [slotBase, slotBase + nSlots)
+// and nodes from nodeBase in the concatenated arrays, its vertices and
+// triangles from vertBase / triBase in the staged arrays (triangle indices
+// are local to the mesh's vertices).
+struct BvhMeshPart
+{
+    int64_t slotBase, nSlots, nodeBase, vertBase, triBase;
+    // Filled by DeviceBvh::stageMeshes: the mesh's tree is assembled as
+    // 2^topLevels subtrees (workgroups subBase ...) plus the nodes above
+    // them; f0[d] / f1[d] are the node counts of the subtrees over
+    // nSlots >> d and (nSlots >> d) + 1 slots, the two sizes at depth d.
+    int64_t subBase = 0;
+    int32_t topLevels = 0, pad = 0;
+    int32_t f0[32] = {}, f1[32] = {};
+};
+// The assembly cuts a mesh's tree into at most 2^kMaxTopLevels subtrees, of
+// at least kMinSubtreeSlots slots when the mesh is larger.
+constexpr int kMaxTopLevels = 8;
+constexpr int64_t kMinSubtreeSlots = 256;
+// Nodes, slot triangles and slot boxes of db from its staged meshes and slot
+// order; db.flags must be zero (build.hip).
+void launchBvhAssemble(const DeviceBvh& db, hipStream_t s);
 
 // Nearest / signed nearest triangle of a point stream (build.hip).
 void launchMeshQuery(const DeviceBvh& bvh, bool signedDist, const double* pts, int64_t n,

@@ -168,6 +168,7 @@ _sig("sdfo_contact_backward_saved", [_vp, _vp, _vp, _i64, _vp, _i64, _P(ContactP
                                      _i64, _vp, _vp, _vp, _vp, _vp])
 
 _sig("sdfo_debug_bvh_export", [_vp, _i64, _vp, _i64, _vp, _vp, _vp, _i32, _vp])
+_sig("sdfo_debug_device_bvh_export", [_i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _vp])
 _sig("sdfo_debug_random_read_floor", [_i32, _i64, _i64, _i32, _P(C.c_double)])
 _sig("sdfo_debug_sort_by_abs", [_vp, _i32, _vp])
 _sig("sdfo_debug_build_phases", [_vp, _i32])
@@ -247,6 +248,31 @@ def debug_bvh_export(verts, tris):
                                    C.byref(depth))
     check(nn if nn < 0 else 0)
     return n6[:nn].copy(), i4[:nn].copy(), perm, depth.value
+
+
+def debug_device_bvh_export(meshes):
+    """The device-assembled BVH of [(verts, tris), ...] concatenated:
+    (nodes6, nodeI4, perm, tri9, box6, depth) -- test support, current device."""
+    import numpy as np
+
+    vs = [np.ascontiguousarray(v, np.float64).reshape(-1, 3) for v, _ in meshes]
+    ts = [np.ascontiguousarray(t, np.int32).reshape(-1, 3) for _, t in meshes]
+    vo = np.concatenate([[0], np.cumsum([len(v) for v in vs])]).astype(np.int64)
+    to = np.concatenate([[0], np.cumsum([len(t) for t in ts])]).astype(np.int64)
+    v = np.ascontiguousarray(np.concatenate(vs))
+    t = np.ascontiguousarray(np.concatenate(ts))
+    cap = 2 * len(t) + len(ts)
+    n6 = np.zeros((cap, 6))
+    i4 = np.zeros((cap, 4), np.int32)
+    perm = np.zeros(len(t), np.int32)
+    tri9 = np.zeros((len(t), 9))
+    box6 = np.zeros((len(t), 6))
+    depth = C.c_int32()
+    p = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+    nn = lib.sdfo_debug_device_bvh_export(len(ts), p(v), p(vo), p(t), p(to), p(n6), p(i4), p(perm),
+                                          p(tri9), p(box6), cap, C.byref(depth))
+    check(nn if nn < 0 else 0)
+    return n6[:nn].copy(), i4[:nn].copy(), perm, tri9, box6, depth.value
 
 
 def check(rc: int) -> None:

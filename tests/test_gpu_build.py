@@ -181,3 +181,59 @@ def test_wave_search_replays_few_voxels(sdfo_gpu):
     n = np.prod(g.spec.shape)
     print(f"[gpu] ring 128: replayed {ph['replayed_voxels']} of {n}; phases {ph}", flush=True)
     assert 0 <= ph["replayed_voxels"] < 0.01 * n
+
+
+def _soup_with_signed_zeros(n_tris, seed):
+    """A triangle soup whose coordinates include +0.0 and -0.0 (the host tree's
+    min / max keep the first of equal arguments, which fixes a zero bound's sign)."""
+    rng = np.random.default_rng(seed)
+    v = rng.uniform(-1.0, 1.0, (3 * n_tris, 3))
+    z = rng.random(v.shape) < 0.2
+    v[z] = np.where(rng.random(int(z.sum())) < 0.5, -0.0, 0.0)
+    return v, np.arange(3 * n_tris, dtype=np.int32).reshape(-1, 3)
+
+
+def test_device_assembled_bvh_equals_host_tree(sdfo_gpu):
+    """The builds assemble the BVH on the device from the host's slot order
+    (shape from the triangle count, boxes folded as on the host): nodes, links,
+    slot triangles and slot boxes equal the host tree bit for bit, for one mesh
+    and for several uneven meshes concatenated as the batched build does."""
+    from sdfo import _lib
+
+    cases = [meshes.torus(0.05, 0.01, 64, 32), meshes.cube(0.1), _soup_with_signed_zeros(5, 1),
+             _soup_with_signed_zeros(9, 2), _soup_with_signed_zeros(1000, 3),
+             meshes.gear(40, 12, 0.02, 0.002, 12), _soup_with_signed_zeros(17, 4),
+             meshes.hollow_cylinder(0.04, 0.03, 0.05, 256, 32)]
+    groups = [[c] for c in cases] + [cases]
+    for gi, group in enumerate(groups):
+        print(f"[gpu] device BVH group {gi}: {[len(t) for _, t in group]} tris", flush=True)
+        n6, i4, perm, tri9, box6, depth = _lib.debug_device_bvh_export(group)
+        node_base = slot_base = 0
+        want_depth = 0
+        for v, t in group:
+            h6, hi4, hperm, hdepth = _lib.debug_bvh_export(v, t)
+            nn, ns = len(h6), len(t)
+            want_depth = max(want_depth, hdepth)
+            hi4 = hi4.copy()
+            leaf = hi4[:, 3] > 0
+            hi4[leaf, 2] += slot_base
+            hi4[~leaf, 0] += node_base
+            hi4[~leaf, 1] += node_base
+            assert np.array_equal(n6[node_base:node_base + nn].view(np.uint64), h6.view(np.uint64))
+            assert np.array_equal(i4[node_base:node_base + nn], hi4)
+            assert np.array_equal(perm[slot_base:slot_base + ns], hperm)
+            vv = np.asarray(v, np.float64).reshape(-1, 3)
+            tt = np.asarray(t, np.int64).reshape(-1, 3)
+            corners = vv[tt[hperm]]  # (ns, 3, 3)
+            assert np.array_equal(tri9[slot_base:slot_base + ns].view(np.uint64),
+                                  corners.reshape(ns, 9).view(np.uint64))
+            a, b, c = corners[:, 0], corners[:, 1], corners[:, 2]
+            lo = np.where(c < np.where(b < a, b, a), c, np.where(b < a, b, a))
+            hm = np.where(a < b, b, a)
+            hi = np.where(hm < c, c, hm)
+            assert np.array_equal(box6[slot_base:slot_base + ns].view(np.uint64),
+                                  np.concatenate([lo, hi], 1).view(np.uint64))
+            node_base += nn
+            slot_base += ns
+        assert len(n6) == node_base
+        assert depth == want_depth
