@@ -402,6 +402,17 @@ def bench_batched_gears(args):
     print(json.dumps(line))
 
 
+def _morton_order(pts, spec):
+    """Order of points by the Morton (z-order) code of their grid cells."""
+    cell = np.floor((pts - np.asarray(spec.bmin)) / np.asarray(spec.voxel)).astype(np.int64)
+    cell = np.clip(cell, 0, (1 << 20) - 1)
+    code = np.zeros(len(pts), np.uint64)
+    for b in range(20):
+        for a in range(3):
+            code |= ((cell[:, a] >> b) & 1).astype(np.uint64) << np.uint64(3 * b + a)
+    return np.argsort(code, kind="stable")
+
+
 def _random_read_floor(nq, shape, q_ms):
     """The memory system's floor for the record-path query: nq uniformly
     random 128-byte reads from an array the size of the grid's cell records
@@ -471,6 +482,10 @@ def main():
     cube = scenes.cube_mesh()
     _lib.debug_skip_cell_records(args.cell_records == "off")
     cube_grid = sdfo.SdfGrid.build_dense(*cube, target_resolution=128, max_resolution=128)
+    # the same grid without cell records: the per-corner gather from the FP32
+    # mirror, for the spatially coherent stream below
+    _lib.debug_skip_cell_records(True)
+    cube_gather = sdfo.SdfGrid.build_dense(*cube, target_resolution=128, max_resolution=128)
     _lib.debug_skip_cell_records(False)
     spec = cube_grid.spec
     nq = args.query_points
@@ -478,6 +493,9 @@ def main():
     # distinct point sets (and outputs) for the passes of one step
     sets_np = [scenes.valid_region_points(spec, nq, seed=1 + r) for r in range(reps)]
     sets = [torch.from_numpy(p).to(dev) for p in sets_np]
+    # the same point sets in Morton order of their cells (a spatially coherent
+    # stream, as a caller that bins its points would hand over)
+    sets_z = [torch.from_numpy(p[_morton_order(p, spec)]).to(dev) for p in sets_np]
     outs = [(torch.empty(nq, dtype=torch.float32, device=dev),
              torch.empty((nq, 3), dtype=torch.float32, device=dev),
              torch.empty(nq, dtype=torch.uint8, device=dev)) for _ in range(reps)]
@@ -539,7 +557,7 @@ def main():
             rec["build_s"].append(time.perf_counter() - t0)
             rec["phases"].append(_lib.debug_build_phases())
             g.close()
-        e = [ev() for _ in range(18)]
+        e = [ev() for _ in range(22)]
         # (2) 1e6-point tricubic queries: each timed segment starts behind a
         # short device spin so the host's launch latency is not in the events.
         # (2a) distinct point sets and outputs, back to back
@@ -556,6 +574,20 @@ def main():
             for _ in range(reps):
                 cube_grid.query_into(sets[0], *outs[0], "Tricubic")
         e[3].record(stream)
+        # (2b') the point sets in Morton order, back to back
+        busy()
+        e[18].record(stream)
+        if "query" in parts:
+            for r in range(reps):
+                cube_grid.query_into(sets_z[r], *outs[r], "Tricubic")
+        e[19].record(stream)
+        # (2b'') the same on the per-corner gather path
+        busy()
+        e[20].record(stream)
+        if "query" in parts:
+            for r in range(reps):
+                cube_gather.query_into(sets_z[r], *outs[r], "Tricubic")
+        e[21].record(stream)
         # (2c) FP64 points, the exact path
         busy()
         e[4].record(stream)
@@ -606,6 +638,8 @@ def main():
         torch.cuda.synchronize()
         rec["query_ms"].append(e[0].elapsed_time(e[1]) / reps)
         rec["query_warm_ms"].append(e[2].elapsed_time(e[3]) / reps)
+        rec["query_z_ms"].append(e[18].elapsed_time(e[19]) / reps)
+        rec["query_zg_ms"].append(e[20].elapsed_time(e[21]) / reps)
         rec["query64_ms"].append(e[4].elapsed_time(e[5]))
         rec["contact_ms"].append(e[6].elapsed_time(e[7]))
         rec["eval_ms"].append(e[8].elapsed_time(e[9]))
@@ -615,7 +649,9 @@ def main():
         rec["lin_ms"].append(e[14].elapsed_time(e[15]))
 
     def new_rec():
-        return dict(build_s=[], query_ms=[], query_warm_ms=[], query64_ms=[], contact_ms=[],
+        return dict(build_s=[], query_ms=[], query_warm_ms=[], query_z_ms=[], query_zg_ms=[],
+                    query64_ms=[],
+                    contact_ms=[],
                     eval_ms=[], fwdbwd_ms=[], fwdbwd_screen_ms=[], plain_ms=[], lin_ms=[],
                     phases=[])
 
@@ -633,6 +669,7 @@ def main():
     K = args.steps
     med = lambda k: max(float(np.median(rec[k])), 1e-9) if rec[k] else 1e-9  # noqa: E731
     q_ms, qw_ms, q64_ms = med("query_ms"), med("query_warm_ms"), med("query64_ms")
+    qz_ms, qzg_ms = med("query_z_ms"), med("query_zg_ms")
     c_ms, e_ms, fb_ms = med("contact_ms"), med("eval_ms"), med("fwdbwd_ms")
     pl_ms, lin_ms, fbs_ms = med("plain_ms"), med("lin_ms"), med("fwdbwd_screen_ms")
     build_s = med("build_s")
@@ -672,6 +709,16 @@ def main():
         "query_kernel_ms": round(q_ms, 4),
         "query_same_points_kernel_ms": round(qw_ms, 4),
         "query_same_points_mpts_per_s": round(nq / (qw_ms * 1e-3) / 1e6, 3),
+        "query_morton_order_kernel_ms": round(qz_ms, 4),
+        "query_morton_order_mpts_per_s": round(nq / (qz_ms * 1e-3) / 1e6, 3),
+        "query_morton_order_roofline": _hbm_roofline(
+            alg_bytes, qz_ms * 1e-3, "queryPairKernel<float, float, 3>, the same five point "
+            "sets in Morton order of their cells (sorted on the host before timing)"),
+        "query_morton_order_gather_kernel_ms": round(qzg_ms, 4),
+        "query_morton_order_gather_mpts_per_s": round(nq / (qzg_ms * 1e-3) / 1e6, 3),
+        "query_morton_order_gather_roofline": _hbm_roofline(
+            alg_bytes, qzg_ms * 1e-3, "queryPairKernel<float, float, 3> gathering corners from "
+            "the FP32 mirror (grid built without cell records), Morton-ordered point sets"),
         "tricubic_f64_points_mpts_per_s": round(nq / (q64_ms * 1e-3) / 1e6, 3),
         "contact_fwd_evals_per_s": round(n_eval / (c_ms * 1e-3), 1),
         "contact_evaluate_world_evals_per_s": round(n_eval / (e_ms * 1e-3), 1),
@@ -723,7 +770,11 @@ def main():
     skipped = {
         "build": ("build_mvoxel_per_s", "build_phases_ms", "build_ms", "build_roofline"),
         "query": ("query_kernel_ms", "query_same_points_kernel_ms", "query_same_points_mpts_per_s",
-                  "tricubic_f64_points_mpts_per_s", "roofline", "query_random_read_floor"),
+                  "query_morton_order_kernel_ms", "query_morton_order_mpts_per_s",
+                  "query_morton_order_roofline", "query_morton_order_gather_kernel_ms",
+                  "query_morton_order_gather_mpts_per_s", "query_morton_order_gather_roofline",
+                  "tricubic_f64_points_mpts_per_s", "roofline",
+                  "query_random_read_floor"),
         "contact": ("contact_fwd_evals_per_s", "contact_evaluate_world_evals_per_s",
                     "contact_linearized_evals_per_s", "linearized_vs_direct", "contact_roofline"),
         "bwd": ("contact_fwdbwd_evals_per_s", "contact_fwdbwd_screening_backward_evals_per_s",

@@ -77,6 +77,12 @@ int sdfo_debug_ca_merge_cap(int32_t first, int32_t second);
  * per-corner gather path against the CPU oracle on small grids. */
 int sdfo_debug_skip_cell_records(int32_t on);
 
+/* Test switch: records variant as the autotuner's decision for key (a key
+ * as the tuner report prints it, e.g. "contact_forward points~2^12
+ * poses~2^8"), so tests can run every launch variant of a kernel and check
+ * that they give the same bits.  sdfo_tuner_reset clears it. */
+int sdfo_debug_tuner_pin(const char* key, int32_t variant);
+
 /* Measurement support: the best time in microseconds (best_us) over reps
  * launches (after one warm-up) of n_reads uniformly random reads of
  * record_bytes-byte records (64 or 128) from a device array of array_bytes,

@@ -389,10 +389,11 @@ __device__ __forceinline__ void forwardBlock(const QGrid& g, const Law& L, const
     }
 }
 
-// The forward kernels: four waves per SIMD (128 registers; +10% over three
-// waves for the 1000-pose forward).
-template <bool Cached, bool Culled, int Kind = kFieldAny>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4)))
+// The forward kernels: W waves per SIMD, four by default (128 registers;
+// +10% over three waves for the 1000-pose forward), three as the tuner's
+// alternative (no spill); the sums are the same either way.
+template <bool Cached, bool Culled, int Kind = kFieldAny, int W = 4>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(W)))
 contactForwardKernel(QGrid g, Law L, const double* __restrict__ X, const double* __restrict__ w,
                      int64_t n, const double* __restrict__ poses, int64_t chunkLen,
                      double* __restrict__ partial, LinCache lc, CullArgs cu,
@@ -403,8 +404,8 @@ contactForwardKernel(QGrid g, Law L, const double* __restrict__ X, const double*
                                               anyLeaf, PairList{}, sh, nullptr);
 }
 
-template <int Kind>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4)))
+template <int Kind, int W = 4>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(W)))
 contactForwardSaveKernel(QGrid g, Law L, const double* __restrict__ X, const double* __restrict__ w,
                          int64_t n, const double* __restrict__ poses, int64_t chunkLen,
                          double* __restrict__ partial, PairList save)
@@ -1246,6 +1247,24 @@ PairList savedPairs(void* saved, const ContactLayout& o)
                     reinterpret_cast<uint8_t*>(b + o.oAct)};
 }
 
+// Occupancy variants of the forward kernels the tuner chooses from (the
+// default first).
+constexpr int kFwdVariants = 2;
+const char* const kFwdVariantNames[kFwdVariants] = {"4waves", "3waves"};
+
+std::string forwardKey(const char* what, int64_t n, int64_t nb)
+{
+    return std::string(what) + " points~2^" + std::to_string(log2Bucket(n)) + " poses~2^" +
+           std::to_string(log2Bucket(nb));
+}
+
+// Persistent grid of the backward's piece kernel the tuner chooses from, in
+// blocks per CU (the default first); the pieces and their sums are fixed by
+// the piece table, so every variant gives the same bits.
+constexpr int kPieceGridCount = 4;
+constexpr int kPieceGrids[kPieceGridCount] = {8, 4, 16, 32};
+const char* const kPieceGridNames[kPieceGridCount] = {"8/CU", "4/CU", "16/CU", "32/CU"};
+
 // Forward launch shared by the plain, linearised and culled entry points;
 // with saved (plain forward only), the pair record is written too.
 template <bool Cached>
@@ -1272,9 +1291,17 @@ void forwardLaunch(const QGrid& q, const Law& L, const double* X, const double* 
     }
     else if (rec)
     {
-        hipLaunchKernelGGL(contactForwardSaveKernel<kFieldAny>, grid, dim3(kBlock), 0, s, q, L, X, w,
-                           n, poses, chunkLen, partial, savedPairs(saved, lay));
-        SDFO_HIP(hipGetLastError());
+        auto launchSave = [&](int var) {
+            if (var == 1)
+                hipLaunchKernelGGL((contactForwardSaveKernel<kFieldAny, 3>), grid, dim3(kBlock), 0, s, q,
+                                   L, X, w, n, poses, chunkLen, partial, savedPairs(saved, lay));
+            else
+                hipLaunchKernelGGL((contactForwardSaveKernel<kFieldAny, 4>), grid, dim3(kBlock), 0, s, q,
+                                   L, X, w, n, poses, chunkLen, partial, savedPairs(saved, lay));
+            SDFO_HIP(hipGetLastError());
+        };
+        launchSave(tunedVariant(forwardKey("contact_forward_save", n, nb), s, kFwdVariants,
+                                kFwdVariantNames, launchSave));
         // the lists and the force finalize in one launch
         const int64_t blocks = nb * chunks + (nb + kBlock - 1) / kBlock;
         hipLaunchKernelGGL(pairListKernel, dim3(unsigned(blocks)), dim3(kBlock), 0, s, n, nb, chunks,
@@ -1284,8 +1311,19 @@ void forwardLaunch(const QGrid& q, const Law& L, const double* X, const double* 
         return;
     }
     else
-        hipLaunchKernelGGL((contactForwardKernel<Cached, false>), grid, dim3(kBlock), 0, s, q, L,
-                           X, w, n, poses, chunkLen, partial, lc, CullArgs{}, nullptr);
+    {
+        auto launchFwd = [&](int var) {
+            if (var == 1)
+                hipLaunchKernelGGL((contactForwardKernel<Cached, false, kFieldAny, 3>), grid, dim3(kBlock),
+                                   0, s, q, L, X, w, n, poses, chunkLen, partial, lc, CullArgs{}, nullptr);
+            else
+                hipLaunchKernelGGL((contactForwardKernel<Cached, false, kFieldAny, 4>), grid, dim3(kBlock),
+                                   0, s, q, L, X, w, n, poses, chunkLen, partial, lc, CullArgs{}, nullptr);
+            SDFO_HIP(hipGetLastError());
+        };
+        launchFwd(tunedVariant(forwardKey(Cached ? "contact_forward_linearized" : "contact_forward", n, nb),
+                               s, kFwdVariants, kFwdVariantNames, launchFwd));
+    }
     SDFO_HIP(hipGetLastError());
     hipLaunchKernelGGL(contactForwardFinalize, dim3(unsigned((nb + 255) / 256)), dim3(256), 0, s,
                        nb, chunks, static_cast<const double*>(partial), out, active);
@@ -1539,18 +1577,22 @@ void launchContactBackward(const sdfo_grid& g, const double* X, const double* w,
                            int64_t(cnt) * chunks, chunks, maxPieces,
                            static_cast<const int32_t*>(gl.count), items, poseStart, total);
         SDFO_HIP(hipGetLastError());
-        // two waves per SIMD (the kernel's register budget), four SIMDs per CU
-        auto pieces = [&](auto kindTag) {
-            hipLaunchKernelGGL(contactBackwardPieceKernel<decltype(kindTag)::value>,
-                               dim3(unsigned(8 * cus)), dim3(64), 0, s, q, L, X, w, n, poses, gOut,
-                               chunks, chunkLen, maxPieces, base,
+        // persistent one-wave blocks: by default eight per CU (two waves per
+        // SIMD, the kernel's register budget, four SIMDs per CU)
+        auto pieces = [&](int var) {
+            if (var < 0 || var >= kPieceGridCount) var = 0;
+            hipLaunchKernelGGL(contactBackwardPieceKernel<kFieldAny>,
+                               dim3(unsigned(kPieceGrids[var] * cus)), dim3(64), 0, s, q, L, X, w, n,
+                               poses, gOut, chunks, chunkLen, maxPieces, base,
                                static_cast<const int32_t*>(gl.list),
                                static_cast<const int32_t*>(gl.count),
                                static_cast<const int32_t*>(items),
                                static_cast<const int32_t*>(total), pieceSum, pointPartial);
+            SDFO_HIP(hipGetLastError());
         };
-        pieces(std::integral_constant<int, kFieldAny>{});
-        SDFO_HIP(hipGetLastError());
+        pieces(tunedVariant("contact_backward_pieces pairs~2^" +
+                                std::to_string(log2Bucket(int64_t(cnt) * n)),
+                            s, kPieceGridCount, kPieceGridNames, pieces));
         // the tail: point reduction, pose finalize and, for a single group,
         // the parameter sum, as the blocks of one launch
         const int64_t nReduce = wantPoint ? (n + kReducePoints - 1) / kReducePoints : 0;

@@ -72,3 +72,10 @@ extern "C" int sdfo_debug_random_read_floor(int32_t record_bytes, int64_t n_read
         return SDFO_ERR_INVALID_ARGUMENT;
     }
 }
+
+extern "C" int sdfo_debug_tuner_pin(const char* key, int32_t variant)
+{
+    if (!key || variant < 0) return SDFO_ERR_INVALID_ARGUMENT;
+    sdfo::tunerRecord(key, {{int(variant), 0.0}}, nullptr);
+    return SDFO_OK;
+}

@@ -186,27 +186,38 @@ __device__ __forceinline__ void pairQuery(const QGrid& g, const P3<TP>* __restri
     }
 }
 
-// Trilinear / Tricubic, two points per thread: both points' corner samples
-// (one 128-byte record each, or eight mirror loads) are requested before
-// either is interpolated, doubling the loads in flight per lane (the gather
-// is latency-bound at one point per lane).  Same arithmetic as queryKernel.
-template <typename TP, typename T, int Mode, int Src, int Block = kPairBlock>
+// Trilinear / Tricubic, Pts points per thread (two by default): every
+// point's corner samples (one 128-byte record each, or eight mirror loads)
+// are requested before any is interpolated, multiplying the loads in flight
+// per lane (the gather is latency-bound at one point per lane).  Same
+// arithmetic as queryKernel.
+template <typename TP, typename T, int Mode, int Src, int Block = kPairBlock, int Pts = kLanePoints>
 __global__ void __launch_bounds__(Block)
 queryPairKernel(QGrid g, const P3<TP>* __restrict__ pts, int64_t n, TP* __restrict__ gap,
                 P3<TP>* __restrict__ grad3, uint8_t* __restrict__ valid,
                 P3<TP>* __restrict__ normal3, P3<TP>* __restrict__ witness3,
                 int32_t* __restrict__ branch)
 {
-    const int64_t i0 = int64_t(blockIdx.x) * (kLanePoints * Block) + threadIdx.x;
-    pairQuery<TP, T, Mode, Src, kLanePoints>(g, pts, i0, Block, n, gap, grad3, valid, normal3, witness3,
-                           branch);
+    const int64_t i0 = int64_t(blockIdx.x) * (Pts * Block) + threadIdx.x;
+    pairQuery<TP, T, Mode, Src, Pts>(g, pts, i0, Block, n, gap, grad3, valid, normal3, witness3,
+                                     branch);
 }
 
-// Block sizes of the record-path query the tuner chooses from (the default
-// first); every variant computes the same outputs.
+// Launch shapes of the record-path query the tuner chooses from (the default
+// first): points per lane x block size.  Every variant computes the same
+// outputs.
 constexpr int kPairBlockCount = 4;
 constexpr int kPairBlocks[kPairBlockCount] = {128, 64, 256, 512};
-const char* const kPairBlockNames[kPairBlockCount] = {"block128", "block64", "block256", "block512"};
+constexpr int kPairShapeCount = 2 * kPairBlockCount;
+// On a grid with cell records the tuner also tries the per-corner gather from
+// the FP32 mirror (the records' eighth of the bytes, which neighbouring
+// points share through the caches when the stream is spatially coherent).
+constexpr int kPairVariantCount = 2 * kPairShapeCount;
+const char* const kPairVariantNames[kPairVariantCount] = {
+    "rec/2pts/block128",    "rec/2pts/block64",    "rec/2pts/block256",    "rec/2pts/block512",
+    "rec/1pt/block128",     "rec/1pt/block64",     "rec/1pt/block256",     "rec/1pt/block512",
+    "gather/2pts/block128", "gather/2pts/block64", "gather/2pts/block256", "gather/2pts/block512",
+    "gather/1pt/block128",  "gather/1pt/block64",  "gather/1pt/block256",  "gather/1pt/block512"};
 
 // One point stream of a batched query: its grid, its points and its range of
 // 128-point work units (one wave each, two points per lane) in the launch.
@@ -324,28 +335,39 @@ void launchQuery(const sdfo_grid& g, int mode, const TP* pts, int64_t n, TP* gap
     {
         const bool rec = q.sparse ? q.scellRec != nullptr : q.cellRec != nullptr;
         const int src = rec ? (q.sparse ? kAnyRec : kDenseRec) : (q.sparse ? kSparseGather : kDenseGather);
-        auto launchBlock = [&](auto srcTag, auto blockTag) {
+        auto launchBlock = [&](auto srcTag, auto blockTag, auto ptsTag) {
             constexpr int Src = decltype(srcTag)::value;
             constexpr int B = decltype(blockTag)::value;
-            const dim3 pg(unsigned((n + kLanePoints * B - 1) / (kLanePoints * B))), pb(B);
+            constexpr int Pts = decltype(ptsTag)::value;
+            const dim3 pg(unsigned((n + Pts * B - 1) / (Pts * B))), pb(B);
             if (cubic)
-                hipLaunchKernelGGL((queryPairKernel<TP, T, SDFO_TRICUBIC, Src, B>), pg, pb, 0, s, q,
-                                   p, n, gap, gr, valid, nm, wt, branch);
+                hipLaunchKernelGGL((queryPairKernel<TP, T, SDFO_TRICUBIC, Src, B, Pts>), pg, pb, 0, s,
+                                   q, p, n, gap, gr, valid, nm, wt, branch);
             else
-                hipLaunchKernelGGL((queryPairKernel<TP, T, SDFO_TRILINEAR, Src, B>), pg, pb, 0, s, q,
-                                   p, n, gap, gr, valid, nm, wt, branch);
+                hipLaunchKernelGGL((queryPairKernel<TP, T, SDFO_TRILINEAR, Src, B, Pts>), pg, pb, 0, s,
+                                   q, p, n, gap, gr, valid, nm, wt, branch);
         };
+        const int nVariants = rec ? kPairVariantCount : kPairShapeCount;
         auto launch = [&](int variant) {
-            auto withSrc = [&](auto srcTag) {
-                switch (variant)
+            if (variant < 0 || variant >= nVariants) variant = 0;
+            const int from = variant >= kPairShapeCount ? (q.sparse ? kSparseGather : kDenseGather) : src;
+            variant %= kPairShapeCount;
+            auto withPts = [&](auto srcTag, auto ptsTag) {
+                switch (variant % kPairBlockCount)
                 {
-                case 1: launchBlock(srcTag, std::integral_constant<int, kPairBlocks[1]>{}); break;
-                case 2: launchBlock(srcTag, std::integral_constant<int, kPairBlocks[2]>{}); break;
-                case 3: launchBlock(srcTag, std::integral_constant<int, kPairBlocks[3]>{}); break;
-                default: launchBlock(srcTag, std::integral_constant<int, kPairBlocks[0]>{}); break;
+                case 1: launchBlock(srcTag, std::integral_constant<int, kPairBlocks[1]>{}, ptsTag); break;
+                case 2: launchBlock(srcTag, std::integral_constant<int, kPairBlocks[2]>{}, ptsTag); break;
+                case 3: launchBlock(srcTag, std::integral_constant<int, kPairBlocks[3]>{}, ptsTag); break;
+                default: launchBlock(srcTag, std::integral_constant<int, kPairBlocks[0]>{}, ptsTag); break;
                 }
             };
-            switch (src)
+            auto withSrc = [&](auto srcTag) {
+                if (variant >= kPairBlockCount)
+                    withPts(srcTag, std::integral_constant<int, 1>{});
+                else
+                    withPts(srcTag, std::integral_constant<int, kLanePoints>{});
+            };
+            switch (from)
             {
             case kDenseRec: withSrc(std::integral_constant<int, kDenseRec>{}); break;
             case kAnyRec: withSrc(std::integral_constant<int, kAnyRec>{}); break;
@@ -354,7 +376,7 @@ void launchQuery(const sdfo_grid& g, int mode, const TP* pts, int64_t n, TP* gap
             }
             SDFO_HIP(hipGetLastError());
         };
-        // the tuner times every block size on this call's own points (each
+        // the tuner times every launch shape on this call's own points (each
         // run writes the same outputs) the first time a size class is seen
         const std::string key = std::string("query ") + (q.sparse ? "sparse " : "dense ") +
                                 (rec ? "" : "gather ") + (cubic ? "tricubic" : "trilinear") +
@@ -365,8 +387,8 @@ void launchQuery(const sdfo_grid& g, int mode, const TP* pts, int64_t n, TP* gap
         {
             launch(0); // warm the first launch out of the timings
             const std::vector<std::pair<int, double>> t =
-                timeVariants(s, kPairBlockCount, [&](int v) { launch(v); });
-            tunerRecord(key, t, kPairBlockNames);
+                timeVariants(s, nVariants, [&](int v) { launch(v); });
+            tunerRecord(key, t, rec ? kPairVariantNames : kPairVariantNames + kPairShapeCount);
             tunerLookup(key, variant);
         }
         launch(variant);

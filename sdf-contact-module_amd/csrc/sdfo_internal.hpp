@@ -470,6 +470,11 @@ double timeOnStream(hipStream_t s, const std::function<void()>& f);
 std::vector<std::pair<int, double>> timeVariants(hipStream_t s, int n,
                                                  const std::function<void(int)>& run,
                                                  int rounds = 3);
+// The tuner's variant for key: the recorded one; else, when the tuner is on
+// and s is not capturing, run(0) once (warm-up), time all n variants and keep
+// the fastest (names label them in the report); else 0, the default.
+int tunedVariant(const std::string& key, hipStream_t s, int n, const char* const* names,
+                 const std::function<void(int)>& run);
 int fillShapeFor(const GridSpec& g, int64_t ntris);
 
 // Build kernels (build.hip): the calling thread's pinned counter that the last

@@ -1,5 +1,6 @@
-// Autotuner: launch variants of a kernel (brick shapes of the dense fill,
-// block sizes of the batched query) are timed on the caller's own work the
+// Autotuner: launch variants of a kernel (brick shapes and step budgets of
+// the dense fill, points per lane and block sizes of the queries, occupancy of
+// the contact forward, persistent grid of the contact backward) are timed on the caller's own work the
 // first time a size class is seen, when the tuner is on, and the fastest is
 // kept for the rest of the process.  Every variant computes the same result;
 // only the schedule differs.
@@ -125,6 +126,18 @@ std::vector<std::pair<int, double>> timeVariants(hipStream_t s, int n,
         for (int v = 0; v < n; ++v)
             t[size_t(v)].second = std::min(t[size_t(v)].second, timeOnStream(s, [&] { run(v); }));
     return t;
+}
+
+int tunedVariant(const std::string& key, hipStream_t s, int n, const char* const* names,
+                 const std::function<void(int)>& run)
+{
+    int v = 0;
+    if (tunerLookup(key, v)) return v;
+    if (!tunerEnabled() || streamCapturing(s)) return 0;
+    run(0);
+    tunerRecord(key, timeVariants(s, n, run), names);
+    tunerLookup(key, v);
+    return v;
 }
 
 void tunerSetMode(int mode) { gMode.store(mode); }

@@ -176,6 +176,7 @@ _sig("sdfo_debug_force_replay", [_i32])
 _sig("sdfo_debug_fill_budget", [_i32, _i32])
 _sig("sdfo_debug_ca_merge_cap", [_i32, _i32])
 _sig("sdfo_debug_skip_cell_records", [_i32])
+_sig("sdfo_debug_tuner_pin", [C.c_char_p, _i32])
 
 BUILD_PHASES = ("setup_grid", "host_bvh", "upload_alloc", "fill_kernel", "hessian_kernel",
                 "mirror_kernels", "total", "replayed_voxels")
@@ -304,6 +305,11 @@ def tuner_set_mode(mode: int):
 
 def tuner_reset():
     check(lib.sdfo_tuner_reset())
+
+
+def debug_tuner_pin(key: str, variant: int):
+    """Test switch: make `variant` the tuner's decision for `key`."""
+    check(lib.sdfo_debug_tuner_pin(key.encode(), int(variant)))
 
 
 def tuner_report() -> str:

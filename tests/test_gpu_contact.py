@@ -395,3 +395,55 @@ def test_saved_path_edge_sizes(O, cube):
     gt, gp, gx, gw = SC.contact_backward(g, X0, w0, Pd, gout, law, saved=rec0)
     assert float(gt.abs().max()) == 0.0 and float(gp.abs().max()) == 0.0
     assert gx.shape == (0, 3) and gw.shape == (0,)
+
+
+def test_tuner_variants_give_the_same_bits(cube):
+    """Every launch variant the autotuner chooses among (forward occupancy,
+    recording-forward occupancy, backward persistent grid, query points per
+    lane x block size) gives the default variant's bits: a tuning run, then
+    each variant pinned in turn."""
+    import sdfo
+
+    g, _ = cube
+    X, w, poses = _scene(40, seed=23)
+    Xd, wd, Pd = _cuda(X, w, poses)
+    law = SC.ContactLaw(kn=1e11, damping=1e4, mu=0.3, regularizer=1e-6, order=2)
+    gout = torch.from_numpy(np.random.default_rng(5).normal(size=(len(poses), 6))).cuda()
+    pts = torch.from_numpy(scenes.valid_region_points(g.spec, 20000, seed=9)).cuda()
+
+    def run():
+        f = SC.contact_forward(g, Xd, wd, Pd, law)
+        fs, rec = SC.contact_forward(g, Xd, wd, Pd, law, save=True)
+        b = SC.contact_backward(g, Xd, wd, Pd, gout, law, saved=rec)
+        q = g.query(pts, "Tricubic")
+        torch.cuda.synchronize()
+        return [f, fs, *[x for x in b if x is not None], *q]
+
+    sdfo.tuner_reset()
+    ref = run()
+    sdfo.tuner_set_mode(1)
+    try:
+        print("[gpu] tuner: contact and query variants", flush=True)
+        tuned = run()
+        rep = sdfo.tuner_report()
+    finally:
+        sdfo.tuner_set_mode(0)
+    print(rep, flush=True)
+    for a, b in zip(tuned, ref):
+        assert torch.equal(a, b)
+    keys = {}
+    for line in rep.strip().splitlines():
+        key, rest = line.split(": ", 1)
+        keys[key] = rest.count("ms")
+    for prefix in ("contact_forward points", "contact_forward_save", "contact_backward_pieces",
+                   "query dense tricubic f32"):
+        assert any(k.startswith(prefix) for k in keys), prefix
+    for key, nvar in keys.items():
+        if not key.startswith(("contact_", "query dense tricubic f32")):
+            continue
+        for v in range(nvar):
+            print(f"[gpu] pinned {key} -> {v}", flush=True)
+            sdfo._lib.debug_tuner_pin(key, v)
+            for a, b in zip(run(), ref):
+                assert torch.equal(a, b), (key, v)
+    sdfo.tuner_reset()
