@@ -436,20 +436,24 @@ int buildHostBvhInto(const double* verts, int64_t nv, const int32_t* tris, int64
                      bool parallel)
 {
     const int n = int(nt);
-    for (int64_t q = 0; q < 3 * nt; ++q)
-        if (tris[q] < 0 || tris[q] >= nv)
-            throw Error(SDFO_ERR_INVALID_ARGUMENT, "triangle references invalid vertex");
     // per-triangle boxes (not when only the order is asked for) and
-    // centroids (in slot-order items), in parallel chunks
+    // centroids (in slot-order items), in parallel chunks that check the
+    // vertex indices first
     ScratchLease lease;
     double* box = nodes ? lease.s->box.get(size_t(n) * 6) : nullptr;
     Item* items = lease.s->items.get(size_t(n));
+    std::atomic<bool> badIndex{false};
     parallelFor(
         n,
         [&](int t0, int t1) {
             for (int t = t0; t < t1; ++t)
             {
                 const int32_t* ix = tris + 3 * size_t(t);
+                if (ix[0] < 0 || ix[0] >= nv || ix[1] < 0 || ix[1] >= nv || ix[2] < 0 || ix[2] >= nv)
+                {
+                    badIndex.store(true, std::memory_order_relaxed);
+                    return;
+                }
                 const double* a = verts + 3 * size_t(ix[0]);
                 const double* b = verts + 3 * size_t(ix[1]);
                 const double* c = verts + 3 * size_t(ix[2]);
@@ -468,6 +472,7 @@ int buildHostBvhInto(const double* verts, int64_t nv, const int32_t* tris, int64
             }
         },
         parallel);
+    if (badIndex.load()) throw Error(SDFO_ERR_INVALID_ARGUMENT, "triangle references invalid vertex");
     int depth = 0;
     if (n > 0)
     {
