@@ -402,6 +402,18 @@ def bench_batched_gears(args):
     print(json.dumps(line))
 
 
+def _tuned_variant(report, prefix):
+    """The variant the autotuner chose (marked *) for the first key that
+    starts with prefix, or "default" when it timed none."""
+    for line in report.splitlines():
+        key, _, rest = line.partition(": ")
+        if key.startswith(prefix):
+            for item in rest.split(", "):
+                if item.endswith("*"):
+                    return item.split("=")[0]
+    return "default"
+
+
 def _morton_order(pts, spec):
     """Order of points by the Morton (z-order) code of their grid cells."""
     cell = np.floor((pts - np.asarray(spec.bmin)) / np.asarray(spec.voxel)).astype(np.int64)
@@ -729,8 +741,10 @@ def main():
         "plain_fwdbwd_evals_per_s": round(n_eval_b / (pl_ms * 1e-3), 1),
         "fwdbwd_vs_plain": round(pl_ms / fb_ms, 3),
         "roofline": _hbm_roofline(alg_bytes, q_ms * 1e-3,
-                                  "queryPairKernel<float, float, 3> (Tricubic, two points per "
-                                  "lane), distinct point sets"),
+                                  "queryPairKernel<float, float, 3> (Tricubic, tuned launch "
+                                  "shape " + _tuned_variant(sdfo.tuner_report(),
+                                                            "query dense tricubic f32") +
+                                  "), distinct point sets"),
         "contact_roofline": _valu_roofline(FWD_FLOPS * n_eval, c_ms * 1e-3,
                                            "contactForwardKernel + finalize (1000 poses)",
                                            "600 flops per (pose, point) evaluation"),

@@ -186,6 +186,18 @@ __device__ __forceinline__ void pairQuery(const QGrid& g, const P3<TP>* __restri
     }
 }
 
+// The chip's 8 XCDs (each with its own L2) take a launch's workgroups in
+// turn: workgroup b runs on XCD b % 8.  Renumbering so that XCD x takes a
+// contiguous eighth of the work (x * q + min(x, r) + b / 8, a bijection of
+// [0, G) for G = 8 q + r) keeps a spatially coherent point stream's grid
+// reads within one XCD's L2 instead of every XCD reading the whole grid.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int64_t xcdBlock(int64_t b, int64_t G)
+{
+    const int64_t q = G / kXcds, r = G % kXcds, x = b % kXcds;
+    return x * q + (x < r ? x : r) + b / kXcds;
+}
+
 // Trilinear / Tricubic, Pts points per thread (two by default): every
 // point's corner samples (one 128-byte record each, or eight mirror loads)
 // are requested before any is interpolated, multiplying the loads in flight
@@ -198,7 +210,7 @@ queryPairKernel(QGrid g, const P3<TP>* __restrict__ pts, int64_t n, TP* __restri
                 P3<TP>* __restrict__ normal3, P3<TP>* __restrict__ witness3,
                 int32_t* __restrict__ branch)
 {
-    const int64_t i0 = int64_t(blockIdx.x) * (Pts * Block) + threadIdx.x;
+    const int64_t i0 = xcdBlock(blockIdx.x, gridDim.x) * (Pts * Block) + threadIdx.x;
     pairQuery<TP, T, Mode, Src, Pts>(g, pts, i0, Block, n, gap, grad3, valid, normal3, witness3,
                                      branch);
 }
@@ -235,7 +247,7 @@ queryBatchedKernel(const StreamDesc* __restrict__ sd, int ns, int64_t units,
                    const P3<float>* __restrict__ pts, float* __restrict__ gap,
                    P3<float>* __restrict__ grad3, uint8_t* __restrict__ valid)
 {
-    const int64_t u = int64_t(blockIdx.x) * (Block / 64) + (threadIdx.x >> 6);
+    const int64_t u = xcdBlock(blockIdx.x, gridDim.x) * (Block / 64) + (threadIdx.x >> 6);
     if (u >= units) return;
     // the stream owning this unit (wave-uniform binary search)
     int lo = 0, hi = ns - 1;
