@@ -516,6 +516,24 @@ __global__ void cellPackKernel(GridSpec g, const float4* __restrict__ pn, float4
     rec[t] = pn[i + int64_t(g.n[0]) * (j + int64_t(g.n[1]) * k)];
 }
 
+// FP64 cell records: the 8 corners (phi, n) of each cell from the FP64 fields,
+// one thread per (cell, corner), in the cell order of cellPackKernel.
+__global__ void cellPack64Kernel(GridSpec g, const double* __restrict__ phi,
+                                 const double* __restrict__ n3, double4* __restrict__ rec)
+{
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int cx = g.n[0] - 1, cy = g.n[1] - 1, cz = g.n[2] - 1;
+    const int64_t cells = int64_t(cx) * cy * cz;
+    if (t >= cells * 8) return;
+    const int64_t cell = t >> 3;
+    const int q = int(t & 7);
+    const int i = int(cell % cx) + (q & 1);
+    const int j = int((cell / cx) % cy) + ((q >> 1) & 1);
+    const int k = int(cell / (int64_t(cx) * cy)) + (q >> 2);
+    const int64_t v = i + int64_t(g.n[0]) * (j + int64_t(g.n[1]) * k);
+    rec[t] = make_double4(phi[v], n3[3 * v], n3[3 * v + 1], n3[3 * v + 2]);
+}
+
 // Sparse cell records: for the cell whose lowest corner is brick-pool voxel
 // t, the 8 corner records (located through the brick table across brick
 // faces) and one validity bit (all 8 corners exist, the rule of
@@ -1189,6 +1207,16 @@ void launchDenseMirror(int64_t N, const double* phi, const double* n3, const dou
 {
     hipLaunchKernelGGL(denseMirrorKernel, dim3(blocksFor(N, 256)), dim3(256), 0, s, N, phi, n3,
                        h6, pn, h6f);
+    SDFO_HIP(hipGetLastError());
+}
+
+void launchCellPack64(const GridSpec& g, const double* phi, const double* n3, double4* rec,
+                      hipStream_t s)
+{
+    const int64_t items = int64_t(g.n[0] - 1) * (g.n[1] - 1) * (g.n[2] - 1) * 8;
+    if (items <= 0) return;
+    hipLaunchKernelGGL(cellPack64Kernel, dim3(unsigned((items + 255) / 256)), dim3(256), 0, s, g, phi,
+                       n3, rec);
     SDFO_HIP(hipGetLastError());
 }
 

@@ -53,6 +53,8 @@ struct QGrid
     // query kernels: phi, n (3 per voxel), H (6 per voxel), candidate phi
     // (S per voxel) and normals (3S per voxel)
     const double *phi64, *n64, *h64, *cphi64, *cn64;
+    // dense, optional: FP64 cell records (the 8 corners' (phi, n) per cell)
+    const double4* cellRec64;
 };
 
 // The 8 packed corner records of cell c (dense or sparse cell records), or
@@ -268,6 +270,17 @@ template <bool F64 = false>
 __device__ __forceinline__ bool cellCorners(const QGrid& g, const Cell& c,
                                             std::conditional_t<F64, double4, float4> (&f)[8])
 {
+    if constexpr (F64)
+    {
+        if (!g.sparse && g.cellRec64 != nullptr)
+        {
+            const double4* r = g.cellRec64 + 8 * (int64_t(c.i0) + int64_t(g.n[0] - 1) *
+                                                                      (c.j0 + int64_t(g.n[1] - 1) * c.k0));
+#pragma unroll
+            for (int q = 0; q < 8; ++q) f[q] = r[q];
+            return true;
+        }
+    }
     if (!F64 && (g.sparse ? g.scellRec != nullptr : g.cellRec != nullptr))
     {
         if constexpr (!F64)

@@ -536,6 +536,45 @@ void launchQueryF32(const sdfo_grid& g, int mode, const float* pts, int64_t n, f
     launchQuery<float, float>(g, mode, pts, n, gap, grad3, valid, normal3, witness3, branch, s);
 }
 
+const double4* cellRecords64(const sdfo_grid& g, hipStream_t s)
+{
+    sdfo_grid::Records64& R = g.rec64;
+    if (g.sparse || !g.cellRec) return nullptr;
+    if (R.done.load(std::memory_order_acquire)) return R.rec;
+    std::lock_guard<std::mutex> lk(R.lock);
+    if (!R.tried)
+    {
+        if (streamCapturing(s)) return nullptr; // no allocation inside a capture
+        R.tried = true;
+        const int64_t cells = int64_t(g.spec.n[0] - 1) * (g.spec.n[1] - 1) * (g.spec.n[2] - 1);
+        const int64_t bytes = cells * 8 * int64_t(sizeof(double4));
+        if (cells <= 0 || bytes > cellRecordLimit()) return nullptr;
+        double4* rec = nullptr;
+        if (hipMallocAsync(reinterpret_cast<void**>(&rec), size_t(bytes), s) != hipSuccess)
+        {
+            (void)hipGetLastError();
+            return nullptr; // out of memory: the gather path
+        }
+        launchCellPack64(g.spec, g.phi, g.n3, rec, s);
+        SDFO_HIP(hipEventCreateWithFlags(&R.ready, hipEventDisableTiming));
+        SDFO_HIP(hipEventRecord(R.ready, s));
+        R.rec = rec;
+        R.bytes = bytes;
+        return rec; // ordered on s
+    }
+    if (!R.rec) return nullptr;
+    const hipError_t q = hipEventQuery(R.ready);
+    if (q == hipSuccess)
+    {
+        R.done.store(true, std::memory_order_release);
+        return R.rec;
+    }
+    if (q != hipErrorNotReady) SDFO_HIP(q);
+    if (streamCapturing(s)) return nullptr; // still packing: gather, wait for nothing
+    SDFO_HIP(hipStreamWaitEvent(s, R.ready, 0));
+    return R.rec;
+}
+
 void launchQueryF64(const sdfo_grid& g, int mode, const double* pts, int64_t n, double* gap,
                     double* grad3, uint8_t* valid, double* normal3, double* witness3,
                     int32_t* branch, hipStream_t s)

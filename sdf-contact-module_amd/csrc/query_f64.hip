@@ -19,7 +19,13 @@ void launchQueryExactF64(const sdfo_grid& g, int mode, const double* pts, int64_
                          int32_t* branch, hipStream_t s)
 {
     if (n <= 0) return;
-    const QGrid q = dev::makeQGrid(g);
+    QGrid q = dev::makeQGrid(g);
+    // Trilinear and Tricubic (and ContactAware without candidates, which
+    // falls back to Tricubic) read the 8 corners of a cell: from the FP64
+    // cell records when the grid has them
+    if (!g.sparse && (mode == SDFO_TRILINEAR || mode == SDFO_TRICUBIC ||
+                      (mode == SDFO_CONTACT_AWARE && q.stride <= 0)))
+        q.cellRec64 = cellRecords64(g, s);
     using V = P3<double>;
     const dim3 grid(unsigned((n + 255) / 256)), block(256);
     const V* p = reinterpret_cast<const V*>(pts);

@@ -346,6 +346,21 @@ struct sdfo_grid
     float4* scellRec = nullptr;           // nb*B^3*8
     unsigned long long* scellMask = nullptr; // nb * ceil(B^3 / 64) words
 
+    // ---- dense: FP64 cell records of the exact FP64-point queries, made on
+    // the first FP64 Trilinear / Tricubic query of a grid that keeps FP32
+    // cell records (cellRecords64 in query.hip), two 128-byte lines per
+    // cell instead of about ten scattered lines from phi and n3
+    struct Records64
+    {
+        std::mutex lock;
+        double4* rec = nullptr;
+        hipEvent_t ready = nullptr;    // the pack kernel's completion
+        std::atomic<bool> done{false}; // ready has completed
+        bool tried = false;
+        int64_t bytes = 0;
+    };
+    mutable Records64 rec64;
+
     int64_t deviceBytes = 0;
     std::vector<void*> allocations;
     // Arrays come from the device's stream-ordered pool, ordered on this stream
@@ -391,6 +406,8 @@ struct sdfo_grid
     ~sdfo_grid()
     {
         if (arena) arena->noteRelease(releaseOrdered, releaseStream);
+        if (rec64.rec) allocations.push_back(rec64.rec);
+        if (rec64.ready) (void)hipEventDestroy(rec64.ready);
         if (allocations.empty()) return;
         if (!releaseOrdered) (void)hipDeviceSynchronize();
         for (void* p : allocations) (void)hipFreeAsync(p, releaseOrdered ? releaseStream : nullptr);
@@ -526,6 +543,9 @@ struct BatchMesh
 void launchDenseFillBatched(const std::vector<BatchMesh>& meshes, const DeviceBvh& bvh,
                             hipStream_t s);
 void launchCellPack(const GridSpec& g, const float4* pn, float4* cellRec, hipStream_t s);
+// FP64 cell records (8 double4 corner records per cell) from the FP64 fields.
+void launchCellPack64(const GridSpec& g, const double* phi, const double* n3, double4* rec,
+                      hipStream_t s);
 void launchSparseCellPack(const GridSpec& g, int bd, int nbx, int nby, int nbz,
                           const int32_t* bijk, const int32_t* table, const float4* spn,
                           int64_t count, int words, float4* rec, unsigned long long* mask,
@@ -562,6 +582,12 @@ void launchQueryF32(const sdfo_grid& g, int mode, const float* pts, int64_t n, f
 void launchQueryExactF64(const sdfo_grid& g, int mode, const double* pts, int64_t n, double* gap,
                          double* grad3, uint8_t* valid, double* normal3, double* witness3,
                          int32_t* branch, hipStream_t s);
+// The FP64 cell records of a dense grid for a query on stream s, made on
+// first use (a grid with FP32 cell records, when twice their bytes fit the
+// record limit; never inside a stream capture): the pack kernel runs on s
+// and a query on another stream waits for it.  nullptr: the query gathers
+// its corners from the FP64 fields.
+const double4* cellRecords64(const sdfo_grid& g, hipStream_t s);
 void launchQueryF64(const sdfo_grid& g, int mode, const double* pts, int64_t n, double* gap,
                     double* grad3, uint8_t* valid, double* normal3, double* witness3,
                     int32_t* branch, hipStream_t s);

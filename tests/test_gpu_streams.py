@@ -93,3 +93,40 @@ def test_stream_ordered_release(sdfo_gpu):
         torch.cuda.synchronize()
         assert torch.isfinite(out).all()
     ref_g.close()
+
+
+def test_fp64_cell_records_made_on_one_stream_used_on_another(sdfo_gpu):
+    """A dense grid's FP64 cell records are made by its first FP64 Tricubic
+    query, on that query's stream; a query issued at once on a second stream
+    waits for them.  Both give the bits of the same grid built without cell
+    records (FP64 corners gathered from phi and n)."""
+    from sdfo import _lib
+
+    v, t = scenes.cube_mesh()
+    cuda = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+    _lib.debug_skip_cell_records(True)
+    try:
+        plain = sdfo_gpu.SdfGrid.build_dense(v, t, target_resolution=96, max_resolution=96)
+    finally:
+        _lib.debug_skip_cell_records(False)
+    g = sdfo_gpu.SdfGrid.build_dense(v, t, target_resolution=96, max_resolution=96)
+    try:
+        pts = [cuda(scenes.valid_region_points(g.spec, 300_000, seed=s, dtype=np.float64))
+               for s in (4, 5)]
+        want = [plain.query(p, mode) for p in pts for mode in ("Tricubic", "Trilinear")]
+        torch.cuda.synchronize()
+        s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+        s1.wait_stream(torch.cuda.current_stream())
+        s2.wait_stream(torch.cuda.current_stream())
+        print("[gpu] FP64 records: first query on s1, second on s2 at once", flush=True)
+        with torch.cuda.stream(s1):
+            a = [g.query(pts[0], mode, stream=s1) for mode in ("Tricubic", "Trilinear")]
+        with torch.cuda.stream(s2):
+            b = [g.query(pts[1], mode, stream=s2) for mode in ("Tricubic", "Trilinear")]
+        torch.cuda.synchronize()
+        for got, exp in zip(a + b, want):
+            for x, y in zip(got, exp):
+                assert torch.equal(x, y)
+    finally:
+        g.close()
+        plain.close()
