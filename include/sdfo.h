@@ -90,7 +90,8 @@ typedef struct sdfo_grid_info
     int32_t block_grid[3];    /* sparse: ceil(n / block_dim) */
     double band_distance;
     double activation_distance;
-    int64_t device_bytes;     /* device memory held by the handle */
+    int64_t device_bytes;     /* device memory held by the handle (with the FP64 cell
+                                 records its first FP64 Trilinear / Tricubic query made) */
 } sdfo_grid_info;
 
 /* Contact law parameters (contacts_sdf.py:300-305; ContactPair fields,

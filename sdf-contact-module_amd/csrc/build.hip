@@ -921,6 +921,7 @@ struct ReplayBuf
     // Bytes of the one block a ReplayBuf carves: the 48-byte counter header,
     // then the replay ids (cap items) and, for 64-lane work units, the heavy
     // list, its seeds and the spill list (units * 64 voxels).
+    static constexpr size_t kHeaderBytes = 48; // the six queue counters and list lengths
     static size_t bytes(int64_t cap, int64_t units)
     {
         auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
@@ -933,7 +934,10 @@ struct ReplayBuf
     // cap: work items (replay ids); units: 64-lane work units (heavy list).
     // The lists come from `mem` (at least bytes(cap, units), 256-byte aligned,
     // not in use by other work on the stream) or from one allocation.
-    ReplayBuf(int64_t cap, hipStream_t st, int64_t units = 0, void* mem = nullptr) : s(st)
+    // zeroed: the caller cleared the counter header of `mem` on this stream.
+    ReplayBuf(int64_t cap, hipStream_t st, int64_t units = 0, void* mem = nullptr,
+              bool zeroed = false)
+        : s(st)
     {
         if (!mem)
         {
@@ -943,7 +947,7 @@ struct ReplayBuf
         auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
         char* p = static_cast<char*>(mem);
         queue = reinterpret_cast<unsigned long long*>(p);
-        SDFO_HIP(hipMemsetAsync(queue, 0, 48, s));
+        if (!(zeroed && mem)) SDFO_HIP(hipMemsetAsync(queue, 0, kHeaderBytes, s));
         r.count = reinterpret_cast<int*>(queue + 1);
         h.queue = queue + 2;
         h.count = reinterpret_cast<int*>(queue + 3);
@@ -1064,13 +1068,13 @@ std::vector<std::string> fillVariantNames()
 }
 
 void runDenseFill(const GridSpec& g, const DeviceBvh& bvh, const DenseFillOut& o, int variant,
-                  bool report, hipStream_t s, void* scratch)
+                  bool report, hipStream_t s, void* scratch, bool zeroed = false)
 {
     const int shape = variant / kBudgetCount;
     const int budget = debugFillBudget() > 0 ? debugFillBudget() : kBudgets[variant % kBudgetCount];
     const BrickShape sh = BrickShape::make(g, kFillShapes[shape][0], kFillShapes[shape][1]);
     const D3 dir = dev::d3(bvh.rayDir[0], bvh.rayDir[1], bvh.rayDir[2]);
-    ReplayBuf rb(g.total(), s, sh.count(), scratch);
+    ReplayBuf rb(g.total(), s, sh.count(), scratch, zeroed);
     const double mu = tieMargin(g);
     hipLaunchKernelGGL(denseFillKernel, dim3(queueGrid(sh.count())), dim3(kFillBlock), 0, s, g,
                        bvh.nodes, bvh.tri9, bvh.box6, dir, mu, sh, o, rb.r, rb.h, rb.queue, budget,
@@ -1115,14 +1119,18 @@ size_t denseFillScratchBytes(const GridSpec& g)
     return b;
 }
 
+size_t denseFillScratchHeaderBytes() { return ReplayBuf::kHeaderBytes; }
+
 void launchDenseFill(const GridSpec& g, const DeviceBvh& bvh, double* phi, double* n3,
-                     double* witness3, int32_t* tri, hipStream_t s, void* scratch)
+                     double* witness3, int32_t* tri, hipStream_t s, void* scratch,
+                     bool scratchZeroed)
 {
     const DenseFillOut o{bvh.triIdx, phi, n3, witness3, tri};
     const std::string key = fillKey(g, bvh.ntris);
     int shape = 0;
     if (!tunerLookup(key, shape) && tunerEnabled() && !streamCapturing(s))
     {
+        scratchZeroed = false; // the timed runs leave their counters behind
         // time every brick shape on this build (each run writes the same,
         // exact fields) and keep the fastest for the size class
         const std::vector<std::pair<int, double>> t = timeVariants(
@@ -1134,7 +1142,7 @@ void launchDenseFill(const GridSpec& g, const DeviceBvh& bvh, double* phi, doubl
         tunerLookup(key, shape);
     }
     if (shape < 0 || shape >= kFillVariants) shape = 0;
-    runDenseFill(g, bvh, o, shape, true, s, scratch);
+    runDenseFill(g, bvh, o, shape, true, s, scratch, scratchZeroed);
 }
 
 void launchDenseFillBatched(const std::vector<BatchMesh>& meshes, const DeviceBvh& bvh,

@@ -317,6 +317,7 @@ std::unique_ptr<sdfo_grid> buildDense(const double* verts, int64_t nv, const int
         const BvhMeshPart part{0, nt, 0, 0, 0};
         db.stageMeshes(verts, nv, tris, nt, &part, 1, s);
         SDFO_HIP(hipMallocAsync(&fillScratch, denseFillScratchBytes(g->spec), s));
+        SDFO_HIP(hipMemsetAsync(fillScratch, 0, denseFillScratchHeaderBytes(), s));
         for (hipEvent_t& e : ev) SDFO_HIP(hipEventCreate(&e));
     };
     bool allocJoined = false;
@@ -345,7 +346,7 @@ std::unique_ptr<sdfo_grid> buildDense(const double* verts, int64_t nv, const int
         db.assemble(hb.slotTri, hb.depth, hb.rayDir, s);
         ph[2] = msSince(t2);
         SDFO_HIP(hipEventRecord(ev[0], s));
-        launchDenseFill(g->spec, db, g->phi, g->n3, g->witness3, g->tri, s, fillScratch);
+        launchDenseFill(g->spec, db, g->phi, g->n3, g->witness3, g->tri, s, fillScratch, true);
         SDFO_HIP(hipEventRecord(ev[1], s));
         launchDenseHessian(g->spec, g->n3, g->h6, s, g->phi, g->pn, g->h6f);
         SDFO_HIP(hipEventRecord(ev[2], s));
@@ -762,7 +763,12 @@ void fillInfo(const sdfo_grid& g, sdfo_grid_info* info)
     info->block_grid[2] = g.nbz;
     info->band_distance = g.band;
     info->activation_distance = g.activation;
-    info->device_bytes = g.deviceBytes;
+    int64_t lazy = 0;
+    {
+        std::lock_guard<std::mutex> lk(g.rec64.lock);
+        lazy = g.rec64.rec ? g.rec64.bytes : 0;
+    }
+    info->device_bytes = g.deviceBytes + lazy;
 }
 
 void checkGrid(const sdfo_grid* g)

@@ -553,9 +553,15 @@ void launchSparseCellPack(const GridSpec& g, int bd, int nbx, int nby, int nbz,
 // `scratch`, when given, holds denseFillScratchBytes(g) bytes (256-byte
 // aligned) that no other work on s uses meanwhile; otherwise the lists of the
 // fill passes are allocated on s.
+// scratch: denseFillScratchBytes(g) of device memory for the fill's lists
+// (else allocated per call); scratchZeroed: its first
+// denseFillScratchHeaderBytes() were cleared on s beforehand, so the fill
+// enqueues no clear of its own (a build clears them while its host BVH runs).
 void launchDenseFill(const GridSpec& g, const DeviceBvh& bvh, double* phi, double* n3,
-                     double* witness3, int32_t* tri, hipStream_t s, void* scratch = nullptr);
+                     double* witness3, int32_t* tri, hipStream_t s, void* scratch = nullptr,
+                     bool scratchZeroed = false);
 size_t denseFillScratchBytes(const GridSpec& g);
+size_t denseFillScratchHeaderBytes();
 // With pn (and h6f) set, also writes the FP32 query mirror of every voxel.
 void launchDenseHessian(const GridSpec& g, const double* n3, double* h6, hipStream_t s,
                         const double* phi = nullptr, float4* pn = nullptr, float* h6f = nullptr);
