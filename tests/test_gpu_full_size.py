@@ -97,6 +97,21 @@ def test_tricubic_1m_points_full_size(O, cube_field):
     assert torch.equal(torch.cat(parts), gap)
 
 
+def test_tricubic_1m_fp64_points_full_size(O, cube_field):
+    """The benchmark's 1e6 points as FP64 points: the exact path (FP64 cell
+    records, made by this first FP64 query) returns the oracle's FP64
+    Tricubic on the FP64 field bit for bit, gap, gradient and valid mask."""
+    g, ref = cube_field
+    print("[gpu] full-size FP64-point Tricubic, 1e6 points", flush=True)
+    pts = scenes.valid_region_points(g.spec, 1_000_000, seed=1).astype(np.float64)
+    gap, grad, valid = g.query(torch.from_numpy(pts).cuda(), "Tricubic")
+    rq = O.query_dense(ograd(O, g.spec), dict(phi=ref["phi"], n=ref["n"], H=ref["H"]),
+                       "Tricubic", pts)
+    assert np.array_equal(valid.cpu().numpy(), rq["valid"])
+    assert np.array_equal(gap.cpu().numpy(), rq["gap"])
+    assert np.array_equal(grad.cpu().numpy(), rq["grad"])
+
+
 def test_contact_1000_poses_full_size(O, cube_field):
     g, ref = cube_field
     print("[gpu] full-size contact, 1000 poses x 7680 points", flush=True)
