@@ -1247,10 +1247,12 @@ PairList savedPairs(void* saved, const ContactLayout& o)
                     reinterpret_cast<uint8_t*>(b + o.oAct)};
 }
 
-// Occupancy variants of the forward kernels the tuner chooses from (the
-// default first).
+// Occupancy variants of the forward kernels the tuner chooses from, the
+// default first: four waves per SIMD for the generic field read, three for
+// the dense Tricubic specialisation (whose register use spills at four).
 constexpr int kFwdVariants = 2;
 const char* const kFwdVariantNames[kFwdVariants] = {"4waves", "3waves"};
+const char* const kFwdVariantNamesDC[kFwdVariants] = {"3waves", "4waves"};
 
 std::string forwardKey(const char* what, int64_t n, int64_t nb)
 {
@@ -1291,17 +1293,26 @@ void forwardLaunch(const QGrid& q, const Law& L, const double* X, const double* 
     }
     else if (rec)
     {
+        const bool dc = fieldKind(q, L) == kFieldDenseCubic;
         auto launchSave = [&](int var) {
-            if (var == 1)
-                hipLaunchKernelGGL((contactForwardSaveKernel<kFieldAny, 3>), grid, dim3(kBlock), 0, s, q,
-                                   L, X, w, n, poses, chunkLen, partial, savedPairs(saved, lay));
+            auto go = [&](auto kindTag, auto wTag) {
+                hipLaunchKernelGGL((contactForwardSaveKernel<decltype(kindTag)::value, decltype(wTag)::value>),
+                                   grid, dim3(kBlock), 0, s, q, L, X, w, n, poses, chunkLen, partial,
+                                   savedPairs(saved, lay));
+            };
+            using DC = std::integral_constant<int, kFieldDenseCubic>;
+            using AnyK = std::integral_constant<int, kFieldAny>;
+            using W3 = std::integral_constant<int, 3>;
+            using W4 = std::integral_constant<int, 4>;
+            if (dc)
+                var == 1 ? go(DC{}, W4{}) : go(DC{}, W3{});
             else
-                hipLaunchKernelGGL((contactForwardSaveKernel<kFieldAny, 4>), grid, dim3(kBlock), 0, s, q,
-                                   L, X, w, n, poses, chunkLen, partial, savedPairs(saved, lay));
+                var == 1 ? go(AnyK{}, W3{}) : go(AnyK{}, W4{});
             SDFO_HIP(hipGetLastError());
         };
-        launchSave(tunedVariant(forwardKey("contact_forward_save", n, nb), s, kFwdVariants,
-                                kFwdVariantNames, launchSave));
+        launchSave(tunedVariant(forwardKey(dc ? "contact_forward_save tricubic" : "contact_forward_save",
+                                           n, nb),
+                                s, kFwdVariants, dc ? kFwdVariantNamesDC : kFwdVariantNames, launchSave));
         // the lists and the force finalize in one launch
         const int64_t blocks = nb * chunks + (nb + kBlock - 1) / kBlock;
         hipLaunchKernelGGL(pairListKernel, dim3(unsigned(blocks)), dim3(kBlock), 0, s, n, nb, chunks,
@@ -1312,17 +1323,30 @@ void forwardLaunch(const QGrid& q, const Law& L, const double* X, const double* 
     }
     else
     {
+        // the direct forward on a dense Tricubic grid with cell records takes
+        // the specialised field read (one record read, no other path)
+        const bool dc = !Cached && fieldKind(q, L) == kFieldDenseCubic;
         auto launchFwd = [&](int var) {
-            if (var == 1)
-                hipLaunchKernelGGL((contactForwardKernel<Cached, false, kFieldAny, 3>), grid, dim3(kBlock),
-                                   0, s, q, L, X, w, n, poses, chunkLen, partial, lc, CullArgs{}, nullptr);
+            auto go = [&](auto kindTag, auto wTag) {
+                hipLaunchKernelGGL((contactForwardKernel<Cached, false, decltype(kindTag)::value,
+                                                         decltype(wTag)::value>),
+                                   grid, dim3(kBlock), 0, s, q, L, X, w, n, poses, chunkLen, partial, lc,
+                                   CullArgs{}, nullptr);
+            };
+            using DC = std::integral_constant<int, kFieldDenseCubic>;
+            using AnyK = std::integral_constant<int, kFieldAny>;
+            using W3 = std::integral_constant<int, 3>;
+            using W4 = std::integral_constant<int, 4>;
+            if (dc)
+                var == 1 ? go(DC{}, W4{}) : go(DC{}, W3{});
             else
-                hipLaunchKernelGGL((contactForwardKernel<Cached, false, kFieldAny, 4>), grid, dim3(kBlock),
-                                   0, s, q, L, X, w, n, poses, chunkLen, partial, lc, CullArgs{}, nullptr);
+                var == 1 ? go(AnyK{}, W3{}) : go(AnyK{}, W4{});
             SDFO_HIP(hipGetLastError());
         };
-        launchFwd(tunedVariant(forwardKey(Cached ? "contact_forward_linearized" : "contact_forward", n, nb),
-                               s, kFwdVariants, kFwdVariantNames, launchFwd));
+        launchFwd(tunedVariant(forwardKey(Cached ? "contact_forward_linearized"
+                                                 : (dc ? "contact_forward tricubic" : "contact_forward"),
+                                          n, nb),
+                               s, kFwdVariants, dc ? kFwdVariantNamesDC : kFwdVariantNames, launchFwd));
     }
     SDFO_HIP(hipGetLastError());
     hipLaunchKernelGGL(contactForwardFinalize, dim3(unsigned((nb + 255) / 256)), dim3(256), 0, s,

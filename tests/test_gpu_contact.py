@@ -435,7 +435,8 @@ def test_tuner_variants_give_the_same_bits(cube):
     for line in rep.strip().splitlines():
         key, rest = line.split(": ", 1)
         keys[key] = rest.count("ms")
-    for prefix in ("contact_forward points", "contact_forward_save", "contact_backward_pieces",
+    for prefix in ("contact_forward tricubic points", "contact_forward_save tricubic",
+                   "contact_backward_pieces",
                    "query dense tricubic f32"):
         assert any(k.startswith(prefix) for k in keys), prefix
     for key, nvar in keys.items():
