@@ -1016,8 +1016,8 @@ __global__ void evaluatePrepKernel(EvalPrep in, int64_t nb, double* __restrict__
 
 // grid = (chunks, pairs).  partial[(pair*chunks + chunk)*7 + k]: F_A, tau_A
 // (world frame) and the active count.
-template <bool Culled, int Kind = kFieldAny>
-__global__ void __launch_bounds__(kBlock)
+template <bool Culled, int Kind = kFieldAny, int W = 0>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(W > 0 ? W : 1, W > 0 ? W : 8)))
 contactEvaluateKernel(QGrid g, Law L, const double* __restrict__ X, const double* __restrict__ w,
                       int64_t n, const double* __restrict__ rec, int64_t chunkLen,
                       double* __restrict__ partial, CullArgs cu, const uint8_t* __restrict__ anyLeaf)
@@ -1260,6 +1260,11 @@ std::string forwardKey(const char* what, int64_t n, int64_t nb)
            std::to_string(log2Bucket(nb));
 }
 
+// Occupancy variants of the world-frame evaluate on the dense Tricubic path
+// (the default first: the compiler's own register budget).
+constexpr int kEvalVariants = 3;
+const char* const kEvalVariantNames[kEvalVariants] = {"auto", "3waves", "4waves"};
+
 // Persistent grid of the backward's piece kernel the tuner chooses from, in
 // blocks per CU (the default first); the pieces and their sums are fixed by
 // the piece table, so every variant gives the same bits.
@@ -1431,10 +1436,25 @@ void evaluateLaunch(const sdfo_grid& g, const double* X, const double* w, int64_
                                chunkLen, partial, cu, static_cast<const uint8_t*>(anyLeaf));
     }
     else if (fieldKind(q, L) == kFieldDenseCubic)
-        hipLaunchKernelGGL((contactEvaluateKernel<false, kFieldDenseCubic>),
-                           dim3(chunks, unsigned(nb)), dim3(kBlock), 0, s, q, L, X, w, n,
-                           static_cast<const double*>(rec), chunkLen, partial, CullArgs{},
-                           static_cast<const uint8_t*>(nullptr));
+    {
+        auto launchEval = [&](int var) {
+            auto go = [&](auto wTag) {
+                hipLaunchKernelGGL((contactEvaluateKernel<false, kFieldDenseCubic, decltype(wTag)::value>),
+                                   dim3(chunks, unsigned(nb)), dim3(kBlock), 0, s, q, L, X, w, n,
+                                   static_cast<const double*>(rec), chunkLen, partial, CullArgs{},
+                                   static_cast<const uint8_t*>(nullptr));
+            };
+            if (var == 1)
+                go(std::integral_constant<int, 3>{});
+            else if (var == 2)
+                go(std::integral_constant<int, 4>{});
+            else
+                go(std::integral_constant<int, 0>{});
+            SDFO_HIP(hipGetLastError());
+        };
+        launchEval(tunedVariant(forwardKey("contact_evaluate tricubic", n, nb), s, kEvalVariants,
+                                kEvalVariantNames, launchEval));
+    }
     else
         hipLaunchKernelGGL((contactEvaluateKernel<false>), dim3(chunks, unsigned(nb)), dim3(kBlock),
                            0, s, q, L, X, w, n, static_cast<const double*>(rec), chunkLen, partial,
@@ -1605,13 +1625,18 @@ void launchContactBackward(const sdfo_grid& g, const double* X, const double* w,
         // SIMD, the kernel's register budget, four SIMDs per CU)
         auto pieces = [&](int var) {
             if (var < 0 || var >= kPieceGridCount) var = 0;
-            hipLaunchKernelGGL(contactBackwardPieceKernel<kFieldAny>,
-                               dim3(unsigned(kPieceGrids[var] * cus)), dim3(64), 0, s, q, L, X, w, n,
-                               poses, gOut, chunks, chunkLen, maxPieces, base,
-                               static_cast<const int32_t*>(gl.list),
-                               static_cast<const int32_t*>(gl.count),
-                               static_cast<const int32_t*>(items),
-                               static_cast<const int32_t*>(total), pieceSum, pointPartial);
+            auto go = [&](auto kindTag) {
+                hipLaunchKernelGGL(contactBackwardPieceKernel<decltype(kindTag)::value>,
+                                   dim3(unsigned(kPieceGrids[var] * cus)), dim3(64), 0, s, q, L, X, w, n,
+                                   poses, gOut, chunks, chunkLen, maxPieces, base,
+                                   static_cast<const int32_t*>(gl.list),
+                                   static_cast<const int32_t*>(gl.count),
+                                   static_cast<const int32_t*>(items),
+                                   static_cast<const int32_t*>(total), pieceSum, pointPartial);
+            };
+            // the generic field read: the dense Tricubic specialisation
+            // compiles to 256 VGPRs (one wave per SIMD) here and is slower
+            go(std::integral_constant<int, kFieldAny>{});
             SDFO_HIP(hipGetLastError());
         };
         pieces(tunedVariant("contact_backward_pieces pairs~2^" +
